@@ -692,7 +692,8 @@ int or_decode_window(const float* d0, const float* d1, const float* d2, uint32_t
                      uint32_t filler, uint32_t max_iterations, int early_stop,
                      uint32_t seg_len, uint32_t warmup, int lin, float* post, int* crc_ok,
                      uint32_t* iterations_used) {
-  if (or_qpp_index(k) < 0 || seg_len == 0 || k % seg_len != 0 || warmup > seg_len) return -1;
+  if (or_qpp_index(k) < 0 || seg_len == 0 || k % seg_len != 0) return -1;
+  if (k / seg_len > 1 && warmup >= seg_len) return -1; /* warm-up stays inside the neighbour */
   const uint32_t nseg = k / seg_len;
   const int g = (nseg > 1) ? (int)warmup : 0;
   uint32_t* perm = (uint32_t*)malloc(sizeof(uint32_t) * k);

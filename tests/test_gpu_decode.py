@@ -1,0 +1,221 @@
+"""HIP decoders against the oracle, the golden vectors of the reference and
+the reference's BER.  Marked gpu: they need an MI355X.
+
+Tolerances (stated per test): exact decoder posterior LLRs within 1e-4
+absolute of the reference (lte_turbo.cpp:736-813) with identical hard
+decisions where |LLR| >= 1e-3; window decoders against the oracle's
+restatement of the same algorithm (or_decode_window) and, for BER, inside
+the reference decoder's 95% Clopper-Pearson interval.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import turbo_cpp_amd as T
+from tests.helpers import assert_llr_parity, bpsk_awgn_llr, clopper_pearson, golden, random_blocks
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def _split(llr, k):
+    d = k + 4
+    return llr[:d], llr[d:2 * d], llr[2 * d:]
+
+
+GOLDEN_DECODES = [(40, 100, 6), (64, 0, 8), (544, 50, 6), (1024, 50, 8), (6144, 100, 8), (6144, 50, 8),
+                  (6144, 25, 8)]
+
+
+@pytest.mark.parametrize("k,eb,it", GOLDEN_DECODES)
+def test_exact_decoder_matches_reference_golden(k, eb, it):
+    g = golden()
+    tag = f"dec_k{k}_eb{eb}_it{it}"
+    _say(tag)
+    llr = g[tag + "_llr"]
+    post, ok, used = T.turbo_decode_code_block_gpu_stage1(T.CodeBlockDescriptor(0, k, k + 4), llr[0], llr[1],
+                                                          llr[2], it, False)
+    assert used == it and not ok
+    assert_llr_parity(post, g[tag + "_post"], atol=1e-4)
+
+
+def test_exact_decoder_early_stop_matches_reference():
+    g = golden()
+    llr = g["es_k1024_llr"]
+    post, ok, used = T.turbo_decode_code_block_gpu_stage1(T.CodeBlockDescriptor(0, 1024, 1028), *llr, 8, True)
+    assert [int(ok), used] == list(g["es_k1024_meta"])
+    assert_llr_parity(post, g["es_k1024_post"], atol=1e-4)
+
+
+def test_exact_batched_equals_single_calls():
+    """Uneven K in one launch: every block equals its single-block decode."""
+    rng = np.random.default_rng(11)
+    ks = [40, 6144, 104, 1024, 544, 40]
+    llrs, singles = [], []
+    for k in ks:
+        cb = rng.integers(0, 2, k).astype(np.uint8)
+        l = np.concatenate(bpsk_awgn_llr(O.encode_cb(cb), k, 0.5, rng))
+        llrs.append(l)
+        singles.append(T.turbo_decode_code_block_gpu_stage1(T.CodeBlockDescriptor(0, k, k + 4), *_split(l, k), 6,
+                                                            False)[0])
+    post, _, its = T.decode_batch_host(ks, np.concatenate(llrs), T.DECODER_EXACT, 6)
+    off = 0
+    for k, s in zip(ks, singles):
+        assert np.array_equal(post[off:off + k].view(np.uint32), s.view(np.uint32))
+        off += k
+    assert list(its) == [6] * len(ks)
+
+
+# ------------------------------------------------------------ window decoders
+WINDOW_CASES = [(6144, 0.5), (6144, 1.0), (1024, 0.5), (40, 0.5), (1952, 0.75), (6080, 0.25)]
+
+
+@pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])
+@pytest.mark.parametrize("k,ebn0", WINDOW_CASES)
+def test_window_decoder_matches_oracle(k, ebn0, decoder):
+    """Same algorithm on both sides (segments, warm-up, NII, arithmetic);
+    they differ only in exp2/log2 rounding (v_exp_f32/v_log_f32 vs libm).
+    Bar: |dLLR| <= 1e-3 + 1e-4 |LLR| and equal hard decisions where |LLR| >= 1e-3."""
+    geo = T.segment_geometry(k, decoder)
+    _say(f"k={k} ebn0={ebn0} decoder={decoder} geometry={geo}")
+    rng = np.random.default_rng(k + int(100 * ebn0))
+    bits, llr = random_blocks(3, k, ebn0, int(rng.integers(1 << 30)), O.encode_cb)
+    post, _, its = T.decode_batch_host([k] * 3, llr, decoder, 8)
+    for i in range(3):
+        ref = O.decode_window(*_split(llr[i], k), k, geo["segment_length"], geo["warmup"],
+                              decoder == T.DECODER_WINDOW_LIN, 0, 8, False)[0]
+        assert_llr_parity(post[i * k:(i + 1) * k], ref, atol=1e-3, rtol=1e-4)
+    assert list(its) == [8, 8, 8]
+
+
+@pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])
+def test_window_single_segment_matches_reference(decoder):
+    """K=40 and K=56 use one segment per block (M=1): no boundaries, so the
+    window decoder is a full-length MAP decoder and must match the reference's
+    posteriors up to rounding."""
+    g = golden()
+    llr = g["dec_k40_eb100_it6_llr"]
+    post, _, _ = T.decode_batch_host([40], llr.reshape(-1), decoder, 6)
+    assert_llr_parity(post, g["dec_k40_eb100_it6_post"], atol=2e-4, rtol=2e-5)
+
+
+def test_window_batched_mixed_sizes_equal_single():
+    """Blocks of different K packed into shared jobs give the same bits as
+    decoding each alone (the packing does not change any lane's arithmetic)."""
+    rng = np.random.default_rng(5)
+    ks = [6144, 40, 40, 1024, 6144, 544, 3200, 40]
+    llrs = []
+    for k in ks:
+        cb = rng.integers(0, 2, k).astype(np.uint8)
+        llrs.append(np.concatenate(bpsk_awgn_llr(O.encode_cb(cb), k, 0.5, rng)))
+    post, _, _ = T.decode_batch_host(ks, np.concatenate(llrs), T.DECODER_WINDOW_LOG, 8)
+    off = 0
+    for k, l in zip(ks, llrs):
+        single, _, _ = T.decode_batch_host([k], l, T.DECODER_WINDOW_LOG, 8)
+        assert np.array_equal(post[off:off + k].view(np.uint32), single.view(np.uint32))
+        off += k
+
+
+def test_window_decoder_is_deterministic():
+    bits, llr = random_blocks(70, 6144, 0.5, 99, O.encode_cb)
+    a, _, _ = T.decode_batch_host([6144] * 70, llr, T.DECODER_WINDOW_LOG, 8)
+    b, _, _ = T.decode_batch_host([6144] * 70, llr, T.DECODER_WINDOW_LOG, 8)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _reference_posteriors(llr, k, iters):
+    if O.ref_available():
+        return O.ref_decode_many(llr, k, iters, min(16, os.cpu_count() or 1))
+    return np.stack([O.decode_cb(*_split(l, k), k, 0, iters, False)[0] for l in llr])
+
+
+@pytest.mark.parametrize("ebn0", [0.5, 0.75])
+def test_window_ber_inside_reference_interval(ebn0):
+    """BER and BLER of the window decoders lie inside the 95% interval of the
+    reference decoder on the same 128 noisy K=6144 blocks (8 iterations)."""
+    n, k = 128, 6144
+    bits, llr = random_blocks(n, k, ebn0, 1234 + int(ebn0 * 100), O.encode_cb)
+    ref = _reference_posteriors(llr, k, 8)
+    ref_err = ((ref >= 0) != bits).sum(1)
+    lo_b, hi_b = clopper_pearson(int(ref_err.sum()), n * k)
+    lo_f, hi_f = clopper_pearson(int((ref_err > 0).sum()), n)
+    _say(f"ebn0={ebn0} reference BER={ref_err.sum() / (n * k):.3g} BLER={(ref_err > 0).mean():.3g}")
+    for dec in (T.DECODER_EXACT, T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN):
+        post, _, _ = T.decode_batch_host([k] * n, llr, dec, 8)
+        err = ((post.reshape(n, k) >= 0) != bits).sum(1)
+        ber, bler = err.sum() / (n * k), (err > 0).mean()
+        _say(f"  decoder={dec} BER={ber:.3g} BLER={bler:.3g}")
+        assert lo_b <= ber <= hi_b, (dec, ber, lo_b, hi_b)
+        assert lo_f <= bler <= hi_f, (dec, bler, lo_f, hi_f)
+
+
+# ------------------------------------------- reference test mirrors (GPU) --
+def _tb(tbs, pattern, iters=6):
+    bits = np.array([(i * pattern[0] + pattern[1]) & 1 for i in range(tbs)], np.uint8)
+    seg = T.segment_transport_block(T.append_crc24a(bits))
+    p = T.LteTurboCodecParams(transport_block_bits=tbs, g_total=sum(3 * (len(c) + 4) for c in seg.code_blocks),
+                              max_iterations=iters)
+    return bits, p, T.encode_transport_block(p, bits)
+
+
+@pytest.mark.parametrize("tbs,pattern", [(40, (7, 3)), (512, (11, 5)), (6112, (19, 11)), (12000, (17, 9))])
+def test_reference_roundtrips_gpu(tbs, pattern):
+    # reference tests/test_main.cpp:132-186, 211-254, 287-302, 336-343
+    bits, p, enc = _tb(tbs, pattern)
+    cpu = T.decode_transport_block(p, enc, T.DecoderBackend.CpuReference)
+    for backend in (T.DecoderBackend.GpuStage1Exact, T.DecoderBackend.GpuStage2Windowed):
+        dec = T.decode_transport_block(p, enc, backend)
+        assert np.array_equal(dec.tb_bits, bits), backend
+        assert dec.transport_block_crc_ok
+        assert dec.posterior_llr is not None and len(dec.posterior_llr) == len(cpu.posterior_llr)
+        assert np.all(np.isfinite(dec.posterior_llr))
+        if backend == T.DecoderBackend.GpuStage1Exact:
+            # test_main.cpp:142 (Stage1 vs CPU, 2e-5) on noise-free +-8 inputs
+            np.testing.assert_allclose(dec.posterior_llr, cpu.posterior_llr, atol=2e-5, rtol=1e-6)
+
+
+def test_tile8_alias_hard_decisions():
+    # reference tests/test_main.cpp:153-175: tile8 hard decisions equal cb_bits
+    bits, p, enc = _tb(40, (7, 3))
+    blk = enc.code_blocks[0]
+    l = T.derate_match_code_block(p, blk.descriptor, blk.rate_matched_bits)
+    post, _, _ = T.turbo_decode_code_block_gpu_stage1_tile8_experimental(blk.descriptor, *_split(l, 64), 6, False)
+    assert np.all(np.isfinite(post))
+    assert np.array_equal((post >= 0).astype(np.uint8), blk.cb_bits)
+
+
+def test_gpu_error_paths():
+    d = T.CodeBlockDescriptor(0, 40, 44)
+    z = np.zeros(44, np.float32)
+    with pytest.raises(ValueError):
+        T.turbo_decode_code_block_gpu_stage1(d, z[:43], z, z, 8, False)
+    with pytest.raises(IndexError):
+        T.turbo_decode_code_block_gpu_stage2(T.CodeBlockDescriptor(0, 41, 45), np.zeros(45, np.float32),
+                                             np.zeros(45, np.float32), np.zeros(45, np.float32), 8, False)
+    with pytest.raises(ValueError):
+        T.decode_batch_host([40], np.zeros(10, np.float32))
+
+
+def test_plan_on_device_buffers():
+    torch = pytest.importorskip("torch")
+    ks = [6144] * 5 + [1024] * 3
+    rng = np.random.default_rng(3)
+    llrs = []
+    for k in ks:
+        cb = rng.integers(0, 2, k).astype(np.uint8)
+        llrs.append(np.concatenate(bpsk_awgn_llr(O.encode_cb(cb), k, 0.75, rng)))
+    host = np.concatenate(llrs)
+    plan = T.Plan(ks, T.DECODER_WINDOW_LOG, 8)
+    x = torch.from_numpy(host).cuda()
+    y = torch.empty(plan.post_floats, dtype=torch.float32, device="cuda")
+    it = torch.zeros(len(ks), dtype=torch.int32, device="cuda")
+    plan.decode(x.data_ptr(), y.data_ptr(), 0, it.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    want, _, _ = T.decode_batch_host(ks, host, T.DECODER_WINDOW_LOG, 8)
+    assert np.array_equal(y.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    assert it.cpu().tolist() == [8] * len(ks)

@@ -1,0 +1,256 @@
+"""Pins the oracle (CPU restatement) before it is trusted as the checker.
+
+Checks: the reference's own known-answer test (QPP table), golden vectors
+written from the compiled reference (tests/golden), the compiled reference
+itself where oracle/_ref exists, and what the mathematics fixes: QPP
+bijection, CRC polynomial and GF(2) linearity, the segmentation closed form,
+trellis termination, the rate-matching permutation, and brute-force APP
+enumeration for the SISO.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.helpers import bpsk_awgn_llr, golden
+
+
+# --------------------------------------------------------------------- QPP --
+def test_qpp_table_known_answers():
+    # reference tests/test_main.cpp:37-44
+    t = O.qpp_table()
+    assert len(t) == 188
+    assert t[0] == (40, 3, 10)
+    assert t[-1] == (6144, 263, 480)
+    ks = [e[0] for e in t]
+    grid = list(range(40, 513, 8)) + list(range(528, 1025, 16)) + list(range(1056, 2049, 32)) + \
+        list(range(2112, 6145, 64))
+    assert ks == grid
+
+
+@pytest.mark.parametrize("k", [e[0] for e in O.qpp_table()])
+def test_qpp_is_bijection_closed_form(k):
+    p = O.qpp_perm(k)
+    _, f1, f2 = [e for e in O.qpp_table() if e[0] == k][0]
+    i = np.arange(k, dtype=np.int64)
+    assert np.array_equal(p, (f1 * i + f2 * i * i) % k)
+    assert np.array_equal(np.sort(p), np.arange(k))
+
+
+# --------------------------------------------------------------------- CRC --
+POLY = {O.CRC24A: 0xC199BF, O.CRC24B: 0xC00063, O.CRC16: 0x8811, O.CRC8: 0xD9}
+LEN = {O.CRC24A: 24, O.CRC24B: 24, O.CRC16: 16, O.CRC8: 8}
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_crc_single_bit_is_generator(kind):
+    # remainder of D^N mod G(D) is G(D) - D^N: the reference's tap array
+    # (lte_turbo.cpp:104-116), which differs from 36.212
+    out = O.crc_append(kind, [1])
+    n = LEN[kind]
+    val = int("".join(str(b) for b in out[1:]), 2)
+    assert val == POLY[kind]
+    assert O.crc_check(kind, out)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_crc_linearity_and_detection(kind):
+    rng = np.random.default_rng(kind)
+    n = LEN[kind]
+    for m in (5, 64, 300):
+        a = rng.integers(0, 2, m).astype(np.uint8)
+        b = rng.integers(0, 2, m).astype(np.uint8)
+        ca, cb, cab = O.crc_append(kind, a), O.crc_append(kind, b), O.crc_append(kind, a ^ b)
+        assert np.array_equal(ca[m:] ^ cb[m:], cab[m:])
+        flipped = ca.copy()
+        flipped[rng.integers(len(flipped))] ^= 1
+        assert not O.crc_check(kind, flipped)  # every single-bit error is detected
+    assert not O.crc_check(kind, np.zeros(n - 1, np.uint8))  # shorter than N
+    assert O.crc_check(kind, np.zeros(n, np.uint8))
+
+
+def test_crc_golden():
+    g = golden()
+    for kind in range(4):
+        for n in (1, 7, 40, 333):
+            assert np.array_equal(O.crc_append(kind, g[f"crc{kind}_n{n}_in"]), g[f"crc{kind}_n{n}_out"])
+
+
+# ------------------------------------------------------------ segmentation --
+@pytest.mark.parametrize("b,c,ks,f", [
+    (64, 1, [64], 0),            # TB 40 + CRC24A
+    (536, 1, [544], 8),          # TB 512
+    (6136, 1, [6144], 8),        # TB 6112
+    (12024, 2, [6016, 6080], 24),  # TB 12000
+    (75400, 13, [5824] * 13, None),  # TB 75376
+])
+def test_segmentation_spot_values(b, c, ks, f):
+    s, blocks = O.segment(np.zeros(b, np.uint8))
+    assert s["c"] == c
+    assert [len(x) for x in blocks] == ks
+    if f is not None:
+        assert s["filler_bits"] == f
+    # 36.212 5.1.2 closed form: B' = B + C L, F = sum K_r - B'
+    l = 0 if c == 1 else 24
+    assert s["filler_bits"] == sum(ks) - (b + c * l)
+
+
+def test_segmentation_golden():
+    g = golden()
+    for b, fields in zip(g["seg_sizes"], g["seg_fields"]):
+        s, blocks = O.segment(g[f"seg_b{b}_in"])
+        assert [s[x] for x in ("c", "c_plus", "c_minus", "k_plus", "k_minus", "filler_bits", "l")] == list(fields)
+        assert np.array_equal(np.concatenate(blocks), g[f"seg_b{b}_blocks"])
+
+
+def test_segmentation_crc24b_per_block():
+    rng = np.random.default_rng(3)
+    s, blocks = O.segment(rng.integers(0, 2, 20000).astype(np.uint8))
+    for r, blk in enumerate(blocks):
+        f = s["filler_bits"] if r == 0 else 0
+        assert O.crc_check(O.CRC24B, blk[f:])
+
+
+# ---------------------------------------------------------------- encoder --
+def _rsc(s, u):
+    s0, s1, s2 = s & 1, (s >> 1) & 1, (s >> 2) & 1
+    fb = u ^ s1 ^ s2
+    return fb ^ s0 ^ s2, (fb << 2) | (s2 << 1) | s1
+
+
+def test_encoder_golden():
+    g = golden()
+    for k, f in [(40, 0), (64, 0), (544, 8), (1024, 0), (6144, 8), (6016, 24)]:
+        d = O.encode_cb(g[f"enc_k{k}_f{f}_in"], f)
+        assert np.array_equal(np.stack(d), g[f"enc_k{k}_f{f}_out"])
+
+
+@pytest.mark.parametrize("k", [40, 104, 1024])
+def test_encoder_parity_and_termination(k):
+    """Independent re-encode of the parity streams and of the tails: both
+    constituent encoders end in state 0 after their three tail steps, with the
+    tail bits multiplexed as lte_turbo.cpp:664-677 does."""
+    rng = np.random.default_rng(k)
+    cb = rng.integers(0, 2, k).astype(np.uint8)
+    d0, d1, d2 = O.encode_cb(cb)
+    perm = O.qpp_perm(k)
+    tails = []
+    for seq, par_stream in ((cb, d1), (cb[perm], d2)):
+        s = 0
+        for i, u in enumerate(seq):
+            p, s = _rsc(s, int(u))
+            assert p == par_stream[i]
+        xs, zs = [], []
+        for _ in range(3):
+            u = ((s >> 1) ^ (s >> 2)) & 1
+            p, s = _rsc(s, u)
+            xs.append(u)
+            zs.append(p)
+        assert s == 0
+        tails.append((xs, zs))
+    (xu, zu), (xl, zl) = tails
+    assert list(d0[k:]) == [xu[0], zu[1], xu[1], zu[2]]
+    assert list(d1[k:]) == [zu[0], xu[1], zu[2], xl[0]]
+    assert list(d2[k:]) == [zl[0], xl[1], zl[1], xl[2]]
+    assert np.array_equal(d0[:k], cb)
+
+
+# ---------------------------------------------------------- rate matching --
+def test_rate_matching_golden():
+    g = golden()
+    for ci, tbs, g_total, rv, r, k, e, f in g["tb_meta"]:
+        key = f"tb{ci}_rv{rv}_cb{r}"
+        cbb = g[f"tb{ci}_rv0_cb{r}_cbbits"]
+        d = O.encode_cb(cbb, int(f))
+        assert np.array_equal(O.rate_match(*d, int(f), int(e), int(rv)), g[key + "_rm"]), key
+        assert np.array_equal(O.derate_match(g[key + "_rm"], int(k) + 4, int(f), int(rv)), g[key + "_derm"]), key
+
+
+@pytest.mark.parametrize("k,f", [(40, 0), (544, 0), (6144, 0), (6144, 8)])
+def test_full_rate_visits_every_bit_once(k, f):
+    """At E = number of non-NULL circular-buffer entries and rv=0 each coded
+    bit is read exactly once, so de-rate matching returns +-8 everywhere except
+    the NULL filler positions of d0/d1."""
+    rng = np.random.default_rng(k + f)
+    cb = rng.integers(0, 2, k).astype(np.uint8)
+    cb[:f] = 0
+    d = O.encode_cb(cb, f)
+    e = 3 * (k + 4) - 2 * f
+    llr = O.derate_match(O.rate_match(*d, f, e, 0), k + 4, f, 0)
+    want = np.concatenate([8.0 * (2.0 * s.astype(np.float32) - 1.0) for s in d])
+    want[:f] = 0.0
+    want[k + 4:k + 4 + f] = 0.0
+    assert np.array_equal(llr, want)
+
+
+# ------------------------------------------------------------- decoder -------
+@pytest.mark.parametrize("k,seed", [(8, 1), (10, 2), (12, 3)])
+def test_siso_matches_brute_force(k, seed):
+    """Log-MAP SISO = exact APP from enumerating all 2^K input words, each
+    terminated by its forced tail (float64), minus systematic and a-priori."""
+    rng = np.random.default_rng(seed)
+    sys_ = rng.normal(0, 2, k).astype(np.float32)
+    par = rng.normal(0, 2, k + 4).astype(np.float32)
+    ap = rng.normal(0, 1, k).astype(np.float32)
+    ts = np.array([rng.normal(), rng.normal(), 0.0, 0.0], np.float32)
+    tp = np.append(rng.normal(0, 1, 3), 0.0).astype(np.float32)
+    ext = O.siso(sys_, par, ap, ts, tp)
+    app = O.brute_app(sys_, par[:k], ap, ts, tp)
+    np.testing.assert_allclose(ext, app - sys_ - ap, atol=2e-4)
+
+
+def _golden_decodes():
+    return [(40, 100, 6), (64, 0, 8), (544, 50, 6), (1024, 50, 8), (6144, 100, 8), (6144, 50, 8), (6144, 25, 8)]
+
+
+@pytest.mark.parametrize("k,eb,it", _golden_decodes())
+def test_decoder_golden_bit_exact(k, eb, it):
+    g = golden()
+    tag = f"dec_k{k}_eb{eb}_it{it}"
+    llr = g[tag + "_llr"]
+    post, _, used = O.decode_cb(llr[0], llr[1], llr[2], k, 0, it, False)
+    assert used == it
+    assert np.array_equal(post.view(np.uint32), g[tag + "_post"].view(np.uint32))
+
+
+def test_decoder_early_stop_golden():
+    g = golden()
+    llr = g["es_k1024_llr"]
+    post, ok, used = O.decode_cb(llr[0], llr[1], llr[2], 1024, 0, 8, True)
+    assert [int(ok), used] == list(g["es_k1024_meta"])
+    assert np.array_equal(post.view(np.uint32), g["es_k1024_post"].view(np.uint32))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("k", [40, 56, 544, 1952, 6144])
+def test_decoder_matches_compiled_reference(k):
+    rng = np.random.default_rng(k)
+    for ebn0 in (0.0, 0.75):
+        cb = rng.integers(0, 2, k).astype(np.uint8)
+        llr = bpsk_awgn_llr(O.encode_cb(cb), k, ebn0, rng)
+        for es in (False, True):
+            a = O.decode_cb(*llr, k, 0, 6, es)
+            b = O.ref_decode_cb(*llr, k, 0, 6, es)
+            assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+            assert a[1:] == b[1:]
+
+
+# ------------------------------------------------- windowed decoder oracle ---
+@pytest.mark.parametrize("k", [40, 544, 6144])
+@pytest.mark.parametrize("lin", [False, True])
+def test_window_single_segment_equals_exact(k, lin):
+    """One segment per block: no boundaries, so the windowed arithmetic must
+    reproduce the exact decoder up to float rounding."""
+    rng = np.random.default_rng(7 * k)
+    cb = rng.integers(0, 2, k).astype(np.uint8)
+    llr = bpsk_awgn_llr(O.encode_cb(cb), k, 0.75, rng)
+    exact = O.decode_cb(*llr, k, 0, 8, False)[0]
+    win = O.decode_window(*llr, k, k, 0, lin, 0, 8, False)[0]
+    np.testing.assert_allclose(win, exact, rtol=2e-5, atol=2e-4)
+
+
+def test_window_decoder_rejects_bad_geometry():
+    z = np.zeros(6148, np.float32)
+    with pytest.raises(ValueError):
+        O.decode_window(z, z, z, 6144, 100)  # does not divide K
+    with pytest.raises(ValueError):
+        O.decode_window(z, z, z, 6144, 96, 96)  # warm-up must stay below S

@@ -1,0 +1,384 @@
+// C ABI of the HIP decoders: plan construction (geometry, job list, device
+// work space), launches, and the per-code-block convenience entry point that
+// backs turbo_decode_code_block_gpu_* in the C++ API.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../host/codec_internal.h"
+#include "decoder_types.h"
+#include "turbo_cpp/c_api.h"
+
+namespace tcdev {
+__global__ void exact_decode_kernel(const CbInfo*, uint32_t, const float*, float*, float*, int32_t*, uint32_t*,
+                                    uint32_t, int32_t);
+__global__ void window_decode_log_kernel(const WinJob*, uint32_t, const uint32_t*, const CbInfo*, const uint32_t*,
+                                         const float*, float*, float2*, int32_t*, uint32_t*, uint32_t);
+__global__ void window_decode_lin_kernel(const WinJob*, uint32_t, const uint32_t*, const CbInfo*, const uint32_t*,
+                                         const float*, float*, float2*, int32_t*, uint32_t*, uint32_t);
+}  // namespace tcdev
+
+namespace {
+
+using turbo_cpp::detail::set_last_error;
+
+struct Failure {
+  tc_status status;
+  std::string what;
+};
+
+void hip_check(hipError_t e, const char* where) {
+  if (e != hipSuccess) throw Failure{TC_ERR_RUNTIME, std::string(where) + ": " + hipGetErrorString(e)};
+}
+
+template <typename F>
+tc_status guarded(F&& f) {
+  try {
+    f();
+    return TC_OK;
+  } catch (const Failure& e) {
+    set_last_error(e.what);
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    set_last_error("out of host memory");
+    return TC_ERR_RUNTIME;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return TC_ERR_RUNTIME;
+  }
+}
+
+void require_device() {
+  int n = 0;
+  const hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    throw Failure{TC_ERR_RUNTIME, "no HIP device available (the GPU backends need an MI355X / gfx950)"};
+}
+
+tc_options normalised(const tc_options* o) {
+  tc_options r{};
+  if (o) r = *o;
+  if (r.min_segment == 0) r.min_segment = 32;
+  if (r.warmup == 0) r.warmup = 32;
+  if (r.decoder < TC_DECODER_EXACT || r.decoder > TC_DECODER_WINDOW_LIN)
+    throw Failure{TC_ERR_INVALID_ARGUMENT, "unknown decoder"};
+  return r;
+}
+
+// Segment geometry of the window decoders: S = the smallest divisor of K
+// with S >= max(min_segment, ceil(K / 64)), so M = K / S <= 64 lanes.
+tc_segment_info geometry(uint32_t k, const tc_options& o) {
+  if (turbo_cpp::detail::qpp_index(k) < 0) throw Failure{TC_ERR_OUT_OF_RANGE, "unsupported QPP block size"};
+  const uint32_t lo = std::max<uint32_t>(o.min_segment, (k + 63) / 64);
+  uint32_t s = lo;
+  while (s < k && k % s != 0) ++s;
+  if (s > k) s = k;
+  if (s > static_cast<uint32_t>(tcdev::kMaxSeg))
+    throw Failure{TC_ERR_INVALID_ARGUMENT, "segment length above " + std::to_string(tcdev::kMaxSeg) + " for K=" +
+                                               std::to_string(k)};
+  tc_segment_info g{};
+  g.k = k;
+  g.segment_length = s;
+  g.segments = k / s;
+  g.blocks_per_wave = tcdev::kWaveLanes / g.segments;
+  g.warmup = (g.segments > 1) ? std::min(o.warmup, s - 1) : 0;
+  return g;
+}
+
+template <typename T>
+T* dev_alloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  hip_check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+  return static_cast<T*>(p);
+}
+
+template <typename T>
+void dev_upload(T* dst, const std::vector<T>& src) {
+  if (!src.empty()) hip_check(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+}  // namespace
+
+struct tc_plan {
+  tc_options opts{};
+  int device = 0;
+  size_t n_cb = 0;
+  size_t llr_floats = 0, post_floats = 0;
+  std::vector<tcdev::CbInfo> cbs;
+  // device
+  tcdev::CbInfo* d_cbs = nullptr;
+  // exact
+  float* d_work = nullptr;
+  // window
+  tcdev::WinJob* d_jobs = nullptr;
+  uint32_t* d_cb_index = nullptr;
+  uint32_t* d_qtab = nullptr;
+  float2* d_scratch = nullptr;
+  uint32_t njobs = 0, grid = 0;
+  size_t lds_bytes = 0;
+
+  ~tc_plan() {
+    for (void* p : {(void*)d_cbs, (void*)d_work, (void*)d_jobs, (void*)d_cb_index, (void*)d_qtab, (void*)d_scratch})
+      if (p) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n, const tc_options& o) {
+  require_device();
+  hip_check(hipGetDevice(&p.device), "hipGetDevice");
+  p.opts = o;
+  p.n_cb = n;
+  p.cbs.resize(n);
+  size_t llr = 0, post = 0, work = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const int qi = turbo_cpp::detail::qpp_index(k[i]);
+    if (qi < 0) throw Failure{TC_ERR_OUT_OF_RANGE, "unsupported QPP block size"};
+    const uint32_t f = filler ? filler[i] : 0;
+    if (f >= k[i]) throw Failure{TC_ERR_INVALID_ARGUMENT, "filler count must be below K"};
+    auto& c = p.cbs[i];
+    c.k = k[i];
+    c.filler = f;
+    c.f1 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f1);
+    c.f2 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f2);
+    c.llr_off = llr;
+    c.post_off = post;
+    c.work_off = work;
+    llr += 3 * (size_t)(k[i] + 4);
+    post += k[i];
+    work += 8 * (size_t)(k[i] + 1) + 4 * (size_t)k[i];
+  }
+  p.llr_floats = llr;
+  p.post_floats = post;
+  p.d_cbs = dev_alloc<tcdev::CbInfo>(n);
+  dev_upload(p.d_cbs, p.cbs);
+  if (o.decoder == TC_DECODER_EXACT) {
+    p.d_work = dev_alloc<float>(work);
+    return;
+  }
+  // ---- window decoders: group by K, pack blocks into 64-lane jobs ----
+  std::map<uint32_t, std::vector<uint32_t>> by_k;
+  for (size_t i = 0; i < n; ++i) by_k[k[i]].push_back(static_cast<uint32_t>(i));
+  std::vector<tcdev::WinJob> jobs;
+  std::vector<uint32_t> cb_index, qtab;
+  uint32_t max_seg = 1;
+  for (const auto& [kk, list] : by_k) {
+    const tc_segment_info g = geometry(kk, o);
+    const int qi = turbo_cpp::detail::qpp_index(kk);
+    const uint32_t f1 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f1);
+    const uint32_t f2 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f2);
+    const uint32_t qoff = static_cast<uint32_t>(qtab.size());
+    for (uint32_t t = 0; t < g.segment_length; ++t) {
+      const uint64_t pi = (static_cast<uint64_t>(f1) * t + static_cast<uint64_t>(f2) * t * t) % kk;
+      const uint32_t r = static_cast<uint32_t>(pi % g.segment_length), c0 = static_cast<uint32_t>(pi / g.segment_length);
+      qtab.push_back((r << 16) | c0);
+    }
+    max_seg = std::max(max_seg, g.segment_length);
+    for (size_t s = 0; s < list.size(); s += g.blocks_per_wave) {
+      tcdev::WinJob j{};
+      j.k = kk;
+      j.seg = g.segment_length;
+      j.nseg = g.segments;
+      j.nblk = static_cast<uint32_t>(std::min<size_t>(g.blocks_per_wave, list.size() - s));
+      j.warm = g.warmup;
+      j.f1 = f1;
+      j.f2 = f2;
+      j.cb_off = static_cast<uint32_t>(cb_index.size());
+      j.qtab_off = qoff;
+      j.div_magic = static_cast<uint32_t>(((1ull << 32) + g.segment_length - 1) / g.segment_length);
+      for (uint32_t b = 0; b < j.nblk; ++b) cb_index.push_back(list[s + b]);
+      jobs.push_back(j);
+    }
+  }
+  // longest segments first: a job's time grows with S + 2g
+  std::stable_sort(jobs.begin(), jobs.end(), [](const tcdev::WinJob& a, const tcdev::WinJob& b) {
+    return a.seg + 2 * a.warm > b.seg + 2 * b.warm;
+  });
+  p.njobs = static_cast<uint32_t>(jobs.size());
+  p.lds_bytes = static_cast<size_t>(max_seg) * tcdev::kLdsRow * sizeof(float);
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
+  int per_cu = 0;
+  const void* fn = (o.decoder == TC_DECODER_WINDOW_LOG) ? (const void*)tcdev::window_decode_log_kernel
+                                                        : (const void*)tcdev::window_decode_lin_kernel;
+  hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, tcdev::kWaveLanes, p.lds_bytes),
+            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  per_cu = std::max(per_cu, 1);
+  p.grid = std::max<uint32_t>(1, std::min<uint32_t>(p.njobs, static_cast<uint32_t>(per_cu) * prop.multiProcessorCount));
+  p.d_jobs = dev_alloc<tcdev::WinJob>(jobs.size());
+  dev_upload(p.d_jobs, jobs);
+  p.d_cb_index = dev_alloc<uint32_t>(cb_index.size());
+  dev_upload(p.d_cb_index, cb_index);
+  p.d_qtab = dev_alloc<uint32_t>(qtab.size());
+  dev_upload(p.d_qtab, qtab);
+  p.d_scratch = dev_alloc<float2>(static_cast<size_t>(p.grid) * tcdev::kSlotFloat2);
+}
+
+void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* iters, hipStream_t s) {
+  if (p.n_cb == 0) return;
+  const tc_options& o = p.opts;
+  if (o.decoder == TC_DECODER_EXACT) {
+    const uint32_t blocks = static_cast<uint32_t>((p.n_cb + 63) / 64);
+    hipLaunchKernelGGL(tcdev::exact_decode_kernel, dim3(blocks), dim3(64), 0, s, p.d_cbs,
+                       static_cast<uint32_t>(p.n_cb), llr, post, p.d_work, crc, iters, o.max_iterations,
+                       o.early_stop);
+  } else if (o.decoder == TC_DECODER_WINDOW_LOG) {
+    hipLaunchKernelGGL(tcdev::window_decode_log_kernel, dim3(p.grid), dim3(64), p.lds_bytes, s, p.d_jobs, p.njobs,
+                       p.d_cb_index, p.d_cbs, p.d_qtab, llr, post, p.d_scratch, crc, iters, o.max_iterations);
+  } else {
+    hipLaunchKernelGGL(tcdev::window_decode_lin_kernel, dim3(p.grid), dim3(64), p.lds_bytes, s, p.d_jobs, p.njobs,
+                       p.d_cb_index, p.d_cbs, p.d_qtab, llr, post, p.d_scratch, crc, iters, o.max_iterations);
+  }
+  hip_check(hipGetLastError(), "kernel launch");
+}
+
+void check_early_stop_support(const tc_options& o) {
+  if (o.early_stop && o.decoder != TC_DECODER_EXACT)
+    throw Failure{TC_ERR_INVALID_ARGUMENT, "early stop is implemented for the exact decoder only"};
+}
+
+// Per-thread cache of single-block plans and staging buffers, so repeated
+// per-code-block calls do not allocate.
+struct OneBlockCache {
+  std::map<std::tuple<uint32_t, uint32_t, int32_t, uint32_t, uint32_t, uint32_t, int32_t>, std::unique_ptr<tc_plan>> plans;
+  float* d_llr = nullptr;
+  float* d_post = nullptr;
+  int32_t* d_crc = nullptr;
+  uint32_t* d_it = nullptr;
+  size_t cap = 0;
+  ~OneBlockCache() {
+    for (void* q : {(void*)d_llr, (void*)d_post, (void*)d_crc, (void*)d_it})
+      if (q) (void)hipFree(q);
+  }
+};
+thread_local OneBlockCache g_one;
+
+}  // namespace
+
+extern "C" {
+
+tc_status tc_device_count(int32_t* count) {
+  return guarded([&] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+tc_status tc_segment_geometry(uint32_t k, const tc_options* opts, tc_segment_info* out) {
+  return guarded([&] { *out = geometry(k, normalised(opts)); });
+}
+
+tc_status tc_plan_create(const uint32_t* k, const uint32_t* filler, size_t n_cb, const tc_options* opts,
+                         tc_plan** plan) {
+  return guarded([&] {
+    if (!plan || (n_cb && !k)) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    const tc_options o = normalised(opts);
+    check_early_stop_support(o);
+    auto p = std::make_unique<tc_plan>();
+    build_plan(*p, k, filler, n_cb, o);
+    *plan = p.release();
+  });
+}
+
+void tc_plan_destroy(tc_plan* plan) { delete plan; }
+size_t tc_plan_llr_floats(const tc_plan* plan) { return plan ? plan->llr_floats : 0; }
+size_t tc_plan_post_floats(const tc_plan* plan) { return plan ? plan->post_floats : 0; }
+
+tc_status tc_plan_decode(tc_plan* plan, const float* llr_dev, float* post_dev, int32_t* crc_ok_dev,
+                         uint32_t* iters_dev, void* hip_stream) {
+  return guarded([&] {
+    if (!plan || (plan->n_cb && (!llr_dev || !post_dev))) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    launch(*plan, llr_dev, post_dev, crc_ok_dev, iters_dev, static_cast<hipStream_t>(hip_stream));
+  });
+}
+
+tc_status tc_decode_batch_host(const uint32_t* k, const uint32_t* filler, size_t n_cb, const tc_options* opts,
+                               const float* llr, float* post, int32_t* crc_ok, uint32_t* iterations_used) {
+  return guarded([&] {
+    const tc_options o = normalised(opts);
+    check_early_stop_support(o);
+    tc_plan p;
+    build_plan(p, k, filler, n_cb, o);
+    if (n_cb == 0) return;
+    float* d_llr = dev_alloc<float>(p.llr_floats);
+    float* d_post = dev_alloc<float>(p.post_floats);
+    int32_t* d_crc = dev_alloc<int32_t>(n_cb);
+    uint32_t* d_it = dev_alloc<uint32_t>(n_cb);
+    try {
+      hip_check(hipMemcpy(d_llr, llr, p.llr_floats * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
+      launch(p, d_llr, d_post, d_crc, d_it, nullptr);
+      hip_check(hipDeviceSynchronize(), "decode");
+      hip_check(hipMemcpy(post, d_post, p.post_floats * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+      if (crc_ok) hip_check(hipMemcpy(crc_ok, d_crc, n_cb * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+      if (iterations_used)
+        hip_check(hipMemcpy(iterations_used, d_it, n_cb * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    } catch (...) {
+      (void)hipFree(d_llr), (void)hipFree(d_post), (void)hipFree(d_crc), (void)hipFree(d_it);
+      throw;
+    }
+    (void)hipFree(d_llr), (void)hipFree(d_post), (void)hipFree(d_crc), (void)hipFree(d_it);
+  });
+}
+
+tc_status tc_decode_code_block(const tc_cb_desc* desc, const float* d0, size_t n0, const float* d1, size_t n1,
+                               const float* d2, size_t n2, const tc_options* opts, float* posterior,
+                               int32_t* crc_ok, uint32_t* iterations_used) {
+  return guarded([&] {
+    if (!desc || !d0 || !d1 || !d2 || !posterior) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    // lte_turbo_cuda.cu:1400-1402 / lte_turbo.cpp:744-747: spans must be d_r long
+    if (n0 != desc->d_r || n1 != desc->d_r || n2 != desc->d_r)
+      throw Failure{TC_ERR_INVALID_ARGUMENT, "invalid LLR sizes for turbo decode"};
+    const uint32_t k = static_cast<uint32_t>(desc->k_r);
+    if (turbo_cpp::detail::qpp_index(k) < 0) throw Failure{TC_ERR_OUT_OF_RANGE, "unsupported QPP block size"};
+    if (desc->d_r < desc->k_r + 4) throw Failure{TC_ERR_INVALID_ARGUMENT, "d_r must be at least K + 4"};
+    const tc_options o = normalised(opts);
+    check_early_stop_support(o);
+    require_device();
+    const uint32_t f = static_cast<uint32_t>(desc->filler_count);
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    auto key = std::make_tuple(k, f, o.decoder, o.max_iterations, o.min_segment, o.warmup * 2 + (o.early_stop != 0), dev);
+    auto& slot = g_one.plans[key];
+    if (!slot) {
+      slot = std::make_unique<tc_plan>();
+      build_plan(*slot, &k, &f, 1, o);
+    }
+    const size_t need = 3 * (size_t)(k + 4);
+    if (g_one.cap < need) {
+      for (void* q : {(void*)g_one.d_llr, (void*)g_one.d_post})
+        if (q) (void)hipFree(q);
+      g_one.d_llr = dev_alloc<float>(need);
+      g_one.d_post = dev_alloc<float>(need);
+      g_one.cap = need;
+    }
+    if (!g_one.d_crc) g_one.d_crc = dev_alloc<int32_t>(1);
+    if (!g_one.d_it) g_one.d_it = dev_alloc<uint32_t>(1);
+    // only the first K+4 entries of each stream are read
+    hip_check(hipMemcpy(g_one.d_llr, d0, (k + 4) * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    hip_check(hipMemcpy(g_one.d_llr + (k + 4), d1, (k + 4) * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    hip_check(hipMemcpy(g_one.d_llr + 2 * (k + 4), d2, (k + 4) * sizeof(float), hipMemcpyHostToDevice),
+              "hipMemcpy H2D");
+    launch(*slot, g_one.d_llr, g_one.d_post, g_one.d_crc, g_one.d_it, nullptr);
+    hip_check(hipDeviceSynchronize(), "decode");
+    hip_check(hipMemcpy(posterior, g_one.d_post, k * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    int32_t ok = 0;
+    uint32_t it = 0;
+    hip_check(hipMemcpy(&ok, g_one.d_crc, sizeof ok, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    hip_check(hipMemcpy(&it, g_one.d_it, sizeof it, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    if (crc_ok) *crc_ok = ok;
+    if (iterations_used) *iterations_used = it;
+  });
+}
+
+}  // extern "C"

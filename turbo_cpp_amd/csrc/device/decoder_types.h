@@ -1,0 +1,47 @@
+// Plain structures shared by the host plan builder and the decoder kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace tcdev {
+
+// One code block of a batch.  llr_off: start of [d0|d1|d2] (3*(K+4) floats);
+// post_off: start of its K posterior LLRs; work_off: start of its private
+// work space (exact decoder only).
+struct CbInfo {
+  uint64_t llr_off;
+  uint64_t post_off;
+  uint64_t work_off;
+  uint32_t k;
+  uint32_t filler;
+  uint32_t f1;
+  uint32_t f2;
+};
+
+// Window decoder: a job is what one 64-lane wavefront decodes at a time:
+// nblk code blocks of the same K, each split into M segments of length S
+// (lane = block * M + segment).
+constexpr int kWaveLanes = 64;
+constexpr int kSubWin = 8;       // W: sub-window of the beta recompute
+constexpr int kMaxSubWins = 24;  // nW <= 24, so S <= 192
+constexpr int kMaxSeg = kSubWin * kMaxSubWins;
+constexpr int kLdsRow = kWaveLanes + 1;  // LDS row stride (odd: conflict-free columns)
+// Per-wave scratch slot, in float2: staged channel LLRs for both halves
+// ([t][lane], S <= kMaxSeg), then in floats the boundary metrics
+// [half][16][lane] and the beta checkpoints [w][8][lane].
+constexpr int kSlotFloat2 = 2 * kMaxSeg * kWaveLanes + 16 * kWaveLanes + 4 * kMaxSubWins * kWaveLanes;
+
+struct WinJob {
+  uint32_t k;
+  uint32_t seg;      // S
+  uint32_t nseg;     // M
+  uint32_t nblk;     // code blocks in this job
+  uint32_t warm;     // warm-up steps g (0 when M == 1)
+  uint32_t f1, f2;
+  uint32_t cb_off;   // first entry in the job's code-block index list
+  uint32_t qtab_off; // first entry of the per-K table of Pi(t) for t < S
+  uint32_t div_magic;// ceil(2^32 / S) for exact i / S with i < 2^16
+  uint32_t pad0, pad1;
+};
+
+}  // namespace tcdev

@@ -1,0 +1,511 @@
+// Window-parallel turbo decoder for gfx950: the throughput path.
+//
+// Work decomposition.  A job is what one 64-lane wavefront decodes from start
+// to finish (all iterations): nblk code blocks of one K, each cut into M
+// segments of S = K / M steps; lane = block * M + segment.  Each lane runs
+// both SISO halves of every iteration over its own segment, lane-serially,
+// with all 8 state metrics in VGPRs.
+//
+// Segment boundaries.  A segment's initial alpha (beta) comes from a g-step
+// warm-up recursion over the neighbouring segment's last (first) g steps,
+// itself started from the metrics that neighbour exported in the previous
+// iteration (next-iteration initialisation).  Segment 0 starts from
+// alpha_0 = state 0 and the last segment from the reference's tail beta
+// (lte_turbo.cpp:357-378, tail read-out :758-773), so M == 1 is a plain
+// full-length pass.  The exported metrics move between lanes with DPP
+// wave_shr:1 / wave_shl:1.
+//
+// Memory.  The wavefront's extrinsics live in LDS for the whole decode, one
+// float per bit in a transposed layout: bit i = q*S + t of block b sits at
+// row t, column b*M + q (row stride 65).  The upper half reads and writes
+// its own column; the lower half reads and writes bit pi(j) for its steps
+// j = q*S + t.  Because S divides K, pi(q*S + t) mod S = pi(t) mod S for
+// every q (the QPP is contention-free), so at each step all lanes hit one
+// row and distinct columns: no bank conflicts.  Row pi(t) mod S and the
+// column offset pi(t) div S come from a small per-K table; the per-lane
+// part of the column is (A_q + B_q t) mod M with A_q = (f1 q + f2 q^2 S)
+// mod M and B_q = 2 f2 q mod M.  The channel LLRs (d0, d1 for the upper
+// half; d0[pi(j)], d2 for the lower) are staged once per job into a
+// per-wave scratch area in [t][lane] order, so every per-step load of a
+// wavefront is one contiguous 512-byte line.
+//
+// Within a segment the backward pass stores beta only at sub-window
+// boundaries (every 8 steps, in the wave's scratch slot); the forward pass
+// recomputes each sub-window's beta into registers before computing alpha
+// and the extrinsics.  The
+// recomputation repeats the same instructions on the same inputs, so it is
+// bit-identical to the first pass.
+//
+// Arithmetic: either max* in the log domain or the probability domain with
+// power-of-two renormalisation; oracle/turbo_oracle.c (or_decode_window)
+// states both step by step and the test suite compares the two.
+#include <hip/hip_runtime.h>
+
+#include "decoder_types.h"
+#include "trellis.h"
+
+#pragma clang fp contract(off)
+
+namespace tcdev {
+namespace {
+
+__device__ __forceinline__ float dpp_from_prev_lane(float v) {  // wave_shr:1
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_from_next_lane(float v) {  // wave_shl:1
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float max8(const float* v) {
+  return fmaxf(fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])), fmaxf(fmaxf(v[4], v[5]), fmaxf(v[6], v[7])));
+}
+
+// ------------------------------------------------------------ log domain --
+struct LogDomain {
+  static constexpr float kUniform = 0.0f;
+  static constexpr float kZero = kNeg;
+  __device__ static __forceinline__ float from_log(float v) { return v; }
+
+  __device__ static __forceinline__ float maxstar(float a, float b) {
+    const float m = fmaxf(a, b);
+    const float d = fabsf(a - b);
+    return m + kLn2 * hw_log2(1.0f + hw_exp2(-d * kLog2e));
+  }
+  __device__ static __forceinline__ void norm(float* v) {
+    const float m = max8(v);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = v[i] - m;
+  }
+  // beta_t from beta_{t+1}
+  __device__ static __forceinline__ void bstep(float* b, float sa, float par) {
+    const float x = 0.5f * sa, y = 0.5f * par;
+    const float g1 = x + y, g2 = x - y;
+    float nb[8];
+#pragma unroll
+    for (unsigned p = 0; p < 8; ++p) {
+      const Tr t0 = rsc(p, 0), t1 = rsc(p, 1);
+      nb[p] = maxstar(b[t0.next] + gsel(0, t0.parity, g1, g2), b[t1.next] + gsel(1, t1.parity, g1, g2));
+    }
+    norm(nb);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b[i] = nb[i];
+  }
+  // alpha_{t+1} from alpha_t; returns the extrinsic of step t when WANT
+  template <bool WANT>
+  __device__ static __forceinline__ float astep(float* a, float sa, float par, const float* bn) {
+    const float x = 0.5f * sa, y = 0.5f * par;
+    const float g1 = x + y, g2 = x - y;
+    float na[8];
+#pragma unroll
+    for (unsigned n = 0; n < 8; ++n) {
+      const unsigned u = in_of(n), p = pred0(n), c = par0(n);
+      na[n] = maxstar(a[p] + gsel(u, c, g1, g2), a[p + 1] + gsel(u, c ^ 1u, g1, g2));
+    }
+    float ext = 0.0f;
+    if constexpr (WANT) {
+      float tv[8];
+#pragma unroll
+      for (int n = 0; n < 8; ++n) tv[n] = na[n] + bn[n];
+      // u(n) = 1 for n in {1,2,4,7}, 0 for {0,3,5,6}
+      const float m1 = fmaxf(fmaxf(fmaxf(fmaxf(kNeg, tv[1]), tv[2]), tv[4]), tv[7]);
+      const float m0 = fmaxf(fmaxf(fmaxf(fmaxf(kNeg, tv[0]), tv[3]), tv[5]), tv[6]);
+      float s1 = 0.0f, s0 = 0.0f;
+      s0 += hw_exp2((tv[0] - m0) * kLog2e);
+      s1 += hw_exp2((tv[1] - m1) * kLog2e);
+      s1 += hw_exp2((tv[2] - m1) * kLog2e);
+      s0 += hw_exp2((tv[3] - m0) * kLog2e);
+      s1 += hw_exp2((tv[4] - m1) * kLog2e);
+      s0 += hw_exp2((tv[5] - m0) * kLog2e);
+      s0 += hw_exp2((tv[6] - m0) * kLog2e);
+      s1 += hw_exp2((tv[7] - m1) * kLog2e);
+      const float l1 = m1 + kLn2 * hw_log2(s1);
+      const float l0 = m0 + kLn2 * hw_log2(s0);
+      ext = (l1 - l0) - sa;
+    }
+    norm(na);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = na[i];
+    return ext;
+  }
+};
+
+// ---------------------------------------------------- probability domain --
+struct LinDomain {
+  static constexpr float kUniform = 1.0f;
+  static constexpr float kZero = 0.0f;
+  __device__ static __forceinline__ float from_log(float v) { return hw_exp2(v * kLog2e); }
+
+  // branch weights exp(gamma - |x| - |y|) = U_u * Q_c
+  __device__ static __forceinline__ void weights(float sa, float par, float& u1, float& u0, float& q1, float& q0) {
+    const float x = 0.5f * sa, y = 0.5f * par;
+    const float ex = hw_exp2((-2.0f * fabsf(x)) * kLog2e);
+    const float ey = hw_exp2((-2.0f * fabsf(y)) * kLog2e);
+    u1 = (x >= 0.0f) ? 1.0f : ex;
+    u0 = (x >= 0.0f) ? ex : 1.0f;
+    q1 = (y >= 0.0f) ? 1.0f : ey;
+    q0 = (y >= 0.0f) ? ey : 1.0f;
+  }
+  // rescale by 2^-e so that the largest entry lies in [0.5, 1)
+  __device__ static __forceinline__ void norm(float* v) {
+    const float m = max8(v);
+    if (m > 0.0f) {
+      const int e = __builtin_amdgcn_frexp_expf(m);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = __builtin_amdgcn_ldexpf(v[i], -e);
+    }
+  }
+  __device__ static __forceinline__ void bstep(float* b, float sa, float par) {
+    float u1, u0, q1, q0;
+    weights(sa, par, u1, u0, q1, q0);
+    const float w[2][2] = {{u0 * q0, u0 * q1}, {u1 * q0, u1 * q1}};
+    float nb[8];
+#pragma unroll
+    for (unsigned p = 0; p < 8; ++p) {
+      const Tr t0 = rsc(p, 0), t1 = rsc(p, 1);
+      nb[p] = __builtin_fmaf(w[1][t1.parity], b[t1.next], w[0][t0.parity] * b[t0.next]);
+    }
+    norm(nb);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b[i] = nb[i];
+  }
+  template <bool WANT>
+  __device__ static __forceinline__ float astep(float* a, float sa, float par, const float* bn) {
+    float u1, u0, q1, q0;
+    weights(sa, par, u1, u0, q1, q0);
+    const float q[2] = {q0, q1}, uu[2] = {u0, u1};
+    float na[8];
+    float s1 = 0.0f, s0 = 0.0f;
+#pragma unroll
+    for (unsigned n = 0; n < 8; ++n) {
+      const unsigned u = in_of(n), p = pred0(n), c = par0(n);
+      const float bb = __builtin_fmaf(a[p + 1], q[c ^ 1u], a[p] * q[c]);
+      if constexpr (WANT) {
+        if (u) s1 = __builtin_fmaf(bb, bn[n], s1);
+        else s0 = __builtin_fmaf(bb, bn[n], s0);
+      }
+      na[n] = uu[u] * bb;
+    }
+    float ext = 0.0f;
+    if constexpr (WANT) {
+      s1 = fmaxf(s1, kFltMin);
+      s0 = fmaxf(s0, kFltMin);
+      ext = kLn2 * (hw_log2(s1) - hw_log2(s0));
+    }
+    norm(na);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = na[i];
+    return ext;
+  }
+};
+
+// Tail beta in the log domain, the reference's rules (lte_turbo.cpp:357-378):
+// every state's forced three-step path to state 0, renormalised to max 0.
+__device__ void tail_beta(float ts0, float ts1, float ts2, float tp0, float tp1, float tp2, float* beta) {
+  const float ts[3] = {ts0, ts1, ts2}, tp[3] = {tp0, tp1, tp2};
+#pragma unroll
+  for (unsigned s = 0; s < 8; ++s) {
+    unsigned cur = s;
+    float metric = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const unsigned u = term_in(cur);
+      const Tr tr = rsc(cur, u);
+      metric += 0.5f * ((u ? 1.0f : -1.0f) * ts[t]) + 0.5f * ((tr.parity ? 1.0f : -1.0f) * tp[t]);
+      cur = tr.next;
+    }
+    beta[s] = metric;  // every state reaches 0 in three forced steps
+  }
+  const float m = max8(beta);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) beta[i] -= m;
+}
+
+struct Lane {
+  uint32_t S, M, g, nW, K;
+  uint32_t q, colb, le;
+  bool first, last, active;
+  uint32_t f1, f2, qtab_off;
+};
+
+// (A_q + B_q t) mod M: the per-lane column offset of pi(q S + t).
+__device__ __forceinline__ uint32_t qpp_col0(const Lane& L, uint32_t qq, uint32_t t) {
+  const uint32_t A = L.f1 * qq + L.f2 * qq * qq * L.S;
+  const uint32_t B = 2u * L.f2 * qq;
+  return (A % L.M + (B % L.M) * t) % L.M;
+}
+__device__ __forceinline__ uint32_t qpp_stride(const Lane& L, uint32_t qq) { return (2u * L.f2 * qq) % L.M; }
+
+template <int H>
+__device__ __forceinline__ uint32_t lds_addr(const Lane& L, const uint32_t* __restrict__ qtab, uint32_t qq,
+                                             uint32_t t, uint32_t y) {
+  if constexpr (H == 0) {
+    return t * kLdsRow + L.colb + qq;
+  } else {
+    const uint32_t u = qtab[L.qtab_off + t];
+    uint32_t c = (u & 0xFFFFu) + y;
+    c = (c >= L.M) ? c - L.M : c;
+    return (u >> 16) * kLdsRow + L.colb + c;
+  }
+}
+__device__ __forceinline__ uint32_t y_back(uint32_t y, uint32_t b, uint32_t m) { return (y >= b) ? y - b : y + m - b; }
+__device__ __forceinline__ uint32_t y_fwd(uint32_t y, uint32_t b, uint32_t m) {
+  y += b;
+  return (y >= m) ? y - m : y;
+}
+
+// One SISO half (H = 0 upper, 1 lower) for this lane's segment.
+// nii: this half's boundary metrics in [alpha 8 | beta 8][lane] order
+// (alpha at step -g, beta at step S+g of the lane's segment); ck: the
+// sub-window checkpoints of beta, [w][8][lane].
+template <class D, int H>
+__device__ __forceinline__ void siso_half(const Lane& L, float* __restrict__ lds, const float2* __restrict__ sc,
+                                          const uint32_t* __restrict__ qtab, float* __restrict__ nii,
+                                          float* __restrict__ ck, bool write_post) {
+  const uint32_t S = L.S, M = L.M, g = L.g;
+  const uint32_t lane = threadIdx.x;
+  // ---------------- backward pass: warm-up over the next segment, then the
+  // segment itself, storing beta at sub-window boundaries
+  float bc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) bc[i] = nii[(8 + i) * kWaveLanes + lane];
+  if (g > 0) {
+    const uint32_t qr = L.last ? L.q : L.q + 1;
+    const int dl = L.last ? 0 : 1;
+    uint32_t y = (H == 1) ? qpp_col0(L, qr, g - 1) : 0u;
+    const uint32_t by = (H == 1) ? qpp_stride(L, qr) : 0u;
+    for (int t = (int)g - 1; t >= 0; --t) {
+      const float2 v = sc[t * kWaveLanes + L.le + dl];
+      const float e = lds[lds_addr<H>(L, qtab, qr, (uint32_t)t, y)];
+      D::bstep(bc, v.x + e, v.y);
+      if constexpr (H == 1) y = y_back(y, by, M);
+    }
+    if (L.last) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) bc[i] = nii[(8 + i) * kWaveLanes + lane];
+    }
+  }
+  float bexp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  {
+    uint32_t y = (H == 1) ? qpp_col0(L, L.q, S - 1) : 0u;
+    const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
+    for (int w = (int)L.nW - 1; w >= 0; --w) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) ck[(w * 8 + i) * kWaveLanes + lane] = bc[i];
+      const uint32_t t0 = w * kSubWin;
+      const uint32_t len = min((uint32_t)kSubWin, S - t0);
+#pragma unroll
+      for (int j = kSubWin - 1; j >= 0; --j) {
+        if ((uint32_t)j < len) {
+          const uint32_t t = t0 + j;
+          const float2 v = sc[t * kWaveLanes + L.le];
+          const float e = lds[lds_addr<H>(L, qtab, L.q, t, y)];
+          D::bstep(bc, v.x + e, v.y);
+          if constexpr (H == 1) y = y_back(y, by, M);
+          if (t == g) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) bexp[i] = bc[i];
+          }
+        }
+      }
+    }
+  }
+  // hand beta at step g to the previous segment (used next iteration)
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const float nb = dpp_from_next_lane(bexp[i]);
+    if (!L.last) nii[(8 + i) * kWaveLanes + lane] = nb;
+  }
+  // ---------------- forward pass: warm-up over the previous segment, then
+  // per sub-window: recompute beta, alpha + extrinsic
+  float ac[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) ac[i] = nii[i * kWaveLanes + lane];
+  if (g > 0) {
+    const uint32_t ql = L.first ? L.q : L.q - 1;
+    const int dl = L.first ? 0 : -1;
+    uint32_t y = (H == 1) ? qpp_col0(L, ql, S - g) : 0u;
+    const uint32_t by = (H == 1) ? qpp_stride(L, ql) : 0u;
+    for (uint32_t t = S - g; t < S; ++t) {
+      const float2 v = sc[t * kWaveLanes + L.le + dl];
+      const float e = lds[lds_addr<H>(L, qtab, ql, t, y)];
+      D::template astep<false>(ac, v.x + e, v.y, nullptr);
+      if constexpr (H == 1) y = y_fwd(y, by, M);
+    }
+    if (L.first) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) ac[i] = nii[i * kWaveLanes + lane];
+    }
+  }
+  float aexp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
+  for (uint32_t w = 0; w < L.nW; ++w) {
+    const uint32_t t0 = w * kSubWin;
+    const uint32_t len = min((uint32_t)kSubWin, S - t0);
+    float bs[kSubWin][8];
+    float isa[kSubWin], ipar[kSubWin];
+    uint32_t addr[kSubWin];
+    float bb[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) bb[i] = ck[(w * 8 + i) * kWaveLanes + lane];
+    uint32_t y = (H == 1) ? qpp_col0(L, L.q, t0 + len - 1) : 0u;
+#pragma unroll
+    for (int j = kSubWin - 1; j >= 0; --j) {
+      if ((uint32_t)j < len) {
+        const uint32_t t = t0 + j;
+        addr[j] = lds_addr<H>(L, qtab, L.q, t, y);
+        const float2 v = sc[t * kWaveLanes + L.le];
+        isa[j] = v.x + lds[addr[j]];
+        ipar[j] = v.y;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) bs[j][i] = bb[i];
+        if (j > 0) D::bstep(bb, isa[j], ipar[j]);
+        if constexpr (H == 1) y = y_back(y, by, M);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kSubWin; ++j) {
+      if ((uint32_t)j < len) {
+        const uint32_t t = t0 + j;
+        if (t == S - g) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) aexp[i] = ac[i];
+        }
+        const float ext = D::template astep<true>(ac, isa[j], ipar[j], bs[j]);
+        const float val = (H == 1 && write_post) ? isa[j] + ext : ext;
+        if (L.active) lds[addr[j]] = val;
+      }
+    }
+  }
+  // hand alpha at step S-g to the next segment
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const float na = dpp_from_prev_lane(aexp[i]);
+    if (!L.first) nii[i * kWaveLanes + lane] = na;
+  }
+}
+
+template <class D>
+__device__ void window_decode(const WinJob* __restrict__ jobs, uint32_t njobs, const uint32_t* __restrict__ cb_index,
+                              const CbInfo* __restrict__ cbs, const uint32_t* __restrict__ qtab,
+                              const float* __restrict__ llr, float* __restrict__ post, float2* __restrict__ scratch,
+                              int32_t* __restrict__ crc_ok, uint32_t* __restrict__ iters, uint32_t max_iterations) {
+  extern __shared__ float lds[];
+  const uint32_t lane = threadIdx.x;
+  float2* sc_up = scratch + (size_t)blockIdx.x * kSlotFloat2;
+  float2* sc_lo = sc_up + kMaxSeg * kWaveLanes;
+  float* nii = reinterpret_cast<float*>(sc_lo + kMaxSeg * kWaveLanes);  // [half][16][64]
+  float* ck = nii + 2 * 16 * kWaveLanes;                                 // [w][8][64]
+  for (uint32_t jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
+    const WinJob J = jobs[jb];
+    Lane L;
+    L.S = J.seg;
+    L.M = J.nseg;
+    L.g = J.warm;
+    L.K = J.k;
+    L.nW = (J.seg + kSubWin - 1) / kSubWin;
+    L.f1 = J.f1;
+    L.f2 = J.f2;
+    L.qtab_off = J.qtab_off;
+    const uint32_t used = J.nblk * J.nseg;
+    L.active = lane < used;
+    L.le = L.active ? lane : used - 1;  // idle lanes shadow the last busy lane
+    const uint32_t b = L.le / L.M;
+    L.q = L.le - b * L.M;
+    L.colb = b * L.M;
+    L.first = L.q == 0;
+    L.last = L.q == L.M - 1;
+    const uint32_t K = J.k, S = J.seg, M = J.nseg;
+    const uint32_t my_cb = cb_index[J.cb_off + b];
+    const CbInfo ci = cbs[my_cb];
+    const float* d0 = llr + ci.llr_off;
+    const float* d1 = d0 + (K + 4);
+    const float* d2 = d1 + (K + 4);
+
+    // ---- stage d0 of every block into LDS (transposed) ----
+    for (uint32_t bb = 0; bb < J.nblk; ++bb) {
+      const float* src = llr + cbs[cb_index[J.cb_off + bb]].llr_off;
+      for (uint32_t i = lane; i < K; i += kWaveLanes) {
+        const uint32_t qq = __umulhi(i, J.div_magic);
+        const uint32_t t = i - qq * S;
+        lds[t * kLdsRow + bb * M + qq] = src[i];
+      }
+    }
+    __syncthreads();
+    // ---- per-wave scratch in [t][lane] order: (d0, d1) and (d0[pi], d2) ----
+    {
+      uint32_t y = qpp_col0(L, L.q, 0);
+      const uint32_t by = qpp_stride(L, L.q);
+      for (uint32_t t = 0; t < S; ++t) {
+        const float xs = lds[t * kLdsRow + L.colb + L.q];
+        const float xp = lds[lds_addr<1>(L, qtab, L.q, t, y)];
+        const uint32_t i = L.q * S + t;
+        const float p1 = d1[i], p2 = d2[i];
+        if (L.active) {
+          sc_up[t * kWaveLanes + lane] = make_float2(xs, p1);
+          sc_lo[t * kWaveLanes + lane] = make_float2(xp, p2);
+        }
+        y = y_fwd(y, by, M);
+      }
+    }
+    // ---- boundary metrics: alpha_0 / tail beta / uniform ----
+    {
+      float tb[2][8];
+      // decoder tail read-out of lte_turbo.cpp:758-773
+      tail_beta(d0[K + 0], d0[K + 2], 0.0f, d1[K + 0], d0[K + 1], d1[K + 2], tb[0]);
+      tail_beta(d1[K + 3], d2[K + 0], d2[K + 2], d2[K + 1], d0[K + 3], d2[K + 3], tb[1]);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          nii[(h * 16 + s) * kWaveLanes + lane] = L.first ? (s == 0 ? D::kUniform : D::kZero) : D::kUniform;
+          nii[(h * 16 + 8 + s) * kWaveLanes + lane] = L.last ? D::from_log(tb[h][s]) : D::kUniform;
+        }
+      }
+    }
+    __syncthreads();  // scratch rows visible to neighbour lanes; d0 staging done
+    for (uint32_t i = lane; i < S * kLdsRow; i += kWaveLanes) lds[i] = 0.0f;
+    __syncthreads();
+
+    for (uint32_t it = 0; it < max_iterations; ++it) {
+      siso_half<D, 0>(L, lds, sc_up, qtab, nii, ck, false);
+      __syncthreads();
+      siso_half<D, 1>(L, lds, sc_lo, qtab, nii + 16 * kWaveLanes, ck, it + 1 == max_iterations);
+      __syncthreads();
+    }
+
+    // ---- posterior out, natural order ----
+    for (uint32_t bb = 0; bb < J.nblk; ++bb) {
+      float* dst = post + cbs[cb_index[J.cb_off + bb]].post_off;
+      for (uint32_t i = lane; i < K; i += kWaveLanes) {
+        const uint32_t qq = __umulhi(i, J.div_magic);
+        const uint32_t t = i - qq * S;
+        dst[i] = lds[t * kLdsRow + bb * M + qq];
+      }
+    }
+    if (L.active && L.first) {
+      if (crc_ok) crc_ok[my_cb] = 0;
+      if (iters) iters[my_cb] = max_iterations;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(64, 2) void window_decode_log_kernel(
+    const WinJob* __restrict__ jobs, uint32_t njobs, const uint32_t* __restrict__ cb_index,
+    const CbInfo* __restrict__ cbs, const uint32_t* __restrict__ qtab, const float* __restrict__ llr,
+    float* __restrict__ post, float2* __restrict__ scratch, int32_t* __restrict__ crc_ok,
+    uint32_t* __restrict__ iters, uint32_t max_iterations) {
+  window_decode<LogDomain>(jobs, njobs, cb_index, cbs, qtab, llr, post, scratch, crc_ok, iters, max_iterations);
+}
+
+__global__ __launch_bounds__(64, 2) void window_decode_lin_kernel(
+    const WinJob* __restrict__ jobs, uint32_t njobs, const uint32_t* __restrict__ cb_index,
+    const CbInfo* __restrict__ cbs, const uint32_t* __restrict__ qtab, const float* __restrict__ llr,
+    float* __restrict__ post, float2* __restrict__ scratch, int32_t* __restrict__ crc_ok,
+    uint32_t* __restrict__ iters, uint32_t max_iterations) {
+  window_decode<LinDomain>(jobs, njobs, cb_index, cbs, qtab, llr, post, scratch, crc_ok, iters, max_iterations);
+}
+
+}  // namespace tcdev

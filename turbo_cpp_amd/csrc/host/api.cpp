@@ -1,0 +1,164 @@
+// C++ API entry points that reach the GPU: the per-code-block decoders
+// (lte_turbo.h:129-155) and decode_transport_block (lte_turbo.h:157-159).
+// They call the C ABI of the HIP library and turn its status codes back
+// into the exception types the reference throws.
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+#include "codec_internal.h"
+#include "turbo_cpp/c_api.h"
+
+namespace turbo_cpp {
+namespace detail {
+std::vector<float> decode_cpu(const CodeBlockDescriptor& desc, std::span<const float> d0, std::span<const float> d1,
+                              std::span<const float> d2, std::uint32_t max_iterations, bool early_stop, bool* crc_ok,
+                              std::uint32_t* iterations_used);
+}  // namespace detail
+
+namespace {
+
+void check(tc_status s) {
+  if (s != TC_OK) detail::throw_status(s, tc_last_error());
+}
+
+tc_cb_desc to_c(const CodeBlockDescriptor& d) {
+  return {d.block_index, d.k_r, d.d_r, d.e_r, d.filler_count, d.qpp_index};
+}
+
+std::vector<float> gpu_decode(int decoder, const CodeBlockDescriptor& desc, std::span<const float> d0,
+                              std::span<const float> d1, std::span<const float> d2, std::uint32_t max_iterations,
+                              bool early_stop, bool* crc_ok, std::uint32_t* iterations_used) {
+  tc_options o{};
+  o.max_iterations = max_iterations;
+  o.early_stop = early_stop ? 1 : 0;
+  o.decoder = decoder;
+  // the windowed decoder has no device CRC24B test yet: early stop uses the exact one
+  if (early_stop) o.decoder = TC_DECODER_EXACT;
+  const tc_cb_desc cd = to_c(desc);
+  std::vector<float> post(desc.k_r, 0.0F);
+  int32_t ok = 0;
+  uint32_t it = 0;
+  check(tc_decode_code_block(&cd, d0.data(), d0.size(), d1.data(), d1.size(), d2.data(), d2.size(), &o, post.data(),
+                             &ok, &it));
+  if (crc_ok) *crc_ok = ok != 0;
+  if (iterations_used && max_iterations > 0) *iterations_used = it;
+  return post;
+}
+
+}  // namespace
+
+// GpuStage1Exact: exact full-length Log-MAP on the GPU (lte_turbo.cpp:815-838).
+std::vector<float> turbo_decode_code_block_gpu_stage1(const CodeBlockDescriptor& desc, std::span<const float> d0,
+                                                      std::span<const float> d1, std::span<const float> d2,
+                                                      std::uint32_t max_iterations, bool early_stop, bool* crc_ok,
+                                                      std::uint32_t* iterations_used) {
+  return gpu_decode(TC_DECODER_EXACT, desc, d0, d1, d2, max_iterations, early_stop, crc_ok, iterations_used);
+}
+
+// GpuStage2Windowed: segment-parallel decoder (lte_turbo.cpp:840-863).
+std::vector<float> turbo_decode_code_block_gpu_stage2(const CodeBlockDescriptor& desc, std::span<const float> d0,
+                                                      std::span<const float> d1, std::span<const float> d2,
+                                                      std::uint32_t max_iterations, bool early_stop, bool* crc_ok,
+                                                      std::uint32_t* iterations_used) {
+  return gpu_decode(TC_DECODER_WINDOW_LOG, desc, d0, d1, d2, max_iterations, early_stop, crc_ok, iterations_used);
+}
+
+// The reference's tile8 variant computes the exact decoder's maths with a
+// different thread mapping (lte_turbo_cuda.cu:298-401); here it is the exact decoder.
+std::vector<float> turbo_decode_code_block_gpu_stage1_tile8_experimental(
+    const CodeBlockDescriptor& desc, std::span<const float> d0, std::span<const float> d1, std::span<const float> d2,
+    std::uint32_t max_iterations, bool early_stop, bool* crc_ok, std::uint32_t* iterations_used) {
+  return gpu_decode(TC_DECODER_EXACT, desc, d0, d1, d2, max_iterations, early_stop, crc_ok, iterations_used);
+}
+
+namespace {
+std::vector<std::uint8_t> hard_bits(const std::vector<float>& post) {
+  std::vector<std::uint8_t> b(post.size());
+  for (std::size_t i = 0; i < post.size(); ++i) b[i] = post[i] >= 0.0F ? 1U : 0U;
+  return b;
+}
+}  // namespace
+
+// lte_turbo.cpp:891-980.  CpuReference decodes block by block; both GPU
+// backends decode all code blocks of the transport block in one batched
+// launch (the reference used one std::async thread per block instead).
+DecodedTransportBlock decode_transport_block(const LteTurboCodecParams& params, const EncodedTransportBlock& encoded,
+                                             DecoderBackend backend) {
+  DecodedTransportBlock out;
+  const std::size_t c = encoded.code_blocks.size();
+  out.posterior_llr = std::vector<float>{};
+  std::vector<std::vector<float>> posts(c);
+  std::vector<std::uint32_t> iters(c, 0);
+  std::vector<bool> ok(c, false);
+  std::vector<std::vector<float>> llrs(c);
+  for (std::size_t i = 0; i < c; ++i) {
+    const auto& blk = encoded.code_blocks[i];
+    llrs[i] = derate_match_code_block(params, blk.descriptor, blk.rate_matched_bits);
+  }
+  if (backend == DecoderBackend::CpuReference) {
+    for (std::size_t i = 0; i < c; ++i) {
+      const auto& d = encoded.code_blocks[i].descriptor;
+      const std::size_t n = d.d_r;
+      bool cok = false;
+      std::uint32_t it = 0;
+      posts[i] = detail::decode_cpu(d, std::span<const float>(llrs[i].data(), n),
+                                    std::span<const float>(llrs[i].data() + n, n),
+                                    std::span<const float>(llrs[i].data() + 2 * n, n), params.max_iterations,
+                                    params.enable_early_stop, &cok, &it);
+      ok[i] = cok;
+      iters[i] = it;
+    }
+  } else {
+    for (std::size_t i = 0; i < c; ++i) {
+      const auto& d = encoded.code_blocks[i].descriptor;
+      if (d.d_r != d.k_r + 4 || llrs[i].size() != 3 * d.d_r)
+        throw std::invalid_argument("invalid LLR sizes for turbo decode");
+    }
+    tc_options o{};
+    o.max_iterations = params.max_iterations;
+    o.early_stop = params.enable_early_stop ? 1 : 0;
+    o.decoder = (backend == DecoderBackend::GpuStage1Exact || params.enable_early_stop) ? TC_DECODER_EXACT
+                                                                                         : TC_DECODER_WINDOW_LOG;
+    std::vector<std::uint32_t> ks(c), fs(c);
+    std::vector<float> packed;
+    std::size_t post_total = 0;
+    for (std::size_t i = 0; i < c; ++i) {
+      ks[i] = static_cast<std::uint32_t>(encoded.code_blocks[i].descriptor.k_r);
+      fs[i] = static_cast<std::uint32_t>(encoded.code_blocks[i].descriptor.filler_count);
+      packed.insert(packed.end(), llrs[i].begin(), llrs[i].end());
+      post_total += ks[i];
+    }
+    std::vector<float> post(post_total);
+    std::vector<int32_t> cok(c, 0);
+    check(tc_decode_batch_host(ks.data(), fs.data(), c, &o, packed.data(), post.data(), cok.data(), iters.data()));
+    std::size_t off = 0;
+    for (std::size_t i = 0; i < c; ++i) {
+      posts[i].assign(post.begin() + static_cast<std::ptrdiff_t>(off),
+                      post.begin() + static_cast<std::ptrdiff_t>(off + ks[i]));
+      off += ks[i];
+      ok[i] = cok[i] != 0;
+      if (params.max_iterations == 0) iters[i] = 0;
+    }
+  }
+  std::vector<std::uint8_t> tb_bits_with_crc;
+  for (std::size_t i = 0; i < c; ++i) {
+    const auto& blk = encoded.code_blocks[i];
+    out.iterations_used = std::max(out.iterations_used, iters[i]);
+    auto bits = hard_bits(posts[i]);
+    bits.erase(bits.begin(), bits.begin() + static_cast<std::ptrdiff_t>(blk.descriptor.filler_count));
+    bool block_ok = true;
+    if (c > 1) {
+      block_ok = check_crc24b(bits);
+      bits.resize(bits.size() - 24);
+    }
+    tb_bits_with_crc.insert(tb_bits_with_crc.end(), bits.begin(), bits.end());
+    out.code_block_crc_ok.push_back(block_ok);
+    out.posterior_llr->insert(out.posterior_llr->end(), posts[i].begin(), posts[i].end());
+  }
+  out.transport_block_crc_ok = check_crc24a(tb_bits_with_crc);
+  out.tb_bits.assign(tb_bits_with_crc.begin(), tb_bits_with_crc.end() - 24);
+  return out;
+}
+
+}  // namespace turbo_cpp
