@@ -175,3 +175,16 @@ def test_segment_geometry_all_table_sizes():
         assert g["blocks_per_wave"] == 64 // m
         assert (g["warmup"] == 0) if m == 1 else (0 < g["warmup"] < s)
     assert T.segment_geometry(6144)["segment_length"] == 96
+
+
+def test_single_hip_runtime_per_process():
+    """Loading the library (and torch after it) maps one HIP runtime only."""
+    import subprocess
+    import sys
+
+    code = ("import turbo_cpp_amd as T; T.lib(); import torch; "
+            "m=open('/proc/self/maps').read(); "
+            "print(len({l.split()[-1] for l in m.splitlines() if 'amdhip64' in l}))")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip().splitlines()[-1] == "1"

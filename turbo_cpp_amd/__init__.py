@@ -57,6 +57,15 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m turbo_cpp_amd.build`")
+        # One HIP runtime per process.  PyTorch wheels carry their own
+        # libamdhip64 whose dependents ask for "libamdhip64.so"; the library
+        # here asks for "libamdhip64.so.7".  Loading torch first lets this
+        # library bind to torch's runtime; loading this library first would
+        # make torch map a second runtime that then finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         L.tc_last_error.restype = C.c_char_p
         L.tc_device_count.argtypes = [C.POINTER(C.c_int32)]

@@ -15,6 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libturbo_cpp_amd.so")
+API_TEST = os.path.join(OUT_DIR, "api_test")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
@@ -56,6 +57,14 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ API test program (tests/cpp/api_test.cpp) linked against the library
+    test_src = os.path.join(ROOT, "tests", "cpp", "api_test.cpp")
+    if os.path.exists(test_src):
+        cmd = ["g++"] + COMMON + [test_src, "-o", API_TEST, "-L", HERE, "-lturbo_cpp_amd",
+                                  "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"api_test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print("built", LIB)
     return LIB
