@@ -535,13 +535,16 @@ int or_decode_cb(const float* d0, const float* d1, const float* d2, uint32_t k,
  * ordinary full-length Log-MAP pass.
  *
  * The arithmetic is the GPU kernel's, written out on the CPU:
- *   x = sa/2, y = par/2, g1 = x + y, g2 = x - y (branch metrics +-g1, +-g2)
- *   log domain:  max*(a,b) = max(a,b) + ln2*log2(1 + 2^(-|a-b|*log2e)),
+ *   log domain, metrics in base-2 units (natural-log metric * log2e):
+ *                x = sa*(log2e/2), y = par*(log2e/2), g1 = x + y, g2 = x - y
+ *                (branch metrics +-g1, +-g2),
+ *                max*(a,b) = max(a,b) + log2(1 + 2^-|a-b|),
  *                metrics renormalised to a maximum of 0 each step,
- *                extrinsic = LSE{alpha~(n)+beta(n) : u(n)=1}
- *                          - LSE{...: u(n)=0} - sa,
- *                LSE(v) = m + ln2*log2(sum 2^((v_i-m)*log2e)).
- *   linear domain: branch weights exp(gamma - |x| - |y|) = U_u * Q_c,
+ *                extrinsic = ln2 * (LSE{alpha~(n)+beta(n) : u(n)=1}
+ *                          - LSE{...: u(n)=0}) - sa,
+ *                LSE(v) = m + log2(sum 2^(v_i-m)).
+ *   linear domain: x = sa/2, y = par/2; branch weights
+ *                exp(gamma - |x| - |y|) = U_u * Q_c with 2^(-|sa| log2e),
  *                metrics rescaled by a power of two to a maximum in [0.5,1)
  *                each step, extrinsic = ln(sum_{u=1} B(n) beta(n)) -
  *                ln(sum_{u=0} B(n) beta(n)) with B(n) the alpha update
@@ -557,10 +560,11 @@ static unsigned tr_u(unsigned n) { return ((n >> 2) ^ (n >> 1) ^ n) & 1u; }
 static unsigned tr_pa(unsigned n) { return (n & 3u) << 1; }
 static unsigned tr_cpa(unsigned n) { return ((n >> 2) ^ (n >> 1)) & 1u; }
 
+static const float kHalfLog2e = 0.721347520444481703680f;
+
 static inline float lg_maxstar(float a, float b) {
   float m = fmaxf(a, b);
-  float d = fabsf(a - b);
-  return m + kLn2 * log2f(1.0f + exp2f(-d * kLog2e));
+  return m + log2f(1.0f + exp2f(-fabsf(a - b)));
 }
 
 /* gamma for (u, c) from g1 = x+y, g2 = x-y */
@@ -576,8 +580,8 @@ static void lg_norm(float* v) {
 }
 
 static inline void lin_weights(float x, float y, float* u1, float* u0, float* q1, float* q0) {
-  float ex = exp2f(-2.0f * fabsf(x) * kLog2e);
-  float ey = exp2f(-2.0f * fabsf(y) * kLog2e);
+  float ex = exp2f(-fabsf(x + x) * kLog2e);
+  float ey = exp2f(-fabsf(y + y) * kLog2e);
   *u1 = (x >= 0.0f) ? 1.0f : ex;
   *u0 = (x >= 0.0f) ? ex : 1.0f;
   *q1 = (y >= 0.0f) ? 1.0f : ey;
@@ -609,7 +613,8 @@ static void seg_siso(const float* in_sa, const float* in_par, int len, int ga, i
     float x = 0.5f * in_sa[t], y = 0.5f * in_par[t];
     float nb[8];
     if (!lin) {
-      float g1 = x + y, g2 = x - y;
+      const float x2 = in_sa[t] * kHalfLog2e, y2 = in_par[t] * kHalfLog2e;
+      float g1 = x2 + y2, g2 = x2 - y2;
       for (unsigned p = 0; p < 8; ++p) {
         unsigned c0, n0, c1, n1;
         rsc_step(p, 0, &c0, &n0);
@@ -641,7 +646,8 @@ static void seg_siso(const float* in_sa, const float* in_par, int len, int ga, i
     const float* bn = bstore + 8 * (size_t)(t + 1);
     float na[8];
     if (!lin) {
-      float g1 = x + y, g2 = x - y;
+      const float x2 = in_sa[t] * kHalfLog2e, y2 = in_par[t] * kHalfLog2e;
+      float g1 = x2 + y2, g2 = x2 - y2;
       float tv[8];
       for (unsigned n = 0; n < 8; ++n) {
         unsigned u = tr_u(n), pa = tr_pa(n), ca = tr_cpa(n);
@@ -656,11 +662,11 @@ static void seg_siso(const float* in_sa, const float* in_par, int len, int ga, i
         }
         float s1 = 0.0f, s0 = 0.0f;
         for (unsigned n = 0; n < 8; ++n) {
-          if (tr_u(n)) s1 += exp2f((tv[n] - m1) * kLog2e);
-          else s0 += exp2f((tv[n] - m0) * kLog2e);
+          if (tr_u(n)) s1 += exp2f(tv[n] - m1);
+          else s0 += exp2f(tv[n] - m0);
         }
-        float l1 = m1 + kLn2 * log2f(s1), l0 = m0 + kLn2 * log2f(s0);
-        ext[t] = (l1 - l0) - in_sa[t];
+        float l1 = m1 + log2f(s1), l0 = m0 + log2f(s0);
+        ext[t] = (l1 - l0) * kLn2 - in_sa[t];
       }
       lg_norm(na);
     } else {
@@ -716,7 +722,7 @@ int or_decode_window(const float* d0, const float* d1, const float* d2, uint32_t
   for (int h = 0; h < 2; ++h)
     for (int s = 0; s < 8; ++s) {
       float v = isfinite(tb[h][s]) ? tb[h][s] : OR_NEG;
-      tb[h][s] = lin ? exp2f(v * kLog2e) : v;
+      tb[h][s] = lin ? exp2f(v * kLog2e) : v * kLog2e;
     }
   const float uni = lin ? 1.0f : 0.0f, zero = lin ? 0.0f : OR_NEG;
   for (size_t i = 0; i < nb; ++i) ab[i] = uni;

@@ -16,6 +16,7 @@ namespace tcdev {
 
 constexpr float kLog2e = 1.44269504088896340736f;
 constexpr float kLn2 = 0.693147180559945309417f;
+constexpr float kHalfLog2e = 0.721347520444481703680f;
 constexpr float kNeg = -1.0e30f;          // finite stand-in for -inf in the window kernels
 constexpr float kFltMin = 1.17549435e-38f;
 

@@ -56,20 +56,37 @@ __device__ __forceinline__ float dpp_from_next_lane(float v) {  // wave_shl:1
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
 }
 
+// 8 metrics of one lane as two float4 (per-lane records keep every access a
+// single base register plus an immediate offset)
+__device__ __forceinline__ void store8(float* __restrict__ p, const float* v) {
+  reinterpret_cast<float4*>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+  reinterpret_cast<float4*>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+__device__ __forceinline__ void load8(const float* __restrict__ p, float* v) {
+  const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+  v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+}
+
 __device__ __forceinline__ float max8(const float* v) {
   return fmaxf(fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])), fmaxf(fmaxf(v[4], v[5]), fmaxf(v[6], v[7])));
 }
 
 // ------------------------------------------------------------ log domain --
+// Metrics in base-2 units (natural-log metric * log2e), so max* is
+// max(a,b) + log2(1 + 2^-|a-b|) with no scaling inside the transcendentals.
 struct LogDomain {
+  struct Br {
+    float g1, g2;  // branch metrics: +-g1 for (u,c) = (1,1)/(0,0), +-g2 for (1,0)/(0,1)
+  };
   static constexpr float kUniform = 0.0f;
   static constexpr float kZero = kNeg;
-  __device__ static __forceinline__ float from_log(float v) { return v; }
-
+  __device__ static __forceinline__ float from_log(float v) { return v * kLog2e; }
+  __device__ static __forceinline__ Br branch(float sa, float par) {
+    const float x = sa * kHalfLog2e, y = par * kHalfLog2e;
+    return {x + y, x - y};
+  }
   __device__ static __forceinline__ float maxstar(float a, float b) {
-    const float m = fmaxf(a, b);
-    const float d = fabsf(a - b);
-    return m + kLn2 * hw_log2(1.0f + hw_exp2(-d * kLog2e));
+    return fmaxf(a, b) + hw_log2(1.0f + hw_exp2(-fabsf(a - b)));
   }
   __device__ static __forceinline__ void norm(float* v) {
     const float m = max8(v);
@@ -77,14 +94,12 @@ struct LogDomain {
     for (int i = 0; i < 8; ++i) v[i] = v[i] - m;
   }
   // beta_t from beta_{t+1}
-  __device__ static __forceinline__ void bstep(float* b, float sa, float par) {
-    const float x = 0.5f * sa, y = 0.5f * par;
-    const float g1 = x + y, g2 = x - y;
+  __device__ static __forceinline__ void bstep(float* b, const Br& r) {
     float nb[8];
 #pragma unroll
     for (unsigned p = 0; p < 8; ++p) {
       const Tr t0 = rsc(p, 0), t1 = rsc(p, 1);
-      nb[p] = maxstar(b[t0.next] + gsel(0, t0.parity, g1, g2), b[t1.next] + gsel(1, t1.parity, g1, g2));
+      nb[p] = maxstar(b[t0.next] + gsel(0, t0.parity, r.g1, r.g2), b[t1.next] + gsel(1, t1.parity, r.g1, r.g2));
     }
     norm(nb);
 #pragma unroll
@@ -92,14 +107,12 @@ struct LogDomain {
   }
   // alpha_{t+1} from alpha_t; returns the extrinsic of step t when WANT
   template <bool WANT>
-  __device__ static __forceinline__ float astep(float* a, float sa, float par, const float* bn) {
-    const float x = 0.5f * sa, y = 0.5f * par;
-    const float g1 = x + y, g2 = x - y;
+  __device__ static __forceinline__ float astep(float* a, const Br& r, float sa, const float* bn) {
     float na[8];
 #pragma unroll
     for (unsigned n = 0; n < 8; ++n) {
       const unsigned u = in_of(n), p = pred0(n), c = par0(n);
-      na[n] = maxstar(a[p] + gsel(u, c, g1, g2), a[p + 1] + gsel(u, c ^ 1u, g1, g2));
+      na[n] = maxstar(a[p] + gsel(u, c, r.g1, r.g2), a[p + 1] + gsel(u, c ^ 1u, r.g1, r.g2));
     }
     float ext = 0.0f;
     if constexpr (WANT) {
@@ -110,17 +123,17 @@ struct LogDomain {
       const float m1 = fmaxf(fmaxf(fmaxf(fmaxf(kNeg, tv[1]), tv[2]), tv[4]), tv[7]);
       const float m0 = fmaxf(fmaxf(fmaxf(fmaxf(kNeg, tv[0]), tv[3]), tv[5]), tv[6]);
       float s1 = 0.0f, s0 = 0.0f;
-      s0 += hw_exp2((tv[0] - m0) * kLog2e);
-      s1 += hw_exp2((tv[1] - m1) * kLog2e);
-      s1 += hw_exp2((tv[2] - m1) * kLog2e);
-      s0 += hw_exp2((tv[3] - m0) * kLog2e);
-      s1 += hw_exp2((tv[4] - m1) * kLog2e);
-      s0 += hw_exp2((tv[5] - m0) * kLog2e);
-      s0 += hw_exp2((tv[6] - m0) * kLog2e);
-      s1 += hw_exp2((tv[7] - m1) * kLog2e);
-      const float l1 = m1 + kLn2 * hw_log2(s1);
-      const float l0 = m0 + kLn2 * hw_log2(s0);
-      ext = (l1 - l0) - sa;
+      s0 += hw_exp2(tv[0] - m0);
+      s1 += hw_exp2(tv[1] - m1);
+      s1 += hw_exp2(tv[2] - m1);
+      s0 += hw_exp2(tv[3] - m0);
+      s1 += hw_exp2(tv[4] - m1);
+      s0 += hw_exp2(tv[5] - m0);
+      s0 += hw_exp2(tv[6] - m0);
+      s1 += hw_exp2(tv[7] - m1);
+      const float l1 = m1 + hw_log2(s1);
+      const float l0 = m0 + hw_log2(s0);
+      ext = (l1 - l0) * kLn2 - sa;
     }
     norm(na);
 #pragma unroll
@@ -131,19 +144,17 @@ struct LogDomain {
 
 // ---------------------------------------------------- probability domain --
 struct LinDomain {
+  struct Br {
+    float u1, u0, q1, q0;  // branch weight exp(gamma - |x| - |y|) = U_u * Q_c
+  };
   static constexpr float kUniform = 1.0f;
   static constexpr float kZero = 0.0f;
   __device__ static __forceinline__ float from_log(float v) { return hw_exp2(v * kLog2e); }
-
-  // branch weights exp(gamma - |x| - |y|) = U_u * Q_c
-  __device__ static __forceinline__ void weights(float sa, float par, float& u1, float& u0, float& q1, float& q0) {
+  __device__ static __forceinline__ Br branch(float sa, float par) {
     const float x = 0.5f * sa, y = 0.5f * par;
-    const float ex = hw_exp2((-2.0f * fabsf(x)) * kLog2e);
-    const float ey = hw_exp2((-2.0f * fabsf(y)) * kLog2e);
-    u1 = (x >= 0.0f) ? 1.0f : ex;
-    u0 = (x >= 0.0f) ? ex : 1.0f;
-    q1 = (y >= 0.0f) ? 1.0f : ey;
-    q0 = (y >= 0.0f) ? ey : 1.0f;
+    const float ex = hw_exp2(-fabsf(x + x) * kLog2e);
+    const float ey = hw_exp2(-fabsf(y + y) * kLog2e);
+    return {(x >= 0.0f) ? 1.0f : ex, (x >= 0.0f) ? ex : 1.0f, (y >= 0.0f) ? 1.0f : ey, (y >= 0.0f) ? ey : 1.0f};
   }
   // rescale by 2^-e so that the largest entry lies in [0.5, 1)
   __device__ static __forceinline__ void norm(float* v) {
@@ -154,10 +165,8 @@ struct LinDomain {
       for (int i = 0; i < 8; ++i) v[i] = __builtin_amdgcn_ldexpf(v[i], -e);
     }
   }
-  __device__ static __forceinline__ void bstep(float* b, float sa, float par) {
-    float u1, u0, q1, q0;
-    weights(sa, par, u1, u0, q1, q0);
-    const float w[2][2] = {{u0 * q0, u0 * q1}, {u1 * q0, u1 * q1}};
+  __device__ static __forceinline__ void bstep(float* b, const Br& r) {
+    const float w[2][2] = {{r.u0 * r.q0, r.u0 * r.q1}, {r.u1 * r.q0, r.u1 * r.q1}};
     float nb[8];
 #pragma unroll
     for (unsigned p = 0; p < 8; ++p) {
@@ -169,10 +178,8 @@ struct LinDomain {
     for (int i = 0; i < 8; ++i) b[i] = nb[i];
   }
   template <bool WANT>
-  __device__ static __forceinline__ float astep(float* a, float sa, float par, const float* bn) {
-    float u1, u0, q1, q0;
-    weights(sa, par, u1, u0, q1, q0);
-    const float q[2] = {q0, q1}, uu[2] = {u0, u1};
+  __device__ static __forceinline__ float astep(float* a, const Br& r, float /*sa*/, const float* bn) {
+    const float q[2] = {r.q0, r.q1}, uu[2] = {r.u0, r.u1};
     float na[8];
     float s1 = 0.0f, s0 = 0.0f;
 #pragma unroll
@@ -253,134 +260,188 @@ __device__ __forceinline__ uint32_t y_fwd(uint32_t y, uint32_t b, uint32_t m) {
   return (y >= m) ? y - m : y;
 }
 
+// Loads of one chunk of up to kSubWin consecutive steps [lo, lo+len) of
+// segment qq: the staged channel pair from the wave's scratch (global) and
+// the extrinsic from LDS.  The global part of the next chunk is issued before
+// the current chunk is computed.
+struct Chunk {
+  float2 v[kSubWin];
+};
+
+__device__ __forceinline__ void load_chunk(const float2* __restrict__ sc, uint32_t col, uint32_t lo, uint32_t len,
+                                           Chunk& c) {
+  const float2* base = sc + lo * kWaveLanes + col;
+  // Opaque to the optimiser: without this the per-lane 64-bit address of
+  // every (chunk, step, column) combination is hoisted out of the loops and
+  // spilled.  One address pair per chunk; steps use immediate offsets.
+  asm volatile("" : "+v"(base));
+#pragma unroll
+  for (int j = 0; j < kSubWin; ++j)
+    if ((uint32_t)j < len) c.v[j] = base[j * kWaveLanes];
+}
+
+// Backward recursion of beta over steps [lo, hi) of segment qq (chunks of
+// kSubWin aligned at lo, last chunk first).  on_chunk(c, beta) runs before
+// chunk c; on_step(t, beta) after each step t (beta then is beta_t).
+template <class D, int H, class OnChunk, class OnStep>
+__device__ __forceinline__ void beta_pass(const Lane& L, const float* __restrict__ lds,
+                                          const float2* __restrict__ sc, const uint32_t* __restrict__ qtab,
+                                          uint32_t qq, uint32_t col, uint32_t lo, uint32_t hi, float* bc,
+                                          OnChunk&& on_chunk, OnStep&& on_step) {
+  if (hi <= lo) return;
+  const uint32_t n = (hi - lo + kSubWin - 1) / kSubWin;
+  const uint32_t by = (H == 1) ? qpp_stride(L, qq) : 0u;
+  Chunk nxt;
+  {
+    const uint32_t c0 = lo + (n - 1) * kSubWin;
+    load_chunk(sc, col, c0, hi - c0, nxt);
+  }
+  for (int c = (int)n - 1; c >= 0; --c) {
+    const uint32_t clo = lo + c * kSubWin;
+    const uint32_t len = min((uint32_t)kSubWin, hi - clo);
+    const Chunk cur = nxt;
+    if (c > 0) load_chunk(sc, col, clo - kSubWin, kSubWin, nxt);
+    on_chunk((uint32_t)c, bc);
+    uint32_t y = (H == 1) ? qpp_col0(L, qq, clo + len - 1) : 0u;
+    float e[kSubWin];
+#pragma unroll
+    for (int j = kSubWin - 1; j >= 0; --j) {
+      if ((uint32_t)j < len) {
+        e[j] = lds[lds_addr<H>(L, qtab, qq, clo + j, y)];
+        if constexpr (H == 1) y = y_back(y, by, L.M);
+      }
+    }
+#pragma unroll
+    for (int j = kSubWin - 1; j >= 0; --j) {
+      if ((uint32_t)j < len) {
+        __builtin_amdgcn_sched_barrier(0);
+        D::bstep(bc, D::branch(cur.v[j].x + e[j], cur.v[j].y));
+        on_step(clo + j, bc);
+      }
+    }
+  }
+}
+
 // One SISO half (H = 0 upper, 1 lower) for this lane's segment.
-// nii: this half's boundary metrics in [alpha 8 | beta 8][lane] order
-// (alpha at step -g, beta at step S+g of the lane's segment); ck: the
-// sub-window checkpoints of beta, [w][8][lane].
+// nii: this lane's boundary metrics of this half, [alpha 8 | beta 8]
+// (alpha at step -g, beta at step S+g of the segment); ck: this lane's beta
+// checkpoints at sub-window boundaries, [w][8].  The neighbours' records
+// are reached with nii_stride (floats between consecutive lanes' records).
 template <class D, int H>
 __device__ __forceinline__ void siso_half(const Lane& L, float* __restrict__ lds, const float2* __restrict__ sc,
                                           const uint32_t* __restrict__ qtab, float* __restrict__ nii,
                                           float* __restrict__ ck, bool write_post) {
-  const uint32_t S = L.S, M = L.M, g = L.g;
-  const uint32_t lane = threadIdx.x;
-  // ---------------- backward pass: warm-up over the next segment, then the
-  // segment itself, storing beta at sub-window boundaries
+  const uint32_t S = L.S, g = L.g;
+  // ---------------- backward pass: warm-up over the next segment's first g
+  // steps, then the segment, storing beta at sub-window boundaries
   float bc[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) bc[i] = nii[(8 + i) * kWaveLanes + lane];
+  load8(nii + 8, bc);
   if (g > 0) {
     const uint32_t qr = L.last ? L.q : L.q + 1;
-    const int dl = L.last ? 0 : 1;
-    uint32_t y = (H == 1) ? qpp_col0(L, qr, g - 1) : 0u;
-    const uint32_t by = (H == 1) ? qpp_stride(L, qr) : 0u;
-    for (int t = (int)g - 1; t >= 0; --t) {
-      const float2 v = sc[t * kWaveLanes + L.le + dl];
-      const float e = lds[lds_addr<H>(L, qtab, qr, (uint32_t)t, y)];
-      D::bstep(bc, v.x + e, v.y);
-      if constexpr (H == 1) y = y_back(y, by, M);
-    }
-    if (L.last) {
-#pragma unroll
-      for (int i = 0; i < 8; ++i) bc[i] = nii[(8 + i) * kWaveLanes + lane];
-    }
+    const uint32_t col = L.last ? L.le : L.le + 1;
+    beta_pass<D, H>(L, lds, sc, qtab, qr, col, 0, g, bc, [](uint32_t, float*) {}, [](uint32_t, const float*) {});
+    if (L.last) load8(nii + 8, bc);
   }
-  float bexp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  {
-    uint32_t y = (H == 1) ? qpp_col0(L, L.q, S - 1) : 0u;
-    const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
-    for (int w = (int)L.nW - 1; w >= 0; --w) {
+  beta_pass<D, H>(
+      L, lds, sc, qtab, L.q, L.le, 0, S, bc,
+      [&](uint32_t w, float* b) { store8(ck + 8 * w, b); },
+      [&](uint32_t t, const float* b) {
+        // t is uniform: the whole wave hands beta at step g to the previous
+        // segment (used next iteration); the last segment keeps its tail beta
+        if (g > 0 && t == g) {
+          float nb[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) ck[(w * 8 + i) * kWaveLanes + lane] = bc[i];
-      const uint32_t t0 = w * kSubWin;
-      const uint32_t len = min((uint32_t)kSubWin, S - t0);
+          for (int i = 0; i < 8; ++i) nb[i] = dpp_from_next_lane(b[i]);
+          if (!L.last) store8(nii + 8, nb);
+        }
+      });
+  // ---------------- forward pass: warm-up over the previous segment's last g
+  // steps, then per sub-window: recompute beta, alpha + extrinsic
+  float ac[8];
+  load8(nii, ac);
+  if (g > 0) {
+    const uint32_t ql = L.first ? L.q : L.q - 1;
+    const uint32_t col = L.first ? L.le : L.le - 1;
+    const uint32_t by = (H == 1) ? qpp_stride(L, ql) : 0u;
+    const uint32_t n = (g + kSubWin - 1) / kSubWin;
+    Chunk nxt;
+    load_chunk(sc, col, S - g, min((uint32_t)kSubWin, g), nxt);
+    for (uint32_t c = 0; c < n; ++c) {
+      const uint32_t clo = S - g + c * kSubWin;
+      const uint32_t len = min((uint32_t)kSubWin, S - clo);
+      const Chunk cur = nxt;
+      if (c + 1 < n) load_chunk(sc, col, clo + kSubWin, min((uint32_t)kSubWin, S - clo - kSubWin), nxt);
+      uint32_t y = (H == 1) ? qpp_col0(L, ql, clo) : 0u;
+      float e[kSubWin];
 #pragma unroll
-      for (int j = kSubWin - 1; j >= 0; --j) {
+      for (int j = 0; j < kSubWin; ++j) {
         if ((uint32_t)j < len) {
-          const uint32_t t = t0 + j;
-          const float2 v = sc[t * kWaveLanes + L.le];
-          const float e = lds[lds_addr<H>(L, qtab, L.q, t, y)];
-          D::bstep(bc, v.x + e, v.y);
-          if constexpr (H == 1) y = y_back(y, by, M);
-          if (t == g) {
+          e[j] = lds[lds_addr<H>(L, qtab, ql, clo + j, y)];
+          if constexpr (H == 1) y = y_fwd(y, by, L.M);
+        }
+      }
 #pragma unroll
-            for (int i = 0; i < 8; ++i) bexp[i] = bc[i];
-          }
+      for (int j = 0; j < kSubWin; ++j) {
+        if ((uint32_t)j < len) {
+          const float sa = cur.v[j].x + e[j];
+          D::template astep<false>(ac, D::branch(sa, cur.v[j].y), sa, nullptr);
         }
       }
     }
+    if (L.first) load8(nii, ac);
   }
-  // hand beta at step g to the previous segment (used next iteration)
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const float nb = dpp_from_next_lane(bexp[i]);
-    if (!L.last) nii[(8 + i) * kWaveLanes + lane] = nb;
-  }
-  // ---------------- forward pass: warm-up over the previous segment, then
-  // per sub-window: recompute beta, alpha + extrinsic
-  float ac[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) ac[i] = nii[i * kWaveLanes + lane];
-  if (g > 0) {
-    const uint32_t ql = L.first ? L.q : L.q - 1;
-    const int dl = L.first ? 0 : -1;
-    uint32_t y = (H == 1) ? qpp_col0(L, ql, S - g) : 0u;
-    const uint32_t by = (H == 1) ? qpp_stride(L, ql) : 0u;
-    for (uint32_t t = S - g; t < S; ++t) {
-      const float2 v = sc[t * kWaveLanes + L.le + dl];
-      const float e = lds[lds_addr<H>(L, qtab, ql, t, y)];
-      D::template astep<false>(ac, v.x + e, v.y, nullptr);
-      if constexpr (H == 1) y = y_fwd(y, by, M);
-    }
-    if (L.first) {
-#pragma unroll
-      for (int i = 0; i < 8; ++i) ac[i] = nii[i * kWaveLanes + lane];
-    }
-  }
-  float aexp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
+  Chunk nxt;
+  load_chunk(sc, L.le, 0, min((uint32_t)kSubWin, S), nxt);
   for (uint32_t w = 0; w < L.nW; ++w) {
     const uint32_t t0 = w * kSubWin;
     const uint32_t len = min((uint32_t)kSubWin, S - t0);
+    const Chunk cur = nxt;
+    if (w + 1 < L.nW) load_chunk(sc, L.le, t0 + kSubWin, min((uint32_t)kSubWin, S - t0 - kSubWin), nxt);
     float bs[kSubWin][8];
-    float isa[kSubWin], ipar[kSubWin];
+    typename D::Br br[kSubWin];
+    float isa[kSubWin];
     uint32_t addr[kSubWin];
     float bb[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) bb[i] = ck[(w * 8 + i) * kWaveLanes + lane];
+    load8(ck + 8 * w, bb);
     uint32_t y = (H == 1) ? qpp_col0(L, L.q, t0 + len - 1) : 0u;
+    float e[kSubWin];
 #pragma unroll
     for (int j = kSubWin - 1; j >= 0; --j) {
       if ((uint32_t)j < len) {
-        const uint32_t t = t0 + j;
-        addr[j] = lds_addr<H>(L, qtab, L.q, t, y);
-        const float2 v = sc[t * kWaveLanes + L.le];
-        isa[j] = v.x + lds[addr[j]];
-        ipar[j] = v.y;
+        addr[j] = lds_addr<H>(L, qtab, L.q, t0 + j, y);
+        e[j] = lds[addr[j]];
+        if constexpr (H == 1) y = y_back(y, by, L.M);
+      }
+    }
+#pragma unroll
+    for (int j = kSubWin - 1; j >= 0; --j) {
+      if ((uint32_t)j < len) {
+        isa[j] = cur.v[j].x + e[j];
+        br[j] = D::branch(isa[j], cur.v[j].y);
 #pragma unroll
         for (int i = 0; i < 8; ++i) bs[j][i] = bb[i];
-        if (j > 0) D::bstep(bb, isa[j], ipar[j]);
-        if constexpr (H == 1) y = y_back(y, by, M);
+        __builtin_amdgcn_sched_barrier(0);
+        if (j > 0) D::bstep(bb, br[j]);
       }
     }
 #pragma unroll
     for (int j = 0; j < kSubWin; ++j) {
       if ((uint32_t)j < len) {
         const uint32_t t = t0 + j;
-        if (t == S - g) {
+        if (g > 0 && t == S - g) {
+          // uniform: hand alpha at step S-g to the next segment
+          float na[8];
 #pragma unroll
-          for (int i = 0; i < 8; ++i) aexp[i] = ac[i];
+          for (int i = 0; i < 8; ++i) na[i] = dpp_from_prev_lane(ac[i]);
+          if (!L.first) store8(nii, na);
         }
-        const float ext = D::template astep<true>(ac, isa[j], ipar[j], bs[j]);
+        __builtin_amdgcn_sched_barrier(0);  // keep steps in order: bounds register pressure
+        const float ext = D::template astep<true>(ac, br[j], isa[j], bs[j]);
         const float val = (H == 1 && write_post) ? isa[j] + ext : ext;
         if (L.active) lds[addr[j]] = val;
       }
     }
-  }
-  // hand alpha at step S-g to the next segment
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const float na = dpp_from_prev_lane(aexp[i]);
-    if (!L.first) nii[i * kWaveLanes + lane] = na;
   }
 }
 
@@ -393,8 +454,9 @@ __device__ void window_decode(const WinJob* __restrict__ jobs, uint32_t njobs, c
   const uint32_t lane = threadIdx.x;
   float2* sc_up = scratch + (size_t)blockIdx.x * kSlotFloat2;
   float2* sc_lo = sc_up + kMaxSeg * kWaveLanes;
-  float* nii = reinterpret_cast<float*>(sc_lo + kMaxSeg * kWaveLanes);  // [half][16][64]
-  float* ck = nii + 2 * 16 * kWaveLanes;                                 // [w][8][64]
+  // per-lane records: boundary metrics [lane][half][16], checkpoints [lane][w][8]
+  float* nii = reinterpret_cast<float*>(sc_lo + kMaxSeg * kWaveLanes) + lane * 32;
+  float* ck = reinterpret_cast<float*>(sc_lo + kMaxSeg * kWaveLanes) + kWaveLanes * 32 + lane * (8 * kMaxSubWins);
   for (uint32_t jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
     const WinJob J = jobs[jb];
     Lane L;
@@ -457,8 +519,8 @@ __device__ void window_decode(const WinJob* __restrict__ jobs, uint32_t njobs, c
       for (int h = 0; h < 2; ++h) {
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-          nii[(h * 16 + s) * kWaveLanes + lane] = L.first ? (s == 0 ? D::kUniform : D::kZero) : D::kUniform;
-          nii[(h * 16 + 8 + s) * kWaveLanes + lane] = L.last ? D::from_log(tb[h][s]) : D::kUniform;
+          nii[h * 16 + s] = L.first ? (s == 0 ? D::kUniform : D::kZero) : D::kUniform;
+          nii[h * 16 + 8 + s] = L.last ? D::from_log(tb[h][s]) : D::kUniform;
         }
       }
     }
@@ -469,7 +531,7 @@ __device__ void window_decode(const WinJob* __restrict__ jobs, uint32_t njobs, c
     for (uint32_t it = 0; it < max_iterations; ++it) {
       siso_half<D, 0>(L, lds, sc_up, qtab, nii, ck, false);
       __syncthreads();
-      siso_half<D, 1>(L, lds, sc_lo, qtab, nii + 16 * kWaveLanes, ck, it + 1 == max_iterations);
+      siso_half<D, 1>(L, lds, sc_lo, qtab, nii + 16, ck, it + 1 == max_iterations);
       __syncthreads();
     }
 
