@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--plain-blocks", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this decoder runs on one GPU; --gpus must be 1 (no multi-GPU path is built)"}))
@@ -196,6 +198,14 @@ def main():
     plain_s = (time.perf_counter() - t0) / nplain
     plain_gbits = K / plain_s / 1e9
     log(f"plain: {plain_s * 1e3:.3f} ms per block -> {plain_gbits * 1e3:.3f} Mbit/s")
+
+    if args.copy_calibration:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 256 MiB read + 256 MiB written per copy kernel")
 
     bits_per_launch = N_CB * K
     achieved = bits_per_launch * ITERS * VALU_SLOTS_PER_BIT_ITER / (kernel_ms / 1e3)
