@@ -262,10 +262,11 @@ __device__ __forceinline__ uint32_t y_fwd(uint32_t y, uint32_t b, uint32_t m) {
 
 // Loads of one chunk of up to kSubWin consecutive steps [lo, lo+len) of
 // segment qq: the staged channel pair from the wave's scratch (global) and
-// the extrinsic from LDS.  The global part of the next chunk is issued before
-// the current chunk is computed.
+// the extrinsic from LDS.  Each pass issues the next chunk's loads before it
+// computes the current one.
 struct Chunk {
   float2 v[kSubWin];
+  float e[kSubWin];
 };
 
 __device__ __forceinline__ void load_chunk(const float2* __restrict__ sc, uint32_t col, uint32_t lo, uint32_t len,
@@ -278,6 +279,25 @@ __device__ __forceinline__ void load_chunk(const float2* __restrict__ sc, uint32
 #pragma unroll
   for (int j = 0; j < kSubWin; ++j)
     if ((uint32_t)j < len) c.v[j] = base[j * kWaveLanes];
+}
+
+// LDS extrinsics of the chunk, issued in the order the pass consumes them;
+// addr (may be null) keeps the addresses for the write-back.
+template <int H, bool BACKWARD>
+__device__ __forceinline__ void load_chunk_lds(const Lane& L, const float* __restrict__ lds,
+                                               const uint32_t* __restrict__ qtab, uint32_t qq, uint32_t by,
+                                               uint32_t lo, uint32_t len, Chunk& c, uint32_t* addr) {
+  uint32_t y = (H == 1) ? qpp_col0(L, qq, BACKWARD ? lo + len - 1 : lo) : 0u;
+#pragma unroll
+  for (int jj = 0; jj < kSubWin; ++jj) {
+    const int j = BACKWARD ? kSubWin - 1 - jj : jj;
+    if ((uint32_t)j < len) {
+      const uint32_t a = lds_addr<H>(L, qtab, qq, lo + j, y);
+      c.e[j] = lds[a];
+      if (addr) addr[j] = a;
+      if constexpr (H == 1) y = BACKWARD ? y_back(y, by, L.M) : y_fwd(y, by, L.M);
+    }
+  }
 }
 
 // Backward recursion of beta over steps [lo, hi) of segment qq (chunks of
@@ -295,27 +315,22 @@ __device__ __forceinline__ void beta_pass(const Lane& L, const float* __restrict
   {
     const uint32_t c0 = lo + (n - 1) * kSubWin;
     load_chunk(sc, col, c0, hi - c0, nxt);
+    load_chunk_lds<H, true>(L, lds, qtab, qq, by, c0, hi - c0, nxt, nullptr);
   }
   for (int c = (int)n - 1; c >= 0; --c) {
     const uint32_t clo = lo + c * kSubWin;
     const uint32_t len = min((uint32_t)kSubWin, hi - clo);
     const Chunk cur = nxt;
-    if (c > 0) load_chunk(sc, col, clo - kSubWin, kSubWin, nxt);
-    on_chunk((uint32_t)c, bc);
-    uint32_t y = (H == 1) ? qpp_col0(L, qq, clo + len - 1) : 0u;
-    float e[kSubWin];
-#pragma unroll
-    for (int j = kSubWin - 1; j >= 0; --j) {
-      if ((uint32_t)j < len) {
-        e[j] = lds[lds_addr<H>(L, qtab, qq, clo + j, y)];
-        if constexpr (H == 1) y = y_back(y, by, L.M);
-      }
+    if (c > 0) {
+      load_chunk(sc, col, clo - kSubWin, kSubWin, nxt);
+      load_chunk_lds<H, true>(L, lds, qtab, qq, by, clo - kSubWin, kSubWin, nxt, nullptr);
     }
+    on_chunk((uint32_t)c, bc);
 #pragma unroll
     for (int j = kSubWin - 1; j >= 0; --j) {
       if ((uint32_t)j < len) {
         __builtin_amdgcn_sched_barrier(0);
-        D::bstep(bc, D::branch(cur.v[j].x + e[j], cur.v[j].y));
+        D::bstep(bc, D::branch(cur.v[j].x + cur.e[j], cur.v[j].y));
         on_step(clo + j, bc);
       }
     }
@@ -366,24 +381,21 @@ __device__ __forceinline__ void siso_half(const Lane& L, float* __restrict__ lds
     const uint32_t n = (g + kSubWin - 1) / kSubWin;
     Chunk nxt;
     load_chunk(sc, col, S - g, min((uint32_t)kSubWin, g), nxt);
+    load_chunk_lds<H, false>(L, lds, qtab, ql, by, S - g, min((uint32_t)kSubWin, g), nxt, nullptr);
     for (uint32_t c = 0; c < n; ++c) {
       const uint32_t clo = S - g + c * kSubWin;
       const uint32_t len = min((uint32_t)kSubWin, S - clo);
       const Chunk cur = nxt;
-      if (c + 1 < n) load_chunk(sc, col, clo + kSubWin, min((uint32_t)kSubWin, S - clo - kSubWin), nxt);
-      uint32_t y = (H == 1) ? qpp_col0(L, ql, clo) : 0u;
-      float e[kSubWin];
-#pragma unroll
-      for (int j = 0; j < kSubWin; ++j) {
-        if ((uint32_t)j < len) {
-          e[j] = lds[lds_addr<H>(L, qtab, ql, clo + j, y)];
-          if constexpr (H == 1) y = y_fwd(y, by, L.M);
-        }
+      if (c + 1 < n) {
+        const uint32_t nlen = min((uint32_t)kSubWin, S - clo - kSubWin);
+        load_chunk(sc, col, clo + kSubWin, nlen, nxt);
+        load_chunk_lds<H, false>(L, lds, qtab, ql, by, clo + kSubWin, nlen, nxt, nullptr);
       }
 #pragma unroll
       for (int j = 0; j < kSubWin; ++j) {
         if ((uint32_t)j < len) {
-          const float sa = cur.v[j].x + e[j];
+          __builtin_amdgcn_sched_barrier(0);
+          const float sa = cur.v[j].x + cur.e[j];
           D::template astep<false>(ac, D::branch(sa, cur.v[j].y), sa, nullptr);
         }
       }
@@ -391,33 +403,34 @@ __device__ __forceinline__ void siso_half(const Lane& L, float* __restrict__ lds
     if (L.first) load8(nii, ac);
   }
   const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
+  // global loads (staged LLRs, beta checkpoint) run one sub-window ahead;
+  // the LDS extrinsics of a sub-window are read at its start
   Chunk nxt;
+  float nck[8];
   load_chunk(sc, L.le, 0, min((uint32_t)kSubWin, S), nxt);
+  load8(ck, nck);
   for (uint32_t w = 0; w < L.nW; ++w) {
     const uint32_t t0 = w * kSubWin;
     const uint32_t len = min((uint32_t)kSubWin, S - t0);
-    const Chunk cur = nxt;
-    if (w + 1 < L.nW) load_chunk(sc, L.le, t0 + kSubWin, min((uint32_t)kSubWin, S - t0 - kSubWin), nxt);
+    Chunk cur;
+#pragma unroll
+    for (int j = 0; j < kSubWin; ++j) cur.v[j] = nxt.v[j];
+    float bb[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) bb[i] = nck[i];
+    if (w + 1 < L.nW) {
+      load_chunk(sc, L.le, t0 + kSubWin, min((uint32_t)kSubWin, S - t0 - kSubWin), nxt);
+      load8(ck + 8 * (w + 1), nck);
+    }
+    uint32_t addr[kSubWin];
+    load_chunk_lds<H, true>(L, lds, qtab, L.q, by, t0, len, cur, addr);
     float bs[kSubWin][8];
     typename D::Br br[kSubWin];
     float isa[kSubWin];
-    uint32_t addr[kSubWin];
-    float bb[8];
-    load8(ck + 8 * w, bb);
-    uint32_t y = (H == 1) ? qpp_col0(L, L.q, t0 + len - 1) : 0u;
-    float e[kSubWin];
 #pragma unroll
     for (int j = kSubWin - 1; j >= 0; --j) {
       if ((uint32_t)j < len) {
-        addr[j] = lds_addr<H>(L, qtab, L.q, t0 + j, y);
-        e[j] = lds[addr[j]];
-        if constexpr (H == 1) y = y_back(y, by, L.M);
-      }
-    }
-#pragma unroll
-    for (int j = kSubWin - 1; j >= 0; --j) {
-      if ((uint32_t)j < len) {
-        isa[j] = cur.v[j].x + e[j];
+        isa[j] = cur.v[j].x + cur.e[j];
         br[j] = D::branch(isa[j], cur.v[j].y);
 #pragma unroll
         for (int i = 0; i < 8; ++i) bs[j][i] = bb[i];
