@@ -219,3 +219,70 @@ def test_plan_on_device_buffers():
     want, _, _ = T.decode_batch_host(ks, host, T.DECODER_WINDOW_LOG, 8)
     assert np.array_equal(y.cpu().numpy().view(np.uint32), want.view(np.uint32))
     assert it.cpu().tolist() == [8] * len(ks)
+
+
+# ------------------------------------------------------- early stop (CRC24B)
+def _crc_blocks(ks, fillers, ebn0s, seed):
+    """Code blocks that carry a CRC24B over bits [F, K) like segmented
+    transport blocks (filler bits zero), BPSK/AWGN LLRs."""
+    rng = np.random.default_rng(seed)
+    bits, llrs = [], []
+    for k, f, e in zip(ks, fillers, ebn0s):
+        payload = rng.integers(0, 2, k - f - 24).astype(np.uint8)
+        cb = np.concatenate([np.zeros(f, np.uint8), O.crc_append(O.CRC24B, payload)])
+        bits.append(cb)
+        llrs.append(np.concatenate(bpsk_awgn_llr(O.encode_cb(cb, f), k, e, rng)))
+    return bits, llrs
+
+
+@pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])
+def test_window_early_stop_matches_oracle(decoder):
+    """Stopping iteration and CRC verdict per block equal the oracle's
+    restatement; posteriors agree to the window-decoder bar.  Mixed K, so
+    some jobs pack several blocks per wavefront that stop at different
+    iterations (and one block at -1 dB that never passes)."""
+    ks = [1024, 6144, 6144, 40, 40, 40, 544, 544, 1024, 104]
+    fs = [8, 0, 24, 0, 0, 0, 8, 0, 0, 0]
+    eb = [1.0, 1.0, -1.0, 2.0, 0.0, 3.0, 1.5, 0.5, 1.2, 2.5]
+    bits, llrs = _crc_blocks(ks, fs, eb, 77)
+    post, crc, its = T.decode_batch_host(ks, np.concatenate(llrs), decoder, 8, True, fillers=fs)
+    off = 0
+    lin = decoder == T.DECODER_WINDOW_LIN
+    for i, (k, f, l) in enumerate(zip(ks, fs, llrs)):
+        geo = T.segment_geometry(k, decoder)
+        ref, ok, used = O.decode_window(*_split(l, k), k, geo["segment_length"], geo["warmup"], lin, f, 8, True)
+        _say(f"block {i}: K={k} F={f} gpu it={its[i]} crc={crc[i]}  oracle it={used} crc={ok}")
+        assert (int(its[i]), bool(crc[i])) == (used, ok)
+        assert_llr_parity(post[off:off + k], ref, atol=1e-3, rtol=1e-4)
+        off += k
+    assert crc.sum() >= 6 and not crc[2]
+
+
+def test_window_early_stop_equals_fixed_iterations():
+    """A block that stops at iteration n has bit-for-bit the posterior of the
+    same decoder run for exactly n iterations without early stop."""
+    ks = [6144, 1024, 40, 40]
+    fs = [0, 8, 0, 0]
+    bits, llrs = _crc_blocks(ks, fs, [1.0, 1.0, 2.0, 1.0], 5)
+    post, crc, its = T.decode_batch_host(ks, np.concatenate(llrs), T.DECODER_WINDOW_LOG, 8, True, fillers=fs)
+    off = 0
+    for i, k in enumerate(ks):
+        assert crc[i] and its[i] < 8
+        fixed, _, _ = T.decode_batch_host([k], llrs[i], T.DECODER_WINDOW_LOG, int(its[i]))
+        assert np.array_equal(post[off:off + k].view(np.uint32), fixed.view(np.uint32))
+        off += k
+
+
+@pytest.mark.parametrize("backend", ["GpuStage1Exact", "GpuStage2Windowed"])
+def test_transport_block_early_stop(backend):
+    """Segmented TB (C=2, CRC24B per block) with early stop on: decoded,
+    CRC-clean, and fewer iterations than the maximum (noise-free inputs)."""
+    bits, p, enc = _tb(12000, (17, 9), iters=8)
+    p.enable_early_stop = True
+    cpu = T.decode_transport_block(p, enc, T.DecoderBackend.CpuReference)
+    dec = T.decode_transport_block(p, enc, getattr(T.DecoderBackend, backend))
+    assert np.array_equal(dec.tb_bits, bits) and dec.transport_block_crc_ok
+    assert dec.iterations_used < 8
+    if backend == "GpuStage1Exact":
+        assert dec.iterations_used == cpu.iterations_used
+    assert all(dec.code_block_crc_ok)

@@ -363,8 +363,6 @@ def _gpu_decode(decoder, descriptor, d0, d1, d2, max_iterations, enable_early_st
     d0, d1, d2 = _spans(d0, d1, d2)
     post = np.zeros(descriptor.k_r, np.float32)
     ok, it = C.c_int32(0), C.c_uint32(0)
-    if enable_early_stop:
-        decoder = DECODER_EXACT  # device CRC24B early stop lives in the exact decoder
     o = options(decoder, max_iterations, enable_early_stop, **kw)
     cd = descriptor.c()
     _check(lib().tc_decode_code_block(C.byref(cd), d0, len(d0), d1, len(d1), d2, len(d2), C.byref(o), post,
@@ -466,8 +464,7 @@ def decode_transport_block(params: LteTurboCodecParams, encoded: EncodedTranspor
             oks.append(ok)
             iters.append(it)
     else:
-        dec = DECODER_EXACT if (backend == DecoderBackend.GpuStage1Exact or params.enable_early_stop) \
-            else DECODER_WINDOW_LOG
+        dec = DECODER_EXACT if backend == DecoderBackend.GpuStage1Exact else DECODER_WINDOW_LOG
         ks = [b.descriptor.k_r for b in blocks]
         post, crc, its = decode_batch_host(ks, np.concatenate(llrs), dec, params.max_iterations,
                                            params.enable_early_stop,

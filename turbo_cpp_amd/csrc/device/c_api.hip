@@ -20,10 +20,10 @@
 namespace tcdev {
 __global__ void exact_decode_kernel(const CbInfo*, uint32_t, const float*, float*, float*, int32_t*, uint32_t*,
                                     uint32_t, int32_t);
-__global__ void window_decode_log_kernel(const WinJob*, uint32_t, const uint32_t*, const CbInfo*, const uint32_t*,
-                                         const float*, float*, float2*, int32_t*, uint32_t*, uint32_t);
-__global__ void window_decode_lin_kernel(const WinJob*, uint32_t, const uint32_t*, const CbInfo*, const uint32_t*,
-                                         const float*, float*, float2*, int32_t*, uint32_t*, uint32_t);
+__global__ void window_decode_log_kernel(WinArgs);
+__global__ void window_decode_lin_kernel(WinArgs);
+__global__ void window_decode_log_es_kernel(WinArgs);
+__global__ void window_decode_lin_es_kernel(WinArgs);
 }  // namespace tcdev
 
 namespace {
@@ -93,6 +93,13 @@ tc_segment_info geometry(uint32_t k, const tc_options& o) {
   return g;
 }
 
+using WinKernel = void (*)(tcdev::WinArgs);
+WinKernel window_kernel(const tc_options& o) {
+  if (o.decoder == TC_DECODER_WINDOW_LOG)
+    return o.early_stop ? tcdev::window_decode_log_es_kernel : tcdev::window_decode_log_kernel;
+  return o.early_stop ? tcdev::window_decode_lin_es_kernel : tcdev::window_decode_lin_kernel;
+}
+
 template <typename T>
 T* dev_alloc(size_t n) {
   void* p = nullptr;
@@ -122,12 +129,14 @@ struct tc_plan {
   tcdev::WinJob* d_jobs = nullptr;
   uint32_t* d_cb_index = nullptr;
   uint32_t* d_qtab = nullptr;
+  uint32_t* d_crc_pow = nullptr;
   float2* d_scratch = nullptr;
   uint32_t njobs = 0, grid = 0;
   size_t lds_bytes = 0;
 
   ~tc_plan() {
-    for (void* p : {(void*)d_cbs, (void*)d_work, (void*)d_jobs, (void*)d_cb_index, (void*)d_qtab, (void*)d_scratch})
+    for (void* p : {(void*)d_cbs, (void*)d_work, (void*)d_jobs, (void*)d_cb_index, (void*)d_qtab, (void*)d_crc_pow,
+                    (void*)d_scratch})
       if (p) (void)hipFree(p);
   }
 };
@@ -205,12 +214,11 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
     return a.seg + 2 * a.warm > b.seg + 2 * b.warm;
   });
   p.njobs = static_cast<uint32_t>(jobs.size());
-  p.lds_bytes = static_cast<size_t>(max_seg) * tcdev::kLdsRow * sizeof(float);
+  p.lds_bytes = (static_cast<size_t>(max_seg) * tcdev::kLdsRow + tcdev::kLdsExtraWords(max_seg)) * sizeof(float);
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
   int per_cu = 0;
-  const void* fn = (o.decoder == TC_DECODER_WINDOW_LOG) ? (const void*)tcdev::window_decode_log_kernel
-                                                        : (const void*)tcdev::window_decode_lin_kernel;
+  const void* fn = reinterpret_cast<const void*>(window_kernel(o));
   hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, tcdev::kWaveLanes, p.lds_bytes),
             "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   per_cu = std::max(per_cu, 1);
@@ -222,6 +230,15 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   p.d_qtab = dev_alloc<uint32_t>(qtab.size());
   dev_upload(p.d_qtab, qtab);
   p.d_scratch = dev_alloc<float2>(static_cast<size_t>(p.grid) * tcdev::kSlotFloat2);
+  // D^m mod G(D) for the CRC24B generator (lte_turbo.cpp:108-111)
+  std::vector<uint32_t> pw(tcdev::kCrcPowLen);
+  uint32_t r = 1;
+  for (int m = 0; m < tcdev::kCrcPowLen; ++m) {
+    pw[m] = r;
+    r = ((r << 1) & 0xFFFFFFu) ^ (((r >> 23) & 1u) ? 0xC00063u : 0u);
+  }
+  p.d_crc_pow = dev_alloc<uint32_t>(pw.size());
+  dev_upload(p.d_crc_pow, pw);
 }
 
 void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* iters, hipStream_t s) {
@@ -232,20 +249,15 @@ void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* i
     hipLaunchKernelGGL(tcdev::exact_decode_kernel, dim3(blocks), dim3(64), 0, s, p.d_cbs,
                        static_cast<uint32_t>(p.n_cb), llr, post, p.d_work, crc, iters, o.max_iterations,
                        o.early_stop);
-  } else if (o.decoder == TC_DECODER_WINDOW_LOG) {
-    hipLaunchKernelGGL(tcdev::window_decode_log_kernel, dim3(p.grid), dim3(64), p.lds_bytes, s, p.d_jobs, p.njobs,
-                       p.d_cb_index, p.d_cbs, p.d_qtab, llr, post, p.d_scratch, crc, iters, o.max_iterations);
   } else {
-    hipLaunchKernelGGL(tcdev::window_decode_lin_kernel, dim3(p.grid), dim3(64), p.lds_bytes, s, p.d_jobs, p.njobs,
-                       p.d_cb_index, p.d_cbs, p.d_qtab, llr, post, p.d_scratch, crc, iters, o.max_iterations);
+    tcdev::WinArgs a{p.d_jobs, p.njobs, p.d_cb_index, p.d_cbs, p.d_qtab, p.d_crc_pow, llr, post, p.d_scratch,
+                     crc, iters, o.max_iterations, o.early_stop};
+    hipLaunchKernelGGL(window_kernel(o), dim3(p.grid), dim3(64), p.lds_bytes, s, a);
   }
   hip_check(hipGetLastError(), "kernel launch");
 }
 
-void check_early_stop_support(const tc_options& o) {
-  if (o.early_stop && o.decoder != TC_DECODER_EXACT)
-    throw Failure{TC_ERR_INVALID_ARGUMENT, "early stop is implemented for the exact decoder only"};
-}
+void check_early_stop_support(const tc_options&) {}
 
 // Per-thread cache of single-block plans and staging buffers, so repeated
 // per-code-block calls do not allocate.

@@ -44,4 +44,25 @@ struct WinJob {
   uint32_t pad0, pad1;
 };
 
+// Arguments of the window kernels.
+struct WinArgs {
+  const WinJob* jobs;
+  uint32_t njobs;
+  const uint32_t* cb_index;
+  const CbInfo* cbs;
+  const uint32_t* qtab;
+  const uint32_t* crc_pow;  // D^m mod G(D) of CRC24B for m < kCrcPowLen
+  const float* llr;
+  float* post;
+  float2* scratch;
+  int32_t* crc_ok;
+  uint32_t* iters;
+  uint32_t max_iterations;
+  int32_t early_stop;
+};
+constexpr int kCrcPowLen = 6144;
+// LDS floats of one window job beyond the extrinsics: hard-decision bitmap
+// ((S*65+31)/32 words) and one CRC partial per lane.
+constexpr int kLdsExtraWords(int s) { return (s * kLdsRow + 31) / 32 + kWaveLanes; }
+
 }  // namespace tcdev

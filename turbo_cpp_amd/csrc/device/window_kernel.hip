@@ -337,15 +337,28 @@ __device__ __forceinline__ void beta_pass(const Lane& L, const float* __restrict
   }
 }
 
+// CRC24B early stop (lte_turbo.cpp:792-804) state of this lane's block.
+// With on set, every lower half writes the posterior of each bit to HBM and
+// its hard decision to an LDS bitmap indexed like the extrinsics; a block
+// whose check passed is done and its lanes stop writing.
+struct EarlyStop {
+  bool on;
+  bool done;
+  float* post_cb;
+  uint32_t* bitmap;
+};
+constexpr uint32_t kDiv65Magic = 66076420u;  // ceil(2^32 / 65): exact a / 65 for a < 2^16
+static_assert(kLdsRow == 65, "kDiv65Magic assumes 65-float LDS rows");
+
 // One SISO half (H = 0 upper, 1 lower) for this lane's segment.
 // nii: this lane's boundary metrics of this half, [alpha 8 | beta 8]
 // (alpha at step -g, beta at step S+g of the segment); ck: this lane's beta
 // checkpoints at sub-window boundaries, [w][8].  The neighbours' records
 // are reached with nii_stride (floats between consecutive lanes' records).
-template <class D, int H>
+template <class D, int H, bool ES>
 __device__ __forceinline__ void siso_half(const Lane& L, float* __restrict__ lds, const float2* __restrict__ sc,
                                           const uint32_t* __restrict__ qtab, float* __restrict__ nii,
-                                          float* __restrict__ ck, bool write_post) {
+                                          float* __restrict__ ck, bool write_post, const EarlyStop& es) {
   const uint32_t S = L.S, g = L.g;
   // ---------------- backward pass: warm-up over the next segment's first g
   // steps, then the segment, storing beta at sub-window boundaries
@@ -452,26 +465,63 @@ __device__ __forceinline__ void siso_half(const Lane& L, float* __restrict__ lds
         __builtin_amdgcn_sched_barrier(0);  // keep steps in order: bounds register pressure
         const float ext = D::template astep<true>(ac, br[j], isa[j], bs[j]);
         const float val = (H == 1 && write_post) ? isa[j] + ext : ext;
-        if (L.active) lds[addr[j]] = val;
+        if (L.active && !es.done) {
+          lds[addr[j]] = val;
+          if constexpr (H == 1 && ES) {
+            if (es.on) {
+              // posterior of bit pi(j) to HBM, its hard decision into the bitmap
+              const float p = isa[j] + ext;
+              const uint32_t row = __umulhi(addr[j], kDiv65Magic);
+              const uint32_t col = addr[j] - row * kLdsRow;
+              es.post_cb[(col - L.colb) * S + row] = p;
+              if (p >= 0.0f) atomicOr(&es.bitmap[addr[j] >> 5], 1u << (addr[j] & 31));
+            }
+          }
+        }
       }
     }
   }
 }
 
-template <class D>
-__device__ void window_decode(const WinJob* __restrict__ jobs, uint32_t njobs, const uint32_t* __restrict__ cb_index,
-                              const CbInfo* __restrict__ cbs, const uint32_t* __restrict__ qtab,
-                              const float* __restrict__ llr, float* __restrict__ post, float2* __restrict__ scratch,
-                              int32_t* __restrict__ crc_ok, uint32_t* __restrict__ iters, uint32_t max_iterations) {
+// remainder test of CRC24B (generator of lte_turbo.cpp:108-111) on the
+// hard decisions of bits [F, K) of every block of the job, from the LDS
+// bitmap: bit i contributes D^(K-1-i) mod G.  Each lane folds its own
+// segment (natural order: r(i+1) = r(i) * D^-1 mod G), then the first lane of
+// each block XORs its M partials.  Returns this lane's block verdict.
+__device__ bool crc24b_job(const Lane& L, const uint32_t* __restrict__ bitmap, uint32_t* __restrict__ partial,
+                           const uint32_t* __restrict__ crc_pow, uint32_t filler) {
+  const uint32_t S = L.S, K = L.K, lane = threadIdx.x;
+  uint32_t x = 0;
+  uint32_t r = crc_pow[K - 1 - L.q * S];
+  for (uint32_t t = 0; t < S; ++t) {
+    const uint32_t i = L.q * S + t;
+    const uint32_t a = t * kLdsRow + L.colb + L.q;
+    if (i >= filler && ((bitmap[a >> 5] >> (a & 31)) & 1u)) x ^= r;
+    r = (r & 1u) ? ((r ^ (0xC00063u | (1u << 24))) >> 1) : (r >> 1);
+  }
+  partial[lane] = x;
+  __syncthreads();
+  uint32_t y = 0;
+  for (uint32_t m = 0; m < L.M; ++m) y ^= partial[L.colb + m];
+  return y == 0 && K - filler >= 24;
+}
+
+template <class D, bool ES>
+__device__ void window_decode(const WinArgs& A) {
   extern __shared__ float lds[];
   const uint32_t lane = threadIdx.x;
-  float2* sc_up = scratch + (size_t)blockIdx.x * kSlotFloat2;
+  const uint32_t* __restrict__ cb_index = A.cb_index;
+  const CbInfo* __restrict__ cbs = A.cbs;
+  const uint32_t* __restrict__ qtab = A.qtab;
+  const float* __restrict__ llr = A.llr;
+  float2* sc_up = A.scratch + (size_t)blockIdx.x * kSlotFloat2;
   float2* sc_lo = sc_up + kMaxSeg * kWaveLanes;
   // per-lane records: boundary metrics [lane][half][16], checkpoints [lane][w][8]
   float* nii = reinterpret_cast<float*>(sc_lo + kMaxSeg * kWaveLanes) + lane * 32;
   float* ck = reinterpret_cast<float*>(sc_lo + kMaxSeg * kWaveLanes) + kWaveLanes * 32 + lane * (8 * kMaxSubWins);
-  for (uint32_t jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
-    const WinJob J = jobs[jb];
+  const bool es_on = ES && A.max_iterations > 0;
+  for (uint32_t jb = blockIdx.x; jb < A.njobs; jb += gridDim.x) {
+    const WinJob J = A.jobs[jb];
     Lane L;
     L.S = J.seg;
     L.M = J.nseg;
@@ -495,6 +545,11 @@ __device__ void window_decode(const WinJob* __restrict__ jobs, uint32_t njobs, c
     const float* d0 = llr + ci.llr_off;
     const float* d1 = d0 + (K + 4);
     const float* d2 = d1 + (K + 4);
+    // LDS: extrinsics [S][65] floats, then the hard-decision bitmap and the
+    // CRC partials (early stop only)
+    uint32_t* bitmap = reinterpret_cast<uint32_t*>(lds + S * kLdsRow);
+    const uint32_t bitmap_words = (S * kLdsRow + 31) / 32;
+    uint32_t* partial = bitmap + bitmap_words;
 
     // ---- stage d0 of every block into LDS (transposed) ----
     for (uint32_t bb = 0; bb < J.nblk; ++bb) {
@@ -541,25 +596,45 @@ __device__ void window_decode(const WinJob* __restrict__ jobs, uint32_t njobs, c
     for (uint32_t i = lane; i < S * kLdsRow; i += kWaveLanes) lds[i] = 0.0f;
     __syncthreads();
 
-    for (uint32_t it = 0; it < max_iterations; ++it) {
-      siso_half<D, 0>(L, lds, sc_up, qtab, nii, ck, false);
+    EarlyStop es{es_on, false, A.post + ci.post_off, bitmap};
+    uint32_t used_it = A.max_iterations;
+    for (uint32_t it = 0; it < A.max_iterations; ++it) {
+      siso_half<D, 0, ES>(L, lds, sc_up, qtab, nii, ck, false, es);
       __syncthreads();
-      siso_half<D, 1>(L, lds, sc_lo, qtab, nii + 16, ck, it + 1 == max_iterations);
+      if (es_on) {
+        for (uint32_t i = lane; i < bitmap_words; i += kWaveLanes) bitmap[i] = 0u;
+        __syncthreads();
+      }
+      siso_half<D, 1, ES>(L, lds, sc_lo, qtab, nii + 16, ck, !es_on && it + 1 == A.max_iterations, es);
       __syncthreads();
-    }
-
-    // ---- posterior out, natural order ----
-    for (uint32_t bb = 0; bb < J.nblk; ++bb) {
-      float* dst = post + cbs[cb_index[J.cb_off + bb]].post_off;
-      for (uint32_t i = lane; i < K; i += kWaveLanes) {
-        const uint32_t qq = __umulhi(i, J.div_magic);
-        const uint32_t t = i - qq * S;
-        dst[i] = lds[t * kLdsRow + bb * M + qq];
+      if (es_on) {
+        const bool ok = crc24b_job(L, bitmap, partial, A.crc_pow, ci.filler);
+        if (ok && !es.done) {
+          es.done = true;
+          used_it = it + 1;
+          if (L.active && L.first) {
+            if (A.crc_ok) A.crc_ok[my_cb] = 1;
+            if (A.iters) A.iters[my_cb] = it + 1;
+          }
+        }
+        if (__syncthreads_and(es.done || !L.active)) break;
       }
     }
-    if (L.active && L.first) {
-      if (crc_ok) crc_ok[my_cb] = 0;
-      if (iters) iters[my_cb] = max_iterations;
+
+    if (!es_on) {
+      // ---- posterior out, natural order ----
+      for (uint32_t bb = 0; bb < J.nblk; ++bb) {
+        float* dst = A.post + cbs[cb_index[J.cb_off + bb]].post_off;
+        for (uint32_t i = lane; i < K; i += kWaveLanes) {
+          const uint32_t qq = __umulhi(i, J.div_magic);
+          const uint32_t t = i - qq * S;
+          dst[i] = lds[t * kLdsRow + bb * M + qq];
+        }
+      }
+    }
+    if (L.active && L.first && !es.done) {
+      if (A.crc_ok) A.crc_ok[my_cb] = 0;
+      if (A.iters) A.iters[my_cb] = used_it;
     }
     __syncthreads();
   }
@@ -567,20 +642,21 @@ __device__ void window_decode(const WinJob* __restrict__ jobs, uint32_t njobs, c
 
 }  // namespace
 
-__global__ __launch_bounds__(64, 2) void window_decode_log_kernel(
-    const WinJob* __restrict__ jobs, uint32_t njobs, const uint32_t* __restrict__ cb_index,
-    const CbInfo* __restrict__ cbs, const uint32_t* __restrict__ qtab, const float* __restrict__ llr,
-    float* __restrict__ post, float2* __restrict__ scratch, int32_t* __restrict__ crc_ok,
-    uint32_t* __restrict__ iters, uint32_t max_iterations) {
-  window_decode<LogDomain>(jobs, njobs, cb_index, cbs, qtab, llr, post, scratch, crc_ok, iters, max_iterations);
+__global__ __launch_bounds__(64, 2) void window_decode_log_kernel(WinArgs args) {
+  window_decode<LogDomain, false>(args);
 }
 
-__global__ __launch_bounds__(64, 2) void window_decode_lin_kernel(
-    const WinJob* __restrict__ jobs, uint32_t njobs, const uint32_t* __restrict__ cb_index,
-    const CbInfo* __restrict__ cbs, const uint32_t* __restrict__ qtab, const float* __restrict__ llr,
-    float* __restrict__ post, float2* __restrict__ scratch, int32_t* __restrict__ crc_ok,
-    uint32_t* __restrict__ iters, uint32_t max_iterations) {
-  window_decode<LinDomain>(jobs, njobs, cb_index, cbs, qtab, llr, post, scratch, crc_ok, iters, max_iterations);
+__global__ __launch_bounds__(64, 2) void window_decode_lin_kernel(WinArgs args) {
+  window_decode<LinDomain, false>(args);
+}
+
+// with CRC24B early stop
+__global__ __launch_bounds__(64, 2) void window_decode_log_es_kernel(WinArgs args) {
+  window_decode<LogDomain, true>(args);
+}
+
+__global__ __launch_bounds__(64, 2) void window_decode_lin_es_kernel(WinArgs args) {
+  window_decode<LinDomain, true>(args);
 }
 
 }  // namespace tcdev

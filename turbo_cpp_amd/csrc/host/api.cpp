@@ -33,8 +33,6 @@ std::vector<float> gpu_decode(int decoder, const CodeBlockDescriptor& desc, std:
   o.max_iterations = max_iterations;
   o.early_stop = early_stop ? 1 : 0;
   o.decoder = decoder;
-  // the windowed decoder has no device CRC24B test yet: early stop uses the exact one
-  if (early_stop) o.decoder = TC_DECODER_EXACT;
   const tc_cb_desc cd = to_c(desc);
   std::vector<float> post(desc.k_r, 0.0F);
   int32_t ok = 0;
@@ -118,8 +116,7 @@ DecodedTransportBlock decode_transport_block(const LteTurboCodecParams& params, 
     tc_options o{};
     o.max_iterations = params.max_iterations;
     o.early_stop = params.enable_early_stop ? 1 : 0;
-    o.decoder = (backend == DecoderBackend::GpuStage1Exact || params.enable_early_stop) ? TC_DECODER_EXACT
-                                                                                         : TC_DECODER_WINDOW_LOG;
+    o.decoder = (backend == DecoderBackend::GpuStage1Exact) ? TC_DECODER_EXACT : TC_DECODER_WINDOW_LOG;
     std::vector<std::uint32_t> ks(c), fs(c);
     std::vector<float> packed;
     std::size_t post_total = 0;
