@@ -5,7 +5,12 @@ rate-1/3 mother code with 12 tail bits (18444 coded bits per block), info
 bits i.i.d. Bernoulli(0.5) from a seed, BPSK (bit c -> 2c-1) over AWGN with
 block i at Eb/N0 = 0.25 * (i mod 7) dB (the 0.0..1.5 dB sweep), fp32 LLRs
 2y/sigma^2, 8 iterations, no early stop.  One step = one batched decode of
-all 4096 blocks with the inputs resident in HBM.
+all 4096 blocks with the inputs resident in HBM.  Each block carries a
+transport block of 6120 payload bits + CRC24A (K = 6144, no filler).
+
+A second leg runs the receive chain on the same blocks as 4096 transport
+blocks (rv 0, E = 18444): soft de-rate matching, the batched decode and
+transport-block assembly (CRC24A), and reports its time and TB CRC pass rate.
 
 Printed: one JSON line (metric, value in info Gbit/s, roofline of the decode
 kernel, the reference CPU decoder timed on this host, the plain
@@ -42,10 +47,13 @@ def log(*a):
 
 
 def make_inputs(n, seed=12345):
+    """Blocks are transport blocks of B = K - 24 payload bits + CRC24A, so the
+    receive-chain leg can check each TB CRC."""
     import turbo_cpp_amd as T
 
     rng = np.random.default_rng(seed)
-    bits = rng.integers(0, 2, (n, K)).astype(np.uint8)
+    payload = rng.integers(0, 2, (n, K - 24)).astype(np.uint8)
+    bits = np.stack([T.append_crc24a(payload[i]) for i in range(n)])
     coded = np.empty((n, 3 * (K + 4)), np.float32)
     for i in range(n):
         d0, d1, d2 = T.encode_code_block_streams(bits[i])
@@ -103,6 +111,119 @@ def cpu_baseline(llr):
             "single_thread_gbit_s": single}
 
 
+def rx_chain_leg(T, torch, stream, llr, decode_ms):
+    """Receive chain on the bench blocks: received LLRs in circular-buffer
+    order (rv 0, every non-NULL entry once) -> TbPlan (de-rate matching,
+    decode, TB assembly)."""
+    n, e, d = N_CB, 3 * (K + 4), K + 4
+    # position map of one block: derate of 1..E gives, per [d0|d1|d2] slot, 1 + its received index
+    where = T.derate_match_soft(np.arange(1, e + 1, dtype=np.float32), K).astype(np.int64) - 1
+    assert (where >= 0).all() and np.unique(where).size == e
+    rx = np.empty((n, e), np.float32)
+    rx[:, where] = llr
+    d_rx = torch.from_numpy(rx.reshape(-1)).cuda()
+    tb = T.TbPlan([(K - 24, e, 0, 1, 2)] * n, T.DECODER_WINDOW_LOG, ITERS)
+    der = T.DeratePlan([K] * n, [e] * n)
+    soft = torch.empty(n * 3 * d, dtype=torch.float32, device="cuda")
+    bits = torch.empty(tb.bits_total, dtype=torch.uint8, device="cuda")
+    ok = torch.empty(n, dtype=torch.int32, device="cuda")
+    s = stream.cuda_stream
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    der_ms = timed(lambda: der.run(d_rx.data_ptr(), soft.data_ptr(), stream=s), 10)
+    # the derate output must be the bench's decoder input
+    assert torch.equal(soft.view(n, 3 * d).cpu(), torch.from_numpy(llr))
+    tb_ms = timed(lambda: tb.decode(d_rx.data_ptr(), bits.data_ptr(), ok.data_ptr(), stream=s), 3)
+    okh = ok.cpu().numpy().astype(bool)
+    ebn0 = np.array([EBN0[i % len(EBN0)] for i in range(n)])
+    der_bytes = n * (4 * e + 4 * 3 * d)  # received LLRs in, decoder input out
+    der_gbs = der_bytes / (der_ms / 1e3) / 1e9
+    log(f"rx chain: derate {der_ms:.3f} ms ({der_gbs:.0f} GB/s), full TB decode {tb_ms:.3f} ms")
+    return {"transport_blocks": n, "tb_bits": K - 24, "ms": tb_ms, "tb_gbit_s": n * (K - 24) / tb_ms / 1e6,
+            "derate_ms": der_ms, "assembly_ms_est": max(tb_ms - der_ms - decode_ms, 0.0),
+            "derate_roofline": {"bound": "hbm", "achieved": der_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": der_gbs / HBM_PEAK_GBS, "bytes_per_launch": der_bytes},
+            "tb_crc_pass": {f"{x:.2f}": float(okh[ebn0 == x].mean()) for x in EBN0}}
+
+
+def diff_leg(T, torch, train_steps):
+    """Differentiable decoder: 2048 blocks of K=1024 (f1=31, f2=64), BPSK/AWGN
+    0.5 dB, 6 iterations with 12 per-half-iteration extrinsic scales (init
+    1.0), BCE loss of the posteriors.  Times forward and adjoint, fits the
+    scales with Adam, and compares held-out BER with the plain decoder."""
+    from turbo_cpp_amd.autograd import WeightedTurboDecoder
+
+    n, k, it = 2048, 1024, 6
+    rng = np.random.default_rng(777)
+
+    def data():
+        bits = rng.integers(0, 2, (n, k)).astype(np.uint8)
+        rate = k / (3 * k + 12)
+        sigma2 = 1.0 / (2 * rate * 10 ** (0.5 / 10))
+        coded = np.stack([np.concatenate(T.encode_code_block_streams(b)) for b in bits]).astype(np.float32)
+        y = (2.0 * coded - 1.0) + np.sqrt(sigma2) * rng.standard_normal(coded.shape, dtype=np.float32)
+        return (torch.from_numpy(bits).cuda().float(), torch.from_numpy((2.0 * y / sigma2).astype(np.float32)).cuda())
+
+    y_tr, x_tr = data()
+    y_te, x_te = data()
+    dec = WeightedTurboDecoder([k] * n, iterations=it)
+    bce = torch.nn.functional.binary_cross_entropy_with_logits
+    stream = torch.cuda.current_stream()
+    a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    x_req = x_tr.clone().requires_grad_(True)
+    fwd, bwd = [], []
+    for rep in range(4):
+        dec.zero_grad()
+        a.record(stream)
+        loss = bce(dec(x_req), y_tr)
+        b.record(stream)
+        loss.backward()
+        c.record(stream)
+        torch.cuda.synchronize()
+        if rep:
+            fwd.append(a.elapsed_time(b))
+            bwd.append(b.elapsed_time(c))
+    fwd_ms, bwd_ms = float(np.mean(fwd)), float(np.mean(bwd))
+
+    def ber(x, y):
+        with torch.no_grad():
+            hard = (dec(x) >= 0).float()
+        return float((hard != y).float().mean()), float((hard != y).any(dim=1).float().mean())
+
+    ber_plain = ber(x_te, y_te)
+    opt = torch.optim.Adam(dec.parameters(), lr=0.01)
+    losses = []
+    t0 = time.perf_counter()
+    for step in range(train_steps):
+        opt.zero_grad()
+        loss = bce(dec(x_tr), y_tr)
+        loss.backward()
+        opt.step()
+        if step % 50 == 0 or step == train_steps - 1:
+            losses.append(round(float(loss.detach()), 6))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    ber_w = ber(x_te, y_te)
+    log(f"diff: fwd {fwd_ms:.2f} ms bwd {bwd_ms:.2f} ms; train {train_steps} steps {train_s:.1f} s; "
+        f"BER plain {ber_plain[0]:.2e} weighted {ber_w[0]:.2e}")
+    return {"workload": "2048 code blocks K=1024, BPSK/AWGN 0.5 dB, 6 iterations, 12 extrinsic scales, BCE",
+            "fwd_ms": fwd_ms, "bwd_ms": bwd_ms, "fwd_gbit_s": n * k / fwd_ms / 1e6, "bwd_gbit_s": n * k / bwd_ms / 1e6,
+            "adam_steps": train_steps, "train_s": train_s, "loss_every_50": losses,
+            "scales": [round(float(v), 4) for v in dec.scales.detach().cpu()],
+            "held_out": {"ber_plain": ber_plain[0], "bler_plain": ber_plain[1], "ber_weighted": ber_w[0],
+                         "bler_weighted": ber_w[1], "blocks": n}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +233,8 @@ def main():
     ap.add_argument("--plain-blocks", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
+    ap.add_argument("--no-rx-chain", action="store_true")
+    ap.add_argument("--train-steps", type=int, default=500, help="Adam steps of the differentiation leg (0: skip it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -199,6 +322,9 @@ def main():
     plain_gbits = K / plain_s / 1e9
     log(f"plain: {plain_s * 1e3:.3f} ms per block -> {plain_gbits * 1e3:.3f} Mbit/s")
 
+    rx_chain = None if args.no_rx_chain else rx_chain_leg(T, torch, stream, llr, kernel_ms)
+    diff = diff_leg(T, torch, args.train_steps) if args.train_steps > 0 else None
+
     if args.copy_calibration:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
         dst = torch.empty_like(src)
@@ -246,6 +372,9 @@ def main():
         "modes": modes,
         "ber": ber,
         "kernel_ms": kernel_ms,
+        "rx_chain": rx_chain,
+        "bwd_gbit_s": diff["bwd_gbit_s"] if diff else None,
+        "differentiation": diff,
     }
     print(json.dumps(out), flush=True)
 

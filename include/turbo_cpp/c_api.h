@@ -100,6 +100,103 @@ tc_status tc_decode_batch_host(const uint32_t* k, const uint32_t* filler, size_t
                                const tc_options* opts, const float* llr, float* post,
                                int32_t* crc_ok, uint32_t* iterations_used);
 
+/* ---- soft de-rate matching with HARQ combining (GPU) ----------------------
+ * Float-LLR form of the reference's de-rate matcher (lte_turbo.cpp:280-325,
+ * called per code block from lte_turbo.cpp:722-734, which takes hard bits and
+ * adds +-8 per bit): received LLR k of a code block (bit 1 > 0) is added onto
+ * its circular-buffer entry, walking from k0(rv) and skipping NULL entries
+ * (repetitions summed in k order), and every entry lands on its stream
+ * position; NULL (filler) positions get 0.  +-8 inputs reproduce the
+ * reference's output bit for bit.  With accumulate != 0 the result is added
+ * to what the output already holds (HARQ soft combining of retransmissions,
+ * e.g. rv 0 then rv 2); positions the transmission does not reach keep their
+ * value.
+ * Input: the E_r LLRs of every code block back to back.  Output: the decoder
+ * plan's input layout ([d0|d1|d2] of K+4 floats per block, blocks back to
+ * back), so a run feeds tc_plan_decode on the same stream directly.
+ * rv (run): -1 uses the per-block rv given at creation (NULL there = 0),
+ * 0..3 applies to every block.  A run allocates nothing; runs on one plan
+ * from several streams do not conflict (the plan is read-only). */
+typedef struct tc_derate_plan tc_derate_plan;
+tc_status tc_derate_plan_create(const uint32_t* k, const uint32_t* filler, const uint64_t* e,
+                                const uint32_t* rv, size_t n_cb, tc_derate_plan** plan);
+void tc_derate_plan_destroy(tc_derate_plan* plan);
+size_t tc_derate_plan_rx_floats(const tc_derate_plan* plan);
+size_t tc_derate_plan_llr_floats(const tc_derate_plan* plan);
+tc_status tc_derate_plan_run(tc_derate_plan* plan, const float* rx_dev, int32_t rv, int32_t accumulate,
+                             float* llr_dev, void* hip_stream);
+/* One code block from host buffers (out: 3 (K+4) floats; read first when
+ * accumulate != 0). */
+tc_status tc_derate_match_soft(const float* rx, uint64_t e, uint32_t k, uint32_t filler, uint32_t rv,
+                               int32_t accumulate, float* out);
+
+/* ---- batched transport-block decoding (GPU) --------------------------------
+ * The reference's decode_transport_block (lte_turbo.cpp:891-980) for many
+ * transport blocks at once, from received LLRs: soft de-rate matching (with
+ * HARQ combining), turbo decoding of every code block in one batched launch,
+ * and TB assembly on the device (hard decisions, filler strip, CRC24B per code
+ * block when C > 1, CRC24A per TB, iterations = max over the TB's blocks).
+ * Segmentation and E_r follow encode_transport_block (lte_turbo.cpp:540-604,
+ * 167-185, 690-720) from each descriptor.
+ * Device buffers of tc_tb_plan_decode: rx = sum E_r floats (blocks of all TBs
+ * back to back, TB order); bits = sum B bytes (0/1, TB payloads back to back);
+ * tb_crc_ok / iterations: one per TB; cb_crc_ok: one per code block;
+ * posterior: sum K_r floats (NULL: kept inside the plan).  Any output may be
+ * NULL.  The plan holds the decoder input between calls: accumulate != 0 adds
+ * this transmission onto it (HARQ), rv as in tc_derate_plan_run.  A call
+ * allocates nothing; calls on one plan must not overlap (shared buffers). */
+typedef struct {
+  uint64_t transport_block_bits; /* B: payload bits without the CRC24A */
+  uint64_t g_total;              /* coded bits for the TB */
+  uint32_t rv;
+  uint32_t num_layers;
+  uint32_t mod_order;
+} tc_tb_desc;
+typedef struct tc_tb_plan tc_tb_plan;
+tc_status tc_tb_plan_create(const tc_tb_desc* tbs, size_t n_tb, const tc_options* opts,
+                            tc_tb_plan** plan);
+void tc_tb_plan_destroy(tc_tb_plan* plan);
+/* sizes[0] = code blocks, [1] = rx floats, [2] = payload bits, [3] = posterior floats. */
+tc_status tc_tb_plan_sizes(const tc_tb_plan* plan, uint64_t sizes[4]);
+/* K_r, F_r and E_r of every code block (each array may be NULL). */
+tc_status tc_tb_plan_code_blocks(const tc_tb_plan* plan, uint32_t* k, uint32_t* filler, uint64_t* e);
+tc_status tc_tb_plan_decode(tc_tb_plan* plan, const float* rx_dev, int32_t rv, int32_t accumulate,
+                            uint8_t* bits_dev, int32_t* tb_crc_ok_dev, int32_t* cb_crc_ok_dev,
+                            uint32_t* iterations_dev, float* posterior_dev, void* hip_stream);
+/* Same from host buffers (one transmission, no HARQ). */
+tc_status tc_decode_transport_blocks_host(const tc_tb_desc* tbs, size_t n_tb, const tc_options* opts,
+                                          const float* rx, uint8_t* bits, int32_t* tb_crc_ok,
+                                          int32_t* cb_crc_ok, uint32_t* iterations, float* posterior);
+
+/* ---- differentiable decoder (GPU) ------------------------------------------
+ * No counterpart in the reference: a sliding-window Log-MAP turbo decoder
+ * whose half-iteration h multiplies its extrinsic output by a learnable scale
+ * w[h] wherever it is used (next half's a-priori input; the last two halves
+ * also in the posterior post = d0 + w[2I-2] e1 + w[2I-1] e2), with the
+ * hand-written adjoint of its alpha/beta/LLR recursions.  With w = 1 and one
+ * segment per block (K <= min_segment) it is the reference decoder
+ * (lte_turbo.cpp:736-813, no early stop).  Segments of S steps
+ * (S = smallest divisor of K >= max(min_segment, K/64)) start their alpha and
+ * beta recursions `warmup` (<= S) steps outside the segment from zero metrics.
+ * fp64 != 0 runs every buffer in double (for finite-difference checks),
+ * else float.  Layouts: llr = [d0|d1|d2] of K+4 per block, post K per block,
+ * scales 2I, saved = the forward's extrinsics for the backward (sizes from
+ * tc_diff_plan_sizes: [0] llr, [1] post, [2] scales, [3] saved elements).
+ * grad_llr is overwritten (same layout as llr); grad_scales gets 2I values.
+ * Calls enqueue on the given stream and allocate nothing; calls on one plan
+ * must not overlap (shared work space). */
+typedef struct tc_diff_plan tc_diff_plan;
+tc_status tc_diff_plan_create(const uint32_t* k, size_t n_cb, uint32_t iterations,
+                              uint32_t min_segment, uint32_t warmup, int32_t fp64,
+                              tc_diff_plan** plan);
+void tc_diff_plan_destroy(tc_diff_plan* plan);
+tc_status tc_diff_plan_sizes(const tc_diff_plan* plan, uint64_t sizes[4]);
+tc_status tc_diff_forward(tc_diff_plan* plan, const void* llr, const void* scales, void* post,
+                          void* saved, void* hip_stream);
+tc_status tc_diff_backward(tc_diff_plan* plan, const void* llr, const void* scales,
+                           const void* saved, const void* grad_post, void* grad_llr,
+                           void* grad_scales, void* hip_stream);
+
 /* ---- host codec (the reference's CPU chain, bit-exact) ---------------------
  * CRC kinds: 0 = CRC24A, 1 = CRC24B, 2 = CRC16, 3 = CRC8 with the
  * reference's generator polynomials (lte_turbo.cpp:104-116). */

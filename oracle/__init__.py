@@ -70,6 +70,9 @@ def lib():
         L.or_compute_e.restype = C.c_int
         L.or_rate_match.argtypes = [_u8p, _u8p, _u8p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint32, _u8p]
         L.or_derate_match.argtypes = [_u8p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint32, _f32p]
+        L.or_derate_match_soft.argtypes = [_f32p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint32, _f32p, C.c_int]
+        L.or_assemble_tb.argtypes = [_f32p, _u32p, _u32p, C.c_size_t, C.c_size_t, _u8p, C.c_void_p,
+                                     C.POINTER(C.c_int)]
         L.or_siso.argtypes = [_f32p, _f32p, _f32p, C.c_uint32, _f32p, _f32p, _f32p]
         L.or_decode_cb.argtypes = [_f32p, _f32p, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
                                    _f32p, C.POINTER(C.c_int), C.POINTER(C.c_uint32)]
@@ -110,6 +113,8 @@ def ref():
         R.ref_tb_roundtrip.argtypes = [C.c_size_t, C.c_size_t, C.c_uint32, C.c_uint32, C.c_int, _u8p, _u8p,
                                        C.POINTER(C.c_int), C.POINTER(C.c_uint32), C.c_void_p,
                                        C.POINTER(C.c_size_t)]
+        R.ref_decode_tb.argtypes = [C.c_size_t, C.c_size_t, C.c_uint32, C.c_uint32, C.c_int, _u8p, _u8p, _u8p,
+                                    C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_uint32), _f32p]
         _ref = R
     return _ref
 
@@ -201,6 +206,30 @@ def derate_match(bits, d, filler, rv):
     return out
 
 
+def derate_match_soft(llr, d, filler, rv, out=None):
+    """Soft de-rate matching; with out given, adds into it (HARQ combining)."""
+    llr = _f32(llr)
+    acc = out is not None
+    if out is None:
+        out = np.zeros(3 * d, np.float32)
+    lib().or_derate_match_soft(llr, len(llr), d, filler, rv, out, int(acc))
+    return out
+
+
+def assemble_tb(post, ks, fillers, b):
+    """Hard decisions, filler strip, CRC24B per block and CRC24A per TB.
+
+    Returns (payload bits (b), per-block CRC ok (bool array), TB CRC ok)."""
+    ks = np.ascontiguousarray(ks, dtype=np.uint32)
+    fs = np.ascontiguousarray(fillers, dtype=np.uint32)
+    bits = np.zeros(b, np.uint8)
+    cb = np.zeros(len(ks), np.int32)
+    tb = C.c_int()
+    if lib().or_assemble_tb(_f32(post), ks, fs, len(ks), b, bits, cb.ctypes.data, C.byref(tb)):
+        raise ValueError("code blocks do not add up to b + 24 bits")
+    return bits, cb.astype(bool), bool(tb.value)
+
+
 def siso(sys, parity, apriori, tail_sys, tail_par):
     k = len(sys)
     ext = np.zeros(k, np.float32)
@@ -279,6 +308,23 @@ def ref_encode_cb(cb_bits, filler=0):
     if ref().ref_encode_cb(cb_bits, k, filler, d[0], d[1], d[2]) != 0:
         raise ValueError("encode failed")
     return d
+
+
+def ref_decode_tb(payload, g_total, rm, rv=0, max_iterations=8, early_stop=False):
+    """The reference's decode_transport_block (CpuReference) on given
+    rate-matched bits.  Returns (bits, cb_ok, tb_ok, iterations, posterior)."""
+    payload = _u8(payload)
+    rm = _u8(rm)
+    bits = np.zeros(len(payload), np.uint8)
+    cb = np.zeros(64, np.int32)
+    tb = C.c_int()
+    it = C.c_uint32()
+    post = np.zeros(len(payload) + 24 * 64 + 6144 * 2, np.float32)
+    st = ref().ref_decode_tb(len(payload), g_total, rv, max_iterations, int(early_stop), payload, rm, bits,
+                             cb.ctypes.data, C.byref(tb), C.byref(it), post)
+    if st:
+        raise ValueError(st)
+    return bits, cb, bool(tb.value), int(it.value), post
 
 
 def ref_encode_tb(bits, g_total, rv=0, nl=1, qm=2):

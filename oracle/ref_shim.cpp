@@ -222,4 +222,34 @@ int ref_tb_roundtrip(size_t tbs, size_t g_total, uint32_t rv, uint32_t max_it, i
   });
 }
 
+// Decode a transport block with the CPU backend from given rate-matched bits
+// (bits of all code blocks back to back, the encoder's lengths): encodes
+// `payload` to obtain the descriptors, replaces the rate-matched bits with
+// `rm`, then decodes.  Outputs payload bits, per-block CRC flags, TB CRC flag,
+// iterations used and the posterior LLRs.
+int ref_decode_tb(size_t tbs, size_t g_total, uint32_t rv, uint32_t max_it, int early,
+                  const uint8_t* payload, const uint8_t* rm, uint8_t* out_bits, int* cb_ok, int* tb_ok,
+                  uint32_t* iters, float* post_out) {
+  return guarded([&] {
+    tc::LteTurboCodecParams p;
+    p.transport_block_bits = tbs;
+    p.g_total = g_total;
+    p.rv = rv;
+    p.max_iterations = max_it;
+    p.enable_early_stop = early != 0;
+    auto enc = tc::encode_transport_block(p, std::span<const uint8_t>(payload, tbs));
+    size_t off = 0;
+    for (auto& cb : enc.code_blocks) {
+      std::memcpy(cb.rate_matched_bits.data(), rm + off, cb.rate_matched_bits.size());
+      off += cb.rate_matched_bits.size();
+    }
+    auto dec = tc::decode_transport_block(p, enc, tc::DecoderBackend::CpuReference);
+    std::memcpy(out_bits, dec.tb_bits.data(), dec.tb_bits.size());
+    for (size_t r = 0; r < dec.code_block_crc_ok.size(); ++r) cb_ok[r] = dec.code_block_crc_ok[r];
+    *tb_ok = dec.transport_block_crc_ok;
+    *iters = dec.iterations_used;
+    std::memcpy(post_out, dec.posterior_llr->data(), dec.posterior_llr->size() * sizeof(float));
+  });
+}
+
 }  // extern "C"

@@ -359,6 +359,74 @@ void or_derate_match(const uint8_t* bits, size_t e, size_t d, size_t filler, uin
   free(m2);
 }
 
+/* Soft-input variant of the de-rate matcher: received LLR k (bit 1 > 0)
+ * instead of +-8 per hard bit (lte_turbo.cpp:280-325, the same circular
+ * buffer walk and accumulation order).  accumulate adds this transmission's
+ * contribution to out (HARQ soft combining) instead of overwriting it. */
+void or_derate_match_soft(const float* llr, size_t e, size_t d, size_t filler, uint32_t rv,
+                          float* out, int accumulate) {
+  size_t rows, kpi = ((d + 31) / 32) * 32, ncb = 3 * kpi;
+  int* m01 = (int*)malloc(sizeof(int) * kpi);
+  int* m2 = (int*)malloc(sizeof(int) * kpi);
+  float* circ = (float*)calloc(ncb, sizeof(float));
+  char* seen = (char*)calloc(ncb, 1);
+  subblock_maps(d, filler, m01, m2, &rows);
+  size_t k0 = k0_of(rows, ncb, rv), k = 0, j = 0;
+  while (k < e) {
+    size_t idx = (k0 + j) % ncb;
+    int st;
+    if (circ_entry(idx, kpi, m01, m2, &st) >= 0) {
+      circ[idx] += llr[k];
+      seen[idx] = 1;
+      ++k;
+    }
+    ++j;
+  }
+  if (!accumulate) memset(out, 0, sizeof(float) * 3 * d);
+  for (size_t idx = 0; idx < ncb; ++idx) {
+    int st;
+    int v = circ_entry(idx, kpi, m01, m2, &st);
+    if (v >= 0 && seen[idx]) out[(size_t)st * d + (size_t)v] += circ[idx];
+  }
+  free(seen);
+  free(circ);
+  free(m01);
+  free(m2);
+}
+
+/* Transport-block assembly after decoding (lte_turbo.cpp:958-979): hard
+ * decisions (LLR >= 0 is 1, :488-494) of each code block's posterior, filler
+ * bits dropped, CRC24B checked and dropped when c > 1 (a single block reports
+ * ok), CRC24A over the concatenation, payload = all but the last 24 bits.
+ * post: the blocks' K_r posteriors back to back.  Returns -1 when the blocks
+ * do not add up to b + 24 bits. */
+int or_assemble_tb(const float* post, const uint32_t* k, const uint32_t* filler, size_t c, size_t b,
+                   uint8_t* bits_out, int* cb_ok, int* tb_ok) {
+  size_t total = 0;
+  for (size_t r = 0; r < c; ++r) total += k[r] - filler[r] - (c > 1 ? 24 : 0);
+  if (total != b + 24) return -1;
+  uint8_t* all = (uint8_t*)malloc(total + 1);
+  uint8_t* blk = (uint8_t*)malloc(6144 + 1);
+  size_t pos = 0, off = 0;
+  for (size_t r = 0; r < c; ++r) {
+    size_t n = k[r] - filler[r];
+    for (size_t i = 0; i < n; ++i) blk[i] = post[off + filler[r] + i] >= 0.0f ? 1 : 0;
+    cb_ok[r] = 1;
+    if (c > 1) {
+      cb_ok[r] = or_crc_check(OR_CRC24B, blk, n);
+      n -= 24;
+    }
+    memcpy(all + pos, blk, n);
+    pos += n;
+    off += k[r];
+  }
+  *tb_ok = or_crc_check(OR_CRC24A, all, total);
+  memcpy(bits_out, all, b);
+  free(blk);
+  free(all);
+  return 0;
+}
+
 /* ------------------------------------------------------ Log-MAP decoder ---
  * max* with the reference's infinity rules (lte_turbo.cpp:380-390). */
 static float maxstar(float x, float y) {

@@ -25,6 +25,12 @@
  *                       circular-buffer entry exactly once (permutation).
  *   or_siso / or_decode golden vectors; brute-force enumeration of all
  *                       2^K paths for K<=12 in float64 (exact APP).
+ *   or_derate_match_soft  +-8 inputs equal the reference's de-rate matcher
+ *                       (golden vectors, every rv, E above and below the
+ *                       circular buffer); additivity of HARQ combining.
+ *   or_assemble_tb      equality with the reference's decode_transport_block
+ *                       (golden vectors with corrupted code blocks: bits,
+ *                       per-block and TB CRC flags).
  *   or_decode_window    the decoder this build runs on the GPU in windowed
  *                       mode; pinned by equality with or_decode when there is
  *                       one window per code block, and by BER.
@@ -67,6 +73,15 @@ void or_rate_match(const uint8_t* d0, const uint8_t* d1, const uint8_t* d2,
                    size_t d, size_t filler, size_t e, uint32_t rv, uint8_t* out);
 void or_derate_match(const uint8_t* bits, size_t e, size_t d, size_t filler,
                      uint32_t rv, float* out /* 3*d: d0|d1|d2 */);
+
+/* Soft de-rate matching of E received LLRs (HARQ: accumulate into out). */
+void or_derate_match_soft(const float* llr, size_t e, size_t d, size_t filler, uint32_t rv,
+                          float* out, int accumulate);
+
+/* Transport-block assembly after decoding: hard decisions, filler strip,
+ * CRC24B per block (c > 1), CRC24A per TB, payload of b bits. */
+int or_assemble_tb(const float* post, const uint32_t* k, const uint32_t* filler, size_t c, size_t b,
+                   uint8_t* bits_out, int* cb_ok, int* tb_ok);
 
 /* One Log-MAP SISO half: sys, apriori K; parity K+4 (first K used). */
 void or_siso(const float* sys, const float* parity, const float* apriori, uint32_t k,

@@ -102,6 +102,36 @@ def main():
     data["es_k1024_llr"] = np.stack(llr)
     data["es_k1024_post"] = post
     data["es_k1024_meta"] = np.array([int(ok), used], np.int64)
+    # transport-block decodes (decode_transport_block, CpuReference) from
+    # rate-matched bits with some bits flipped, so that some code blocks and
+    # transport blocks fail their CRC
+    tbd_cases = [(40, None, 0.0, 8, False), (512, 1000, 0.25, 6, False), (12000, None, 0.0, 4, True),
+                 (12000, 30000, 0.2, 8, False), (20000, None, 0.2, 8, True), (40000, 90000, 0.12, 3, False)]
+    tbd_meta = []
+    for ci, (tbs, g, flip, it, es) in enumerate(tbd_cases):
+        payload = rng.integers(0, 2, tbs).astype(np.uint8)
+        seg, _ = oracle.ref_segment(oracle.ref_crc_append(0, payload))
+        ks = [seg["k_minus"]] * seg["c_minus"] + [seg["k_plus"]] * seg["c_plus"]
+        g_total = g if g is not None else sum(3 * (k + 4) for k in ks)
+        cbs = oracle.ref_encode_tb(payload, g_total)
+        rm = []
+        for r, c in enumerate(cbs):
+            p = flip * ((r + ci) % 3) / 2.0  # blocks corrupted to different degrees, some clean
+            noise = (rng.random(c["e"]) < p).astype(np.uint8)
+            rm.append(c["rm"] ^ noise)
+        rm = np.concatenate(rm)
+        bits, cb_ok, tb_ok, used, post = oracle.ref_decode_tb(payload, g_total, rm, 0, it, es)
+        nk = sum(c["k"] for c in cbs)
+        tbd_meta.append([ci, tbs, g_total, it, int(es), len(cbs), int(tb_ok), used])
+        data[f"tbd{ci}_payload"] = payload
+        data[f"tbd{ci}_rm"] = rm
+        data[f"tbd{ci}_bits"] = bits
+        data[f"tbd{ci}_cb_ok"] = cb_ok[:len(cbs)].astype(np.int64)
+        data[f"tbd{ci}_post"] = post[:nk]
+        data[f"tbd{ci}_k"] = np.array([c["k"] for c in cbs], np.int64)
+        data[f"tbd{ci}_f"] = np.array([c["filler"] for c in cbs], np.int64)
+        data[f"tbd{ci}_e"] = np.array([c["e"] for c in cbs], np.int64)
+    data["tbd_meta"] = np.array(tbd_meta, np.int64)
     out = os.path.join(HERE, "reference_vectors.npz")
     np.savez_compressed(out, **data)
     print("wrote", out, os.path.getsize(out), "bytes,", len(data), "arrays")

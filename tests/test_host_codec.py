@@ -164,6 +164,22 @@ def test_gpu_entry_points_fail_loudly_without_device():
             fn(d, z, z, z, 8, False)
     with pytest.raises(RuntimeError):
         T.Plan([6144])
+    with pytest.raises(RuntimeError):
+        T.DeratePlan([6144], [18444])
+    with pytest.raises(RuntimeError):
+        T.derate_match_soft(np.zeros(100, np.float32), 40)
+
+
+def test_soft_derate_argument_errors_before_device():
+    # argument checks come before the device is touched: same errors with or without a GPU
+    with pytest.raises(ValueError):
+        T.derate_match_soft(np.zeros(10, np.float32), 40, 0, 4)  # rv outside 0..3
+    with pytest.raises(IndexError):
+        T.derate_match_soft(np.zeros(10, np.float32), 41, 0, 0)  # K not a QPP size
+    with pytest.raises(ValueError):
+        T.DeratePlan([40], [0])  # E = 0
+    with pytest.raises(ValueError):
+        T.DeratePlan([40], [10], fillers=[40])  # F >= K
 
 
 def test_segment_geometry_all_table_sizes():
@@ -188,3 +204,13 @@ def test_single_hip_runtime_per_process():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=300)
     assert r.returncode == 0, r.stderr
     assert r.stdout.strip().splitlines()[-1] == "1"
+
+
+def test_tb_plan_argument_errors_before_device():
+    # descriptor checks come before the device is touched
+    with pytest.raises(ValueError):
+        T.TbPlan([(0, 100, 0, 1, 2)])
+    with pytest.raises(ValueError):
+        T.TbPlan([(40, 101, 0, 1, 2)])  # g_total not divisible by layers * Qm
+    with pytest.raises(ValueError):
+        T.TbPlan([(40, 100, 4, 1, 2)])

@@ -165,6 +165,124 @@ def test_rate_matching_golden():
         assert np.array_equal(O.derate_match(g[key + "_rm"], int(k) + 4, int(f), int(rv)), g[key + "_derm"]), key
 
 
+def test_soft_derate_of_pm8_is_reference_derate_golden():
+    """+-8 per hard bit through the soft de-rate matcher reproduces the
+    reference's de-rate matcher (lte_turbo.cpp:280-325) bit for bit, for every
+    rv, with puncturing (E < N_valid) and repetition (E > N_valid)."""
+    g = golden()
+    for ci, tbs, g_total, rv, r, k, e, f in g["tb_meta"]:
+        key = f"tb{ci}_rv{rv}_cb{r}"
+        rx = np.where(g[key + "_rm"] > 0, 8.0, -8.0).astype(np.float32)
+        soft = O.derate_match_soft(rx, int(k) + 4, int(f), int(rv))
+        assert np.array_equal(soft.view(np.uint32), g[key + "_derm"].view(np.uint32)), key
+
+
+@pytest.mark.parametrize("k,f,e", [(40, 0, 50), (544, 8, 3000), (1024, 0, 7000)])
+def test_soft_derate_harq_combining_is_additive(k, f, e):
+    """HARQ combining: accumulating rv2 onto rv0 equals the sum of the two
+    separate de-rate matchings (small-integer LLRs, so float sums are exact),
+    and a single LLR routed alone lands on exactly one output position."""
+    rng = np.random.default_rng(k)
+    d = k + 4
+    x0 = rng.integers(-20, 21, e).astype(np.float32)
+    x2 = rng.integers(-20, 21, e).astype(np.float32)
+    a = O.derate_match_soft(x0, d, f, 0)
+    b = O.derate_match_soft(x2, d, f, 2)
+    comb = O.derate_match_soft(x2, d, f, 2, out=a.copy())
+    assert np.array_equal(comb, a + b)
+    # a one-hot received vector reaches the output once per repetition of its entry
+    nv = 3 * d - 2 * f
+    for j in (0, e // 2, e - 1):
+        hot = np.zeros(e, np.float32)
+        hot[j] = 1.0
+        out = O.derate_match_soft(hot, d, f, 1)
+        assert np.count_nonzero(out) == 1 and out.sum() == 1.0
+        assert out[:f].sum() == 0 and out[d:d + f].sum() == 0  # filler positions stay 0
+    assert nv == np.count_nonzero(O.derate_match_soft(np.ones(max(e, nv), np.float32), d, f, 0))
+
+
+def _oracle_tb_receive(g, ci):
+    """The oracle's receive chain for golden TB decode case ci: +-8 soft
+    de-rate matching, exact decode per block, TB assembly."""
+    _, tbs, g_total, it, es, c, _, _ = g["tbd_meta"][ci]
+    ks, fs, es_ = g[f"tbd{ci}_k"], g[f"tbd{ci}_f"], g[f"tbd{ci}_e"]
+    rm = g[f"tbd{ci}_rm"]
+    posts, iters, off = [], [], 0
+    for k, f, e in zip(ks, fs, es_):
+        rx = np.where(rm[off:off + e] > 0, 8.0, -8.0).astype(np.float32)
+        off += e
+        llr = O.derate_match_soft(rx, int(k) + 4, int(f), 0)
+        d = int(k) + 4
+        p, ok, used = O.decode_cb(llr[:d], llr[d:2 * d], llr[2 * d:], int(k), int(f), int(it), bool(es))
+        posts.append(p)
+        iters.append(used)
+    post = np.concatenate(posts)
+    bits, cb_ok, tb_ok = O.assemble_tb(post, ks, fs, int(tbs))
+    return post, bits, cb_ok, tb_ok, max(iters)
+
+
+@pytest.mark.parametrize("ci", range(6))
+def test_tb_receive_chain_matches_reference_golden(ci):
+    """Soft de-rate matching + decode + TB assembly of the oracle equal the
+    reference's decode_transport_block (lte_turbo.cpp:891-980) on corrupted
+    transport blocks: payload bits, per-block and TB CRC flags, iterations
+    used and the posterior LLRs bit for bit."""
+    g = golden()
+    post, bits, cb_ok, tb_ok, used = _oracle_tb_receive(g, ci)
+    meta = g["tbd_meta"][ci]
+    assert np.array_equal(post.view(np.uint32), g[f"tbd{ci}_post"].view(np.uint32))
+    assert np.array_equal(bits, g[f"tbd{ci}_bits"])
+    assert np.array_equal(cb_ok, g[f"tbd{ci}_cb_ok"].astype(bool))
+    assert tb_ok == bool(meta[6]) and used == meta[7]
+
+
+def test_non_converged_block_is_ill_conditioned():
+    """Why the GPU tests compare posteriors only on blocks that converge: in
+    the reference decoder (the oracle is bit-exact with it) a 1-ulp change of
+    one input LLR of the non-converged block of golden TB case 4 moves its
+    posteriors by more than 1e-4, while converged blocks do not move."""
+    g = golden()
+    ci = 4
+    ks, fs, es = g[f"tbd{ci}_k"], g[f"tbd{ci}_f"], g[f"tbd{ci}_e"]
+    rm = g[f"tbd{ci}_rm"]
+    off = 0
+    moved = []
+    for k, f, e in zip(ks, fs, es):
+        rx = np.where(rm[off:off + e] > 0, 8.0, -8.0).astype(np.float32)
+        off += e
+        llr = O.derate_match_soft(rx, int(k) + 4, int(f), 0)
+        d = int(k) + 4
+        a, ok, _ = O.decode_cb(llr[:d], llr[d:2 * d], llr[2 * d:], int(k), int(f), 8, True)
+        llr[5] = np.nextafter(llr[5], np.float32(100))
+        b, _, _ = O.decode_cb(llr[:d], llr[d:2 * d], llr[2 * d:], int(k), int(f), 8, True)
+        moved.append((ok, float(np.max(np.abs(a - b)))))
+    assert any(not ok and m > 1e-4 for ok, m in moved)
+    assert all(m < 1e-4 for ok, m in moved if ok)
+
+
+def test_assemble_tb_detects_single_bit_errors():
+    """One flipped hard decision in a segmented TB fails exactly its block's
+    CRC24B, and the TB CRC24A unless the bit is part of the block's own CRC24B
+    (dropped before the TB check); filler LLRs are ignored."""
+    rng = np.random.default_rng(5)
+    payload = rng.integers(0, 2, 12000).astype(np.uint8)
+    tb = O.crc_append(0, payload)
+    seg, blocks = O.segment(tb)
+    ks = [seg["k_minus"]] * seg["c_minus"] + [seg["k_plus"]] * seg["c_plus"]
+    fs = [seg["filler_bits"]] + [0] * (len(ks) - 1)
+    post = (2.0 * np.concatenate(blocks).astype(np.float32) - 1.0) * 3.0
+    post[:fs[0]] = -post[:fs[0]]  # filler decisions do not matter
+    bits, cb_ok, tb_ok = O.assemble_tb(post, ks, fs, len(payload))
+    assert np.array_equal(bits, payload) and cb_ok.all() and tb_ok
+    for pos in (fs[0], ks[0] - 1, ks[0] + 17, sum(ks) - 1):
+        bad = post.copy()
+        bad[pos] = -bad[pos]
+        bits, cb_ok, tb_ok = O.assemble_tb(bad, ks, fs, len(payload))
+        blk = 0 if pos < ks[0] else 1
+        in_cb_crc = (pos - (0 if blk == 0 else ks[0])) >= ks[blk] - 24
+        assert tb_ok == in_cb_crc and list(cb_ok) == [i != blk for i in range(len(ks))]
+
+
 @pytest.mark.parametrize("k,f", [(40, 0), (544, 0), (6144, 0), (6144, 8)])
 def test_full_rate_visits_every_bit_once(k, f):
     """At E = number of non-NULL circular-buffer entries and rv=0 each coded
@@ -254,3 +372,56 @@ def test_window_decoder_rejects_bad_geometry():
         O.decode_window(z, z, z, 6144, 100)  # does not divide K
     with pytest.raises(ValueError):
         O.decode_window(z, z, z, 6144, 96, 96)  # warm-up must stay below S
+
+
+# ------------------------------------------- differentiable decoder (FP64) --
+def _sw_inputs(k, n, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    llr = []
+    for _ in range(n):
+        cb = rng.integers(0, 2, k).astype(np.uint8)
+        d = O.encode_cb(cb, 0)
+        llr.append(np.concatenate([(2.0 * (2.0 * x - 1.0) + rng.normal(0, 1.3, k + 4)) for x in d]))
+    return torch.tensor(np.stack(llr), dtype=torch.float64)
+
+
+def test_sw_reference_single_segment_is_the_reference_decoder():
+    """One segment and unit scales: the FP64 sliding-window decoder is the
+    reference decoder (or_decode_cb, float32): within 1e-4 relative."""
+    import torch
+    from oracle import sw_reverse as R
+    k = 64
+    x = _sw_inputs(k, 2, 1)
+    post = R.decode(x, torch.ones(12, dtype=torch.float64), k, np.array(O.qpp_perm(k), np.int64), k, 0, 6)
+    for i in range(2):
+        ref, _, _ = O.decode_cb(*np.split(x[i].numpy().astype(np.float32), 3), k, 0, 6)
+        assert np.max(np.abs(post[i].numpy() - ref) / (1 + np.abs(ref))) < 1e-4
+
+
+def test_sw_reference_full_warmup_equals_single_segment():
+    """Segments whose warm-ups reach both block ends compute exact metrics:
+    5 segments of 8 with warm-up 40 equal one segment of 40 (FP64, 1e-9)."""
+    import torch
+    from oracle import sw_reverse as R
+    k = 40
+    x = _sw_inputs(k, 2, 2)
+    w = torch.linspace(0.7, 1.2, 12, dtype=torch.float64)
+    perm = np.array(O.qpp_perm(k), np.int64)
+    a = R.decode(x, w, k, perm, 40, 0, 6)
+    b = R.decode(x, w, k, perm, 8, 40, 6)
+    c = R.decode(x, w, k, perm, 8, 4, 6)
+    assert float((a - b).abs().max()) < 1e-9
+    assert float((a - c).abs().max()) > 1e-6  # short warm-ups do change the metrics
+
+
+def test_sw_reference_gradcheck():
+    """Central finite differences of the FP64 reference (K=40, 5 segments)."""
+    import torch
+    from oracle import sw_reverse as R
+    k = 40
+    x = (_sw_inputs(k, 1, 3) * 0.3).requires_grad_(True)
+    w = torch.linspace(0.7, 1.2, 4, dtype=torch.float64).requires_grad_(True)
+    perm = np.array(O.qpp_perm(k), np.int64)
+    assert torch.autograd.gradcheck(lambda a, b: R.decode(a, b, k, perm, 8, 8, 2), (x, w), eps=1e-6, atol=1e-6)
+

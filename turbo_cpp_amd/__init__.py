@@ -38,6 +38,11 @@ class TcOptions(C.Structure):
                 ("min_segment", C.c_uint32), ("warmup", C.c_uint32)]
 
 
+class TcTbDesc(C.Structure):
+    _fields_ = [("transport_block_bits", C.c_uint64), ("g_total", C.c_uint64), ("rv", C.c_uint32),
+                ("num_layers", C.c_uint32), ("mod_order", C.c_uint32)]
+
+
 class TcSegmentInfo(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("k", "segment_length", "segments", "blocks_per_wave", "warmup")]
 
@@ -94,6 +99,26 @@ def lib() -> C.CDLL:
         L.tc_derate_match.argtypes = [_u8p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, _f32p]
         L.tc_decode_code_block_cpu.argtypes = [C.POINTER(TcCbDesc), _f32p, _f32p, _f32p, C.c_uint32, C.c_int32,
                                                _f32p, C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]
+        L.tc_tb_plan_create.argtypes = [C.POINTER(TcTbDesc), C.c_size_t, C.POINTER(TcOptions),
+                                        C.POINTER(C.c_void_p)]
+        L.tc_tb_plan_destroy.argtypes = [C.c_void_p]
+        L.tc_tb_plan_destroy.restype = None
+        L.tc_tb_plan_sizes.argtypes = [C.c_void_p, _u64p]
+        L.tc_tb_plan_code_blocks.argtypes = [C.c_void_p, _u32p, _u32p, _u64p]
+        L.tc_tb_plan_decode.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32] + [C.c_void_p] * 6
+        L.tc_decode_transport_blocks_host.argtypes = [C.POINTER(TcTbDesc), C.c_size_t, C.POINTER(TcOptions), _f32p,
+                                                      _u8p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.tc_derate_plan_create.argtypes = [_u32p, C.c_void_p, _u64p, C.c_void_p, C.c_size_t,
+                                            C.POINTER(C.c_void_p)]
+        L.tc_derate_plan_destroy.argtypes = [C.c_void_p]
+        L.tc_derate_plan_destroy.restype = None
+        L.tc_derate_plan_rx_floats.argtypes = [C.c_void_p]
+        L.tc_derate_plan_rx_floats.restype = C.c_size_t
+        L.tc_derate_plan_llr_floats.argtypes = [C.c_void_p]
+        L.tc_derate_plan_llr_floats.restype = C.c_size_t
+        L.tc_derate_plan_run.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
+        L.tc_derate_match_soft.argtypes = [_f32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32,
+                                           _f32p]
         _lib = L
     return _lib
 
@@ -449,10 +474,177 @@ class Plan:
             pass
 
 
+def derate_match_soft(rx, k: int, filler: int = 0, rv: int = 0, out=None) -> np.ndarray:
+    """Soft de-rate matching of one code block on the GPU (tc_derate_match_soft).
+
+    rx: the E received LLRs (bit 1 > 0).  Returns [d0|d1|d2] (3(K+4) floats).
+    With ``out`` given, the result is added into it (HARQ combining)."""
+    rx = _f32(rx).reshape(-1)
+    acc = out is not None
+    if out is None:
+        out = np.zeros(3 * (k + 4), np.float32)
+    elif out.dtype != np.float32 or not out.flags.c_contiguous or out.size != 3 * (k + 4):
+        raise ValueError("out must be a contiguous float32 array of 3(K+4)")
+    _check(lib().tc_derate_match_soft(rx, rx.size, k, filler, rv, int(acc), out))
+    return out
+
+
+class DeratePlan:
+    """Batched soft de-rate matching on the GPU (tc_derate_plan_*).
+
+    ``run`` takes device pointers: ``rx`` holds the E_r received LLRs of every
+    block back to back, ``llr`` receives the decoder input layout that
+    :class:`Plan` reads.  ``rv=-1`` uses the per-block rv given here."""
+
+    def __init__(self, ks: Sequence[int], es: Sequence[int], fillers=None, rvs=None):
+        self.ks = np.ascontiguousarray(ks, dtype=np.uint32)
+        self.es = np.ascontiguousarray(es, dtype=np.uint64)
+        if self.es.size != self.ks.size:
+            raise ValueError("one E per code block")
+        self._fil = None if fillers is None else np.ascontiguousarray(fillers, dtype=np.uint32)
+        self._rv = None if rvs is None else np.ascontiguousarray(rvs, dtype=np.uint32)
+        h = C.c_void_p()
+        _check(lib().tc_derate_plan_create(self.ks, None if self._fil is None else self._fil.ctypes.data, self.es,
+                                           None if self._rv is None else self._rv.ctypes.data, len(self.ks),
+                                           C.byref(h)))
+        self._h = h
+        self.rx_floats = lib().tc_derate_plan_rx_floats(h)
+        self.llr_floats = lib().tc_derate_plan_llr_floats(h)
+
+    def run(self, rx_ptr: int, llr_ptr: int, rv: int = -1, accumulate: bool = False, stream: int = 0):
+        _check(lib().tc_derate_plan_run(self._h, rx_ptr or None, rv, int(accumulate), llr_ptr or None,
+                                        stream or None))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tc_derate_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _tb_descs(tbs):
+    arr = (TcTbDesc * max(len(tbs), 1))()
+    for i, t in enumerate(tbs):
+        if isinstance(t, LteTurboCodecParams):
+            t = (t.transport_block_bits, t.g_total, t.rv, t.num_layers, t.mod_order)
+        arr[i] = TcTbDesc(*[int(x) for x in t])
+    return arr
+
+
+class TbPlan:
+    """Batched transport-block decoding on the GPU (tc_tb_plan_*).
+
+    ``tbs``: LteTurboCodecParams or tuples (B, g_total, rv, num_layers,
+    mod_order).  ``decode`` takes device pointers; outputs may be 0."""
+
+    def __init__(self, tbs, decoder: int = DECODER_WINDOW_LOG, max_iterations: int = 8,
+                 early_stop: bool = False, min_segment: int = 0, warmup: int = 0):
+        self.n_tb = len(tbs)
+        self._descs = _tb_descs(tbs)
+        o = options(decoder, max_iterations, early_stop, min_segment, warmup)
+        h = C.c_void_p()
+        _check(lib().tc_tb_plan_create(self._descs, self.n_tb, C.byref(o), C.byref(h)))
+        self._h = h
+        sz = np.zeros(4, np.uint64)
+        _check(lib().tc_tb_plan_sizes(h, sz))
+        self.n_cb, self.rx_floats, self.bits_total, self.post_floats = (int(x) for x in sz)
+        self.ks = np.zeros(max(self.n_cb, 1), np.uint32)
+        self.fillers = np.zeros(max(self.n_cb, 1), np.uint32)
+        self.es = np.zeros(max(self.n_cb, 1), np.uint64)
+        _check(lib().tc_tb_plan_code_blocks(h, self.ks, self.fillers, self.es))
+        self.ks, self.fillers, self.es = self.ks[:self.n_cb], self.fillers[:self.n_cb], self.es[:self.n_cb]
+
+    def decode(self, rx_ptr: int, bits_ptr: int = 0, tb_crc_ptr: int = 0, cb_crc_ptr: int = 0,
+               iters_ptr: int = 0, post_ptr: int = 0, rv: int = -1, accumulate: bool = False, stream: int = 0):
+        _check(lib().tc_tb_plan_decode(self._h, rx_ptr or None, rv, int(accumulate), bits_ptr or None,
+                                       tb_crc_ptr or None, cb_crc_ptr or None, iters_ptr or None,
+                                       post_ptr or None, stream or None))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tc_tb_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def decode_transport_blocks(tbs, rx, decoder: int = DECODER_WINDOW_LOG, max_iterations: int = 8,
+                            early_stop: bool = False):
+    """Decode many transport blocks from host LLRs on the GPU.
+
+    Returns (payload bits per TB (list), tb_crc_ok bool[n_tb], cb_crc_ok
+    bool[n_cb], iterations uint32[n_tb], posterior float32[sum K])."""
+    descs = _tb_descs(tbs)
+    n = len(tbs)
+    rx = _f32(rx).reshape(-1)
+    b = [int(descs[i].transport_block_bits) for i in range(n)]
+    bits = np.zeros(max(sum(b), 1), np.uint8)
+    tb = np.zeros(max(n, 1), np.int32)
+    o = options(decoder, max_iterations, early_stop)
+    # sizes come from a plan-free query: segmentation on the host
+    ks, ncb = [], 0
+    for i in range(n):
+        seg = _segment_sizes(b[i] + 24)
+        ks += [seg[4]] * seg[2] + [seg[3]] * seg[1]
+    cb = np.zeros(max(len(ks), 1), np.int32)
+    its = np.zeros(max(n, 1), np.uint32)
+    post = np.zeros(max(sum(ks), 1), np.float32)
+    _check(lib().tc_decode_transport_blocks_host(descs, n, C.byref(o), rx, bits, tb.ctypes.data, cb.ctypes.data,
+                                                 its.ctypes.data, post.ctypes.data))
+    out, off = [], 0
+    for x in b:
+        out.append(bits[off:off + x].copy())
+        off += x
+    return out, tb[:n].astype(bool), cb[:len(ks)].astype(bool), its[:n], post[:sum(ks)]
+
+
+def _standard_geometry(params: LteTurboCodecParams, encoded: EncodedTransportBlock) -> bool:
+    """True when the blocks are those encode_transport_block makes for params."""
+    try:
+        if params.transport_block_bits == 0:
+            return False
+        c, cp, cm, kp, km, f, _ = _segment_sizes(params.transport_block_bits + 24)
+        if c != len(encoded.code_blocks):
+            return False
+        for r, b in enumerate(encoded.code_blocks):
+            d = b.descriptor
+            k = km if r < cm else kp
+            e = compute_e(r, params.g_total, c, params.num_layers, params.mod_order)
+            if (d.k_r, d.d_r, d.filler_count, d.e_r, len(b.rate_matched_bits)) != (k, k + 4, f if r == 0 else 0, e, e):
+                return False
+        return True
+    except (ValueError, IndexError, RuntimeError):
+        return False
+
+
+def _segment_sizes(b: int):
+    """(c, c_plus, c_minus, k_plus, k_minus, filler, l) of a b-bit TB (CRC included)."""
+    seg = np.zeros(7, np.uint64)
+    _check(lib().tc_segment(np.zeros(b, np.uint8), b, seg, None))
+    return [int(x) for x in seg]
+
+
 def decode_transport_block(params: LteTurboCodecParams, encoded: EncodedTransportBlock,
                            backend: DecoderBackend) -> DecodedTransportBlock:
-    """lte_turbo.cpp:891-980 (GPU backends batch all code blocks in one launch)."""
+    """lte_turbo.cpp:891-980.  GPU backends batch all code blocks in one launch;
+    for blocks laid out as encode_transport_block makes them, de-rate
+    matching and TB assembly run on the device too."""
     blocks = encoded.code_blocks
+    if backend != DecoderBackend.CpuReference and _standard_geometry(params, encoded):
+        rx = np.concatenate([np.where(_u8(b.rate_matched_bits) > 0, 8.0, -8.0).astype(np.float32) for b in blocks])
+        dec = DECODER_EXACT if backend == DecoderBackend.GpuStage1Exact else DECODER_WINDOW_LOG
+        bits, tb_ok, cb_ok, its, post = decode_transport_blocks([params], rx, dec, params.max_iterations,
+                                                                params.enable_early_stop)
+        return DecodedTransportBlock(bits[0], [bool(x) for x in cb_ok], bool(tb_ok[0]), int(its[0]), post)
     llrs = [derate_match_code_block(params, b.descriptor, b.rate_matched_bits) for b in blocks]
     posts, oks, iters = [], [], []
     if backend == DecoderBackend.CpuReference:

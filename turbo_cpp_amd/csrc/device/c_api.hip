@@ -15,6 +15,7 @@
 
 #include "../host/codec_internal.h"
 #include "decoder_types.h"
+#include "hip_util.h"
 #include "turbo_cpp/c_api.h"
 
 namespace tcdev {
@@ -28,40 +29,7 @@ __global__ void window_decode_lin_es_kernel(WinArgs);
 
 namespace {
 
-using turbo_cpp::detail::set_last_error;
-
-struct Failure {
-  tc_status status;
-  std::string what;
-};
-
-void hip_check(hipError_t e, const char* where) {
-  if (e != hipSuccess) throw Failure{TC_ERR_RUNTIME, std::string(where) + ": " + hipGetErrorString(e)};
-}
-
-template <typename F>
-tc_status guarded(F&& f) {
-  try {
-    f();
-    return TC_OK;
-  } catch (const Failure& e) {
-    set_last_error(e.what);
-    return e.status;
-  } catch (const std::bad_alloc&) {
-    set_last_error("out of host memory");
-    return TC_ERR_RUNTIME;
-  } catch (const std::exception& e) {
-    set_last_error(e.what());
-    return TC_ERR_RUNTIME;
-  }
-}
-
-void require_device() {
-  int n = 0;
-  const hipError_t e = hipGetDeviceCount(&n);
-  if (e != hipSuccess || n == 0)
-    throw Failure{TC_ERR_RUNTIME, "no HIP device available (the GPU backends need an MI355X / gfx950)"};
-}
+using namespace tchost;
 
 tc_options normalised(const tc_options* o) {
   tc_options r{};
@@ -98,19 +66,6 @@ WinKernel window_kernel(const tc_options& o) {
   if (o.decoder == TC_DECODER_WINDOW_LOG)
     return o.early_stop ? tcdev::window_decode_log_es_kernel : tcdev::window_decode_log_kernel;
   return o.early_stop ? tcdev::window_decode_lin_es_kernel : tcdev::window_decode_lin_kernel;
-}
-
-template <typename T>
-T* dev_alloc(size_t n) {
-  void* p = nullptr;
-  if (n == 0) n = 1;
-  hip_check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
-  return static_cast<T*>(p);
-}
-
-template <typename T>
-void dev_upload(T* dst, const std::vector<T>& src) {
-  if (!src.empty()) hip_check(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
 }
 
 }  // namespace

@@ -1,0 +1,42 @@
+// Plain structures of the differentiable decoder (sliding-window Log-MAP with
+// per-half-iteration extrinsic scales, forward and adjoint kernels).
+#pragma once
+
+#include <cstdint>
+
+namespace tcdev {
+
+// One wave of work: nblk code blocks of the same K, each split into M
+// segments of S steps (lane = block * M + segment); warm-up g <= S steps.
+struct DiffJob {
+  uint32_t k, seg, nseg, nblk, warm, f1, f2, cb_first;
+};
+
+// A code block: offsets (in elements) of its channel LLRs [d0|d1|d2]
+// (3(K+4)), its posteriors (K), its saved extrinsics (2I x K) and its two
+// extrinsic-adjoint buffers (2 x K).
+struct DiffCb {
+  uint64_t llr_off, post_off, save_off, ebar_off;
+};
+
+template <typename R>
+struct DiffArgs {
+  const DiffJob* jobs;
+  uint32_t njobs;
+  uint32_t halves;  // 2I
+  const DiffCb* cbs;
+  const R* llr;
+  const R* scales;  // [2I]
+  R* post;
+  R* saved;
+  // backward only
+  const R* grad_post;
+  R* grad_llr;
+  R* ebar;
+  R* dscale_part;  // [njobs][2I]
+  R* scratch;      // per lane, slot_elems each
+  uint32_t slot_elems;
+  uint32_t max_seg, max_warm;
+};
+
+}  // namespace tcdev

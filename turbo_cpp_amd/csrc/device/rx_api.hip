@@ -1,0 +1,322 @@
+// C ABI of the receive chain on the GPU: soft de-rate matching with HARQ
+// combining, and batched transport-block decoding (de-rate matching, turbo
+// decoding and TB assembly chained on one stream).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <stdexcept>
+#include <memory>
+#include <utility>
+#include <vector>
+
+#include "../host/codec_internal.h"
+#include "hip_util.h"
+#include "rx_types.h"
+#include "turbo_cpp/c_api.h"
+
+namespace tcdev {
+__global__ void derate_soft_kernel(const DerateCb*, const uint16_t*, const float*, float*, int32_t, int32_t);
+__global__ void tb_assemble_kernel(const TbInfo*, const TbCb*, const float*, const uint32_t*, uint32_t,
+                                   const uint32_t*, const uint32_t*, uint8_t*, int32_t*, int32_t*, uint32_t*);
+}  // namespace tcdev
+
+using namespace tchost;
+
+struct tc_derate_plan {
+  size_t n_cb = 0;
+  size_t rx_floats = 0, llr_floats = 0;
+  uint32_t max_valid = 0;
+  tcdev::DerateCb* d_cbs = nullptr;
+  uint16_t* d_rank = nullptr;
+  ~tc_derate_plan() {
+    for (void* p : {(void*)d_cbs, (void*)d_rank})
+      if (p) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+void build_derate_plan(tc_derate_plan& p, const uint32_t* k, const uint32_t* filler, const uint64_t* e,
+                       const uint32_t* rv, size_t n) {
+  if (n && (!k || !e)) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+  std::map<std::pair<uint32_t, uint32_t>, std::pair<uint32_t, turbo_cpp::detail::DerateGeometry>> tables;
+  std::vector<uint16_t> rank;
+  std::vector<tcdev::DerateCb> cbs(n);
+  uint64_t in = 0, out = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const uint32_t f = filler ? filler[i] : 0;
+    if (turbo_cpp::detail::qpp_index(k[i]) < 0) throw Failure{TC_ERR_OUT_OF_RANGE, "unsupported QPP block size"};
+    if (f >= k[i]) throw Failure{TC_ERR_INVALID_ARGUMENT, "filler count must be below K"};
+    if (e[i] == 0 || e[i] > 0xFFFFFFFFull) throw Failure{TC_ERR_INVALID_ARGUMENT, "E must be in [1, 2^32)"};
+    if (rv && rv[i] > 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "rv must be 0..3"};
+    const uint32_t d = k[i] + 4;
+    auto it = tables.find({d, f});
+    if (it == tables.end()) {
+      auto g = turbo_cpp::detail::derate_geometry(d, f);
+      const uint32_t off = static_cast<uint32_t>(rank.size());
+      rank.insert(rank.end(), g.rank.begin(), g.rank.end());
+      it = tables.emplace(std::make_pair(d, f), std::make_pair(off, std::move(g))).first;
+    }
+    const auto& g = it->second.second;
+    tcdev::DerateCb& c = cbs[i];
+    c.in_off = in;
+    c.out_off = out;
+    c.e = static_cast<uint32_t>(e[i]);
+    c.d = d;
+    c.n_valid = g.n_valid;
+    c.rank_off = it->second.first;
+    std::copy(g.start, g.start + 4, c.start);
+    c.rv = rv ? rv[i] : 0;
+    c.pad = 0;
+    p.max_valid = std::max(p.max_valid, g.n_valid);
+    in += e[i];
+    out += 3ull * d;
+  }
+  p.n_cb = n;
+  p.rx_floats = in;
+  p.llr_floats = out;
+  if (n == 0) return;
+  require_device();
+  p.d_cbs = dev_alloc<tcdev::DerateCb>(n);
+  dev_upload(p.d_cbs, cbs);
+  p.d_rank = dev_alloc<uint16_t>(rank.size());
+  dev_upload(p.d_rank, rank);
+}
+
+void run_derate(const tc_derate_plan& p, const float* rx, int32_t rv, int32_t accumulate, float* llr, hipStream_t s) {
+  if (rv < -1 || rv > 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "rv must be -1 (per block) or 0..3"};
+  if (p.n_cb == 0) return;
+  if (!rx || !llr) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+  if (p.n_cb > 0x7FFFFFFFull) throw Failure{TC_ERR_INVALID_ARGUMENT, "too many code blocks"};
+  const size_t lds = static_cast<size_t>(p.max_valid) * sizeof(float);
+  hipLaunchKernelGGL(tcdev::derate_soft_kernel, dim3(static_cast<uint32_t>(p.n_cb)), dim3(tcdev::kDerateThreads), lds,
+                     s, p.d_cbs, p.d_rank, rx, llr, rv, accumulate ? 1 : 0);
+  hip_check(hipGetLastError(), "de-rate matching launch");
+}
+
+}  // namespace
+
+struct tc_tb_plan {
+  size_t n_tb = 0, n_cb = 0;
+  uint64_t bits_total = 0;
+  uint32_t max_iterations = 0;
+  std::vector<uint32_t> ks, fs;
+  std::vector<uint64_t> es;
+  tc_derate_plan* derate = nullptr;
+  tc_plan* decoder = nullptr;
+  tcdev::TbInfo* d_tbs = nullptr;
+  tcdev::TbCb* d_cbs = nullptr;
+  uint32_t* d_pow24a = nullptr;
+  uint32_t* d_pow24b = nullptr;
+  float* d_soft = nullptr;  // decoder input; the HARQ soft buffer between calls
+  float* d_post = nullptr;  // posteriors when the caller passes none
+  uint32_t* d_cb_iters = nullptr;
+  ~tc_tb_plan() {
+    tc_derate_plan_destroy(derate);
+    tc_plan_destroy(decoder);
+    for (void* p : {(void*)d_tbs, (void*)d_cbs, (void*)d_pow24a, (void*)d_pow24b, (void*)d_soft, (void*)d_post,
+                    (void*)d_cb_iters})
+      if (p) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+// D^m mod G(D) for m < n, G = D^24 + poly (24 low coefficients).
+std::vector<uint32_t> crc_pow_table(uint32_t poly, size_t n) {
+  std::vector<uint32_t> t(n);
+  uint32_t r = 1;
+  for (size_t m = 0; m < n; ++m) {
+    t[m] = r;
+    r = ((r << 1) & 0xFFFFFFu) ^ (((r >> 23) & 1u) ? poly : 0u);
+  }
+  return t;
+}
+
+void check_status(tc_status s) {
+  if (s != TC_OK) throw Failure{s, turbo_cpp::detail::last_error()};
+}
+
+void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_options* opts) {
+  if (n_tb && !tbs) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+  std::vector<tcdev::TbInfo> info(n_tb);
+  std::vector<tcdev::TbCb> cbs;
+  std::vector<uint32_t> rvs;
+  uint64_t bits = 0, post = 0, max_len = 24;
+  for (size_t t = 0; t < n_tb; ++t) {
+    const tc_tb_desc& d = tbs[t];
+    if (d.transport_block_bits == 0) throw Failure{TC_ERR_INVALID_ARGUMENT, "transport block cannot be empty"};
+    if (d.rv > 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "rv must be 0..3"};
+    if (d.transport_block_bits > 0xFFFFFFFFull - 24) throw Failure{TC_ERR_INVALID_ARGUMENT, "transport block too long"};
+    turbo_cpp::SegmentationResult seg;
+    std::vector<uint64_t> e(0);
+    try {
+      seg = turbo_cpp::detail::segmentation_sizes(d.transport_block_bits + 24);
+      for (size_t r = 0; r < seg.c; ++r)
+        e.push_back(turbo_cpp::detail::compute_e(r, d.g_total, seg.c, d.num_layers, d.mod_order));
+    } catch (const std::invalid_argument& ex) {
+      throw Failure{TC_ERR_INVALID_ARGUMENT, ex.what()};
+    } catch (const std::runtime_error& ex) {
+      throw Failure{TC_ERR_INVALID_ARGUMENT, ex.what()};
+    }
+    info[t] = {bits, static_cast<uint32_t>(d.transport_block_bits), static_cast<uint32_t>(p.ks.size()),
+               static_cast<uint32_t>(seg.c), 0};
+    for (size_t r = 0; r < seg.c; ++r) {
+      const uint32_t k = static_cast<uint32_t>(r < seg.c_minus ? seg.k_minus : seg.k_plus);
+      const uint32_t f = static_cast<uint32_t>(r == 0 ? seg.filler_bits : 0);
+      if (e[r] == 0) throw Failure{TC_ERR_INVALID_ARGUMENT, "g_total leaves a code block without coded bits"};
+      p.ks.push_back(k);
+      p.fs.push_back(f);
+      p.es.push_back(e[r]);
+      rvs.push_back(d.rv);
+      cbs.push_back({post, k, f});
+      post += k;
+    }
+    bits += d.transport_block_bits;
+    max_len = std::max<uint64_t>(max_len, d.transport_block_bits + 24);
+  }
+  p.n_tb = n_tb;
+  p.n_cb = p.ks.size();
+  p.bits_total = bits;
+  const tc_options o = opts ? *opts : tc_options{8, 0, TC_DECODER_WINDOW_LOG, 0, 0};
+  p.max_iterations = o.max_iterations;
+  check_status(tc_derate_plan_create(p.ks.data(), p.fs.data(), p.es.data(), rvs.data(), p.n_cb, &p.derate));
+  check_status(tc_plan_create(p.ks.data(), p.fs.data(), p.n_cb, &o, &p.decoder));
+  if (n_tb == 0) return;
+  p.d_tbs = dev_alloc<tcdev::TbInfo>(n_tb);
+  dev_upload(p.d_tbs, info);
+  p.d_cbs = dev_alloc<tcdev::TbCb>(cbs.size());
+  dev_upload(p.d_cbs, cbs);
+  const auto pa = crc_pow_table(tcdev::kCrc24APoly, max_len);
+  const auto pb = crc_pow_table(tcdev::kCrc24BPoly, 6144);
+  p.d_pow24a = dev_alloc<uint32_t>(pa.size());
+  dev_upload(p.d_pow24a, pa);
+  p.d_pow24b = dev_alloc<uint32_t>(pb.size());
+  dev_upload(p.d_pow24b, pb);
+  p.d_soft = dev_alloc<float>(tc_derate_plan_llr_floats(p.derate));
+  p.d_post = dev_alloc<float>(post);
+  p.d_cb_iters = dev_alloc<uint32_t>(p.n_cb);
+}
+
+void run_tb(tc_tb_plan& p, const float* rx, int32_t rv, int32_t accumulate, uint8_t* bits, int32_t* tb_ok,
+            int32_t* cb_ok, uint32_t* iters, float* post, hipStream_t s) {
+  if (p.n_tb == 0) return;
+  if (!rx) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+  float* post_buf = post ? post : p.d_post;
+  run_derate(*p.derate, rx, rv, accumulate, p.d_soft, s);
+  check_status(tc_plan_decode(p.decoder, p.d_soft, post_buf, nullptr, p.d_cb_iters, s));
+  if (p.n_tb > 0x7FFFFFFFull) throw Failure{TC_ERR_INVALID_ARGUMENT, "too many transport blocks"};
+  hipLaunchKernelGGL(tcdev::tb_assemble_kernel, dim3(static_cast<uint32_t>(p.n_tb)), dim3(tcdev::kTbThreads), 0, s,
+                     p.d_tbs, p.d_cbs, post_buf, p.d_cb_iters, p.max_iterations, p.d_pow24a, p.d_pow24b, bits, tb_ok,
+                     cb_ok, iters);
+  hip_check(hipGetLastError(), "transport block assembly launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+tc_status tc_derate_plan_create(const uint32_t* k, const uint32_t* filler, const uint64_t* e, const uint32_t* rv,
+                                size_t n_cb, tc_derate_plan** plan) {
+  return guarded([&] {
+    if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    auto p = std::make_unique<tc_derate_plan>();
+    build_derate_plan(*p, k, filler, e, rv, n_cb);
+    *plan = p.release();
+  });
+}
+
+void tc_derate_plan_destroy(tc_derate_plan* plan) { delete plan; }
+size_t tc_derate_plan_rx_floats(const tc_derate_plan* plan) { return plan ? plan->rx_floats : 0; }
+size_t tc_derate_plan_llr_floats(const tc_derate_plan* plan) { return plan ? plan->llr_floats : 0; }
+
+tc_status tc_derate_plan_run(tc_derate_plan* plan, const float* rx_dev, int32_t rv, int32_t accumulate,
+                             float* llr_dev, void* hip_stream) {
+  return guarded([&] {
+    if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    run_derate(*plan, rx_dev, rv, accumulate, llr_dev, static_cast<hipStream_t>(hip_stream));
+  });
+}
+
+tc_status tc_derate_match_soft(const float* rx, uint64_t e, uint32_t k, uint32_t filler, uint32_t rv,
+                               int32_t accumulate, float* out) {
+  return guarded([&] {
+    if (!rx || !out) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    tc_derate_plan p;
+    build_derate_plan(p, &k, &filler, &e, &rv, 1);
+    DevBuf<float> d_rx(e), d_out(p.llr_floats);
+    hip_check(hipMemcpy(d_rx.p, rx, e * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    if (accumulate)
+      hip_check(hipMemcpy(d_out.p, out, p.llr_floats * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    run_derate(p, d_rx.p, -1, accumulate, d_out.p, nullptr);
+    hip_check(hipDeviceSynchronize(), "de-rate matching");
+    hip_check(hipMemcpy(out, d_out.p, p.llr_floats * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  });
+}
+
+tc_status tc_tb_plan_create(const tc_tb_desc* tbs, size_t n_tb, const tc_options* opts, tc_tb_plan** plan) {
+  return guarded([&] {
+    if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    auto p = std::make_unique<tc_tb_plan>();
+    build_tb_plan(*p, tbs, n_tb, opts);
+    *plan = p.release();
+  });
+}
+
+void tc_tb_plan_destroy(tc_tb_plan* plan) { delete plan; }
+
+tc_status tc_tb_plan_sizes(const tc_tb_plan* plan, uint64_t sizes[4]) {
+  return guarded([&] {
+    if (!plan || !sizes) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    sizes[0] = plan->n_cb;
+    sizes[1] = tc_derate_plan_rx_floats(plan->derate);
+    sizes[2] = plan->bits_total;
+    sizes[3] = tc_plan_post_floats(plan->decoder);
+  });
+}
+
+tc_status tc_tb_plan_code_blocks(const tc_tb_plan* plan, uint32_t* k, uint32_t* filler, uint64_t* e) {
+  return guarded([&] {
+    if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    if (k) std::copy(plan->ks.begin(), plan->ks.end(), k);
+    if (filler) std::copy(plan->fs.begin(), plan->fs.end(), filler);
+    if (e) std::copy(plan->es.begin(), plan->es.end(), e);
+  });
+}
+
+tc_status tc_tb_plan_decode(tc_tb_plan* plan, const float* rx_dev, int32_t rv, int32_t accumulate, uint8_t* bits_dev,
+                            int32_t* tb_crc_ok_dev, int32_t* cb_crc_ok_dev, uint32_t* iterations_dev,
+                            float* posterior_dev, void* hip_stream) {
+  return guarded([&] {
+    if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    run_tb(*plan, rx_dev, rv, accumulate, bits_dev, tb_crc_ok_dev, cb_crc_ok_dev, iterations_dev, posterior_dev,
+           static_cast<hipStream_t>(hip_stream));
+  });
+}
+
+tc_status tc_decode_transport_blocks_host(const tc_tb_desc* tbs, size_t n_tb, const tc_options* opts, const float* rx,
+                                          uint8_t* bits, int32_t* tb_crc_ok, int32_t* cb_crc_ok,
+                                          uint32_t* iterations, float* posterior) {
+  return guarded([&] {
+    tc_tb_plan p;
+    build_tb_plan(p, tbs, n_tb, opts);
+    if (n_tb == 0) return;
+    if (!rx || !bits || !tb_crc_ok) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    const size_t nrx = tc_derate_plan_rx_floats(p.derate), npost = tc_plan_post_floats(p.decoder);
+    DevBuf<float> d_rx(nrx), d_post(npost);
+    DevBuf<uint8_t> d_bits(p.bits_total);
+    DevBuf<int32_t> d_tb(n_tb), d_cb(p.n_cb);
+    DevBuf<uint32_t> d_it(n_tb);
+    hip_check(hipMemcpy(d_rx.p, rx, nrx * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    run_tb(p, d_rx.p, -1, 0, d_bits.p, d_tb.p, d_cb.p, d_it.p, d_post.p, nullptr);
+    hip_check(hipDeviceSynchronize(), "transport block decode");
+    hip_check(hipMemcpy(bits, d_bits.p, p.bits_total, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    hip_check(hipMemcpy(tb_crc_ok, d_tb.p, n_tb * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    if (cb_crc_ok) hip_check(hipMemcpy(cb_crc_ok, d_cb.p, p.n_cb * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    if (iterations) hip_check(hipMemcpy(iterations, d_it.p, n_tb * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    if (posterior) hip_check(hipMemcpy(posterior, d_post.p, npost * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  });
+}
+
+}  // extern "C"

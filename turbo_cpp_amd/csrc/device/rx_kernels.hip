@@ -1,0 +1,115 @@
+// Receive-chain kernels: soft de-rate matching with HARQ combining, and
+// transport-block assembly.
+//
+// Soft de-rate matching restates lte_turbo.cpp:280-325 for float LLRs.  The
+// reference walks the circular buffer from k0(rv), skips NULL entries, adds
+// +-8 per received hard bit onto the entry, then copies every entry to its
+// stream position.  Since the non-NULL entries are visited in a fixed cyclic
+// order, received value k lands on rank (start[rv] + k) mod n_valid; a
+// workgroup per code block therefore
+//   1. sums the received values per rank in k order into LDS (coalesced reads;
+//      more than one term only when E > n_valid, i.e. repetition), and
+//   2. writes each [d0|d1|d2] position from the LDS entry of its rank
+//      (coalesced writes, one gather from LDS).
+// The summation order (0 + first + second + ...) is the reference's, so the
+// output is bit-identical to it for +-8 inputs.
+#include <hip/hip_runtime.h>
+
+#include "rx_types.h"
+
+namespace tcdev {
+
+__global__ __launch_bounds__(kDerateThreads) void derate_soft_kernel(const DerateCb* __restrict__ cbs,
+                                                                     const uint16_t* __restrict__ rank,
+                                                                     const float* __restrict__ rx,
+                                                                     float* __restrict__ out, int32_t rv_override,
+                                                                     int32_t accumulate) {
+  extern __shared__ float acc[];
+  const DerateCb c = cbs[blockIdx.x];
+  const float* in = rx + c.in_off;
+  const uint32_t nv = c.n_valid, e = c.e;
+  if (e <= nv) {
+    for (uint32_t m = threadIdx.x; m < e; m += kDerateThreads) acc[m] = 0.0f + in[m];
+  } else {
+    for (uint32_t m = threadIdx.x; m < nv; m += kDerateThreads) {
+      float v = 0.0f;
+      for (uint32_t k = m; k < e; k += nv) v += in[k];
+      acc[m] = v;
+    }
+  }
+  __syncthreads();
+  const uint32_t rv = (rv_override >= 0 ? static_cast<uint32_t>(rv_override) : c.rv) & 3u;
+  const uint32_t n0 = c.start[rv];
+  const uint16_t* rk = rank + c.rank_off;
+  float* o = out + c.out_off;
+  const uint32_t n = 3 * c.d;
+  for (uint32_t i = threadIdx.x; i < n; i += kDerateThreads) {
+    const uint32_t r = rk[i];
+    const uint32_t m = (r >= n0) ? r - n0 : r + nv - n0;
+    const bool seen = (r != kNullRank) && m < e;
+    if (accumulate) {
+      if (seen) o[i] = o[i] + acc[m];
+    } else {
+      o[i] = seen ? acc[m] : 0.0f;
+    }
+  }
+}
+
+// XOR over the workgroup (kTbThreads lanes); every thread gets the result.
+__device__ uint32_t block_xor(uint32_t x, uint32_t* red) {
+  for (int o = 32; o > 0; o >>= 1) x ^= __shfl_xor(x, o, 64);
+  const uint32_t wave = threadIdx.x >> 6;
+  __syncthreads();  // red[] free from the previous reduction
+  if ((threadIdx.x & 63) == 0) red[wave] = x;
+  __syncthreads();
+  uint32_t y = 0;
+  for (int w = 0; w < kTbThreads / 64; ++w) y ^= red[w];
+  return y;
+}
+
+// Transport-block assembly (lte_turbo.cpp:958-979), one workgroup per TB:
+// hard decision LLR >= 0 -> 1 (:488-494), filler bits dropped, CRC24B over
+// each block when c > 1 (its 24 parity bits then dropped), CRC24A over the
+// concatenation, payload = all but the last 24 bits, iterations = max over
+// the blocks.  A CRC is checked as the remainder of the whole sequence
+// a_0 .. a_{n-1} (a_0 the highest power): XOR of D^(n-1-i) mod G over the
+// ones, from the tables pow24a / pow24b; zero remainder = pass, as the
+// reference's long division (:137-155).
+__global__ __launch_bounds__(kTbThreads) void tb_assemble_kernel(
+    const TbInfo* __restrict__ tbs, const TbCb* __restrict__ cbs, const float* __restrict__ post,
+    const uint32_t* __restrict__ cb_iters, uint32_t max_iterations, const uint32_t* __restrict__ pow24a,
+    const uint32_t* __restrict__ pow24b, uint8_t* __restrict__ bits, int32_t* __restrict__ tb_ok,
+    int32_t* __restrict__ cb_ok, uint32_t* __restrict__ tb_iters) {
+  __shared__ uint32_t red[kTbThreads / 64];
+  const TbInfo t = tbs[blockIdx.x];
+  const uint32_t len = t.b + 24;  // bits under the CRC24A
+  const bool seg = t.c > 1;
+  uint32_t tb_x = 0, pos = 0, it_max = 0;
+  for (uint32_t r = 0; r < t.c; ++r) {
+    const TbCb cb = cbs[t.cb_first + r];
+    const uint32_t n = cb.k - cb.filler;     // after the filler strip
+    const uint32_t np = seg ? n - 24 : n;    // bits that go to the TB
+    const float* p = post + cb.post_off + cb.filler;
+    uint32_t cb_x = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += kTbThreads) {
+      const bool one = p[i] >= 0.0f;
+      if (one && seg) cb_x ^= pow24b[n - 1 - i];
+      if (i < np) {
+        const uint32_t q = pos + i;
+        if (one) tb_x ^= pow24a[len - 1 - q];
+        if (bits && q < t.b) bits[t.bits_off + q] = one ? 1 : 0;
+      }
+    }
+    if (seg) cb_x = block_xor(cb_x, red);
+    if (cb_ok && threadIdx.x == 0) cb_ok[t.cb_first + r] = seg ? (cb_x == 0 && n >= 24) : 1;
+    if (cb_iters) it_max = max(it_max, cb_iters[t.cb_first + r]);
+    pos += np;
+  }
+  tb_x = block_xor(tb_x, red);
+  if (threadIdx.x == 0) {
+    if (tb_ok) tb_ok[blockIdx.x] = (tb_x == 0 && pos == len) ? 1 : 0;
+    if (tb_iters) tb_iters[blockIdx.x] = max_iterations ? it_max : 0;
+  }
+}
+
+}  // namespace tcdev

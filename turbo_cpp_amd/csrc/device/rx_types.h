@@ -1,0 +1,54 @@
+// Plain structures shared by the receive-chain plan builders and kernels:
+// soft de-rate matching with HARQ combining, and transport-block assembly.
+#pragma once
+
+#include <cstdint>
+
+namespace tcdev {
+
+// One code block of a soft de-rate matching batch.  Received LLR k of the
+// block (in_off + k, k < e) lands on circular-buffer rank
+// (start[rv] + k) mod n_valid; rank_off locates the (K, F) table that gives
+// the rank of each [d0|d1|d2] output position (0xFFFF = NULL entry).
+struct DerateCb {
+  uint64_t in_off;
+  uint64_t out_off;
+  uint32_t e;
+  uint32_t d;
+  uint32_t n_valid;
+  uint32_t rank_off;
+  uint32_t start[4];
+  uint32_t rv;
+  uint32_t pad;
+};
+static_assert(sizeof(DerateCb) == 56, "DerateCb layout");
+
+// Transport-block assembly: one TB of B payload bits carried by code blocks
+// cb_first .. cb_first + c - 1; its payload lands at bits_off of the output.
+struct TbInfo {
+  uint64_t bits_off;
+  uint32_t b;
+  uint32_t cb_first;
+  uint32_t c;
+  uint32_t pad;
+};
+static_assert(sizeof(TbInfo) == 24, "TbInfo layout");
+
+// A code block as the assembly sees it: its K posteriors at post_off.
+struct TbCb {
+  uint64_t post_off;
+  uint32_t k;
+  uint32_t filler;
+};
+static_assert(sizeof(TbCb) == 16, "TbCb layout");
+
+constexpr int kTbThreads = 256;
+// D^m mod G(D) tables: CRC24B for m < 6144 (a code block), CRC24A up to the
+// longest transport block of a plan.
+constexpr uint32_t kCrc24APoly = 0xC199BFu;  // lte_turbo.cpp:104-107, D^24 implicit
+constexpr uint32_t kCrc24BPoly = 0xC00063u;  // lte_turbo.cpp:108-111
+
+constexpr uint16_t kNullRank = 0xFFFF;
+constexpr int kDerateThreads = 1024;
+
+}  // namespace tcdev

@@ -78,11 +78,73 @@ std::vector<std::uint8_t> hard_bits(const std::vector<float>& post) {
 }
 }  // namespace
 
-// lte_turbo.cpp:891-980.  CpuReference decodes block by block; both GPU
+namespace {
+// True when the code blocks are those encode_transport_block makes for
+// `params` (lte_turbo.cpp:690-720), so the device pipeline, which derives the
+// geometry from the parameters, sees the same blocks.
+bool standard_geometry(const LteTurboCodecParams& params, const EncodedTransportBlock& encoded) {
+  try {
+    if (params.transport_block_bits == 0) return false;
+    const auto seg = detail::segmentation_sizes(params.transport_block_bits + 24);
+    if (seg.c != encoded.code_blocks.size()) return false;
+    for (std::size_t r = 0; r < seg.c; ++r) {
+      const auto& d = encoded.code_blocks[r].descriptor;
+      const std::size_t k = r < seg.c_minus ? seg.k_minus : seg.k_plus;
+      const std::size_t f = r == 0 ? seg.filler_bits : 0;
+      const std::size_t e = detail::compute_e(r, params.g_total, seg.c, params.num_layers, params.mod_order);
+      if (d.k_r != k || d.d_r != k + 4 || d.filler_count != f || d.e_r != e ||
+          encoded.code_blocks[r].rate_matched_bits.size() != e)
+        return false;
+    }
+    return true;
+  } catch (const std::exception&) {
+    return false;
+  }
+}
+
+// The whole receive chain on the device: +-8 per received bit (the
+// reference's de-rate input, lte_turbo.cpp:280-325), soft de-rate matching,
+// batched decode and TB assembly.
+DecodedTransportBlock decode_on_device(const LteTurboCodecParams& params, const EncodedTransportBlock& encoded,
+                                       int decoder) {
+  std::vector<float> rx;
+  std::size_t nk = 0;
+  for (const auto& blk : encoded.code_blocks) {
+    for (std::uint8_t b : blk.rate_matched_bits) rx.push_back(b ? 8.0F : -8.0F);
+    nk += blk.descriptor.k_r;
+  }
+  const std::size_t c = encoded.code_blocks.size();
+  tc_tb_desc t{params.transport_block_bits, params.g_total, params.rv, params.num_layers, params.mod_order};
+  tc_options o{};
+  o.max_iterations = params.max_iterations;
+  o.early_stop = params.enable_early_stop ? 1 : 0;
+  o.decoder = decoder;
+  DecodedTransportBlock out;
+  out.tb_bits.resize(params.transport_block_bits);
+  std::vector<int32_t> cb_ok(c, 0);
+  std::vector<float> post(nk);
+  int32_t tb_ok = 0;
+  uint32_t iters = 0;
+  check(tc_decode_transport_blocks_host(&t, 1, &o, rx.data(), out.tb_bits.data(), &tb_ok, cb_ok.data(), &iters,
+                                        post.data()));
+  for (int32_t v : cb_ok) out.code_block_crc_ok.push_back(v != 0);
+  out.transport_block_crc_ok = tb_ok != 0;
+  out.iterations_used = iters;
+  out.posterior_llr = std::move(post);
+  return out;
+}
+}  // namespace
+
+// lte_turbo.cpp:891-980.  CpuReference decodes block by block.  Both GPU
 // backends decode all code blocks of the transport block in one batched
-// launch (the reference used one std::async thread per block instead).
+// launch (the reference used one std::async thread per block instead); for
+// transport blocks laid out as encode_transport_block makes them, de-rate
+// matching and TB assembly run on the device too.
 DecodedTransportBlock decode_transport_block(const LteTurboCodecParams& params, const EncodedTransportBlock& encoded,
                                              DecoderBackend backend) {
+  if (backend != DecoderBackend::CpuReference && standard_geometry(params, encoded))
+    return decode_on_device(params, encoded,
+                            backend == DecoderBackend::GpuStage1Exact ? TC_DECODER_EXACT : TC_DECODER_WINDOW_LOG);
   DecodedTransportBlock out;
   const std::size_t c = encoded.code_blocks.size();
   out.posterior_llr = std::vector<float>{};

@@ -185,18 +185,6 @@ std::size_t start_position(const CircularBuffer& cb, std::uint32_t rv) {
   return cb.rows * (2 * ceil_div(cb.ncb, 8 * cb.rows) * rv + 2);
 }
 
-std::size_t compute_e(std::size_t r, std::size_t g_total, std::size_t c, std::size_t num_layers,
-                      std::size_t mod_order) {
-  // lte_turbo.cpp:167-185
-  if (c == 0) throw std::invalid_argument("code block count must be positive");
-  if (num_layers == 0 || mod_order == 0)
-    throw std::invalid_argument("num_layers and mod_order must be positive");
-  const std::size_t nq = num_layers * mod_order;
-  if (g_total % nq != 0) throw std::invalid_argument("g_total must be divisible by num_layers * mod_order");
-  const std::size_t gp = g_total / nq, gamma = gp % c;
-  return (r + gamma + 1 <= c) ? nq * (gp / c) : nq * ceil_div(gp, c);
-}
-
 std::vector<std::uint8_t> rate_match_bits(std::span<const std::uint8_t> d0, std::span<const std::uint8_t> d1,
                                           std::span<const std::uint8_t> d2, std::size_t filler,
                                           std::size_t e, std::uint32_t rv) {
@@ -317,6 +305,38 @@ void siso(std::span<const float> sys, std::span<const float> parity, std::span<c
 
 }  // namespace
 
+std::size_t compute_e(std::size_t r, std::size_t g_total, std::size_t c, std::size_t num_layers,
+                      std::size_t mod_order) {
+  // lte_turbo.cpp:167-185
+  if (c == 0) throw std::invalid_argument("code block count must be positive");
+  if (num_layers == 0 || mod_order == 0)
+    throw std::invalid_argument("num_layers and mod_order must be positive");
+  const std::size_t nq = num_layers * mod_order;
+  if (g_total % nq != 0) throw std::invalid_argument("g_total must be divisible by num_layers * mod_order");
+  const std::size_t gp = g_total / nq, gamma = gp % c;
+  return (r + gamma + 1 <= c) ? nq * (gp / c) : nq * ceil_div(gp, c);
+}
+
+DerateGeometry derate_geometry(std::size_t d, std::size_t filler) {
+  const auto cb = make_circular_buffer(d, filler);
+  DerateGeometry g;
+  g.rank.assign(3 * d, 0xFFFFU);
+  std::vector<std::uint32_t> before(cb.ncb + 1, 0);  // non-NULL entries in [0, j)
+  std::uint32_t n = 0;
+  for (std::size_t j = 0; j < cb.ncb; ++j) {
+    before[j] = n;
+    const std::int32_t s = cb.src[j];
+    if (s < 0) continue;
+    if (n > 0xFFFEU) throw std::invalid_argument("circular buffer too long for soft de-rate matching");
+    g.rank[(s >> 16) * d + (s & 0xFFFF)] = static_cast<std::uint16_t>(n++);
+  }
+  before[cb.ncb] = n;
+  if (n == 0) throw std::invalid_argument("circular buffer holds no bits");
+  g.n_valid = n;
+  for (std::uint32_t rv = 0; rv < 4; ++rv) g.start[rv] = before[start_position(cb, rv) % cb.ncb] % n;
+  return g;
+}
+
 void tail_beta_log(const float tail_sys[3], const float tail_par[3], float beta[8]) {
   for (unsigned s = 0; s < 8; ++s) {
     unsigned cur = s;
@@ -419,12 +439,12 @@ bool check_crc24b(std::span<const std::uint8_t> b) { return detail::crc_check(1,
 bool check_crc16(std::span<const std::uint8_t> b) { return detail::crc_check(2, b); }
 bool check_crc8(std::span<const std::uint8_t> b) { return detail::crc_check(3, b); }
 
-SegmentationResult segment_transport_block(std::span<const std::uint8_t> bits) {
-  // 36.212 5.1.2 as at lte_turbo.cpp:540-604
-  if (bits.empty()) throw std::invalid_argument("transport block cannot be empty");
+namespace detail {
+SegmentationResult segmentation_sizes(std::size_t b) {
+  // 36.212 5.1.2 as at lte_turbo.cpp:540-604 (sizes only)
+  if (b == 0) throw std::invalid_argument("transport block cannot be empty");
   constexpr std::size_t Z = 6144;
   SegmentationResult r;
-  const std::size_t b = bits.size();
   std::size_t bp = b;
   if (b > Z) {
     r.l = 24;
@@ -447,6 +467,13 @@ SegmentationResult segment_transport_block(std::span<const std::uint8_t> bits) {
     r.c_plus = r.c - r.c_minus;
   }
   r.filler_bits = r.c_plus * r.k_plus + r.c_minus * r.k_minus - bp;
+  return r;
+}
+}  // namespace detail
+
+SegmentationResult segment_transport_block(std::span<const std::uint8_t> bits) {
+  // 36.212 5.1.2 as at lte_turbo.cpp:540-604
+  SegmentationResult r = detail::segmentation_sizes(bits.size());
   std::size_t cursor = 0;
   for (std::size_t i = 0; i < r.c; ++i) {
     const std::size_t kr = (i < r.c_minus) ? r.k_minus : r.k_plus;

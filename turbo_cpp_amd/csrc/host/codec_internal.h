@@ -42,6 +42,27 @@ void tail_beta_log(const float tail_sys[3], const float tail_par[3], float beta[
 void decoder_tail_llrs(const float* d0, const float* d1, const float* d2, std::size_t k,
                        float out[12]);
 
+// E_r of code block r (lte_turbo.cpp:167-185); invalid_argument on bad input.
+std::size_t compute_e(std::size_t r, std::size_t g_total, std::size_t c, std::size_t num_layers,
+                      std::size_t mod_order);
+
+// Segmentation sizes (C, C+/-, K+/-, F, L) of a transport block of b bits
+// including its CRC24A (lte_turbo.cpp:540-604); code_blocks stays empty.
+SegmentationResult segmentation_sizes(std::size_t b);
+
+// Soft de-rate matching geometry of one code block (d = K + 4 per stream,
+// filler bits F), from the circular buffer of lte_turbo.cpp:195-264:
+// rank[s * d + i] is the position of stream s entry i among the non-NULL
+// circular-buffer entries (0xFFFF for the NULL filler entries of d(0)/d(1)),
+// n_valid their count, start[rv] the rank of the first entry read from k0(rv).
+// Received bit k of a transmission lands on rank (start[rv] + k) mod n_valid.
+struct DerateGeometry {
+  std::vector<std::uint16_t> rank;
+  std::uint32_t n_valid = 0;
+  std::uint32_t start[4] = {0, 0, 0, 0};
+};
+DerateGeometry derate_geometry(std::size_t d, std::size_t filler);
+
 // Thread-local last-error message used by the C ABI.
 void set_last_error(const std::string& msg);
 const char* last_error();
