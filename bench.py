@@ -131,7 +131,8 @@ def rx_chain_leg(T, torch, stream, llr, decode_ms):
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
     def timed(fn, reps):
-        fn()
+        for _ in range(3):
+            fn()
         torch.cuda.synchronize()
         a.record(stream)
         for _ in range(reps):
@@ -143,17 +144,192 @@ def rx_chain_leg(T, torch, stream, llr, decode_ms):
     der_ms = timed(lambda: der.run(d_rx.data_ptr(), soft.data_ptr(), stream=s), 10)
     # the derate output must be the bench's decoder input
     assert torch.equal(soft.view(n, 3 * d).cpu(), torch.from_numpy(llr))
-    tb_ms = timed(lambda: tb.decode(d_rx.data_ptr(), bits.data_ptr(), ok.data_ptr(), stream=s), 3)
+    tb_ms = timed(lambda: tb.decode(d_rx.data_ptr(), bits.data_ptr(), ok.data_ptr(), stream=s), 5)
     okh = ok.cpu().numpy().astype(bool)
     ebn0 = np.array([EBN0[i % len(EBN0)] for i in range(n)])
     der_bytes = n * (4 * e + 4 * 3 * d)  # received LLRs in, decoder input out
     der_gbs = der_bytes / (der_ms / 1e3) / 1e9
     log(f"rx chain: derate {der_ms:.3f} ms ({der_gbs:.0f} GB/s), full TB decode {tb_ms:.3f} ms")
     return {"transport_blocks": n, "tb_bits": K - 24, "ms": tb_ms, "tb_gbit_s": n * (K - 24) / tb_ms / 1e6,
-            "derate_ms": der_ms, "assembly_ms_est": max(tb_ms - der_ms - decode_ms, 0.0),
+            "derate_ms": der_ms, "decode_ms_main_loop": decode_ms,
             "derate_roofline": {"bound": "hbm", "achieved": der_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": der_gbs / HBM_PEAK_GBS, "bytes_per_launch": der_bytes},
             "tb_crc_pass": {f"{x:.2f}": float(okh[ebn0 == x].mean()) for x in EBN0}}
+
+
+def _graph_time(torch, fn, reps):
+    """Capture fn (which enqueues on the current stream) in a HIP graph and
+    return the mean replay time in ms."""
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        fn()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        fn()
+    g.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps, g
+
+
+def _plain_sample(T, torch, ks, llr_dev, offs, iters):
+    """The window decoder launched once per code block (one plan per K),
+    sequentially, with CRC24B early stop: info Gbit/s on the given blocks."""
+    plans = {}
+    post = torch.empty(max(ks), dtype=torch.float32, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    for k in set(ks):
+        plans[k] = T.Plan([k], T.DECODER_WINDOW_LOG, iters, early_stop=True)
+    for k, o in zip(ks, offs):  # warm-up
+        plans[k].decode(llr_dev.data_ptr() + 4 * o, post.data_ptr(), 0, 0, s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, o in zip(ks, offs):
+        plans[k].decode(llr_dev.data_ptr() + 4 * o, post.data_ptr(), 0, 0, s)
+    torch.cuda.synchronize()
+    return sum(ks) / (time.perf_counter() - t0) / 1e9
+
+
+def _margin_n0(qm, b, g, margin_db):
+    """Noise variance for unit-energy symbols at Es/N0 = Shannon limit of the
+    transport block's spectral efficiency (qm b / g bits per symbol) + margin."""
+    se = qm * b / g
+    esn0 = (2.0 ** se - 1.0) * 10.0 ** (margin_db / 10.0)
+    return 1.0 / esn0
+
+
+def uneven_leg(T, torch, n_cb_target=20000, seed=31):
+    """Uneven batch: transport blocks of B uniform in [1000, 75376] bits (CRC24A,
+    segmentation with filler and CRC24B), code rate uniform in [0.3, 0.9],
+    QPSK/16QAM/64QAM uniform, rv uniform 0..3, AWGN at Es/N0 = Shannon limit +
+    uniform [1, 3] dB, max-log LLRs; about 20k code blocks per launch;
+    window decoder with CRC24B early stop, max 8 iterations.  Data are made on
+    the GPU (encoder, channel); the timed region (de-rate matching, decode, TB
+    assembly) is one captured HIP graph."""
+    rng = np.random.default_rng(seed)
+    tbs, n0s, ncb = [], [], 0
+    while ncb < n_cb_target:
+        b = int(rng.integers(1000, 75377))
+        qm = int(rng.choice([2, 4, 6]))
+        rate = float(rng.uniform(0.3, 0.9))
+        c = T._segment_sizes(b + 24)[0]
+        g = max(int(round(b / rate)) // qm * qm, qm * c)
+        tbs.append((b, g, int(rng.integers(0, 4)), 1, qm))
+        n0s.append(_margin_n0(qm, b, g, float(rng.uniform(1.0, 3.0))))
+        ncb += c
+    plan = T.TbPlan(tbs, T.DECODER_WINDOW_LOG, 8, early_stop=True)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    pay = torch.randint(0, 2, (plan.bits_total,), dtype=torch.uint8, device="cuda", generator=gen)
+    rm = torch.empty(plan.rx_floats, dtype=torch.uint8, device="cuda")
+    plan.encode(pay.data_ptr(), rm.data_ptr())
+    segs, i = [], 0
+    for t, n0 in zip(tbs, n0s):
+        for _ in range(T._segment_sizes(t[0] + 24)[0]):
+            segs.append((int(plan.es[i]), t[4], 0, n0))
+            i += 1
+    ch = T.ChannelPlan(segs)
+    llr = torch.empty(plan.rx_floats, dtype=torch.float32, device="cuda")
+    ch.run(seed, rm.data_ptr(), llr.data_ptr())
+    bits = torch.empty(plan.bits_total, dtype=torch.uint8, device="cuda")
+    ok = torch.empty(plan.n_tb, dtype=torch.int32, device="cuda")
+    its = torch.empty(plan.n_tb, dtype=torch.int32, device="cuda")
+
+    def run():
+        plan.decode(llr.data_ptr(), bits.data_ptr(), ok.data_ptr(), 0, its.data_ptr(),
+                    stream=torch.cuda.current_stream().cuda_stream)
+
+    ms, _ = _graph_time(torch, run, 3)
+    okh = ok.cpu().numpy().astype(bool)
+    payload_ber = float((bits != pay).float().mean())
+    # plain: the first 64 blocks' decoder inputs (from a de-rate plan), one launch per block
+    der = T.DeratePlan(plan.ks, plan.es, plan.fillers, [tbs[j][2] for j in range(len(tbs))
+                                                        for _ in range(T._segment_sizes(tbs[j][0] + 24)[0])])
+    soft = torch.empty(der.llr_floats, dtype=torch.float32, device="cuda")
+    der.run(llr.data_ptr(), soft.data_ptr())
+    soffs = np.concatenate([[0], np.cumsum(3 * (plan.ks.astype(np.int64) + 4))[:-1]])
+    plain = _plain_sample(T, torch, [int(k) for k in plan.ks[:64]], soft, [int(o) for o in soffs[:64]], 8)
+    info = sum(t[0] for t in tbs)
+    log(f"uneven: {len(tbs)} TBs {plan.n_cb} CBs, {ms:.2f} ms -> {info / ms / 1e6:.3f} Gbit/s; "
+        f"TB CRC pass {okh.mean():.3f}; plain {plain:.4f} Gbit/s")
+    return {"workload": "uneven TBs B~U[1000,75376], rate U[0.3,0.9], QPSK/16QAM/64QAM, rv U{0..3}, AWGN "
+                        "at Shannon limit + U[1,3] dB, max-log LLRs, early stop, max 8 iterations",
+            "transport_blocks": len(tbs), "code_blocks": plan.n_cb, "info_bits": info,
+            "code_block_bits": int(plan.ks.sum()), "ms": ms, "info_gbit_s": info / ms / 1e6,
+            "cb_gbit_s": float(plan.ks.sum()) / ms / 1e6, "mean_iterations_tb_max": float(its.float().mean()),
+            "tb_crc_pass": float(okh.mean()), "payload_ber": payload_ber, "graph": True,
+            "plain_gbit_s": plain, "batched_vs_plain": (float(plan.ks.sum()) / ms / 1e6) / plain}
+
+
+def mixed_leg(T, torch, n_cb=65536, seed=41):
+    """Large mixed load: 65536 code blocks, 70% K=6144 and 30% uniform over
+    the QPP table (each a transport block of B = K - 24, no filler); 16QAM over
+    i.i.d. Rayleigh fading with known CSI, Eb/N0 uniform in [2, 6] dB per
+    block; two transmissions of E = 3(K+4)/2 (rounded to 16QAM symbols) with
+    rv 0 and rv 2, soft-combined in fp32 before one decode; CRC24B early stop,
+    max 8 iterations.  Timed (one captured HIP graph): both de-rate
+    matchings, the decode and TB assembly."""
+    rng = np.random.default_rng(seed)
+    table = [int(e[0]) for e in T.qpp_interleaver_table()]
+    ks = np.where(rng.random(n_cb) < 0.7, 6144, rng.choice(table, n_cb)).astype(np.int64)
+    es = (3 * (ks + 4) // 2) // 4 * 4
+    ebn0 = rng.uniform(2.0, 6.0, n_cb)
+    tbs = [(int(k) - 24, int(e), 0, 1, 4) for k, e in zip(ks, es)]
+    plan = T.TbPlan(tbs, T.DECODER_WINDOW_LOG, 8, early_stop=True)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    pay = torch.randint(0, 2, (plan.bits_total,), dtype=torch.uint8, device="cuda", generator=gen)
+    # Es/N0 = Eb/N0 + 10 log10(qm B / E) per transmission
+    n0 = 1.0 / (10.0 ** (ebn0 / 10.0) * 4.0 * (ks - 24) / es)
+    ch = T.ChannelPlan([(int(e), 4, 1, float(v)) for e, v in zip(es, n0)])
+    rx = []
+    for rv, sd in ((0, seed), (2, seed + 1)):
+        rm = torch.empty(plan.rx_floats, dtype=torch.uint8, device="cuda")
+        plan.encode(pay.data_ptr(), rm.data_ptr(), rv=rv)
+        y = torch.empty(plan.rx_floats, dtype=torch.float32, device="cuda")
+        ch.run(sd, rm.data_ptr(), y.data_ptr())
+        rx.append(y)
+        del rm
+    bits = torch.empty(plan.bits_total, dtype=torch.uint8, device="cuda")
+    ok = torch.empty(plan.n_tb, dtype=torch.int32, device="cuda")
+    its = torch.empty(plan.n_tb, dtype=torch.int32, device="cuda")
+
+    def run():
+        s = torch.cuda.current_stream().cuda_stream
+        plan.soft_combine(rx[0].data_ptr(), rv=0, stream=s)
+        plan.soft_combine(rx[1].data_ptr(), rv=2, accumulate=True, stream=s)
+        plan.decode(0, bits.data_ptr(), ok.data_ptr(), 0, its.data_ptr(), stream=s)
+
+    ms, _ = _graph_time(torch, run, 2)
+    okh = ok.cpu().numpy().astype(bool)
+    mean_it = float(its.float().mean())
+    kb = int(ks.sum())
+    gbit = kb / ms / 1e6
+    # algorithmic HBM bytes: two received LLR sets, soft buffer written twice and read
+    # once (first combine overwrites, second reads and writes), posteriors, payload bits
+    hbm = 2 * 4 * int(es.sum()) + 4 * 3 * int((3 * (ks + 4)).sum()) + 4 * kb + plan.bits_total
+    valu = kb * mean_it * 864 / (ms / 1e3)
+    # plain path on 64 blocks: one launch per block on the combined decoder input
+    der = T.DeratePlan(plan.ks[:64], plan.es[:64])
+    soft = torch.empty(der.llr_floats, dtype=torch.float32, device="cuda")
+    der.run(rx[0].data_ptr(), soft.data_ptr(), rv=0)
+    der.run(rx[1].data_ptr(), soft.data_ptr(), rv=2, accumulate=True)
+    soffs = np.concatenate([[0], np.cumsum(3 * (plan.ks[:64].astype(np.int64) + 4))[:-1]])
+    plain = _plain_sample(T, torch, [int(k) for k in plan.ks[:64]], soft, [int(o) for o in soffs], 8)
+    log(f"mixed: {n_cb} CBs {ms:.1f} ms -> {gbit:.3f} Gbit/s (mean iterations {mean_it:.2f}), "
+        f"BLER {1 - okh.mean():.4f}; plain {plain:.4f} Gbit/s")
+    return {"workload": "65536 CBs (70% K=6144, 30% U(QPP table)), 16QAM, Rayleigh with known CSI, "
+                        "Eb/N0 U[2,6] dB, HARQ rv0+rv2 (E=3(K+4)/2 each) combined in fp32, early stop, max 8",
+            "code_blocks": n_cb, "code_block_bits": kb, "ms": ms, "gbit_s": gbit,
+            "mean_iterations": mean_it, "bler": float(1 - okh.mean()), "graph": True,
+            "hbm_gb_s": hbm / (ms / 1e3) / 1e9, "hbm_frac": hbm / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "valu_frac": valu / VALU_PEAK, "plain_gbit_s": plain, "batched_vs_plain": gbit / plain}
 
 
 def diff_leg(T, torch, train_steps):
@@ -235,6 +411,7 @@ def main():
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-rx-chain", action="store_true")
     ap.add_argument("--train-steps", type=int, default=500, help="Adam steps of the differentiation leg (0: skip it)")
+    ap.add_argument("--no-scenarios", action="store_true", help="skip the uneven-batch and mixed-load legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -324,6 +501,8 @@ def main():
 
     rx_chain = None if args.no_rx_chain else rx_chain_leg(T, torch, stream, llr, kernel_ms)
     diff = diff_leg(T, torch, args.train_steps) if args.train_steps > 0 else None
+    uneven = None if args.no_scenarios else uneven_leg(T, torch)
+    mixed = None if args.no_scenarios else mixed_leg(T, torch)
 
     if args.copy_calibration:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -375,6 +554,8 @@ def main():
         "rx_chain": rx_chain,
         "bwd_gbit_s": diff["bwd_gbit_s"] if diff else None,
         "differentiation": diff,
+        "uneven_batch": uneven,
+        "mixed_load": mixed,
     }
     print(json.dumps(out), flush=True)
 

@@ -143,8 +143,10 @@ tc_status tc_derate_match_soft(const float* rx, uint64_t e, uint32_t k, uint32_t
  * tb_crc_ok / iterations: one per TB; cb_crc_ok: one per code block;
  * posterior: sum K_r floats (NULL: kept inside the plan).  Any output may be
  * NULL.  The plan holds the decoder input between calls: accumulate != 0 adds
- * this transmission onto it (HARQ), rv as in tc_derate_plan_run.  A call
- * allocates nothing; calls on one plan must not overlap (shared buffers). */
+ * this transmission onto it (HARQ), rv as in tc_derate_plan_run; rx NULL
+ * decodes the held input as it stands (e.g. after tc_tb_plan_soft_combine).
+ * A call allocates nothing; calls on one plan must not overlap (shared
+ * buffers). */
 typedef struct {
   uint64_t transport_block_bits; /* B: payload bits without the CRC24A */
   uint64_t g_total;              /* coded bits for the TB */
@@ -163,10 +165,53 @@ tc_status tc_tb_plan_code_blocks(const tc_tb_plan* plan, uint32_t* k, uint32_t* 
 tc_status tc_tb_plan_decode(tc_tb_plan* plan, const float* rx_dev, int32_t rv, int32_t accumulate,
                             uint8_t* bits_dev, int32_t* tb_crc_ok_dev, int32_t* cb_crc_ok_dev,
                             uint32_t* iterations_dev, float* posterior_dev, void* hip_stream);
+/* The transmit side of the same plan, bit-exact with encode_transport_block
+ * (lte_turbo.cpp:690-720): CRC24A per TB, segmentation with filler and
+ * CRC24B, turbo encoder with the reference's tail multiplexing, rate
+ * matching.  payload = sum B bytes (0/1, TB order); rm_bits = sum E_r bytes
+ * (the reference's rate_matched_bits of every block back to back); cb_bits
+ * (may be NULL) = sum K_r bytes (the reference's cb_bits).  rv as in
+ * tc_derate_plan_run. */
+tc_status tc_tb_plan_encode(tc_tb_plan* plan, const uint8_t* payload_dev, int32_t rv,
+                            uint8_t* rm_bits_dev, uint8_t* cb_bits_dev, void* hip_stream);
+tc_status tc_encode_transport_blocks_host(const tc_tb_desc* tbs, size_t n_tb, const uint8_t* payload,
+                                          uint8_t* rm_bits, uint8_t* cb_bits);
+/* Soft de-rate matching of one transmission into the plan's held decoder
+ * input without decoding (HARQ: combine rv 0 and rv 2, then decode with
+ * rx NULL). */
+tc_status tc_tb_plan_soft_combine(tc_tb_plan* plan, const float* rx_dev, int32_t rv, int32_t accumulate,
+                                  void* hip_stream);
 /* Same from host buffers (one transmission, no HARQ). */
 tc_status tc_decode_transport_blocks_host(const tc_tb_desc* tbs, size_t n_tb, const tc_options* opts,
                                           const float* rx, uint8_t* bits, int32_t* tb_crc_ok,
                                           int32_t* cb_crc_ok, uint32_t* iterations, float* posterior);
+
+/* ---- channel front end (GPU) -----------------------------------------------
+ * No counterpart in the reference (it has no channel code): Gray QAM mapping
+ * of 3GPP TS 36.211 7.1 (qm 2/4/6, unit average energy; bits b0,b2,b4 on I,
+ * b1,b3,b5 on Q), AWGN or i.i.d. Rayleigh fading known at the receiver
+ * (h ~ CN(0,1) per symbol), noise CN(0, n0), and the max-log demapper
+ * LLR = (min_{b=0} |y-hs|^2 - min_{b=1} |y-hs|^2) / n0 (bit 1 > 0).
+ * Segments lie back to back: bits (and LLRs) sum n_bits, symbols sum
+ * n_bits/qm; y and h hold (re, im) per symbol.  tc_channel_llr maps, fades,
+ * adds noise from (seed, symbol index) and demaps in one pass (y, h may be
+ * NULL); tc_qam_demap demaps given y (h NULL = 1).  Both only enqueue. */
+typedef struct {
+  uint64_t n_bits;   /* multiple of qm */
+  uint32_t qm;       /* 2, 4 or 6 */
+  uint32_t rayleigh; /* 0: AWGN, 1: Rayleigh */
+  float n0;          /* noise variance per complex symbol */
+  uint32_t pad;
+} tc_channel_seg;
+typedef struct tc_channel_plan tc_channel_plan;
+tc_status tc_channel_plan_create(const tc_channel_seg* segs, size_t n_seg, tc_channel_plan** plan);
+void tc_channel_plan_destroy(tc_channel_plan* plan);
+/* sizes[0] = bits, sizes[1] = symbols */
+tc_status tc_channel_plan_sizes(const tc_channel_plan* plan, uint64_t sizes[2]);
+tc_status tc_channel_llr(tc_channel_plan* plan, uint64_t seed, const uint8_t* bits_dev, float* llr_dev,
+                         float* y_dev, float* h_dev, void* hip_stream);
+tc_status tc_qam_demap(tc_channel_plan* plan, const float* y_dev, const float* h_dev, float* llr_dev,
+                       void* hip_stream);
 
 /* ---- differentiable decoder (GPU) ------------------------------------------
  * No counterpart in the reference: a sliding-window Log-MAP turbo decoder

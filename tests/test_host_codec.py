@@ -214,3 +214,13 @@ def test_tb_plan_argument_errors_before_device():
         T.TbPlan([(40, 101, 0, 1, 2)])  # g_total not divisible by layers * Qm
     with pytest.raises(ValueError):
         T.TbPlan([(40, 100, 4, 1, 2)])
+
+
+def test_channel_plan_argument_errors_before_device():
+    with pytest.raises(ValueError):
+        T.ChannelPlan([(10, 3, 0, 1.0)])  # qm
+    with pytest.raises(ValueError):
+        T.ChannelPlan([(10, 4, 0, 1.0)])  # n_bits not a multiple of qm
+    with pytest.raises(ValueError):
+        T.ChannelPlan([(12, 4, 0, 0.0)])  # n0
+

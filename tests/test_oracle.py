@@ -425,3 +425,32 @@ def test_sw_reference_gradcheck():
     perm = np.array(O.qpp_perm(k), np.int64)
     assert torch.autograd.gradcheck(lambda a, b: R.decode(a, b, k, perm, 8, 8, 2), (x, w), eps=1e-6, atol=1e-6)
 
+
+# ------------------------------------------------------ channel front end --
+def test_channel_oracle_constellations_and_closed_forms():
+    """36.211 7.1 spot values, unit energy and the Gray property of the QAM
+    tables; the brute-force max-log LLR reduces to the QPSK closed form
+    -2 sqrt(2) Re(z) |h|^2 / N0 (bit 1 = negative level)."""
+    from oracle import channel as CH
+    p4 = CH.constellation(4) * np.sqrt(10)
+    assert p4[0b0000] == 1 + 1j and p4[0b0001] == 1 + 3j and p4[0b0010] == 3 + 1j and p4[0b1111] == -3 - 3j
+    p6 = CH.constellation(6) * np.sqrt(42)
+    for bits, val in ((0b000000, 3 + 3j), (0b000001, 3 + 1j), (0b000011, 1 + 1j), (0b001000, 5 + 3j),
+                      (0b000101, 3 + 7j), (0b111111, -7 - 7j)):
+        assert abs(p6[bits] - val) < 1e-9
+    for qm in (2, 4, 6):
+        p = CH.constellation(qm)
+        assert abs(np.mean(np.abs(p) ** 2) - 1.0) < 1e-12
+        for v in range(len(p)):
+            d = np.abs(p - p[v])
+            d[v] = 9
+            for u in np.flatnonzero(np.isclose(d, d.min())):
+                assert bin(u ^ v).count("1") == 1
+    rng = np.random.default_rng(0)
+    y = rng.normal(size=50) + 1j * rng.normal(size=50)
+    h = rng.normal(size=50) + 1j * rng.normal(size=50)
+    llr = CH.maxlog_llr(y, h, 0.5, 2).reshape(-1, 2)
+    z = y * np.conj(h) / np.abs(h) ** 2
+    g = np.abs(h) ** 2 / 0.5
+    assert np.allclose(llr[:, 0], -2 * np.sqrt(2) * z.real * g) and np.allclose(llr[:, 1], -2 * np.sqrt(2) * z.imag * g)
+

@@ -43,6 +43,11 @@ class TcTbDesc(C.Structure):
                 ("num_layers", C.c_uint32), ("mod_order", C.c_uint32)]
 
 
+class TcChannelSeg(C.Structure):
+    _fields_ = [("n_bits", C.c_uint64), ("qm", C.c_uint32), ("rayleigh", C.c_uint32), ("n0", C.c_float),
+                ("pad", C.c_uint32)]
+
+
 class TcSegmentInfo(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("k", "segment_length", "segments", "blocks_per_wave", "warmup")]
 
@@ -106,8 +111,17 @@ def lib() -> C.CDLL:
         L.tc_tb_plan_sizes.argtypes = [C.c_void_p, _u64p]
         L.tc_tb_plan_code_blocks.argtypes = [C.c_void_p, _u32p, _u32p, _u64p]
         L.tc_tb_plan_decode.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32] + [C.c_void_p] * 6
+        L.tc_tb_plan_soft_combine.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
+        L.tc_tb_plan_encode.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.tc_encode_transport_blocks_host.argtypes = [C.POINTER(TcTbDesc), C.c_size_t, _u8p, _u8p, C.c_void_p]
         L.tc_decode_transport_blocks_host.argtypes = [C.POINTER(TcTbDesc), C.c_size_t, C.POINTER(TcOptions), _f32p,
                                                       _u8p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.tc_channel_plan_create.argtypes = [C.POINTER(TcChannelSeg), C.c_size_t, C.POINTER(C.c_void_p)]
+        L.tc_channel_plan_destroy.argtypes = [C.c_void_p]
+        L.tc_channel_plan_destroy.restype = None
+        L.tc_channel_plan_sizes.argtypes = [C.c_void_p, _u64p]
+        L.tc_channel_llr.argtypes = [C.c_void_p, C.c_uint64] + [C.c_void_p] * 5
+        L.tc_qam_demap.argtypes = [C.c_void_p] * 5
         L.tc_derate_plan_create.argtypes = [_u32p, C.c_void_p, _u64p, C.c_void_p, C.c_size_t,
                                             C.POINTER(C.c_void_p)]
         L.tc_derate_plan_destroy.argtypes = [C.c_void_p]
@@ -474,6 +488,42 @@ class Plan:
             pass
 
 
+class ChannelPlan:
+    """Channel front end on the GPU (tc_channel_*): segments of (n_bits, qm,
+    rayleigh, n0) back to back.  ``run`` maps coded bits to QAM, fades, adds
+    noise and demaps (max-log) in one pass; ``demap`` demaps given y (and h)."""
+
+    def __init__(self, segs):
+        arr = (TcChannelSeg * max(len(segs), 1))()
+        for i, (nb, qm, ray, n0) in enumerate(segs):
+            arr[i] = TcChannelSeg(int(nb), int(qm), int(bool(ray)), float(n0), 0)
+        self._segs = arr
+        h = C.c_void_p()
+        _check(lib().tc_channel_plan_create(arr, len(segs), C.byref(h)))
+        self._h = h
+        sz = np.zeros(2, np.uint64)
+        _check(lib().tc_channel_plan_sizes(h, sz))
+        self.n_bits, self.n_sym = int(sz[0]), int(sz[1])
+
+    def run(self, seed: int, bits_ptr: int, llr_ptr: int, y_ptr: int = 0, h_ptr: int = 0, stream: int = 0):
+        _check(lib().tc_channel_llr(self._h, seed, bits_ptr or None, llr_ptr or None, y_ptr or None, h_ptr or None,
+                                    stream or None))
+
+    def demap(self, y_ptr: int, h_ptr: int, llr_ptr: int, stream: int = 0):
+        _check(lib().tc_qam_demap(self._h, y_ptr or None, h_ptr or None, llr_ptr or None, stream or None))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tc_channel_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def derate_match_soft(rx, k: int, filler: int = 0, rv: int = 0, out=None) -> np.ndarray:
     """Soft de-rate matching of one code block on the GPU (tc_derate_match_soft).
 
@@ -565,6 +615,16 @@ class TbPlan:
                                        tb_crc_ptr or None, cb_crc_ptr or None, iters_ptr or None,
                                        post_ptr or None, stream or None))
 
+    def soft_combine(self, rx_ptr: int, rv: int = -1, accumulate: bool = False, stream: int = 0):
+        """De-rate match one transmission into the held decoder input (no decode)."""
+        _check(lib().tc_tb_plan_soft_combine(self._h, rx_ptr or None, rv, int(accumulate), stream or None))
+
+    def encode(self, payload_ptr: int, rm_ptr: int, cb_ptr: int = 0, rv: int = -1, stream: int = 0):
+        """Transmit side: payload bits (bits_total bytes) -> rate-matched bits
+        (rx_floats bytes) and optionally code-block bits (post_floats bytes)."""
+        _check(lib().tc_tb_plan_encode(self._h, payload_ptr or None, rv, rm_ptr or None, cb_ptr or None,
+                                       stream or None))
+
     def close(self):
         if getattr(self, "_h", None):
             lib().tc_tb_plan_destroy(self._h)
@@ -624,6 +684,27 @@ def _standard_geometry(params: LteTurboCodecParams, encoded: EncodedTransportBlo
         return True
     except (ValueError, IndexError, RuntimeError):
         return False
+
+
+def encode_transport_blocks(tbs, payloads):
+    """GPU encode of many transport blocks (tc_encode_transport_blocks_host).
+
+    payloads: one 0/1 array of B bits per TB.  Returns (rate-matched bits of
+    every code block back to back, code-block bits back to back)."""
+    descs = _tb_descs(tbs)
+    n = len(tbs)
+    pay = np.concatenate([_u8(p) for p in payloads]) if n else np.zeros(1, np.uint8)
+    ne = nk = 0
+    for i in range(n):
+        c, cp, cm, kp, km, f, _ = _segment_sizes(int(descs[i].transport_block_bits) + 24)
+        ks = [km] * cm + [kp] * cp
+        nk += sum(ks)
+        ne += sum(compute_e(r, int(descs[i].g_total), c, int(descs[i].num_layers), int(descs[i].mod_order))
+                  for r in range(c))
+    rm = np.zeros(max(ne, 1), np.uint8)
+    cb = np.zeros(max(nk, 1), np.uint8)
+    _check(lib().tc_encode_transport_blocks_host(descs, n, pay, rm, cb.ctypes.data))
+    return rm[:ne], cb[:nk]
 
 
 def _segment_sizes(b: int):

@@ -22,7 +22,8 @@ ARCH = "gfx950"
 
 HOST_SOURCES = ["host/codec.cpp", "host/api.cpp"]
 DEVICE_SOURCES = ["device/exact_kernel.hip", "device/window_kernel.hip", "device/c_api.hip",
-                  "device/rx_kernels.hip", "device/rx_api.hip", "device/diff_kernel.hip",
+                  "device/rx_kernels.hip", "device/rx_api.hip", "device/tx_kernels.hip", "device/channel.hip",
+                  "device/diff_kernel.hip",
                   "device/diff_api.hip"]
 
 COMMON = ["-O3", "-std=c++20", "-fPIC", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include")]

@@ -159,7 +159,7 @@ void backward(const tc_diff_plan& p, const void* llr, const void* scales, const 
   hipLaunchKernelGGL(tcdev::diff_backward_kernel<R>, dim3(p.grid), dim3(64), 0, s,
                      args<R>(p, llr, scales, nullptr, const_cast<void*>(saved), grad_post, grad_llr));
   hip_check(hipGetLastError(), "differentiable decoder backward launch");
-  hipLaunchKernelGGL(tcdev::diff_scale_reduce_kernel<R>, dim3(1), dim3(kMaxHalves), 0, s,
+  hipLaunchKernelGGL(tcdev::diff_scale_reduce_kernel<R>, dim3(p.halves), dim3(256), 0, s,
                      static_cast<const R*>(p.d_part), p.njobs, p.halves, static_cast<R*>(grad_scales));
   hip_check(hipGetLastError(), "scale gradient reduction launch");
 }

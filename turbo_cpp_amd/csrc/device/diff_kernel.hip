@@ -619,14 +619,23 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
   }
 }
 
-// dscales[h] = sum over jobs of the per-job partial sums, in job order.
+// dscales[h] = sum over jobs of the per-job partial sums: one workgroup per
+// h, thread t sums jobs t, t + 256, ... in order, then a fixed tree over
+// the threads (deterministic).
 template <typename R>
-__global__ void diff_scale_reduce_kernel(const R* part, uint32_t njobs, uint32_t halves, R* out) {
-  const uint32_t h = threadIdx.x;
-  if (h >= halves) return;
+__global__ __launch_bounds__(256) void diff_scale_reduce_kernel(const R* part, uint32_t njobs, uint32_t halves,
+                                                                R* out) {
+  __shared__ R red[256];
+  const uint32_t h = blockIdx.x;
   R s = 0;
-  for (uint32_t j = 0; j < njobs; ++j) s += part[(size_t)j * halves + h];
-  out[h] = s;
+  for (uint32_t j = threadIdx.x; j < njobs; j += 256) s += part[(size_t)j * halves + h];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[h] = red[0];
 }
 
 template __global__ void diff_forward_kernel<float>(DiffArgs<float>);

@@ -1,6 +1,8 @@
-// C ABI of the receive chain on the GPU: soft de-rate matching with HARQ
-// combining, and batched transport-block decoding (de-rate matching, turbo
-// decoding and TB assembly chained on one stream).
+// C ABI of the transport-block chain on the GPU: soft de-rate matching with
+// HARQ combining, batched transport-block decoding (de-rate matching, turbo
+// decoding and TB assembly chained on one stream) and batched
+// transport-block encoding (CRC24A, segmentation with CRC24B, turbo encoder,
+// rate matching).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,6 +21,10 @@ namespace tcdev {
 __global__ void derate_soft_kernel(const DerateCb*, const uint16_t*, const float*, float*, int32_t, int32_t);
 __global__ void tb_assemble_kernel(const TbInfo*, const TbCb*, const float*, const uint32_t*, uint32_t,
                                    const uint32_t*, const uint32_t*, uint8_t*, int32_t*, int32_t*, uint32_t*);
+__global__ void tb_crc24a_kernel(const TbInfo*, const uint8_t*, const uint32_t*, uint32_t*);
+__global__ void cb_encode_kernel(const TxCb*, const TbInfo*, const uint8_t*, const uint32_t*, const uint32_t*,
+                                 uint8_t*, uint8_t*);
+__global__ void rate_match_kernel(const TxCb*, const uint32_t*, const uint8_t*, int32_t, uint8_t*);
 }  // namespace tcdev
 
 using namespace tchost;
@@ -112,11 +118,16 @@ struct tc_tb_plan {
   float* d_soft = nullptr;  // decoder input; the HARQ soft buffer between calls
   float* d_post = nullptr;  // posteriors when the caller passes none
   uint32_t* d_cb_iters = nullptr;
+  // transmit side
+  tcdev::TxCb* d_tx = nullptr;
+  uint32_t* d_valid = nullptr;   // per (K, F): non-NULL circular-buffer entries in order
+  uint8_t* d_streams = nullptr;  // encoder output [d0|d1|d2] per block
+  uint32_t* d_tb_crc = nullptr;  // CRC24A per TB
   ~tc_tb_plan() {
     tc_derate_plan_destroy(derate);
     tc_plan_destroy(decoder);
     for (void* p : {(void*)d_tbs, (void*)d_cbs, (void*)d_pow24a, (void*)d_pow24b, (void*)d_soft, (void*)d_post,
-                    (void*)d_cb_iters})
+                    (void*)d_cb_iters, (void*)d_tx, (void*)d_valid, (void*)d_streams, (void*)d_tb_crc})
       if (p) (void)hipFree(p);
   }
 };
@@ -143,7 +154,13 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
   std::vector<tcdev::TbInfo> info(n_tb);
   std::vector<tcdev::TbCb> cbs;
   std::vector<uint32_t> rvs;
-  uint64_t bits = 0, post = 0, max_len = 24;
+  std::vector<tcdev::TxCb> tx;
+  std::vector<uint32_t> valid;
+  struct ValidInfo {
+    uint32_t off, n, start[4];
+  };
+  std::map<std::pair<uint32_t, uint32_t>, ValidInfo> valid_of;
+  uint64_t bits = 0, post = 0, max_len = 24, d_bytes = 0, rm_bits = 0;
   for (size_t t = 0; t < n_tb; ++t) {
     const tc_tb_desc& d = tbs[t];
     if (d.transport_block_bits == 0) throw Failure{TC_ERR_INVALID_ARGUMENT, "transport block cannot be empty"};
@@ -162,6 +179,7 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
     }
     info[t] = {bits, static_cast<uint32_t>(d.transport_block_bits), static_cast<uint32_t>(p.ks.size()),
                static_cast<uint32_t>(seg.c), 0};
+    uint32_t cursor = 0;
     for (size_t r = 0; r < seg.c; ++r) {
       const uint32_t k = static_cast<uint32_t>(r < seg.c_minus ? seg.k_minus : seg.k_plus);
       const uint32_t f = static_cast<uint32_t>(r == 0 ? seg.filler_bits : 0);
@@ -171,6 +189,41 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
       p.es.push_back(e[r]);
       rvs.push_back(d.rv);
       cbs.push_back({post, k, f});
+      tcdev::TxCb x{};
+      x.cb_off = post;
+      x.d_off = d_bytes;
+      x.rm_off = rm_bits;
+      x.tb = static_cast<uint32_t>(t);
+      x.k = k;
+      x.filler = f;
+      x.cursor = cursor;
+      x.seg = seg.c > 1 ? 1u : 0u;
+      x.payload = k - f - 24 * x.seg;
+      const auto& qe = turbo_cpp::detail::kQppTable[turbo_cpp::detail::qpp_index(k)];
+      x.f1 = static_cast<uint32_t>(qe.f1);
+      x.f2 = static_cast<uint32_t>(qe.f2);
+      x.e = static_cast<uint32_t>(e[r]);
+      x.rv = d.rv;
+      auto it = valid_of.find({k, f});
+      if (it == valid_of.end()) {
+        const auto g = turbo_cpp::detail::derate_geometry(k + 4, f);
+        std::vector<uint32_t> v(g.n_valid);
+        for (uint32_t s2 = 0; s2 < 3; ++s2)
+          for (uint32_t i = 0; i < k + 4; ++i) {
+            const uint16_t rk = g.rank[s2 * (k + 4) + i];
+            if (rk != tcdev::kNullRank) v[rk] = (s2 << 16) | i;
+          }
+        ValidInfo vi{static_cast<uint32_t>(valid.size()), g.n_valid, {g.start[0], g.start[1], g.start[2], g.start[3]}};
+        valid.insert(valid.end(), v.begin(), v.end());
+        it = valid_of.emplace(std::make_pair(k, f), vi).first;
+      }
+      x.valid_off = it->second.off;
+      x.n_valid = it->second.n;
+      std::copy(it->second.start, it->second.start + 4, x.start);
+      tx.push_back(x);
+      cursor += x.payload;
+      d_bytes += 3ull * (k + 4);
+      rm_bits += e[r];
       post += k;
     }
     bits += d.transport_block_bits;
@@ -197,14 +250,36 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
   p.d_soft = dev_alloc<float>(tc_derate_plan_llr_floats(p.derate));
   p.d_post = dev_alloc<float>(post);
   p.d_cb_iters = dev_alloc<uint32_t>(p.n_cb);
+  p.d_tx = dev_alloc<tcdev::TxCb>(tx.size());
+  dev_upload(p.d_tx, tx);
+  p.d_valid = dev_alloc<uint32_t>(valid.size());
+  dev_upload(p.d_valid, valid);
+  p.d_streams = dev_alloc<uint8_t>(d_bytes);
+  p.d_tb_crc = dev_alloc<uint32_t>(n_tb);
+}
+
+void run_encode(tc_tb_plan& p, const uint8_t* payload, int32_t rv, uint8_t* rm, uint8_t* cb_bits, hipStream_t s) {
+  if (rv < -1 || rv > 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "rv must be -1 (per block) or 0..3"};
+  if (p.n_tb == 0) return;
+  if (!payload || !rm) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+  if (p.n_tb > 0x7FFFFFFFull || p.n_cb > 0x7FFFFFFFull) throw Failure{TC_ERR_INVALID_ARGUMENT, "batch too large"};
+  hipLaunchKernelGGL(tcdev::tb_crc24a_kernel, dim3(static_cast<uint32_t>(p.n_tb)), dim3(256), 0, s, p.d_tbs, payload,
+                     p.d_pow24a, p.d_tb_crc);
+  hip_check(hipGetLastError(), "CRC24A launch");
+  hipLaunchKernelGGL(tcdev::cb_encode_kernel, dim3(static_cast<uint32_t>(p.n_cb)), dim3(64), 0, s, p.d_tx, p.d_tbs,
+                     payload, p.d_tb_crc, p.d_pow24b, cb_bits, p.d_streams);
+  hip_check(hipGetLastError(), "encoder launch");
+  hipLaunchKernelGGL(tcdev::rate_match_kernel, dim3(static_cast<uint32_t>(p.n_cb)), dim3(256), 0, s, p.d_tx,
+                     p.d_valid, p.d_streams, rv, rm);
+  hip_check(hipGetLastError(), "rate matching launch");
 }
 
 void run_tb(tc_tb_plan& p, const float* rx, int32_t rv, int32_t accumulate, uint8_t* bits, int32_t* tb_ok,
             int32_t* cb_ok, uint32_t* iters, float* post, hipStream_t s) {
+  if (rv < -1 || rv > 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "rv must be -1 (per block) or 0..3"};
   if (p.n_tb == 0) return;
-  if (!rx) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
   float* post_buf = post ? post : p.d_post;
-  run_derate(*p.derate, rx, rv, accumulate, p.d_soft, s);
+  if (rx) run_derate(*p.derate, rx, rv, accumulate, p.d_soft, s);  // else: decode the soft buffer as it stands
   check_status(tc_plan_decode(p.decoder, p.d_soft, post_buf, nullptr, p.d_cb_iters, s));
   if (p.n_tb > 0x7FFFFFFFull) throw Failure{TC_ERR_INVALID_ARGUMENT, "too many transport blocks"};
   hipLaunchKernelGGL(tcdev::tb_assemble_kernel, dim3(static_cast<uint32_t>(p.n_tb)), dim3(tcdev::kTbThreads), 0, s,
@@ -292,6 +367,41 @@ tc_status tc_tb_plan_decode(tc_tb_plan* plan, const float* rx_dev, int32_t rv, i
     if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
     run_tb(*plan, rx_dev, rv, accumulate, bits_dev, tb_crc_ok_dev, cb_crc_ok_dev, iterations_dev, posterior_dev,
            static_cast<hipStream_t>(hip_stream));
+  });
+}
+
+tc_status tc_tb_plan_soft_combine(tc_tb_plan* plan, const float* rx_dev, int32_t rv, int32_t accumulate,
+                                  void* hip_stream) {
+  return guarded([&] {
+    if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    if (plan->n_tb == 0) return;
+    if (!rx_dev) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    run_derate(*plan->derate, rx_dev, rv, accumulate, plan->d_soft, static_cast<hipStream_t>(hip_stream));
+  });
+}
+
+tc_status tc_tb_plan_encode(tc_tb_plan* plan, const uint8_t* payload_dev, int32_t rv, uint8_t* rm_bits_dev,
+                            uint8_t* cb_bits_dev, void* hip_stream) {
+  return guarded([&] {
+    if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    run_encode(*plan, payload_dev, rv, rm_bits_dev, cb_bits_dev, static_cast<hipStream_t>(hip_stream));
+  });
+}
+
+tc_status tc_encode_transport_blocks_host(const tc_tb_desc* tbs, size_t n_tb, const uint8_t* payload,
+                                          uint8_t* rm_bits, uint8_t* cb_bits) {
+  return guarded([&] {
+    tc_tb_plan p;
+    build_tb_plan(p, tbs, n_tb, nullptr);
+    if (n_tb == 0) return;
+    if (!payload || !rm_bits) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    const size_t nrm = tc_derate_plan_rx_floats(p.derate), ncb = tc_plan_post_floats(p.decoder);
+    DevBuf<uint8_t> d_pay(p.bits_total), d_rm(nrm), d_cb(ncb);
+    hip_check(hipMemcpy(d_pay.p, payload, p.bits_total, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    run_encode(p, d_pay.p, -1, d_rm.p, cb_bits ? d_cb.p : nullptr, nullptr);
+    hip_check(hipDeviceSynchronize(), "transport block encode");
+    hip_check(hipMemcpy(rm_bits, d_rm.p, nrm, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    if (cb_bits) hip_check(hipMemcpy(cb_bits, d_cb.p, ncb, hipMemcpyDeviceToHost), "hipMemcpy D2H");
   });
 }
 

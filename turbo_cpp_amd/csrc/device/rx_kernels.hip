@@ -28,8 +28,21 @@ __global__ __launch_bounds__(kDerateThreads) void derate_soft_kernel(const Derat
   const DerateCb c = cbs[blockIdx.x];
   const float* in = rx + c.in_off;
   const uint32_t nv = c.n_valid, e = c.e;
+  constexpr uint32_t U = 4;  // loads in flight per thread
   if (e <= nv) {
-    for (uint32_t m = threadIdx.x; m < e; m += kDerateThreads) acc[m] = 0.0f + in[m];
+    for (uint32_t base = 0; base < e; base += U * kDerateThreads) {
+      float v[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t m = base + u * kDerateThreads + threadIdx.x;
+        v[u] = m < e ? in[m] : 0.0f;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t m = base + u * kDerateThreads + threadIdx.x;
+        if (m < e) acc[m] = 0.0f + v[u];
+      }
+    }
   } else {
     for (uint32_t m = threadIdx.x; m < nv; m += kDerateThreads) {
       float v = 0.0f;
@@ -43,14 +56,26 @@ __global__ __launch_bounds__(kDerateThreads) void derate_soft_kernel(const Derat
   const uint16_t* rk = rank + c.rank_off;
   float* o = out + c.out_off;
   const uint32_t n = 3 * c.d;
-  for (uint32_t i = threadIdx.x; i < n; i += kDerateThreads) {
-    const uint32_t r = rk[i];
-    const uint32_t m = (r >= n0) ? r - n0 : r + nv - n0;
-    const bool seen = (r != kNullRank) && m < e;
-    if (accumulate) {
-      if (seen) o[i] = o[i] + acc[m];
-    } else {
-      o[i] = seen ? acc[m] : 0.0f;
+  for (uint32_t base = 0; base < n; base += U * kDerateThreads) {
+    uint32_t r[U];
+    float prev[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t i = base + u * kDerateThreads + threadIdx.x;
+      r[u] = i < n ? rk[i] : kNullRank;
+      prev[u] = (accumulate && i < n) ? o[i] : 0.0f;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t i = base + u * kDerateThreads + threadIdx.x;
+      if (i >= n) continue;
+      const uint32_t m = (r[u] >= n0) ? r[u] - n0 : r[u] + nv - n0;
+      const bool seen = (r[u] != kNullRank) && m < e;
+      if (accumulate) {
+        if (seen) o[i] = prev[u] + acc[m];
+      } else {
+        o[i] = seen ? acc[m] : 0.0f;
+      }
     }
   }
 }

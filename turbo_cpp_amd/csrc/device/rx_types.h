@@ -48,6 +48,19 @@ constexpr int kTbThreads = 256;
 constexpr uint32_t kCrc24APoly = 0xC199BFu;  // lte_turbo.cpp:104-107, D^24 implicit
 constexpr uint32_t kCrc24BPoly = 0xC00063u;  // lte_turbo.cpp:108-111
 
+// Transmit side of a code block (GPU encoder and rate matcher).  Its bits
+// are filler zeros, then payload bits [cursor, cursor + payload) of the TB
+// with its CRC24A, then (seg) a CRC24B over those payload bits.
+struct TxCb {
+  uint64_t cb_off;   // first code-block bit in the code-block bit buffer (sum of K)
+  uint64_t d_off;    // first byte of [d0|d1|d2] (3 (K + 4) bytes)
+  uint64_t rm_off;   // first rate-matched bit (sum of E)
+  uint32_t tb, k, filler, cursor, payload, seg;
+  uint32_t f1, f2;
+  uint32_t e, n_valid, valid_off, rv;
+  uint32_t start[4];
+};
+
 constexpr uint16_t kNullRank = 0xFFFF;
 constexpr int kDerateThreads = 1024;
 
