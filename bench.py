@@ -1,20 +1,27 @@
 """Throughput benchmark: batched LTE turbo decoding on one MI355X.
 
-Workload (uniform batch): 4096 code blocks of K = 6144 (f1=263, f2=480),
-rate-1/3 mother code with 12 tail bits (18444 coded bits per block), info
-bits i.i.d. Bernoulli(0.5) from a seed, BPSK (bit c -> 2c-1) over AWGN with
-block i at Eb/N0 = 0.25 * (i mod 7) dB (the 0.0..1.5 dB sweep), fp32 LLRs
-2y/sigma^2, 8 iterations, no early stop.  One step = one batched decode of
-all 4096 blocks with the inputs resident in HBM.  Each block carries a
-transport block of 6120 payload bits + CRC24A (K = 6144, no filler).
-
-A second leg runs the receive chain on the same blocks as 4096 transport
-blocks (rv 0, E = 18444): soft de-rate matching, the batched decode and
-transport-block assembly (CRC24A), and reports its time and TB CRC pass rate.
+Headline workload (uniform batch): 4096 code blocks of K = 6144 (f1=263,
+f2=480), rate-1/3 mother code with 12 tail bits (18444 coded bits per block),
+each block a transport block of 6120 seeded Bernoulli(0.5) payload bits +
+CRC24A, BPSK (bit c -> 2c-1) over AWGN with block i at Eb/N0 = 0.25 * (i mod 7)
+dB (the 0.0..1.5 dB sweep), fp32 LLRs 2y/sigma^2, 8 iterations, no early stop.
+One step = one batched decode of all 4096 blocks with the inputs resident in
+HBM.  The window geometry (segment length, warm-up, waves per CU) is the
+autotuner's BER-gated choice for K = 6144.
 
 Printed: one JSON line (metric, value in info Gbit/s, roofline of the decode
 kernel, the reference CPU decoder timed on this host, the plain
-one-launch-per-block GPU decoder, BER/BLER per Eb/N0 for each decoder).
+one-launch-per-block GPU decoder, BER/BLER per Eb/N0 for each decoder) with
+further legs:
+  rx_chain         the same blocks as 4096 transport blocks through soft de-rate
+                   matching, the decode and TB assembly;
+  differentiation  2048 x K=1024, 6 iterations, 12 learnable extrinsic scales:
+                   forward / adjoint time (bwd Gbit/s), 500 Adam steps,
+                   held-out BER against the plain decoder;
+  uneven_batch     ~20k code blocks of transport blocks of uneven size,
+                   QPSK/16QAM/64QAM, rv 0..3, early stop, HIP graph;
+  mixed_load       65536 code blocks (70% K=6144), 16QAM over Rayleigh,
+                   HARQ rv0+rv2 combining, early stop, HIP graph.
 """
 from __future__ import annotations
 
@@ -249,6 +256,12 @@ def uneven_leg(T, torch, n_cb_target=20000, seed=31):
     ms, _ = _graph_time(torch, run, 3)
     okh = ok.cpu().numpy().astype(bool)
     payload_ber = float((bits != pay).float().mean())
+    rvs = np.array([t[2] for t in tbs])
+    qms = np.array([t[4] for t in tbs])
+    rates = np.array([t[0] / t[1] for t in tbs])
+    pass_by = {"rv": {str(r): float(okh[rvs == r].mean()) for r in range(4)},
+               "qm": {str(q): float(okh[qms == q].mean()) for q in (2, 4, 6)},
+               "rate<0.6": float(okh[rates < 0.6].mean()), "rate>=0.6": float(okh[rates >= 0.6].mean())}
     # plain: the first 64 blocks' decoder inputs (from a de-rate plan), one launch per block
     der = T.DeratePlan(plan.ks, plan.es, plan.fillers, [tbs[j][2] for j in range(len(tbs))
                                                         for _ in range(T._segment_sizes(tbs[j][0] + 24)[0])])
@@ -264,24 +277,27 @@ def uneven_leg(T, torch, n_cb_target=20000, seed=31):
             "transport_blocks": len(tbs), "code_blocks": plan.n_cb, "info_bits": info,
             "code_block_bits": int(plan.ks.sum()), "ms": ms, "info_gbit_s": info / ms / 1e6,
             "cb_gbit_s": float(plan.ks.sum()) / ms / 1e6, "mean_iterations_tb_max": float(its.float().mean()),
-            "tb_crc_pass": float(okh.mean()), "payload_ber": payload_ber, "graph": True,
+            "tb_crc_pass": float(okh.mean()), "tb_crc_pass_by": pass_by, "payload_ber": payload_ber,
+            "graph": True,
             "plain_gbit_s": plain, "batched_vs_plain": (float(plan.ks.sum()) / ms / 1e6) / plain}
 
 
 def mixed_leg(T, torch, n_cb=65536, seed=41):
     """Large mixed load: 65536 code blocks, 70% K=6144 and 30% uniform over
-    the QPP table (each a transport block of B = K - 24, no filler); 16QAM over
+    the QPP table, each K - 24 random bits + CRC24B (code_block_crc
+    descriptors, so early stop applies as to segmented blocks); 16QAM over
     i.i.d. Rayleigh fading with known CSI, Eb/N0 uniform in [2, 6] dB per
     block; two transmissions of E = 3(K+4)/2 (rounded to 16QAM symbols) with
     rv 0 and rv 2, soft-combined in fp32 before one decode; CRC24B early stop,
-    max 8 iterations.  Timed (one captured HIP graph): both de-rate
-    matchings, the decode and TB assembly."""
+    max 8 iterations.  Timed (one captured HIP graph): the decode with both
+    transmissions de-rate matched and combined in its input staging, and TB
+    assembly."""
     rng = np.random.default_rng(seed)
     table = [int(e[0]) for e in T.qpp_interleaver_table()]
     ks = np.where(rng.random(n_cb) < 0.7, 6144, rng.choice(table, n_cb)).astype(np.int64)
     es = (3 * (ks + 4) // 2) // 4 * 4
     ebn0 = rng.uniform(2.0, 6.0, n_cb)
-    tbs = [(int(k) - 24, int(e), 0, 1, 4) for k, e in zip(ks, es)]
+    tbs = [(int(k) - 24, int(e), 0, 1, 4, 1) for k, e in zip(ks, es)]  # K - 24 payload bits + CRC24B
     plan = T.TbPlan(tbs, T.DECODER_WINDOW_LOG, 8, early_stop=True)
     gen = torch.Generator(device="cuda").manual_seed(seed)
     pay = torch.randint(0, 2, (plan.bits_total,), dtype=torch.uint8, device="cuda", generator=gen)
@@ -300,20 +316,18 @@ def mixed_leg(T, torch, n_cb=65536, seed=41):
     ok = torch.empty(plan.n_tb, dtype=torch.int32, device="cuda")
     its = torch.empty(plan.n_tb, dtype=torch.int32, device="cuda")
 
-    def run():
-        s = torch.cuda.current_stream().cuda_stream
-        plan.soft_combine(rx[0].data_ptr(), rv=0, stream=s)
-        plan.soft_combine(rx[1].data_ptr(), rv=2, accumulate=True, stream=s)
-        plan.decode(0, bits.data_ptr(), ok.data_ptr(), 0, its.data_ptr(), stream=s)
+    def run():  # both transmissions de-rate matched and combined inside the decoder's input staging
+        plan.decode_multi([rx[0].data_ptr(), rx[1].data_ptr()], [0, 2], bits.data_ptr(), ok.data_ptr(), 0,
+                          its.data_ptr(), stream=torch.cuda.current_stream().cuda_stream)
 
     ms, _ = _graph_time(torch, run, 2)
     okh = ok.cpu().numpy().astype(bool)
     mean_it = float(its.float().mean())
     kb = int(ks.sum())
     gbit = kb / ms / 1e6
-    # algorithmic HBM bytes: two received LLR sets, soft buffer written twice and read
-    # once (first combine overwrites, second reads and writes), posteriors, payload bits
-    hbm = 2 * 4 * int(es.sum()) + 4 * 3 * int((3 * (ks + 4)).sum()) + 4 * kb + plan.bits_total
+    # algorithmic HBM bytes: two received LLR sets read, the combined decoder input written
+    # once (the plan's HARQ soft buffer), posteriors written and read, payload bits
+    hbm = 2 * 4 * int(es.sum()) + 4 * int((3 * (ks + 4)).sum()) + 2 * 4 * kb + plan.bits_total
     valu = kb * mean_it * 864 / (ms / 1e3)
     # plain path on 64 blocks: one launch per block on the combined decoder input
     der = T.DeratePlan(plan.ks[:64], plan.es[:64])
@@ -412,6 +426,7 @@ def main():
     ap.add_argument("--no-rx-chain", action="store_true")
     ap.add_argument("--train-steps", type=int, default=500, help="Adam steps of the differentiation leg (0: skip it)")
     ap.add_argument("--no-scenarios", action="store_true", help="skip the uneven-batch and mixed-load legs")
+    ap.add_argument("--no-autotune", action="store_true", help="default geometry instead of the tuned one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -425,14 +440,22 @@ def main():
 
     dec = T.DECODER_WINDOW_LOG if args.decoder == "log" else T.DECODER_WINDOW_LIN
     other = T.DECODER_WINDOW_LIN if dec == T.DECODER_WINDOW_LOG else T.DECODER_WINDOW_LOG
+    tuned = None
+    tuned_opts = {}
+    if not args.no_autotune:
+        from turbo_cpp_amd import autotune
+        log("autotuning segment length / warm-up / waves per CU")
+        tuned = autotune.tune(K, dec, ITERS, n_blocks=N_CB, ebn0_db=0.5)
+        tuned_opts = autotune.options_for(tuned)
+        log("autotune choice:", tuned["best"])
     log("generating", N_CB, "blocks")
     bits, llr, ebn0 = make_inputs(N_CB)
     ks = [K] * N_CB
     x = torch.from_numpy(llr.reshape(-1)).cuda()
     y = torch.empty(N_CB * K, dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
-    plan = T.Plan(ks, dec, ITERS)
-    geo = T.segment_geometry(K, dec)
+    plan = T.Plan(ks, dec, ITERS, **tuned_opts)
+    geo = T.segment_geometry(K, dec, tuned_opts.get("min_segment", 0), tuned_opts.get("warmup", 0))
 
     def step(p=plan, out=y):
         p.decode(x.data_ptr(), out.data_ptr(), 0, 0, stream.cuda_stream)
@@ -487,7 +510,7 @@ def main():
 
     # plain: same window decoder, one launch (and one plan) per code block
     nplain = args.plain_blocks
-    p1 = T.Plan([K], dec, ITERS)
+    p1 = T.Plan([K], dec, ITERS, **tuned_opts)
     for i in range(2):
         p1.decode(x.data_ptr(), y.data_ptr(), 0, 0, stream.cuda_stream)
     torch.cuda.synchronize()
@@ -541,7 +564,7 @@ def main():
                                "Eb/N0 0.0..1.5 dB (0.25 dB steps), fp32 LLRs, 8 iterations, no early stop",
                    "code_blocks": N_CB, "k": K, "iterations": ITERS, "decoder": f"window-{args.decoder}",
                    "segment_length": geo["segment_length"], "segments_per_block": geo["segments"],
-                   "warmup_steps": geo["warmup"]},
+                   "warmup_steps": geo["warmup"], "waves_per_cu": tuned_opts.get("waves_per_cu", 0)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain": {"gbit_s": plain_gbits, "ms_per_block": plain_s * 1e3, "blocks_timed": nplain,
@@ -554,6 +577,11 @@ def main():
         "rx_chain": rx_chain,
         "bwd_gbit_s": diff["bwd_gbit_s"] if diff else None,
         "differentiation": diff,
+        "autotune": None if tuned is None else {
+            "choice": tuned["best"], "exact_bit_errors": tuned["exact_bit_errors"], "interval": tuned["interval"],
+            "calibration": f"{tuned['n_blocks']} blocks K={K}, QPSK/AWGN {tuned['ebn0_db']} dB",
+            "table": [{k: c[k] for k in ("segment_length", "warmup", "waves_per_cu", "ms", "bit_errors", "qualifies")}
+                      for c in tuned["table"]]},
         "uneven_batch": uneven,
         "mixed_load": mixed,
     }

@@ -54,6 +54,7 @@ typedef struct tc_options {
   int32_t decoder;         /* TC_DECODER_* */
   uint32_t min_segment;    /* window decoders: lower bound on the segment length, 0 = 32 */
   uint32_t warmup;         /* window decoders: warm-up steps per boundary, 0 = 32 */
+  uint32_t waves_per_cu;   /* window decoders: resident waves per CU, 0 = as many as fit */
 } tc_options;
 
 /* Geometry the window decoders chose for one K (for tests and reports). */
@@ -130,6 +131,21 @@ tc_status tc_derate_plan_run(tc_derate_plan* plan, const float* rx_dev, int32_t 
 tc_status tc_derate_match_soft(const float* rx, uint64_t e, uint32_t k, uint32_t filler, uint32_t rv,
                                int32_t accumulate, float* out);
 
+/* Decode straight from received LLRs: the window decoders de-rate match
+ * (tc_derate_plan semantics) inside their input staging, so no decoder input
+ * is written before the decode.  `derate` must cover the same blocks (K,
+ * filler) as `plan`.  The decoder input is, per position,
+ *   (accumulate ? keep : 0) + de-rate(rx_dev[0], rv[0]) + ... + de-rate(rx_dev[n_tx-1], rv[n_tx-1])
+ * (HARQ combining of up to 4 transmissions; rv NULL or -1 = the blocks' own),
+ * bit-identical to running tc_derate_plan_run per transmission into keep and
+ * decoding keep.  keep (may be NULL unless accumulate) receives that input.
+ * The exact decoder has no fused staging: it de-rate matches into keep
+ * (required) and decodes it. */
+tc_status tc_plan_decode_rx(tc_plan* plan, const tc_derate_plan* derate, uint32_t n_tx,
+                            const float* const* rx_dev, const int32_t* rv, int32_t accumulate,
+                            float* keep_dev, float* post_dev, int32_t* crc_ok_dev,
+                            uint32_t* iters_dev, void* hip_stream);
+
 /* ---- batched transport-block decoding (GPU) --------------------------------
  * The reference's decode_transport_block (lte_turbo.cpp:891-980) for many
  * transport blocks at once, from received LLRs: soft de-rate matching (with
@@ -153,6 +169,11 @@ typedef struct {
   uint32_t rv;
   uint32_t num_layers;
   uint32_t mod_order;
+  /* 1: no CRC24A and no segmentation -- one code block of K = B + 24 bits
+   * whose last 24 bits are its CRC24B, the layout of the reference's
+   * segmented code blocks (early stop applies); for code-block-level batches.
+   * tb_crc_ok then reports that CRC24B. */
+  uint32_t code_block_crc;
 } tc_tb_desc;
 typedef struct tc_tb_plan tc_tb_plan;
 tc_status tc_tb_plan_create(const tc_tb_desc* tbs, size_t n_tb, const tc_options* opts,
@@ -180,6 +201,13 @@ tc_status tc_encode_transport_blocks_host(const tc_tb_desc* tbs, size_t n_tb, co
  * input without decoding (HARQ: combine rv 0 and rv 2, then decode with
  * rx NULL). */
 tc_status tc_tb_plan_soft_combine(tc_tb_plan* plan, const float* rx_dev, int32_t rv, int32_t accumulate,
+                                  void* hip_stream);
+/* HARQ in one call: the held decoder input becomes the combination of n_tx
+ * (1..4) transmissions (rx_dev[t] with rv[t], rv NULL = the descriptors'),
+ * then decode and assembly as tc_tb_plan_decode. */
+tc_status tc_tb_plan_decode_multi(tc_tb_plan* plan, uint32_t n_tx, const float* const* rx_dev,
+                                  const int32_t* rv, uint8_t* bits_dev, int32_t* tb_crc_ok_dev,
+                                  int32_t* cb_crc_ok_dev, uint32_t* iterations_dev, float* posterior_dev,
                                   void* hip_stream);
 /* Same from host buffers (one transmission, no HARQ). */
 tc_status tc_decode_transport_blocks_host(const tc_tb_desc* tbs, size_t n_tb, const tc_options* opts,

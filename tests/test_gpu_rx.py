@@ -407,3 +407,112 @@ def test_tb_pipeline_graph_capture_replays_match_eager():
         torch.cuda.synchronize()
         for a, b in zip(eager, other):
             assert torch.equal(a, b), rep
+
+
+def test_code_block_crc_mode():
+    """code_block_crc descriptors: one code block of K = B + 24 ending in a
+    CRC24B.  Encoding equals the host chain (append_crc24b, encoder, rate
+    matching); decoding stops early on that CRC24B and reports it as the
+    TB check."""
+    import torch
+    rng = np.random.default_rng(17)
+    ks = [6144, 1024, 40, 3200, 6144]
+    tbs = [(k - 24, int(3 * (k + 4) * 0.8) // 2 * 2, i % 4, 1, 2, 1) for i, k in enumerate(ks)]
+    pays = [rng.integers(0, 2, k - 24).astype(np.uint8) for k in ks]
+    rm, cb = T.encode_transport_blocks(tbs, pays)
+    roff = coff = 0
+    for (b, g, rv, _, _, _), pay, k in zip(tbs, pays, ks):
+        bits = T.append_crc24b(pay)
+        d = T.encode_code_block_streams(bits)
+        ref = O.rate_match(*d, 0, g, rv)
+        assert np.array_equal(cb[coff:coff + k], bits)
+        assert np.array_equal(rm[roff:roff + g], ref)
+        roff += g
+        coff += k
+    plan = T.TbPlan(tbs, max_iterations=8, early_stop=True)
+    rx = torch.from_numpy((2.0 * rm.astype(np.float32) - 1.0) * 3.0).cuda()
+    bits = torch.empty(plan.bits_total, dtype=torch.uint8, device="cuda")
+    ok = torch.zeros(plan.n_tb, dtype=torch.int32, device="cuda")
+    it = torch.zeros(plan.n_tb, dtype=torch.int32, device="cuda")
+    _say("code-block CRC mode")
+    plan.decode(rx.data_ptr(), bits.data_ptr(), ok.data_ptr(), 0, it.data_ptr())
+    torch.cuda.synchronize()
+    assert bool(ok.all()) and np.array_equal(bits.cpu().numpy(), np.concatenate(pays))
+    assert int(it.max()) < 8  # early stop fired
+
+
+@pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN, T.DECODER_EXACT])
+def test_fused_derate_decode_equals_separate(decoder):
+    """De-rate matching inside the decoder's input staging (tc_plan_decode_rx)
+    is bit-identical to the de-rate kernel followed by a decode: one
+    transmission, HARQ of two (rv 0 + rv 2) in one call, and accumulation
+    onto a held input; the held input written by the fused path equals the
+    de-rate kernel's output.  Mixed K, fillers, repetition, early stop."""
+    import torch
+    ks, fs, es, _, rng = _mixed_batch(23)
+    n = len(ks)
+    its = 3 if decoder == T.DECODER_EXACT else 6
+    rx0 = torch.from_numpy(rng.normal(0, 2.5, sum(es)).astype(np.float32)).cuda()
+    rx2 = torch.from_numpy(rng.normal(0, 2.5, sum(es)).astype(np.float32)).cuda()
+    der = T.DeratePlan(ks, es, fs)
+    plan = T.Plan(ks, decoder, its, early_stop=True, fillers=fs)
+    s = torch.cuda.current_stream().cuda_stream
+
+    def z(m, dt=torch.float32):
+        return torch.zeros(m, dtype=dt, device="cuda")
+
+    # separate: de-rate kernel, then decode
+    soft = z(der.llr_floats)
+    der.run(rx0.data_ptr(), soft.data_ptr(), rv=0, stream=s)
+    soft1 = soft.clone()
+    der.run(rx2.data_ptr(), soft.data_ptr(), rv=2, accumulate=True, stream=s)
+    post_a, crc_a, it_a = z(plan.post_floats), z(n, torch.int32), z(n, torch.int32)
+    plan.decode(soft1.data_ptr(), post_a.data_ptr(), crc_a.data_ptr(), it_a.data_ptr(), s)
+    post_b = z(plan.post_floats)
+    plan.decode(soft.data_ptr(), post_b.data_ptr(), 0, 0, s)
+    _say("fused decode", decoder)
+    # fused, one transmission
+    keep, post_c, crc_c, it_c = z(der.llr_floats), z(plan.post_floats), z(n, torch.int32), z(n, torch.int32)
+    plan.decode_rx(der, [rx0.data_ptr()], post_c.data_ptr(), rvs=[0], keep_ptr=keep.data_ptr(),
+                   crc_ptr=crc_c.data_ptr(), iters_ptr=it_c.data_ptr(), stream=s)
+    torch.cuda.synchronize()
+    assert torch.equal(post_c, post_a) and torch.equal(crc_c, crc_a) and torch.equal(it_c, it_a)
+    assert torch.equal(keep, soft1)
+    # fused, HARQ of two transmissions in one call
+    keep2, post_d = z(der.llr_floats), z(plan.post_floats)
+    plan.decode_rx(der, [rx0.data_ptr(), rx2.data_ptr()], post_d.data_ptr(), rvs=[0, 2], keep_ptr=keep2.data_ptr(),
+                   stream=s)
+    torch.cuda.synchronize()
+    assert torch.equal(post_d, post_b) and torch.equal(keep2, soft)
+    # fused, accumulating onto a held input
+    keep3, post_e = soft1.clone(), z(plan.post_floats)
+    plan.decode_rx(der, [rx2.data_ptr()], post_e.data_ptr(), rvs=[2], accumulate=True, keep_ptr=keep3.data_ptr(),
+                   stream=s)
+    torch.cuda.synchronize()
+    assert torch.equal(post_e, post_b) and torch.equal(keep3, soft)
+
+
+def test_tb_decode_multi_equals_combine_then_decode():
+    import torch
+    tbs, rx, _ = _tb_batch(41, n_tb=10)
+    plan = T.TbPlan(tbs, max_iterations=6, early_stop=True)
+    rng = np.random.default_rng(5)
+    r0 = torch.from_numpy(rx).cuda()
+    r2 = torch.from_numpy(rx[rng.permutation(rx.size)]).cuda()
+    s = torch.cuda.current_stream().cuda_stream
+    outs = []
+    for mode in ("combine", "multi"):
+        bits = torch.zeros(plan.bits_total, dtype=torch.uint8, device="cuda")
+        post = torch.zeros(plan.post_floats, dtype=torch.float32, device="cuda")
+        ok = torch.zeros(plan.n_tb, dtype=torch.int32, device="cuda")
+        if mode == "combine":
+            plan.soft_combine(r0.data_ptr(), rv=0, stream=s)
+            plan.soft_combine(r2.data_ptr(), rv=2, accumulate=True, stream=s)
+            plan.decode(0, bits.data_ptr(), ok.data_ptr(), post_ptr=post.data_ptr(), stream=s)
+        else:
+            plan.decode_multi([r0.data_ptr(), r2.data_ptr()], [0, 2], bits.data_ptr(), ok.data_ptr(),
+                              post_ptr=post.data_ptr(), stream=s)
+        torch.cuda.synchronize()
+        outs.append((bits, post, ok))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)

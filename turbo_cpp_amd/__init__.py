@@ -35,12 +35,12 @@ class TcCbDesc(C.Structure):
 
 class TcOptions(C.Structure):
     _fields_ = [("max_iterations", C.c_uint32), ("early_stop", C.c_int32), ("decoder", C.c_int32),
-                ("min_segment", C.c_uint32), ("warmup", C.c_uint32)]
+                ("min_segment", C.c_uint32), ("warmup", C.c_uint32), ("waves_per_cu", C.c_uint32)]
 
 
 class TcTbDesc(C.Structure):
     _fields_ = [("transport_block_bits", C.c_uint64), ("g_total", C.c_uint64), ("rv", C.c_uint32),
-                ("num_layers", C.c_uint32), ("mod_order", C.c_uint32)]
+                ("num_layers", C.c_uint32), ("mod_order", C.c_uint32), ("code_block_crc", C.c_uint32)]
 
 
 class TcChannelSeg(C.Structure):
@@ -111,6 +111,9 @@ def lib() -> C.CDLL:
         L.tc_tb_plan_sizes.argtypes = [C.c_void_p, _u64p]
         L.tc_tb_plan_code_blocks.argtypes = [C.c_void_p, _u32p, _u32p, _u64p]
         L.tc_tb_plan_decode.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32] + [C.c_void_p] * 6
+        L.tc_plan_decode_rx.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int32,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.tc_tb_plan_decode_multi.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p] + [C.c_void_p] * 6
         L.tc_tb_plan_soft_combine.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
         L.tc_tb_plan_encode.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.tc_encode_transport_blocks_host.argtypes = [C.POINTER(TcTbDesc), C.c_size_t, _u8p, _u8p, C.c_void_p]
@@ -394,8 +397,8 @@ def turbo_decode_code_block_cpu(descriptor: CodeBlockDescriptor, d0, d1, d2, max
 
 
 def options(decoder: int = DECODER_WINDOW_LOG, max_iterations: int = 8, early_stop: bool = False,
-            min_segment: int = 0, warmup: int = 0) -> TcOptions:
-    return TcOptions(max_iterations, int(early_stop), decoder, min_segment, warmup)
+            min_segment: int = 0, warmup: int = 0, waves_per_cu: int = 0) -> TcOptions:
+    return TcOptions(max_iterations, int(early_stop), decoder, min_segment, warmup, waves_per_cu)
 
 
 def _gpu_decode(decoder, descriptor, d0, d1, d2, max_iterations, enable_early_stop, **kw):
@@ -461,10 +464,11 @@ class Plan:
     ``torch.cuda.current_stream().cuda_stream``)."""
 
     def __init__(self, ks: Sequence[int], decoder: int = DECODER_WINDOW_LOG, max_iterations: int = 8,
-                 early_stop: bool = False, fillers=None, min_segment: int = 0, warmup: int = 0):
+                 early_stop: bool = False, fillers=None, min_segment: int = 0, warmup: int = 0,
+                 waves_per_cu: int = 0):
         self.ks = np.ascontiguousarray(ks, dtype=np.uint32)
         self._fil = None if fillers is None else np.ascontiguousarray(fillers, dtype=np.uint32)
-        o = options(decoder, max_iterations, early_stop, min_segment, warmup)
+        o = options(decoder, max_iterations, early_stop, min_segment, warmup, waves_per_cu)
         h = C.c_void_p()
         _check(lib().tc_plan_create(self.ks, None if self._fil is None else self._fil.ctypes.data, len(self.ks),
                                     C.byref(o), C.byref(h)))
@@ -475,6 +479,16 @@ class Plan:
     def decode(self, llr_ptr: int, post_ptr: int, crc_ptr: int = 0, iters_ptr: int = 0, stream: int = 0):
         _check(lib().tc_plan_decode(self._h, llr_ptr, post_ptr, crc_ptr or None, iters_ptr or None,
                                     stream or None))
+
+    def decode_rx(self, derate: "DeratePlan", rx_ptrs, post_ptr: int, rvs=None, accumulate: bool = False,
+                  keep_ptr: int = 0, crc_ptr: int = 0, iters_ptr: int = 0, stream: int = 0):
+        """Decode from received LLRs of 1..4 transmissions (device pointers),
+        de-rate matched inside the decoder's input staging (tc_plan_decode_rx)."""
+        n = len(rx_ptrs)
+        ptrs = (C.c_void_p * max(n, 1))(*rx_ptrs)
+        rv = None if rvs is None else (C.c_int32 * n)(*rvs)
+        _check(lib().tc_plan_decode_rx(self._h, derate._h, n, ptrs, rv, int(accumulate), keep_ptr or None,
+                                       post_ptr or None, crc_ptr or None, iters_ptr or None, stream or None))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -590,7 +604,8 @@ class TbPlan:
     """Batched transport-block decoding on the GPU (tc_tb_plan_*).
 
     ``tbs``: LteTurboCodecParams or tuples (B, g_total, rv, num_layers,
-    mod_order).  ``decode`` takes device pointers; outputs may be 0."""
+    mod_order[, code_block_crc]).  ``decode`` takes device pointers; outputs
+    may be 0."""
 
     def __init__(self, tbs, decoder: int = DECODER_WINDOW_LOG, max_iterations: int = 8,
                  early_stop: bool = False, min_segment: int = 0, warmup: int = 0):
@@ -614,6 +629,16 @@ class TbPlan:
         _check(lib().tc_tb_plan_decode(self._h, rx_ptr or None, rv, int(accumulate), bits_ptr or None,
                                        tb_crc_ptr or None, cb_crc_ptr or None, iters_ptr or None,
                                        post_ptr or None, stream or None))
+
+    def decode_multi(self, rx_ptrs, rvs=None, bits_ptr: int = 0, tb_crc_ptr: int = 0, cb_crc_ptr: int = 0,
+                     iters_ptr: int = 0, post_ptr: int = 0, stream: int = 0):
+        """HARQ in one call: combine 1..4 transmissions (rv per transmission) and decode."""
+        n = len(rx_ptrs)
+        ptrs = (C.c_void_p * max(n, 1))(*rx_ptrs)
+        rv = None if rvs is None else (C.c_int32 * n)(*rvs)
+        _check(lib().tc_tb_plan_decode_multi(self._h, n, ptrs, rv, bits_ptr or None, tb_crc_ptr or None,
+                                             cb_crc_ptr or None, iters_ptr or None, post_ptr or None,
+                                             stream or None))
 
     def soft_combine(self, rx_ptr: int, rv: int = -1, accumulate: bool = False, stream: int = 0):
         """De-rate match one transmission into the held decoder input (no decode)."""

@@ -1,0 +1,132 @@
+"""Autotuner of the window decoders' launch geometry.
+
+Knobs (tc_options): the segment length S (via min_segment; S is the smallest
+divisor of K >= max(min_segment, K/64), which also fixes the segments per
+block M = K/S and the blocks per 64-lane wave 64/M), the warm-up steps per
+segment boundary, and the resident waves per CU (occupancy).
+
+For one K the tuner makes a calibration batch on the GPU (random payloads of
+K - 24 bits with CRC24A, the GPU encoder, full circular buffer at rv 0, QPSK
+over AWGN at the given Eb/N0, max-log LLRs, soft de-rate matching), decodes it
+with the exact full-length decoder (the reference's algorithm) and with every
+candidate, and times each candidate's decode with HIP events.  A candidate
+qualifies when its bit-error count lies inside the 95% Clopper-Pearson
+interval of the exact decoder's count on the same blocks; the fastest
+qualifying candidate is returned with the whole table.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+from dataclasses import asdict, dataclass
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import (DECODER_EXACT, DECODER_WINDOW_LOG, ChannelPlan, DeratePlan, Plan, TbPlan, segment_geometry)
+
+
+@dataclass
+class Candidate:
+    min_segment: int
+    warmup: int
+    waves_per_cu: int
+    segment_length: int = 0
+    segments: int = 0
+    blocks_per_wave: int = 0
+    ms: float = float("nan")
+    bit_errors: int = -1
+    qualifies: bool = False
+
+
+def _beta_ppf(q, a, b):
+    from scipy.stats import beta
+    return float(beta.ppf(q, a, b))
+
+
+def clopper_pearson(k: int, n: int, alpha: float = 0.05):
+    lo = 0.0 if k == 0 else _beta_ppf(alpha / 2, k, n - k + 1)
+    hi = 1.0 if k == n else _beta_ppf(1 - alpha / 2, k + 1, n - k)
+    return lo, hi
+
+
+def calibration_batch(k: int, n_blocks: int, ebn0_db: float, seed: int = 1):
+    """Decoder inputs ([d0|d1|d2] per block, device) and the code-block bits (device)."""
+    import torch
+    plan = TbPlan([(k - 24, 3 * (k + 4), 0, 1, 2)] * n_blocks, max_iterations=1)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    pay = torch.randint(0, 2, (plan.bits_total,), dtype=torch.uint8, device="cuda", generator=gen)
+    rm = torch.empty(plan.rx_floats, dtype=torch.uint8, device="cuda")
+    cb = torch.empty(plan.post_floats, dtype=torch.uint8, device="cuda")
+    plan.encode(pay.data_ptr(), rm.data_ptr(), cb.data_ptr())
+    rate = k / (3.0 * (k + 4))
+    n0 = 1.0 / (2.0 * rate * 10.0 ** (ebn0_db / 10.0))  # QPSK: Es = 1 carries 2 coded bits
+    ch = ChannelPlan([(int(e), 2, 0, n0) for e in plan.es])
+    rx = torch.empty(plan.rx_floats, dtype=torch.float32, device="cuda")
+    ch.run(seed, rm.data_ptr(), rx.data_ptr())
+    der = DeratePlan(plan.ks, plan.es)
+    llr = torch.empty(der.llr_floats, dtype=torch.float32, device="cuda")
+    der.run(rx.data_ptr(), llr.data_ptr())
+    torch.cuda.synchronize()
+    return llr, cb
+
+
+def _decode(plan: Plan, llr, post, reps: int):
+    import torch
+    s = torch.cuda.current_stream()
+    plan.decode(llr.data_ptr(), post.data_ptr(), 0, 0, s.cuda_stream)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(reps):
+        plan.decode(llr.data_ptr(), post.data_ptr(), 0, 0, s.cuda_stream)
+    b.record(s)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def tune(k: int, decoder: int = DECODER_WINDOW_LOG, iterations: int = 8, n_blocks: int = 2048,
+         ebn0_db: float = 0.5, min_segments: Sequence[int] = (32, 64, 96, 128, 192),
+         warmups: Sequence[int] = (16, 24, 32), waves: Sequence[int] = (0, 4, 2), reps: int = 3,
+         seed: int = 1) -> Dict:
+    """Returns {"best": {...}, "table": [...], "exact_bit_errors": n, "interval": [lo, hi], ...}."""
+    import torch
+    llr, cb = calibration_batch(k, n_blocks, ebn0_db, seed)
+    bits = cb.to(torch.bool)
+    post = torch.empty(n_blocks * k, dtype=torch.float32, device="cuda")
+    exact = Plan([k] * n_blocks, DECODER_EXACT, iterations)
+    exact.decode(llr.data_ptr(), post.data_ptr(), 0, 0, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    ref_err = int(((post >= 0) != bits).sum())
+    nbits = n_blocks * k
+    lo, hi = clopper_pearson(ref_err, nbits)
+    seen = set()
+    table: List[Candidate] = []
+    for ms, g, w in itertools.product(min_segments, warmups, waves):
+        try:
+            geo = segment_geometry(k, decoder, ms, g)
+        except ValueError:
+            continue  # segment length outside what the kernel supports
+        key = (geo["segment_length"], geo["warmup"], w)
+        if key in seen:
+            continue
+        seen.add(key)
+        c = Candidate(ms, g, w, geo["segment_length"], geo["segments"], geo["blocks_per_wave"])
+        plan = Plan([k] * n_blocks, decoder, iterations, min_segment=ms, warmup=g, waves_per_cu=w)
+        c.ms = _decode(plan, llr, post, reps)
+        c.bit_errors = int(((post >= 0) != bits).sum())
+        c.qualifies = lo * nbits <= c.bit_errors <= hi * nbits
+        table.append(c)
+        del plan
+    ok = [c for c in table if c.qualifies]
+    best = min(ok or table, key=lambda c: c.ms)
+    return {"k": k, "decoder": decoder, "iterations": iterations, "n_blocks": n_blocks, "ebn0_db": ebn0_db,
+            "exact_bit_errors": ref_err, "bits": nbits,
+            "interval": [math.floor(lo * nbits), math.ceil(hi * nbits)],
+            "best": asdict(best), "table": [asdict(c) for c in table]}
+
+
+def options_for(result: Dict) -> Dict[str, int]:
+    """Keyword arguments for Plan / options() from a tune() result."""
+    b = result["best"]
+    return {"min_segment": b["min_segment"], "warmup": b["warmup"], "waves_per_cu": b["waves_per_cu"]}

@@ -15,6 +15,7 @@
 
 #include "../host/codec_internal.h"
 #include "decoder_types.h"
+#include "derate_plan.h"
 #include "hip_util.h"
 #include "turbo_cpp/c_api.h"
 
@@ -177,6 +178,7 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, tcdev::kWaveLanes, p.lds_bytes),
             "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   per_cu = std::max(per_cu, 1);
+  if (o.waves_per_cu) per_cu = std::min<int>(per_cu, static_cast<int>(o.waves_per_cu));
   p.grid = std::max<uint32_t>(1, std::min<uint32_t>(p.njobs, static_cast<uint32_t>(per_cu) * prop.multiProcessorCount));
   p.d_jobs = dev_alloc<tcdev::WinJob>(jobs.size());
   dev_upload(p.d_jobs, jobs);
@@ -196,7 +198,8 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   dev_upload(p.d_crc_pow, pw);
 }
 
-void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* iters, hipStream_t s) {
+void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* iters, hipStream_t s,
+            const tcdev::RxIn* rx = nullptr) {
   if (p.n_cb == 0) return;
   const tc_options& o = p.opts;
   if (o.decoder == TC_DECODER_EXACT) {
@@ -206,7 +209,8 @@ void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* i
                        o.early_stop);
   } else {
     tcdev::WinArgs a{p.d_jobs, p.njobs, p.d_cb_index, p.d_cbs, p.d_qtab, p.d_crc_pow, llr, post, p.d_scratch,
-                     crc, iters, o.max_iterations, o.early_stop};
+                     crc, iters, o.max_iterations, o.early_stop, tcdev::RxIn{}};
+    if (rx) a.rx = *rx;
     hipLaunchKernelGGL(window_kernel(o), dim3(p.grid), dim3(64), p.lds_bytes, s, a);
   }
   hip_check(hipGetLastError(), "kernel launch");
@@ -214,10 +218,16 @@ void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* i
 
 void check_early_stop_support(const tc_options&) {}
 
+void check_status(tc_status st) {
+  if (st != TC_OK) throw Failure{st, turbo_cpp::detail::last_error()};
+}
+
 // Per-thread cache of single-block plans and staging buffers, so repeated
 // per-code-block calls do not allocate.
 struct OneBlockCache {
-  std::map<std::tuple<uint32_t, uint32_t, int32_t, uint32_t, uint32_t, uint32_t, int32_t>, std::unique_ptr<tc_plan>> plans;
+  std::map<std::tuple<uint32_t, uint32_t, int32_t, uint32_t, uint32_t, uint32_t, int32_t, uint32_t>,
+           std::unique_ptr<tc_plan>>
+      plans;
   float* d_llr = nullptr;
   float* d_post = nullptr;
   int32_t* d_crc = nullptr;
@@ -270,6 +280,46 @@ tc_status tc_plan_decode(tc_plan* plan, const float* llr_dev, float* post_dev, i
   });
 }
 
+tc_status tc_plan_decode_rx(tc_plan* plan, const tc_derate_plan* derate, uint32_t n_tx, const float* const* rx_dev,
+                            const int32_t* rv, int32_t accumulate, float* keep_dev, float* post_dev,
+                            int32_t* crc_ok_dev, uint32_t* iters_dev, void* hip_stream) {
+  return guarded([&] {
+    if (!plan || !derate) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    if (n_tx == 0 || n_tx > static_cast<uint32_t>(tcdev::kMaxTx))
+      throw Failure{TC_ERR_INVALID_ARGUMENT, "n_tx must be 1..4"};
+    if (plan->n_cb != derate->n_cb) throw Failure{TC_ERR_INVALID_ARGUMENT, "plans cover different code blocks"};
+    for (size_t i = 0; i < plan->n_cb; ++i)
+      if (plan->cbs[i].k != derate->ks[i] || plan->cbs[i].filler != derate->fs[i])
+        throw Failure{TC_ERR_INVALID_ARGUMENT, "plans cover different code blocks"};
+    if (plan->n_cb == 0) return;
+    if (!rx_dev || !post_dev || (accumulate && !keep_dev)) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    tcdev::RxIn x{};
+    for (uint32_t t = 0; t < static_cast<uint32_t>(tcdev::kMaxTx); ++t) {
+      const int32_t r = (rv && t < n_tx) ? rv[t] : -1;
+      if (r < -1 || r > 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "rv must be -1 (per block) or 0..3"};
+      x.rv[t] = r < 0 ? tcdev::kBlockRv : static_cast<uint32_t>(r);
+      x.rx[t] = t < n_tx ? rx_dev[t] : nullptr;
+      if (t < n_tx && !x.rx[t]) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    }
+    x.n_tx = n_tx;
+    x.base = accumulate ? 1 : 0;
+    x.keep = keep_dev;
+    x.rank = derate->d_rank;
+    x.dcb = derate->d_cbs;
+    const auto s = static_cast<hipStream_t>(hip_stream);
+    if (plan->opts.decoder == TC_DECODER_EXACT) {
+      // no fused staging in the exact decoder: de-rate into keep, then decode it
+      if (!keep_dev) throw Failure{TC_ERR_INVALID_ARGUMENT, "the exact decoder needs keep"};
+      for (uint32_t t = 0; t < n_tx; ++t)
+        check_status(tc_derate_plan_run(const_cast<tc_derate_plan*>(derate), rx_dev[t], rv ? rv[t] : -1,
+                                        (t > 0 || accumulate) ? 1 : 0, keep_dev, hip_stream));
+      launch(*plan, keep_dev, post_dev, crc_ok_dev, iters_dev, s);
+      return;
+    }
+    launch(*plan, nullptr, post_dev, crc_ok_dev, iters_dev, s, &x);
+  });
+}
+
 tc_status tc_decode_batch_host(const uint32_t* k, const uint32_t* filler, size_t n_cb, const tc_options* opts,
                                const float* llr, float* post, int32_t* crc_ok, uint32_t* iterations_used) {
   return guarded([&] {
@@ -315,7 +365,8 @@ tc_status tc_decode_code_block(const tc_cb_desc* desc, const float* d0, size_t n
     const uint32_t f = static_cast<uint32_t>(desc->filler_count);
     int dev = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
-    auto key = std::make_tuple(k, f, o.decoder, o.max_iterations, o.min_segment, o.warmup * 2 + (o.early_stop != 0), dev);
+    auto key = std::make_tuple(k, f, o.decoder, o.max_iterations, o.min_segment, o.warmup * 2 + (o.early_stop != 0), dev,
+                               o.waves_per_cu);
     auto& slot = g_one.plans[key];
     if (!slot) {
       slot = std::make_unique<tc_plan>();

@@ -3,6 +3,8 @@
 
 #include <cstdint>
 
+#include "rx_types.h"
+
 namespace tcdev {
 
 // One code block of a batch.  llr_off: start of [d0|d1|d2] (3*(K+4) floats);
@@ -44,6 +46,25 @@ struct WinJob {
   uint32_t pad0, pad1;
 };
 
+// Received LLRs de-rate matched inside the window decoder's input staging
+// (n_tx > 0).  The decoder input of block c at stream s, index i is
+//   (base ? keep[c][s][i] : 0) + sum over transmissions t of the soft de-rate
+//   value of rx[t] with redundancy version rv[t] (kBlockRv: the block's own),
+// in that order, i.e. what tc_derate_plan_run would leave in keep after one
+// run per transmission (the first without accumulation unless base).  keep,
+// if set, receives the result in the decoder input layout.
+constexpr int kMaxTx = 4;
+constexpr uint32_t kBlockRv = 0xFFFFFFFFu;
+struct RxIn {
+  const float* rx[kMaxTx];
+  uint32_t rv[kMaxTx];
+  uint32_t n_tx;
+  int32_t base;
+  float* keep;
+  const uint16_t* rank;
+  const DerateCb* dcb;
+};
+
 // Arguments of the window kernels.
 struct WinArgs {
   const WinJob* jobs;
@@ -59,6 +80,7 @@ struct WinArgs {
   uint32_t* iters;
   uint32_t max_iterations;
   int32_t early_stop;
+  RxIn rx;  // n_tx == 0: the decoder input is llr
 };
 constexpr int kCrcPowLen = 6144;
 // LDS floats of one window job beyond the extrinsics: hard-decision bitmap

@@ -14,6 +14,7 @@
 
 #include "../host/codec_internal.h"
 #include "hip_util.h"
+#include "derate_plan.h"
 #include "rx_types.h"
 #include "turbo_cpp/c_api.h"
 
@@ -29,17 +30,6 @@ __global__ void rate_match_kernel(const TxCb*, const uint32_t*, const uint8_t*, 
 
 using namespace tchost;
 
-struct tc_derate_plan {
-  size_t n_cb = 0;
-  size_t rx_floats = 0, llr_floats = 0;
-  uint32_t max_valid = 0;
-  tcdev::DerateCb* d_cbs = nullptr;
-  uint16_t* d_rank = nullptr;
-  ~tc_derate_plan() {
-    for (void* p : {(void*)d_cbs, (void*)d_rank})
-      if (p) (void)hipFree(p);
-  }
-};
 
 namespace {
 
@@ -76,6 +66,8 @@ void build_derate_plan(tc_derate_plan& p, const uint32_t* k, const uint32_t* fil
     c.rv = rv ? rv[i] : 0;
     c.pad = 0;
     p.max_valid = std::max(p.max_valid, g.n_valid);
+    p.ks.push_back(k[i]);
+    p.fs.push_back(f);
     in += e[i];
     out += 3ull * d;
   }
@@ -169,7 +161,17 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
     turbo_cpp::SegmentationResult seg;
     std::vector<uint64_t> e(0);
     try {
-      seg = turbo_cpp::detail::segmentation_sizes(d.transport_block_bits + 24);
+      if (d.code_block_crc) {
+        if (turbo_cpp::detail::qpp_index(d.transport_block_bits + 24) < 0)
+          throw Failure{TC_ERR_OUT_OF_RANGE, "code_block_crc needs B + 24 to be a QPP block size"};
+        seg.c = 1;
+        seg.c_plus = 1;
+        seg.k_plus = d.transport_block_bits + 24;
+        seg.filler_bits = 0;
+        seg.l = 24;
+      } else {
+        seg = turbo_cpp::detail::segmentation_sizes(d.transport_block_bits + 24);
+      }
       for (size_t r = 0; r < seg.c; ++r)
         e.push_back(turbo_cpp::detail::compute_e(r, d.g_total, seg.c, d.num_layers, d.mod_order));
     } catch (const std::invalid_argument& ex) {
@@ -178,7 +180,7 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
       throw Failure{TC_ERR_INVALID_ARGUMENT, ex.what()};
     }
     info[t] = {bits, static_cast<uint32_t>(d.transport_block_bits), static_cast<uint32_t>(p.ks.size()),
-               static_cast<uint32_t>(seg.c), 0};
+               static_cast<uint32_t>(seg.c), d.code_block_crc ? 1u : 0u};
     uint32_t cursor = 0;
     for (size_t r = 0; r < seg.c; ++r) {
       const uint32_t k = static_cast<uint32_t>(r < seg.c_minus ? seg.k_minus : seg.k_plus);
@@ -197,7 +199,7 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
       x.k = k;
       x.filler = f;
       x.cursor = cursor;
-      x.seg = seg.c > 1 ? 1u : 0u;
+      x.seg = (seg.c > 1 || d.code_block_crc) ? 1u : 0u;
       x.payload = k - f - 24 * x.seg;
       const auto& qe = turbo_cpp::detail::kQppTable[turbo_cpp::detail::qpp_index(k)];
       x.f1 = static_cast<uint32_t>(qe.f1);
@@ -232,7 +234,7 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
   p.n_tb = n_tb;
   p.n_cb = p.ks.size();
   p.bits_total = bits;
-  const tc_options o = opts ? *opts : tc_options{8, 0, TC_DECODER_WINDOW_LOG, 0, 0};
+  const tc_options o = opts ? *opts : tc_options{8, 0, TC_DECODER_WINDOW_LOG, 0, 0, 0};
   p.max_iterations = o.max_iterations;
   check_status(tc_derate_plan_create(p.ks.data(), p.fs.data(), p.es.data(), rvs.data(), p.n_cb, &p.derate));
   check_status(tc_plan_create(p.ks.data(), p.fs.data(), p.n_cb, &o, &p.decoder));
@@ -274,13 +276,20 @@ void run_encode(tc_tb_plan& p, const uint8_t* payload, int32_t rv, uint8_t* rm, 
   hip_check(hipGetLastError(), "rate matching launch");
 }
 
-void run_tb(tc_tb_plan& p, const float* rx, int32_t rv, int32_t accumulate, uint8_t* bits, int32_t* tb_ok,
-            int32_t* cb_ok, uint32_t* iters, float* post, hipStream_t s) {
-  if (rv < -1 || rv > 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "rv must be -1 (per block) or 0..3"};
+// n_tx received transmissions (0: decode the held input as it stands): the
+// decoder de-rate matches them in its input staging and refreshes the held
+// input (the HARQ soft buffer), then the TB assembly runs.
+void run_tb(tc_tb_plan& p, uint32_t n_tx, const float* const* rx, const int32_t* rv, int32_t accumulate,
+            uint8_t* bits, int32_t* tb_ok, int32_t* cb_ok, uint32_t* iters, float* post, hipStream_t s) {
+  for (uint32_t t = 0; rv && t < n_tx; ++t)
+    if (rv[t] < -1 || rv[t] > 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "rv must be -1 (per block) or 0..3"};
   if (p.n_tb == 0) return;
   float* post_buf = post ? post : p.d_post;
-  if (rx) run_derate(*p.derate, rx, rv, accumulate, p.d_soft, s);  // else: decode the soft buffer as it stands
-  check_status(tc_plan_decode(p.decoder, p.d_soft, post_buf, nullptr, p.d_cb_iters, s));
+  if (n_tx)
+    check_status(tc_plan_decode_rx(p.decoder, p.derate, n_tx, rx, rv, accumulate, p.d_soft, post_buf, nullptr,
+                                   p.d_cb_iters, s));
+  else
+    check_status(tc_plan_decode(p.decoder, p.d_soft, post_buf, nullptr, p.d_cb_iters, s));
   if (p.n_tb > 0x7FFFFFFFull) throw Failure{TC_ERR_INVALID_ARGUMENT, "too many transport blocks"};
   hipLaunchKernelGGL(tcdev::tb_assemble_kernel, dim3(static_cast<uint32_t>(p.n_tb)), dim3(tcdev::kTbThreads), 0, s,
                      p.d_tbs, p.d_cbs, post_buf, p.d_cb_iters, p.max_iterations, p.d_pow24a, p.d_pow24b, bits, tb_ok,
@@ -365,7 +374,22 @@ tc_status tc_tb_plan_decode(tc_tb_plan* plan, const float* rx_dev, int32_t rv, i
                             float* posterior_dev, void* hip_stream) {
   return guarded([&] {
     if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
-    run_tb(*plan, rx_dev, rv, accumulate, bits_dev, tb_crc_ok_dev, cb_crc_ok_dev, iterations_dev, posterior_dev,
+    if (rv < -1 || rv > 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "rv must be -1 (per block) or 0..3"};
+    const float* rxs[1] = {rx_dev};
+    const int32_t rvs[1] = {rv};
+    run_tb(*plan, rx_dev ? 1u : 0u, rxs, rvs, accumulate, bits_dev, tb_crc_ok_dev, cb_crc_ok_dev, iterations_dev,
+           posterior_dev, static_cast<hipStream_t>(hip_stream));
+  });
+}
+
+tc_status tc_tb_plan_decode_multi(tc_tb_plan* plan, uint32_t n_tx, const float* const* rx_dev, const int32_t* rv,
+                                  uint8_t* bits_dev, int32_t* tb_crc_ok_dev, int32_t* cb_crc_ok_dev,
+                                  uint32_t* iterations_dev, float* posterior_dev, void* hip_stream) {
+  return guarded([&] {
+    if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    if (n_tx == 0 || n_tx > 4) throw Failure{TC_ERR_INVALID_ARGUMENT, "n_tx must be 1..4"};
+    if (!rx_dev) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    run_tb(*plan, n_tx, rx_dev, rv, 0, bits_dev, tb_crc_ok_dev, cb_crc_ok_dev, iterations_dev, posterior_dev,
            static_cast<hipStream_t>(hip_stream));
   });
 }
@@ -419,7 +443,8 @@ tc_status tc_decode_transport_blocks_host(const tc_tb_desc* tbs, size_t n_tb, co
     DevBuf<int32_t> d_tb(n_tb), d_cb(p.n_cb);
     DevBuf<uint32_t> d_it(n_tb);
     hip_check(hipMemcpy(d_rx.p, rx, nrx * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
-    run_tb(p, d_rx.p, -1, 0, d_bits.p, d_tb.p, d_cb.p, d_it.p, d_post.p, nullptr);
+    const float* rxs[1] = {d_rx.p};
+    run_tb(p, 1, rxs, nullptr, 0, d_bits.p, d_tb.p, d_cb.p, d_it.p, d_post.p, nullptr);
     hip_check(hipDeviceSynchronize(), "transport block decode");
     hip_check(hipMemcpy(bits, d_bits.p, p.bits_total, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     hip_check(hipMemcpy(tb_crc_ok, d_tb.p, n_tb * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");

@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kTbThreads) void tb_assemble_kernel(
   __shared__ uint32_t red[kTbThreads / 64];
   const TbInfo t = tbs[blockIdx.x];
   const uint32_t len = t.b + 24;  // bits under the CRC24A
-  const bool seg = t.c > 1;
+  const bool seg = t.c > 1 || t.cb_crc;  // code blocks carry a CRC24B
   uint32_t tb_x = 0, pos = 0, it_max = 0;
   for (uint32_t r = 0; r < t.c; ++r) {
     const TbCb cb = cbs[t.cb_first + r];
@@ -126,13 +126,14 @@ __global__ __launch_bounds__(kTbThreads) void tb_assemble_kernel(
       }
     }
     if (seg) cb_x = block_xor(cb_x, red);
+    if (t.cb_crc) tb_x = (cb_x == 0 && n >= 24) ? 0u : 1u;  // the block's CRC24B is the TB's check
     if (cb_ok && threadIdx.x == 0) cb_ok[t.cb_first + r] = seg ? (cb_x == 0 && n >= 24) : 1;
     if (cb_iters) it_max = max(it_max, cb_iters[t.cb_first + r]);
     pos += np;
   }
-  tb_x = block_xor(tb_x, red);
+  if (!t.cb_crc) tb_x = block_xor(tb_x, red);
   if (threadIdx.x == 0) {
-    if (tb_ok) tb_ok[blockIdx.x] = (tb_x == 0 && pos == len) ? 1 : 0;
+    if (tb_ok) tb_ok[blockIdx.x] = (tb_x == 0 && (pos == len || t.cb_crc)) ? 1 : 0;
     if (tb_iters) tb_iters[blockIdx.x] = max_iterations ? it_max : 0;
   }
 }

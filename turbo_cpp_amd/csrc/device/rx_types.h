@@ -25,12 +25,13 @@ static_assert(sizeof(DerateCb) == 56, "DerateCb layout");
 
 // Transport-block assembly: one TB of B payload bits carried by code blocks
 // cb_first .. cb_first + c - 1; its payload lands at bits_off of the output.
+// cb_crc: a single code block of B + 24 bits ending in a CRC24B (no CRC24A).
 struct TbInfo {
   uint64_t bits_off;
   uint32_t b;
   uint32_t cb_first;
   uint32_t c;
-  uint32_t pad;
+  uint32_t cb_crc;  // 1: one code block carrying a CRC24B instead of a CRC24A
 };
 static_assert(sizeof(TbInfo) == 24, "TbInfo layout");
 

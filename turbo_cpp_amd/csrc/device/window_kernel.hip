@@ -207,7 +207,7 @@ struct LinDomain {
 
 // Tail beta in the log domain, the reference's rules (lte_turbo.cpp:357-378):
 // every state's forced three-step path to state 0, renormalised to max 0.
-__device__ void tail_beta(float ts0, float ts1, float ts2, float tp0, float tp1, float tp2, float* beta) {
+__device__ __forceinline__ void tail_beta(float ts0, float ts1, float ts2, float tp0, float tp1, float tp2, float* beta) {
   const float ts[3] = {ts0, ts1, ts2}, tp[3] = {tp0, tp1, tp2};
 #pragma unroll
   for (unsigned s = 0; s < 8; ++s) {
@@ -488,7 +488,7 @@ __device__ __forceinline__ void siso_half(const Lane& L, float* __restrict__ lds
 // bitmap: bit i contributes D^(K-1-i) mod G.  Each lane folds its own
 // segment (natural order: r(i+1) = r(i) * D^-1 mod G), then the first lane of
 // each block XORs its M partials.  Returns this lane's block verdict.
-__device__ bool crc24b_job(const Lane& L, const uint32_t* __restrict__ bitmap, uint32_t* __restrict__ partial,
+__device__ __forceinline__ bool crc24b_job(const Lane& L, const uint32_t* __restrict__ bitmap, uint32_t* __restrict__ partial,
                            const uint32_t* __restrict__ crc_pow, uint32_t filler) {
   const uint32_t S = L.S, K = L.K, lane = threadIdx.x;
   uint32_t x = 0;
@@ -506,8 +506,68 @@ __device__ bool crc24b_job(const Lane& L, const uint32_t* __restrict__ bitmap, u
   return y == 0 && K - filler >= 24;
 }
 
+// One block's view of the received LLRs for staging-time de-rate matching
+// (scalars only, so it lives in registers).
+struct RxBlk {
+  uint64_t in_off;
+  uint32_t e, nv;
+  const uint16_t* rank;  // rank of each [d0|d1|d2] position, kNullRank = NULL entry
+  uint32_t n00, n01, n02, n03;  // first rank of each transmission
+};
+
+__device__ __forceinline__ uint32_t start_of(const RxBlk& B, int t) {
+  return t == 0 ? B.n00 : (t == 1 ? B.n01 : (t == 2 ? B.n02 : B.n03));
+}
+
+__device__ __forceinline__ void rx_block(const RxIn& X, uint32_t cb, RxBlk& r) {
+  const DerateCb* c = X.dcb + cb;
+  r.in_off = c->in_off;
+  r.e = c->e;
+  r.nv = c->n_valid;
+  r.rank = X.rank + c->rank_off;
+  const uint32_t own = c->rv;
+  r.n00 = c->start[(X.rv[0] == kBlockRv ? own : X.rv[0]) & 3u];
+  r.n01 = c->start[(X.rv[1] == kBlockRv ? own : X.rv[1]) & 3u];
+  r.n02 = c->start[(X.rv[2] == kBlockRv ? own : X.rv[2]) & 3u];
+  r.n03 = c->start[(X.rv[3] == kBlockRv ? own : X.rv[3]) & 3u];
+}
+
+// Soft de-rate value of decoder-input position pos (= s (K+4) + i) of one
+// block: received values landing on its circular-buffer rank summed in
+// reception order per transmission, transmissions added in order (the
+// order of tc_derate_plan_run, so the result is bit-identical to it).
+__device__ __forceinline__ float rx_value(const RxIn& X, const RxBlk& B, const float* keep_blk, uint32_t pos) {
+  float v = X.base ? keep_blk[pos] : 0.0f;
+  const uint32_t r = B.rank[pos];
+  if (r == kNullRank) return v;
+#pragma unroll
+  for (int t = 0; t < kMaxTx; ++t) {
+    if ((uint32_t)t >= X.n_tx) break;
+    const uint32_t n0 = start_of(B, t);
+    const uint32_t m = r >= n0 ? r - n0 : r + B.nv - n0;
+    if (m < B.e) {
+      const float* in = X.rx[t] + B.in_off;
+      float a = 0.0f;
+      for (uint32_t k = m; k < B.e; k += B.nv) a += in[k];
+      v = (t == 0 && !X.base) ? a : v + a;
+    }
+  }
+  return v;
+}
+
+// Decoder input value at position pos of block cb: from llr, or de-rate
+// matched from the received LLRs (then also written to keep if set).
+__device__ __forceinline__ float chan_in(const WinArgs& A, const RxBlk& B, uint64_t llr_off, uint32_t pos,
+                                         bool write_keep) {
+  if (A.rx.n_tx == 0) return A.llr[llr_off + pos];
+  float* keep = A.rx.keep ? A.rx.keep + llr_off : nullptr;
+  const float v = rx_value(A.rx, B, keep, pos);
+  if (keep && write_keep) keep[pos] = v;
+  return v;
+}
+
 template <class D, bool ES>
-__device__ void window_decode(const WinArgs& A) {
+__device__ __forceinline__ void window_decode(const WinArgs& A) {
   extern __shared__ float lds[];
   const uint32_t lane = threadIdx.x;
   const uint32_t* __restrict__ cb_index = A.cb_index;
@@ -542,9 +602,7 @@ __device__ void window_decode(const WinArgs& A) {
     const uint32_t K = J.k, S = J.seg, M = J.nseg;
     const uint32_t my_cb = cb_index[J.cb_off + b];
     const CbInfo ci = cbs[my_cb];
-    const float* d0 = llr + ci.llr_off;
-    const float* d1 = d0 + (K + 4);
-    const float* d2 = d1 + (K + 4);
+    const bool fused = A.rx.n_tx > 0;
     // LDS: extrinsics [S][65] floats, then the hard-decision bitmap and the
     // CRC partials (early stop only)
     uint32_t* bitmap = reinterpret_cast<uint32_t*>(lds + S * kLdsRow);
@@ -553,14 +611,20 @@ __device__ void window_decode(const WinArgs& A) {
 
     // ---- stage d0 of every block into LDS (transposed) ----
     for (uint32_t bb = 0; bb < J.nblk; ++bb) {
-      const float* src = llr + cbs[cb_index[J.cb_off + bb]].llr_off;
+      const uint32_t cbb = cb_index[J.cb_off + bb];
+      const uint64_t off = cbs[cbb].llr_off;
+      RxBlk B{};
+      if (fused) rx_block(A.rx, cbb, B);
       for (uint32_t i = lane; i < K; i += kWaveLanes) {
         const uint32_t qq = __umulhi(i, J.div_magic);
         const uint32_t t = i - qq * S;
-        lds[t * kLdsRow + bb * M + qq] = src[i];
+        lds[t * kLdsRow + bb * M + qq] = fused ? chan_in(A, B, off, i, true) : llr[off + i];
       }
     }
     __syncthreads();
+    RxBlk Bm{};
+    if (fused) rx_block(A.rx, my_cb, Bm);
+    const uint32_t D4 = K + 4;
     // ---- per-wave scratch in [t][lane] order: (d0, d1) and (d0[pi], d2) ----
     {
       uint32_t y = qpp_col0(L, L.q, 0);
@@ -569,7 +633,9 @@ __device__ void window_decode(const WinArgs& A) {
         const float xs = lds[t * kLdsRow + L.colb + L.q];
         const float xp = lds[lds_addr<1>(L, qtab, L.q, t, y)];
         const uint32_t i = L.q * S + t;
-        const float p1 = d1[i], p2 = d2[i];
+        // idle lanes (shadowing the last busy lane) neither read nor write the held input
+        const float p1 = (fused && !L.active) ? 0.0f : chan_in(A, Bm, ci.llr_off, D4 + i, true);
+        const float p2 = (fused && !L.active) ? 0.0f : chan_in(A, Bm, ci.llr_off, 2 * D4 + i, true);
         if (L.active) {
           sc_up[t * kWaveLanes + lane] = make_float2(xs, p1);
           sc_lo[t * kWaveLanes + lane] = make_float2(xp, p2);
@@ -580,9 +646,14 @@ __device__ void window_decode(const WinArgs& A) {
     // ---- boundary metrics: alpha_0 / tail beta / uniform ----
     {
       float tb[2][8];
+      float tl[3][4];  // d0, d1, d2 at K .. K+3
       // decoder tail read-out of lte_turbo.cpp:758-773
-      tail_beta(d0[K + 0], d0[K + 2], 0.0f, d1[K + 0], d0[K + 1], d1[K + 2], tb[0]);
-      tail_beta(d1[K + 3], d2[K + 0], d2[K + 2], d2[K + 1], d0[K + 3], d2[K + 3], tb[1]);
+#pragma unroll
+      for (uint32_t sgm = 0; sgm < 3; ++sgm)
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) tl[sgm][j] = chan_in(A, Bm, ci.llr_off, sgm * D4 + K + j, false);
+      tail_beta(tl[0][0], tl[0][2], 0.0f, tl[1][0], tl[0][1], tl[1][2], tb[0]);
+      tail_beta(tl[1][3], tl[2][0], tl[2][2], tl[2][1], tl[0][3], tl[2][3], tb[1]);
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
 #pragma unroll
@@ -593,6 +664,14 @@ __device__ void window_decode(const WinArgs& A) {
       }
     }
     __syncthreads();  // scratch rows visible to neighbour lanes; d0 staging done
+    if (fused && A.rx.keep && L.active && L.first) {
+      // every lane of the block has read the tails; one recomputes and writes them
+      for (uint32_t sgm = 0; sgm < 3; ++sgm)
+        for (uint32_t j = 0; j < 4; ++j) {
+          const uint32_t pos = sgm * D4 + K + j;
+          A.rx.keep[ci.llr_off + pos] = chan_in(A, Bm, ci.llr_off, pos, false);
+        }
+    }
     for (uint32_t i = lane; i < S * kLdsRow; i += kWaveLanes) lds[i] = 0.0f;
     __syncthreads();
 
