@@ -118,7 +118,7 @@ def cpu_baseline(llr):
             "single_thread_gbit_s": single}
 
 
-def rx_chain_leg(T, torch, stream, llr, decode_ms):
+def rx_chain_leg(T, torch, stream, llr, decode_ms, tuned):
     """Receive chain on the bench blocks: received LLRs in circular-buffer
     order (rv 0, every non-NULL entry once) -> TbPlan (de-rate matching,
     decode, TB assembly)."""
@@ -129,7 +129,8 @@ def rx_chain_leg(T, torch, stream, llr, decode_ms):
     rx = np.empty((n, e), np.float32)
     rx[:, where] = llr
     d_rx = torch.from_numpy(rx.reshape(-1)).cuda()
-    tb = T.TbPlan([(K - 24, e, 0, 1, 2)] * n, T.DECODER_WINDOW_LOG, ITERS)
+    tb = T.TbPlan([(K - 24, e, 0, 1, 2)] * n, T.DECODER_WINDOW_LOG, ITERS, **tuned)
+    tb.set_hold(False)  # one transmission: no HARQ soft buffer to keep
     der = T.DeratePlan([K] * n, [e] * n)
     soft = torch.empty(n * 3 * d, dtype=torch.float32, device="cuda")
     bits = torch.empty(tb.bits_total, dtype=torch.uint8, device="cuda")
@@ -205,6 +206,11 @@ def _plain_sample(T, torch, ks, llr_dev, offs, iters):
     return sum(ks) / (time.perf_counter() - t0) / 1e9
 
 
+# Extra Es/N0 for bit-interleaved Gray QAM with max-log LLRs over the Shannon
+# limit of the same spectral efficiency (grows with the order)
+QAM_GAP_DB = {2: 0.0, 4: 1.5, 6: 3.5}
+
+
 def _margin_n0(qm, b, g, margin_db):
     """Noise variance for unit-energy symbols at Es/N0 = Shannon limit of the
     transport block's spectral efficiency (qm b / g bits per symbol) + margin."""
@@ -213,11 +219,12 @@ def _margin_n0(qm, b, g, margin_db):
     return 1.0 / esn0
 
 
-def uneven_leg(T, torch, n_cb_target=20000, seed=31):
+def uneven_leg(T, torch, geo, n_cb_target=20000, seed=31):
     """Uneven batch: transport blocks of B uniform in [1000, 75376] bits (CRC24A,
     segmentation with filler and CRC24B), code rate uniform in [0.3, 0.9],
     QPSK/16QAM/64QAM uniform, rv uniform 0..3, AWGN at Es/N0 = Shannon limit +
-    uniform [1, 3] dB, max-log LLRs; about 20k code blocks per launch;
+    uniform [1, 3] dB + a QAM gap (0 / 1.5 / 3.5 dB for QPSK / 16QAM / 64QAM),
+    max-log LLRs; about 20k code blocks per launch;
     window decoder with CRC24B early stop, max 8 iterations.  Data are made on
     the GPU (encoder, channel); the timed region (de-rate matching, decode, TB
     assembly) is one captured HIP graph."""
@@ -230,9 +237,10 @@ def uneven_leg(T, torch, n_cb_target=20000, seed=31):
         c = T._segment_sizes(b + 24)[0]
         g = max(int(round(b / rate)) // qm * qm, qm * c)
         tbs.append((b, g, int(rng.integers(0, 4)), 1, qm))
-        n0s.append(_margin_n0(qm, b, g, float(rng.uniform(1.0, 3.0))))
+        n0s.append(_margin_n0(qm, b, g, float(rng.uniform(1.0, 3.0)) + QAM_GAP_DB[qm]))
         ncb += c
-    plan = T.TbPlan(tbs, T.DECODER_WINDOW_LOG, 8, early_stop=True)
+    plan = T.TbPlan(tbs, T.DECODER_WINDOW_LOG, 8, early_stop=True, **geo)
+    plan.set_hold(False)  # first transmissions only
     gen = torch.Generator(device="cuda").manual_seed(seed)
     pay = torch.randint(0, 2, (plan.bits_total,), dtype=torch.uint8, device="cuda", generator=gen)
     rm = torch.empty(plan.rx_floats, dtype=torch.uint8, device="cuda")
@@ -273,7 +281,8 @@ def uneven_leg(T, torch, n_cb_target=20000, seed=31):
     log(f"uneven: {len(tbs)} TBs {plan.n_cb} CBs, {ms:.2f} ms -> {info / ms / 1e6:.3f} Gbit/s; "
         f"TB CRC pass {okh.mean():.3f}; plain {plain:.4f} Gbit/s")
     return {"workload": "uneven TBs B~U[1000,75376], rate U[0.3,0.9], QPSK/16QAM/64QAM, rv U{0..3}, AWGN "
-                        "at Shannon limit + U[1,3] dB, max-log LLRs, early stop, max 8 iterations",
+                        "at Shannon limit + U[1,3] dB + QAM gap 0/1.5/3.5 dB, max-log LLRs, early stop, max 8 "
+                        "iterations",
             "transport_blocks": len(tbs), "code_blocks": plan.n_cb, "info_bits": info,
             "code_block_bits": int(plan.ks.sum()), "ms": ms, "info_gbit_s": info / ms / 1e6,
             "cb_gbit_s": float(plan.ks.sum()) / ms / 1e6, "mean_iterations_tb_max": float(its.float().mean()),
@@ -282,7 +291,7 @@ def uneven_leg(T, torch, n_cb_target=20000, seed=31):
             "plain_gbit_s": plain, "batched_vs_plain": (float(plan.ks.sum()) / ms / 1e6) / plain}
 
 
-def mixed_leg(T, torch, n_cb=65536, seed=41):
+def mixed_leg(T, torch, geo, n_cb=65536, seed=41):
     """Large mixed load: 65536 code blocks, 70% K=6144 and 30% uniform over
     the QPP table, each K - 24 random bits + CRC24B (code_block_crc
     descriptors, so early stop applies as to segmented blocks); 16QAM over
@@ -298,7 +307,8 @@ def mixed_leg(T, torch, n_cb=65536, seed=41):
     es = (3 * (ks + 4) // 2) // 4 * 4
     ebn0 = rng.uniform(2.0, 6.0, n_cb)
     tbs = [(int(k) - 24, int(e), 0, 1, 4, 1) for k, e in zip(ks, es)]  # K - 24 payload bits + CRC24B
-    plan = T.TbPlan(tbs, T.DECODER_WINDOW_LOG, 8, early_stop=True)
+    plan = T.TbPlan(tbs, T.DECODER_WINDOW_LOG, 8, early_stop=True, **geo)
+    plan.set_hold(False)  # both transmissions are combined inside the decode
     gen = torch.Generator(device="cuda").manual_seed(seed)
     pay = torch.randint(0, 2, (plan.bits_total,), dtype=torch.uint8, device="cuda", generator=gen)
     # Es/N0 = Eb/N0 + 10 log10(qm B / E) per transmission
@@ -325,9 +335,9 @@ def mixed_leg(T, torch, n_cb=65536, seed=41):
     mean_it = float(its.float().mean())
     kb = int(ks.sum())
     gbit = kb / ms / 1e6
-    # algorithmic HBM bytes: two received LLR sets read, the combined decoder input written
-    # once (the plan's HARQ soft buffer), posteriors written and read, payload bits
-    hbm = 2 * 4 * int(es.sum()) + 4 * int((3 * (ks + 4)).sum()) + 2 * 4 * kb + plan.bits_total
+    # algorithmic HBM bytes: two received LLR sets read, posteriors written and read (TB
+    # assembly), payload bits written
+    hbm = 2 * 4 * int(es.sum()) + 2 * 4 * kb + plan.bits_total
     valu = kb * mean_it * 864 / (ms / 1e3)
     # plain path on 64 blocks: one launch per block on the combined decoder input
     der = T.DeratePlan(plan.ks[:64], plan.es[:64])
@@ -426,7 +436,9 @@ def main():
     ap.add_argument("--no-rx-chain", action="store_true")
     ap.add_argument("--train-steps", type=int, default=500, help="Adam steps of the differentiation leg (0: skip it)")
     ap.add_argument("--no-scenarios", action="store_true", help="skip the uneven-batch and mixed-load legs")
-    ap.add_argument("--no-autotune", action="store_true", help="default geometry instead of the tuned one")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the tuner; use --min-segment/--warmup-steps")
+    ap.add_argument("--min-segment", type=int, default=0, help="with --no-autotune (0: library default 32)")
+    ap.add_argument("--warmup-steps", type=int, default=0, help="with --no-autotune (0: library default 32)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -442,7 +454,9 @@ def main():
     other = T.DECODER_WINDOW_LIN if dec == T.DECODER_WINDOW_LOG else T.DECODER_WINDOW_LOG
     tuned = None
     tuned_opts = {}
-    if not args.no_autotune:
+    if args.no_autotune:
+        tuned_opts = {"min_segment": args.min_segment, "warmup": args.warmup_steps, "waves_per_cu": 0}
+    else:
         from turbo_cpp_amd import autotune
         log("autotuning segment length / warm-up / waves per CU")
         tuned = autotune.tune(K, dec, ITERS, n_blocks=N_CB, ebn0_db=0.5)
@@ -522,10 +536,11 @@ def main():
     plain_gbits = K / plain_s / 1e9
     log(f"plain: {plain_s * 1e3:.3f} ms per block -> {plain_gbits * 1e3:.3f} Mbit/s")
 
-    rx_chain = None if args.no_rx_chain else rx_chain_leg(T, torch, stream, llr, kernel_ms)
+    geo_kw = {k: v for k, v in tuned_opts.items() if k in ("min_segment", "warmup")}
+    rx_chain = None if args.no_rx_chain else rx_chain_leg(T, torch, stream, llr, kernel_ms, geo_kw)
     diff = diff_leg(T, torch, args.train_steps) if args.train_steps > 0 else None
-    uneven = None if args.no_scenarios else uneven_leg(T, torch)
-    mixed = None if args.no_scenarios else mixed_leg(T, torch)
+    uneven = None if args.no_scenarios else uneven_leg(T, torch, geo_kw)
+    mixed = None if args.no_scenarios else mixed_leg(T, torch, geo_kw)
 
     if args.copy_calibration:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)

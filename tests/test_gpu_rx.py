@@ -516,3 +516,29 @@ def test_tb_decode_multi_equals_combine_then_decode():
         outs.append((bits, post, ok))
     for a, b in zip(*outs):
         assert torch.equal(a, b)
+
+
+def test_tb_plan_without_held_input():
+    """set_hold(False): same decode results, no accumulation allowed; rx NULL
+    after soft_combine still works."""
+    import torch
+    tbs, rx, _ = _tb_batch(51, n_tb=8)
+    outs = []
+    for hold in (True, False):
+        plan = T.TbPlan(tbs, max_iterations=6, early_stop=True)
+        plan.set_hold(hold)
+        r = torch.from_numpy(rx).cuda()
+        post = torch.zeros(plan.post_floats, dtype=torch.float32, device="cuda")
+        ok = torch.zeros(plan.n_tb, dtype=torch.int32, device="cuda")
+        plan.decode(r.data_ptr(), 0, ok.data_ptr(), post_ptr=post.data_ptr())
+        torch.cuda.synchronize()
+        outs.append((post, ok))
+        if not hold:
+            with pytest.raises(ValueError):
+                plan.decode(r.data_ptr(), accumulate=True)
+            plan.soft_combine(r.data_ptr())
+            post2 = torch.zeros_like(post)
+            plan.decode(0, post_ptr=post2.data_ptr())
+            torch.cuda.synchronize()
+            assert torch.equal(post2, post)
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])

@@ -114,6 +114,7 @@ def lib() -> C.CDLL:
         L.tc_plan_decode_rx.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int32,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.tc_tb_plan_decode_multi.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p] + [C.c_void_p] * 6
+        L.tc_tb_plan_set_hold.argtypes = [C.c_void_p, C.c_int32]
         L.tc_tb_plan_soft_combine.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
         L.tc_tb_plan_encode.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.tc_encode_transport_blocks_host.argtypes = [C.POINTER(TcTbDesc), C.c_size_t, _u8p, _u8p, C.c_void_p]
@@ -639,6 +640,10 @@ class TbPlan:
         _check(lib().tc_tb_plan_decode_multi(self._h, n, ptrs, rv, bits_ptr or None, tb_crc_ptr or None,
                                              cb_crc_ptr or None, iters_ptr or None, post_ptr or None,
                                              stream or None))
+
+    def set_hold(self, hold: bool):
+        """Whether decodes from received LLRs keep the decoder input for HARQ (default True)."""
+        _check(lib().tc_tb_plan_set_hold(self._h, int(hold)))
 
     def soft_combine(self, rx_ptr: int, rv: int = -1, accumulate: bool = False, stream: int = 0):
         """De-rate match one transmission into the held decoder input (no decode)."""

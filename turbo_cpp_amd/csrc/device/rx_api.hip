@@ -99,6 +99,7 @@ struct tc_tb_plan {
   size_t n_tb = 0, n_cb = 0;
   uint64_t bits_total = 0;
   uint32_t max_iterations = 0;
+  bool exact = false;  // exact decoder: de-rate matching goes through d_soft
   std::vector<uint32_t> ks, fs;
   std::vector<uint64_t> es;
   tc_derate_plan* derate = nullptr;
@@ -108,6 +109,7 @@ struct tc_tb_plan {
   uint32_t* d_pow24a = nullptr;
   uint32_t* d_pow24b = nullptr;
   float* d_soft = nullptr;  // decoder input; the HARQ soft buffer between calls
+  bool hold = true;         // decodes from received LLRs refresh d_soft
   float* d_post = nullptr;  // posteriors when the caller passes none
   uint32_t* d_cb_iters = nullptr;
   // transmit side
@@ -236,6 +238,7 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
   p.bits_total = bits;
   const tc_options o = opts ? *opts : tc_options{8, 0, TC_DECODER_WINDOW_LOG, 0, 0, 0};
   p.max_iterations = o.max_iterations;
+  p.exact = o.decoder == TC_DECODER_EXACT;
   check_status(tc_derate_plan_create(p.ks.data(), p.fs.data(), p.es.data(), rvs.data(), p.n_cb, &p.derate));
   check_status(tc_plan_create(p.ks.data(), p.fs.data(), p.n_cb, &o, &p.decoder));
   if (n_tb == 0) return;
@@ -285,9 +288,12 @@ void run_tb(tc_tb_plan& p, uint32_t n_tx, const float* const* rx, const int32_t*
     if (rv[t] < -1 || rv[t] > 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "rv must be -1 (per block) or 0..3"};
   if (p.n_tb == 0) return;
   float* post_buf = post ? post : p.d_post;
+  if (accumulate && !p.hold)
+    throw Failure{TC_ERR_INVALID_ARGUMENT, "the plan does not hold its decoder input (tc_tb_plan_set_hold)"};
+  const bool exact = p.exact;
   if (n_tx)
-    check_status(tc_plan_decode_rx(p.decoder, p.derate, n_tx, rx, rv, accumulate, p.d_soft, post_buf, nullptr,
-                                   p.d_cb_iters, s));
+    check_status(tc_plan_decode_rx(p.decoder, p.derate, n_tx, rx, rv, accumulate,
+                                   (p.hold || exact) ? p.d_soft : nullptr, post_buf, nullptr, p.d_cb_iters, s));
   else
     check_status(tc_plan_decode(p.decoder, p.d_soft, post_buf, nullptr, p.d_cb_iters, s));
   if (p.n_tb > 0x7FFFFFFFull) throw Failure{TC_ERR_INVALID_ARGUMENT, "too many transport blocks"};
@@ -379,6 +385,13 @@ tc_status tc_tb_plan_decode(tc_tb_plan* plan, const float* rx_dev, int32_t rv, i
     const int32_t rvs[1] = {rv};
     run_tb(*plan, rx_dev ? 1u : 0u, rxs, rvs, accumulate, bits_dev, tb_crc_ok_dev, cb_crc_ok_dev, iterations_dev,
            posterior_dev, static_cast<hipStream_t>(hip_stream));
+  });
+}
+
+tc_status tc_tb_plan_set_hold(tc_tb_plan* plan, int32_t hold) {
+  return guarded([&] {
+    if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    plan->hold = hold != 0;
   });
 }
 

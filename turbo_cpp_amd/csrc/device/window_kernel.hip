@@ -536,9 +536,9 @@ __device__ __forceinline__ void rx_block(const RxIn& X, uint32_t cb, RxBlk& r) {
 // block: received values landing on its circular-buffer rank summed in
 // reception order per transmission, transmissions added in order (the
 // order of tc_derate_plan_run, so the result is bit-identical to it).
-__device__ __forceinline__ float rx_value(const RxIn& X, const RxBlk& B, const float* keep_blk, uint32_t pos) {
+__device__ __forceinline__ float rx_from_rank(const RxIn& X, const RxBlk& B, const float* keep_blk, uint32_t pos,
+                                             uint32_t r) {
   float v = X.base ? keep_blk[pos] : 0.0f;
-  const uint32_t r = B.rank[pos];
   if (r == kNullRank) return v;
 #pragma unroll
   for (int t = 0; t < kMaxTx; ++t) {
@@ -555,6 +555,10 @@ __device__ __forceinline__ float rx_value(const RxIn& X, const RxBlk& B, const f
   return v;
 }
 
+__device__ __forceinline__ float rx_value(const RxIn& X, const RxBlk& B, const float* keep_blk, uint32_t pos) {
+  return rx_from_rank(X, B, keep_blk, pos, B.rank[pos]);
+}
+
 // Decoder input value at position pos of block cb: from llr, or de-rate
 // matched from the received LLRs (then also written to keep if set).
 __device__ __forceinline__ float chan_in(const WinArgs& A, const RxBlk& B, uint64_t llr_off, uint32_t pos,
@@ -564,6 +568,20 @@ __device__ __forceinline__ float chan_in(const WinArgs& A, const RxBlk& B, uint6
   const float v = rx_value(A.rx, B, keep, pos);
   if (keep && write_keep) keep[pos] = v;
   return v;
+}
+
+// kStageBatch positions at once: all rank loads, then all value loads, so a
+// lane keeps that many gathers in flight while staging.
+constexpr uint32_t kStageBatch = 8;
+__device__ __forceinline__ void rx_batch(const RxIn& X, const RxBlk& B, uint64_t llr_off, const uint32_t (&pos)[kStageBatch],
+                                         uint32_t valid_mask, float (&out)[kStageBatch]) {
+  const float* keep = X.keep ? X.keep + llr_off : nullptr;
+  uint32_t r[kStageBatch];
+#pragma unroll
+  for (uint32_t u = 0; u < kStageBatch; ++u) r[u] = ((valid_mask >> u) & 1u) ? B.rank[pos[u]] : kNullRank;
+#pragma unroll
+  for (uint32_t u = 0; u < kStageBatch; ++u)
+    out[u] = ((valid_mask >> u) & 1u) ? rx_from_rank(X, B, keep, pos[u], r[u]) : 0.0f;
 }
 
 template <class D, bool ES>
@@ -613,12 +631,34 @@ __device__ __forceinline__ void window_decode(const WinArgs& A) {
     for (uint32_t bb = 0; bb < J.nblk; ++bb) {
       const uint32_t cbb = cb_index[J.cb_off + bb];
       const uint64_t off = cbs[cbb].llr_off;
+      if (!fused) {
+        for (uint32_t i = lane; i < K; i += kWaveLanes) {
+          const uint32_t qq = __umulhi(i, J.div_magic);
+          const uint32_t t = i - qq * S;
+          lds[t * kLdsRow + bb * M + qq] = llr[off + i];
+        }
+        continue;
+      }
       RxBlk B{};
-      if (fused) rx_block(A.rx, cbb, B);
-      for (uint32_t i = lane; i < K; i += kWaveLanes) {
-        const uint32_t qq = __umulhi(i, J.div_magic);
-        const uint32_t t = i - qq * S;
-        lds[t * kLdsRow + bb * M + qq] = fused ? chan_in(A, B, off, i, true) : llr[off + i];
+      rx_block(A.rx, cbb, B);
+      for (uint32_t base = 0; base < K; base += kStageBatch * kWaveLanes) {
+        uint32_t pos[kStageBatch], mask = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kStageBatch; ++u) {
+          pos[u] = base + u * kWaveLanes + lane;
+          mask |= (pos[u] < K ? 1u : 0u) << u;
+        }
+        float v[kStageBatch];
+        rx_batch(A.rx, B, off, pos, mask, v);
+#pragma unroll
+        for (uint32_t u = 0; u < kStageBatch; ++u) {
+          if (!((mask >> u) & 1u)) continue;
+          const uint32_t i = pos[u];
+          const uint32_t qq = __umulhi(i, J.div_magic);
+          const uint32_t t = i - qq * S;
+          lds[t * kLdsRow + bb * M + qq] = v[u];
+          if (A.rx.keep) A.rx.keep[off + i] = v[u];
+        }
       }
     }
     __syncthreads();
@@ -629,18 +669,52 @@ __device__ __forceinline__ void window_decode(const WinArgs& A) {
     {
       uint32_t y = qpp_col0(L, L.q, 0);
       const uint32_t by = qpp_stride(L, L.q);
-      for (uint32_t t = 0; t < S; ++t) {
-        const float xs = lds[t * kLdsRow + L.colb + L.q];
-        const float xp = lds[lds_addr<1>(L, qtab, L.q, t, y)];
-        const uint32_t i = L.q * S + t;
-        // idle lanes (shadowing the last busy lane) neither read nor write the held input
-        const float p1 = (fused && !L.active) ? 0.0f : chan_in(A, Bm, ci.llr_off, D4 + i, true);
-        const float p2 = (fused && !L.active) ? 0.0f : chan_in(A, Bm, ci.llr_off, 2 * D4 + i, true);
-        if (L.active) {
-          sc_up[t * kWaveLanes + lane] = make_float2(xs, p1);
-          sc_lo[t * kWaveLanes + lane] = make_float2(xp, p2);
+      if (fused) {
+        // parities of this lane's segment, kStageBatch steps at a time; idle
+        // lanes (shadowing the last busy lane) neither read nor write the held input
+        for (uint32_t t0 = 0; t0 < S; t0 += kStageBatch / 2) {
+          uint32_t pos[kStageBatch], mask = 0;
+#pragma unroll
+          for (uint32_t u = 0; u < kStageBatch / 2; ++u) {
+            const uint32_t i = L.q * S + t0 + u;
+            pos[2 * u] = D4 + i;
+            pos[2 * u + 1] = 2 * D4 + i;
+            const uint32_t ok = (L.active && t0 + u < S) ? 3u : 0u;
+            mask |= ok << (2 * u);
+          }
+          float v[kStageBatch];
+          rx_batch(A.rx, Bm, ci.llr_off, pos, mask, v);
+#pragma unroll
+          for (uint32_t u = 0; u < kStageBatch / 2; ++u) {
+            const uint32_t t = t0 + u;
+            if (t >= S) break;
+            const float xs = lds[t * kLdsRow + L.colb + L.q];
+            const float xp = lds[lds_addr<1>(L, qtab, L.q, t, y)];
+            if (L.active) {
+              sc_up[t * kWaveLanes + lane] = make_float2(xs, v[2 * u]);
+              sc_lo[t * kWaveLanes + lane] = make_float2(xp, v[2 * u + 1]);
+              if (A.rx.keep) {
+                A.rx.keep[ci.llr_off + pos[2 * u]] = v[2 * u];
+                A.rx.keep[ci.llr_off + pos[2 * u + 1]] = v[2 * u + 1];
+              }
+            }
+            y = y_fwd(y, by, M);
+          }
         }
-        y = y_fwd(y, by, M);
+      } else {
+        const float* d1 = llr + ci.llr_off + D4;
+        const float* d2 = d1 + D4;
+        for (uint32_t t = 0; t < S; ++t) {
+          const float xs = lds[t * kLdsRow + L.colb + L.q];
+          const float xp = lds[lds_addr<1>(L, qtab, L.q, t, y)];
+          const uint32_t i = L.q * S + t;
+          const float p1 = d1[i], p2 = d2[i];
+          if (L.active) {
+            sc_up[t * kWaveLanes + lane] = make_float2(xs, p1);
+            sc_lo[t * kWaveLanes + lane] = make_float2(xp, p2);
+          }
+          y = y_fwd(y, by, M);
+        }
       }
     }
     // ---- boundary metrics: alpha_0 / tail beta / uniform ----
