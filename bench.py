@@ -550,11 +550,21 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 256 MiB read + 256 MiB written per copy kernel")
 
+    # HBM traffic per launch of the headline kernel from its rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    # (copy-calibrated), recorded in profiles/ for this workload and geometry
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "window_decode_counters_k6144_x4096.md")
+    if dec == T.DECODER_WINDOW_LOG and os.path.exists(prof):
+        for line in open(prof):
+            if line.startswith("traffic_bytes_per_launch_log:"):
+                traffic = float(line.split(":")[1])
     bits_per_launch = N_CB * K
     achieved = bits_per_launch * ITERS * VALU_SLOTS_PER_BIT_ITER / (kernel_ms / 1e3)
     hbm_achieved = bits_per_launch * (16 + 48 / K) / (kernel_ms / 1e3) / 1e9
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12, "unit": "Tops/s",
-                "frac": achieved / VALU_PEAK, "traffic": None,
+                "frac": achieved / VALU_PEAK, "traffic": traffic,
+                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
+                                "profiles/window_decode_counters_k6144_x4096.md)",
                 "note": "VALU lane-slots in the reference's pairwise Log-MAP form (864/bit/iteration, "
                         "transcendentals x4) per second; no MFMA work exists in this path",
                 "hbm": {"achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -253,24 +253,80 @@ __device__ void tail_inputs(uint32_t h, const R* d0, const R* d1, const R* d2, u
 }
 
 // alpha over local steps 0 .. g + S, stored at al[j] (alpha_t for t = t_of(j)).
+// The next step's inputs are loaded before this step's stores, so their
+// latency overlaps the recursion.
 template <typename R>
 __device__ void alpha_pass(const Lane<R>& L, const Layout& Y) {
   R a[8];
   const bool from_zero_state = (int)(L.q * L.S) - (int)L.g <= 0;
 #pragma unroll
   for (int s = 0; s < 8; ++s) a[s] = from_zero_state ? (s == 0 ? R(0) : R(kNegF)) : R(0);
-  for (uint32_t j = 0; j < L.g + L.S; ++j) {
+  const uint32_t n = L.g + L.S;
+  R sa_n = L.at(Y.sa, 0), pr_n = L.at(Y.pr, 0);
+  for (uint32_t j = 0; j < n; ++j) {
+    const R sa = sa_n, pr = pr_n;
+    if (j + 1 < n) {
+      sa_n = L.at(Y.sa, j + 1);
+      pr_n = L.at(Y.pr, j + 1);
+    }
 #pragma unroll
     for (int s = 0; s < 8; ++s) L.at(Y.al + s * (Y.g + Y.S + 1), j) = a[s];
     if (L.t_of(j) >= 0) {
       R o[8];
-      alpha_step(a, R(0.5) * L.at(Y.sa, j), R(0.5) * L.at(Y.pr, j), o);
+      alpha_step(a, R(0.5) * sa, R(0.5) * pr, o);
 #pragma unroll
       for (int s = 0; s < 8; ++s) a[s] = o[s];
     }
   }
 #pragma unroll
-  for (int s = 0; s < 8; ++s) L.at(Y.al + s * (Y.g + Y.S + 1), L.g + L.S) = a[s];
+  for (int s = 0; s < 8; ++s) L.at(Y.al + s * (Y.g + Y.S + 1), n) = a[s];
+}
+
+// Forward beta pass that also emits the extrinsics of the own steps
+// (alpha from the alpha pass, beta_{t+1} in registers); beta is not stored.
+template <typename R>
+__device__ void beta_ext_pass(const Lane<R>& L, const Layout& Y, const R (&ts)[3], const R (&tp)[3], uint32_t h,
+                              R* e) {
+  R b[8];
+  const bool from_tail = L.q * L.S + L.S + L.g >= L.K;
+  if (from_tail) {
+    tail_beta(ts, tp, b);
+  } else {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) b[s] = 0;
+  }
+  const uint32_t top = 2 * L.g + L.S;
+  const uint32_t own_end = L.g + L.S;  // own steps: g <= j < g + S
+  R sa_n = L.at(Y.sa, top - 1), pr_n = L.at(Y.pr, top - 1), a_n[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) a_n[s] = (top - 1 < own_end) ? L.at(Y.al + s * (Y.g + Y.S + 1), top - 1) : R(0);
+  for (uint32_t j = top; j-- > L.g;) {
+    const R sa = sa_n, pr = pr_n;
+    R a[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) a[s] = a_n[s];
+    if (j > L.g) {
+      sa_n = L.at(Y.sa, j - 1);
+      pr_n = L.at(Y.pr, j - 1);
+      if (j - 1 < own_end) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) a_n[s] = L.at(Y.al + s * (Y.g + Y.S + 1), j - 1);
+      }
+    }
+    const int t = L.t_of(j);
+    if (j < own_end) {
+      R Lu[2];
+      llr_terms(a, b, R(0.5) * sa, R(0.5) * pr, Lu, (R(*)[8]) nullptr);
+      const uint32_t pos = (h & 1u) ? qpp((uint32_t)t, L.K, L.f1, L.f2) : (uint32_t)t;
+      e[pos] = Lu[1] - Lu[0] - sa;
+    }
+    if (t < (int)L.K) {
+      R o[8];
+      beta_step(b, R(0.5) * sa, R(0.5) * pr, o);
+#pragma unroll
+      for (int s = 0; s < 8; ++s) b[s] = o[s];
+    }
+  }
 }
 
 // beta over local steps g + S + g - 1 down to g, stored at be[j - g]
@@ -341,18 +397,7 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs<R> A) {
         tail_inputs(h, d0, d1, d2, L.K, ts, tp);
         stage(L, Y, h, d0, d1, d2, sv, A.scales);
         alpha_pass(L, Y);
-        beta_pass(L, Y, ts, tp);
-        R* e = sv + (size_t)h * L.K;
-        for (uint32_t j = L.g; j < L.g + L.S; ++j) {
-          R a[8], b[8], Lu[2];
-          load8(L, Y.al, Y.g + Y.S + 1, j, a);
-          load8(L, Y.be, Y.S + Y.g + 1, j + 1 - L.g, b);
-          const R sa = L.at(Y.sa, j);
-          llr_terms(a, b, R(0.5) * sa, R(0.5) * L.at(Y.pr, j), Lu, (R(*)[8]) nullptr);
-          const uint32_t t = (uint32_t)L.t_of(j);
-          const uint32_t pos = (h & 1u) ? qpp(t, L.K, L.f1, L.f2) : t;
-          e[pos] = Lu[1] - Lu[0] - sa;
-        }
+        beta_ext_pass(L, Y, ts, tp, h, sv + (size_t)h * L.K);
       }
       __syncthreads();
     }
@@ -439,21 +484,37 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
         // alpha adjoint, backwards over local steps g + S - 1 .. 0; output
         // adjoints of the own steps on the way
         R ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        // inputs of step j are loaded during step j + 1 (their latency overlaps the arithmetic)
+        const uint32_t jtop = L.g + L.S - 1;
+        R nx_a[8], nx_b[8], nx_sa = L.at(Y.sa, jtop), nx_pr = L.at(Y.pr, jtop), nx_e = L.at(Y.eb, jtop - L.g);
+        load8(L, Y.al, Y.g + Y.S + 1, jtop, nx_a);
+        load8(L, Y.be, Y.S + Y.g + 1, jtop + 1 - L.g, nx_b);
         for (uint32_t j = L.g + L.S; j-- > 0;) {
           if (L.t_of(j) < 0) break;
-          R a[8];
-          load8(L, Y.al, Y.g + Y.S + 1, j, a);
-          const R hs = R(0.5) * L.at(Y.sa, j), hp = R(0.5) * L.at(Y.pr, j);
+          R a[8], b[8];
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            a[s] = nx_a[s];
+            b[s] = nx_b[s];
+          }
+          const R hs = R(0.5) * nx_sa, hp = R(0.5) * nx_pr, e = nx_e;
+          if (j > 0 && L.t_of(j - 1) >= 0) {
+            load8(L, Y.al, Y.g + Y.S + 1, j - 1, nx_a);
+            nx_sa = L.at(Y.sa, j - 1);
+            nx_pr = L.at(Y.pr, j - 1);
+            if (j - 1 >= L.g) {
+              load8(L, Y.be, Y.S + Y.g + 1, j - L.g, nx_b);
+              nx_e = L.at(Y.eb, j - 1 - L.g);
+            }
+          }
           R dg[8][2];
 #pragma unroll
           for (int s = 0; s < 8; ++s) dg[s][0] = dg[s][1] = 0;
           R abn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
           R dsa_direct = 0;
           if (j >= L.g) {
-            R b[8], Lu[2], P[2][8];
-            load8(L, Y.be, Y.S + Y.g + 1, j + 1 - L.g, b);
+            R Lu[2], P[2][8];
             llr_terms(a, b, hs, hp, Lu, P);
-            const R e = L.at(Y.eb, j - L.g);
             R bo[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (unsigned s = 0; s < 8; ++s)
@@ -500,20 +561,29 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
               ds += u ? R(0.5) * dg[s][u] : R(-0.5) * dg[s][u];
               dp += par(s, u) ? R(0.5) * dg[s][u] : R(-0.5) * dg[s][u];
             }
-          L.at(Y.dsa, j) += ds;
-          L.at(Y.dpr, j) += dp;
+          L.at(Y.dsa, j) = ds;  // first writer of these positions (all zero before)
+          L.at(Y.dpr, j) = dp;
         }
         // beta adjoint, forwards over local steps g .. g + S + g - 1
         R bb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (uint32_t j = L.g; j < 2 * L.g + L.S; ++j) {
+        const uint32_t jend = 2 * L.g + L.S;
+        R nb_b[8], nb_o[8], nb_sa = L.at(Y.sa, L.g), nb_pr = L.at(Y.pr, L.g);
+        load8(L, Y.be, Y.S + Y.g + 1, 1, nb_b);
+        load8(L, Y.bo, Y.S, 0, nb_o);
+        for (uint32_t j = L.g; j < jend; ++j) {
           if (L.t_of(j) >= (int)K) break;
-          R b[8];
-          load8(L, Y.be, Y.S + Y.g + 1, j + 1 - L.g, b);
-          const R hs = R(0.5) * L.at(Y.sa, j), hp = R(0.5) * L.at(Y.pr, j);
-          R bn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          if (j < L.g + L.S) {
+          R b[8], bn[8];
 #pragma unroll
-            for (int s = 0; s < 8; ++s) bn[s] = L.at(Y.bo + s * Y.S, j - L.g);
+          for (int s = 0; s < 8; ++s) {
+            b[s] = nb_b[s];
+            bn[s] = (j < L.g + L.S) ? nb_o[s] : R(0);
+          }
+          const R hs = R(0.5) * nb_sa, hp = R(0.5) * nb_pr;
+          if (j + 1 < jend && L.t_of(j + 1) < (int)K) {
+            load8(L, Y.be, Y.S + Y.g + 1, j + 2 - L.g, nb_b);
+            nb_sa = L.at(Y.sa, j + 1);
+            nb_pr = L.at(Y.pr, j + 1);
+            if (j + 1 < L.g + L.S) load8(L, Y.bo, Y.S, j + 1 - L.g, nb_o);
           }
           R ds = 0, dp = 0;
 #pragma unroll

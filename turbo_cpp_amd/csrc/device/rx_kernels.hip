@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kDerateThreads) void derate_soft_kernel(const Derat
   }
   __syncthreads();
   const uint32_t rv = (rv_override >= 0 ? static_cast<uint32_t>(rv_override) : c.rv) & 3u;
-  const uint32_t n0 = c.start[rv];
+  const uint32_t n0 = cbs[blockIdx.x].start[rv];  // indexed in global memory: no private copy
   const uint16_t* rk = rank + c.rank_off;
   float* o = out + c.out_off;
   const uint32_t n = 3 * c.d;

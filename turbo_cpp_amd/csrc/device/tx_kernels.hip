@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void rate_match_kernel(const TxCb* __restrict_
                                                          uint8_t* __restrict__ rm) {
   const TxCb b = cbs[blockIdx.x];
   const uint32_t rv = (rv_override >= 0 ? (uint32_t)rv_override : b.rv) & 3u;
-  const uint32_t n0 = b.start[rv], nv = b.n_valid, D = b.k + 4;
+  const uint32_t n0 = cbs[blockIdx.x].start[rv], nv = b.n_valid, D = b.k + 4;  // no private copy
   const uint32_t* v = valid + b.valid_off;
   const uint8_t* d = dsrc + b.d_off;
   for (uint32_t k = threadIdx.x; k < b.e; k += 256) {
