@@ -87,7 +87,7 @@ def _decode(plan: Plan, llr, post, reps: int):
 
 def tune(k: int, decoder: int = DECODER_WINDOW_LOG, iterations: int = 8, n_blocks: int = 2048,
          ebn0_db: float = 0.5, min_segments: Sequence[int] = (32, 64, 96, 128, 192),
-         warmups: Sequence[int] = (16, 24, 32), waves: Sequence[int] = (0, 4, 2), reps: int = 3,
+         warmups: Sequence[int] = (8, 16, 24, 32), waves: Sequence[int] = (0, 4, 2), reps: int = 3,
          seed: int = 1) -> Dict:
     """Returns {"best": {...}, "table": [...], "exact_bit_errors": n, "interval": [lo, hi], ...}."""
     import torch

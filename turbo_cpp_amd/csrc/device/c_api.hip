@@ -199,7 +199,7 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
 }
 
 void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* iters, hipStream_t s,
-            const tcdev::RxIn* rx = nullptr) {
+            const tcdev::RxIn* rx = nullptr, const tcdev::TbOut* tb = nullptr) {
   if (p.n_cb == 0) return;
   const tc_options& o = p.opts;
   if (o.decoder == TC_DECODER_EXACT) {
@@ -211,10 +211,23 @@ void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* i
     tcdev::WinArgs a{p.d_jobs, p.njobs, p.d_cb_index, p.d_cbs, p.d_qtab, p.d_crc_pow, llr, post, p.d_scratch,
                      crc, iters, o.max_iterations, o.early_stop, tcdev::RxIn{}};
     if (rx) a.rx = *rx;
+    a.tb = tcdev::TbOut{};
+    if (tb) a.tb = *tb;
     hipLaunchKernelGGL(window_kernel(o), dim3(p.grid), dim3(64), p.lds_bytes, s, a);
   }
   hip_check(hipGetLastError(), "kernel launch");
 }
+
+}  // namespace
+
+bool tchost::launch_window(tc_plan* plan, const float* llr, float* post, int32_t* crc_ok, uint32_t* iters,
+                           hipStream_t stream, const tcdev::RxIn* rx, const tcdev::TbOut* tb) {
+  if (plan->opts.decoder == TC_DECODER_EXACT) return false;
+  launch(*plan, llr, post, crc_ok, iters, stream, rx, tb);
+  return true;
+}
+
+namespace {
 
 void check_early_stop_support(const tc_options&) {}
 

@@ -81,6 +81,7 @@ struct WinArgs {
   uint32_t max_iterations;
   int32_t early_stop;
   RxIn rx;  // n_tx == 0: the decoder input is llr
+  TbOut tb;  // on == 0: posteriors only
 };
 constexpr int kCrcPowLen = 6144;
 // LDS floats of one window job beyond the extrinsics: hard-decision bitmap

@@ -1,5 +1,7 @@
-// The soft de-rate matching plan, shared by its own entry points and the
-// window decoders' fused input staging.
+// Internal plan interfaces shared by the C ABI translation units: the soft
+// de-rate matching plan (its own entry points and the window decoders'
+// fused input staging) and the decode plan's launch with fused inputs and
+// transport-block assembly.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,3 +24,17 @@ struct tc_derate_plan {
       if (p) (void)hipFree(p);
   }
 };
+
+struct tc_plan;
+namespace tcdev {
+struct RxIn;
+}
+
+namespace tchost {
+// Launch a decode plan's window decoder with optional fused inputs (rx) and
+// TB assembly in its epilogue (tb).  Returns false, launching nothing, for an
+// exact-decoder plan.
+bool launch_window(tc_plan* plan, const float* llr, float* post, int32_t* crc_ok, uint32_t* iters,
+                   hipStream_t stream, const tcdev::RxIn* rx, const tcdev::TbOut* tb);
+}  // namespace tchost
+

@@ -14,6 +14,7 @@
 
 #include "../host/codec_internal.h"
 #include "hip_util.h"
+#include "decoder_types.h"
 #include "derate_plan.h"
 #include "rx_types.h"
 #include "turbo_cpp/c_api.h"
@@ -22,6 +23,7 @@ namespace tcdev {
 __global__ void derate_soft_kernel(const DerateCb*, const uint16_t*, const float*, float*, int32_t, int32_t);
 __global__ void tb_assemble_kernel(const TbInfo*, const TbCb*, const float*, const uint32_t*, uint32_t,
                                    const uint32_t*, const uint32_t*, uint8_t*, int32_t*, int32_t*, uint32_t*);
+__global__ void tb_acc_zero_kernel(uint32_t*, uint32_t);
 __global__ void tb_crc24a_kernel(const TbInfo*, const uint8_t*, const uint32_t*, uint32_t*);
 __global__ void cb_encode_kernel(const TxCb*, const TbInfo*, const uint8_t*, const uint32_t*, const uint32_t*,
                                  uint8_t*, uint8_t*);
@@ -117,11 +119,15 @@ struct tc_tb_plan {
   uint32_t* d_valid = nullptr;   // per (K, F): non-NULL circular-buffer entries in order
   uint8_t* d_streams = nullptr;  // encoder output [d0|d1|d2] per block
   uint32_t* d_tb_crc = nullptr;  // CRC24A per TB
+  // TB assembly in the window decoder's epilogue
+  tcdev::TbCbMap* d_map = nullptr;
+  uint32_t* d_acc = nullptr;  // 3 words per TB
   ~tc_tb_plan() {
     tc_derate_plan_destroy(derate);
     tc_plan_destroy(decoder);
     for (void* p : {(void*)d_tbs, (void*)d_cbs, (void*)d_pow24a, (void*)d_pow24b, (void*)d_soft, (void*)d_post,
-                    (void*)d_cb_iters, (void*)d_tx, (void*)d_valid, (void*)d_streams, (void*)d_tb_crc})
+                    (void*)d_cb_iters, (void*)d_tx, (void*)d_valid, (void*)d_streams, (void*)d_tb_crc,
+                    (void*)d_map, (void*)d_acc})
       if (p) (void)hipFree(p);
   }
 };
@@ -149,6 +155,7 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
   std::vector<tcdev::TbCb> cbs;
   std::vector<uint32_t> rvs;
   std::vector<tcdev::TxCb> tx;
+  std::vector<tcdev::TbCbMap> map;
   std::vector<uint32_t> valid;
   struct ValidInfo {
     uint32_t off, n, start[4];
@@ -225,6 +232,7 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
       x.n_valid = it->second.n;
       std::copy(it->second.start, it->second.start + 4, x.start);
       tx.push_back(x);
+      map.push_back({static_cast<uint32_t>(t), cursor, x.payload, x.seg});
       cursor += x.payload;
       d_bytes += 3ull * (k + 4);
       rm_bits += e[r];
@@ -261,6 +269,9 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
   dev_upload(p.d_valid, valid);
   p.d_streams = dev_alloc<uint8_t>(d_bytes);
   p.d_tb_crc = dev_alloc<uint32_t>(n_tb);
+  p.d_map = dev_alloc<tcdev::TbCbMap>(map.size());
+  dev_upload(p.d_map, map);
+  p.d_acc = dev_alloc<uint32_t>(3 * n_tb);
 }
 
 void run_encode(tc_tb_plan& p, const uint8_t* payload, int32_t rv, uint8_t* rm, uint8_t* cb_bits, hipStream_t s) {
@@ -291,6 +302,34 @@ void run_tb(tc_tb_plan& p, uint32_t n_tx, const float* const* rx, const int32_t*
   if (accumulate && !p.hold)
     throw Failure{TC_ERR_INVALID_ARGUMENT, "the plan does not hold its decoder input (tc_tb_plan_set_hold)"};
   const bool exact = p.exact;
+  if (!exact) {
+    // window decoders: de-rate matching in the input staging, TB assembly in the epilogue
+    if (p.n_tb > 0x7FFFFFFFull / 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "too many transport blocks"};
+    const uint32_t nacc = static_cast<uint32_t>(3 * p.n_tb);
+    hipLaunchKernelGGL(tcdev::tb_acc_zero_kernel, dim3((nacc + 255) / 256), dim3(256), 0, s, p.d_acc, nacc);
+    hip_check(hipGetLastError(), "accumulator clear launch");
+    tcdev::TbOut tb{1, p.max_iterations, p.d_tbs, p.d_map, p.d_pow24a, p.d_pow24b, bits, tb_ok, cb_ok, iters,
+                    p.d_acc};
+    tcdev::RxIn x{};
+    if (n_tx) {
+      if (n_tx > static_cast<uint32_t>(tcdev::kMaxTx)) throw Failure{TC_ERR_INVALID_ARGUMENT, "n_tx must be 1..4"};
+      for (uint32_t t = 0; t < static_cast<uint32_t>(tcdev::kMaxTx); ++t) {
+        const int32_t r = (rv && t < n_tx) ? rv[t] : -1;
+        x.rv[t] = r < 0 ? tcdev::kBlockRv : static_cast<uint32_t>(r);
+        x.rx[t] = t < n_tx ? rx[t] : nullptr;
+        if (t < n_tx && !x.rx[t]) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+      }
+      x.n_tx = n_tx;
+      x.base = accumulate ? 1 : 0;
+      x.keep = p.hold ? p.d_soft : nullptr;
+      x.rank = p.derate->d_rank;
+      x.dcb = p.derate->d_cbs;
+    }
+    if (!tchost::launch_window(p.decoder, n_tx ? nullptr : p.d_soft, post_buf, nullptr, p.d_cb_iters, s,
+                               n_tx ? &x : nullptr, &tb))
+      throw Failure{TC_ERR_RUNTIME, "window decoder expected"};
+    return;
+  }
   if (n_tx)
     check_status(tc_plan_decode_rx(p.decoder, p.derate, n_tx, rx, rv, accumulate,
                                    (p.hold || exact) ? p.d_soft : nullptr, post_buf, nullptr, p.d_cb_iters, s));

@@ -138,4 +138,11 @@ __global__ __launch_bounds__(kTbThreads) void tb_assemble_kernel(
   }
 }
 
+// Clears the per-TB accumulators of the decoder-epilogue assembly (a kernel,
+// so a captured graph records no memset).
+__global__ void tb_acc_zero_kernel(uint32_t* acc, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) acc[i] = 0u;
+}
+
 }  // namespace tcdev

@@ -44,6 +44,29 @@ struct TbCb {
 static_assert(sizeof(TbCb) == 16, "TbCb layout");
 
 constexpr int kTbThreads = 256;
+
+// Transport-block assembly done by the window decoder after a block's last
+// iteration (the TB plan's fast path): where the block's payload goes in its
+// TB and which CRCs it carries.
+struct TbCbMap {
+  uint32_t tb;     // transport block index
+  uint32_t pos;    // first payload bit of the block within its TB (with CRC24A)
+  uint32_t np;     // bits the block contributes to the TB (K - F - 24 * seg)
+  uint32_t seg;    // 1: the block ends in a CRC24B
+};
+struct TbOut {
+  int32_t on;
+  uint32_t max_iterations;
+  const TbInfo* tbs;
+  const TbCbMap* map;
+  const uint32_t* pow24a;
+  const uint32_t* pow24b;
+  uint8_t* bits;
+  int32_t* tb_ok;
+  int32_t* cb_ok;
+  uint32_t* tb_iters;
+  uint32_t* acc;  // per TB: CRC24A remainder (XOR), blocks done, max iterations
+};
 // D^m mod G(D) tables: CRC24B for m < 6144 (a code block), CRC24A up to the
 // longest transport block of a plan.
 constexpr uint32_t kCrc24APoly = 0xC199BFu;  // lte_turbo.cpp:104-107, D^24 implicit

@@ -584,6 +584,52 @@ __device__ __forceinline__ void rx_batch(const RxIn& X, const RxBlk& B, uint64_t
     out[u] = ((valid_mask >> u) & 1u) ? rx_from_rank(X, B, keep, pos[u], r[u]) : 0.0f;
 }
 
+// Transport-block assembly of one decoded block (lte_turbo.cpp:958-979), run
+// by its wave right after the block's posteriors are written: hard decisions
+// (LLR >= 0), filler strip, CRC24B over the block when it carries one, the
+// block's share of its TB's CRC24A (XOR of D^(len-1-p) mod G over its ones at
+// TB positions p), payload bits out.  Per-TB partial remainders, block counts
+// and iteration maxima meet in acc[] through atomics; the last block of a TB
+// finalises it.
+__device__ __forceinline__ void tb_epilogue(const WinArgs& A, uint32_t cb, const float* post, uint32_t K,
+                                            uint32_t filler) {
+  const TbOut& T = A.tb;
+  const uint32_t lane = threadIdx.x;
+  const TbCbMap m = T.map[cb];
+  const TbInfo t = T.tbs[m.tb];
+  const uint32_t n = K - filler, len = t.b + 24;
+  uint32_t cbx = 0, tbx = 0;
+  for (uint32_t q = lane; q < n; q += kWaveLanes) {
+    const bool one = post[filler + q] >= 0.0f;
+    if (one && m.seg) cbx ^= T.pow24b[n - 1 - q];
+    if (q < m.np) {
+      const uint32_t p = m.pos + q;
+      if (one) tbx ^= T.pow24a[len - 1 - p];
+      if (T.bits && p < t.b) T.bits[t.bits_off + p] = one ? 1 : 0;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    cbx ^= __shfl_xor(cbx, o, 64);
+    tbx ^= __shfl_xor(tbx, o, 64);
+  }
+  if (lane == 0) {
+    const bool cb_pass = m.seg ? (cbx == 0 && n >= 24) : true;
+    if (T.cb_ok) T.cb_ok[cb] = cb_pass ? 1 : 0;
+    uint32_t* acc = T.acc + 3 * m.tb;
+    if (t.cb_crc) tbx = cb_pass ? 0u : 1u;  // a code-block-CRC descriptor is checked by its CRC24B
+    atomicXor(acc, tbx);
+    atomicMax(acc + 2, A.iters ? A.iters[cb] : 0u);
+    __threadfence();
+    if (atomicAdd(acc + 1, 1u) == t.c - 1) {  // last block of this TB
+      __threadfence();
+      const uint32_t rem = atomicOr(acc, 0u), itmax = atomicOr(acc + 2, 0u);
+      if (T.tb_ok) T.tb_ok[m.tb] = rem == 0 ? 1 : 0;
+      if (T.tb_iters) T.tb_iters[m.tb] = T.max_iterations ? itmax : 0u;
+    }
+  }
+  (void)len;
+}
+
 template <class D, bool ES>
 __device__ __forceinline__ void window_decode(const WinArgs& A) {
   extern __shared__ float lds[];
@@ -788,6 +834,15 @@ __device__ __forceinline__ void window_decode(const WinArgs& A) {
     if (L.active && L.first && !es.done) {
       if (A.crc_ok) A.crc_ok[my_cb] = 0;
       if (A.iters) A.iters[my_cb] = used_it;
+    }
+    if (A.tb.on) {
+      __threadfence_block();  // this wave's posterior and iteration stores before it reads them back
+      __syncthreads();
+      for (uint32_t bb = 0; bb < J.nblk; ++bb) {
+        const uint32_t cbb = cb_index[J.cb_off + bb];
+        const CbInfo cbi = cbs[cbb];
+        tb_epilogue(A, cbb, A.post + cbi.post_off, K, cbi.filler);
+      }
     }
     __syncthreads();
   }
