@@ -64,6 +64,9 @@ typedef struct tc_segment_info {
   uint32_t segments;       /* M = K / S: lanes per code block */
   uint32_t blocks_per_wave;/* code blocks packed into one 64-lane wavefront */
   uint32_t warmup;         /* effective warm-up steps (0 when M == 1) */
+  uint32_t extrinsic_bits; /* width of the extrinsics held between
+                              half-iterations: 16 (fp16, saturated at
+                              +-65504, round to nearest even) */
 } tc_segment_info;
 
 const char* tc_last_error(void);

@@ -78,9 +78,11 @@ def lib():
                                    _f32p, C.POINTER(C.c_int), C.POINTER(C.c_uint32)]
         L.or_decode_cb.restype = C.c_int
         L.or_decode_window.argtypes = [_f32p, _f32p, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
-                                       C.c_uint32, C.c_uint32, C.c_int, _f32p, C.POINTER(C.c_int),
-                                       C.POINTER(C.c_uint32)]
+                                       C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, _f32p,
+                                       C.POINTER(C.c_int), C.POINTER(C.c_uint32)]
         L.or_decode_window.restype = C.c_int
+        L.or_round_half.argtypes = [C.c_float]
+        L.or_round_half.restype = C.c_float
         L.or_brute_app.argtypes = [_f32p, _f32p, _f32p, C.c_uint32, _f32p, _f32p, _f64p]
         _lib = L
     return _lib
@@ -247,13 +249,22 @@ def decode_cb(d0, d1, d2, k, filler=0, max_iterations=8, early_stop=False):
 
 
 def decode_window(d0, d1, d2, k, seg_len, warmup=0, linear=False, filler=0, max_iterations=8,
-                  early_stop=False):
+                  early_stop=False, ex_bits=32):
+    """The windowed decoder (or_decode_window); ex_bits=16 holds the
+    extrinsics between half-iterations as fp16, as the GPU does for long
+    segments (segment_geometry()["extrinsic_bits"])."""
     post = np.zeros(k, np.float32)
     ok, it = C.c_int(0), C.c_uint32(0)
     if lib().or_decode_window(_f32(d0), _f32(d1), _f32(d2), k, filler, max_iterations, int(early_stop),
-                              seg_len, warmup, int(linear), post, C.byref(ok), C.byref(it)) != 0:
-        raise ValueError("bad K / segment length")
+                              seg_len, warmup, int(linear), ex_bits, post, C.byref(ok), C.byref(it)) != 0:
+        raise ValueError("bad K / segment length / extrinsic width")
     return post, bool(ok.value), it.value
+
+
+def round_half(x):
+    """or_round_half over an array (float32 in, float32 out)."""
+    f = lib().or_round_half
+    return np.array([f(float(v)) for v in np.asarray(x, np.float32).ravel()], np.float32)
 
 
 def brute_app(sys, parity, apriori, tail_sys, tail_par):

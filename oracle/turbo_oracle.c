@@ -762,18 +762,39 @@ static void seg_siso(const float* in_sa, const float* in_par, int len, int ga, i
   }
 }
 
+float or_round_half(float x) {
+  if (x > 65504.0f) x = 65504.0f;
+  if (x < -65504.0f) x = -65504.0f;
+  const float ax = fabsf(x);
+  if (ax < 6.103515625e-05f) { /* fp16 subnormals: multiples of 2^-24 */
+    const float q = rintf(ax * 16777216.0f) * 5.9604644775390625e-08f;
+    return copysignf(q, x);
+  }
+  /* keep 10 of the 23 fraction bits, ties to even; a carry moves into the
+   * exponent as it should */
+  uint32_t u;
+  memcpy(&u, &x, sizeof u);
+  u += 0x0FFFu + ((u >> 13) & 1u);
+  u &= ~0x1FFFu;
+  float r;
+  memcpy(&r, &u, sizeof r);
+  return r;
+}
+
 int or_decode_window(const float* d0, const float* d1, const float* d2, uint32_t k,
                      uint32_t filler, uint32_t max_iterations, int early_stop,
-                     uint32_t seg_len, uint32_t warmup, int lin, float* post, int* crc_ok,
-                     uint32_t* iterations_used) {
+                     uint32_t seg_len, uint32_t warmup, int lin, uint32_t ex_bits, float* post,
+                     int* crc_ok, uint32_t* iterations_used) {
   if (or_qpp_index(k) < 0 || seg_len == 0 || k % seg_len != 0) return -1;
+  if (ex_bits != 32 && ex_bits != 16) return -1;
   if (k / seg_len > 1 && warmup >= seg_len) return -1; /* warm-up stays inside the neighbour */
   const uint32_t nseg = k / seg_len;
   const int g = (nseg > 1) ? (int)warmup : 0;
   uint32_t* perm = (uint32_t*)malloc(sizeof(uint32_t) * k);
   or_qpp_perm(k, perm);
-  float* e = (float*)calloc(3 * (size_t)k, sizeof(float)); /* E1 | E2 | ext */
-  float *e1 = e, *e2 = e + k, *ex = e + 2 * k;
+  /* E1 | E2 as held between halves | ext | E2 as computed */
+  float* e = (float*)calloc(4 * (size_t)k, sizeof(float));
+  float *e1 = e, *e2 = e + k, *ex = e + 2 * k, *e2raw = e + 3 * (size_t)k;
   float* sa = (float*)malloc(sizeof(float) * k);
   float* si = (float*)malloc(sizeof(float) * k);
   float* bst = (float*)malloc(sizeof(float) * 8 * (seg_len + 1));
@@ -821,13 +842,18 @@ int or_decode_window(const float* d0, const float* d1, const float* d2, uint32_t
         if (m > 0) memcpy(nab + ((size_t)(h * 2 + 1) * nseg + m - 1) * 8, bexp, sizeof bexp);
       }
       memcpy(ab, nab, sizeof(float) * nb);
+      /* the held extrinsics feed the next half; the posterior takes the
+       * fresh lower extrinsic */
       if (h == 0) {
-        memcpy(e1, ex, sizeof(float) * k);
+        for (uint32_t i = 0; i < k; ++i) e1[i] = (ex_bits == 16) ? or_round_half(ex[i]) : ex[i];
       } else {
-        for (uint32_t j = 0; j < k; ++j) e2[perm[j]] = ex[j];
+        for (uint32_t j = 0; j < k; ++j) {
+          e2[perm[j]] = (ex_bits == 16) ? or_round_half(ex[j]) : ex[j];
+          e2raw[perm[j]] = ex[j];
+        }
       }
     }
-    for (uint32_t i = 0; i < k; ++i) post[i] = d0[i] + e1[i] + e2[i];
+    for (uint32_t i = 0; i < k; ++i) post[i] = d0[i] + e1[i] + e2raw[i];
     if (iterations_used) *iterations_used = it + 1;
     if (early_stop) {
       ok = cb_crc_passes(post, k, filler, bits);

@@ -34,6 +34,9 @@
  *   or_decode_window    the decoder this build runs on the GPU in windowed
  *                       mode; pinned by equality with or_decode when there is
  *                       one window per code block, and by BER.
+ *   or_round_half       equality with numpy's float32 -> float16 -> float32
+ *                       conversion (IEEE round to nearest even) on normal,
+ *                       subnormal, tie and saturating inputs.
  */
 #ifndef TURBO_ORACLE_H
 #define TURBO_ORACLE_H
@@ -92,11 +95,16 @@ int or_decode_cb(const float* d0, const float* d1, const float* d2, uint32_t k,
                  uint32_t filler, uint32_t max_iterations, int early_stop,
                  float* posterior, int* crc_ok, uint32_t* iterations_used);
 
-/* Windowed decoder used by the GPU batched path (see turbo_oracle.c). */
+/* Windowed decoder used by the GPU batched path (see turbo_oracle.c).
+ * ex_bits: 32, or 16 to hold the extrinsics between half-iterations as fp16. */
 int or_decode_window(const float* d0, const float* d1, const float* d2, uint32_t k,
                      uint32_t filler, uint32_t max_iterations, int early_stop,
-                     uint32_t seg_len, uint32_t warmup, int linear_domain,
+                     uint32_t seg_len, uint32_t warmup, int linear_domain, uint32_t ex_bits,
                      float* posterior, int* crc_ok, uint32_t* iterations_used);
+
+/* x saturated at +-65504 and rounded to the nearest fp16 value (ties to
+ * even, fp16 subnormals kept), returned as a float. */
+float or_round_half(float x);
 
 /* Exact APP of the upper constituent code by enumerating 2^K inputs (K<=16). */
 void or_brute_app(const float* sys, const float* parity, const float* apriori, uint32_t k,

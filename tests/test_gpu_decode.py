@@ -72,6 +72,21 @@ def test_exact_batched_equals_single_calls():
 
 
 # ------------------------------------------------------------ window decoders
+def _window_bar(geo, ref):
+    """(atol, rtol, hard_tol) of window decoder vs oracle.  exp2/log2
+    rounding (v_exp_f32/v_log_f32 against libm) alone leaves ~1e-4 relative;
+    with extrinsics held as fp16, such a difference can move a held value
+    across an fp16 rounding boundary, i.e. by one fp16 step (2^-11 of the
+    held value, not of the bit's own LLR), which the following
+    half-iterations spread to neighbouring bits: the absolute bar scales
+    with the largest LLR of the block."""
+    assert geo["extrinsic_bits"] in (16, 32)
+    if geo["extrinsic_bits"] == 32:
+        return 1e-3, 1e-4, 1e-3
+    a = 1e-3 * max(1.0, float(np.abs(ref).max()))
+    return a, 2e-3, a
+
+
 WINDOW_CASES = [(6144, 0.5), (6144, 1.0), (1024, 0.5), (40, 0.5), (1952, 0.75), (6080, 0.25)]
 
 
@@ -80,7 +95,8 @@ WINDOW_CASES = [(6144, 0.5), (6144, 1.0), (1024, 0.5), (40, 0.5), (1952, 0.75), 
 def test_window_decoder_matches_oracle(k, ebn0, decoder):
     """Same algorithm on both sides (segments, warm-up, NII, arithmetic);
     they differ only in exp2/log2 rounding (v_exp_f32/v_log_f32 vs libm).
-    Bar: |dLLR| <= 1e-3 + 1e-4 |LLR| and equal hard decisions where |LLR| >= 1e-3."""
+    Bar (_window_bar), extrinsics held as fp16: 1e-3 max|LLR| + 2e-3 |LLR|,
+    hard decisions equal where |LLR| >= 1e-3 max|LLR|."""
     geo = T.segment_geometry(k, decoder)
     _say(f"k={k} ebn0={ebn0} decoder={decoder} geometry={geo}")
     rng = np.random.default_rng(k + int(100 * ebn0))
@@ -88,20 +104,29 @@ def test_window_decoder_matches_oracle(k, ebn0, decoder):
     post, _, its = T.decode_batch_host([k] * 3, llr, decoder, 8)
     for i in range(3):
         ref = O.decode_window(*_split(llr[i], k), k, geo["segment_length"], geo["warmup"],
-                              decoder == T.DECODER_WINDOW_LIN, 0, 8, False)[0]
-        assert_llr_parity(post[i * k:(i + 1) * k], ref, atol=1e-3, rtol=1e-4)
+                              decoder == T.DECODER_WINDOW_LIN, 0, 8, False, geo["extrinsic_bits"])[0]
+        atol, rtol, hard = _window_bar(geo, ref)
+        assert_llr_parity(post[i * k:(i + 1) * k], ref, atol=atol, rtol=rtol, hard_tol=hard)
     assert list(its) == [8, 8, 8]
 
 
 @pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])
 def test_window_single_segment_matches_reference(decoder):
-    """K=40 and K=56 use one segment per block (M=1): no boundaries, so the
-    window decoder is a full-length MAP decoder and must match the reference's
-    posteriors up to rounding."""
+    """K=40 uses one segment per block (M=1): no boundaries, so the window
+    decoder is a full-length MAP decoder.  It matches the oracle's restatement
+    with fp16-held extrinsics at the window bar, and the reference's
+    posteriors to the fp16 rounding of the held extrinsics (the oracle with
+    32-bit extrinsics matches the reference to 2e-4: test_oracle)."""
     g = golden()
-    llr = g["dec_k40_eb100_it6_llr"]
-    post, _, _ = T.decode_batch_host([40], llr.reshape(-1), decoder, 6)
-    assert_llr_parity(post, g["dec_k40_eb100_it6_post"], atol=2e-4, rtol=2e-5)
+    llr = g["dec_k40_eb100_it6_llr"].reshape(-1)
+    geo = T.segment_geometry(40, decoder)
+    assert geo["segments"] == 1 and geo["extrinsic_bits"] == 16
+    post, _, _ = T.decode_batch_host([40], llr, decoder, 6)
+    ref = O.decode_window(*_split(llr, 40), 40, 40, 0, decoder == T.DECODER_WINDOW_LIN, 0, 6, False, 16)[0]
+    atol, rtol, hard = _window_bar(geo, ref)
+    assert_llr_parity(post, ref, atol=atol, rtol=rtol, hard_tol=hard)
+    gold = g["dec_k40_eb100_it6_post"]
+    assert_llr_parity(post, gold, atol=2e-3 * np.abs(gold).max(), rtol=2e-3, hard_tol=2e-3 * np.abs(gold).max())
 
 
 def test_window_batched_mixed_sizes_equal_single():
@@ -250,10 +275,12 @@ def test_window_early_stop_matches_oracle(decoder):
     lin = decoder == T.DECODER_WINDOW_LIN
     for i, (k, f, l) in enumerate(zip(ks, fs, llrs)):
         geo = T.segment_geometry(k, decoder)
-        ref, ok, used = O.decode_window(*_split(l, k), k, geo["segment_length"], geo["warmup"], lin, f, 8, True)
+        ref, ok, used = O.decode_window(*_split(l, k), k, geo["segment_length"], geo["warmup"], lin, f, 8, True,
+                                        geo["extrinsic_bits"])
         _say(f"block {i}: K={k} F={f} gpu it={its[i]} crc={crc[i]}  oracle it={used} crc={ok}")
         assert (int(its[i]), bool(crc[i])) == (used, ok)
-        assert_llr_parity(post[off:off + k], ref, atol=1e-3, rtol=1e-4)
+        atol, rtol, hard = _window_bar(geo, ref)
+        assert_llr_parity(post[off:off + k], ref, atol=atol, rtol=rtol, hard_tol=hard)
         off += k
     assert crc.sum() >= 6 and not crc[2]
 

@@ -366,12 +366,51 @@ def test_window_single_segment_equals_exact(k, lin):
     np.testing.assert_allclose(win, exact, rtol=2e-5, atol=2e-4)
 
 
+def test_round_half_equals_ieee_conversion():
+    """or_round_half against numpy's float16 conversion (IEEE 754 round to
+    nearest even): random normals across the fp16 range, fp16 subnormals,
+    exact ties between neighbouring fp16 values, and saturation."""
+    rng = np.random.default_rng(11)
+    x = np.concatenate([
+        rng.standard_normal(20000).astype(np.float32) * np.float32(10.0) ** rng.integers(-6, 5, 20000),
+        rng.uniform(-6.2e-5, 6.2e-5, 5000).astype(np.float32),
+        np.float32([0.0, -0.0, 65504.0, -65504.0, 65519.0, 1e9, -1e9, 6.1035156e-05, 5.96e-08, 2.98e-08]),
+    ]).astype(np.float32)
+    # exact midpoints between consecutive fp16 values (ties)
+    h = rng.integers(1, 0x7BFF, 4000).astype(np.uint16).view(np.float16).astype(np.float32)
+    nxt = np.nextafter(h.astype(np.float16), np.float16(np.inf)).astype(np.float32)
+    x = np.concatenate([x, (h + nxt) / 2, -(h + nxt) / 2]).astype(np.float32)
+    want = np.clip(x, -65504, 65504).astype(np.float16).astype(np.float32)
+    got = O.round_half(x)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_window_half_extrinsics_close_to_float():
+    """Holding the extrinsics as fp16 moves posteriors by about the fp16
+    rounding of what is held (relative 2^-11), not more, and leaves the hard
+    decisions of a converged block unchanged; with 32 bits the argument is
+    a no-op."""
+    k = 6144
+    rng = np.random.default_rng(3)
+    cb = rng.integers(0, 2, k).astype(np.uint8)
+    llr = bpsk_awgn_llr(O.encode_cb(cb), k, 1.0, rng)
+    full = O.decode_window(*llr, k, 96, 16, False, 0, 8, False)[0]
+    same = O.decode_window(*llr, k, 96, 16, False, 0, 8, False, 32)[0]
+    half = O.decode_window(*llr, k, 96, 16, False, 0, 8, False, 16)[0]
+    assert np.array_equal(full.view(np.uint32), same.view(np.uint32))
+    assert not np.array_equal(full, half)
+    assert np.array_equal(full >= 0, half >= 0) and np.array_equal(half >= 0, cb.astype(bool))
+    np.testing.assert_allclose(half, full, rtol=2e-3, atol=2e-2)
+
+
 def test_window_decoder_rejects_bad_geometry():
     z = np.zeros(6148, np.float32)
     with pytest.raises(ValueError):
         O.decode_window(z, z, z, 6144, 100)  # does not divide K
     with pytest.raises(ValueError):
         O.decode_window(z, z, z, 6144, 96, 96)  # warm-up must stay below S
+    with pytest.raises(ValueError):
+        O.decode_window(z, z, z, 6144, 96, 16, ex_bits=8)
 
 
 # ------------------------------------------- differentiable decoder (FP64) --

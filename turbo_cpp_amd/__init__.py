@@ -49,7 +49,8 @@ class TcChannelSeg(C.Structure):
 
 
 class TcSegmentInfo(C.Structure):
-    _fields_ = [(n, C.c_uint32) for n in ("k", "segment_length", "segments", "blocks_per_wave", "warmup")]
+    _fields_ = [(n, C.c_uint32) for n in ("k", "segment_length", "segments", "blocks_per_wave", "warmup",
+                                          "extrinsic_bits")]
 
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
@@ -433,7 +434,7 @@ def segment_geometry(k: int, decoder: int = DECODER_WINDOW_LOG, min_segment: int
     g = TcSegmentInfo()
     _check(lib().tc_segment_geometry(k, C.byref(o), C.byref(g)))
     return dict(k=g.k, segment_length=g.segment_length, segments=g.segments,
-                blocks_per_wave=g.blocks_per_wave, warmup=g.warmup)
+                blocks_per_wave=g.blocks_per_wave, warmup=g.warmup, extrinsic_bits=g.extrinsic_bits)
 
 
 def decode_batch_host(ks: Sequence[int], llr: np.ndarray, decoder: int = DECODER_WINDOW_LOG,

@@ -59,6 +59,7 @@ tc_segment_info geometry(uint32_t k, const tc_options& o) {
   g.segments = k / s;
   g.blocks_per_wave = tcdev::kWaveLanes / g.segments;
   g.warmup = (g.segments > 1) ? std::min(o.warmup, s - 1) : 0;
+  g.extrinsic_bits = tcdev::kExtrinsicBits;
   return g;
 }
 
@@ -87,12 +88,13 @@ struct tc_plan {
   uint32_t* d_qtab = nullptr;
   uint32_t* d_crc_pow = nullptr;
   float2* d_scratch = nullptr;
+  uint32_t* d_sched = nullptr;
   uint32_t njobs = 0, grid = 0;
   size_t lds_bytes = 0;
 
   ~tc_plan() {
     for (void* p : {(void*)d_cbs, (void*)d_work, (void*)d_jobs, (void*)d_cb_index, (void*)d_qtab, (void*)d_crc_pow,
-                    (void*)d_scratch})
+                    (void*)d_scratch, (void*)d_sched})
       if (p) (void)hipFree(p);
   }
 };
@@ -136,7 +138,6 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   for (size_t i = 0; i < n; ++i) by_k[k[i]].push_back(static_cast<uint32_t>(i));
   std::vector<tcdev::WinJob> jobs;
   std::vector<uint32_t> cb_index, qtab;
-  uint32_t max_seg = 1;
   for (const auto& [kk, list] : by_k) {
     const tc_segment_info g = geometry(kk, o);
     const int qi = turbo_cpp::detail::qpp_index(kk);
@@ -148,7 +149,6 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
       const uint32_t r = static_cast<uint32_t>(pi % g.segment_length), c0 = static_cast<uint32_t>(pi / g.segment_length);
       qtab.push_back((r << 16) | c0);
     }
-    max_seg = std::max(max_seg, g.segment_length);
     for (size_t s = 0; s < list.size(); s += g.blocks_per_wave) {
       tcdev::WinJob j{};
       j.k = kk;
@@ -170,7 +170,9 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
     return a.seg + 2 * a.warm > b.seg + 2 * b.warm;
   });
   p.njobs = static_cast<uint32_t>(jobs.size());
-  p.lds_bytes = (static_cast<size_t>(max_seg) * tcdev::kLdsRow + tcdev::kLdsExtraWords(max_seg)) * sizeof(float);
+  p.lds_bytes = 0;
+  for (const auto& j : jobs)
+    p.lds_bytes = std::max<size_t>(p.lds_bytes, static_cast<size_t>(tcdev::job_lds_bytes(static_cast<int>(j.seg))));
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
   int per_cu = 0;
@@ -187,6 +189,8 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   p.d_qtab = dev_alloc<uint32_t>(qtab.size());
   dev_upload(p.d_qtab, qtab);
   p.d_scratch = dev_alloc<float2>(static_cast<size_t>(p.grid) * tcdev::kSlotFloat2);
+  p.d_sched = dev_alloc<uint32_t>(2);
+  hip_check(hipMemset(p.d_sched, 0, 2 * sizeof(uint32_t)), "hipMemset");
   // D^m mod G(D) for the CRC24B generator (lte_turbo.cpp:108-111)
   std::vector<uint32_t> pw(tcdev::kCrcPowLen);
   uint32_t r = 1;
@@ -213,6 +217,7 @@ void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* i
     if (rx) a.rx = *rx;
     a.tb = tcdev::TbOut{};
     if (tb) a.tb = *tb;
+    a.sched = p.d_sched;
     hipLaunchKernelGGL(window_kernel(o), dim3(p.grid), dim3(64), p.lds_bytes, s, a);
   }
   hip_check(hipGetLastError(), "kernel launch");

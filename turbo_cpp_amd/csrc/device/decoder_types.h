@@ -27,11 +27,13 @@ constexpr int kWaveLanes = 64;
 constexpr int kSubWin = 8;       // W: sub-window of the beta recompute
 constexpr int kMaxSubWins = 24;  // nW <= 24, so S <= 192
 constexpr int kMaxSeg = kSubWin * kMaxSubWins;
-constexpr int kLdsRow = kWaveLanes + 1;  // LDS row stride (odd: conflict-free columns)
+constexpr int kExRow = kWaveLanes;  // LDS extrinsic row: one entry per lane
 // Per-wave scratch slot, in float2: staged channel LLRs for both halves
 // ([t][lane], S <= kMaxSeg), then in floats the boundary metrics
-// [half][16][lane] and the beta checkpoints [w][8][lane].
-constexpr int kSlotFloat2 = 2 * kMaxSeg * kWaveLanes + 16 * kWaveLanes + 4 * kMaxSubWins * kWaveLanes;
+// [lane][half][16], the beta checkpoints [lane][w][8] and the posteriors
+// [t][lane] plus one dummy slot per lane.
+constexpr int kSlotFloat2 = 2 * kMaxSeg * kWaveLanes + 16 * kWaveLanes + 4 * kMaxSubWins * kWaveLanes +
+                            (kMaxSeg + 1) * kWaveLanes / 2;
 
 struct WinJob {
   uint32_t k;
@@ -82,10 +84,19 @@ struct WinArgs {
   int32_t early_stop;
   RxIn rx;  // n_tx == 0: the decoder input is llr
   TbOut tb;  // on == 0: posteriors only
+  // Job queue: [0] next job, [1] waves that found it empty.  Both read zero
+  // at launch; the last wave to leave puts them back to zero, so the kernel
+  // needs no clearing launch (and no memset node under graph capture).
+  uint32_t* sched;
 };
 constexpr int kCrcPowLen = 6144;
-// LDS floats of one window job beyond the extrinsics: hard-decision bitmap
-// ((S*65+31)/32 words) and one CRC partial per lane.
-constexpr int kLdsExtraWords(int s) { return (s * kLdsRow + 31) / 32 + kWaveLanes; }
+// LDS of one window job: fp16 extrinsics [S][64] plus one dummy slot per
+// lane, a hard-decision bitmap over the same index space, and one CRC
+// partial per lane.  At S = 96 this is 13.3 KiB, so LDS does not keep a CU
+// below the eight waves (two per SIMD) that the register budget allows.
+constexpr int job_lds_bytes(int s) {
+  return (s * kExRow + kWaveLanes) * 2 + (s * kExRow + kWaveLanes) / 8 + kWaveLanes * 4;
+}
+constexpr int kExtrinsicBits = 16;
 
 }  // namespace tcdev

@@ -16,8 +16,8 @@
 // wave_shr:1 / wave_shl:1.
 //
 // Memory.  The wavefront's extrinsics live in LDS for the whole decode, one
-// float per bit in a transposed layout: bit i = q*S + t of block b sits at
-// row t, column b*M + q (row stride 65).  The upper half reads and writes
+// fp16 per bit in a transposed layout: bit i = q*S + t of block b sits at
+// row t, column b*M + q (rows of 64).  The upper half reads and writes
 // its own column; the lower half reads and writes bit pi(j) for its steps
 // j = q*S + t.  Because S divides K, pi(q*S + t) mod S = pi(t) mod S for
 // every q (the QPP is contention-free), so at each step all lanes hit one
@@ -27,7 +27,9 @@
 // mod M and B_q = 2 f2 q mod M.  The channel LLRs (d0, d1 for the upper
 // half; d0[pi(j)], d2 for the lower) are staged once per job into a
 // per-wave scratch area in [t][lane] order, so every per-step load of a
-// wavefront is one contiguous 512-byte line.
+// wavefront is one contiguous 512-byte line.  Each lower half leaves the
+// posteriors (fp32) in the same scratch slot at the bits' extrinsic
+// addresses; the job ends with one copy in natural order.
 //
 // Within a segment the backward pass stores beta only at sub-window
 // boundaries (every 8 steps, in the wave's scratch slot); the forward pass
@@ -246,14 +248,21 @@ template <int H>
 __device__ __forceinline__ uint32_t lds_addr(const Lane& L, const uint32_t* __restrict__ qtab, uint32_t qq,
                                              uint32_t t, uint32_t y) {
   if constexpr (H == 0) {
-    return t * kLdsRow + L.colb + qq;
+    return t * kExRow + L.colb + qq;
   } else {
     const uint32_t u = qtab[L.qtab_off + t];
     uint32_t c = (u & 0xFFFFu) + y;
     c = (c >= L.M) ? c - L.M : c;
-    return (u >> 16) * kLdsRow + L.colb + c;
+    return (u >> 16) * kExRow + L.colb + c;
   }
 }
+
+// Extrinsics in LDS are _Float16, saturated at +-65504 and rounded to
+// nearest even (v_cvt_f16_f32 in the default rounding mode).
+__device__ __forceinline__ _Float16 to_ex(float v) {
+  return static_cast<_Float16>(__builtin_fminf(__builtin_fmaxf(v, -65504.0f), 65504.0f));
+}
+
 __device__ __forceinline__ uint32_t y_back(uint32_t y, uint32_t b, uint32_t m) { return (y >= b) ? y - b : y + m - b; }
 __device__ __forceinline__ uint32_t y_fwd(uint32_t y, uint32_t b, uint32_t m) {
   y += b;
@@ -263,37 +272,43 @@ __device__ __forceinline__ uint32_t y_fwd(uint32_t y, uint32_t b, uint32_t m) {
 // Loads of one chunk of up to kSubWin consecutive steps [lo, lo+len) of
 // segment qq: the staged channel pair from the wave's scratch (global) and
 // the extrinsic from LDS.  Each pass issues the next chunk's loads before it
-// computes the current one.
+// computes the current one.  FULL: len == kSubWin, no per-step guards.
 struct Chunk {
   float2 v[kSubWin];
   float e[kSubWin];
 };
 
+typedef __attribute__((address_space(1))) const float GlobalFloat;
+
+template <bool FULL = false>
 __device__ __forceinline__ void load_chunk(const float2* __restrict__ sc, uint32_t col, uint32_t lo, uint32_t len,
                                            Chunk& c) {
-  const float2* base = sc + lo * kWaveLanes + col;
+  const GlobalFloat* base = (const GlobalFloat*)(sc + lo * kWaveLanes + col);
   // Opaque to the optimiser: without this the per-lane 64-bit address of
   // every (chunk, step, column) combination is hoisted out of the loops and
-  // spilled.  One address pair per chunk; steps use immediate offsets.
+  // spilled.  One address pair per chunk; steps use immediate offsets.  The
+  // pointer stays in the global address space: flat loads would also count
+  // against the LDS wait counter, so every LDS wait of the sub-window would
+  // drain the next sub-window's prefetch.
   asm volatile("" : "+v"(base));
 #pragma unroll
   for (int j = 0; j < kSubWin; ++j)
-    if ((uint32_t)j < len) c.v[j] = base[j * kWaveLanes];
+    if (FULL || (uint32_t)j < len) c.v[j] = make_float2(base[2 * j * kWaveLanes], base[2 * j * kWaveLanes + 1]);
 }
 
 // LDS extrinsics of the chunk, issued in the order the pass consumes them;
 // addr (may be null) keeps the addresses for the write-back.
-template <int H, bool BACKWARD>
-__device__ __forceinline__ void load_chunk_lds(const Lane& L, const float* __restrict__ lds,
+template <int H, bool BACKWARD, bool FULL>
+__device__ __forceinline__ void load_chunk_lds(const Lane& L, const _Float16* __restrict__ ex,
                                                const uint32_t* __restrict__ qtab, uint32_t qq, uint32_t by,
                                                uint32_t lo, uint32_t len, Chunk& c, uint32_t* addr) {
   uint32_t y = (H == 1) ? qpp_col0(L, qq, BACKWARD ? lo + len - 1 : lo) : 0u;
 #pragma unroll
   for (int jj = 0; jj < kSubWin; ++jj) {
     const int j = BACKWARD ? kSubWin - 1 - jj : jj;
-    if ((uint32_t)j < len) {
+    if (FULL || (uint32_t)j < len) {
       const uint32_t a = lds_addr<H>(L, qtab, qq, lo + j, y);
-      c.e[j] = lds[a];
+      c.e[j] = static_cast<float>(ex[a]);
       if (addr) addr[j] = a;
       if constexpr (H == 1) y = BACKWARD ? y_back(y, by, L.M) : y_fwd(y, by, L.M);
     }
@@ -302,12 +317,14 @@ __device__ __forceinline__ void load_chunk_lds(const Lane& L, const float* __res
 
 // Backward recursion of beta over steps [lo, hi) of segment qq (chunks of
 // kSubWin aligned at lo, last chunk first).  on_chunk(c, beta) runs before
-// chunk c; on_step(t, beta) after each step t (beta then is beta_t).
-template <class D, int H, class OnChunk, class OnStep>
-__device__ __forceinline__ void beta_pass(const Lane& L, const float* __restrict__ lds,
+// chunk c; on_hand(beta) right after step hand_t (beta then is beta_hand_t;
+// hand_t outside [lo, hi): never).  A chunk of kSubWin steps that does not
+// hold hand_t strictly inside runs without per-step guards.
+template <class D, int H, class OnChunk, class OnHand>
+__device__ __forceinline__ void beta_pass(const Lane& L, const _Float16* __restrict__ ex,
                                           const float2* __restrict__ sc, const uint32_t* __restrict__ qtab,
                                           uint32_t qq, uint32_t col, uint32_t lo, uint32_t hi, float* bc,
-                                          OnChunk&& on_chunk, OnStep&& on_step) {
+                                          uint32_t hand_t, OnChunk&& on_chunk, OnHand&& on_hand) {
   if (hi <= lo) return;
   const uint32_t n = (hi - lo + kSubWin - 1) / kSubWin;
   const uint32_t by = (H == 1) ? qpp_stride(L, qq) : 0u;
@@ -315,95 +332,66 @@ __device__ __forceinline__ void beta_pass(const Lane& L, const float* __restrict
   {
     const uint32_t c0 = lo + (n - 1) * kSubWin;
     load_chunk(sc, col, c0, hi - c0, nxt);
-    load_chunk_lds<H, true>(L, lds, qtab, qq, by, c0, hi - c0, nxt, nullptr);
+    load_chunk_lds<H, true, false>(L, ex, qtab, qq, by, c0, hi - c0, nxt, nullptr);
   }
   for (int c = (int)n - 1; c >= 0; --c) {
     const uint32_t clo = lo + c * kSubWin;
     const uint32_t len = min((uint32_t)kSubWin, hi - clo);
     const Chunk cur = nxt;
     if (c > 0) {
-      load_chunk(sc, col, clo - kSubWin, kSubWin, nxt);
-      load_chunk_lds<H, true>(L, lds, qtab, qq, by, clo - kSubWin, kSubWin, nxt, nullptr);
+      load_chunk<true>(sc, col, clo - kSubWin, kSubWin, nxt);
+      load_chunk_lds<H, true, true>(L, ex, qtab, qq, by, clo - kSubWin, kSubWin, nxt, nullptr);
     }
     on_chunk((uint32_t)c, bc);
+    if (len == kSubWin && (hand_t <= clo || hand_t >= clo + kSubWin)) {
 #pragma unroll
-    for (int j = kSubWin - 1; j >= 0; --j) {
-      if ((uint32_t)j < len) {
+      for (int j = kSubWin - 1; j >= 0; --j) {
         __builtin_amdgcn_sched_barrier(0);
         D::bstep(bc, D::branch(cur.v[j].x + cur.e[j], cur.v[j].y));
-        on_step(clo + j, bc);
+      }
+      if (hand_t == clo) on_hand(bc);
+    } else {
+#pragma unroll
+      for (int j = kSubWin - 1; j >= 0; --j) {
+        if ((uint32_t)j < len) {
+          __builtin_amdgcn_sched_barrier(0);
+          D::bstep(bc, D::branch(cur.v[j].x + cur.e[j], cur.v[j].y));
+          if (clo + j == hand_t) on_hand(bc);
+        }
       }
     }
   }
 }
 
-// CRC24B early stop (lte_turbo.cpp:792-804) state of this lane's block.
-// With on set, every lower half writes the posterior of each bit to HBM and
-// its hard decision to an LDS bitmap indexed like the extrinsics; a block
-// whose check passed is done and its lanes stop writing.
-struct EarlyStop {
-  bool on;
-  bool done;
-  float* post_cb;
-  uint32_t* bitmap;
-};
-constexpr uint32_t kDiv65Magic = 66076420u;  // ceil(2^32 / 65): exact a / 65 for a < 2^16
-static_assert(kLdsRow == 65, "kDiv65Magic assumes 65-float LDS rows");
-
-// One SISO half (H = 0 upper, 1 lower) for this lane's segment.
-// nii: this lane's boundary metrics of this half, [alpha 8 | beta 8]
-// (alpha at step -g, beta at step S+g of the segment); ck: this lane's beta
-// checkpoints at sub-window boundaries, [w][8].  The neighbours' records
-// are reached with nii_stride (floats between consecutive lanes' records).
-template <class D, int H, bool ES>
-__device__ __forceinline__ void siso_half(const Lane& L, float* __restrict__ lds, const float2* __restrict__ sc,
-                                          const uint32_t* __restrict__ qtab, float* __restrict__ nii,
-                                          float* __restrict__ ck, bool write_post, const EarlyStop& es) {
-  const uint32_t S = L.S, g = L.g;
-  // ---------------- backward pass: warm-up over the next segment's first g
-  // steps, then the segment, storing beta at sub-window boundaries
-  float bc[8];
-  load8(nii + 8, bc);
-  if (g > 0) {
-    const uint32_t qr = L.last ? L.q : L.q + 1;
-    const uint32_t col = L.last ? L.le : L.le + 1;
-    beta_pass<D, H>(L, lds, sc, qtab, qr, col, 0, g, bc, [](uint32_t, float*) {}, [](uint32_t, const float*) {});
-    if (L.last) load8(nii + 8, bc);
-  }
-  beta_pass<D, H>(
-      L, lds, sc, qtab, L.q, L.le, 0, S, bc,
-      [&](uint32_t w, float* b) { store8(ck + 8 * w, b); },
-      [&](uint32_t t, const float* b) {
-        // t is uniform: the whole wave hands beta at step g to the previous
-        // segment (used next iteration); the last segment keeps its tail beta
-        if (g > 0 && t == g) {
-          float nb[8];
+// Forward recursion of alpha without extrinsics (the warm-up) over steps
+// [lo, hi) of segment qq, chunks of kSubWin aligned at lo.
+template <class D, int H>
+__device__ __forceinline__ void alpha_warmup(const Lane& L, const _Float16* __restrict__ ex,
+                                             const float2* __restrict__ sc, const uint32_t* __restrict__ qtab,
+                                             uint32_t qq, uint32_t col, uint32_t lo, uint32_t hi, float* ac) {
+  if (hi <= lo) return;
+  const uint32_t by = (H == 1) ? qpp_stride(L, qq) : 0u;
+  const uint32_t n = (hi - lo + kSubWin - 1) / kSubWin;
+  Chunk nxt;
+  load_chunk(sc, col, lo, min((uint32_t)kSubWin, hi - lo), nxt);
+  load_chunk_lds<H, false, false>(L, ex, qtab, qq, by, lo, min((uint32_t)kSubWin, hi - lo), nxt, nullptr);
+  for (uint32_t c = 0; c < n; ++c) {
+    const uint32_t clo = lo + c * kSubWin;
+    const uint32_t len = min((uint32_t)kSubWin, hi - clo);
+    const Chunk cur = nxt;
+    if (c + 1 < n) {
+      const uint32_t nlen = min((uint32_t)kSubWin, hi - clo - kSubWin);
+      load_chunk(sc, col, clo + kSubWin, nlen, nxt);
+      load_chunk_lds<H, false, false>(L, ex, qtab, qq, by, clo + kSubWin, nlen, nxt, nullptr);
+    }
+    if (len == kSubWin) {
 #pragma unroll
-          for (int i = 0; i < 8; ++i) nb[i] = dpp_from_next_lane(b[i]);
-          if (!L.last) store8(nii + 8, nb);
-        }
-      });
-  // ---------------- forward pass: warm-up over the previous segment's last g
-  // steps, then per sub-window: recompute beta, alpha + extrinsic
-  float ac[8];
-  load8(nii, ac);
-  if (g > 0) {
-    const uint32_t ql = L.first ? L.q : L.q - 1;
-    const uint32_t col = L.first ? L.le : L.le - 1;
-    const uint32_t by = (H == 1) ? qpp_stride(L, ql) : 0u;
-    const uint32_t n = (g + kSubWin - 1) / kSubWin;
-    Chunk nxt;
-    load_chunk(sc, col, S - g, min((uint32_t)kSubWin, g), nxt);
-    load_chunk_lds<H, false>(L, lds, qtab, ql, by, S - g, min((uint32_t)kSubWin, g), nxt, nullptr);
-    for (uint32_t c = 0; c < n; ++c) {
-      const uint32_t clo = S - g + c * kSubWin;
-      const uint32_t len = min((uint32_t)kSubWin, S - clo);
-      const Chunk cur = nxt;
-      if (c + 1 < n) {
-        const uint32_t nlen = min((uint32_t)kSubWin, S - clo - kSubWin);
-        load_chunk(sc, col, clo + kSubWin, nlen, nxt);
-        load_chunk_lds<H, false>(L, lds, qtab, ql, by, clo + kSubWin, nlen, nxt, nullptr);
+      for (int j = 0; j < kSubWin; ++j) {
+        __builtin_amdgcn_sched_barrier(0);
+        const float sa = cur.v[j].x + cur.e[j];
+        D::template astep<false>(ac, D::branch(sa, cur.v[j].y), sa, nullptr);
       }
+    } else {
 #pragma unroll
       for (int j = 0; j < kSubWin; ++j) {
         if ((uint32_t)j < len) {
@@ -413,9 +401,120 @@ __device__ __forceinline__ void siso_half(const Lane& L, float* __restrict__ lds
         }
       }
     }
+  }
+}
+
+// CRC24B early stop (lte_turbo.cpp:792-804) state of this lane's block.  A
+// block whose check passed is done: its lanes stop writing (their stores go
+// to per-lane dummy slots), so its posteriors stay those of the iteration
+// that passed.
+struct EarlyStop {
+  bool on;
+  bool done;
+  uint32_t* bitmap;  // hard decisions, indexed like the extrinsics
+};
+static_assert(kExRow == 64, "extrinsic address = row * 64 + column");
+
+// Sub-window w of the forward pass: recompute beta from the checkpoint bb,
+// then alpha + extrinsic per step.  Writes go to index wa(j) = the bit's
+// extrinsic address, or a per-lane dummy slot for lanes that must not write:
+// the fp16 extrinsic to LDS (upper half: E1 of the bit; lower: E2) and, in
+// the lower half, the fp32 posterior to post_sc.
+template <class D, int H, bool ES, bool FULL>
+__device__ __forceinline__ void alpha_subwindow(const Lane& L, _Float16* __restrict__ ex, const Chunk& cur,
+                                                const uint32_t* addr, float* bb, float* ac, uint32_t t0,
+                                                uint32_t len, uint32_t hand_t, float* __restrict__ nii,
+                                                uint32_t dummy, bool wr, float* __restrict__ post_sc,
+                                                const EarlyStop& es) {
+  float bs[kSubWin][8];
+  typename D::Br br[kSubWin];
+  float isa[kSubWin];
+#pragma unroll
+  for (int j = kSubWin - 1; j >= 0; --j) {
+    if (FULL || (uint32_t)j < len) {
+      isa[j] = cur.v[j].x + cur.e[j];
+      br[j] = D::branch(isa[j], cur.v[j].y);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) bs[j][i] = bb[i];
+      __builtin_amdgcn_sched_barrier(0);
+      if (j > 0) D::bstep(bb, br[j]);
+    }
+  }
+  auto hand = [&]() {
+    // uniform: hand alpha at step S-g to the next segment
+    float na[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) na[i] = dpp_from_prev_lane(ac[i]);
+    if (!L.first) store8(nii, na);
+  };
+  if (FULL && hand_t == t0) hand();
+#pragma unroll
+  for (int j = 0; j < kSubWin; ++j) {
+    if (FULL || (uint32_t)j < len) {
+      if (!FULL && t0 + j == hand_t) hand();
+      __builtin_amdgcn_sched_barrier(0);  // keep steps in order: bounds register pressure
+      const float ext = D::template astep<true>(ac, br[j], isa[j], bs[j]);
+      const uint32_t a = wr ? addr[j] : dummy;
+      ex[a] = to_ex(ext);
+      if constexpr (H == 1) {
+        const float p = isa[j] + ext;
+        post_sc[a] = p;
+        if constexpr (ES) {
+          if (es.on && p >= 0.0f) atomicOr(&es.bitmap[a >> 5], 1u << (a & 31));
+        }
+      }
+    }
+  }
+}
+
+// One SISO half (H = 0 upper, 1 lower) for this lane's segment.
+// nii: this lane's boundary metrics of this half, [alpha 8 | beta 8]
+// (alpha at step -g, beta at step S+g of the segment); ck: this lane's beta
+// checkpoints at sub-window boundaries, [w][8].  The lower half also leaves
+// each bit's posterior in post_sc (fp32, at the bit's extrinsic address).
+template <class D, int H, bool ES>
+__device__ __forceinline__ void siso_half(const Lane& L, _Float16* __restrict__ ex, const float2* __restrict__ sc,
+                                          const uint32_t* __restrict__ qtab, float* __restrict__ nii,
+                                          float* __restrict__ ck, float* __restrict__ post_sc,
+                                          const EarlyStop& es) {
+  const uint32_t S = L.S, g = L.g;
+  const uint32_t lane = threadIdx.x;
+  constexpr uint32_t kNever = 0xFFFFFFFFu;
+  // ---------------- backward pass: warm-up over the next segment's first g
+  // steps, then the segment, storing beta at sub-window boundaries
+  float bc[8];
+  load8(nii + 8, bc);
+  if (g > 0) {
+    const uint32_t qr = L.last ? L.q : L.q + 1;
+    const uint32_t col = L.last ? L.le : L.le + 1;
+    beta_pass<D, H>(L, ex, sc, qtab, qr, col, 0, g, bc, kNever, [](uint32_t, float*) {}, [](const float*) {});
+    if (L.last) load8(nii + 8, bc);
+  }
+  beta_pass<D, H>(
+      L, ex, sc, qtab, L.q, L.le, 0, S, bc, g > 0 ? g : kNever,
+      [&](uint32_t w, float* b) { store8(ck + 8 * w, b); },
+      [&](const float* b) {
+        // the whole wave hands beta at step g to the previous segment (used
+        // next iteration); the last segment keeps its tail beta
+        float nb[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) nb[i] = dpp_from_next_lane(b[i]);
+        if (!L.last) store8(nii + 8, nb);
+      });
+  // ---------------- forward pass: warm-up over the previous segment's last g
+  // steps, then per sub-window: recompute beta, alpha + extrinsic
+  float ac[8];
+  load8(nii, ac);
+  if (g > 0) {
+    const uint32_t ql = L.first ? L.q : L.q - 1;
+    const uint32_t col = L.first ? L.le : L.le - 1;
+    alpha_warmup<D, H>(L, ex, sc, qtab, ql, col, S - g, S, ac);
     if (L.first) load8(nii, ac);
   }
   const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
+  const uint32_t hand_t = g > 0 ? S - g : kNever;
+  const uint32_t dummy = S * kExRow + lane;
+  const bool wr = L.active && !es.done;
   // global loads (staged LLRs, beta checkpoint) run one sub-window ahead;
   // the LDS extrinsics of a sub-window are read at its start
   Chunk nxt;
@@ -436,49 +535,12 @@ __device__ __forceinline__ void siso_half(const Lane& L, float* __restrict__ lds
       load8(ck + 8 * (w + 1), nck);
     }
     uint32_t addr[kSubWin];
-    load_chunk_lds<H, true>(L, lds, qtab, L.q, by, t0, len, cur, addr);
-    float bs[kSubWin][8];
-    typename D::Br br[kSubWin];
-    float isa[kSubWin];
-#pragma unroll
-    for (int j = kSubWin - 1; j >= 0; --j) {
-      if ((uint32_t)j < len) {
-        isa[j] = cur.v[j].x + cur.e[j];
-        br[j] = D::branch(isa[j], cur.v[j].y);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) bs[j][i] = bb[i];
-        __builtin_amdgcn_sched_barrier(0);
-        if (j > 0) D::bstep(bb, br[j]);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kSubWin; ++j) {
-      if ((uint32_t)j < len) {
-        const uint32_t t = t0 + j;
-        if (g > 0 && t == S - g) {
-          // uniform: hand alpha at step S-g to the next segment
-          float na[8];
-#pragma unroll
-          for (int i = 0; i < 8; ++i) na[i] = dpp_from_prev_lane(ac[i]);
-          if (!L.first) store8(nii, na);
-        }
-        __builtin_amdgcn_sched_barrier(0);  // keep steps in order: bounds register pressure
-        const float ext = D::template astep<true>(ac, br[j], isa[j], bs[j]);
-        const float val = (H == 1 && write_post) ? isa[j] + ext : ext;
-        if (L.active && !es.done) {
-          lds[addr[j]] = val;
-          if constexpr (H == 1 && ES) {
-            if (es.on) {
-              // posterior of bit pi(j) to HBM, its hard decision into the bitmap
-              const float p = isa[j] + ext;
-              const uint32_t row = __umulhi(addr[j], kDiv65Magic);
-              const uint32_t col = addr[j] - row * kLdsRow;
-              es.post_cb[(col - L.colb) * S + row] = p;
-              if (p >= 0.0f) atomicOr(&es.bitmap[addr[j] >> 5], 1u << (addr[j] & 31));
-            }
-          }
-        }
-      }
+    if (len == kSubWin && (hand_t <= t0 || hand_t >= t0 + kSubWin)) {
+      load_chunk_lds<H, true, true>(L, ex, qtab, L.q, by, t0, len, cur, addr);
+      alpha_subwindow<D, H, ES, true>(L, ex, cur, addr, bb, ac, t0, len, hand_t, nii, dummy, wr, post_sc, es);
+    } else {
+      load_chunk_lds<H, true, false>(L, ex, qtab, L.q, by, t0, len, cur, addr);
+      alpha_subwindow<D, H, ES, false>(L, ex, cur, addr, bb, ac, t0, len, hand_t, nii, dummy, wr, post_sc, es);
     }
   }
 }
@@ -495,7 +557,7 @@ __device__ __forceinline__ bool crc24b_job(const Lane& L, const uint32_t* __rest
   uint32_t r = crc_pow[K - 1 - L.q * S];
   for (uint32_t t = 0; t < S; ++t) {
     const uint32_t i = L.q * S + t;
-    const uint32_t a = t * kLdsRow + L.colb + L.q;
+    const uint32_t a = t * kExRow + L.colb + L.q;
     if (i >= filler && ((bitmap[a >> 5] >> (a & 31)) & 1u)) x ^= r;
     r = (r & 1u) ? ((r ^ (0xC00063u | (1u << 24))) >> 1) : (r >> 1);
   }
@@ -570,17 +632,18 @@ __device__ __forceinline__ float chan_in(const WinArgs& A, const RxBlk& B, uint6
   return v;
 }
 
-// kStageBatch positions at once: all rank loads, then all value loads, so a
-// lane keeps that many gathers in flight while staging.
-constexpr uint32_t kStageBatch = 8;
-__device__ __forceinline__ void rx_batch(const RxIn& X, const RxBlk& B, uint64_t llr_off, const uint32_t (&pos)[kStageBatch],
-                                         uint32_t valid_mask, float (&out)[kStageBatch]) {
+// kStageSteps steps of the three streams at once: all rank loads, then all
+// value loads, so a lane keeps that many gathers in flight while staging.
+constexpr uint32_t kStageSteps = 4;
+template <uint32_t N>
+__device__ __forceinline__ void rx_batch(const RxIn& X, const RxBlk& B, uint64_t llr_off, const uint32_t (&pos)[N],
+                                         uint32_t valid_mask, float (&out)[N]) {
   const float* keep = X.keep ? X.keep + llr_off : nullptr;
-  uint32_t r[kStageBatch];
+  uint32_t r[N];
 #pragma unroll
-  for (uint32_t u = 0; u < kStageBatch; ++u) r[u] = ((valid_mask >> u) & 1u) ? B.rank[pos[u]] : kNullRank;
+  for (uint32_t u = 0; u < N; ++u) r[u] = ((valid_mask >> u) & 1u) ? B.rank[pos[u]] : kNullRank;
 #pragma unroll
-  for (uint32_t u = 0; u < kStageBatch; ++u)
+  for (uint32_t u = 0; u < N; ++u)
     out[u] = ((valid_mask >> u) & 1u) ? rx_from_rank(X, B, keep, pos[u], r[u]) : 0.0f;
 }
 
@@ -630,222 +693,228 @@ __device__ __forceinline__ void tb_epilogue(const WinArgs& A, uint32_t cb, const
   (void)len;
 }
 
+// Jobs are taken from a counter rather than by a fixed stride: waves that
+// share a SIMD run slower than waves alone on one, and early-stopped blocks
+// finish early, so a fixed assignment leaves the fast waves idle at the end.
+__device__ __forceinline__ uint32_t next_job(uint32_t* sched) {
+  uint32_t j = 0;
+  if (threadIdx.x == 0) j = atomicAdd(sched, 1u);
+  return __builtin_amdgcn_readfirstlane(j);
+}
+// Every wave has taken its last (empty) ticket before it counts itself out,
+// so the last one out can reset the queue for the next launch.
+__device__ __forceinline__ void leave_queue(uint32_t* sched) {
+  if (threadIdx.x == 0 && atomicAdd(sched + 1, 1u) == gridDim.x - 1) {
+    atomicExch(sched, 0u);
+    atomicExch(sched + 1, 0u);
+  }
+}
+
+// One job: stage its inputs, run every iteration, write its outputs.
 template <class D, bool ES>
-__device__ __forceinline__ void window_decode(const WinArgs& A) {
-  extern __shared__ float lds[];
+__device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _Float16* __restrict__ ex,
+                                           float2* __restrict__ sc_up, float2* __restrict__ sc_lo,
+                                           float* __restrict__ nii, float* __restrict__ ck,
+                                           float* __restrict__ post_sc) {
   const uint32_t lane = threadIdx.x;
   const uint32_t* __restrict__ cb_index = A.cb_index;
   const CbInfo* __restrict__ cbs = A.cbs;
   const uint32_t* __restrict__ qtab = A.qtab;
-  const float* __restrict__ llr = A.llr;
-  float2* sc_up = A.scratch + (size_t)blockIdx.x * kSlotFloat2;
-  float2* sc_lo = sc_up + kMaxSeg * kWaveLanes;
-  // per-lane records: boundary metrics [lane][half][16], checkpoints [lane][w][8]
-  float* nii = reinterpret_cast<float*>(sc_lo + kMaxSeg * kWaveLanes) + lane * 32;
-  float* ck = reinterpret_cast<float*>(sc_lo + kMaxSeg * kWaveLanes) + kWaveLanes * 32 + lane * (8 * kMaxSubWins);
   const bool es_on = ES && A.max_iterations > 0;
-  for (uint32_t jb = blockIdx.x; jb < A.njobs; jb += gridDim.x) {
-    const WinJob J = A.jobs[jb];
-    Lane L;
-    L.S = J.seg;
-    L.M = J.nseg;
-    L.g = J.warm;
-    L.K = J.k;
-    L.nW = (J.seg + kSubWin - 1) / kSubWin;
-    L.f1 = J.f1;
-    L.f2 = J.f2;
-    L.qtab_off = J.qtab_off;
-    const uint32_t used = J.nblk * J.nseg;
-    L.active = lane < used;
-    L.le = L.active ? lane : used - 1;  // idle lanes shadow the last busy lane
-    const uint32_t b = L.le / L.M;
-    L.q = L.le - b * L.M;
-    L.colb = b * L.M;
-    L.first = L.q == 0;
-    L.last = L.q == L.M - 1;
-    const uint32_t K = J.k, S = J.seg, M = J.nseg;
-    const uint32_t my_cb = cb_index[J.cb_off + b];
-    const CbInfo ci = cbs[my_cb];
-    const bool fused = A.rx.n_tx > 0;
-    // LDS: extrinsics [S][65] floats, then the hard-decision bitmap and the
-    // CRC partials (early stop only)
-    uint32_t* bitmap = reinterpret_cast<uint32_t*>(lds + S * kLdsRow);
-    const uint32_t bitmap_words = (S * kLdsRow + 31) / 32;
-    uint32_t* partial = bitmap + bitmap_words;
+  Lane L;
+  L.S = J.seg;
+  L.M = J.nseg;
+  L.g = J.warm;
+  L.K = J.k;
+  L.nW = (J.seg + kSubWin - 1) / kSubWin;
+  L.f1 = J.f1;
+  L.f2 = J.f2;
+  L.qtab_off = J.qtab_off;
+  const uint32_t used = J.nblk * J.nseg;
+  L.active = lane < used;
+  L.le = L.active ? lane : used - 1;  // idle lanes shadow the last busy lane
+  const uint32_t b = L.le / L.M;
+  L.q = L.le - b * L.M;
+  L.colb = b * L.M;
+  L.first = L.q == 0;
+  L.last = L.q == L.M - 1;
+  const uint32_t K = J.k, S = J.seg, M = J.nseg;
+  const uint32_t my_cb = cb_index[J.cb_off + b];
+  const CbInfo ci = cbs[my_cb];
+  const bool fused = A.rx.n_tx > 0;
+  // LDS: extrinsics [S][64] + one dummy slot per lane (fp16), then the
+  // hard-decision bitmap over the same index space and the CRC partials
+  // (early stop only)
+  const uint32_t ex_slots = S * kExRow + kWaveLanes;
+  uint32_t* bitmap = reinterpret_cast<uint32_t*>(ex + ex_slots);
+  const uint32_t bitmap_words = ex_slots / 32;
+  uint32_t* partial = bitmap + bitmap_words;
+  float* up_f = reinterpret_cast<float*>(sc_up);
+  float* lo_f = reinterpret_cast<float*>(sc_lo);
+  RxBlk Bm{};
+  if (fused) rx_block(A.rx, my_cb, Bm);
+  const uint32_t D4 = K + 4;
 
-    // ---- stage d0 of every block into LDS (transposed) ----
-    for (uint32_t bb = 0; bb < J.nblk; ++bb) {
-      const uint32_t cbb = cb_index[J.cb_off + bb];
-      const uint64_t off = cbs[cbb].llr_off;
-      if (!fused) {
-        for (uint32_t i = lane; i < K; i += kWaveLanes) {
-          const uint32_t qq = __umulhi(i, J.div_magic);
-          const uint32_t t = i - qq * S;
-          lds[t * kLdsRow + bb * M + qq] = llr[off + i];
-        }
-        continue;
+  // ---- this lane's segment of the decoder input into the wave's scratch in
+  // [t][lane] order: (d0, d1) in sc_up, d2 in sc_lo.y.  Idle lanes (shadowing
+  // the last busy lane) neither read nor write the held input.
+  {
+    float* keep = (fused && A.rx.keep) ? A.rx.keep + ci.llr_off : nullptr;
+    for (uint32_t t0 = 0; t0 < S; t0 += kStageSteps) {
+      constexpr uint32_t N = 3 * kStageSteps;
+      uint32_t pos[N], mask = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < kStageSteps; ++u) {
+        const uint32_t i = L.q * S + t0 + u;
+        pos[3 * u] = i;
+        pos[3 * u + 1] = D4 + i;
+        pos[3 * u + 2] = 2 * D4 + i;
+        mask |= ((L.active && t0 + u < S) ? 7u : 0u) << (3 * u);
       }
-      RxBlk B{};
-      rx_block(A.rx, cbb, B);
-      for (uint32_t base = 0; base < K; base += kStageBatch * kWaveLanes) {
-        uint32_t pos[kStageBatch], mask = 0;
+      float v[N];
+      if (fused) {
+        rx_batch(A.rx, Bm, ci.llr_off, pos, mask, v);
+      } else {
 #pragma unroll
-        for (uint32_t u = 0; u < kStageBatch; ++u) {
-          pos[u] = base + u * kWaveLanes + lane;
-          mask |= (pos[u] < K ? 1u : 0u) << u;
-        }
-        float v[kStageBatch];
-        rx_batch(A.rx, B, off, pos, mask, v);
+        for (uint32_t u = 0; u < N; ++u) v[u] = ((mask >> u) & 1u) ? A.llr[ci.llr_off + pos[u]] : 0.0f;
+      }
 #pragma unroll
-        for (uint32_t u = 0; u < kStageBatch; ++u) {
-          if (!((mask >> u) & 1u)) continue;
-          const uint32_t i = pos[u];
-          const uint32_t qq = __umulhi(i, J.div_magic);
-          const uint32_t t = i - qq * S;
-          lds[t * kLdsRow + bb * M + qq] = v[u];
-          if (A.rx.keep) A.rx.keep[off + i] = v[u];
+      for (uint32_t u = 0; u < kStageSteps; ++u) {
+        const uint32_t t = t0 + u;
+        if (((mask >> (3 * u)) & 1u) == 0) continue;
+        sc_up[t * kWaveLanes + lane] = make_float2(v[3 * u], v[3 * u + 1]);
+        lo_f[2 * (t * kWaveLanes + lane) + 1] = v[3 * u + 2];
+        if (keep) {
+          keep[pos[3 * u]] = v[3 * u];
+          keep[pos[3 * u + 1]] = v[3 * u + 1];
+          keep[pos[3 * u + 2]] = v[3 * u + 2];
         }
       }
     }
-    __syncthreads();
-    RxBlk Bm{};
-    if (fused) rx_block(A.rx, my_cb, Bm);
-    const uint32_t D4 = K + 4;
-    // ---- per-wave scratch in [t][lane] order: (d0, d1) and (d0[pi], d2) ----
-    {
-      uint32_t y = qpp_col0(L, L.q, 0);
-      const uint32_t by = qpp_stride(L, L.q);
-      if (fused) {
-        // parities of this lane's segment, kStageBatch steps at a time; idle
-        // lanes (shadowing the last busy lane) neither read nor write the held input
-        for (uint32_t t0 = 0; t0 < S; t0 += kStageBatch / 2) {
-          uint32_t pos[kStageBatch], mask = 0;
+  }
+  // ---- boundary metrics: alpha_0 / tail beta / uniform ----
+  {
+    float tb[2][8];
+    float tl[3][4];  // d0, d1, d2 at K .. K+3
+    // decoder tail read-out of lte_turbo.cpp:758-773
 #pragma unroll
-          for (uint32_t u = 0; u < kStageBatch / 2; ++u) {
-            const uint32_t i = L.q * S + t0 + u;
-            pos[2 * u] = D4 + i;
-            pos[2 * u + 1] = 2 * D4 + i;
-            const uint32_t ok = (L.active && t0 + u < S) ? 3u : 0u;
-            mask |= ok << (2 * u);
-          }
-          float v[kStageBatch];
-          rx_batch(A.rx, Bm, ci.llr_off, pos, mask, v);
+    for (uint32_t sgm = 0; sgm < 3; ++sgm)
 #pragma unroll
-          for (uint32_t u = 0; u < kStageBatch / 2; ++u) {
-            const uint32_t t = t0 + u;
-            if (t >= S) break;
-            const float xs = lds[t * kLdsRow + L.colb + L.q];
-            const float xp = lds[lds_addr<1>(L, qtab, L.q, t, y)];
-            if (L.active) {
-              sc_up[t * kWaveLanes + lane] = make_float2(xs, v[2 * u]);
-              sc_lo[t * kWaveLanes + lane] = make_float2(xp, v[2 * u + 1]);
-              if (A.rx.keep) {
-                A.rx.keep[ci.llr_off + pos[2 * u]] = v[2 * u];
-                A.rx.keep[ci.llr_off + pos[2 * u + 1]] = v[2 * u + 1];
-              }
-            }
-            y = y_fwd(y, by, M);
-          }
-        }
-      } else {
-        const float* d1 = llr + ci.llr_off + D4;
-        const float* d2 = d1 + D4;
-        for (uint32_t t = 0; t < S; ++t) {
-          const float xs = lds[t * kLdsRow + L.colb + L.q];
-          const float xp = lds[lds_addr<1>(L, qtab, L.q, t, y)];
-          const uint32_t i = L.q * S + t;
-          const float p1 = d1[i], p2 = d2[i];
-          if (L.active) {
-            sc_up[t * kWaveLanes + lane] = make_float2(xs, p1);
-            sc_lo[t * kWaveLanes + lane] = make_float2(xp, p2);
-          }
+      for (uint32_t j = 0; j < 4; ++j) tl[sgm][j] = chan_in(A, Bm, ci.llr_off, sgm * D4 + K + j, false);
+    tail_beta(tl[0][0], tl[0][2], 0.0f, tl[1][0], tl[0][1], tl[1][2], tb[0]);
+    tail_beta(tl[1][3], tl[2][0], tl[2][2], tl[2][1], tl[0][3], tl[2][3], tb[1]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        nii[h * 16 + s] = L.first ? (s == 0 ? D::kUniform : D::kZero) : D::kUniform;
+        nii[h * 16 + 8 + s] = L.last ? D::from_log(tb[h][s]) : D::kUniform;
+      }
+    }
+  }
+  __syncthreads();  // every lane's d0 in sc_up before the gather below
+  if (fused && A.rx.keep && L.active && L.first) {
+    // every lane of the block has read the tails; one recomputes and writes them
+    for (uint32_t sgm = 0; sgm < 3; ++sgm)
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t pos = sgm * D4 + K + j;
+        A.rx.keep[ci.llr_off + pos] = chan_in(A, Bm, ci.llr_off, pos, false);
+      }
+  }
+  // ---- interleaved systematic: sc_lo.x at step t = d0[pi(q S + t)], which
+  // sits in sc_up at the extrinsic address of that bit ----
+  {
+    uint32_t y = qpp_col0(L, L.q, 0);
+    const uint32_t by = qpp_stride(L, L.q);
+    for (uint32_t t0 = 0; t0 < S; t0 += kSubWin) {
+      float xp[kSubWin];
+#pragma unroll
+      for (uint32_t u = 0; u < kSubWin; ++u) {
+        if (t0 + u < S) {
+          xp[u] = up_f[2 * lds_addr<1>(L, qtab, L.q, t0 + u, y)];
           y = y_fwd(y, by, M);
         }
       }
-    }
-    // ---- boundary metrics: alpha_0 / tail beta / uniform ----
-    {
-      float tb[2][8];
-      float tl[3][4];  // d0, d1, d2 at K .. K+3
-      // decoder tail read-out of lte_turbo.cpp:758-773
 #pragma unroll
-      for (uint32_t sgm = 0; sgm < 3; ++sgm)
-#pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) tl[sgm][j] = chan_in(A, Bm, ci.llr_off, sgm * D4 + K + j, false);
-      tail_beta(tl[0][0], tl[0][2], 0.0f, tl[1][0], tl[0][1], tl[1][2], tb[0]);
-      tail_beta(tl[1][3], tl[2][0], tl[2][2], tl[2][1], tl[0][3], tl[2][3], tb[1]);
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-          nii[h * 16 + s] = L.first ? (s == 0 ? D::kUniform : D::kZero) : D::kUniform;
-          nii[h * 16 + 8 + s] = L.last ? D::from_log(tb[h][s]) : D::kUniform;
-        }
-      }
+      for (uint32_t u = 0; u < kSubWin; ++u)
+        if (t0 + u < S && L.active) lo_f[2 * ((t0 + u) * kWaveLanes + lane)] = xp[u];
     }
-    __syncthreads();  // scratch rows visible to neighbour lanes; d0 staging done
-    if (fused && A.rx.keep && L.active && L.first) {
-      // every lane of the block has read the tails; one recomputes and writes them
-      for (uint32_t sgm = 0; sgm < 3; ++sgm)
-        for (uint32_t j = 0; j < 4; ++j) {
-          const uint32_t pos = sgm * D4 + K + j;
-          A.rx.keep[ci.llr_off + pos] = chan_in(A, Bm, ci.llr_off, pos, false);
-        }
-    }
-    for (uint32_t i = lane; i < S * kLdsRow; i += kWaveLanes) lds[i] = 0.0f;
-    __syncthreads();
-
-    EarlyStop es{es_on, false, A.post + ci.post_off, bitmap};
-    uint32_t used_it = A.max_iterations;
-    for (uint32_t it = 0; it < A.max_iterations; ++it) {
-      siso_half<D, 0, ES>(L, lds, sc_up, qtab, nii, ck, false, es);
-      __syncthreads();
-      if (es_on) {
-        for (uint32_t i = lane; i < bitmap_words; i += kWaveLanes) bitmap[i] = 0u;
-        __syncthreads();
-      }
-      siso_half<D, 1, ES>(L, lds, sc_lo, qtab, nii + 16, ck, !es_on && it + 1 == A.max_iterations, es);
-      __syncthreads();
-      if (es_on) {
-        const bool ok = crc24b_job(L, bitmap, partial, A.crc_pow, ci.filler);
-        if (ok && !es.done) {
-          es.done = true;
-          used_it = it + 1;
-          if (L.active && L.first) {
-            if (A.crc_ok) A.crc_ok[my_cb] = 1;
-            if (A.iters) A.iters[my_cb] = it + 1;
-          }
-        }
-        if (__syncthreads_and(es.done || !L.active)) break;
-      }
-    }
-
-    if (!es_on) {
-      // ---- posterior out, natural order ----
-      for (uint32_t bb = 0; bb < J.nblk; ++bb) {
-        float* dst = A.post + cbs[cb_index[J.cb_off + bb]].post_off;
-        for (uint32_t i = lane; i < K; i += kWaveLanes) {
-          const uint32_t qq = __umulhi(i, J.div_magic);
-          const uint32_t t = i - qq * S;
-          dst[i] = lds[t * kLdsRow + bb * M + qq];
-        }
-      }
-    }
-    if (L.active && L.first && !es.done) {
-      if (A.crc_ok) A.crc_ok[my_cb] = 0;
-      if (A.iters) A.iters[my_cb] = used_it;
-    }
-    if (A.tb.on) {
-      __threadfence_block();  // this wave's posterior and iteration stores before it reads them back
-      __syncthreads();
-      for (uint32_t bb = 0; bb < J.nblk; ++bb) {
-        const uint32_t cbb = cb_index[J.cb_off + bb];
-        const CbInfo cbi = cbs[cbb];
-        tb_epilogue(A, cbb, A.post + cbi.post_off, K, cbi.filler);
-      }
-    }
-    __syncthreads();
   }
+  {
+    uint32_t* exw = reinterpret_cast<uint32_t*>(ex);
+    for (uint32_t i = lane; i < ex_slots / 2; i += kWaveLanes) exw[i] = 0u;
+  }
+  __syncthreads();
+
+  EarlyStop es{es_on, false, bitmap};
+  uint32_t used_it = A.max_iterations;
+  for (uint32_t it = 0; it < A.max_iterations; ++it) {
+    siso_half<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
+    __syncthreads();
+    if (es_on) {
+      for (uint32_t i = lane; i < bitmap_words; i += kWaveLanes) bitmap[i] = 0u;
+      __syncthreads();
+    }
+    // every lower half leaves the posteriors in post_sc; a block that has
+    // stopped no longer overwrites its own
+    siso_half<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
+    __syncthreads();
+    if (es_on) {
+      const bool ok = crc24b_job(L, bitmap, partial, A.crc_pow, ci.filler);
+      if (ok && !es.done) {
+        es.done = true;
+        used_it = it + 1;
+        if (L.active && L.first) {
+          if (A.crc_ok) A.crc_ok[my_cb] = 1;
+          if (A.iters) A.iters[my_cb] = it + 1;
+        }
+      }
+      if (__syncthreads_and(es.done || !L.active)) break;
+    }
+  }
+
+  // ---- posterior out, natural order (zero when no iteration ran) ----
+  for (uint32_t bb = 0; bb < J.nblk; ++bb) {
+    float* dst = A.post + cbs[cb_index[J.cb_off + bb]].post_off;
+    for (uint32_t i = lane; i < K; i += kWaveLanes) {
+      const uint32_t qq = __umulhi(i, J.div_magic);
+      const uint32_t t = i - qq * S;
+      dst[i] = A.max_iterations ? post_sc[t * kExRow + bb * M + qq] : 0.0f;
+    }
+  }
+  if (L.active && L.first && !es.done) {
+    if (A.crc_ok) A.crc_ok[my_cb] = 0;
+    if (A.iters) A.iters[my_cb] = used_it;
+  }
+  if (A.tb.on) {
+    __threadfence_block();  // this wave's posterior and iteration stores before it reads them back
+    __syncthreads();
+    for (uint32_t bb = 0; bb < J.nblk; ++bb) {
+      const uint32_t cbb = cb_index[J.cb_off + bb];
+      const CbInfo cbi = cbs[cbb];
+      tb_epilogue(A, cbb, A.post + cbi.post_off, K, cbi.filler);
+    }
+  }
+  __syncthreads();
+}
+
+template <class D, bool ES>
+__device__ __forceinline__ void window_decode(const WinArgs& A) {
+  extern __shared__ float lds[];
+  const uint32_t lane = threadIdx.x;
+  float2* sc_up = A.scratch + (size_t)blockIdx.x * kSlotFloat2;
+  float2* sc_lo = sc_up + kMaxSeg * kWaveLanes;
+  // per-lane records: boundary metrics [lane][half][16], checkpoints [lane][w][8]
+  float* recs = reinterpret_cast<float*>(sc_lo + kMaxSeg * kWaveLanes);
+  float* nii = recs + lane * 32;
+  float* ck = recs + kWaveLanes * 32 + lane * (8 * kMaxSubWins);
+  // posteriors [S][64] + one dummy slot per lane
+  float* post_sc = recs + kWaveLanes * 32 + kWaveLanes * 8 * kMaxSubWins;
+  for (uint32_t jb = next_job(A.sched); jb < A.njobs; jb = next_job(A.sched)) {
+    const WinJob J = A.jobs[jb];
+    window_job<D, ES>(A, J, reinterpret_cast<_Float16*>(lds), sc_up, sc_lo, nii, ck, post_sc);
+  }
+  leave_queue(A.sched);
 }
 
 }  // namespace
