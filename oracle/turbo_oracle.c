@@ -607,7 +607,7 @@ int or_decode_cb(const float* d0, const float* d1, const float* d2, uint32_t k,
  *                x = sa*(log2e/2), y = par*(log2e/2), g1 = x + y, g2 = x - y
  *                (branch metrics +-g1, +-g2),
  *                max*(a,b) = max(a,b) + log2(1 + 2^-|a-b|),
- *                metrics renormalised to a maximum of 0 each step,
+ *                metrics renormalised to state 0's metric each step,
  *                extrinsic = ln2 * (LSE{alpha~(n)+beta(n) : u(n)=1}
  *                          - LSE{...: u(n)=0}) - sa,
  *                LSE(v) = m + log2(sum 2^(v_i-m)).
@@ -641,9 +641,9 @@ static inline float gsel(unsigned u, unsigned c, float g1, float g2) {
   return c ? -g2 : -g1;
 }
 
+/* metrics relative to state 0 */
 static void lg_norm(float* v) {
-  float m = v[0];
-  for (int i = 1; i < 8; ++i) m = fmaxf(m, v[i]);
+  const float m = v[0];
   for (int i = 0; i < 8; ++i) v[i] = v[i] - m;
 }
 

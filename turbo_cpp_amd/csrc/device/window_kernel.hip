@@ -83,6 +83,8 @@ struct LogDomain {
   static constexpr float kUniform = 0.0f;
   static constexpr float kZero = kNeg;
   __device__ static __forceinline__ float from_log(float v) { return v * kLog2e; }
+  // the staged parity LLR, as the branch metrics take it
+  __device__ static __forceinline__ float stage_par(float par) { return par; }
   __device__ static __forceinline__ Br branch(float sa, float par) {
     const float x = sa * kHalfLog2e, y = par * kHalfLog2e;
     return {x + y, x - y};
@@ -90,10 +92,15 @@ struct LogDomain {
   __device__ static __forceinline__ float maxstar(float a, float b) {
     return fmaxf(a, b) + hw_log2(1.0f + hw_exp2(-fabsf(a - b)));
   }
+  // metrics relative to state 0 (any reference state keeps max* exact; the
+  // spread between states stays bounded because every state reaches every
+  // other in three steps); state 0 is then exactly 0, which the next step's
+  // additions fold away
   __device__ static __forceinline__ void norm(float* v) {
-    const float m = max8(v);
+    const float m = v[0];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = v[i] - m;
+    for (int i = 1; i < 8; ++i) v[i] = v[i] - m;
+    v[0] = 0.0f;
   }
   // beta_t from beta_{t+1}
   __device__ static __forceinline__ void bstep(float* b, const Br& r) {
@@ -124,9 +131,9 @@ struct LogDomain {
       // u(n) = 1 for n in {1,2,4,7}, 0 for {0,3,5,6}
       const float m1 = fmaxf(fmaxf(fmaxf(fmaxf(kNeg, tv[1]), tv[2]), tv[4]), tv[7]);
       const float m0 = fmaxf(fmaxf(fmaxf(fmaxf(kNeg, tv[0]), tv[3]), tv[5]), tv[6]);
-      float s1 = 0.0f, s0 = 0.0f;
-      s0 += hw_exp2(tv[0] - m0);
-      s1 += hw_exp2(tv[1] - m1);
+      // 0 + x == x for the non-negative x here: summed as 0.0f + x + ...
+      float s0 = hw_exp2(tv[0] - m0);
+      float s1 = hw_exp2(tv[1] - m1);
       s1 += hw_exp2(tv[2] - m1);
       s0 += hw_exp2(tv[3] - m0);
       s1 += hw_exp2(tv[4] - m1);
@@ -152,20 +159,27 @@ struct LinDomain {
   static constexpr float kUniform = 1.0f;
   static constexpr float kZero = 0.0f;
   __device__ static __forceinline__ float from_log(float v) { return hw_exp2(v * kLog2e); }
-  __device__ static __forceinline__ Br branch(float sa, float par) {
-    const float x = 0.5f * sa, y = 0.5f * par;
-    const float ex = hw_exp2(-fabsf(x + x) * kLog2e);
+  // The parity weight depends on the channel only, so it is staged once per
+  // job: e^-|par| (y = par/2, y + y = par exactly) with the sign bit set
+  // when y < 0; the sign bit, not the value, carries it (the weight may
+  // underflow to zero).
+  __device__ static __forceinline__ float stage_par(float par) {
+    const float y = 0.5f * par;
     const float ey = hw_exp2(-fabsf(y + y) * kLog2e);
-    return {(x >= 0.0f) ? 1.0f : ex, (x >= 0.0f) ? ex : 1.0f, (y >= 0.0f) ? 1.0f : ey, (y >= 0.0f) ? ey : 1.0f};
+    return (y >= 0.0f) ? ey : -ey;
   }
-  // rescale by 2^-e so that the largest entry lies in [0.5, 1)
+  __device__ static __forceinline__ Br branch(float sa, float sp) {
+    const float x = 0.5f * sa;
+    const float ex = hw_exp2(-fabsf(x + x) * kLog2e);
+    const bool yp = (__builtin_bit_cast(uint32_t, sp) >> 31) == 0u;
+    return {(x >= 0.0f) ? 1.0f : ex, (x >= 0.0f) ? ex : 1.0f, yp ? 1.0f : -sp, yp ? sp : 1.0f};
+  }
+  // rescale by 2^-e so that the largest entry lies in [0.5, 1) (no branch:
+  // an all-zero vector has e = 0 and stays as it is)
   __device__ static __forceinline__ void norm(float* v) {
-    const float m = max8(v);
-    if (m > 0.0f) {
-      const int e = __builtin_amdgcn_frexp_expf(m);
+    const int e = __builtin_amdgcn_frexp_expf(max8(v));
 #pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = __builtin_amdgcn_ldexpf(v[i], -e);
-    }
+    for (int i = 0; i < 8; ++i) v[i] = __builtin_amdgcn_ldexpf(v[i], -e);
   }
   __device__ static __forceinline__ void bstep(float* b, const Br& r) {
     const float w[2][2] = {{r.u0 * r.q0, r.u0 * r.q1}, {r.u1 * r.q0, r.u1 * r.q1}};
@@ -260,7 +274,7 @@ __device__ __forceinline__ uint32_t lds_addr(const Lane& L, const uint32_t* __re
 // Extrinsics in LDS are _Float16, saturated at +-65504 and rounded to
 // nearest even (v_cvt_f16_f32 in the default rounding mode).
 __device__ __forceinline__ _Float16 to_ex(float v) {
-  return static_cast<_Float16>(__builtin_fminf(__builtin_fmaxf(v, -65504.0f), 65504.0f));
+  return static_cast<_Float16>(__builtin_amdgcn_fmed3f(v, -65504.0f, 65504.0f));
 }
 
 __device__ __forceinline__ uint32_t y_back(uint32_t y, uint32_t b, uint32_t m) { return (y >= b) ? y - b : y + m - b; }
@@ -782,8 +796,8 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _F
       for (uint32_t u = 0; u < kStageSteps; ++u) {
         const uint32_t t = t0 + u;
         if (((mask >> (3 * u)) & 1u) == 0) continue;
-        sc_up[t * kWaveLanes + lane] = make_float2(v[3 * u], v[3 * u + 1]);
-        lo_f[2 * (t * kWaveLanes + lane) + 1] = v[3 * u + 2];
+        sc_up[t * kWaveLanes + lane] = make_float2(v[3 * u], D::stage_par(v[3 * u + 1]));
+        lo_f[2 * (t * kWaveLanes + lane) + 1] = D::stage_par(v[3 * u + 2]);
         if (keep) {
           keep[pos[3 * u]] = v[3 * u];
           keep[pos[3 * u + 1]] = v[3 * u + 1];
