@@ -648,7 +648,9 @@ __device__ __forceinline__ float chan_in(const WinArgs& A, const RxBlk& B, uint6
 
 // kStageSteps steps of the three streams at once: all rank loads, then all
 // value loads, so a lane keeps that many gathers in flight while staging.
-constexpr uint32_t kStageSteps = 4;
+constexpr uint32_t kStageSteps = 8;
+// posteriors copied out per lane and batch
+constexpr uint32_t kCopyBatch = 16;
 template <uint32_t N>
 __device__ __forceinline__ void rx_batch(const RxIn& X, const RxBlk& B, uint64_t llr_off, const uint32_t (&pos)[N],
                                          uint32_t valid_mask, float (&out)[N]) {
@@ -676,13 +678,30 @@ __device__ __forceinline__ void tb_epilogue(const WinArgs& A, uint32_t cb, const
   const TbInfo t = T.tbs[m.tb];
   const uint32_t n = K - filler, len = t.b + 24;
   uint32_t cbx = 0, tbx = 0;
-  for (uint32_t q = lane; q < n; q += kWaveLanes) {
-    const bool one = post[filler + q] >= 0.0f;
-    if (one && m.seg) cbx ^= T.pow24b[n - 1 - q];
-    if (q < m.np) {
-      const uint32_t p = m.pos + q;
-      if (one) tbx ^= T.pow24a[len - 1 - p];
-      if (T.bits && p < t.b) T.bits[t.bits_off + p] = one ? 1 : 0;
+  // kCopyBatch bits per lane and batch: all loads first, then the XORs
+  for (uint32_t base = 0; base < n; base += kCopyBatch * kWaveLanes) {
+    float pv[kCopyBatch];
+    uint32_t pb[kCopyBatch], pa[kCopyBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kCopyBatch; ++u) {
+      const uint32_t q = base + u * kWaveLanes + lane;
+      const bool in = q < n, in_tb = q < m.np;
+      pv[u] = in ? post[filler + q] : -1.0f;
+      pb[u] = (in && m.seg) ? T.pow24b[n - 1 - q] : 0u;
+      pa[u] = in_tb ? T.pow24a[len - 1 - (m.pos + q)] : 0u;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kCopyBatch; ++u) {
+      const uint32_t q = base + u * kWaveLanes + lane;
+      const bool one = pv[u] >= 0.0f;
+      if (one) {
+        cbx ^= pb[u];
+        tbx ^= pa[u];
+      }
+      if (q < m.np) {
+        const uint32_t p = m.pos + q;
+        if (T.bits && p < t.b) T.bits[t.bits_off + p] = one ? 1 : 0;
+      }
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -887,13 +906,24 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _F
     }
   }
 
-  // ---- posterior out, natural order (zero when no iteration ran) ----
+  // ---- posterior out, natural order (zero when no iteration ran), with
+  // kCopyBatch loads in flight per lane ----
   for (uint32_t bb = 0; bb < J.nblk; ++bb) {
     float* dst = A.post + cbs[cb_index[J.cb_off + bb]].post_off;
-    for (uint32_t i = lane; i < K; i += kWaveLanes) {
-      const uint32_t qq = __umulhi(i, J.div_magic);
-      const uint32_t t = i - qq * S;
-      dst[i] = A.max_iterations ? post_sc[t * kExRow + bb * M + qq] : 0.0f;
+    for (uint32_t base = 0; base < K; base += kCopyBatch * kWaveLanes) {
+      float v[kCopyBatch];
+#pragma unroll
+      for (uint32_t u = 0; u < kCopyBatch; ++u) {
+        const uint32_t i = base + u * kWaveLanes + lane;
+        const uint32_t qq = __umulhi(i, J.div_magic);
+        const uint32_t t = i - qq * S;
+        v[u] = (i < K && A.max_iterations) ? post_sc[t * kExRow + bb * M + qq] : 0.0f;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kCopyBatch; ++u) {
+        const uint32_t i = base + u * kWaveLanes + lane;
+        if (i < K) dst[i] = v[u];
+      }
     }
   }
   if (L.active && L.first && !es.done) {
