@@ -26,8 +26,9 @@
 // part of the column is (A_q + B_q t) mod M with A_q = (f1 q + f2 q^2 S)
 // mod M and B_q = 2 f2 q mod M.  The channel LLRs (d0, d1 for the upper
 // half; d0[pi(j)], d2 for the lower) are staged once per job into a
-// per-wave scratch area in [t][lane] order, so every per-step load of a
-// wavefront is one contiguous 512-byte line.  Each lower half leaves the
+// per-wave scratch area in [t][lane] order (one plane of floats per input
+// stream), so every per-step load of a wavefront is one contiguous 256-byte
+// row per plane.  Each lower half leaves the
 // posteriors (fp32) in the same scratch slot at the bits' extrinsic
 // addresses; the job ends with one copy in natural order.
 //
@@ -293,9 +294,12 @@ struct Chunk {
 };
 
 typedef __attribute__((address_space(1))) const float GlobalFloat;
+// A half's staged inputs are two planes of [t][lane] floats, kPlane apart:
+// systematic (+ a-priori source) and parity.
+constexpr uint32_t kPlane = kMaxSeg * kWaveLanes;
 
 template <bool FULL = false>
-__device__ __forceinline__ void load_chunk(const float2* __restrict__ sc, uint32_t col, uint32_t lo, uint32_t len,
+__device__ __forceinline__ void load_chunk(const float* __restrict__ sc, uint32_t col, uint32_t lo, uint32_t len,
                                            Chunk& c) {
   const GlobalFloat* base = (const GlobalFloat*)(sc + lo * kWaveLanes + col);
   // Opaque to the optimiser: without this the per-lane 64-bit address of
@@ -307,7 +311,7 @@ __device__ __forceinline__ void load_chunk(const float2* __restrict__ sc, uint32
   asm volatile("" : "+v"(base));
 #pragma unroll
   for (int j = 0; j < kSubWin; ++j)
-    if (FULL || (uint32_t)j < len) c.v[j] = make_float2(base[2 * j * kWaveLanes], base[2 * j * kWaveLanes + 1]);
+    if (FULL || (uint32_t)j < len) c.v[j] = make_float2(base[j * kWaveLanes], base[kPlane + j * kWaveLanes]);
 }
 
 // LDS extrinsics of the chunk, issued in the order the pass consumes them;
@@ -336,7 +340,7 @@ __device__ __forceinline__ void load_chunk_lds(const Lane& L, const _Float16* __
 // hold hand_t strictly inside runs without per-step guards.
 template <class D, int H, class OnChunk, class OnHand>
 __device__ __forceinline__ void beta_pass(const Lane& L, const _Float16* __restrict__ ex,
-                                          const float2* __restrict__ sc, const uint32_t* __restrict__ qtab,
+                                          const float* __restrict__ sc, const uint32_t* __restrict__ qtab,
                                           uint32_t qq, uint32_t col, uint32_t lo, uint32_t hi, float* bc,
                                           uint32_t hand_t, OnChunk&& on_chunk, OnHand&& on_hand) {
   if (hi <= lo) return;
@@ -381,7 +385,7 @@ __device__ __forceinline__ void beta_pass(const Lane& L, const _Float16* __restr
 // [lo, hi) of segment qq, chunks of kSubWin aligned at lo.
 template <class D, int H>
 __device__ __forceinline__ void alpha_warmup(const Lane& L, const _Float16* __restrict__ ex,
-                                             const float2* __restrict__ sc, const uint32_t* __restrict__ qtab,
+                                             const float* __restrict__ sc, const uint32_t* __restrict__ qtab,
                                              uint32_t qq, uint32_t col, uint32_t lo, uint32_t hi, float* ac) {
   if (hi <= lo) return;
   const uint32_t by = (H == 1) ? qpp_stride(L, qq) : 0u;
@@ -487,7 +491,7 @@ __device__ __forceinline__ void alpha_subwindow(const Lane& L, _Float16* __restr
 // checkpoints at sub-window boundaries, [w][8].  The lower half also leaves
 // each bit's posterior in post_sc (fp32, at the bit's extrinsic address).
 template <class D, int H, bool ES>
-__device__ __forceinline__ void siso_half(const Lane& L, _Float16* __restrict__ ex, const float2* __restrict__ sc,
+__device__ __forceinline__ void siso_half(const Lane& L, _Float16* __restrict__ ex, const float* __restrict__ sc,
                                           const uint32_t* __restrict__ qtab, float* __restrict__ nii,
                                           float* __restrict__ ck, float* __restrict__ post_sc,
                                           const EarlyStop& es) {
@@ -743,10 +747,130 @@ __device__ __forceinline__ void leave_queue(uint32_t* sched) {
   }
 }
 
+// Transposes between a block's natural order and the wave's [t][lane]
+// scratch rows go through the job's LDS (the extrinsic area, idle outside
+// the iterations, (S + 1) * 32 floats with the dummy slots): R = ceil(S/2)
+// rows of 64 floats per pass, column c of row u at u*64 + (c ^ (u mod 64)),
+// so that a run of consecutive rows of one column (the natural order) and a
+// row across the columns are both conflict-free.  Global accesses are then
+// runs of consecutive floats on the natural side and whole rows on the
+// scratch side.
+__device__ __forceinline__ uint32_t tile_at(uint32_t u, uint32_t c) { return u * kWaveLanes + (c ^ (u & 63u)); }
+constexpr uint32_t kTileBatch = 16;
+
+// One-block job (M segments): the decoder input of the block into the scratch
+// planes: d0 and d1 into the upper half's, d2 and d0[pi] (gathered from the
+// tile while it holds d0 rows) into the lower half's; fused de-rate matching
+// when A.rx.n_tx > 0 (then also the held input, written in natural order).
+template <class D>
+__device__ __forceinline__ void stage_tiled(const WinArgs& A, const Lane& L, const CbInfo& ci, const RxBlk& B,
+                                            bool fused, const uint32_t* __restrict__ qtab, float* __restrict__ tile,
+                                            float* __restrict__ up, float* __restrict__ lo) {
+  const uint32_t lane = threadIdx.x, S = L.S, M = L.M, D4 = L.K + 4;
+  const uint32_t R = (S + 1) / 2;
+  float* keep = (fused && A.rx.keep) ? A.rx.keep + ci.llr_off : nullptr;
+  const uint32_t by = qpp_stride(L, L.q);
+  for (uint32_t sgm = 0; sgm < 3; ++sgm) {
+    for (uint32_t r0 = 0; r0 < S; r0 += R) {
+      const uint32_t rp = min(R, S - r0);
+      const uint32_t mg = 0xFFFFFFFFu / rp + 1u;  // ceil(2^32 / rp): exact idx / rp for idx < 2^16
+      const uint32_t n = M * rp;
+      for (uint32_t base = 0; base < n; base += kTileBatch * kWaveLanes) {
+        uint32_t pos[kTileBatch], tix[kTileBatch], mask = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kTileBatch; ++k) {
+          const uint32_t idx = base + k * kWaveLanes + lane;
+          const uint32_t q = __umulhi(idx, mg), u = idx - q * rp;
+          pos[k] = sgm * D4 + q * S + r0 + u;
+          tix[k] = tile_at(u, q);
+          mask |= (idx < n ? 1u : 0u) << k;
+        }
+        float v[kTileBatch];
+        if (fused) {
+          rx_batch(A.rx, B, ci.llr_off, pos, mask, v);
+        } else {
+#pragma unroll
+          for (uint32_t k = 0; k < kTileBatch; ++k) v[k] = ((mask >> k) & 1u) ? A.llr[ci.llr_off + pos[k]] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kTileBatch; ++k) {
+          if ((mask >> k) & 1u) {
+            tile[tix[k]] = v[k];
+            if (keep) keep[pos[k]] = v[k];
+          }
+        }
+      }
+      __syncthreads();
+      // rows r0 .. r0+rp-1 of this lane's column
+      float* dst = (sgm == 0) ? up : (sgm == 1 ? up + kPlane : lo + kPlane);
+      for (uint32_t u0 = 0; u0 < rp; u0 += kSubWin) {
+        float v[kSubWin];
+#pragma unroll
+        for (uint32_t k = 0; k < kSubWin; ++k)
+          if (u0 + k < rp) v[k] = tile[tile_at(u0 + k, lane)];
+#pragma unroll
+        for (uint32_t k = 0; k < kSubWin; ++k) {
+          if (u0 + k < rp && L.active)
+            dst[(r0 + u0 + k) * kWaveLanes + lane] = (sgm == 0) ? v[k] : D::stage_par(v[k]);
+        }
+      }
+      if (sgm == 0) {
+        // lower-half systematic: step t of segment q reads d0 at pi(q S + t),
+        // i.e. row pi(t) mod S (uniform) at this lane's permuted column
+        uint32_t y = qpp_col0(L, L.q, 0);
+        for (uint32_t t = 0; t < S; ++t) {
+          const uint32_t qv = qtab[L.qtab_off + t];
+          const uint32_t row = qv >> 16;
+          if (row >= r0 && row < r0 + rp) {
+            uint32_t c = (qv & 0xFFFFu) + y;
+            c = (c >= M) ? c - M : c;
+            const float x = tile[tile_at(row - r0, c)];
+            if (L.active) lo[t * kWaveLanes + lane] = x;
+          }
+          y = y_fwd(y, by, M);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// One-block job: posteriors from the scratch rows (post_sc) to dst in
+// natural order, zero when no iteration ran.
+__device__ __forceinline__ void copy_out_tiled(const Lane& L, bool any, const float* __restrict__ post_sc,
+                                               float* __restrict__ tile, float* __restrict__ dst) {
+  const uint32_t lane = threadIdx.x, S = L.S, M = L.M;
+  const uint32_t R = (S + 1) / 2;
+  for (uint32_t r0 = 0; r0 < S; r0 += R) {
+    const uint32_t rp = min(R, S - r0);
+    for (uint32_t u0 = 0; u0 < rp; u0 += kTileBatch) {
+      float v[kTileBatch];
+#pragma unroll
+      for (uint32_t k = 0; k < kTileBatch; ++k)
+        if (u0 + k < rp) v[k] = post_sc[(r0 + u0 + k) * kExRow + lane];
+#pragma unroll
+      for (uint32_t k = 0; k < kTileBatch; ++k)
+        if (u0 + k < rp) tile[tile_at(u0 + k, lane)] = v[k];
+    }
+    __syncthreads();
+    const uint32_t mg = 0xFFFFFFFFu / rp + 1u;
+    const uint32_t n = M * rp;
+    for (uint32_t base = 0; base < n; base += kTileBatch * kWaveLanes) {
+#pragma unroll
+      for (uint32_t k = 0; k < kTileBatch; ++k) {
+        const uint32_t idx = base + k * kWaveLanes + lane;
+        const uint32_t q = __umulhi(idx, mg), u = idx - q * rp;
+        if (idx < n) dst[q * S + r0 + u] = any ? tile[tile_at(u, q)] : 0.0f;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // One job: stage its inputs, run every iteration, write its outputs.
 template <class D, bool ES>
 __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _Float16* __restrict__ ex,
-                                           float2* __restrict__ sc_up, float2* __restrict__ sc_lo,
+                                           float* __restrict__ sc_up, float* __restrict__ sc_lo,
                                            float* __restrict__ nii, float* __restrict__ ck,
                                            float* __restrict__ post_sc) {
   const uint32_t lane = threadIdx.x;
@@ -782,16 +906,18 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _F
   uint32_t* bitmap = reinterpret_cast<uint32_t*>(ex + ex_slots);
   const uint32_t bitmap_words = ex_slots / 32;
   uint32_t* partial = bitmap + bitmap_words;
-  float* up_f = reinterpret_cast<float*>(sc_up);
-  float* lo_f = reinterpret_cast<float*>(sc_lo);
   RxBlk Bm{};
   if (fused) rx_block(A.rx, my_cb, Bm);
   const uint32_t D4 = K + 4;
 
-  // ---- this lane's segment of the decoder input into the wave's scratch in
-  // [t][lane] order: (d0, d1) in sc_up, d2 in sc_lo.y.  Idle lanes (shadowing
-  // the last busy lane) neither read nor write the held input.
-  {
+  // ---- the decoder input into the wave's scratch planes in [t][lane]
+  // order: d0 and d1 in the upper half's, d2 in the lower half's parity
+  // plane.  One-block jobs transpose through LDS;
+  // otherwise each lane stages its own segment.  Idle lanes (shadowing the
+  // last busy lane) neither read nor write the held input.
+  if (J.nblk == 1) {
+    stage_tiled<D>(A, L, ci, Bm, fused, qtab, reinterpret_cast<float*>(ex), sc_up, sc_lo);
+  } else {
     float* keep = (fused && A.rx.keep) ? A.rx.keep + ci.llr_off : nullptr;
     for (uint32_t t0 = 0; t0 < S; t0 += kStageSteps) {
       constexpr uint32_t N = 3 * kStageSteps;
@@ -815,8 +941,9 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _F
       for (uint32_t u = 0; u < kStageSteps; ++u) {
         const uint32_t t = t0 + u;
         if (((mask >> (3 * u)) & 1u) == 0) continue;
-        sc_up[t * kWaveLanes + lane] = make_float2(v[3 * u], D::stage_par(v[3 * u + 1]));
-        lo_f[2 * (t * kWaveLanes + lane) + 1] = D::stage_par(v[3 * u + 2]);
+        sc_up[t * kWaveLanes + lane] = v[3 * u];
+        sc_up[kPlane + t * kWaveLanes + lane] = D::stage_par(v[3 * u + 1]);
+        sc_lo[kPlane + t * kWaveLanes + lane] = D::stage_par(v[3 * u + 2]);
         if (keep) {
           keep[pos[3 * u]] = v[3 * u];
           keep[pos[3 * u + 1]] = v[3 * u + 1];
@@ -854,9 +981,10 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _F
         A.rx.keep[ci.llr_off + pos] = chan_in(A, Bm, ci.llr_off, pos, false);
       }
   }
-  // ---- interleaved systematic: sc_lo.x at step t = d0[pi(q S + t)], which
-  // sits in sc_up at the extrinsic address of that bit ----
-  {
+  // ---- several-block jobs: interleaved systematic, the lower half's plane
+  // at step t = d0[pi(q S + t)], which sits in the upper half's plane at the
+  // extrinsic address of that bit ----
+  if (J.nblk > 1) {
     uint32_t y = qpp_col0(L, L.q, 0);
     const uint32_t by = qpp_stride(L, L.q);
     for (uint32_t t0 = 0; t0 < S; t0 += kSubWin) {
@@ -864,13 +992,13 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _F
 #pragma unroll
       for (uint32_t u = 0; u < kSubWin; ++u) {
         if (t0 + u < S) {
-          xp[u] = up_f[2 * lds_addr<1>(L, qtab, L.q, t0 + u, y)];
+          xp[u] = sc_up[lds_addr<1>(L, qtab, L.q, t0 + u, y)];
           y = y_fwd(y, by, M);
         }
       }
 #pragma unroll
       for (uint32_t u = 0; u < kSubWin; ++u)
-        if (t0 + u < S && L.active) lo_f[2 * ((t0 + u) * kWaveLanes + lane)] = xp[u];
+        if (t0 + u < S && L.active) sc_lo[(t0 + u) * kWaveLanes + lane] = xp[u];
     }
   }
   {
@@ -908,7 +1036,9 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _F
 
   // ---- posterior out, natural order (zero when no iteration ran), with
   // kCopyBatch loads in flight per lane ----
-  for (uint32_t bb = 0; bb < J.nblk; ++bb) {
+  if (J.nblk == 1)
+    copy_out_tiled(L, A.max_iterations > 0, post_sc, reinterpret_cast<float*>(ex), A.post + ci.post_off);
+  else for (uint32_t bb = 0; bb < J.nblk; ++bb) {
     float* dst = A.post + cbs[cb_index[J.cb_off + bb]].post_off;
     for (uint32_t base = 0; base < K; base += kCopyBatch * kWaveLanes) {
       float v[kCopyBatch];
@@ -946,10 +1076,11 @@ template <class D, bool ES>
 __device__ __forceinline__ void window_decode(const WinArgs& A) {
   extern __shared__ float lds[];
   const uint32_t lane = threadIdx.x;
-  float2* sc_up = A.scratch + (size_t)blockIdx.x * kSlotFloat2;
-  float2* sc_lo = sc_up + kMaxSeg * kWaveLanes;
+  // staged inputs: two planes per half
+  float* sc_up = reinterpret_cast<float*>(A.scratch + (size_t)blockIdx.x * kSlotFloat2);
+  float* sc_lo = sc_up + 2 * kPlane;
   // per-lane records: boundary metrics [lane][half][16], checkpoints [lane][w][8]
-  float* recs = reinterpret_cast<float*>(sc_lo + kMaxSeg * kWaveLanes);
+  float* recs = sc_lo + 2 * kPlane;
   float* nii = recs + lane * 32;
   float* ck = recs + kWaveLanes * 32 + lane * (8 * kMaxSubWins);
   // posteriors [S][64] + one dummy slot per lane
