@@ -259,13 +259,22 @@ __device__ __forceinline__ uint32_t qpp_col0(const Lane& L, uint32_t qq, uint32_
 }
 __device__ __forceinline__ uint32_t qpp_stride(const Lane& L, uint32_t qq) { return (2u * L.f2 * qq) % L.M; }
 
+// The per-K table is read-only for the whole launch: read through the
+// constant address space, so that a wave-uniform index becomes a scalar load
+// (waited on with lgkmcnt) instead of a vector load, whose wait would also
+// drain every prefetched vector load behind it.
+typedef __attribute__((address_space(4))) const uint32_t ConstU32;
+__device__ __forceinline__ uint32_t qtab_at(const uint32_t* qtab, uint32_t i) {
+  return ((ConstU32*)qtab)[i];
+}
+
 template <int H>
 __device__ __forceinline__ uint32_t lds_addr(const Lane& L, const uint32_t* __restrict__ qtab, uint32_t qq,
                                              uint32_t t, uint32_t y) {
   if constexpr (H == 0) {
     return t * kExRow + L.colb + qq;
   } else {
-    const uint32_t u = qtab[L.qtab_off + t];
+    const uint32_t u = qtab_at(qtab, L.qtab_off + t);
     uint32_t c = (u & 0xFFFFu) + y;
     c = (c >= L.M) ? c - L.M : c;
     return (u >> 16) * kExRow + L.colb + c;
@@ -819,7 +828,7 @@ __device__ __forceinline__ void stage_tiled(const WinArgs& A, const Lane& L, con
         // i.e. row pi(t) mod S (uniform) at this lane's permuted column
         uint32_t y = qpp_col0(L, L.q, 0);
         for (uint32_t t = 0; t < S; ++t) {
-          const uint32_t qv = qtab[L.qtab_off + t];
+          const uint32_t qv = qtab_at(qtab, L.qtab_off + t);
           const uint32_t row = qv >> 16;
           if (row >= r0 && row < r0 + rp) {
             uint32_t c = (qv & 0xFFFFu) + y;
