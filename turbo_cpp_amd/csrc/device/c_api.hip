@@ -165,6 +165,8 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
       jobs.push_back(j);
     }
   }
+  // padding: the window kernels read a chunk's kSubWin entries at once
+  qtab.resize(qtab.size() + tcdev::kSubWin, 0u);
   // longest segments first: a job's time grows with S + 2g
   std::stable_sort(jobs.begin(), jobs.end(), [](const tcdev::WinJob& a, const tcdev::WinJob& b) {
     return a.seg + 2 * a.warm > b.seg + 2 * b.warm;

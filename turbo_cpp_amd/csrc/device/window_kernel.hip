@@ -268,17 +268,21 @@ __device__ __forceinline__ uint32_t qtab_at(const uint32_t* qtab, uint32_t i) {
   return ((ConstU32*)qtab)[i];
 }
 
+// u: the table entry of step t (lower half only)
 template <int H>
-__device__ __forceinline__ uint32_t lds_addr(const Lane& L, const uint32_t* __restrict__ qtab, uint32_t qq,
-                                             uint32_t t, uint32_t y) {
+__device__ __forceinline__ uint32_t lds_addr_q(const Lane& L, uint32_t u, uint32_t qq, uint32_t t, uint32_t y) {
   if constexpr (H == 0) {
     return t * kExRow + L.colb + qq;
   } else {
-    const uint32_t u = qtab_at(qtab, L.qtab_off + t);
     uint32_t c = (u & 0xFFFFu) + y;
     c = (c >= L.M) ? c - L.M : c;
     return (u >> 16) * kExRow + L.colb + c;
   }
+}
+template <int H>
+__device__ __forceinline__ uint32_t lds_addr(const Lane& L, const uint32_t* __restrict__ qtab, uint32_t qq,
+                                             uint32_t t, uint32_t y) {
+  return lds_addr_q<H>(L, H == 1 ? qtab_at(qtab, L.qtab_off + t) : 0u, qq, t, y);
 }
 
 // Extrinsics in LDS are _Float16, saturated at +-65504 and rounded to
@@ -330,11 +334,23 @@ __device__ __forceinline__ void load_chunk_lds(const Lane& L, const _Float16* __
                                                const uint32_t* __restrict__ qtab, uint32_t qq, uint32_t by,
                                                uint32_t lo, uint32_t len, Chunk& c, uint32_t* addr) {
   uint32_t y = (H == 1) ? qpp_col0(L, qq, BACKWARD ? lo + len - 1 : lo) : 0u;
+  // the chunk's kSubWin table entries first, in one scalar batch (the table
+  // is padded by kSubWin entries), then the LDS reads back to back
+  uint32_t qv[kSubWin];
+  if constexpr (H == 1) {
+    // opaque per chunk, so the compiler does not hoist (and then spill) the
+    // table addresses of every chunk out of the loops
+    uint32_t off = __builtin_amdgcn_readfirstlane(L.qtab_off + lo);  // wave-uniform
+    asm volatile("" : "+s"(off));
+    const uint32_t* qt = qtab + off;
+#pragma unroll
+    for (int j = 0; j < kSubWin; ++j) qv[j] = qtab_at(qt, j);
+  }
 #pragma unroll
   for (int jj = 0; jj < kSubWin; ++jj) {
     const int j = BACKWARD ? kSubWin - 1 - jj : jj;
     if (FULL || (uint32_t)j < len) {
-      const uint32_t a = lds_addr<H>(L, qtab, qq, lo + j, y);
+      const uint32_t a = lds_addr_q<H>(L, qv[j], qq, lo + j, y);
       c.e[j] = static_cast<float>(ex[a]);
       if (addr) addr[j] = a;
       if constexpr (H == 1) y = BACKWARD ? y_back(y, by, L.M) : y_fwd(y, by, L.M);
