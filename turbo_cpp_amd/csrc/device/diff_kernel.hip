@@ -43,15 +43,19 @@ namespace {
 
 template <typename R>
 struct Num;
+// float: the hardware exp2 / log2 / reciprocal (about 1 ulp); double: libm
+// and IEEE division, for the finite-difference and FP64 reference checks.
 template <>
 struct Num<float> {
   static __device__ float ex(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
   static __device__ float ln(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+  static __device__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 };
 template <>
 struct Num<double> {
   static __device__ double ex(double x) { return exp(x); }
   static __device__ double ln(double x) { return log(x); }
+  static __device__ double rcp(double x) { return 1.0 / x; }
 };
 
 constexpr float kNegF = -1.0e30f;
@@ -78,7 +82,7 @@ template <typename R>
 __device__ inline void jac_w(R a, R b, R& wa, R& wb) {
   const R d = a > b ? a - b : b - a;
   const R e = Num<R>::ex(-d);
-  const R big = R(1) / (R(1) + e);
+  const R big = Num<R>::rcp(R(1) + e);
   const R small = e * big;
   wa = a >= b ? big : small;
   wb = a >= b ? small : big;
@@ -144,7 +148,7 @@ __device__ inline void llr_terms(const R (&a)[8], const R (&b)[8], R hs, R hp, R
     }
     L[u] = mx + Num<R>::ln(sum);
     if (P) {
-      const R inv = R(1) / sum;
+      const R inv = Num<R>::rcp(sum);
 #pragma unroll
       for (unsigned s = 0; s < 8; ++s) P[u][s] = m[s] * inv;
     }
@@ -169,11 +173,29 @@ __device__ inline void tail_beta(const R (&ts)[3], const R (&tp)[3], R (&o)[8]) 
   normalise(o);
 }
 
-// QPP address Pi(i) = (f1 i + f2 i^2) mod K.
-__device__ inline uint32_t qpp(uint32_t i, uint32_t k, uint32_t f1, uint32_t f2) {
-  const uint64_t v = (uint64_t)f1 * i + (uint64_t)f2 * i % k * i;
-  return (uint32_t)(v % k);
-}
+// QPP address Pi(i) = (f1 i + f2 i^2) mod K, walked step by step:
+// Pi(i+1) - Pi(i) = f1 + f2 (2i + 1) and that difference grows by 2 f2, all
+// mod K (K <= 6144, so every product below fits 32 bits).  One modulo per
+// walk instead of a 64-bit modulo per step.
+struct QppWalk {
+  uint32_t pi, d, d2, k;  // Pi(i), Pi(i+1) - Pi(i), 2 f2, all mod k
+  __device__ QppWalk(uint32_t i, uint32_t kk, uint32_t f1, uint32_t f2) : k(kk) {
+    const uint32_t im = i % kk;
+    pi = (f1 % kk * im % kk + f2 % kk * im % kk * im) % kk;
+    d = (f1 % kk + f2 % kk * ((2 * im + 1) % kk)) % kk;
+    d2 = 2 * f2 % kk;
+  }
+  __device__ void next() {
+    pi += d;
+    pi = pi >= k ? pi - k : pi;
+    d += d2;
+    d = d >= k ? d - k : d;
+  }
+  __device__ void prev() {
+    d = d >= d2 ? d - d2 : d + k - d2;
+    pi = pi >= d ? pi - d : pi + k - d;
+  }
+};
 
 // Per-lane scratch fields, each [j][lane] with j < its length.
 struct Layout {
@@ -214,6 +236,8 @@ __device__ void stage(const Lane<R>& L, const Layout& Y, uint32_t h, const R* d0
                       const R* sv, const R* scales) {
   const R w = h ? scales[h - 1] : R(0);
   const R* eprev = sv + (size_t)(h ? h - 1 : 0) * L.K;
+  const int t0 = L.t_of(0);
+  QppWalk qw(t0 > 0 ? (uint32_t)t0 : 0u, L.K, L.f1, L.f2);
   for (uint32_t j = 0; j < Y.P; ++j) {
     const int t = L.t_of(j);
     R sa = 0, pr = 0;
@@ -222,10 +246,11 @@ __device__ void stage(const Lane<R>& L, const Layout& Y, uint32_t h, const R* d0
         sa = d0[t] + (h ? w * eprev[t] : R(0));
         pr = d1[t];
       } else {
-        const uint32_t pi = qpp((uint32_t)t, L.K, L.f1, L.f2);
+        const uint32_t pi = qw.pi;
         sa = d0[pi] + w * eprev[pi];
         pr = d2[t];
       }
+      qw.next();
     }
     L.at(Y.sa, j) = sa;
     L.at(Y.pr, j) = pr;
@@ -300,6 +325,8 @@ __device__ void beta_ext_pass(const Lane<R>& L, const Layout& Y, const R (&ts)[3
   R sa_n = L.at(Y.sa, top - 1), pr_n = L.at(Y.pr, top - 1), a_n[8];
 #pragma unroll
   for (int s = 0; s < 8; ++s) a_n[s] = (top - 1 < own_end) ? L.at(Y.al + s * (Y.g + Y.S + 1), top - 1) : R(0);
+  // Pi of the own steps, from the last one down
+  QppWalk qw((uint32_t)L.t_of(own_end - 1), L.K, L.f1, L.f2);
   for (uint32_t j = top; j-- > L.g;) {
     const R sa = sa_n, pr = pr_n;
     R a[8];
@@ -317,8 +344,9 @@ __device__ void beta_ext_pass(const Lane<R>& L, const Layout& Y, const R (&ts)[3
     if (j < own_end) {
       R Lu[2];
       llr_terms(a, b, R(0.5) * sa, R(0.5) * pr, Lu, (R(*)[8]) nullptr);
-      const uint32_t pos = (h & 1u) ? qpp((uint32_t)t, L.K, L.f1, L.f2) : (uint32_t)t;
+      const uint32_t pos = (h & 1u) ? qw.pi : (uint32_t)t;
       e[pos] = Lu[1] - Lu[0] - sa;
+      if (j > L.g) qw.prev();
     }
     if (t < (int)L.K) {
       R o[8];
@@ -474,9 +502,12 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
           for (uint32_t t = lo; t < hi; ++t) prev[t] = (h - 1 == H - 2) ? A.scales[H - 2] * gp[t] : R(0);
         tail_inputs(h, d0, d1, d2, K, ts, tp);
         stage(L, Y, h, d0, d1, d2, sv, A.scales);
-        for (uint32_t j = L.g; j < L.g + L.S; ++j) {
-          const uint32_t t = (uint32_t)L.t_of(j);
-          L.at(Y.eb, j - L.g) = cur[(h & 1u) ? qpp(t, K, L.f1, L.f2) : t];
+        {
+          QppWalk qw(lo, K, L.f1, L.f2);
+          for (uint32_t j = L.g; j < L.g + L.S; ++j, qw.next()) {
+            const uint32_t t = (uint32_t)L.t_of(j);
+            L.at(Y.eb, j - L.g) = cur[(h & 1u) ? qw.pi : t];
+          }
         }
         for (uint32_t j = 0; j < Y.P; ++j) L.at(Y.dsa, j) = L.at(Y.dpr, j) = 0;
         alpha_pass(L, Y);
@@ -629,7 +660,8 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
       if (L.active) {
         const R w = h ? A.scales[h - 1] : R(0);
         const R* eprev = sv + (size_t)(h ? h - 1 : 0) * K;
-        for (uint32_t j = L.g; j < L.g + L.S; ++j) {
+        QppWalk qw(lo, K, L.f1, L.f2);
+        for (uint32_t j = L.g; j < L.g + L.S; ++j, qw.next()) {
           const uint32_t t = (uint32_t)L.t_of(j);
           R ds = L.at(Y.dsa, j), dp = L.at(Y.dpr, j);
           if (L.q + 1 < L.M && j >= L.S) {  // t in [qS + S - g, qS + S): lane q+1's local j - S
@@ -648,7 +680,7 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
               wsum += ds * eprev[t];
             }
           } else {
-            const uint32_t pi = qpp(t, K, L.f1, L.f2);
+            const uint32_t pi = qw.pi;
             g0[pi] += ds;
             g2[t] += dp;
             prev[pi] += w * ds;
