@@ -357,33 +357,6 @@ __device__ void beta_ext_pass(const Lane<R>& L, const Layout& Y, const R (&ts)[3
   }
 }
 
-// beta over local steps g + S + g - 1 down to g, stored at be[j - g]
-// (beta_t for t = t_of(j)), from the tail beta or zero metrics.
-template <typename R>
-__device__ void beta_pass(const Lane<R>& L, const Layout& Y, const R (&ts)[3], const R (&tp)[3]) {
-  R b[8];
-  const bool from_tail = L.q * L.S + L.S + L.g >= L.K;
-  if (from_tail) {
-    tail_beta(ts, tp, b);
-  } else {
-#pragma unroll
-    for (int s = 0; s < 8; ++s) b[s] = 0;
-  }
-  const uint32_t top = 2 * L.g + L.S;  // local index of the beta start
-  for (uint32_t j = top; j-- > L.g;) {
-#pragma unroll
-    for (int s = 0; s < 8; ++s) L.at(Y.be + s * (Y.S + Y.g + 1), j + 1 - L.g) = b[s];
-    if (L.t_of(j) < (int)L.K) {
-      R o[8];
-      beta_step(b, R(0.5) * L.at(Y.sa, j), R(0.5) * L.at(Y.pr, j), o);
-#pragma unroll
-      for (int s = 0; s < 8; ++s) b[s] = o[s];
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < 8; ++s) L.at(Y.be + s * (Y.S + Y.g + 1), 0) = b[s];
-}
-
 template <typename R>
 __device__ inline void load8(const Lane<R>& L, uint32_t field, uint32_t stride, uint32_t j, R (&v)[8]) {
 #pragma unroll
@@ -511,31 +484,61 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
         }
         for (uint32_t j = 0; j < Y.P; ++j) L.at(Y.dsa, j) = L.at(Y.dpr, j) = 0;
         alpha_pass(L, Y);
-        beta_pass(L, Y, ts, tp);
+        // beta over the window's warm-up above the own steps (local 2g + S - 1
+        // down to g + S), stored at be[j + 1 - g] (beta_{j+1}); the own steps'
+        // beta runs inside the alpha-adjoint loop below, an independent chain
+        // beside the adjoint recursion, and is stored the same way
+        R bcur[8];
+        if (lo + L.S + L.g >= K) {
+          tail_beta(ts, tp, bcur);
+        } else {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) bcur[s] = 0;
+        }
+        for (uint32_t j = 2 * L.g + L.S; j-- > L.g + L.S;) {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) L.at(Y.be + s * (Y.S + Y.g + 1), j + 1 - L.g) = bcur[s];
+          if (L.t_of(j) < (int)K) {
+            R o[8];
+            beta_step(bcur, R(0.5) * L.at(Y.sa, j), R(0.5) * L.at(Y.pr, j), o);
+#pragma unroll
+            for (int s = 0; s < 8; ++s) bcur[s] = o[s];
+          }
+        }
         // alpha adjoint, backwards over local steps g + S - 1 .. 0; output
         // adjoints of the own steps on the way
         R ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         // inputs of step j are loaded during step j + 1 (their latency overlaps the arithmetic)
         const uint32_t jtop = L.g + L.S - 1;
-        R nx_a[8], nx_b[8], nx_sa = L.at(Y.sa, jtop), nx_pr = L.at(Y.pr, jtop), nx_e = L.at(Y.eb, jtop - L.g);
+        R nx_a[8], nx_sa = L.at(Y.sa, jtop), nx_pr = L.at(Y.pr, jtop), nx_e = L.at(Y.eb, jtop - L.g);
         load8(L, Y.al, Y.g + Y.S + 1, jtop, nx_a);
-        load8(L, Y.be, Y.S + Y.g + 1, jtop + 1 - L.g, nx_b);
         for (uint32_t j = L.g + L.S; j-- > 0;) {
           if (L.t_of(j) < 0) break;
           R a[8], b[8];
 #pragma unroll
           for (int s = 0; s < 8; ++s) {
             a[s] = nx_a[s];
-            b[s] = nx_b[s];
+            b[s] = bcur[s];  // beta_{j+1} while j >= g
           }
           const R hs = R(0.5) * nx_sa, hp = R(0.5) * nx_pr, e = nx_e;
           if (j > 0 && L.t_of(j - 1) >= 0) {
             load8(L, Y.al, Y.g + Y.S + 1, j - 1, nx_a);
             nx_sa = L.at(Y.sa, j - 1);
             nx_pr = L.at(Y.pr, j - 1);
-            if (j - 1 >= L.g) {
-              load8(L, Y.be, Y.S + Y.g + 1, j - L.g, nx_b);
-              nx_e = L.at(Y.eb, j - 1 - L.g);
+            if (j - 1 >= L.g) nx_e = L.at(Y.eb, j - 1 - L.g);
+          }
+          if (j >= L.g) {
+            // own step: store beta_{j+1}, step beta to beta_j (its last value,
+            // beta at local g, goes to be[0])
+#pragma unroll
+            for (int s = 0; s < 8; ++s) L.at(Y.be + s * (Y.S + Y.g + 1), j + 1 - L.g) = b[s];
+            R o[8];
+            beta_step(b, hs, hp, o);
+#pragma unroll
+            for (int s = 0; s < 8; ++s) bcur[s] = o[s];
+            if (j == L.g) {
+#pragma unroll
+              for (int s = 0; s < 8; ++s) L.at(Y.be + s * (Y.S + Y.g + 1), 0) = bcur[s];
             }
           }
           R dg[8][2];
