@@ -363,6 +363,120 @@ __device__ inline void load8(const Lane<R>& L, uint32_t field, uint32_t stride, 
   for (int s = 0; s < 8; ++s) v[s] = L.at(field + s * stride, j);
 }
 
+// alpha over local steps 0 .. g + S with only every 8th stored: al[c] =
+// alpha at local 8c (the same recursion as alpha_pass).
+template <typename R>
+__device__ void alpha_ckpt_pass(const Lane<R>& L, const Layout& Y) {
+  R a[8];
+  const bool from_zero_state = (int)(L.q * L.S) - (int)L.g <= 0;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) a[s] = from_zero_state ? (s == 0 ? R(0) : R(kNegF)) : R(0);
+  const uint32_t n = L.g + L.S;
+  for (uint32_t j = 0; j < n; ++j) {
+    if ((j & 7u) == 0) {
+#pragma unroll
+      for (int s = 0; s < 8; ++s) L.at(Y.al + s * (Y.g + Y.S + 1), j / 8) = a[s];
+    }
+    if (L.t_of(j) >= 0) {
+      R o[8];
+      alpha_step(a, R(0.5) * L.at(Y.sa, j), R(0.5) * L.at(Y.pr, j), o);
+#pragma unroll
+      for (int s = 0; s < 8; ++s) a[s] = o[s];
+    }
+  }
+}
+
+// One step of the alpha adjoint at local j (alpha_j = a, beta_{j+1} = b,
+// half gammas hs, hp): the output adjoint of the extrinsic e when the step is
+// the lane's own (then the beta output adjoints of the step go to bo[j - g]),
+// then the adjoint of alpha_{j+1} pulled back through its two incoming
+// transitions; ab holds the adjoint of alpha_{j+1} on entry and of alpha_j on
+// exit; ds, dp: the adjoints of sys + la and par of the step.
+template <typename R>
+__device__ inline void alpha_adjoint_step(const R (&a)[8], const R (&b)[8], R hs, R hp, R e, bool own, R (&ab)[8],
+                                          R& ds_out, R& dp_out, const Lane<R>& L, const Layout& Y, uint32_t j) {
+  R dg[8][2];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) dg[s][0] = dg[s][1] = 0;
+  R abn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  R dsa_direct = 0;
+  if (own) {
+    R Lu[2], P[2][8];
+    llr_terms(a, b, hs, hp, Lu, P);
+    R bo[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (unsigned s = 0; s < 8; ++s)
+#pragma unroll
+      for (unsigned u = 0; u < 2; ++u) {
+        const R dm = u ? e * P[1][s] : -e * P[0][s];
+        dg[s][u] += dm;
+        abn[s] += dm;
+        bo[nxt(s, u)] += dm;
+      }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) L.at(Y.bo + s * Y.S, j - L.g) = bo[s];
+    dsa_direct = -e;
+  }
+  // alpha_{t+1}(n) = jac over its two incoming transitions
+  bool seen[8] = {false, false, false, false, false, false, false, false};
+  unsigned fs[8], fu[8];
+#pragma unroll
+  for (unsigned s = 0; s < 8; ++s)
+#pragma unroll
+    for (unsigned u = 0; u < 2; ++u) {
+      const unsigned n = nxt(s, u);
+      if (!seen[n]) {
+        fs[n] = s;
+        fu[n] = u;
+        seen[n] = true;
+      } else {
+        const unsigned s0 = fs[n], u0 = fu[n];
+        R w0, w1;
+        jac_w(a[s0] + gam(u0, par(s0, u0), hs, hp), a[s] + gam(u, par(s, u), hs, hp), w0, w1);
+        dg[s0][u0] += ab[n] * w0;
+        abn[s0] += ab[n] * w0;
+        dg[s][u] += ab[n] * w1;
+        abn[s] += ab[n] * w1;
+      }
+    }
+#pragma unroll
+  for (int s = 0; s < 8; ++s) ab[s] = abn[s];
+  R ds = dsa_direct, dp = 0;
+#pragma unroll
+  for (unsigned s = 0; s < 8; ++s)
+#pragma unroll
+    for (unsigned u = 0; u < 2; ++u) {
+      ds += u ? R(0.5) * dg[s][u] : R(-0.5) * dg[s][u];
+      dp += par(s, u) ? R(0.5) * dg[s][u] : R(-0.5) * dg[s][u];
+    }
+  ds_out = ds;
+  dp_out = dp;
+}
+
+// One step of the beta adjoint at local j (beta_{j+1} = b): bb holds the
+// adjoint of beta_j on entry (bn: the step's output adjoints of beta_{j+1})
+// and of beta_{j+1} on exit.
+template <typename R>
+__device__ inline void beta_adjoint_step(const R (&b)[8], R hs, R hp, R (&bb)[8], R (&bn)[8], R& ds_out,
+                                         R& dp_out) {
+  R ds = 0, dp = 0;
+#pragma unroll
+  for (unsigned s = 0; s < 8; ++s) {
+    R w0, w1;
+    jac_w(b[nxt(s, 0)] + gam(0u, par(s, 0), hs, hp), b[nxt(s, 1)] + gam(1u, par(s, 1), hs, hp), w0, w1);
+    const R d0g = bb[s] * w0, d1g = bb[s] * w1;
+    bn[nxt(s, 0)] += d0g;
+    bn[nxt(s, 1)] += d1g;
+    ds += R(0.5) * (d1g - d0g);
+    dp += par(s, 0) ? R(0.5) * d0g : R(-0.5) * d0g;
+    dp += par(s, 1) ? R(0.5) * d1g : R(-0.5) * d1g;
+  }
+#pragma unroll
+  for (int s = 0; s < 8; ++s) bb[s] = bn[s];
+  ds_out = ds;
+  dp_out = dp;
+}
+
 template <typename R>
 __device__ Lane<R> make_lane(const DiffJob& J, const DiffArgs<R>& A) {
   Lane<R> L;
@@ -483,11 +597,16 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
           }
         }
         for (uint32_t j = 0; j < Y.P; ++j) L.at(Y.dsa, j) = L.at(Y.dpr, j) = 0;
-        alpha_pass(L, Y);
+        // alpha at every 8th local step only (checkpoints in the al field,
+        // al[c] = alpha at local 8c); the adjoint sweeps recompute each
+        // sub-window's metrics into registers, which keeps the wave's scratch
+        // small enough to stay in the caches
+        alpha_ckpt_pass(L, Y);
         // beta over the window's warm-up above the own steps (local 2g + S - 1
-        // down to g + S), stored at be[j + 1 - g] (beta_{j+1}); the own steps'
-        // beta runs inside the alpha-adjoint loop below, an independent chain
-        // beside the adjoint recursion, and is stored the same way
+        // down to g + S); the own steps' beta runs inside the alpha-adjoint
+        // loop below, an independent chain beside the adjoint recursion.
+        // Checkpoints for the beta-adjoint sweep: beta at local g + i for i a
+        // multiple of 8 or i = S + g, at be[(i + 7) / 8]
         R bcur[8];
         if (lo + L.S + L.g >= K) {
           tail_beta(ts, tp, bcur);
@@ -495,9 +614,15 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
 #pragma unroll
           for (int s = 0; s < 8; ++s) bcur[s] = 0;
         }
-        for (uint32_t j = 2 * L.g + L.S; j-- > L.g + L.S;) {
+        const uint32_t bst = Y.S + Y.g + 1;
+        auto beta_ck = [&](uint32_t i, const R (&b)[8]) {
+          if ((i & 7u) == 0 || i == L.S + L.g) {
 #pragma unroll
-          for (int s = 0; s < 8; ++s) L.at(Y.be + s * (Y.S + Y.g + 1), j + 1 - L.g) = bcur[s];
+            for (int s = 0; s < 8; ++s) L.at(Y.be + s * bst, (i + 7) / 8) = b[s];
+          }
+        };
+        for (uint32_t j = 2 * L.g + L.S; j-- > L.g + L.S;) {
+          beta_ck(j + 1 - L.g, bcur);
           if (L.t_of(j) < (int)K) {
             R o[8];
             beta_step(bcur, R(0.5) * L.at(Y.sa, j), R(0.5) * L.at(Y.pr, j), o);
@@ -505,136 +630,104 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
             for (int s = 0; s < 8; ++s) bcur[s] = o[s];
           }
         }
-        // alpha adjoint, backwards over local steps g + S - 1 .. 0; output
+        // alpha adjoint, backwards over local steps g + S - 1 .. 0 in
+        // sub-windows of 8 (alpha recomputed from the checkpoint); output
         // adjoints of the own steps on the way
         R ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        // inputs of step j are loaded during step j + 1 (their latency overlaps the arithmetic)
-        const uint32_t jtop = L.g + L.S - 1;
-        R nx_a[8], nx_sa = L.at(Y.sa, jtop), nx_pr = L.at(Y.pr, jtop), nx_e = L.at(Y.eb, jtop - L.g);
-        load8(L, Y.al, Y.g + Y.S + 1, jtop, nx_a);
-        for (uint32_t j = L.g + L.S; j-- > 0;) {
-          if (L.t_of(j) < 0) break;
-          R a[8], b[8];
+        const uint32_t n_al = L.g + L.S;
+        for (int c = (int)((n_al + 7) / 8) - 1; c >= 0; --c) {
+          const uint32_t j0 = 8u * (uint32_t)c;
+          const uint32_t len = min(8u, n_al - j0);
+          if (L.t_of(j0 + len - 1) < 0) break;  // segment 0: nothing before t = 0
+          R areg[8][8], hsr[8], hpr[8], er[8];
+          {
+            R a[8];
+            load8(L, Y.al, Y.g + Y.S + 1, (uint32_t)c, a);
 #pragma unroll
-          for (int s = 0; s < 8; ++s) {
-            a[s] = nx_a[s];
-            b[s] = bcur[s];  // beta_{j+1} while j >= g
-          }
-          const R hs = R(0.5) * nx_sa, hp = R(0.5) * nx_pr, e = nx_e;
-          if (j > 0 && L.t_of(j - 1) >= 0) {
-            load8(L, Y.al, Y.g + Y.S + 1, j - 1, nx_a);
-            nx_sa = L.at(Y.sa, j - 1);
-            nx_pr = L.at(Y.pr, j - 1);
-            if (j - 1 >= L.g) nx_e = L.at(Y.eb, j - 1 - L.g);
-          }
-          if (j >= L.g) {
-            // own step: store beta_{j+1}, step beta to beta_j (its last value,
-            // beta at local g, goes to be[0])
+            for (uint32_t jj = 0; jj < 8; ++jj) {
+              if (jj < len) {
+                const uint32_t j = j0 + jj;
+                hsr[jj] = R(0.5) * L.at(Y.sa, j);
+                hpr[jj] = R(0.5) * L.at(Y.pr, j);
+                er[jj] = (j >= L.g) ? L.at(Y.eb, j - L.g) : R(0);
 #pragma unroll
-            for (int s = 0; s < 8; ++s) L.at(Y.be + s * (Y.S + Y.g + 1), j + 1 - L.g) = b[s];
-            R o[8];
-            beta_step(b, hs, hp, o);
+                for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
+                if (jj + 1 < len && L.t_of(j) >= 0) {
+                  R o[8];
+                  alpha_step(a, hsr[jj], hpr[jj], o);
 #pragma unroll
-            for (int s = 0; s < 8; ++s) bcur[s] = o[s];
-            if (j == L.g) {
-#pragma unroll
-              for (int s = 0; s < 8; ++s) L.at(Y.be + s * (Y.S + Y.g + 1), 0) = bcur[s];
-            }
-          }
-          R dg[8][2];
-#pragma unroll
-          for (int s = 0; s < 8; ++s) dg[s][0] = dg[s][1] = 0;
-          R abn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          R dsa_direct = 0;
-          if (j >= L.g) {
-            R Lu[2], P[2][8];
-            llr_terms(a, b, hs, hp, Lu, P);
-            R bo[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (unsigned s = 0; s < 8; ++s)
-#pragma unroll
-              for (unsigned u = 0; u < 2; ++u) {
-                const R dm = u ? e * P[1][s] : -e * P[0][s];
-                dg[s][u] += dm;
-                abn[s] += dm;
-                bo[nxt(s, u)] += dm;
-              }
-#pragma unroll
-            for (int s = 0; s < 8; ++s) L.at(Y.bo + s * Y.S, j - L.g) = bo[s];
-            dsa_direct = -e;
-          }
-          // alpha_{t+1}(n) = jac over its two incoming transitions
-          bool seen[8] = {false, false, false, false, false, false, false, false};
-          unsigned fs[8], fu[8];
-#pragma unroll
-          for (unsigned s = 0; s < 8; ++s)
-#pragma unroll
-            for (unsigned u = 0; u < 2; ++u) {
-              const unsigned n = nxt(s, u);
-              if (!seen[n]) {
-                fs[n] = s;
-                fu[n] = u;
-                seen[n] = true;
-              } else {
-                const unsigned s0 = fs[n], u0 = fu[n];
-                R w0, w1;
-                jac_w(a[s0] + gam(u0, par(s0, u0), hs, hp), a[s] + gam(u, par(s, u), hs, hp), w0, w1);
-                dg[s0][u0] += ab[n] * w0;
-                abn[s0] += ab[n] * w0;
-                dg[s][u] += ab[n] * w1;
-                abn[s] += ab[n] * w1;
+                  for (int s = 0; s < 8; ++s) a[s] = o[s];
+                }
               }
             }
+          }
 #pragma unroll
-          for (int s = 0; s < 8; ++s) ab[s] = abn[s];
-          R ds = dsa_direct, dp = 0;
+          for (int jj = 7; jj >= 0; --jj) {
+            const uint32_t j = j0 + (uint32_t)jj;
+            if ((uint32_t)jj >= len || L.t_of(j) < 0) continue;
+            R b[8];
 #pragma unroll
-          for (unsigned s = 0; s < 8; ++s)
+            for (int s = 0; s < 8; ++s) b[s] = bcur[s];  // beta_{j+1} while j >= g
+            const R hs = hsr[jj], hp = hpr[jj], e = er[jj];
+            if (j >= L.g) {
+              // own step: checkpoint beta_{j+1}, step beta to beta_j
+              beta_ck(j + 1 - L.g, b);
+              R o[8];
+              beta_step(b, hs, hp, o);
 #pragma unroll
-            for (unsigned u = 0; u < 2; ++u) {
-              ds += u ? R(0.5) * dg[s][u] : R(-0.5) * dg[s][u];
-              dp += par(s, u) ? R(0.5) * dg[s][u] : R(-0.5) * dg[s][u];
+              for (int s = 0; s < 8; ++s) bcur[s] = o[s];
             }
-          L.at(Y.dsa, j) = ds;  // first writer of these positions (all zero before)
-          L.at(Y.dpr, j) = dp;
+            R ds, dp;
+            alpha_adjoint_step(areg[jj], b, hs, hp, e, j >= L.g, ab, ds, dp, L, Y, j);
+            L.at(Y.dsa, j) = ds;  // first writer of these positions (all zero before)
+            L.at(Y.dpr, j) = dp;
+          }
         }
-        // beta adjoint, forwards over local steps g .. g + S + g - 1
+        // beta adjoint, forwards over local steps g .. g + S + g - 1 in
+        // sub-windows of 8 (beta recomputed from the checkpoint at the top)
         R bb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const uint32_t jend = 2 * L.g + L.S;
-        R nb_b[8], nb_o[8], nb_sa = L.at(Y.sa, L.g), nb_pr = L.at(Y.pr, L.g);
-        load8(L, Y.be, Y.S + Y.g + 1, 1, nb_b);
-        load8(L, Y.bo, Y.S, 0, nb_o);
-        for (uint32_t j = L.g; j < jend; ++j) {
-          if (L.t_of(j) >= (int)K) break;
-          R b[8], bn[8];
+        for (uint32_t jb0 = L.g; jb0 < jend; jb0 += 8) {
+          if (L.t_of(jb0) >= (int)K) break;
+          const uint32_t len = min(8u, jend - jb0);
+          const uint32_t itop = jb0 + len - L.g;  // beta at local jb0 + len = g + itop
+          R breg[8][8], hsr[8], hpr[8];
+          {
+            R b[8];
+            load8(L, Y.be, bst, (itop + 7) / 8, b);
 #pragma unroll
-          for (int s = 0; s < 8; ++s) {
-            b[s] = nb_b[s];
-            bn[s] = (j < L.g + L.S) ? nb_o[s] : R(0);
-          }
-          const R hs = R(0.5) * nb_sa, hp = R(0.5) * nb_pr;
-          if (j + 1 < jend && L.t_of(j + 1) < (int)K) {
-            load8(L, Y.be, Y.S + Y.g + 1, j + 2 - L.g, nb_b);
-            nb_sa = L.at(Y.sa, j + 1);
-            nb_pr = L.at(Y.pr, j + 1);
-            if (j + 1 < L.g + L.S) load8(L, Y.bo, Y.S, j + 1 - L.g, nb_o);
-          }
-          R ds = 0, dp = 0;
+            for (int jj = 7; jj >= 0; --jj) {
+              if ((uint32_t)jj < len) {
+                const uint32_t j = jb0 + (uint32_t)jj;
+                hsr[jj] = R(0.5) * L.at(Y.sa, j);
+                hpr[jj] = R(0.5) * L.at(Y.pr, j);
 #pragma unroll
-          for (unsigned s = 0; s < 8; ++s) {
-            R w0, w1;
-            jac_w(b[nxt(s, 0)] + gam(0u, par(s, 0), hs, hp), b[nxt(s, 1)] + gam(1u, par(s, 1), hs, hp), w0, w1);
-            const R d0g = bb[s] * w0, d1g = bb[s] * w1;
-            bn[nxt(s, 0)] += d0g;
-            bn[nxt(s, 1)] += d1g;
-            ds += R(0.5) * (d1g - d0g);
-            dp += par(s, 0) ? R(0.5) * d0g : R(-0.5) * d0g;
-            dp += par(s, 1) ? R(0.5) * d1g : R(-0.5) * d1g;
+                for (int s = 0; s < 8; ++s) breg[jj][s] = b[s];  // beta at local j + 1
+                if (jj > 0 && L.t_of(j) < (int)K) {
+                  R o[8];
+                  beta_step(b, hsr[jj], hpr[jj], o);
+#pragma unroll
+                  for (int s = 0; s < 8; ++s) b[s] = o[s];
+                }
+              }
+            }
           }
 #pragma unroll
-          for (int s = 0; s < 8; ++s) bb[s] = bn[s];
-          L.at(Y.dsa, j) += ds;
-          L.at(Y.dpr, j) += dp;
+          for (uint32_t jj = 0; jj < 8; ++jj) {
+            const uint32_t j = jb0 + jj;
+            if (jj >= len || L.t_of(j) >= (int)K) break;
+            R bn[8];
+            if (j < L.g + L.S) {
+              load8(L, Y.bo, Y.S, j - L.g, bn);
+            } else {
+#pragma unroll
+              for (int s = 0; s < 8; ++s) bn[s] = 0;
+            }
+            R ds, dp;
+            beta_adjoint_step(breg[jj], hsr[jj], hpr[jj], bb, bn, ds, dp);
+            L.at(Y.dsa, j) += ds;
+            L.at(Y.dpr, j) += dp;
+          }
         }
         // beta of the window top: the tail beta when it reaches K
         to_tail = lo + L.S + L.g >= K;
