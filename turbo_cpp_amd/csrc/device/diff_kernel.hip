@@ -238,6 +238,7 @@ __device__ void stage(const Lane<R>& L, const Layout& Y, uint32_t h, const R* d0
   const R* eprev = sv + (size_t)(h ? h - 1 : 0) * L.K;
   const int t0 = L.t_of(0);
   QppWalk qw(t0 > 0 ? (uint32_t)t0 : 0u, L.K, L.f1, L.f2);
+#pragma unroll 8
   for (uint32_t j = 0; j < Y.P; ++j) {
     const int t = L.t_of(j);
     R sa = 0, pr = 0;
@@ -372,14 +373,20 @@ __device__ void alpha_ckpt_pass(const Lane<R>& L, const Layout& Y) {
 #pragma unroll
   for (int s = 0; s < 8; ++s) a[s] = from_zero_state ? (s == 0 ? R(0) : R(kNegF)) : R(0);
   const uint32_t n = L.g + L.S;
+  R sa_n = L.at(Y.sa, 0), pr_n = L.at(Y.pr, 0);  // the next step's inputs, one step ahead
   for (uint32_t j = 0; j < n; ++j) {
+    const R sa = sa_n, pr = pr_n;
+    if (j + 1 < n) {
+      sa_n = L.at(Y.sa, j + 1);
+      pr_n = L.at(Y.pr, j + 1);
+    }
     if ((j & 7u) == 0) {
 #pragma unroll
       for (int s = 0; s < 8; ++s) L.at(Y.al + s * (Y.g + Y.S + 1), j / 8) = a[s];
     }
     if (L.t_of(j) >= 0) {
       R o[8];
-      alpha_step(a, R(0.5) * L.at(Y.sa, j), R(0.5) * L.at(Y.pr, j), o);
+      alpha_step(a, R(0.5) * sa, R(0.5) * pr, o);
 #pragma unroll
       for (int s = 0; s < 8; ++s) a[s] = o[s];
     }
@@ -591,12 +598,15 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
         stage(L, Y, h, d0, d1, d2, sv, A.scales);
         {
           QppWalk qw(lo, K, L.f1, L.f2);
+#pragma unroll 8
           for (uint32_t j = L.g; j < L.g + L.S; ++j, qw.next()) {
             const uint32_t t = (uint32_t)L.t_of(j);
             L.at(Y.eb, j - L.g) = cur[(h & 1u) ? qw.pi : t];
           }
         }
-        for (uint32_t j = 0; j < Y.P; ++j) L.at(Y.dsa, j) = L.at(Y.dpr, j) = 0;
+        // the alpha-adjoint sweep is the first writer of local steps below
+        // g + S; the beta-adjoint sweep adds to those above
+        for (uint32_t j = L.g + L.S; j < Y.P; ++j) L.at(Y.dsa, j) = L.at(Y.dpr, j) = 0;
         // alpha at every 8th local step only (checkpoints in the al field,
         // al[c] = alpha at local 8c); the adjoint sweeps recompute each
         // sub-window's metrics into registers, which keeps the wave's scratch
@@ -621,11 +631,17 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
             for (int s = 0; s < 8; ++s) L.at(Y.be + s * bst, (i + 7) / 8) = b[s];
           }
         };
+        R wsa = L.at(Y.sa, 2 * L.g + L.S - 1), wpr = L.at(Y.pr, 2 * L.g + L.S - 1);
         for (uint32_t j = 2 * L.g + L.S; j-- > L.g + L.S;) {
+          const R sa = wsa, pr = wpr;
+          if (j > L.g + L.S) {
+            wsa = L.at(Y.sa, j - 1);
+            wpr = L.at(Y.pr, j - 1);
+          }
           beta_ck(j + 1 - L.g, bcur);
           if (L.t_of(j) < (int)K) {
             R o[8];
-            beta_step(bcur, R(0.5) * L.at(Y.sa, j), R(0.5) * L.at(Y.pr, j), o);
+            beta_step(bcur, R(0.5) * sa, R(0.5) * pr, o);
 #pragma unroll
             for (int s = 0; s < 8; ++s) bcur[s] = o[s];
           }
@@ -712,21 +728,31 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
               }
             }
           }
+          // the sub-window's output adjoints and running sys/par adjoints, all
+          // loaded before its first step
+          R bor[8][8], dsr[8], dpr[8];
+#pragma unroll
+          for (uint32_t jj = 0; jj < 8; ++jj) {
+            const uint32_t j = jb0 + jj;
+            if (jj < len) {
+              if (j < L.g + L.S) {
+                load8(L, Y.bo, Y.S, j - L.g, bor[jj]);
+              } else {
+#pragma unroll
+                for (int s = 0; s < 8; ++s) bor[jj][s] = 0;
+              }
+              dsr[jj] = L.at(Y.dsa, j);
+              dpr[jj] = L.at(Y.dpr, j);
+            }
+          }
 #pragma unroll
           for (uint32_t jj = 0; jj < 8; ++jj) {
             const uint32_t j = jb0 + jj;
             if (jj >= len || L.t_of(j) >= (int)K) break;
-            R bn[8];
-            if (j < L.g + L.S) {
-              load8(L, Y.bo, Y.S, j - L.g, bn);
-            } else {
-#pragma unroll
-              for (int s = 0; s < 8; ++s) bn[s] = 0;
-            }
             R ds, dp;
-            beta_adjoint_step(breg[jj], hsr[jj], hpr[jj], bb, bn, ds, dp);
-            L.at(Y.dsa, j) += ds;
-            L.at(Y.dpr, j) += dp;
+            beta_adjoint_step(breg[jj], hsr[jj], hpr[jj], bb, bor[jj], ds, dp);
+            L.at(Y.dsa, j) = dsr[jj] + ds;
+            L.at(Y.dpr, j) = dpr[jj] + dp;
           }
         }
         // beta of the window top: the tail beta when it reaches K
@@ -757,6 +783,7 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
         const R w = h ? A.scales[h - 1] : R(0);
         const R* eprev = sv + (size_t)(h ? h - 1 : 0) * K;
         QppWalk qw(lo, K, L.f1, L.f2);
+#pragma unroll 4
         for (uint32_t j = L.g; j < L.g + L.S; ++j, qw.next()) {
           const uint32_t t = (uint32_t)L.t_of(j);
           R ds = L.at(Y.dsa, j), dp = L.at(Y.dpr, j);
