@@ -20,8 +20,8 @@
 #include "turbo_cpp/c_api.h"
 
 namespace tcdev {
-__global__ void exact_decode_kernel(const CbInfo*, uint32_t, const float*, float*, float*, int32_t*, uint32_t*,
-                                    uint32_t, int32_t);
+__global__ void exact_decode_kernel(const CbInfo*, const uint32_t*, uint32_t, const float*, float*, float*, int32_t*,
+                                    uint32_t*, uint32_t, int32_t);
 __global__ void window_decode_log_kernel(WinArgs);
 __global__ void window_decode_lin_kernel(WinArgs);
 __global__ void window_decode_log_es_kernel(WinArgs);
@@ -82,6 +82,8 @@ struct tc_plan {
   tcdev::CbInfo* d_cbs = nullptr;
   // exact
   float* d_work = nullptr;
+  uint32_t* d_groups = nullptr;
+  uint32_t n_groups = 0;
   // window
   tcdev::WinJob* d_jobs = nullptr;
   uint32_t* d_cb_index = nullptr;
@@ -93,7 +95,7 @@ struct tc_plan {
   size_t lds_bytes = 0;
 
   ~tc_plan() {
-    for (void* p : {(void*)d_cbs, (void*)d_work, (void*)d_jobs, (void*)d_cb_index, (void*)d_qtab, (void*)d_crc_pow,
+    for (void* p : {(void*)d_cbs, (void*)d_work, (void*)d_groups, (void*)d_jobs, (void*)d_cb_index, (void*)d_qtab, (void*)d_crc_pow,
                     (void*)d_scratch, (void*)d_sched})
       if (p) (void)hipFree(p);
   }
@@ -123,7 +125,7 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
     c.work_off = work;
     llr += 3 * (size_t)(k[i] + 4);
     post += k[i];
-    work += 8 * (size_t)(k[i] + 1) + 4 * (size_t)k[i];
+    work += 16 * (size_t)(k[i] + 1) + 4 * (size_t)k[i];
   }
   p.llr_floats = llr;
   p.post_floats = post;
@@ -131,6 +133,18 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   dev_upload(p.d_cbs, p.cbs);
   if (o.decoder == TC_DECODER_EXACT) {
     p.d_work = dev_alloc<float>(work);
+    // groups of up to eight blocks of one K, one wave each
+    std::map<uint32_t, std::vector<uint32_t>> by_k;
+    for (size_t i = 0; i < n; ++i) by_k[k[i]].push_back(static_cast<uint32_t>(i));
+    std::vector<uint32_t> groups;
+    for (const auto& [kk, list] : by_k) {
+      (void)kk;
+      for (size_t s = 0; s < list.size(); s += 8)
+        for (size_t x = 0; x < 8; ++x) groups.push_back(s + x < list.size() ? list[s + x] : tcdev::kNoBlock);
+    }
+    p.n_groups = static_cast<uint32_t>(groups.size() / 8);
+    p.d_groups = dev_alloc<uint32_t>(groups.size());
+    dev_upload(p.d_groups, groups);
     return;
   }
   // ---- window decoders: group by K, pack blocks into 64-lane jobs ----
@@ -209,10 +223,8 @@ void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* i
   if (p.n_cb == 0) return;
   const tc_options& o = p.opts;
   if (o.decoder == TC_DECODER_EXACT) {
-    const uint32_t blocks = static_cast<uint32_t>((p.n_cb + 63) / 64);
-    hipLaunchKernelGGL(tcdev::exact_decode_kernel, dim3(blocks), dim3(64), 0, s, p.d_cbs,
-                       static_cast<uint32_t>(p.n_cb), llr, post, p.d_work, crc, iters, o.max_iterations,
-                       o.early_stop);
+    hipLaunchKernelGGL(tcdev::exact_decode_kernel, dim3(p.n_groups), dim3(64), 0, s, p.d_cbs, p.d_groups,
+                       p.n_groups, llr, post, p.d_work, crc, iters, o.max_iterations, o.early_stop);
   } else {
     tcdev::WinArgs a{p.d_jobs, p.njobs, p.d_cb_index, p.d_cbs, p.d_qtab, p.d_crc_pow, llr, post, p.d_scratch,
                      crc, iters, o.max_iterations, o.early_stop, tcdev::RxIn{}};

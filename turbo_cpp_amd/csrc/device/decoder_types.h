@@ -9,7 +9,7 @@ namespace tcdev {
 
 // One code block of a batch.  llr_off: start of [d0|d1|d2] (3*(K+4) floats);
 // post_off: start of its K posterior LLRs; work_off: start of its private
-// work space (exact decoder only).
+// work space (exact decoder only: 16 (K+1) + 4K floats).
 struct CbInfo {
   uint64_t llr_off;
   uint64_t post_off;
@@ -19,6 +19,10 @@ struct CbInfo {
   uint32_t f1;
   uint32_t f2;
 };
+
+// Exact decoder: eight code blocks of one K per wave (8 lanes per block),
+// listed eight indices per group; kNoBlock fills a short group.
+constexpr uint32_t kNoBlock = 0xFFFFFFFFu;
 
 // Window decoder: a job is what one 64-lane wavefront decodes at a time:
 // nblk code blocks of the same K, each split into M segments of length S

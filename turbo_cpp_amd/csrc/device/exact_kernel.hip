@@ -1,11 +1,26 @@
-// Exact Log-MAP turbo decoder: one lane per code block, full-length forward
-// and backward recursions with no windowing, evaluated in the reference's
-// operation order (lte_turbo.cpp:380-486, 736-813) so that posterior LLRs
-// agree with turbo_decode_code_block_cpu to rounding of expf/log1pf.
+// Exact Log-MAP turbo decoder: full-length forward and backward recursions
+// with no windowing, evaluated in the reference's operation order
+// (lte_turbo.cpp:380-486, 736-813) so that posterior LLRs agree with
+// turbo_decode_code_block_cpu to rounding of expf/log1pf.
 //
 // This is the parity path behind GpuStage1Exact; the window kernel is the
-// throughput path.  Work space per code block (float offsets from work_off):
-//   beta  (K+1)*8   backward metrics, all steps (read back by the forward pass)
+// throughput path.
+//
+// Layout: one wave decodes up to eight code blocks of one K, eight lanes per
+// block, lane = 8 * block + state.  Each lane carries its own state's metric
+// through the recursions: a step reads the two predecessor (successor)
+// metrics of the block's other lanes with ds_bpermute and evaluates exactly
+// the reference's expression for its state (the two transitions in order of
+// increasing source state, the reference's max* with its infinity rules), and
+// the renormalisation takes the block's maximum across its eight lanes with
+// DPP.  The alpha and beta recursions of a half-iteration are independent and
+// run in the same loop (alpha at step i, beta at step K-1-i), storing every
+// step's metrics; the extrinsics then follow in a pass parallel over steps,
+// each lane doing the reference's sequential LLR accumulation for its steps.
+//
+// Work space per code block (float offsets from work_off):
+//   alpha (K+1)*8   forward metrics, all steps
+//   beta  (K+1)*8   backward metrics, all steps
 //   e1    K         upper extrinsic, natural order
 //   e2    K         lower extrinsic, natural order (= next a-priori)
 //   si    K         systematic LLRs in interleaved order
@@ -33,103 +48,130 @@ __device__ __forceinline__ float jac(float x, float y) {
   return hi + log1pf(expf(lo - hi));
 }
 
-// lte_turbo.cpp:392-402
-__device__ __forceinline__ void renorm(float* m) {
-  float mx = m[0];
-#pragma unroll
-  for (int i = 1; i < 8; ++i) mx = (mx < m[i]) ? m[i] : mx;
-  if (!fin(mx)) return;
-#pragma unroll
-  for (int i = 0; i < 8; ++i)
-    if (fin(m[i])) m[i] -= mx;
-}
-
 __device__ __forceinline__ float bm(unsigned u, unsigned p, float sa, float lp) {
   return 0.5f * ((u ? 1.0f : -1.0f) * sa + (p ? 1.0f : -1.0f) * lp);
 }
 
-// Tail beta (lte_turbo.cpp:357-378) from three systematic/parity tail LLRs.
-__device__ void tail_beta(const float* ts, const float* tp, float* beta) {
-#pragma unroll
-  for (unsigned s = 0; s < 8; ++s) {
-    unsigned cur = s;
-    float metric = 0.0f;
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      const unsigned u = term_in(cur);
-      const Tr tr = rsc(cur, u);
-      metric += 0.5f * ((u ? 1.0f : -1.0f) * ts[t]) + 0.5f * ((tr.parity ? 1.0f : -1.0f) * tp[t]);
-      cur = tr.next;
-    }
-    beta[s] = (cur == 0) ? metric : -kInf;
-  }
-  renorm(beta);
+// metric of lane src (a lane of the same wave)
+__device__ __forceinline__ float from_lane(float v, uint32_t src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(static_cast<int>(src << 2), __builtin_bit_cast(int, v)));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+// maximum over the eight lanes of a block (xor 1, xor 2 within quads, then
+// the mirror within eight lanes)
+__device__ __forceinline__ float group_max(float v) {
+  v = fmaxf(v, dpp<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = fmaxf(v, dpp<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = fmaxf(v, dpp<0x141>(v));  // row_half_mirror
+  return v;
+}
+// lte_turbo.cpp:392-402: subtract the maximum from the finite metrics
+__device__ __forceinline__ float renorm(float m) {
+  const float mx = group_max(m);
+  return (fin(mx) && fin(m)) ? m - mx : m;
 }
 
-// One SISO half.  sys[t], ap[t] (t < K), par[t]; the extrinsic of step t is
-// written to ext[perm ? perm(t) : t] where perm is the QPP when interleaved.
-__device__ void siso(const float* sys, const float* ap, const float* par, uint32_t k,
-                     const float* ts, const float* tp, float* beta, float* ext, bool scatter,
-                     uint32_t f1, uint32_t f2) {
-  float b[8];
-  tail_beta(ts, tp, b);
-  float4* bst = reinterpret_cast<float4*>(beta);
-  bst[2 * k] = make_float4(b[0], b[1], b[2], b[3]);
-  bst[2 * k + 1] = make_float4(b[4], b[5], b[6], b[7]);
-  for (uint32_t t = k; t-- > 0;) {
-    const float sa = sys[t] + ap[t];
-    const float lp = par[t];
-    float nb[8];
-#pragma unroll
-    for (unsigned s = 0; s < 8; ++s) {
-      const Tr t0 = rsc(s, 0), t1 = rsc(s, 1);
-      const float c0 = b[t0.next] + bm(0, t0.parity, sa, lp);
-      const float c1 = b[t1.next] + bm(1, t1.parity, sa, lp);
-      // the reference skips non-finite successors; jac() does the same
-      nb[s] = jac(fin(b[t0.next]) ? c0 : -kInf, fin(b[t1.next]) ? c1 : -kInf);
-    }
-    renorm(nb);
-#pragma unroll
-    for (int s = 0; s < 8; ++s) b[s] = nb[s];
-    bst[2 * t] = make_float4(b[0], b[1], b[2], b[3]);
-    bst[2 * t + 1] = make_float4(b[4], b[5], b[6], b[7]);
+// QPP Pi(t) for t = t0, t0 + 8, t0 + 16, ... : Pi(t + 8) - Pi(t) =
+// 8 f1 + f2 (16 t + 64), which grows by 128 f2 per stride (all mod K).
+struct QppStride8 {
+  uint32_t pi, d, dd, k;
+  __device__ QppStride8(uint32_t t0, uint32_t kk, uint32_t f1, uint32_t f2) : k(kk) {
+    pi = (f1 % kk * t0 % kk + f2 % kk * t0 % kk * t0) % kk;
+    d = (8u * (f1 % kk) % kk + f2 % kk * ((16u * t0 + 64u) % kk)) % kk;
+    dd = 128u * (f2 % kk) % kk;
   }
-  float a[8] = {0.0f, -kInf, -kInf, -kInf, -kInf, -kInf, -kInf, -kInf};
-  // forward: alpha, LLR; QPP by recurrence pi(t+1) = pi(t) + d(t), d(t+1) = d(t) + 2 f2
-  uint32_t pi = 0, d = (f1 + f2) % k;
-  const uint32_t d2 = (2 * f2) % k;
-  for (uint32_t t = 0; t < k; ++t) {
+  __device__ void next() {
+    pi += d;
+    pi = pi >= k ? pi - k : pi;
+    d += dd;
+    d = d >= k ? d - k : d;
+  }
+};
+
+// Tail beta of state st (lte_turbo.cpp:357-378) from three systematic/parity
+// tail LLRs, renormalised across the block.
+__device__ float tail_beta(unsigned st, const float* ts, const float* tp) {
+  unsigned cur = st;
+  float metric = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const unsigned u = term_in(cur);
+    const Tr tr = rsc(cur, u);
+    metric += 0.5f * ((u ? 1.0f : -1.0f) * ts[t]) + 0.5f * ((tr.parity ? 1.0f : -1.0f) * tp[t]);
+    cur = tr.next;
+  }
+  return renorm((cur == 0) ? metric : -kInf);
+}
+
+// One SISO half for this lane's block.  sys[t], ap[t] (t < K), par[t]; the
+// extrinsic of step t is written to ext[scatter ? Pi(t) : t].
+__device__ void siso(unsigned st, uint32_t gbase, const float* sys, const float* ap, const float* par, uint32_t k,
+                     const float* ts, const float* tp, float* alpha, float* beta, float* ext, bool scatter,
+                     uint32_t f1, uint32_t f2, bool active) {
+  // this lane's transitions: backward from st to its successors n0 (u=0)
+  // and n1 (u=1); forward into st from predecessors p, p+1 with input u
+  const Tr t0 = rsc(st, 0), t1 = rsc(st, 1);
+  const unsigned u = in_of(st), p = pred0(st), c = par0(st);
+  float b = tail_beta(st, ts, tp);
+  float a = (st == 0) ? 0.0f : -kInf;
+  if (active) {
+    beta[8 * (size_t)k + st] = b;
+    alpha[st] = a;
+  }
+  float sa_b = sys[k - 1] + ap[k - 1], lp_b = par[k - 1];
+  float sa_a = sys[0] + ap[0], lp_a = par[0];
+  for (uint32_t i = 0; i < k; ++i) {
+    const uint32_t tb = k - 1 - i, ta = i;
+    const float sab = sa_b, lpb = lp_b, saa = sa_a, lpa = lp_a;
+    if (i + 1 < k) {  // next step's inputs, one step ahead
+      sa_b = sys[tb - 1] + ap[tb - 1];
+      lp_b = par[tb - 1];
+      sa_a = sys[ta + 1] + ap[ta + 1];
+      lp_a = par[ta + 1];
+    }
+    // beta_tb(st) from beta_{tb+1}; the reference skips non-finite successors
+    const float b0 = from_lane(b, gbase + t0.next), b1 = from_lane(b, gbase + t1.next);
+    const float cb0 = b0 + bm(0, t0.parity, sab, lpb);
+    const float cb1 = b1 + bm(1, t1.parity, sab, lpb);
+    // alpha_{ta+1}(st) from alpha_ta
+    const float a0 = from_lane(a, gbase + p), a1 = from_lane(a, gbase + p + 1);
+    const float ca0 = a0 + bm(u, c, saa, lpa);
+    const float ca1 = a1 + bm(u, c ^ 1u, saa, lpa);
+    b = renorm(jac(fin(b0) ? cb0 : -kInf, fin(b1) ? cb1 : -kInf));
+    a = renorm(jac(fin(a0) ? ca0 : -kInf, fin(a1) ? ca1 : -kInf));
+    if (active) {
+      beta[8 * (size_t)tb + st] = b;
+      alpha[8 * (size_t)(ta + 1) + st] = a;
+    }
+  }
+  __syncthreads();  // every lane's metrics stored before the extrinsic pass reads them
+  // extrinsics: this lane takes steps st, st + 8, ...; the reference's
+  // accumulation order over (state, input) within each step
+  QppStride8 q(st, k, f1, f2);
+  for (uint32_t t = st; t < k; t += 8, q.next()) {
+    const float4* ap4 = reinterpret_cast<const float4*>(alpha + 8 * (size_t)t);
+    const float4* bp4 = reinterpret_cast<const float4*>(beta + 8 * (size_t)(t + 1));
+    const float4 al = ap4[0], ah = ap4[1], bl = bp4[0], bh = bp4[1];
+    const float av[8] = {al.x, al.y, al.z, al.w, ah.x, ah.y, ah.z, ah.w};
+    const float bn[8] = {bl.x, bl.y, bl.z, bl.w, bh.x, bh.y, bh.z, bh.w};
     const float sa = sys[t] + ap[t];
     const float lp = par[t];
-    const float4 bl = bst[2 * (t + 1)], bh = bst[2 * (t + 1) + 1];
-    const float bn[8] = {bl.x, bl.y, bl.z, bl.w, bh.x, bh.y, bh.z, bh.w};
     float l[2] = {-kInf, -kInf};
 #pragma unroll
     for (unsigned s = 0; s < 8; ++s) {
 #pragma unroll
-      for (unsigned u = 0; u < 2; ++u) {
-        const Tr tr = rsc(s, u);
-        const float m = a[s] + bm(u, tr.parity, sa, lp) + bn[tr.next];
-        l[u] = fin(l[u]) ? jac(l[u], m) : m;
+      for (unsigned uu = 0; uu < 2; ++uu) {
+        const Tr tr = rsc(s, uu);
+        const float m = av[s] + bm(uu, tr.parity, sa, lp) + bn[tr.next];
+        l[uu] = fin(l[uu]) ? jac(l[uu], m) : m;
       }
     }
-    ext[scatter ? pi : t] = (l[1] - l[0]) - sys[t] - ap[t];
-    float na[8];
-#pragma unroll
-    for (unsigned n = 0; n < 8; ++n) {
-      const unsigned u = in_of(n), p = pred0(n), c = par0(n);
-      const float ca = a[p] + bm(u, c, sa, lp);
-      const float cb = a[p + 1] + bm(u, c ^ 1u, sa, lp);
-      na[n] = jac(fin(a[p]) ? ca : -kInf, fin(a[p + 1]) ? cb : -kInf);
-    }
-    renorm(na);
-#pragma unroll
-    for (int s = 0; s < 8; ++s) a[s] = na[s];
-    pi += d;
-    if (pi >= k) pi -= k;
-    d += d2;
-    if (d >= k) d -= k;
+    if (active) ext[scatter ? q.pi : t] = (l[1] - l[0]) - sys[t] - ap[t];
   }
+  __syncthreads();
 }
 
 // CRC24B early-stop test on hard decisions of the posterior, filler bits
@@ -153,67 +195,76 @@ __device__ bool crc24b_ok(const float* post, uint32_t k, uint32_t filler) {
 
 }  // namespace
 
-__global__ __launch_bounds__(64) void exact_decode_kernel(const CbInfo* __restrict__ cbs, uint32_t n_cb,
+// groups: eight code-block indices per wave (kNoBlock where the group has
+// fewer), all of one K.
+__global__ __launch_bounds__(64) void exact_decode_kernel(const CbInfo* __restrict__ cbs,
+                                                          const uint32_t* __restrict__ groups, uint32_t n_groups,
                                                           const float* __restrict__ llr, float* __restrict__ post,
                                                           float* __restrict__ work, int32_t* __restrict__ crc_ok,
                                                           uint32_t* __restrict__ iters, uint32_t max_iterations,
                                                           int32_t early_stop) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_cb) return;
-  const CbInfo cb = cbs[i];
-  const uint32_t k = cb.k;
-  const float* d0 = llr + cb.llr_off;
-  const float* d1 = d0 + (k + 4);
-  const float* d2 = d1 + (k + 4);
-  float* out = post + cb.post_off;
-  float* beta = work + cb.work_off;
-  float* e1 = beta + 8 * (size_t)(k + 1);
-  float* e2 = e1 + k;
-  float* si = e2 + k;
-  float* ia = si + k;
-  // decoder tail read-out of lte_turbo.cpp:758-773
-  const float us[3] = {d0[k + 0], d0[k + 2], 0.0f};
-  const float up[3] = {d1[k + 0], d0[k + 1], d1[k + 2]};
-  const float ls[3] = {d1[k + 3], d2[k + 0], d2[k + 2]};
-  const float lp[3] = {d2[k + 1], d0[k + 3], d2[k + 3]};
-  {
-    uint32_t pi = 0, d = (cb.f1 + cb.f2) % k;
-    const uint32_t dd = (2 * cb.f2) % k;
-    for (uint32_t t = 0; t < k; ++t) {
-      si[t] = d0[pi];
-      e2[t] = 0.0f;
-      out[t] = 0.0f;
-      pi += d;
-      if (pi >= k) pi -= k;
-      d += dd;
-      if (d >= k) d -= k;
-    }
-  }
-  bool ok = false;
-  uint32_t used = 0;
-  for (uint32_t it = 0; it < max_iterations; ++it) {
-    siso(d0, e2, d1, k, us, up, beta, e1, false, cb.f1, cb.f2);
-    {
-      uint32_t pi = 0, d = (cb.f1 + cb.f2) % k;
-      const uint32_t dd = (2 * cb.f2) % k;
-      for (uint32_t t = 0; t < k; ++t) {
-        ia[t] = e1[pi];
-        pi += d;
-        if (pi >= k) pi -= k;
-        d += dd;
-        if (d >= k) d -= k;
+  const uint32_t lane = threadIdx.x, g = lane >> 3, st = lane & 7u, gbase = lane & ~7u;
+  for (uint32_t gi = blockIdx.x; gi < n_groups; gi += gridDim.x) {
+    const uint32_t id = groups[8 * gi + g];
+    const bool active = id != kNoBlock;
+    const CbInfo cb = cbs[active ? id : groups[8 * gi]];  // idle blocks shadow the group's first
+    const uint32_t k = cb.k;
+    const float* d0 = llr + cb.llr_off;
+    const float* d1 = d0 + (k + 4);
+    const float* d2 = d1 + (k + 4);
+    float* out = post + cb.post_off;
+    float* alpha = work + cb.work_off;
+    float* beta = alpha + 8 * (size_t)(k + 1);
+    float* e1 = beta + 8 * (size_t)(k + 1);
+    float* e2 = e1 + k;
+    float* si = e2 + k;
+    float* ia = si + k;
+    // decoder tail read-out of lte_turbo.cpp:758-773
+    const float us[3] = {d0[k + 0], d0[k + 2], 0.0f};
+    const float up[3] = {d1[k + 0], d0[k + 1], d1[k + 2]};
+    const float ls[3] = {d1[k + 3], d2[k + 0], d2[k + 2]};
+    const float lp[3] = {d2[k + 1], d0[k + 3], d2[k + 3]};
+    if (active) {
+      QppStride8 q(st, k, cb.f1, cb.f2);
+      for (uint32_t t = st; t < k; t += 8, q.next()) {
+        si[t] = d0[q.pi];
+        e2[t] = 0.0f;
+        out[t] = 0.0f;
       }
     }
-    siso(si, ia, d2, k, ls, lp, beta, e2, true, cb.f1, cb.f2);
-    for (uint32_t t = 0; t < k; ++t) out[t] = d0[t] + e1[t] + e2[t];
-    used = it + 1;
-    if (early_stop) {
-      ok = crc24b_ok(out, k, cb.filler);
-      if (ok) break;
+    __syncthreads();
+    // a block whose check passed is done: its lanes stop writing, so its
+    // posteriors stay those of the passing iteration, while the wave goes on
+    // for its other blocks
+    bool ok = false, done = false;
+    uint32_t used = 0;
+    for (uint32_t it = 0; it < max_iterations; ++it) {
+      const bool wr = active && !done;
+      siso(st, gbase, d0, e2, d1, k, us, up, alpha, beta, e1, false, cb.f1, cb.f2, wr);
+      if (wr) {
+        QppStride8 q(st, k, cb.f1, cb.f2);
+        for (uint32_t t = st; t < k; t += 8, q.next()) ia[t] = e1[q.pi];
+      }
+      __syncthreads();
+      siso(st, gbase, si, ia, d2, k, ls, lp, alpha, beta, e2, true, cb.f1, cb.f2, wr);
+      if (wr)
+        for (uint32_t t = st; t < k; t += 8) out[t] = d0[t] + e1[t] + e2[t];
+      __syncthreads();
+      if (!done) used = it + 1;
+      if (early_stop) {
+        // one lane per block runs the reference's bitwise division; the
+        // verdict goes to all eight lanes of the block
+        const bool mine = (st == 0 && wr) ? crc24b_ok(out, k, cb.filler) : false;
+        if (from_lane(mine ? 1.0f : 0.0f, gbase) != 0.0f) ok = done = true;
+        if (__syncthreads_and(done || !active)) break;
+      }
     }
+    if (active && st == 0) {
+      if (crc_ok) crc_ok[id] = ok ? 1 : 0;
+      if (iters) iters[id] = used;
+    }
+    __syncthreads();
   }
-  if (crc_ok) crc_ok[i] = ok ? 1 : 0;
-  if (iters) iters[i] = used;
 }
 
 }  // namespace tcdev
