@@ -133,16 +133,17 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   dev_upload(p.d_cbs, p.cbs);
   if (o.decoder == TC_DECODER_EXACT) {
     p.d_work = dev_alloc<float>(work);
-    // groups of up to eight blocks of one K, one wave each
+    // groups of up to kExactBlocksPerWave blocks of one K, one wave each
+    constexpr size_t B = tcdev::kExactBlocksPerWave;
     std::map<uint32_t, std::vector<uint32_t>> by_k;
     for (size_t i = 0; i < n; ++i) by_k[k[i]].push_back(static_cast<uint32_t>(i));
     std::vector<uint32_t> groups;
     for (const auto& [kk, list] : by_k) {
       (void)kk;
-      for (size_t s = 0; s < list.size(); s += 8)
-        for (size_t x = 0; x < 8; ++x) groups.push_back(s + x < list.size() ? list[s + x] : tcdev::kNoBlock);
+      for (size_t s = 0; s < list.size(); s += B)
+        for (size_t x = 0; x < B; ++x) groups.push_back(s + x < list.size() ? list[s + x] : tcdev::kNoBlock);
     }
-    p.n_groups = static_cast<uint32_t>(groups.size() / 8);
+    p.n_groups = static_cast<uint32_t>(groups.size() / B);
     p.d_groups = dev_alloc<uint32_t>(groups.size());
     dev_upload(p.d_groups, groups);
     return;

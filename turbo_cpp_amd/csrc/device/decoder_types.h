@@ -20,9 +20,10 @@ struct CbInfo {
   uint32_t f2;
 };
 
-// Exact decoder: eight code blocks of one K per wave (8 lanes per block),
-// listed eight indices per group; kNoBlock fills a short group.
+// Exact decoder: kExactBlocksPerWave code blocks of one K per wave, listed
+// that many indices per group; kNoBlock fills a short group.
 constexpr uint32_t kNoBlock = 0xFFFFFFFFu;
+constexpr uint32_t kExactBlocksPerWave = 2;
 
 // Window decoder: a job is what one 64-lane wavefront decodes at a time:
 // nblk code blocks of the same K, each split into M segments of length S

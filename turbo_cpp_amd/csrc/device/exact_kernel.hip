@@ -6,8 +6,10 @@
 // This is the parity path behind GpuStage1Exact; the window kernel is the
 // throughput path.
 //
-// Layout: one wave decodes up to eight code blocks of one K, eight lanes per
-// block, lane = 8 * block + state.  Each lane carries its own state's metric
+// Layout: one wave decodes up to kExactBlocksPerWave code blocks of one K,
+// kExactLanes lanes per block; the first eight lanes of a block carry one
+// state each through the recursions, and all of the block's lanes share its
+// extrinsic pass.  Each recursion lane carries its own state's metric
 // through the recursions: a step reads the two predecessor (successor)
 // metrics of the block's other lanes with ds_bpermute and evaluates exactly
 // the reference's expression for its state (the two transitions in order of
@@ -17,6 +19,8 @@
 // run in the same loop (alpha at step i, beta at step K-1-i), storing every
 // step's metrics; the extrinsics then follow in a pass parallel over steps,
 // each lane doing the reference's sequential LLR accumulation for its steps.
+// Two blocks per wave (rather than eight) give two waves per SIMD over a
+// 4096-block batch and four times the lanes per block in the extrinsic pass.
 //
 // Work space per code block (float offsets from work_off):
 //   alpha (K+1)*8   forward metrics, all steps
@@ -74,14 +78,18 @@ __device__ __forceinline__ float renorm(float m) {
   return (fin(mx) && fin(m)) ? m - mx : m;
 }
 
-// QPP Pi(t) for t = t0, t0 + 8, t0 + 16, ... : Pi(t + 8) - Pi(t) =
-// 8 f1 + f2 (16 t + 64), which grows by 128 f2 per stride (all mod K).
-struct QppStride8 {
+constexpr uint32_t kExactLanes = 64 / kExactBlocksPerWave;
+
+// QPP Pi(t) for t = t0, t0 + L, t0 + 2L, ... (L = kExactLanes): Pi(t + L) -
+// Pi(t) = L f1 + f2 (2 L t + L^2), which grows by 2 L^2 f2 per stride (all
+// mod K; K <= 6144 keeps every product within 32 bits).
+struct QppStride {
   uint32_t pi, d, dd, k;
-  __device__ QppStride8(uint32_t t0, uint32_t kk, uint32_t f1, uint32_t f2) : k(kk) {
+  __device__ QppStride(uint32_t t0, uint32_t kk, uint32_t f1, uint32_t f2) : k(kk) {
+    constexpr uint32_t L = kExactLanes;
     pi = (f1 % kk * t0 % kk + f2 % kk * t0 % kk * t0) % kk;
-    d = (8u * (f1 % kk) % kk + f2 % kk * ((16u * t0 + 64u) % kk)) % kk;
-    dd = 128u * (f2 % kk) % kk;
+    d = (L * (f1 % kk) % kk + f2 % kk * ((2u * L * t0 + L * L) % kk)) % kk;
+    dd = 2u * L * L % kk * (f2 % kk) % kk;
   }
   __device__ void next() {
     pi += d;
@@ -108,16 +116,20 @@ __device__ float tail_beta(unsigned st, const float* ts, const float* tp) {
 
 // One SISO half for this lane's block.  sys[t], ap[t] (t < K), par[t]; the
 // extrinsic of step t is written to ext[scatter ? Pi(t) : t].
-__device__ void siso(unsigned st, uint32_t gbase, const float* sys, const float* ap, const float* par, uint32_t k,
+// r: the lane's index within its block; lanes r < 8 (state st = r) run the
+// recursions.
+__device__ void siso(unsigned r, uint32_t gbase, const float* sys, const float* ap, const float* par, uint32_t k,
                      const float* ts, const float* tp, float* alpha, float* beta, float* ext, bool scatter,
                      uint32_t f1, uint32_t f2, bool active) {
+  const unsigned st = r & 7u;
+  const bool rec = active && r < 8;  // stores of the recursions
   // this lane's transitions: backward from st to its successors n0 (u=0)
   // and n1 (u=1); forward into st from predecessors p, p+1 with input u
   const Tr t0 = rsc(st, 0), t1 = rsc(st, 1);
   const unsigned u = in_of(st), p = pred0(st), c = par0(st);
   float b = tail_beta(st, ts, tp);
   float a = (st == 0) ? 0.0f : -kInf;
-  if (active) {
+  if (rec) {
     beta[8 * (size_t)k + st] = b;
     alpha[st] = a;
   }
@@ -142,16 +154,16 @@ __device__ void siso(unsigned st, uint32_t gbase, const float* sys, const float*
     const float ca1 = a1 + bm(u, c ^ 1u, saa, lpa);
     b = renorm(jac(fin(b0) ? cb0 : -kInf, fin(b1) ? cb1 : -kInf));
     a = renorm(jac(fin(a0) ? ca0 : -kInf, fin(a1) ? ca1 : -kInf));
-    if (active) {
+    if (rec) {
       beta[8 * (size_t)tb + st] = b;
       alpha[8 * (size_t)(ta + 1) + st] = a;
     }
   }
   __syncthreads();  // every lane's metrics stored before the extrinsic pass reads them
-  // extrinsics: this lane takes steps st, st + 8, ...; the reference's
+  // extrinsics: this lane takes steps r, r + L, ...; the reference's
   // accumulation order over (state, input) within each step
-  QppStride8 q(st, k, f1, f2);
-  for (uint32_t t = st; t < k; t += 8, q.next()) {
+  QppStride q(r, k, f1, f2);
+  for (uint32_t t = r; t < k; t += kExactLanes, q.next()) {
     const float4* ap4 = reinterpret_cast<const float4*>(alpha + 8 * (size_t)t);
     const float4* bp4 = reinterpret_cast<const float4*>(beta + 8 * (size_t)(t + 1));
     const float4 al = ap4[0], ah = ap4[1], bl = bp4[0], bh = bp4[1];
@@ -195,19 +207,20 @@ __device__ bool crc24b_ok(const float* post, uint32_t k, uint32_t filler) {
 
 }  // namespace
 
-// groups: eight code-block indices per wave (kNoBlock where the group has
-// fewer), all of one K.
-__global__ __launch_bounds__(64) void exact_decode_kernel(const CbInfo* __restrict__ cbs,
+// groups: kExactBlocksPerWave code-block indices per wave (kNoBlock where
+// the group has fewer), all of one K.
+__global__ __launch_bounds__(64, 2) void exact_decode_kernel(const CbInfo* __restrict__ cbs,
                                                           const uint32_t* __restrict__ groups, uint32_t n_groups,
                                                           const float* __restrict__ llr, float* __restrict__ post,
                                                           float* __restrict__ work, int32_t* __restrict__ crc_ok,
                                                           uint32_t* __restrict__ iters, uint32_t max_iterations,
                                                           int32_t early_stop) {
-  const uint32_t lane = threadIdx.x, g = lane >> 3, st = lane & 7u, gbase = lane & ~7u;
+  constexpr uint32_t B = kExactBlocksPerWave;
+  const uint32_t lane = threadIdx.x, g = lane / kExactLanes, r = lane % kExactLanes, gbase = g * kExactLanes;
   for (uint32_t gi = blockIdx.x; gi < n_groups; gi += gridDim.x) {
-    const uint32_t id = groups[8 * gi + g];
+    const uint32_t id = groups[B * gi + g];
     const bool active = id != kNoBlock;
-    const CbInfo cb = cbs[active ? id : groups[8 * gi]];  // idle blocks shadow the group's first
+    const CbInfo cb = cbs[active ? id : groups[B * gi]];  // idle blocks shadow the group's first
     const uint32_t k = cb.k;
     const float* d0 = llr + cb.llr_off;
     const float* d1 = d0 + (k + 4);
@@ -225,8 +238,8 @@ __global__ __launch_bounds__(64) void exact_decode_kernel(const CbInfo* __restri
     const float ls[3] = {d1[k + 3], d2[k + 0], d2[k + 2]};
     const float lp[3] = {d2[k + 1], d0[k + 3], d2[k + 3]};
     if (active) {
-      QppStride8 q(st, k, cb.f1, cb.f2);
-      for (uint32_t t = st; t < k; t += 8, q.next()) {
+      QppStride q(r, k, cb.f1, cb.f2);
+      for (uint32_t t = r; t < k; t += kExactLanes, q.next()) {
         si[t] = d0[q.pi];
         e2[t] = 0.0f;
         out[t] = 0.0f;
@@ -240,26 +253,26 @@ __global__ __launch_bounds__(64) void exact_decode_kernel(const CbInfo* __restri
     uint32_t used = 0;
     for (uint32_t it = 0; it < max_iterations; ++it) {
       const bool wr = active && !done;
-      siso(st, gbase, d0, e2, d1, k, us, up, alpha, beta, e1, false, cb.f1, cb.f2, wr);
+      siso(r, gbase, d0, e2, d1, k, us, up, alpha, beta, e1, false, cb.f1, cb.f2, wr);
       if (wr) {
-        QppStride8 q(st, k, cb.f1, cb.f2);
-        for (uint32_t t = st; t < k; t += 8, q.next()) ia[t] = e1[q.pi];
+        QppStride q(r, k, cb.f1, cb.f2);
+        for (uint32_t t = r; t < k; t += kExactLanes, q.next()) ia[t] = e1[q.pi];
       }
       __syncthreads();
-      siso(st, gbase, si, ia, d2, k, ls, lp, alpha, beta, e2, true, cb.f1, cb.f2, wr);
+      siso(r, gbase, si, ia, d2, k, ls, lp, alpha, beta, e2, true, cb.f1, cb.f2, wr);
       if (wr)
-        for (uint32_t t = st; t < k; t += 8) out[t] = d0[t] + e1[t] + e2[t];
+        for (uint32_t t = r; t < k; t += kExactLanes) out[t] = d0[t] + e1[t] + e2[t];
       __syncthreads();
       if (!done) used = it + 1;
       if (early_stop) {
         // one lane per block runs the reference's bitwise division; the
         // verdict goes to all eight lanes of the block
-        const bool mine = (st == 0 && wr) ? crc24b_ok(out, k, cb.filler) : false;
+        const bool mine = (r == 0 && wr) ? crc24b_ok(out, k, cb.filler) : false;
         if (from_lane(mine ? 1.0f : 0.0f, gbase) != 0.0f) ok = done = true;
         if (__syncthreads_and(done || !active)) break;
       }
     }
-    if (active && st == 0) {
+    if (active && r == 0) {
       if (crc_ok) crc_ok[id] = ok ? 1 : 0;
       if (iters) iters[id] = used;
     }
