@@ -133,31 +133,46 @@ __device__ void siso(unsigned r, uint32_t gbase, const float* sys, const float* 
     beta[8 * (size_t)k + st] = b;
     alpha[st] = a;
   }
-  float sa_b = sys[k - 1] + ap[k - 1], lp_b = par[k - 1];
-  float sa_a = sys[0] + ap[0], lp_a = par[0];
-  for (uint32_t i = 0; i < k; ++i) {
-    const uint32_t tb = k - 1 - i, ta = i;
-    const float sab = sa_b, lpb = lp_b, saa = sa_a, lpa = lp_a;
-    if (i + 1 < k) {  // next step's inputs, one step ahead
-      sa_b = sys[tb - 1] + ap[tb - 1];
-      lp_b = par[tb - 1];
-      sa_a = sys[ta + 1] + ap[ta + 1];
-      lp_a = par[ta + 1];
+  // inputs by chunks of kExactLanes steps, loaded one chunk ahead, one step
+  // per lane of the block (alpha side: step i0 + r; beta side: step
+  // k - 1 - i0 - r), and handed to the recursion lanes with ds_bpermute
+  float ca_s, ca_p, cb_s, cb_p, na_s = 0.0f, na_p = 0.0f, nb_s = 0.0f, nb_p = 0.0f;
+  auto load_chunk = [&](uint32_t i0, float& as, float& apv, float& bs, float& bp) {
+    const uint32_t i = i0 + r;
+    if (i < k) {
+      const uint32_t tb = k - 1 - i;
+      as = sys[i] + ap[i];
+      apv = par[i];
+      bs = sys[tb] + ap[tb];
+      bp = par[tb];
+    } else {
+      as = apv = bs = bp = 0.0f;
     }
-    // beta_tb(st) from beta_{tb+1}; the reference skips non-finite successors
-    const float b0 = from_lane(b, gbase + t0.next), b1 = from_lane(b, gbase + t1.next);
-    const float cb0 = b0 + bm(0, t0.parity, sab, lpb);
-    const float cb1 = b1 + bm(1, t1.parity, sab, lpb);
-    // alpha_{ta+1}(st) from alpha_ta
-    const float a0 = from_lane(a, gbase + p), a1 = from_lane(a, gbase + p + 1);
-    const float ca0 = a0 + bm(u, c, saa, lpa);
-    const float ca1 = a1 + bm(u, c ^ 1u, saa, lpa);
-    b = renorm(jac(fin(b0) ? cb0 : -kInf, fin(b1) ? cb1 : -kInf));
-    a = renorm(jac(fin(a0) ? ca0 : -kInf, fin(a1) ? ca1 : -kInf));
-    if (rec) {
-      beta[8 * (size_t)tb + st] = b;
-      alpha[8 * (size_t)(ta + 1) + st] = a;
+  };
+  load_chunk(0, ca_s, ca_p, cb_s, cb_p);
+  for (uint32_t i0 = 0; i0 < k; i0 += kExactLanes) {
+    if (i0 + kExactLanes < k) load_chunk(i0 + kExactLanes, na_s, na_p, nb_s, nb_p);
+    const uint32_t n = min(kExactLanes, k - i0);
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint32_t i = i0 + j, tb = k - 1 - i, ta = i;
+      const float saa = from_lane(ca_s, gbase + j), lpa = from_lane(ca_p, gbase + j);
+      const float sab = from_lane(cb_s, gbase + j), lpb = from_lane(cb_p, gbase + j);
+      // beta_tb(st) from beta_{tb+1}; the reference skips non-finite successors
+      const float b0 = from_lane(b, gbase + t0.next), b1 = from_lane(b, gbase + t1.next);
+      const float cb0 = b0 + bm(0, t0.parity, sab, lpb);
+      const float cb1 = b1 + bm(1, t1.parity, sab, lpb);
+      // alpha_{ta+1}(st) from alpha_ta
+      const float a0 = from_lane(a, gbase + p), a1 = from_lane(a, gbase + p + 1);
+      const float ca0 = a0 + bm(u, c, saa, lpa);
+      const float ca1 = a1 + bm(u, c ^ 1u, saa, lpa);
+      b = renorm(jac(fin(b0) ? cb0 : -kInf, fin(b1) ? cb1 : -kInf));
+      a = renorm(jac(fin(a0) ? ca0 : -kInf, fin(a1) ? ca1 : -kInf));
+      if (rec) {
+        beta[8 * (size_t)tb + st] = b;
+        alpha[8 * (size_t)(ta + 1) + st] = a;
+      }
     }
+    ca_s = na_s, ca_p = na_p, cb_s = nb_s, cb_p = nb_p;
   }
   __syncthreads();  // every lane's metrics stored before the extrinsic pass reads them
   // extrinsics: this lane takes steps r, r + L, ...; the reference's
