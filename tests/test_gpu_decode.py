@@ -260,6 +260,29 @@ def _crc_blocks(ks, fillers, ebn0s, seed):
     return bits, llrs
 
 
+def test_exact_early_stop_blocks_sharing_a_wave():
+    """The exact decoder runs two blocks of one K per wave in lockstep; a block
+    whose CRC24B passes must freeze (posterior and iteration count of the
+    passing iteration) while its partner goes on.  Each block of the batch
+    equals its own single-block decode with early stop, bit for bit, and the
+    verdicts match the reference's decoder on the CPU."""
+    ks = [1024, 1024, 1024, 1024, 6144, 6144, 40]
+    fs = [0, 8, 0, 0, 0, 24, 0]
+    eb = [2.0, -1.0, 0.6, 1.5, -1.0, 1.0, 3.0]
+    _, llrs = _crc_blocks(ks, fs, eb, 91)
+    post, crc, its = T.decode_batch_host(ks, np.concatenate(llrs), T.DECODER_EXACT, 8, True, fillers=fs)
+    off = 0
+    for k, f, l, c, i in zip(ks, fs, llrs, crc, its):
+        d = T.CodeBlockDescriptor(0, k, k + 4, filler_count=f)
+        single, ok, used = T.turbo_decode_code_block_gpu_stage1(d, *_split(l, k), 8, True)
+        cpu, ok_cpu, used_cpu = T.turbo_decode_code_block_cpu(d, *_split(l, k), 8, True)
+        _say(f"K={k} F={f}: batch it={int(i)} crc={bool(c)} single it={used} crc={ok} cpu it={used_cpu} crc={ok_cpu}")
+        assert (int(i), bool(c)) == (used, ok) == (used_cpu, ok_cpu)
+        assert np.array_equal(post[off:off + k].view(np.uint32), single.view(np.uint32))
+        off += k
+    assert crc.sum() >= 3 and not crc[1] and not crc[4]
+
+
 @pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])
 def test_window_early_stop_matches_oracle(decoder):
     """Stopping iteration and CRC verdict per block equal the oracle's
