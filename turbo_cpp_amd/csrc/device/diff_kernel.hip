@@ -71,16 +71,18 @@ __host__ __device__ constexpr unsigned par(unsigned s, unsigned u) {
 // input bit of the termination step from state s (lte_turbo.cpp:343-353)
 __host__ __device__ constexpr unsigned term_in(unsigned s) { return ((s >> 1) ^ (s >> 2)) & 1u; }
 
-// Jacobian logarithm and the weights of its two arguments.
+// Jacobian logarithm and the weights of its two arguments (max and |a - b|:
+// a - b and b - a differ only in sign, so this is the same value as taking
+// the larger minus the smaller).
 template <typename R>
 __device__ inline R jac(R a, R b) {
-  const R m = a > b ? a : b;
-  const R d = a > b ? a - b : b - a;
+  const R m = fmax(a, b);
+  const R d = fabs(a - b);
   return m + Num<R>::ln(R(1) + Num<R>::ex(-d));
 }
 template <typename R>
 __device__ inline void jac_w(R a, R b, R& wa, R& wb) {
-  const R d = a > b ? a - b : b - a;
+  const R d = fabs(a - b);
   const R e = Num<R>::ex(-d);
   const R big = Num<R>::rcp(R(1) + e);
   const R small = e * big;
@@ -97,7 +99,7 @@ template <typename R>
 __device__ inline void normalise(R (&v)[8]) {
   R m = v[0];
 #pragma unroll
-  for (int s = 1; s < 8; ++s) m = v[s] > m ? v[s] : m;
+  for (int s = 1; s < 8; ++s) m = fmax(v[s], m);
 #pragma unroll
   for (int s = 0; s < 8; ++s) v[s] -= m;
 }
@@ -138,7 +140,7 @@ __device__ inline void llr_terms(const R (&a)[8], const R (&b)[8], R hs, R hp, R
 #pragma unroll
     for (unsigned s = 0; s < 8; ++s) {
       m[s] = a[s] + gam(u, par(s, u), hs, hp) + b[nxt(s, u)];
-      mx = m[s] > mx ? m[s] : mx;
+      mx = fmax(m[s], mx);
     }
     R sum = 0;
 #pragma unroll
@@ -191,6 +193,14 @@ struct QppWalk {
     d += d2;
     d = d >= k ? d - k : d;
   }
+  // next() when c, else nothing (no branch)
+  __device__ void next_if(bool c) {
+    uint32_t p = pi + d, dd = d + d2;
+    p = p >= k ? p - k : p;
+    dd = dd >= k ? dd - k : dd;
+    pi = c ? p : pi;
+    d = c ? dd : d;
+  }
   __device__ void prev() {
     d = d >= d2 ? d - d2 : d + k - d2;
     pi = pi >= d ? pi - d : pi + k - d;
@@ -220,42 +230,149 @@ struct Lane {
   uint32_t K, S, M, g, q, f1, f2;
   bool active;
   uint32_t lane;
-  R* sc;  // this wave's scratch, element (field + j) * 64 + lane
-  __device__ R& at(uint32_t field, uint32_t j) const { return sc[(size_t)(field + j) * 64 + lane]; }
-  __device__ R& nb(uint32_t field, uint32_t j, uint32_t other) const { return sc[(size_t)(field + j) * 64 + other]; }
+  R* sc;  // this wave's scratch (uniform), element (field + j) * 64 + lane
+  // 32-bit byte offsets from the wave's base (a slot is far below 4 GiB):
+  // one 32-bit offset per field and the steps as immediate offsets, not a
+  // 64-bit address per access
+  __device__ R& nb(uint32_t field, uint32_t j, uint32_t other) const {
+    return *reinterpret_cast<R*>(reinterpret_cast<char*>(sc) + (uint32_t)(((field + j) * 64u + other) * sizeof(R)));
+  }
+  __device__ R& at(uint32_t field, uint32_t j) const { return nb(field, j, lane); }
   // local step j covers t = qS - g + j
   __device__ int t_of(uint32_t j) const { return (int)(q * S) - (int)g + (int)j; }
 };
 
+// The lane with its scratch base and lane index made opaque to the
+// optimiser.  Each pass starts from such a copy, so the per-step scratch
+// offsets of a pass are not hoisted out of the half-iteration loop (where
+// they would hold registers through every other pass).
+template <typename R>
+__device__ __forceinline__ Lane<R> fresh(const Lane<R>& L0) {
+  Lane<R> L = L0;
+  asm volatile("" : "+s"(L.sc), "+v"(L.lane));
+  return L;
+}
+
+// Memory order and control flow of every sequential pass below.  A job is
+// one wave and a batch rarely has more jobs than SIMDs, so no other wave
+// hides a wait.  Each pass therefore walks its steps in chunks of 8 with two
+// input buffers used in turn: the loads of chunk c + 1 go out before chunk c
+// is computed and stored, and no register copy sits between a load and its
+// use (a copy would wait for the load at the copy).  The vector-memory
+// counter retires loads and stores in issue order, so a load issued behind a
+// store would also wait for that store; issued ahead, it waits only for
+// stores of earlier chunks.  Loop bounds are wave-uniform (all lanes of a job
+// share K, S and g); steps that are outside the block for some lanes
+// (t < 0 in the first segment, t >= K in the last) are computed by every
+// lane and their results selected away, so no per-lane branch splits a
+// chunk.  Values computed that way are only ever written to positions that
+// no later pass reads.
+
+// Runs body(j0, buf) for j0 = lo, lo + 8, ... < hi with load(j0, buf) one
+// chunk ahead, alternating two buffers.
+template <class Buf, class Load, class Body>
+__device__ __forceinline__ void chunks_up(uint32_t lo, uint32_t hi, Load&& load, Body&& body) {
+  if (lo >= hi) return;
+  Buf b0, b1;
+  load(lo, b0);
+  for (uint32_t j0 = lo; j0 < hi; j0 += 16) {
+    if (j0 + 8 < hi) load(j0 + 8, b1);
+    body(j0, b0);
+    if (j0 + 8 >= hi) break;
+    if (j0 + 16 < hi) load(j0 + 16, b0);
+    body(j0 + 8, b1);
+  }
+}
+// The same from the top down: body(top, buf) covers steps top - 1 down to
+// max(top - 8, lo).
+template <class Buf, class Load, class Body>
+__device__ __forceinline__ void chunks_down(int lo, int hi, Load&& load, Body&& body) {
+  if (hi <= lo) return;
+  Buf b0, b1;
+  load(hi, b0);
+  for (int top = hi; top > lo; top -= 16) {
+    if (top - 8 > lo) load(top - 8, b1);
+    body(top, b0);
+    if (top - 8 <= lo) break;
+    if (top - 16 > lo) load(top - 16, b0);
+    body(top - 8, b1);
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void select8(bool c, const R (&x)[8], R (&v)[8]) {
+#pragma unroll
+  for (int s = 0; s < 8; ++s) v[s] = c ? x[s] : v[s];
+}
+
+// Staged inputs (sys + la, par) of up to 8 consecutive local steps: entry jj
+// is step j0 + jj (ascending) or j0 - jj (descending).
+template <typename R>
+struct In8 {
+  R sa[8], pr[8];
+};
+template <typename R, bool DOWN = false>
+__device__ __forceinline__ void load_in8(const Lane<R>& L, const Layout& Y, int j0, int n, In8<R>& c) {
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj)
+    if (jj < n) {
+      const uint32_t j = (uint32_t)(DOWN ? j0 - jj : j0 + jj);
+      c.sa[jj] = L.at(Y.sa, j);
+      c.pr[jj] = L.at(Y.pr, j);
+    }
+}
+
 // Stage sys + la and par of one half over the lane's window.  Upper half
 // (h even): natural order, la = w[h-1] e_{h-1}[t].  Lower half: position i
 // reads d0[Pi(i)], d2[i] and la = w[h-1] e_{h-1}[Pi(i)] (extrinsics are kept
-// in natural order).
+// in natural order).  Steps outside [0, K) stage zeros (their loads read a
+// clamped, valid position).
 template <typename R>
-__device__ void stage(const Lane<R>& L, const Layout& Y, uint32_t h, const R* d0, const R* d1, const R* d2,
-                      const R* sv, const R* scales) {
+struct StageIn {
+  R s[8], e[8], p[8];  // d0, e_{h-1} (at t or Pi(t)), parity
+};
+
+template <typename R>
+__device__ __forceinline__ void stage(const Lane<R>& L0, const Layout& Y, uint32_t h, const R* d0, const R* d1,
+                                      const R* d2, const R* sv, const R* scales) {
+  const Lane<R> L = fresh(L0);
   const R w = h ? scales[h - 1] : R(0);
   const R* eprev = sv + (size_t)(h ? h - 1 : 0) * L.K;
+  const bool lower = (h & 1u) != 0;
+  const R* dp = lower ? d2 : d1;
   const int t0 = L.t_of(0);
   QppWalk qw(t0 > 0 ? (uint32_t)t0 : 0u, L.K, L.f1, L.f2);
-#pragma unroll 8
-  for (uint32_t j = 0; j < Y.P; ++j) {
+  auto in_block = [&](uint32_t j) {
     const int t = L.t_of(j);
-    R sa = 0, pr = 0;
-    if (t >= 0 && t < (int)L.K) {
-      if ((h & 1u) == 0) {
-        sa = d0[t] + (h ? w * eprev[t] : R(0));
-        pr = d1[t];
-      } else {
-        const uint32_t pi = qw.pi;
-        sa = d0[pi] + w * eprev[pi];
-        pr = d2[t];
+    return t >= 0 && t < (int)L.K;
+  };
+  auto load = [&](uint32_t j0, StageIn<R>& c) {
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      if (j0 + jj < Y.P) {
+        const int t = L.t_of(j0 + jj);
+        const uint32_t tc = (uint32_t)min(max(t, 0), (int)L.K - 1);
+        const uint32_t i = lower ? qw.pi : tc;
+        c.s[jj] = d0[i];
+        c.e[jj] = h ? eprev[i] : R(0);
+        c.p[jj] = dp[tc];
+        qw.next_if(in_block(j0 + jj));
       }
-      qw.next();
     }
-    L.at(Y.sa, j) = sa;
-    L.at(Y.pr, j) = pr;
-  }
+  };
+  auto body = [&](uint32_t j0, const StageIn<R>& c) {
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      const uint32_t j = j0 + jj;
+      if (j < Y.P) {
+        const R sa = lower ? c.s[jj] + w * c.e[jj] : c.s[jj] + (h ? w * c.e[jj] : R(0));
+        const bool v = in_block(j);
+        L.at(Y.sa, j) = v ? sa : R(0);
+        L.at(Y.pr, j) = v ? c.p[jj] : R(0);
+      }
+    }
+  };
+  chunks_up<StageIn<R>>(0, Y.P, load, body);
 }
 
 template <typename R>
@@ -278,41 +395,50 @@ __device__ void tail_inputs(uint32_t h, const R* d0, const R* d1, const R* d2, u
   }
 }
 
-// alpha over local steps 0 .. g + S, stored at al[j] (alpha_t for t = t_of(j)).
-// The next step's inputs are loaded before this step's stores, so their
-// latency overlaps the recursion.
+// alpha over local steps 0 .. g + S, stored at al[j] (alpha_t for t = t_of(j));
+// before step 0 of the block (t < 0) alpha stays at its start value.
 template <typename R>
-__device__ void alpha_pass(const Lane<R>& L, const Layout& Y) {
+__device__ __forceinline__ void alpha_pass(const Lane<R>& L0, const Layout& Y) {
+  const Lane<R> L = fresh(L0);
   R a[8];
   const bool from_zero_state = (int)(L.q * L.S) - (int)L.g <= 0;
 #pragma unroll
   for (int s = 0; s < 8; ++s) a[s] = from_zero_state ? (s == 0 ? R(0) : R(kNegF)) : R(0);
   const uint32_t n = L.g + L.S;
-  R sa_n = L.at(Y.sa, 0), pr_n = L.at(Y.pr, 0);
-  for (uint32_t j = 0; j < n; ++j) {
-    const R sa = sa_n, pr = pr_n;
-    if (j + 1 < n) {
-      sa_n = L.at(Y.sa, j + 1);
-      pr_n = L.at(Y.pr, j + 1);
+  const uint32_t st = Y.g + Y.S + 1;
+  auto load = [&](uint32_t j0, In8<R>& c) { load_in8(L, Y, (int)j0, (int)min(8u, n - j0), c); };
+  auto body = [&](uint32_t j0, const In8<R>& c) {
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      const uint32_t j = j0 + jj;
+      if (j < n) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) L.at(Y.al + s * st, j) = a[s];
+        R o[8];
+        alpha_step(a, R(0.5) * c.sa[jj], R(0.5) * c.pr[jj], o);
+        select8(L.t_of(j) >= 0, o, a);
+      }
     }
+  };
+  chunks_up<In8<R>>(0, n, load, body);
 #pragma unroll
-    for (int s = 0; s < 8; ++s) L.at(Y.al + s * (Y.g + Y.S + 1), j) = a[s];
-    if (L.t_of(j) >= 0) {
-      R o[8];
-      alpha_step(a, R(0.5) * sa, R(0.5) * pr, o);
-#pragma unroll
-      for (int s = 0; s < 8; ++s) a[s] = o[s];
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < 8; ++s) L.at(Y.al + s * (Y.g + Y.S + 1), n) = a[s];
+  for (int s = 0; s < 8; ++s) L.at(Y.al + s * st, n) = a[s];
 }
 
 // Forward beta pass that also emits the extrinsics of the own steps
 // (alpha from the alpha pass, beta_{t+1} in registers); beta is not stored.
+// Chunks of 8 steps from the top down, with the alphas of the own steps.
+// Past the block end (t >= K) beta stays at its start value.
 template <typename R>
-__device__ void beta_ext_pass(const Lane<R>& L, const Layout& Y, const R (&ts)[3], const R (&tp)[3], uint32_t h,
-                              R* e) {
+struct BetaIn {
+  In8<R> in;
+  R al[8][8];
+};
+
+template <typename R>
+__device__ __forceinline__ void beta_ext_pass(const Lane<R>& L0, const Layout& Y, const R (&ts)[3], const R (&tp)[3],
+                                              uint32_t h, R* e) {
+  const Lane<R> L = fresh(L0);
   R b[8];
   const bool from_tail = L.q * L.S + L.S + L.g >= L.K;
   if (from_tail) {
@@ -321,45 +447,47 @@ __device__ void beta_ext_pass(const Lane<R>& L, const Layout& Y, const R (&ts)[3
 #pragma unroll
     for (int s = 0; s < 8; ++s) b[s] = 0;
   }
-  const uint32_t top = 2 * L.g + L.S;
-  const uint32_t own_end = L.g + L.S;  // own steps: g <= j < g + S
-  R sa_n = L.at(Y.sa, top - 1), pr_n = L.at(Y.pr, top - 1), a_n[8];
-#pragma unroll
-  for (int s = 0; s < 8; ++s) a_n[s] = (top - 1 < own_end) ? L.at(Y.al + s * (Y.g + Y.S + 1), top - 1) : R(0);
+  const int top = (int)(2 * L.g + L.S);
+  const int own_end = (int)(L.g + L.S);  // own steps: g <= j < g + S
+  const int g = (int)L.g;
+  const uint32_t st = Y.g + Y.S + 1;
   // Pi of the own steps, from the last one down
-  QppWalk qw((uint32_t)L.t_of(own_end - 1), L.K, L.f1, L.f2);
-  for (uint32_t j = top; j-- > L.g;) {
-    const R sa = sa_n, pr = pr_n;
-    R a[8];
+  QppWalk qw((uint32_t)L.t_of((uint32_t)own_end - 1), L.K, L.f1, L.f2);
+  auto load = [&](int hi, BetaIn<R>& c) {
+    load_in8<R, true>(L, Y, hi - 1, min(8, hi - g), c.in);
 #pragma unroll
-    for (int s = 0; s < 8; ++s) a[s] = a_n[s];
-    if (j > L.g) {
-      sa_n = L.at(Y.sa, j - 1);
-      pr_n = L.at(Y.pr, j - 1);
-      if (j - 1 < own_end) {
+    for (int jj = 0; jj < 8; ++jj) {
+      const int j = hi - 1 - jj;
+      if (j >= g && j < own_end) {
 #pragma unroll
-        for (int s = 0; s < 8; ++s) a_n[s] = L.at(Y.al + s * (Y.g + Y.S + 1), j - 1);
+        for (int s = 0; s < 8; ++s) c.al[jj][s] = L.at(Y.al + s * st, (uint32_t)j);
       }
     }
-    const int t = L.t_of(j);
-    if (j < own_end) {
-      R Lu[2];
-      llr_terms(a, b, R(0.5) * sa, R(0.5) * pr, Lu, (R(*)[8]) nullptr);
-      const uint32_t pos = (h & 1u) ? qw.pi : (uint32_t)t;
-      e[pos] = Lu[1] - Lu[0] - sa;
-      if (j > L.g) qw.prev();
-    }
-    if (t < (int)L.K) {
-      R o[8];
-      beta_step(b, R(0.5) * sa, R(0.5) * pr, o);
+  };
+  auto body = [&](int hi, const BetaIn<R>& c) {
 #pragma unroll
-      for (int s = 0; s < 8; ++s) b[s] = o[s];
+    for (int jj = 0; jj < 8; ++jj) {
+      const int j = hi - 1 - jj;
+      if (j >= g) {
+        const R sa = c.in.sa[jj], pr = c.in.pr[jj];
+        const int t = L.t_of((uint32_t)j);
+        if (j < own_end) {
+          R Lu[2];
+          llr_terms(c.al[jj], b, R(0.5) * sa, R(0.5) * pr, Lu, (R(*)[8]) nullptr);
+          e[(h & 1u) ? qw.pi : (uint32_t)t] = Lu[1] - Lu[0] - sa;
+          if (j > g) qw.prev();
+        }
+        R o[8];
+        beta_step(b, R(0.5) * sa, R(0.5) * pr, o);
+        select8(t < (int)L.K, o, b);
+      }
     }
-  }
+  };
+  chunks_down<BetaIn<R>>(g, top, load, body);
 }
 
 template <typename R>
-__device__ inline void load8(const Lane<R>& L, uint32_t field, uint32_t stride, uint32_t j, R (&v)[8]) {
+__device__ __forceinline__ void load8(const Lane<R>& L, uint32_t field, uint32_t stride, uint32_t j, R (&v)[8]) {
 #pragma unroll
   for (int s = 0; s < 8; ++s) v[s] = L.at(field + s * stride, j);
 }
@@ -367,30 +495,28 @@ __device__ inline void load8(const Lane<R>& L, uint32_t field, uint32_t stride, 
 // alpha over local steps 0 .. g + S with only every 8th stored: al[c] =
 // alpha at local 8c (the same recursion as alpha_pass).
 template <typename R>
-__device__ void alpha_ckpt_pass(const Lane<R>& L, const Layout& Y) {
+__device__ __forceinline__ void alpha_ckpt_pass(const Lane<R>& L0, const Layout& Y) {
+  const Lane<R> L = fresh(L0);
   R a[8];
   const bool from_zero_state = (int)(L.q * L.S) - (int)L.g <= 0;
 #pragma unroll
   for (int s = 0; s < 8; ++s) a[s] = from_zero_state ? (s == 0 ? R(0) : R(kNegF)) : R(0);
   const uint32_t n = L.g + L.S;
-  R sa_n = L.at(Y.sa, 0), pr_n = L.at(Y.pr, 0);  // the next step's inputs, one step ahead
-  for (uint32_t j = 0; j < n; ++j) {
-    const R sa = sa_n, pr = pr_n;
-    if (j + 1 < n) {
-      sa_n = L.at(Y.sa, j + 1);
-      pr_n = L.at(Y.pr, j + 1);
-    }
-    if ((j & 7u) == 0) {
+  auto load = [&](uint32_t j0, In8<R>& c) { load_in8(L, Y, (int)j0, (int)min(8u, n - j0), c); };
+  auto body = [&](uint32_t j0, const In8<R>& c) {
 #pragma unroll
-      for (int s = 0; s < 8; ++s) L.at(Y.al + s * (Y.g + Y.S + 1), j / 8) = a[s];
-    }
-    if (L.t_of(j) >= 0) {
-      R o[8];
-      alpha_step(a, R(0.5) * sa, R(0.5) * pr, o);
+    for (int s = 0; s < 8; ++s) L.at(Y.al + s * (Y.g + Y.S + 1), j0 / 8) = a[s];
 #pragma unroll
-      for (int s = 0; s < 8; ++s) a[s] = o[s];
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      const uint32_t j = j0 + jj;
+      if (j < n) {
+        R o[8];
+        alpha_step(a, R(0.5) * c.sa[jj], R(0.5) * c.pr[jj], o);
+        select8(L.t_of(j) >= 0, o, a);
+      }
     }
-  }
+  };
+  chunks_up<In8<R>>(0, n, load, body);
 }
 
 // One step of the alpha adjoint at local j (alpha_j = a, beta_{j+1} = b,
@@ -502,6 +628,7 @@ __device__ Lane<R> make_lane(const DiffJob& J, const DiffArgs<R>& A) {
 
 }  // namespace
 
+
 template <typename R>
 __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs<R> A) {
   const Layout Y(A.max_seg, A.max_warm);
@@ -529,11 +656,318 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs<R> A) {
       const R* e2 = sv + (size_t)(H - 1) * L.K;
       R* post = A.post + C.post_off;
       const R w1 = A.scales[H - 2], w2 = A.scales[H - 1];
-      for (uint32_t t = L.q * L.S; t < L.q * L.S + L.S; ++t) post[t] = d0[t] + w1 * e1[t] + w2 * e2[t];
+      const uint32_t hi = L.q * L.S + L.S;
+      for (uint32_t t0 = L.q * L.S; t0 < hi; t0 += 8) {
+        R x[8], y1[8], y2[8];
+#pragma unroll
+        for (uint32_t jj = 0; jj < 8; ++jj)
+          if (t0 + jj < hi) {
+            x[jj] = d0[t0 + jj];
+            y1[jj] = e1[t0 + jj];
+            y2[jj] = e2[t0 + jj];
+          }
+#pragma unroll
+        for (uint32_t jj = 0; jj < 8; ++jj)
+          if (t0 + jj < hi) post[t0 + jj] = x[jj] + w1 * y1[jj] + w2 * y2[jj];
+      }
     }
     __syncthreads();
   }
 }
+
+namespace {
+
+// Posterior adjoint of one block segment: grad d0 = grad post, the adjoint
+// of the last half's extrinsic, and this lane's terms of the last two scales'
+// adjoints (post = d0 + w[H-2] e_{H-2} + w[H-1] e_{H-1}).
+template <typename R>
+__device__ __forceinline__ void post_adjoint(const DiffArgs<R>& A, const Lane<R>& L, const R* sv, const R* gp, R* g0,
+                                             R* g1, R* g2, R* el, R& s1, R& s2) {
+  const uint32_t H = A.halves, K = L.K, lo = L.q * L.S, hi = lo + L.S;
+  const R* e1 = sv + (size_t)(H - 2) * K;
+  const R* e2 = sv + (size_t)(H - 1) * K;
+  const R wl = A.scales[H - 1];
+  for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
+    R gv[8], x1[8], x2[8];
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj)
+      if (t0 + jj < hi) {
+        gv[jj] = gp[t0 + jj];
+        x1[jj] = e1[t0 + jj];
+        x2[jj] = e2[t0 + jj];
+      }
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj)
+      if (t0 + jj < hi) {
+        const uint32_t t = t0 + jj;
+        g0[t] = gv[jj];
+        g1[t] = 0;
+        g2[t] = 0;
+        el[t] = wl * gv[jj];
+        s1 += gv[jj] * x1[jj];
+        s2 += gv[jj] * x2[jj];
+      }
+  }
+  if (L.q == L.M - 1)
+    for (uint32_t t = K; t < K + 4; ++t) g0[t] = g1[t] = g2[t] = 0;
+}
+
+// Runs body(c, buf) for c = n - 1 down to 0 with load(c, buf) one ahead,
+// alternating two buffers.
+template <class Buf, class Load, class Body>
+__device__ __forceinline__ void subwindows_down(int n, Load&& load, Body&& body) {
+  if (n <= 0) return;
+  Buf b0, b1;
+  load(n - 1, b0);
+  for (int c = n - 1; c >= 0; c -= 2) {
+    if (c >= 1) load(c - 1, b1);
+    body(c, b0);
+    if (c < 1) break;
+    if (c >= 2) load(c - 2, b0);
+    body(c - 1, b1);
+  }
+}
+
+// Alpha adjoint, backwards over local steps g + S - 1 .. 0 in sub-windows of
+// 8 (alpha recomputed from the checkpoint), the output adjoints of the own
+// steps on the way, and beta of the own steps (bcur: beta at local g + S on
+// entry), whose checkpoints beta_ck keeps for the beta-adjoint sweep.
+// Writes dsa / dpr of local steps below g + S (their first writer).
+template <typename R>
+struct AlphaIn {
+  R a0[8], hs[8], hp[8], e[8];
+};
+
+template <typename R, class Ck>
+__device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Layout& Y, R (&bcur)[8], Ck&& beta_ck) {
+  const Lane<R> L = fresh(L0);
+  R ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t n_al = L.g + L.S;
+  auto load = [&](int c, AlphaIn<R>& x) {
+    const uint32_t j0 = 8u * (uint32_t)c, len = min(8u, n_al - j0);
+    load8(L, Y.al, Y.g + Y.S + 1, (uint32_t)c, x.a0);
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      if (jj < len) {
+        const uint32_t j = j0 + jj;
+        x.hs[jj] = R(0.5) * L.at(Y.sa, j);
+        x.hp[jj] = R(0.5) * L.at(Y.pr, j);
+        x.e[jj] = (j >= L.g) ? L.at(Y.eb, j - L.g) : R(0);
+      }
+    }
+  };
+  auto body = [&](int c, const AlphaIn<R>& x) {
+    const uint32_t j0 = 8u * (uint32_t)c, len = min(8u, n_al - j0);
+    R areg[8][8];
+    {
+      R a[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) a[s] = x.a0[s];
+#pragma unroll
+      for (uint32_t jj = 0; jj < 8; ++jj) {
+        if (jj < len) {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
+          if (jj + 1 < len) {
+            R o[8];
+            alpha_step(a, x.hs[jj], x.hp[jj], o);
+            select8(L.t_of(j0 + jj) >= 0, o, a);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int jj = 7; jj >= 0; --jj) {
+      if ((uint32_t)jj < len) {
+        const uint32_t j = j0 + (uint32_t)jj;
+        R b[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) b[s] = bcur[s];  // beta_{j+1} while j >= g
+        const R hs = x.hs[jj], hp = x.hp[jj];
+        if (j >= L.g) {
+          // own step (inside the block for every lane): checkpoint
+          // beta_{j+1}, step beta to beta_j
+          beta_ck(j + 1 - L.g, b);
+          R o[8];
+          beta_step(b, hs, hp, o);
+#pragma unroll
+          for (int s = 0; s < 8; ++s) bcur[s] = o[s];
+        }
+        R ds, dp;
+        alpha_adjoint_step(areg[jj], b, hs, hp, x.e[jj], j >= L.g, ab, ds, dp, L, Y, j);
+        L.at(Y.dsa, j) = ds;
+        L.at(Y.dpr, j) = dp;
+      }
+    }
+  };
+  subwindows_down<AlphaIn<R>>((int)((n_al + 7) / 8), load, body);
+}
+
+// Beta adjoint, forwards over local steps g .. g + S + g - 1 in sub-windows
+// of 8 (beta recomputed from the checkpoint at the top); adds to dsa / dpr.
+// Returns in bb the adjoint of beta at the window top (at the block end,
+// that of the tail beta).
+template <typename R>
+struct BetaRe {
+  R b0[8], hs[8], hp[8];
+};
+
+template <typename R>
+__device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layout& Y, R (&bb)[8]) {
+  const Lane<R> L = fresh(L0);
+  const uint32_t jend = 2 * L.g + L.S;
+  const uint32_t bst = Y.S + Y.g + 1;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) bb[s] = 0;
+  auto load = [&](uint32_t jb0, BetaRe<R>& x) {
+    const uint32_t len = min(8u, jend - jb0);
+    const uint32_t itop = jb0 + len - L.g;  // beta at local jb0 + len = g + itop
+    load8(L, Y.be, bst, (itop + 7) / 8, x.b0);
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      if (jj < len) {
+        x.hs[jj] = R(0.5) * L.at(Y.sa, jb0 + jj);
+        x.hp[jj] = R(0.5) * L.at(Y.pr, jb0 + jj);
+      }
+    }
+  };
+  auto body = [&](uint32_t jb0, const BetaRe<R>& x) {
+    const uint32_t len = min(8u, jend - jb0);
+    // the sub-window's output adjoints and running sys/par adjoints, loaded
+    // ahead of the recompute
+    R bor[8][8], dsr[8], dpr[8];
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      const uint32_t j = jb0 + jj;
+      if (jj < len) {
+        if (j < L.g + L.S) {
+          load8(L, Y.bo, Y.S, j - L.g, bor[jj]);
+        } else {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) bor[jj][s] = 0;
+        }
+        dsr[jj] = L.at(Y.dsa, j);
+        dpr[jj] = L.at(Y.dpr, j);
+      }
+    }
+    R breg[8][8];
+    {
+      R b[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) b[s] = x.b0[s];
+#pragma unroll
+      for (int jj = 7; jj >= 0; --jj) {
+        if ((uint32_t)jj < len) {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) breg[jj][s] = b[s];  // beta at local j + 1
+          if (jj > 0) {
+            R o[8];
+            beta_step(b, x.hs[jj], x.hp[jj], o);
+            select8(L.t_of(jb0 + (uint32_t)jj) < (int)L.K, o, b);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      if (jj < len) {
+        const uint32_t j = jb0 + jj;
+        R nb[8], bn[8], ds, dp;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          nb[s] = bb[s];
+          bn[s] = bor[jj][s];
+        }
+        beta_adjoint_step(breg[jj], x.hs[jj], x.hp[jj], nb, bn, ds, dp);
+        // past the block end the adjoint stops (beta is not stepped there)
+        select8(L.t_of(j) < (int)L.K, nb, bb);
+        L.at(Y.dsa, j) = dsr[jj] + ds;
+        L.at(Y.dpr, j) = dpr[jj] + dp;
+      }
+    }
+  };
+  chunks_up<BetaRe<R>>(L.g, jend, load, body);
+}
+
+// Combine of one half: own window, next segment's alpha warm-up, previous
+// segment's beta warm-up (in that order), added into the LLR adjoints and the
+// previous half's extrinsic adjoint; returns this lane's term of the scale
+// adjoint.  Read-modify-writes in chunks of 8 steps (t, or Pi(t), is a
+// permutation, so no two steps share a position).
+template <typename R>
+struct CombIn {
+  R ds[8], dp[8], dsn[8], dpn[8], dsp[8], dpp[8], gs[8], gq[8], pv[8], ep[8];
+  uint32_t at[8];
+};
+
+template <typename R>
+__device__ __forceinline__ R combine(const Lane<R>& L0, const Layout& Y, uint32_t h, R w, const R* eprev, R* g0,
+                                     R* gpar, R* prev) {
+  const Lane<R> L = fresh(L0);
+  const bool lower = (h & 1u) != 0;
+  const bool with_prev = h != 0;  // the first upper half has no a-priori input
+  const uint32_t own_end = L.g + L.S;
+  const bool has_next = L.q + 1 < L.M, has_prev = L.q > 0;
+  const uint32_t ln = min(L.lane + 1, 63u), lp = L.lane > 0 ? L.lane - 1 : 0u;
+  QppWalk qw(L.q * L.S, L.K, L.f1, L.f2);
+  R wsum = 0;
+  auto load = [&](uint32_t j0, CombIn<R>& x) {
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      const uint32_t j = j0 + jj;
+      if (j < own_end) {
+        const uint32_t t = (uint32_t)L.t_of(j);
+        const uint32_t i = lower ? qw.pi : t;
+        qw.next();
+        x.at[jj] = i;
+        x.ds[jj] = L.at(Y.dsa, j);
+        x.dp[jj] = L.at(Y.dpr, j);
+        if (j >= L.S) {  // t in [qS + S - g, qS + S): lane q+1's local j - S
+          x.dsn[jj] = L.nb(Y.dsa, j - L.S, ln);
+          x.dpn[jj] = L.nb(Y.dpr, j - L.S, ln);
+        }
+        if (j < 2 * L.g) {  // t in [qS, qS + g): lane q-1's local j + S
+          x.dsp[jj] = L.nb(Y.dsa, j + L.S, lp);
+          x.dpp[jj] = L.nb(Y.dpr, j + L.S, lp);
+        }
+        x.gs[jj] = g0[i];
+        x.gq[jj] = gpar[t];
+        if (with_prev) {
+          x.pv[jj] = prev[i];
+          x.ep[jj] = eprev[i];
+        }
+      }
+    }
+  };
+  auto body = [&](uint32_t j0, const CombIn<R>& x) {
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      const uint32_t j = j0 + jj;
+      if (j < own_end) {
+        const uint32_t t = (uint32_t)L.t_of(j), i = x.at[jj];
+        R ds = x.ds[jj], dp = x.dp[jj];
+        if (j >= L.S) {
+          ds = has_next ? ds + x.dsn[jj] : ds;
+          dp = has_next ? dp + x.dpn[jj] : dp;
+        }
+        if (j < 2 * L.g) {
+          ds = has_prev ? ds + x.dsp[jj] : ds;
+          dp = has_prev ? dp + x.dpp[jj] : dp;
+        }
+        g0[i] = x.gs[jj] + ds;
+        gpar[t] = x.gq[jj] + dp;
+        if (with_prev) {
+          prev[i] = x.pv[jj] + w * ds;
+          wsum += ds * x.ep[jj];
+        }
+      }
+    }
+  };
+  chunks_up<CombIn<R>>(L.g, own_end, load, body);
+  return wsum;
+}
+
+}  // namespace
 
 template <typename R>
 __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
@@ -556,24 +990,9 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
     R* eb[2] = {A.ebar + C.ebar_off, A.ebar + C.ebar_off + K};
     R* dsc = A.dscale_part + (size_t)jb * H;
     const uint32_t lo = L.q * L.S, hi = lo + L.S;
-    // posterior: post = d0 + w[H-2] e_{H-2} + w[H-1] e_{H-1}
     {
       R s1 = 0, s2 = 0;
-      if (L.active) {
-        const R* e1 = sv + (size_t)(H - 2) * K;
-        const R* e2 = sv + (size_t)(H - 1) * K;
-        for (uint32_t t = lo; t < hi; ++t) {
-          const R gpt = gp[t];
-          g0[t] = gpt;
-          g1[t] = 0;
-          g2[t] = 0;
-          eb[(H - 1) & 1u][t] = A.scales[H - 1] * gpt;
-          s1 += gpt * e1[t];
-          s2 += gpt * e2[t];
-        }
-        if (L.q == L.M - 1)
-          for (uint32_t t = K; t < D; ++t) g0[t] = g1[t] = g2[t] = 0;
-      }
+      if (L.active) post_adjoint(A, L, sv, gp, g0, g1, g2, eb[(H - 1) & 1u], s1, s2);
       for (int o = 32; o > 0; o >>= 1) {
         s1 += __shfl_xor(s1, o, 64);
         s2 += __shfl_xor(s2, o, 64);
@@ -588,20 +1007,43 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
     for (uint32_t h = H; h-- > 0;) {
       R* cur = eb[h & 1u];
       R* prev = eb[(h + 1) & 1u];
-      R ts[3] = {0, 0, 0}, tp[3] = {0, 0, 0};
-      bool to_tail = false;
+      const bool lower = (h & 1u) != 0;
       if (L.active) {
         // adjoint of e_{h-1}: starts at the posterior term for h - 1 == H - 2
-        if (h >= 1)
-          for (uint32_t t = lo; t < hi; ++t) prev[t] = (h - 1 == H - 2) ? A.scales[H - 2] * gp[t] : R(0);
+        if (h >= 1) {
+          if (h - 1 == H - 2) {
+            const R wp = A.scales[H - 2];
+            for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
+              R gv[8];
+#pragma unroll
+              for (uint32_t jj = 0; jj < 8; ++jj)
+                if (t0 + jj < hi) gv[jj] = gp[t0 + jj];
+#pragma unroll
+              for (uint32_t jj = 0; jj < 8; ++jj)
+                if (t0 + jj < hi) prev[t0 + jj] = wp * gv[jj];
+            }
+          } else {
+            for (uint32_t t = lo; t < hi; ++t) prev[t] = 0;
+          }
+        }
+        R ts[3], tp[3];
         tail_inputs(h, d0, d1, d2, K, ts, tp);
         stage(L, Y, h, d0, d1, d2, sv, A.scales);
         {
+          // the adjoints of the own steps' extrinsics, into the eb field
           QppWalk qw(lo, K, L.f1, L.f2);
-#pragma unroll 8
-          for (uint32_t j = L.g; j < L.g + L.S; ++j, qw.next()) {
-            const uint32_t t = (uint32_t)L.t_of(j);
-            L.at(Y.eb, j - L.g) = cur[(h & 1u) ? qw.pi : t];
+          for (uint32_t j0 = L.g; j0 < L.g + L.S; j0 += 8) {
+            R v[8];
+#pragma unroll
+            for (uint32_t jj = 0; jj < 8; ++jj)
+              if (j0 + jj < L.g + L.S) {
+                const uint32_t t = (uint32_t)L.t_of(j0 + jj);
+                v[jj] = cur[lower ? qw.pi : t];
+                qw.next();
+              }
+#pragma unroll
+            for (uint32_t jj = 0; jj < 8; ++jj)
+              if (j0 + jj < L.g + L.S) L.at(Y.eb, j0 + jj - L.g) = v[jj];
           }
         }
         // the alpha-adjoint sweep is the first writer of local steps below
@@ -614,7 +1056,7 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
         alpha_ckpt_pass(L, Y);
         // beta over the window's warm-up above the own steps (local 2g + S - 1
         // down to g + S); the own steps' beta runs inside the alpha-adjoint
-        // loop below, an independent chain beside the adjoint recursion.
+        // sweep, an independent chain beside the adjoint recursion.
         // Checkpoints for the beta-adjoint sweep: beta at local g + i for i a
         // multiple of 8 or i = S + g, at be[(i + 7) / 8]
         R bcur[8];
@@ -631,134 +1073,29 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
             for (int s = 0; s < 8; ++s) L.at(Y.be + s * bst, (i + 7) / 8) = b[s];
           }
         };
-        R wsa = L.at(Y.sa, 2 * L.g + L.S - 1), wpr = L.at(Y.pr, 2 * L.g + L.S - 1);
-        for (uint32_t j = 2 * L.g + L.S; j-- > L.g + L.S;) {
-          const R sa = wsa, pr = wpr;
-          if (j > L.g + L.S) {
-            wsa = L.at(Y.sa, j - 1);
-            wpr = L.at(Y.pr, j - 1);
-          }
-          beta_ck(j + 1 - L.g, bcur);
-          if (L.t_of(j) < (int)K) {
-            R o[8];
-            beta_step(bcur, R(0.5) * sa, R(0.5) * pr, o);
+        {
+          const int top = (int)(2 * L.g + L.S), bot = (int)(L.g + L.S);
+          auto load = [&](int hw, In8<R>& c) { load_in8<R, true>(L, Y, hw - 1, min(8, hw - bot), c); };
+          auto body = [&](int hw, const In8<R>& c) {
 #pragma unroll
-            for (int s = 0; s < 8; ++s) bcur[s] = o[s];
-          }
-        }
-        // alpha adjoint, backwards over local steps g + S - 1 .. 0 in
-        // sub-windows of 8 (alpha recomputed from the checkpoint); output
-        // adjoints of the own steps on the way
-        R ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        const uint32_t n_al = L.g + L.S;
-        for (int c = (int)((n_al + 7) / 8) - 1; c >= 0; --c) {
-          const uint32_t j0 = 8u * (uint32_t)c;
-          const uint32_t len = min(8u, n_al - j0);
-          if (L.t_of(j0 + len - 1) < 0) break;  // segment 0: nothing before t = 0
-          R areg[8][8], hsr[8], hpr[8], er[8];
-          {
-            R a[8];
-            load8(L, Y.al, Y.g + Y.S + 1, (uint32_t)c, a);
-#pragma unroll
-            for (uint32_t jj = 0; jj < 8; ++jj) {
-              if (jj < len) {
-                const uint32_t j = j0 + jj;
-                hsr[jj] = R(0.5) * L.at(Y.sa, j);
-                hpr[jj] = R(0.5) * L.at(Y.pr, j);
-                er[jj] = (j >= L.g) ? L.at(Y.eb, j - L.g) : R(0);
-#pragma unroll
-                for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
-                if (jj + 1 < len && L.t_of(j) >= 0) {
-                  R o[8];
-                  alpha_step(a, hsr[jj], hpr[jj], o);
-#pragma unroll
-                  for (int s = 0; s < 8; ++s) a[s] = o[s];
-                }
+            for (int jj = 0; jj < 8; ++jj) {
+              const int j = hw - 1 - jj;
+              if (j >= bot) {
+                beta_ck((uint32_t)j + 1 - L.g, bcur);
+                R o[8];
+                beta_step(bcur, R(0.5) * c.sa[jj], R(0.5) * c.pr[jj], o);
+                select8(L.t_of((uint32_t)j) < (int)K, o, bcur);
               }
             }
-          }
-#pragma unroll
-          for (int jj = 7; jj >= 0; --jj) {
-            const uint32_t j = j0 + (uint32_t)jj;
-            if ((uint32_t)jj >= len || L.t_of(j) < 0) continue;
-            R b[8];
-#pragma unroll
-            for (int s = 0; s < 8; ++s) b[s] = bcur[s];  // beta_{j+1} while j >= g
-            const R hs = hsr[jj], hp = hpr[jj], e = er[jj];
-            if (j >= L.g) {
-              // own step: checkpoint beta_{j+1}, step beta to beta_j
-              beta_ck(j + 1 - L.g, b);
-              R o[8];
-              beta_step(b, hs, hp, o);
-#pragma unroll
-              for (int s = 0; s < 8; ++s) bcur[s] = o[s];
-            }
-            R ds, dp;
-            alpha_adjoint_step(areg[jj], b, hs, hp, e, j >= L.g, ab, ds, dp, L, Y, j);
-            L.at(Y.dsa, j) = ds;  // first writer of these positions (all zero before)
-            L.at(Y.dpr, j) = dp;
-          }
+          };
+          chunks_down<In8<R>>(bot, top, load, body);
         }
-        // beta adjoint, forwards over local steps g .. g + S + g - 1 in
-        // sub-windows of 8 (beta recomputed from the checkpoint at the top)
-        R bb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        const uint32_t jend = 2 * L.g + L.S;
-        for (uint32_t jb0 = L.g; jb0 < jend; jb0 += 8) {
-          if (L.t_of(jb0) >= (int)K) break;
-          const uint32_t len = min(8u, jend - jb0);
-          const uint32_t itop = jb0 + len - L.g;  // beta at local jb0 + len = g + itop
-          R breg[8][8], hsr[8], hpr[8];
-          {
-            R b[8];
-            load8(L, Y.be, bst, (itop + 7) / 8, b);
-#pragma unroll
-            for (int jj = 7; jj >= 0; --jj) {
-              if ((uint32_t)jj < len) {
-                const uint32_t j = jb0 + (uint32_t)jj;
-                hsr[jj] = R(0.5) * L.at(Y.sa, j);
-                hpr[jj] = R(0.5) * L.at(Y.pr, j);
-#pragma unroll
-                for (int s = 0; s < 8; ++s) breg[jj][s] = b[s];  // beta at local j + 1
-                if (jj > 0 && L.t_of(j) < (int)K) {
-                  R o[8];
-                  beta_step(b, hsr[jj], hpr[jj], o);
-#pragma unroll
-                  for (int s = 0; s < 8; ++s) b[s] = o[s];
-                }
-              }
-            }
-          }
-          // the sub-window's output adjoints and running sys/par adjoints, all
-          // loaded before its first step
-          R bor[8][8], dsr[8], dpr[8];
-#pragma unroll
-          for (uint32_t jj = 0; jj < 8; ++jj) {
-            const uint32_t j = jb0 + jj;
-            if (jj < len) {
-              if (j < L.g + L.S) {
-                load8(L, Y.bo, Y.S, j - L.g, bor[jj]);
-              } else {
-#pragma unroll
-                for (int s = 0; s < 8; ++s) bor[jj][s] = 0;
-              }
-              dsr[jj] = L.at(Y.dsa, j);
-              dpr[jj] = L.at(Y.dpr, j);
-            }
-          }
-#pragma unroll
-          for (uint32_t jj = 0; jj < 8; ++jj) {
-            const uint32_t j = jb0 + jj;
-            if (jj >= len || L.t_of(j) >= (int)K) break;
-            R ds, dp;
-            beta_adjoint_step(breg[jj], hsr[jj], hpr[jj], bb, bor[jj], ds, dp);
-            L.at(Y.dsa, j) = dsr[jj] + ds;
-            L.at(Y.dpr, j) = dpr[jj] + dp;
-          }
-        }
+        alpha_adjoint_sweep(L, Y, bcur, beta_ck);
+        R bb[8];
+        beta_adjoint_sweep(L, Y, bb);
         // beta of the window top: the tail beta when it reaches K
-        to_tail = lo + L.S + L.g >= K;
         R dts[3] = {0, 0, 0}, dtp[3] = {0, 0, 0};
-        if (to_tail) {
+        if (lo + L.S + L.g >= K) {
 #pragma unroll
           for (unsigned s = 0; s < 8; ++s) {
             unsigned c = s;
@@ -777,39 +1114,11 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
         }
       }
       __syncthreads();
-      // combine: own window, next segment's alpha warm-up, previous segment's beta warm-up
       R wsum = 0;
       if (L.active) {
         const R w = h ? A.scales[h - 1] : R(0);
         const R* eprev = sv + (size_t)(h ? h - 1 : 0) * K;
-        QppWalk qw(lo, K, L.f1, L.f2);
-#pragma unroll 4
-        for (uint32_t j = L.g; j < L.g + L.S; ++j, qw.next()) {
-          const uint32_t t = (uint32_t)L.t_of(j);
-          R ds = L.at(Y.dsa, j), dp = L.at(Y.dpr, j);
-          if (L.q + 1 < L.M && j >= L.S) {  // t in [qS + S - g, qS + S): lane q+1's local j - S
-            ds += L.nb(Y.dsa, j - L.S, L.lane + 1);
-            dp += L.nb(Y.dpr, j - L.S, L.lane + 1);
-          }
-          if (L.q > 0 && j < 2 * L.g) {  // t in [qS, qS + g): lane q-1's local j + S
-            ds += L.nb(Y.dsa, j + L.S, L.lane - 1);
-            dp += L.nb(Y.dpr, j + L.S, L.lane - 1);
-          }
-          if ((h & 1u) == 0) {
-            g0[t] += ds;
-            g1[t] += dp;
-            if (h) {
-              prev[t] += w * ds;
-              wsum += ds * eprev[t];
-            }
-          } else {
-            const uint32_t pi = qw.pi;
-            g0[pi] += ds;
-            g2[t] += dp;
-            prev[pi] += w * ds;
-            wsum += ds * eprev[pi];
-          }
-        }
+        wsum = combine(L, Y, h, w, eprev, g0, lower ? g2 : g1, prev);
         if (L.q == L.M - 1) {
           // tail LLR adjoints, in segment order
           R dts[3] = {0, 0, 0}, dtp[3] = {0, 0, 0};
@@ -821,7 +1130,7 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
               dtp[k3] += L.nb(Y.tl, 3 + k3, other);
             }
           }
-          if ((h & 1u) == 0) {
+          if (!lower) {
             g0[K] += dts[0];
             g0[K + 2] += dts[1];
             g1[K] += dtp[0];
