@@ -27,6 +27,7 @@ struct tc_diff_plan {
   bool fp64 = false;
   size_t n_cb = 0, esize = 4;
   uint32_t halves = 0, njobs = 0, grid = 0, max_seg = 0, max_warm = 0, slot_elems = 0;
+  size_t lds_bytes = 0;  // dynamic LDS per workgroup (0: no LDS path)
   uint64_t llr_elems = 0, post_elems = 0, saved_elems = 0, ebar_elems = 0;
   tcdev::DiffJob* d_jobs = nullptr;
   tcdev::DiffCb* d_cbs = nullptr;
@@ -87,6 +88,7 @@ void build(tc_diff_plan& p, const uint32_t* k, size_t n, uint32_t iterations, ui
   // entries of a reordered copy of the descriptors
   std::vector<tcdev::DiffJob> jobs;
   std::vector<tcdev::DiffCb> ordered;
+  size_t lds_elems = 0;
   for (uint32_t kk : order) {
     const auto& idx = by_k[kk];
     const uint32_t lo_seg = std::max<uint32_t>(min_segment, (kk + 63) / 64);
@@ -105,6 +107,8 @@ void build(tc_diff_plan& p, const uint32_t* k, size_t n, uint32_t iterations, ui
     }
     p.max_seg = std::max(p.max_seg, s);
     p.max_warm = std::max(p.max_warm, g);
+    lds_elems = std::max<size_t>(lds_elems, (size_t)tcdev::kDiffLdsArrays * std::min<size_t>(per, idx.size()) *
+                                                tcdev::diff_lds_stride(kk));
   }
   p.njobs = static_cast<uint32_t>(jobs.size());
   p.slot_elems = slot_elems(p.max_seg, p.max_warm);
@@ -114,6 +118,13 @@ void build(tc_diff_plan& p, const uint32_t* k, size_t n, uint32_t iterations, ui
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
   p.grid = std::max<uint32_t>(1, std::min<uint32_t>(p.njobs, 4u * static_cast<uint32_t>(cus)));
+  // LDS copies of the job's blocks when they leave room for four jobs per CU
+  // (the resident count above); larger jobs gather from global memory
+  int lds_per_cu = 0;
+  hip_check(hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev),
+            "hipDeviceGetAttribute");
+  const size_t need = lds_elems * p.esize;
+  p.lds_bytes = (need > 0 && need <= static_cast<size_t>(lds_per_cu) / 4) ? need : 0;
   p.d_jobs = dev_alloc<tcdev::DiffJob>(jobs.size());
   dev_upload(p.d_jobs, jobs);
   p.d_cbs = dev_alloc<tcdev::DiffCb>(ordered.size());
@@ -143,12 +154,13 @@ tcdev::DiffArgs<R> args(const tc_diff_plan& p, const void* llr, const void* scal
   a.slot_elems = p.slot_elems;
   a.max_seg = p.max_seg;
   a.max_warm = p.max_warm;
+  a.use_lds = p.lds_bytes > 0;
   return a;
 }
 
 template <typename R>
 void forward(const tc_diff_plan& p, const void* llr, const void* scales, void* post, void* saved, hipStream_t s) {
-  hipLaunchKernelGGL(tcdev::diff_forward_kernel<R>, dim3(p.grid), dim3(64), 0, s,
+  hipLaunchKernelGGL(tcdev::diff_forward_kernel<R>, dim3(p.grid), dim3(64), p.lds_bytes, s,
                      args<R>(p, llr, scales, post, saved, nullptr, nullptr));
   hip_check(hipGetLastError(), "differentiable decoder forward launch");
 }
@@ -156,7 +168,7 @@ void forward(const tc_diff_plan& p, const void* llr, const void* scales, void* p
 template <typename R>
 void backward(const tc_diff_plan& p, const void* llr, const void* scales, const void* saved, const void* grad_post,
               void* grad_llr, void* grad_scales, hipStream_t s) {
-  hipLaunchKernelGGL(tcdev::diff_backward_kernel<R>, dim3(p.grid), dim3(64), 0, s,
+  hipLaunchKernelGGL(tcdev::diff_backward_kernel<R>, dim3(p.grid), dim3(64), p.lds_bytes, s,
                      args<R>(p, llr, scales, nullptr, const_cast<void*>(saved), grad_post, grad_llr));
   hip_check(hipGetLastError(), "differentiable decoder backward launch");
   hipLaunchKernelGGL(tcdev::diff_scale_reduce_kernel<R>, dim3(p.halves), dim3(256), 0, s,

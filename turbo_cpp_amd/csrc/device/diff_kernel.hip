@@ -242,17 +242,21 @@ struct Lane {
   __device__ int t_of(uint32_t j) const { return (int)(q * S) - (int)g + (int)j; }
 };
 
-// The lane with its scratch base and lane index made opaque to the
-// optimiser.  Each pass starts from such a copy, so the per-step scratch
+// The lane with its lane index made opaque to the optimiser.  Each pass starts from such a copy, so the per-step scratch
 // offsets of a pass are not hoisted out of the half-iteration loop (where
 // they would hold registers through every other pass).
 template <typename R>
 __device__ __forceinline__ Lane<R> fresh(const Lane<R>& L0) {
   Lane<R> L = L0;
-  asm volatile("" : "+s"(L.sc), "+v"(L.lane));
+  asm volatile("" : "+v"(L.lane));
   return L;
 }
 
+// Inlining: the recursion passes are inlined into the kernels; the LDS fills
+// and write-backs and the global-memory fallbacks of staging and combine are
+// separate functions, so their batches of loads do not raise the register
+// count of the kernel around the passes.
+//
 // Memory order and control flow of every sequential pass below.  A job is
 // one wave and a batch rarely has more jobs than SIMDs, so no other wave
 // hides a wait.  Each pass therefore walks its steps in chunks of 8 with two
@@ -333,7 +337,7 @@ struct StageIn {
 };
 
 template <typename R>
-__device__ __forceinline__ void stage(const Lane<R>& L0, const Layout& Y, uint32_t h, const R* d0, const R* d1,
+__device__ __attribute__((noinline)) void stage(const Lane<R>& L0, const Layout& Y, uint32_t h, const R* d0, const R* d1,
                                       const R* d2, const R* sv, const R* scales) {
   const Lane<R> L = fresh(L0);
   const R w = h ? scales[h - 1] : R(0);
@@ -375,6 +379,89 @@ __device__ __forceinline__ void stage(const Lane<R>& L0, const Layout& Y, uint32
   chunks_up<StageIn<R>>(0, Y.P, load, body);
 }
 
+// LDS copies of per-block arrays of the job (all lanes take part in filling
+// and writing back, in natural order, so every global access of a fill is a
+// contiguous 256-byte row per wave).  Used when the plan's jobs fit (use_lds);
+// then staging and the adjoint combine read and update these copies instead
+// of gathering with a stride of S elements between lanes.
+template <typename R>
+struct BlockLds {
+  R* base;
+  uint32_t kp, nblk;  // diff_lds_stride(K), blocks of the job
+  __device__ R& at(uint32_t arr, uint32_t b, uint32_t i) const {
+    return base[(arr * nblk + b) * kp + i + (i >> 5)];
+  }
+};
+
+template <typename R>
+__device__ __forceinline__ BlockLds<R> block_lds(R* base, const DiffJob& J) {
+  return BlockLds<R>{base, diff_lds_stride(J.k), J.nblk};
+}
+
+constexpr uint32_t kFillBatch = 16;  // loads in flight per lane and array
+
+// Fill of the staging arrays: 0 = sys + la at natural position i (the lower
+// half reads it at Pi(t)), 1 = the half's parity stream.  Same arithmetic as
+// the per-lane staging.
+template <typename R>
+__device__ __attribute__((noinline)) void stage_fill(const DiffArgs<R>& A, const DiffJob& J, uint32_t h, const BlockLds<R>& X) {
+  const R w = h ? A.scales[h - 1] : R(0);
+  const bool lower = (h & 1u) != 0;
+  const uint32_t K = J.k;
+  for (uint32_t b = 0; b < J.nblk; ++b) {
+    const DiffCb C = A.cbs[J.cb_first + b];
+    const R* d0 = A.llr + C.llr_off;
+    const R* dp = d0 + (size_t)(lower ? 2 : 1) * (K + 4);
+    const R* ep = A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K;
+    for (uint32_t i0 = threadIdx.x; i0 < K; i0 += 64 * kFillBatch) {
+      R sv[kFillBatch], ev[kFillBatch], pv[kFillBatch];
+#pragma unroll
+      for (uint32_t u = 0; u < kFillBatch; ++u) {
+        const uint32_t i = i0 + 64 * u;
+        if (i < K) {
+          sv[u] = d0[i];
+          ev[u] = h ? ep[i] : R(0);
+          pv[u] = dp[i];
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kFillBatch; ++u) {
+        const uint32_t i = i0 + 64 * u;
+        if (i < K) {
+          X.at(0, b, i) = lower ? sv[u] + w * ev[u] : sv[u] + (h ? w * ev[u] : R(0));
+          X.at(1, b, i) = pv[u];
+        }
+      }
+    }
+  }
+}
+
+// Per-lane staging from the LDS copies (see stage for the values).
+template <typename R>
+__device__ __forceinline__ void stage_from_lds(const Lane<R>& L0, const Layout& Y, uint32_t h, const BlockLds<R>& X,
+                                               uint32_t blk) {
+  const Lane<R> L = fresh(L0);
+  const bool lower = (h & 1u) != 0;
+  const int t0 = L.t_of(0);
+  QppWalk qw(t0 > 0 ? (uint32_t)t0 : 0u, L.K, L.f1, L.f2);
+  for (uint32_t j0 = 0; j0 < Y.P; j0 += 8) {
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      const uint32_t j = j0 + jj;
+      if (j < Y.P) {
+        const int t = L.t_of(j);
+        const bool v = t >= 0 && t < (int)L.K;
+        const uint32_t tc = (uint32_t)min(max(t, 0), (int)L.K - 1);
+        const R sa = X.at(0, blk, lower ? qw.pi : tc);
+        const R pr = X.at(1, blk, tc);
+        L.at(Y.sa, j) = v ? sa : R(0);
+        L.at(Y.pr, j) = v ? pr : R(0);
+        qw.next_if(v);
+      }
+    }
+  }
+}
+
 template <typename R>
 __device__ void tail_inputs(uint32_t h, const R* d0, const R* d1, const R* d2, uint32_t K, R (&ts)[3], R (&tp)[3]) {
   // the decoder's tail read-out (lte_turbo.cpp:758-773)
@@ -395,44 +482,32 @@ __device__ void tail_inputs(uint32_t h, const R* d0, const R* d1, const R* d2, u
   }
 }
 
-// alpha over local steps 0 .. g + S, stored at al[j] (alpha_t for t = t_of(j));
-// before step 0 of the block (t < 0) alpha stays at its start value.
-template <typename R>
-__device__ __forceinline__ void alpha_pass(const Lane<R>& L0, const Layout& Y) {
-  const Lane<R> L = fresh(L0);
-  R a[8];
-  const bool from_zero_state = (int)(L.q * L.S) - (int)L.g <= 0;
-#pragma unroll
-  for (int s = 0; s < 8; ++s) a[s] = from_zero_state ? (s == 0 ? R(0) : R(kNegF)) : R(0);
-  const uint32_t n = L.g + L.S;
-  const uint32_t st = Y.g + Y.S + 1;
-  auto load = [&](uint32_t j0, In8<R>& c) { load_in8(L, Y, (int)j0, (int)min(8u, n - j0), c); };
-  auto body = [&](uint32_t j0, const In8<R>& c) {
-#pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
-      const uint32_t j = j0 + jj;
-      if (j < n) {
-#pragma unroll
-        for (int s = 0; s < 8; ++s) L.at(Y.al + s * st, j) = a[s];
-        R o[8];
-        alpha_step(a, R(0.5) * c.sa[jj], R(0.5) * c.pr[jj], o);
-        select8(L.t_of(j) >= 0, o, a);
-      }
-    }
-  };
-  chunks_up<In8<R>>(0, n, load, body);
-#pragma unroll
-  for (int s = 0; s < 8; ++s) L.at(Y.al + s * st, n) = a[s];
+// Runs body(c, buf) for c = hi - 1 down to lo with load(c, buf) one ahead,
+// alternating two buffers.
+template <class Buf, class Load, class Body>
+__device__ __forceinline__ void subwindows_down(int lo, int hi, Load&& load, Body&& body) {
+  if (hi <= lo) return;
+  Buf b0, b1;
+  load(hi - 1, b0);
+  for (int c = hi - 1; c >= lo; c -= 2) {
+    if (c - 1 >= lo) load(c - 1, b1);
+    body(c, b0);
+    if (c - 1 < lo) break;
+    if (c - 2 >= lo) load(c - 2, b0);
+    body(c - 1, b1);
+  }
 }
 
-// Forward beta pass that also emits the extrinsics of the own steps
-// (alpha from the alpha pass, beta_{t+1} in registers); beta is not stored.
-// Chunks of 8 steps from the top down, with the alphas of the own steps.
-// Past the block end (t >= K) beta stays at its start value.
+// Forward beta pass that also emits the extrinsics of the own steps: beta
+// from the top of the window down, in sub-windows [8c, 8c + 8) of local
+// steps; for a sub-window with own steps, alpha is recomputed from its
+// checkpoint (al[c], alpha at local 8c, from alpha_ckpt_pass) into
+// registers first.  Past the block end (t >= K) beta stays at its start
+// value; beta is not stored.
 template <typename R>
-struct BetaIn {
-  In8<R> in;
-  R al[8][8];
+struct BetaExtIn {
+  In8<R> in;  // steps 8c + jj
+  R a0[8];    // alpha at local 8c
 };
 
 template <typename R>
@@ -453,27 +528,38 @@ __device__ __forceinline__ void beta_ext_pass(const Lane<R>& L0, const Layout& Y
   const uint32_t st = Y.g + Y.S + 1;
   // Pi of the own steps, from the last one down
   QppWalk qw((uint32_t)L.t_of((uint32_t)own_end - 1), L.K, L.f1, L.f2);
-  auto load = [&](int hi, BetaIn<R>& c) {
-    load_in8<R, true>(L, Y, hi - 1, min(8, hi - g), c.in);
+  auto load = [&](int c, BetaExtIn<R>& x) {
+    const int j0 = 8 * c;
+    load_in8(L, Y, j0, min(8, top - j0), x.in);
+    if (j0 < own_end) load8(L, Y.al, st, (uint32_t)c, x.a0);
+  };
+  auto body = [&](int c, const BetaExtIn<R>& x) {
+    const int j0 = 8 * c;
+    R areg[8][8];
+    if (j0 < own_end) {
+      R a[8];
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
-      const int j = hi - 1 - jj;
-      if (j >= g && j < own_end) {
+      for (int s = 0; s < 8; ++s) a[s] = x.a0[s];
 #pragma unroll
-        for (int s = 0; s < 8; ++s) c.al[jj][s] = L.at(Y.al + s * st, (uint32_t)j);
+      for (int jj = 0; jj < 8; ++jj) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
+        if (jj + 1 < 8 && j0 + jj + 1 < own_end) {
+          R o[8];
+          alpha_step(a, R(0.5) * x.in.sa[jj], R(0.5) * x.in.pr[jj], o);
+          select8(L.t_of((uint32_t)(j0 + jj)) >= 0, o, a);
+        }
       }
     }
-  };
-  auto body = [&](int hi, const BetaIn<R>& c) {
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
-      const int j = hi - 1 - jj;
-      if (j >= g) {
-        const R sa = c.in.sa[jj], pr = c.in.pr[jj];
+    for (int jj = 7; jj >= 0; --jj) {
+      const int j = j0 + jj;
+      if (j >= g && j < top) {
+        const R sa = x.in.sa[jj], pr = x.in.pr[jj];
         const int t = L.t_of((uint32_t)j);
         if (j < own_end) {
           R Lu[2];
-          llr_terms(c.al[jj], b, R(0.5) * sa, R(0.5) * pr, Lu, (R(*)[8]) nullptr);
+          llr_terms(areg[jj], b, R(0.5) * sa, R(0.5) * pr, Lu, (R(*)[8]) nullptr);
           e[(h & 1u) ? qw.pi : (uint32_t)t] = Lu[1] - Lu[0] - sa;
           if (j > g) qw.prev();
         }
@@ -483,7 +569,7 @@ __device__ __forceinline__ void beta_ext_pass(const Lane<R>& L0, const Layout& Y
       }
     }
   };
-  chunks_down<BetaIn<R>>(g, top, load, body);
+  subwindows_down<BetaExtIn<R>>(g / 8, (top + 7) / 8, load, body);
 }
 
 template <typename R>
@@ -493,7 +579,8 @@ __device__ __forceinline__ void load8(const Lane<R>& L, uint32_t field, uint32_t
 }
 
 // alpha over local steps 0 .. g + S with only every 8th stored: al[c] =
-// alpha at local 8c (the same recursion as alpha_pass).
+// alpha at local 8c.  Alpha stays at its start value before step 0 of the
+// block (t < 0).
 template <typename R>
 __device__ __forceinline__ void alpha_ckpt_pass(const Lane<R>& L0, const Layout& Y) {
   const Lane<R> L = fresh(L0);
@@ -631,21 +718,32 @@ __device__ Lane<R> make_lane(const DiffJob& J, const DiffArgs<R>& A) {
 
 template <typename R>
 __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs<R> A) {
+  extern __shared__ __attribute__((aligned(16))) char diff_lds[];
   const Layout Y(A.max_seg, A.max_warm);
   for (uint32_t jb = blockIdx.x; jb < A.njobs; jb += gridDim.x) {
     const DiffJob J = A.jobs[jb];
     const Lane<R> L = make_lane(J, A);
-    const DiffCb C = A.cbs[J.cb_first + (L.active ? L.lane / J.nseg : 0)];
+    const uint32_t blk = L.active ? L.lane / J.nseg : 0;
+    const DiffCb C = A.cbs[J.cb_first + blk];
     const R* d0 = A.llr + C.llr_off;
     const R* d1 = d0 + (L.K + 4);
     const R* d2 = d1 + (L.K + 4);
     R* sv = A.saved + C.save_off;
+    const BlockLds<R> X = block_lds(reinterpret_cast<R*>(diff_lds), J);
     for (uint32_t h = 0; h < A.halves; ++h) {
+      if (A.use_lds) {
+        stage_fill(A, J, h, X);
+        __syncthreads();
+      }
       if (L.active) {
         R ts[3], tp[3];
         tail_inputs(h, d0, d1, d2, L.K, ts, tp);
-        stage(L, Y, h, d0, d1, d2, sv, A.scales);
-        alpha_pass(L, Y);
+        if (A.use_lds) {
+          stage_from_lds(L, Y, h, X, blk);
+        } else {
+          stage(L, Y, h, d0, d1, d2, sv, A.scales);
+        }
+        alpha_ckpt_pass(L, Y);
         beta_ext_pass(L, Y, ts, tp, h, sv + (size_t)h * L.K);
       }
       __syncthreads();
@@ -712,20 +810,27 @@ __device__ __forceinline__ void post_adjoint(const DiffArgs<R>& A, const Lane<R>
     for (uint32_t t = K; t < K + 4; ++t) g0[t] = g1[t] = g2[t] = 0;
 }
 
-// Runs body(c, buf) for c = n - 1 down to 0 with load(c, buf) one ahead,
-// alternating two buffers.
-template <class Buf, class Load, class Body>
-__device__ __forceinline__ void subwindows_down(int n, Load&& load, Body&& body) {
-  if (n <= 0) return;
-  Buf b0, b1;
-  load(n - 1, b0);
-  for (int c = n - 1; c >= 0; c -= 2) {
-    if (c >= 1) load(c - 1, b1);
-    body(c, b0);
-    if (c < 1) break;
-    if (c >= 2) load(c - 2, b0);
-    body(c - 1, b1);
-  }
+// Beta over the window's warm-up above the own steps (local 2g + S - 1 down
+// to g + S), from bcur (the start value at local 2g + S) to beta at local
+// g + S; beta_ck(i, beta) keeps the checkpoints on the way.
+template <typename R, class Ck>
+__device__ __forceinline__ void beta_warmup(const Lane<R>& L0, const Layout& Y, R (&bcur)[8], Ck&& beta_ck) {
+  const Lane<R> L = fresh(L0);
+  const int top = (int)(2 * L.g + L.S), bot = (int)(L.g + L.S);
+  auto load = [&](int hw, In8<R>& c) { load_in8<R, true>(L, Y, hw - 1, min(8, hw - bot), c); };
+  auto body = [&](int hw, const In8<R>& c) {
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+      const int j = hw - 1 - jj;
+      if (j >= bot) {
+        beta_ck((uint32_t)j + 1 - L.g, bcur);
+        R o[8];
+        beta_step(bcur, R(0.5) * c.sa[jj], R(0.5) * c.pr[jj], o);
+        select8(L.t_of((uint32_t)j) < (int)L.K, o, bcur);
+      }
+    }
+  };
+  chunks_down<In8<R>>(bot, top, load, body);
 }
 
 // Alpha adjoint, backwards over local steps g + S - 1 .. 0 in sub-windows of
@@ -800,7 +905,7 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
       }
     }
   };
-  subwindows_down<AlphaIn<R>>((int)((n_al + 7) / 8), load, body);
+  subwindows_down<AlphaIn<R>>(0, (int)((n_al + 7) / 8), load, body);
 }
 
 // Beta adjoint, forwards over local steps g .. g + S + g - 1 in sub-windows
@@ -901,7 +1006,7 @@ struct CombIn {
 };
 
 template <typename R>
-__device__ __forceinline__ R combine(const Lane<R>& L0, const Layout& Y, uint32_t h, R w, const R* eprev, R* g0,
+__device__ __attribute__((noinline)) R combine(const Lane<R>& L0, const Layout& Y, uint32_t h, R w, const R* eprev, R* g0,
                                      R* gpar, R* prev) {
   const Lane<R> L = fresh(L0);
   const bool lower = (h & 1u) != 0;
@@ -967,10 +1072,149 @@ __device__ __forceinline__ R combine(const Lane<R>& L0, const Layout& Y, uint32_
   return wsum;
 }
 
+// LDS path of the combine.  Fill: arrays 0 = grad d0, 1 = grad of the
+// half's parity stream, 2 = the adjoint of e_{h-1} started as the per-lane
+// path starts it (the posterior term for h - 1 == H - 2, else zero), 3 =
+// e_{h-1}.  The lanes then add into the copies in the order of combine; the
+// write-back stores arrays 0-2.
+template <typename R>
+__device__ __attribute__((noinline)) void combine_fill(const DiffArgs<R>& A, const DiffJob& J, uint32_t h, const BlockLds<R>& X) {
+  const uint32_t H = A.halves, K = J.k;
+  const bool lower = (h & 1u) != 0, with_prev = h != 0;
+  const bool from_post = with_prev && h - 1 == H - 2;
+  const R wp = A.scales[H - 2];
+  for (uint32_t b = 0; b < J.nblk; ++b) {
+    const DiffCb C = A.cbs[J.cb_first + b];
+    const R* g0 = A.grad_llr + C.llr_off;
+    const R* gq = g0 + (size_t)(lower ? 2 : 1) * (K + 4);
+    const R* gp = A.grad_post + C.post_off;
+    const R* ep = A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K;
+    for (uint32_t i0 = threadIdx.x; i0 < K; i0 += 64 * kFillBatch) {
+      R x0[kFillBatch], x1[kFillBatch], x2[kFillBatch], x3[kFillBatch];
+#pragma unroll
+      for (uint32_t u = 0; u < kFillBatch; ++u) {
+        const uint32_t i = i0 + 64 * u;
+        if (i < K) {
+          x0[u] = g0[i];
+          x1[u] = gq[i];
+          x2[u] = from_post ? gp[i] : R(0);
+          x3[u] = with_prev ? ep[i] : R(0);
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kFillBatch; ++u) {
+        const uint32_t i = i0 + 64 * u;
+        if (i < K) {
+          X.at(0, b, i) = x0[u];
+          X.at(1, b, i) = x1[u];
+          X.at(2, b, i) = from_post ? wp * x2[u] : R(0);
+          X.at(3, b, i) = x3[u];
+        }
+      }
+    }
+  }
+}
+
+template <typename R>
+__device__ __attribute__((noinline)) void combine_store(const DiffArgs<R>& A, const DiffJob& J, uint32_t h, const BlockLds<R>& X) {
+  const uint32_t K = J.k;
+  const bool lower = (h & 1u) != 0, with_prev = h != 0;
+  for (uint32_t b = 0; b < J.nblk; ++b) {
+    const DiffCb C = A.cbs[J.cb_first + b];
+    R* g0 = A.grad_llr + C.llr_off;
+    R* gq = g0 + (size_t)(lower ? 2 : 1) * (K + 4);
+    R* prev = A.ebar + C.ebar_off + (size_t)((h + 1) & 1u) * K;
+    for (uint32_t i = threadIdx.x; i < K; i += 64) {
+      g0[i] = X.at(0, b, i);
+      gq[i] = X.at(1, b, i);
+      if (with_prev) prev[i] = X.at(2, b, i);
+    }
+  }
+}
+
+// Per-lane part of the LDS combine (the same sums, in the same order, as
+// combine).
+template <typename R>
+__device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uint32_t h, R w, const BlockLds<R>& X,
+                                         uint32_t blk) {
+  const Lane<R> L = fresh(L0);
+  const bool lower = (h & 1u) != 0;
+  const bool with_prev = h != 0;
+  const uint32_t own_end = L.g + L.S;
+  const bool has_next = L.q + 1 < L.M, has_prev = L.q > 0;
+  const uint32_t ln = min(L.lane + 1, 63u), lp = L.lane > 0 ? L.lane - 1 : 0u;
+  QppWalk qw(L.q * L.S, L.K, L.f1, L.f2);
+  R wsum = 0;
+  for (uint32_t j0 = L.g; j0 < own_end; j0 += 8) {
+    R ds[8], dp[8], dsn[8], dpn[8], dsp[8], dpp[8];
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      const uint32_t j = j0 + jj;
+      if (j < own_end) {
+        ds[jj] = L.at(Y.dsa, j);
+        dp[jj] = L.at(Y.dpr, j);
+        if (j >= L.S) {  // t in [qS + S - g, qS + S): lane q+1's local j - S
+          dsn[jj] = L.nb(Y.dsa, j - L.S, ln);
+          dpn[jj] = L.nb(Y.dpr, j - L.S, ln);
+        }
+        if (j < 2 * L.g) {  // t in [qS, qS + g): lane q-1's local j + S
+          dsp[jj] = L.nb(Y.dsa, j + L.S, lp);
+          dpp[jj] = L.nb(Y.dpr, j + L.S, lp);
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      const uint32_t j = j0 + jj;
+      if (j < own_end) {
+        const uint32_t t = (uint32_t)L.t_of(j);
+        const uint32_t i = lower ? qw.pi : t;
+        qw.next();
+        R a = ds[jj], c = dp[jj];
+        if (j >= L.S) {
+          a = has_next ? a + dsn[jj] : a;
+          c = has_next ? c + dpn[jj] : c;
+        }
+        if (j < 2 * L.g) {
+          a = has_prev ? a + dsp[jj] : a;
+          c = has_prev ? c + dpp[jj] : c;
+        }
+        X.at(0, blk, i) = X.at(0, blk, i) + a;
+        X.at(1, blk, t) = X.at(1, blk, t) + c;
+        if (with_prev) {
+          X.at(2, blk, i) = X.at(2, blk, i) + w * a;
+          wsum += a * X.at(3, blk, i);
+        }
+      }
+    }
+  }
+  return wsum;
+}
+
+// LDS path of the extrinsic-adjoint gather: array 0 = the adjoint of e_h.
+template <typename R>
+__device__ __attribute__((noinline)) void eb_fill(const DiffArgs<R>& A, const DiffJob& J, uint32_t h, const BlockLds<R>& X) {
+  const uint32_t K = J.k;
+  for (uint32_t b = 0; b < J.nblk; ++b) {
+    const DiffCb C = A.cbs[J.cb_first + b];
+    const R* cur = A.ebar + C.ebar_off + (size_t)(h & 1u) * K;
+    for (uint32_t i0 = threadIdx.x; i0 < K; i0 += 64 * kFillBatch) {
+      R x[kFillBatch];
+#pragma unroll
+      for (uint32_t u = 0; u < kFillBatch; ++u)
+        if (i0 + 64 * u < K) x[u] = cur[i0 + 64 * u];
+#pragma unroll
+      for (uint32_t u = 0; u < kFillBatch; ++u)
+        if (i0 + 64 * u < K) X.at(0, b, i0 + 64 * u) = x[u];
+    }
+  }
+}
+
 }  // namespace
 
 template <typename R>
 __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
+  extern __shared__ __attribute__((aligned(16))) char diff_lds[];
   const Layout Y(A.max_seg, A.max_warm);
   const uint32_t H = A.halves;
   for (uint32_t jb = blockIdx.x; jb < A.njobs; jb += gridDim.x) {
@@ -990,6 +1234,7 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
     R* eb[2] = {A.ebar + C.ebar_off, A.ebar + C.ebar_off + K};
     R* dsc = A.dscale_part + (size_t)jb * H;
     const uint32_t lo = L.q * L.S, hi = lo + L.S;
+    const BlockLds<R> X = block_lds(reinterpret_cast<R*>(diff_lds), J);
     {
       R s1 = 0, s2 = 0;
       if (L.active) post_adjoint(A, L, sv, gp, g0, g1, g2, eb[(H - 1) & 1u], s1, s2);
@@ -1008,8 +1253,20 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
       R* cur = eb[h & 1u];
       R* prev = eb[(h + 1) & 1u];
       const bool lower = (h & 1u) != 0;
-      if (L.active) {
+      R ts[3], tp[3];
+      if (A.use_lds) {
+        stage_fill(A, J, h, X);
+        __syncthreads();
+        if (L.active) {
+          tail_inputs(h, d0, d1, d2, K, ts, tp);
+          stage_from_lds(L, Y, h, X, blk);
+        }
+        __syncthreads();
+        eb_fill(A, J, h, X);
+        __syncthreads();
+      } else if (L.active) {
         // adjoint of e_{h-1}: starts at the posterior term for h - 1 == H - 2
+        // (the LDS path starts it in its combine fill)
         if (h >= 1) {
           if (h - 1 == H - 2) {
             const R wp = A.scales[H - 2];
@@ -1026,9 +1283,10 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
             for (uint32_t t = lo; t < hi; ++t) prev[t] = 0;
           }
         }
-        R ts[3], tp[3];
         tail_inputs(h, d0, d1, d2, K, ts, tp);
         stage(L, Y, h, d0, d1, d2, sv, A.scales);
+      }
+      if (L.active) {
         {
           // the adjoints of the own steps' extrinsics, into the eb field
           QppWalk qw(lo, K, L.f1, L.f2);
@@ -1038,7 +1296,8 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
             for (uint32_t jj = 0; jj < 8; ++jj)
               if (j0 + jj < L.g + L.S) {
                 const uint32_t t = (uint32_t)L.t_of(j0 + jj);
-                v[jj] = cur[lower ? qw.pi : t];
+                const uint32_t i = lower ? qw.pi : t;
+                v[jj] = A.use_lds ? X.at(0, blk, i) : cur[i];
                 qw.next();
               }
 #pragma unroll
@@ -1073,23 +1332,7 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
             for (int s = 0; s < 8; ++s) L.at(Y.be + s * bst, (i + 7) / 8) = b[s];
           }
         };
-        {
-          const int top = (int)(2 * L.g + L.S), bot = (int)(L.g + L.S);
-          auto load = [&](int hw, In8<R>& c) { load_in8<R, true>(L, Y, hw - 1, min(8, hw - bot), c); };
-          auto body = [&](int hw, const In8<R>& c) {
-#pragma unroll
-            for (int jj = 0; jj < 8; ++jj) {
-              const int j = hw - 1 - jj;
-              if (j >= bot) {
-                beta_ck((uint32_t)j + 1 - L.g, bcur);
-                R o[8];
-                beta_step(bcur, R(0.5) * c.sa[jj], R(0.5) * c.pr[jj], o);
-                select8(L.t_of((uint32_t)j) < (int)K, o, bcur);
-              }
-            }
-          };
-          chunks_down<In8<R>>(bot, top, load, body);
-        }
+        beta_warmup(L, Y, bcur, beta_ck);
         alpha_adjoint_sweep(L, Y, bcur, beta_ck);
         R bb[8];
         beta_adjoint_sweep(L, Y, bb);
@@ -1115,10 +1358,18 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
       }
       __syncthreads();
       R wsum = 0;
-      if (L.active) {
-        const R w = h ? A.scales[h - 1] : R(0);
+      const R w = h ? A.scales[h - 1] : R(0);
+      if (A.use_lds) {
+        combine_fill(A, J, h, X);
+        __syncthreads();
+        if (L.active) wsum = combine_lds(L, Y, h, w, X, blk);
+        __syncthreads();
+        combine_store(A, J, h, X);
+      } else if (L.active) {
         const R* eprev = sv + (size_t)(h ? h - 1 : 0) * K;
         wsum = combine(L, Y, h, w, eprev, g0, lower ? g2 : g1, prev);
+      }
+      if (L.active) {
         if (L.q == L.M - 1) {
           // tail LLR adjoints, in segment order
           R dts[3] = {0, 0, 0}, dtp[3] = {0, 0, 0};

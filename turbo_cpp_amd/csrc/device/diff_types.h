@@ -37,6 +37,16 @@ struct DiffArgs {
   R* scratch;      // per lane, slot_elems each
   uint32_t slot_elems;
   uint32_t max_seg, max_warm;
+  // 1: every job's blocks fit kDiffLdsArrays padded block arrays in the
+  // dynamic LDS of its workgroup (staging and the adjoint combine then go
+  // through LDS with coalesced global accesses)
+  int32_t use_lds;
 };
+
+// LDS copies per block: kDiffLdsArrays arrays of diff_lds_stride(K)
+// elements, element i at i + i / 32 (one pad per 32 elements, so lanes that
+// read positions a multiple of 32 apart use different banks).
+constexpr uint32_t kDiffLdsArrays = 4;
+__host__ __device__ constexpr uint32_t diff_lds_stride(uint32_t k) { return k + (k >> 5) + 1; }
 
 }  // namespace tcdev
