@@ -337,7 +337,7 @@ struct StageIn {
 };
 
 template <typename R>
-__device__ __attribute__((noinline)) void stage(const Lane<R>& L0, const Layout& Y, uint32_t h, const R* d0, const R* d1,
+__device__ __attribute__((noinline)) void stage(const Lane<R> L0, const Layout Y, uint32_t h, const R* d0, const R* d1,
                                       const R* d2, const R* sv, const R* scales) {
   const Lane<R> L = fresh(L0);
   const R w = h ? scales[h - 1] : R(0);
@@ -404,7 +404,7 @@ constexpr uint32_t kFillBatch = 16;  // loads in flight per lane and array
 // half reads it at Pi(t)), 1 = the half's parity stream.  Same arithmetic as
 // the per-lane staging.
 template <typename R>
-__device__ __attribute__((noinline)) void stage_fill(const DiffArgs<R>& A, const DiffJob& J, uint32_t h, const BlockLds<R>& X) {
+__device__ __attribute__((noinline)) void stage_fill(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const R w = h ? A.scales[h - 1] : R(0);
   const bool lower = (h & 1u) != 0;
   const uint32_t K = J.k;
@@ -1006,7 +1006,7 @@ struct CombIn {
 };
 
 template <typename R>
-__device__ __attribute__((noinline)) R combine(const Lane<R>& L0, const Layout& Y, uint32_t h, R w, const R* eprev, R* g0,
+__device__ __attribute__((noinline)) R combine(const Lane<R> L0, const Layout Y, uint32_t h, R w, const R* eprev, R* g0,
                                      R* gpar, R* prev) {
   const Lane<R> L = fresh(L0);
   const bool lower = (h & 1u) != 0;
@@ -1078,7 +1078,7 @@ __device__ __attribute__((noinline)) R combine(const Lane<R>& L0, const Layout& 
 // e_{h-1}.  The lanes then add into the copies in the order of combine; the
 // write-back stores arrays 0-2.
 template <typename R>
-__device__ __attribute__((noinline)) void combine_fill(const DiffArgs<R>& A, const DiffJob& J, uint32_t h, const BlockLds<R>& X) {
+__device__ __attribute__((noinline)) void combine_fill(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const uint32_t H = A.halves, K = J.k;
   const bool lower = (h & 1u) != 0, with_prev = h != 0;
   const bool from_post = with_prev && h - 1 == H - 2;
@@ -1116,7 +1116,7 @@ __device__ __attribute__((noinline)) void combine_fill(const DiffArgs<R>& A, con
 }
 
 template <typename R>
-__device__ __attribute__((noinline)) void combine_store(const DiffArgs<R>& A, const DiffJob& J, uint32_t h, const BlockLds<R>& X) {
+__device__ __attribute__((noinline)) void combine_store(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const uint32_t K = J.k;
   const bool lower = (h & 1u) != 0, with_prev = h != 0;
   for (uint32_t b = 0; b < J.nblk; ++b) {
@@ -1193,7 +1193,7 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
 
 // LDS path of the extrinsic-adjoint gather: array 0 = the adjoint of e_h.
 template <typename R>
-__device__ __attribute__((noinline)) void eb_fill(const DiffArgs<R>& A, const DiffJob& J, uint32_t h, const BlockLds<R>& X) {
+__device__ __attribute__((noinline)) void eb_fill(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const uint32_t K = J.k;
   for (uint32_t b = 0; b < J.nblk; ++b) {
     const DiffCb C = A.cbs[J.cb_first + b];
