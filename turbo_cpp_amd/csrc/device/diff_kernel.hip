@@ -388,10 +388,28 @@ template <typename R>
 struct BlockLds {
   R* base;
   uint32_t kp, nblk;  // diff_lds_stride(K), blocks of the job
-  __device__ R& at(uint32_t arr, uint32_t b, uint32_t i) const {
-    return base[(arr * nblk + b) * kp + i + (i >> 5)];
-  }
+  __device__ uint32_t idx(uint32_t arr, uint32_t b, uint32_t i) const { return (arr * nblk + b) * kp + i + (i >> 5); }
+  __device__ R& at(uint32_t arr, uint32_t b, uint32_t i) const { return base[idx(arr, b, i)]; }
 };
+// Address-space-qualified views for the out-of-line fills, where the
+// optimiser cannot see that a pointer argument is global or LDS memory (a
+// generic pointer would make every access a flat one).
+template <typename R>
+using GlobalPtr = __attribute__((address_space(1))) R*;
+template <typename R>
+using LdsPtr = __attribute__((address_space(3))) R*;
+template <typename R>
+__device__ __forceinline__ GlobalPtr<R> global_view(R* p) {
+  return (GlobalPtr<R>)p;
+}
+template <typename R>
+__device__ __forceinline__ GlobalPtr<const R> global_view(const R* p) {
+  return (GlobalPtr<const R>)p;
+}
+template <typename R>
+__device__ __forceinline__ LdsPtr<R> lds_view(const BlockLds<R>& X) {
+  return (LdsPtr<R>)X.base;
+}
 
 template <typename R>
 __device__ __forceinline__ BlockLds<R> block_lds(R* base, const DiffJob& J) {
@@ -408,11 +426,12 @@ __device__ __attribute__((noinline)) void stage_fill(const DiffArgs<R> A, const 
   const R w = h ? A.scales[h - 1] : R(0);
   const bool lower = (h & 1u) != 0;
   const uint32_t K = J.k;
+  const LdsPtr<R> xl = lds_view(X);
   for (uint32_t b = 0; b < J.nblk; ++b) {
     const DiffCb C = A.cbs[J.cb_first + b];
-    const R* d0 = A.llr + C.llr_off;
-    const R* dp = d0 + (size_t)(lower ? 2 : 1) * (K + 4);
-    const R* ep = A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K;
+    const GlobalPtr<const R> d0 = global_view(A.llr + C.llr_off);
+    const GlobalPtr<const R> dp = d0 + (size_t)(lower ? 2 : 1) * (K + 4);
+    const GlobalPtr<const R> ep = global_view(A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K);
     for (uint32_t i0 = threadIdx.x; i0 < K; i0 += 64 * kFillBatch) {
       R sv[kFillBatch], ev[kFillBatch], pv[kFillBatch];
 #pragma unroll
@@ -428,8 +447,8 @@ __device__ __attribute__((noinline)) void stage_fill(const DiffArgs<R> A, const 
       for (uint32_t u = 0; u < kFillBatch; ++u) {
         const uint32_t i = i0 + 64 * u;
         if (i < K) {
-          X.at(0, b, i) = lower ? sv[u] + w * ev[u] : sv[u] + (h ? w * ev[u] : R(0));
-          X.at(1, b, i) = pv[u];
+          xl[X.idx(0, b, i)] = lower ? sv[u] + w * ev[u] : sv[u] + (h ? w * ev[u] : R(0));
+          xl[X.idx(1, b, i)] = pv[u];
         }
       }
     }
@@ -1083,12 +1102,13 @@ __device__ __attribute__((noinline)) void combine_fill(const DiffArgs<R> A, cons
   const bool lower = (h & 1u) != 0, with_prev = h != 0;
   const bool from_post = with_prev && h - 1 == H - 2;
   const R wp = A.scales[H - 2];
+  const LdsPtr<R> xl = lds_view(X);
   for (uint32_t b = 0; b < J.nblk; ++b) {
     const DiffCb C = A.cbs[J.cb_first + b];
-    const R* g0 = A.grad_llr + C.llr_off;
-    const R* gq = g0 + (size_t)(lower ? 2 : 1) * (K + 4);
-    const R* gp = A.grad_post + C.post_off;
-    const R* ep = A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K;
+    const GlobalPtr<const R> g0 = global_view<R>(A.grad_llr + C.llr_off);
+    const GlobalPtr<const R> gq = g0 + (size_t)(lower ? 2 : 1) * (K + 4);
+    const GlobalPtr<const R> gp = global_view(A.grad_post + C.post_off);
+    const GlobalPtr<const R> ep = global_view(A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K);
     for (uint32_t i0 = threadIdx.x; i0 < K; i0 += 64 * kFillBatch) {
       R x0[kFillBatch], x1[kFillBatch], x2[kFillBatch], x3[kFillBatch];
 #pragma unroll
@@ -1105,10 +1125,10 @@ __device__ __attribute__((noinline)) void combine_fill(const DiffArgs<R> A, cons
       for (uint32_t u = 0; u < kFillBatch; ++u) {
         const uint32_t i = i0 + 64 * u;
         if (i < K) {
-          X.at(0, b, i) = x0[u];
-          X.at(1, b, i) = x1[u];
-          X.at(2, b, i) = from_post ? wp * x2[u] : R(0);
-          X.at(3, b, i) = x3[u];
+          xl[X.idx(0, b, i)] = x0[u];
+          xl[X.idx(1, b, i)] = x1[u];
+          xl[X.idx(2, b, i)] = from_post ? wp * x2[u] : R(0);
+          xl[X.idx(3, b, i)] = x3[u];
         }
       }
     }
@@ -1119,15 +1139,16 @@ template <typename R>
 __device__ __attribute__((noinline)) void combine_store(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const uint32_t K = J.k;
   const bool lower = (h & 1u) != 0, with_prev = h != 0;
+  const LdsPtr<R> xl = lds_view(X);
   for (uint32_t b = 0; b < J.nblk; ++b) {
     const DiffCb C = A.cbs[J.cb_first + b];
-    R* g0 = A.grad_llr + C.llr_off;
-    R* gq = g0 + (size_t)(lower ? 2 : 1) * (K + 4);
-    R* prev = A.ebar + C.ebar_off + (size_t)((h + 1) & 1u) * K;
+    const GlobalPtr<R> g0 = global_view(A.grad_llr + C.llr_off);
+    const GlobalPtr<R> gq = g0 + (size_t)(lower ? 2 : 1) * (K + 4);
+    const GlobalPtr<R> prev = global_view(A.ebar + C.ebar_off + (size_t)((h + 1) & 1u) * K);
     for (uint32_t i = threadIdx.x; i < K; i += 64) {
-      g0[i] = X.at(0, b, i);
-      gq[i] = X.at(1, b, i);
-      if (with_prev) prev[i] = X.at(2, b, i);
+      g0[i] = xl[X.idx(0, b, i)];
+      gq[i] = xl[X.idx(1, b, i)];
+      if (with_prev) prev[i] = xl[X.idx(2, b, i)];
     }
   }
 }
@@ -1195,9 +1216,10 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
 template <typename R>
 __device__ __attribute__((noinline)) void eb_fill(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const uint32_t K = J.k;
+  const LdsPtr<R> xl = lds_view(X);
   for (uint32_t b = 0; b < J.nblk; ++b) {
     const DiffCb C = A.cbs[J.cb_first + b];
-    const R* cur = A.ebar + C.ebar_off + (size_t)(h & 1u) * K;
+    const GlobalPtr<const R> cur = global_view<R>(A.ebar + C.ebar_off + (size_t)(h & 1u) * K);
     for (uint32_t i0 = threadIdx.x; i0 < K; i0 += 64 * kFillBatch) {
       R x[kFillBatch];
 #pragma unroll
@@ -1205,7 +1227,7 @@ __device__ __attribute__((noinline)) void eb_fill(const DiffArgs<R> A, const Dif
         if (i0 + 64 * u < K) x[u] = cur[i0 + 64 * u];
 #pragma unroll
       for (uint32_t u = 0; u < kFillBatch; ++u)
-        if (i0 + 64 * u < K) X.at(0, b, i0 + 64 * u) = x[u];
+        if (i0 + 64 * u < K) xl[X.idx(0, b, i0 + 64 * u)] = x[u];
     }
   }
 }
