@@ -93,7 +93,7 @@ def cpu_baseline(llr):
     threads = max(1, min(16, os.cpu_count() or 1))
     if oracle.ref_available():
         kind = "reference"
-        n = 6 * threads
+        n = 30 * threads  # ~37 ms per block and thread: 15-20 s of CPU work in about a second
         sample = np.ascontiguousarray(llr[:n])
         oracle.ref_decode_many(sample[:1], K, ITERS, 1)  # warm-up
         t0 = time.perf_counter()
