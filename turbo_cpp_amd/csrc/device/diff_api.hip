@@ -107,8 +107,8 @@ void build(tc_diff_plan& p, const uint32_t* k, size_t n, uint32_t iterations, ui
     }
     p.max_seg = std::max(p.max_seg, s);
     p.max_warm = std::max(p.max_warm, g);
-    lds_elems = std::max<size_t>(lds_elems, (size_t)tcdev::kDiffLdsArrays * std::min<size_t>(per, idx.size()) *
-                                                tcdev::diff_lds_stride(kk));
+    lds_elems = std::max<size_t>(
+        lds_elems, tcdev::diff_lds_elems(kk, g, static_cast<uint32_t>(std::min<size_t>(per, idx.size()))));
   }
   p.njobs = static_cast<uint32_t>(jobs.size());
   p.slot_elems = slot_elems(p.max_seg, p.max_warm);

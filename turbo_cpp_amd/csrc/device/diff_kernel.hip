@@ -225,12 +225,24 @@ struct Layout {
   }
 };
 
+// Address-space-qualified pointers, for code where the optimiser cannot see
+// that a pointer is global or LDS memory (a generic pointer would make every
+// access a flat one).
+template <typename R>
+using GlobalPtr = __attribute__((address_space(1))) R*;
+template <typename R>
+using LdsPtr = __attribute__((address_space(3))) R*;
+
 template <typename R>
 struct Lane {
   uint32_t K, S, M, g, q, f1, f2;
   bool active;
   uint32_t lane;
   R* sc;  // this wave's scratch (uniform), element (field + j) * 64 + lane
+  // staged inputs: the block's LDS pairs (lds; local step j is position
+  // p = qS + j, pair at p + p / 32) or the per-lane sa / pr fields
+  bool lds;
+  LdsPtr<R> xs;
   // 32-bit byte offsets from the wave's base (a slot is far below 4 GiB):
   // one 32-bit offset per field and the steps as immediate offsets, not a
   // 64-bit address per access
@@ -317,13 +329,25 @@ struct In8 {
 };
 template <typename R, bool DOWN = false>
 __device__ __forceinline__ void load_in8(const Lane<R>& L, const Layout& Y, int j0, int n, In8<R>& c) {
+  if (L.lds) {
+    const uint32_t pb = L.q * L.S;
 #pragma unroll
-  for (int jj = 0; jj < 8; ++jj)
-    if (jj < n) {
-      const uint32_t j = (uint32_t)(DOWN ? j0 - jj : j0 + jj);
-      c.sa[jj] = L.at(Y.sa, j);
-      c.pr[jj] = L.at(Y.pr, j);
-    }
+    for (int jj = 0; jj < 8; ++jj)
+      if (jj < n) {
+        const uint32_t p = pb + (uint32_t)(DOWN ? j0 - jj : j0 + jj);
+        const uint32_t i = 2 * (p + (p >> 5));
+        c.sa[jj] = L.xs[i];
+        c.pr[jj] = L.xs[i + 1];
+      }
+  } else {
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj)
+      if (jj < n) {
+        const uint32_t j = (uint32_t)(DOWN ? j0 - jj : j0 + jj);
+        c.sa[jj] = L.at(Y.sa, j);
+        c.pr[jj] = L.at(Y.pr, j);
+      }
+  }
 }
 
 // Stage sys + la and par of one half over the lane's window.  Upper half
@@ -388,16 +412,12 @@ template <typename R>
 struct BlockLds {
   R* base;
   uint32_t kp, nblk;  // diff_lds_stride(K), blocks of the job
+  uint32_t kq;        // diff_stage_pairs(K, g)
   __device__ uint32_t idx(uint32_t arr, uint32_t b, uint32_t i) const { return (arr * nblk + b) * kp + i + (i >> 5); }
   __device__ R& at(uint32_t arr, uint32_t b, uint32_t i) const { return base[idx(arr, b, i)]; }
+  // first element of block b's staged pairs (after arrays 0 and 1)
+  __device__ uint32_t stage_base(uint32_t b) const { return 2 * nblk * kp + 2 * b * kq; }
 };
-// Address-space-qualified views for the out-of-line fills, where the
-// optimiser cannot see that a pointer argument is global or LDS memory (a
-// generic pointer would make every access a flat one).
-template <typename R>
-using GlobalPtr = __attribute__((address_space(1))) R*;
-template <typename R>
-using LdsPtr = __attribute__((address_space(3))) R*;
 template <typename R>
 __device__ __forceinline__ GlobalPtr<R> global_view(R* p) {
   return (GlobalPtr<R>)p;
@@ -413,7 +433,7 @@ __device__ __forceinline__ LdsPtr<R> lds_view(const BlockLds<R>& X) {
 
 template <typename R>
 __device__ __forceinline__ BlockLds<R> block_lds(R* base, const DiffJob& J) {
-  return BlockLds<R>{base, diff_lds_stride(J.k), J.nblk};
+  return BlockLds<R>{base, diff_lds_stride(J.k), J.nblk, diff_stage_pairs(J.k, J.warm)};
 }
 
 constexpr uint32_t kFillBatch = 16;  // loads in flight per lane and array
@@ -455,26 +475,31 @@ __device__ __attribute__((noinline)) void stage_fill(const DiffArgs<R> A, const 
   }
 }
 
-// Per-lane staging from the LDS copies (see stage for the values).
+// Staged pairs of the half from arrays 0 and 1 (see stage for the values):
+// pair p of block b holds (sys + la, parity) at position t = p - g of the
+// half's order, zeros outside [0, K).  Each lane takes a run of consecutive
+// positions (the lower half's interleaver walked by its recurrence).
 template <typename R>
-__device__ __forceinline__ void stage_from_lds(const Lane<R>& L0, const Layout& Y, uint32_t h, const BlockLds<R>& X,
-                                               uint32_t blk) {
-  const Lane<R> L = fresh(L0);
+__device__ __attribute__((noinline)) void stage_gather(const DiffJob J, uint32_t h, const BlockLds<R> X) {
+  const LdsPtr<R> xl = lds_view(X);
   const bool lower = (h & 1u) != 0;
-  const int t0 = L.t_of(0);
-  QppWalk qw(t0 > 0 ? (uint32_t)t0 : 0u, L.K, L.f1, L.f2);
-  for (uint32_t j0 = 0; j0 < Y.P; j0 += 8) {
-#pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
-      const uint32_t j = j0 + jj;
-      if (j < Y.P) {
-        const int t = L.t_of(j);
-        const bool v = t >= 0 && t < (int)L.K;
-        const uint32_t tc = (uint32_t)min(max(t, 0), (int)L.K - 1);
-        const R sa = X.at(0, blk, lower ? qw.pi : tc);
-        const R pr = X.at(1, blk, tc);
-        L.at(Y.sa, j) = v ? sa : R(0);
-        L.at(Y.pr, j) = v ? pr : R(0);
+  const uint32_t K = J.k, g = J.warm, n = K + 2 * g;
+  const uint32_t run = (n + 63) / 64, p0 = threadIdx.x * run;
+  const int t0 = (int)p0 - (int)g;
+  for (uint32_t b = 0; b < J.nblk; ++b) {
+    const uint32_t sb = X.stage_base(b);
+    QppWalk qw(t0 > 0 ? (uint32_t)t0 : 0u, K, J.f1, J.f2);
+    for (uint32_t u = 0; u < run; ++u) {
+      const uint32_t p = p0 + u;
+      if (p < n) {
+        const int t = (int)p - (int)g;
+        const bool v = t >= 0 && t < (int)K;
+        const uint32_t tc = v ? (uint32_t)t : 0u;
+        const R sa = xl[X.idx(0, b, lower ? qw.pi : tc)];
+        const R pr = xl[X.idx(1, b, tc)];
+        const uint32_t i = sb + 2 * (p + (p >> 5));
+        xl[i] = v ? sa : R(0);
+        xl[i + 1] = v ? pr : R(0);
         qw.next_if(v);
       }
     }
@@ -517,12 +542,12 @@ __device__ __forceinline__ void subwindows_down(int lo, int hi, Load&& load, Bod
   }
 }
 
-// Forward beta pass that also emits the extrinsics of the own steps: beta
-// from the top of the window down, in sub-windows [8c, 8c + 8) of local
-// steps; for a sub-window with own steps, alpha is recomputed from its
-// checkpoint (al[c], alpha at local 8c, from alpha_ckpt_pass) into
-// registers first.  Past the block end (t >= K) beta stays at its start
-// value; beta is not stored.
+// Forward beta pass over the own steps that also emits their extrinsics:
+// beta from local g + S (b: beta there, after the warm-up of
+// alpha_ckpt_beta_warm_pass) down to local g, in sub-windows [8c, 8c + 8) of
+// local steps; alpha of each sub-window is recomputed from its checkpoint
+// (al[c], alpha at local 8c) into registers first.  Past the block end
+// (t >= K) beta stays at its start value; beta is not stored.
 template <typename R>
 struct BetaExtIn {
   In8<R> in;  // steps 8c + jj
@@ -530,18 +555,8 @@ struct BetaExtIn {
 };
 
 template <typename R>
-__device__ __forceinline__ void beta_ext_pass(const Lane<R>& L0, const Layout& Y, const R (&ts)[3], const R (&tp)[3],
-                                              uint32_t h, R* e) {
+__device__ __forceinline__ void beta_ext_pass(const Lane<R>& L0, const Layout& Y, R (&b)[8], uint32_t h, R* e) {
   const Lane<R> L = fresh(L0);
-  R b[8];
-  const bool from_tail = L.q * L.S + L.S + L.g >= L.K;
-  if (from_tail) {
-    tail_beta(ts, tp, b);
-  } else {
-#pragma unroll
-    for (int s = 0; s < 8; ++s) b[s] = 0;
-  }
-  const int top = (int)(2 * L.g + L.S);
   const int own_end = (int)(L.g + L.S);  // own steps: g <= j < g + S
   const int g = (int)L.g;
   const uint32_t st = Y.g + Y.S + 1;
@@ -549,13 +564,13 @@ __device__ __forceinline__ void beta_ext_pass(const Lane<R>& L0, const Layout& Y
   QppWalk qw((uint32_t)L.t_of((uint32_t)own_end - 1), L.K, L.f1, L.f2);
   auto load = [&](int c, BetaExtIn<R>& x) {
     const int j0 = 8 * c;
-    load_in8(L, Y, j0, min(8, top - j0), x.in);
-    if (j0 < own_end) load8(L, Y.al, st, (uint32_t)c, x.a0);
+    load_in8(L, Y, j0, min(8, own_end - j0), x.in);
+    load8(L, Y.al, st, (uint32_t)c, x.a0);
   };
   auto body = [&](int c, const BetaExtIn<R>& x) {
     const int j0 = 8 * c;
     R areg[8][8];
-    if (j0 < own_end) {
+    {
       R a[8];
 #pragma unroll
       for (int s = 0; s < 8; ++s) a[s] = x.a0[s];
@@ -573,10 +588,10 @@ __device__ __forceinline__ void beta_ext_pass(const Lane<R>& L0, const Layout& Y
 #pragma unroll
     for (int jj = 7; jj >= 0; --jj) {
       const int j = j0 + jj;
-      if (j >= g && j < top) {
+      if (j >= g && j < own_end) {
         const R sa = x.in.sa[jj], pr = x.in.pr[jj];
         const int t = L.t_of((uint32_t)j);
-        if (j < own_end) {
+        {
           R Lu[2];
           llr_terms(areg[jj], b, R(0.5) * sa, R(0.5) * pr, Lu, (R(*)[8]) nullptr);
           e[(h & 1u) ? qw.pi : (uint32_t)t] = Lu[1] - Lu[0] - sa;
@@ -588,7 +603,7 @@ __device__ __forceinline__ void beta_ext_pass(const Lane<R>& L0, const Layout& Y
       }
     }
   };
-  subwindows_down<BetaExtIn<R>>(g / 8, (top + 7) / 8, load, body);
+  subwindows_down<BetaExtIn<R>>(g / 8, (own_end + 7) / 8, load, body);
 }
 
 template <typename R>
@@ -597,19 +612,36 @@ __device__ __forceinline__ void load8(const Lane<R>& L, uint32_t field, uint32_t
   for (int s = 0; s < 8; ++s) v[s] = L.at(field + s * stride, j);
 }
 
-// alpha over local steps 0 .. g + S with only every 8th stored: al[c] =
-// alpha at local 8c.  Alpha stays at its start value before step 0 of the
-// block (t < 0).
+// Alpha over local steps 0 .. g + S with only every 8th stored (al[c] =
+// alpha at local 8c; alpha stays at its start value before step 0 of the
+// block, t < 0) and, in the same loop, beta over the warm-up above the own
+// steps: local 2g + S - 1 down to g + S, from bcur (beta at local 2g + S) to
+// beta at local g + S, with beta_ck(i, beta) called before each beta step
+// (i = j + 1 - g for step j) to keep checkpoints.  The two recursions are
+// independent; a job is one wave and rarely shares its SIMD, so each chunk
+// interleaves them to give the SIMD two dependency chains to issue from.
+// Every value is the one the two recursions compute run one after the other.
 template <typename R>
-__device__ __forceinline__ void alpha_ckpt_pass(const Lane<R>& L0, const Layout& Y) {
+struct AlphaBetaIn {
+  In8<R> up;  // alpha inputs, local j0 + jj
+  In8<R> dn;  // beta inputs, local 2g + S - 1 - (j0 + jj)
+};
+
+template <typename R, class Ck>
+__device__ __forceinline__ void alpha_ckpt_beta_warm_pass(const Lane<R>& L0, const Layout& Y, R (&bcur)[8],
+                                                          Ck&& beta_ck) {
   const Lane<R> L = fresh(L0);
   R a[8];
   const bool from_zero_state = (int)(L.q * L.S) - (int)L.g <= 0;
 #pragma unroll
   for (int s = 0; s < 8; ++s) a[s] = from_zero_state ? (s == 0 ? R(0) : R(kNegF)) : R(0);
   const uint32_t n = L.g + L.S;
-  auto load = [&](uint32_t j0, In8<R>& c) { load_in8(L, Y, (int)j0, (int)min(8u, n - j0), c); };
-  auto body = [&](uint32_t j0, const In8<R>& c) {
+  const int top = (int)(2 * L.g + L.S);
+  auto load = [&](uint32_t j0, AlphaBetaIn<R>& c) {
+    load_in8(L, Y, (int)j0, (int)min(8u, n - j0), c.up);
+    if (j0 < L.g) load_in8<R, true>(L, Y, top - 1 - (int)j0, min(8, (int)L.g - (int)j0), c.dn);
+  };
+  auto body = [&](uint32_t j0, const AlphaBetaIn<R>& c) {
 #pragma unroll
     for (int s = 0; s < 8; ++s) L.at(Y.al + s * (Y.g + Y.S + 1), j0 / 8) = a[s];
 #pragma unroll
@@ -617,12 +649,19 @@ __device__ __forceinline__ void alpha_ckpt_pass(const Lane<R>& L0, const Layout&
       const uint32_t j = j0 + jj;
       if (j < n) {
         R o[8];
-        alpha_step(a, R(0.5) * c.sa[jj], R(0.5) * c.pr[jj], o);
+        alpha_step(a, R(0.5) * c.up.sa[jj], R(0.5) * c.up.pr[jj], o);
         select8(L.t_of(j) >= 0, o, a);
+      }
+      if (j < L.g) {
+        const int jb = top - 1 - (int)j;
+        beta_ck((uint32_t)jb + 1 - L.g, bcur);
+        R o[8];
+        beta_step(bcur, R(0.5) * c.dn.sa[jj], R(0.5) * c.dn.pr[jj], o);
+        select8(L.t_of((uint32_t)jb) < (int)L.K, o, bcur);
       }
     }
   };
-  chunks_up<In8<R>>(0, n, load, body);
+  chunks_up<AlphaBetaIn<R>>(0, n, load, body);
 }
 
 // One step of the alpha adjoint at local j (alpha_j = a, beta_{j+1} = b,
@@ -729,6 +768,16 @@ __device__ Lane<R> make_lane(const DiffJob& J, const DiffArgs<R>& A) {
   L.f2 = J.f2;
   L.active = L.lane / J.nseg < J.nblk;
   L.sc = A.scratch + (size_t)blockIdx.x * 64 * A.slot_elems;
+  L.lds = false;
+  L.xs = nullptr;
+  return L;
+}
+
+// The lane reading its staged inputs from the block's LDS pairs.
+template <typename R>
+__device__ __forceinline__ Lane<R> with_lds(Lane<R> L, const BlockLds<R>& X, uint32_t blk) {
+  L.lds = true;
+  L.xs = lds_view(X) + X.stage_base(blk);
   return L;
 }
 
@@ -741,29 +790,35 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs<R> A) {
   const Layout Y(A.max_seg, A.max_warm);
   for (uint32_t jb = blockIdx.x; jb < A.njobs; jb += gridDim.x) {
     const DiffJob J = A.jobs[jb];
-    const Lane<R> L = make_lane(J, A);
-    const uint32_t blk = L.active ? L.lane / J.nseg : 0;
+    const Lane<R> L0 = make_lane(J, A);
+    const uint32_t blk = L0.active ? L0.lane / J.nseg : 0;
+    const BlockLds<R> X = block_lds(reinterpret_cast<R*>(diff_lds), J);
+    const Lane<R> L = A.use_lds ? with_lds(L0, X, blk) : L0;
     const DiffCb C = A.cbs[J.cb_first + blk];
     const R* d0 = A.llr + C.llr_off;
     const R* d1 = d0 + (L.K + 4);
     const R* d2 = d1 + (L.K + 4);
     R* sv = A.saved + C.save_off;
-    const BlockLds<R> X = block_lds(reinterpret_cast<R*>(diff_lds), J);
     for (uint32_t h = 0; h < A.halves; ++h) {
       if (A.use_lds) {
         stage_fill(A, J, h, X);
+        __syncthreads();
+        stage_gather(J, h, X);
         __syncthreads();
       }
       if (L.active) {
         R ts[3], tp[3];
         tail_inputs(h, d0, d1, d2, L.K, ts, tp);
-        if (A.use_lds) {
-          stage_from_lds(L, Y, h, X, blk);
+        if (!A.use_lds) stage(L, Y, h, d0, d1, d2, sv, A.scales);
+        R b[8];
+        if (L.q * L.S + L.S + L.g >= L.K) {
+          tail_beta(ts, tp, b);
         } else {
-          stage(L, Y, h, d0, d1, d2, sv, A.scales);
+#pragma unroll
+          for (int s = 0; s < 8; ++s) b[s] = 0;
         }
-        alpha_ckpt_pass(L, Y);
-        beta_ext_pass(L, Y, ts, tp, h, sv + (size_t)h * L.K);
+        alpha_ckpt_beta_warm_pass(L, Y, b, [](uint32_t, const R (&)[8]) {});
+        beta_ext_pass(L, Y, b, h, sv + (size_t)h * L.K);
       }
       __syncthreads();
     }
@@ -829,29 +884,6 @@ __device__ __forceinline__ void post_adjoint(const DiffArgs<R>& A, const Lane<R>
     for (uint32_t t = K; t < K + 4; ++t) g0[t] = g1[t] = g2[t] = 0;
 }
 
-// Beta over the window's warm-up above the own steps (local 2g + S - 1 down
-// to g + S), from bcur (the start value at local 2g + S) to beta at local
-// g + S; beta_ck(i, beta) keeps the checkpoints on the way.
-template <typename R, class Ck>
-__device__ __forceinline__ void beta_warmup(const Lane<R>& L0, const Layout& Y, R (&bcur)[8], Ck&& beta_ck) {
-  const Lane<R> L = fresh(L0);
-  const int top = (int)(2 * L.g + L.S), bot = (int)(L.g + L.S);
-  auto load = [&](int hw, In8<R>& c) { load_in8<R, true>(L, Y, hw - 1, min(8, hw - bot), c); };
-  auto body = [&](int hw, const In8<R>& c) {
-#pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
-      const int j = hw - 1 - jj;
-      if (j >= bot) {
-        beta_ck((uint32_t)j + 1 - L.g, bcur);
-        R o[8];
-        beta_step(bcur, R(0.5) * c.sa[jj], R(0.5) * c.pr[jj], o);
-        select8(L.t_of((uint32_t)j) < (int)L.K, o, bcur);
-      }
-    }
-  };
-  chunks_down<In8<R>>(bot, top, load, body);
-}
-
 // Alpha adjoint, backwards over local steps g + S - 1 .. 0 in sub-windows of
 // 8 (alpha recomputed from the checkpoint), the output adjoints of the own
 // steps on the way, and beta of the own steps (bcur: beta at local g + S on
@@ -859,7 +891,8 @@ __device__ __forceinline__ void beta_warmup(const Lane<R>& L0, const Layout& Y, 
 // Writes dsa / dpr of local steps below g + S (their first writer).
 template <typename R>
 struct AlphaIn {
-  R a0[8], hs[8], hp[8], e[8];
+  R a0[8], e[8];
+  In8<R> in;  // staged inputs (halved where used)
 };
 
 template <typename R, class Ck>
@@ -870,12 +903,11 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
   auto load = [&](int c, AlphaIn<R>& x) {
     const uint32_t j0 = 8u * (uint32_t)c, len = min(8u, n_al - j0);
     load8(L, Y.al, Y.g + Y.S + 1, (uint32_t)c, x.a0);
+    load_in8(L, Y, (int)j0, (int)len, x.in);
 #pragma unroll
     for (uint32_t jj = 0; jj < 8; ++jj) {
       if (jj < len) {
         const uint32_t j = j0 + jj;
-        x.hs[jj] = R(0.5) * L.at(Y.sa, j);
-        x.hp[jj] = R(0.5) * L.at(Y.pr, j);
         x.e[jj] = (j >= L.g) ? L.at(Y.eb, j - L.g) : R(0);
       }
     }
@@ -894,7 +926,7 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
           for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
           if (jj + 1 < len) {
             R o[8];
-            alpha_step(a, x.hs[jj], x.hp[jj], o);
+            alpha_step(a, R(0.5) * x.in.sa[jj], R(0.5) * x.in.pr[jj], o);
             select8(L.t_of(j0 + jj) >= 0, o, a);
           }
         }
@@ -907,7 +939,7 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
         R b[8];
 #pragma unroll
         for (int s = 0; s < 8; ++s) b[s] = bcur[s];  // beta_{j+1} while j >= g
-        const R hs = x.hs[jj], hp = x.hp[jj];
+        const R hs = R(0.5) * x.in.sa[jj], hp = R(0.5) * x.in.pr[jj];
         if (j >= L.g) {
           // own step (inside the block for every lane): checkpoint
           // beta_{j+1}, step beta to beta_j
@@ -933,7 +965,8 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
 // that of the tail beta).
 template <typename R>
 struct BetaRe {
-  R b0[8], hs[8], hp[8];
+  R b0[8];
+  In8<R> in;  // staged inputs (halved where used)
 };
 
 template <typename R>
@@ -947,13 +980,7 @@ __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layo
     const uint32_t len = min(8u, jend - jb0);
     const uint32_t itop = jb0 + len - L.g;  // beta at local jb0 + len = g + itop
     load8(L, Y.be, bst, (itop + 7) / 8, x.b0);
-#pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
-      if (jj < len) {
-        x.hs[jj] = R(0.5) * L.at(Y.sa, jb0 + jj);
-        x.hp[jj] = R(0.5) * L.at(Y.pr, jb0 + jj);
-      }
-    }
+    load_in8(L, Y, (int)jb0, (int)len, x.in);
   };
   auto body = [&](uint32_t jb0, const BetaRe<R>& x) {
     const uint32_t len = min(8u, jend - jb0);
@@ -986,7 +1013,7 @@ __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layo
           for (int s = 0; s < 8; ++s) breg[jj][s] = b[s];  // beta at local j + 1
           if (jj > 0) {
             R o[8];
-            beta_step(b, x.hs[jj], x.hp[jj], o);
+            beta_step(b, R(0.5) * x.in.sa[jj], R(0.5) * x.in.pr[jj], o);
             select8(L.t_of(jb0 + (uint32_t)jj) < (int)L.K, o, b);
           }
         }
@@ -1002,7 +1029,7 @@ __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layo
           nb[s] = bb[s];
           bn[s] = bor[jj][s];
         }
-        beta_adjoint_step(breg[jj], x.hs[jj], x.hp[jj], nb, bn, ds, dp);
+        beta_adjoint_step(breg[jj], R(0.5) * x.in.sa[jj], R(0.5) * x.in.pr[jj], nb, bn, ds, dp);
         // past the block end the adjoint stops (beta is not stepped there)
         select8(L.t_of(j) < (int)L.K, nb, bb);
         L.at(Y.dsa, j) = dsr[jj] + ds;
@@ -1241,8 +1268,10 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
   const uint32_t H = A.halves;
   for (uint32_t jb = blockIdx.x; jb < A.njobs; jb += gridDim.x) {
     const DiffJob J = A.jobs[jb];
-    const Lane<R> L = make_lane(J, A);
-    const uint32_t blk = L.active ? L.lane / J.nseg : 0;
+    const Lane<R> L0 = make_lane(J, A);
+    const uint32_t blk = L0.active ? L0.lane / J.nseg : 0;
+    const BlockLds<R> X = block_lds(reinterpret_cast<R*>(diff_lds), J);
+    const Lane<R> L = A.use_lds ? with_lds(L0, X, blk) : L0;
     const DiffCb C = A.cbs[J.cb_first + blk];
     const uint32_t K = L.K, D = K + 4;
     const R* d0 = A.llr + C.llr_off;
@@ -1256,7 +1285,6 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
     R* eb[2] = {A.ebar + C.ebar_off, A.ebar + C.ebar_off + K};
     R* dsc = A.dscale_part + (size_t)jb * H;
     const uint32_t lo = L.q * L.S, hi = lo + L.S;
-    const BlockLds<R> X = block_lds(reinterpret_cast<R*>(diff_lds), J);
     {
       R s1 = 0, s2 = 0;
       if (L.active) post_adjoint(A, L, sv, gp, g0, g1, g2, eb[(H - 1) & 1u], s1, s2);
@@ -1279,10 +1307,8 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
       if (A.use_lds) {
         stage_fill(A, J, h, X);
         __syncthreads();
-        if (L.active) {
-          tail_inputs(h, d0, d1, d2, K, ts, tp);
-          stage_from_lds(L, Y, h, X, blk);
-        }
+        stage_gather(J, h, X);
+        if (L.active) tail_inputs(h, d0, d1, d2, K, ts, tp);
         __syncthreads();
         eb_fill(A, J, h, X);
         __syncthreads();
@@ -1330,16 +1356,15 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
         // the alpha-adjoint sweep is the first writer of local steps below
         // g + S; the beta-adjoint sweep adds to those above
         for (uint32_t j = L.g + L.S; j < Y.P; ++j) L.at(Y.dsa, j) = L.at(Y.dpr, j) = 0;
-        // alpha at every 8th local step only (checkpoints in the al field,
-        // al[c] = alpha at local 8c); the adjoint sweeps recompute each
-        // sub-window's metrics into registers, which keeps the wave's scratch
-        // small enough to stay in the caches
-        alpha_ckpt_pass(L, Y);
         // beta over the window's warm-up above the own steps (local 2g + S - 1
-        // down to g + S); the own steps' beta runs inside the alpha-adjoint
-        // sweep, an independent chain beside the adjoint recursion.
-        // Checkpoints for the beta-adjoint sweep: beta at local g + i for i a
-        // multiple of 8 or i = S + g, at be[(i + 7) / 8]
+        // down to g + S), run beside the alpha checkpoint pass (alpha at every
+        // 8th local step only, al[c] = alpha at local 8c; the adjoint sweeps
+        // recompute each sub-window's metrics into registers, which keeps the
+        // wave's scratch small enough to stay in the caches).  The own steps'
+        // beta runs inside the alpha-adjoint sweep, an independent chain
+        // beside the adjoint recursion.  Checkpoints for the beta-adjoint
+        // sweep: beta at local g + i for i a multiple of 8 or i = S + g, at
+        // be[(i + 7) / 8]
         R bcur[8];
         if (lo + L.S + L.g >= K) {
           tail_beta(ts, tp, bcur);
@@ -1354,7 +1379,7 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
             for (int s = 0; s < 8; ++s) L.at(Y.be + s * bst, (i + 7) / 8) = b[s];
           }
         };
-        beta_warmup(L, Y, bcur, beta_ck);
+        alpha_ckpt_beta_warm_pass(L, Y, bcur, beta_ck);
         alpha_adjoint_sweep(L, Y, bcur, beta_ck);
         R bb[8];
         beta_adjoint_sweep(L, Y, bb);

@@ -37,9 +37,10 @@ struct DiffArgs {
   R* scratch;      // per lane, slot_elems each
   uint32_t slot_elems;
   uint32_t max_seg, max_warm;
-  // 1: every job's blocks fit kDiffLdsArrays padded block arrays in the
+  // 1: every job's blocks fit their LDS copies (diff_lds_elems) in the
   // dynamic LDS of its workgroup (staging and the adjoint combine then go
-  // through LDS with coalesced global accesses)
+  // through LDS with coalesced global accesses, and the recursion passes
+  // read their inputs from LDS)
   int32_t use_lds;
 };
 
@@ -48,5 +49,16 @@ struct DiffArgs {
 // read positions a multiple of 32 apart use different banks).
 constexpr uint32_t kDiffLdsArrays = 4;
 __host__ __device__ constexpr uint32_t diff_lds_stride(uint32_t k) { return k + (k >> 5) + 1; }
+// Staged inputs of one half per block, read by the recursion passes: pairs
+// (sys + la, parity) by position p = t + g in the half's order (the lower
+// half's sys + la already gathered through the interleaver), g zero pairs
+// either side of the block, pair p at p + p / 32; the blocks' pair arrays
+// follow arrays 0 and 1 (they overlap arrays 2 and 3, which only the
+// adjoint's combine uses, after the passes).
+__host__ __device__ constexpr uint32_t diff_stage_pairs(uint32_t k, uint32_t g) { return diff_lds_stride(k + 2 * g); }
+// LDS elements of a job of nblk blocks of size k with warm-up g
+__host__ __device__ constexpr uint32_t diff_lds_elems(uint32_t k, uint32_t g, uint32_t nblk) {
+  return nblk * (2 * diff_lds_stride(k) + 2 * diff_stage_pairs(k, g));
+}
 
 }  // namespace tcdev
