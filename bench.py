@@ -416,7 +416,22 @@ def diff_leg(T, torch, train_steps):
     ber_w = ber(x_te, y_te)
     log(f"diff: fwd {fwd_ms:.2f} ms bwd {bwd_ms:.2f} ms; train {train_steps} steps {train_s:.1f} s; "
         f"BER plain {ber_plain[0]:.2e} weighted {ber_w[0]:.2e}")
+    # VALU busy, memory-wait share and HBM-side traffic of the two kernels at this workload, from
+    # their rocprofv3 counter runs (copy-calibrated), recorded in profiles/
+    prof = os.path.join(ROOT, "profiles", "diff_decoder_k1024_x2048.md")
+    counters = {}
+    if os.path.exists(prof):
+        keys = ("valu_busy_fwd", "valu_busy_bwd", "wait_frac_fwd", "wait_frac_bwd", "traffic_bytes_fwd",
+                "traffic_bytes_bwd")
+        for line in open(prof):
+            for key in keys:
+                if line.startswith(key + ":"):
+                    counters[key] = float(line.split(":")[1])
+        counters["from"] = "profiles/diff_decoder_k1024_x2048.md"
     return {"workload": "2048 code blocks K=1024, BPSK/AWGN 0.5 dB, 6 iterations, 12 extrinsic scales, BCE",
+            "timing": "fwd: decoder forward + BCE; bwd: loss.backward() (adjoint kernel, scale reduction, BCE "
+                      "gradient), HIP events on the launch stream",
+            "counters": counters,
             "fwd_ms": fwd_ms, "bwd_ms": bwd_ms, "fwd_gbit_s": n * k / fwd_ms / 1e6, "bwd_gbit_s": n * k / bwd_ms / 1e6,
             "adam_steps": train_steps, "train_s": train_s, "loss_every_50": losses,
             "scales": [round(float(v), 4) for v in dec.scales.detach().cpu()],
