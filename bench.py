@@ -42,9 +42,11 @@ N_CB = 4096
 ITERS = 8
 EBN0 = [0.25 * i for i in range(7)]
 # Algorithmic work per info bit per iteration in the reference's pairwise
-# Log-MAP form: 384 simple ops + 120 transcendental ops weighted x4 = 864
-# VALU lane-slots; HBM bytes per block: 12(K+4) in + 4K out.
-VALU_SLOTS_PER_BIT_ITER = 864
+# Log-MAP form: 384 simple ops + 120 transcendental ops, each transcendental
+# priced at the two issue cycles v_exp_f32 / v_log_f32 take on gfx950
+# (SQ_ACTIVE_INST_VALU - SQ_INSTS_VALU of the counter runs) = 624 VALU
+# lane-slots; HBM bytes per block: 12(K+4) in + 4K out.
+VALU_SLOTS_PER_BIT_ITER = 384 + 120 * 2
 VALU_PEAK = 256 * 64 * 2.4e9  # lane-slots/s: 256 CUs x 64 lanes x 2.4 GHz, unpacked FP32
 HBM_PEAK_GBS = 8000.0
 
@@ -338,7 +340,7 @@ def mixed_leg(T, torch, geo, n_cb=65536, seed=41):
     # algorithmic HBM bytes: two received LLR sets read, posteriors written and read (TB
     # assembly), payload bits written
     hbm = 2 * 4 * int(es.sum()) + 2 * 4 * kb + plan.bits_total
-    valu = kb * mean_it * 864 / (ms / 1e3)
+    valu = kb * mean_it * VALU_SLOTS_PER_BIT_ITER / (ms / 1e3)
     # plain path on 64 blocks: one launch per block on the combined decoder input
     der = T.DeratePlan(plan.ks[:64], plan.es[:64])
     soft = torch.empty(der.llr_floats, dtype=torch.float32, device="cuda")
@@ -587,8 +589,9 @@ def main():
                 "frac": achieved / VALU_PEAK, "traffic": traffic,
                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
                                 "profiles/window_decode_counters_k6144_x4096.md)",
-                "note": "VALU lane-slots in the reference's pairwise Log-MAP form (864/bit/iteration, "
-                        "transcendentals x4) per second; no MFMA work exists in this path",
+                "note": "VALU lane-slots in the reference's pairwise Log-MAP form (624/bit/iteration: 384 "
+                        "simple + 120 transcendental at 2 issue cycles) per second; no MFMA work exists in "
+                        "this path",
                 "hbm": {"achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": hbm_achieved / HBM_PEAK_GBS}}
     if counters:
