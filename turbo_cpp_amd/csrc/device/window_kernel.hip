@@ -69,7 +69,7 @@ __device__ __forceinline__ float dpp_from_next_lane(float v) {  // wave_shl:1
 // SIMD), keeps their progress even: 4096 blocks of K=6144 decode 6% faster,
 // where the tail of the queue was 9% of the launch.  Called at every chunk of
 // both recursion passes (set in one pass only, a priority would hold through
-// the other).
+// the other).  The log-domain decoder only (D::kShareSimd).
 __device__ __forceinline__ void share_simd() {
   const uint32_t slot = __builtin_amdgcn_s_getreg((3 << 11) | 4) & 1u;  // HW_ID, wave slot bits [3:0]
   const uint32_t t = (uint32_t)(__builtin_amdgcn_s_memrealtime() >> 14);
@@ -104,6 +104,7 @@ struct LogDomain {
   };
   static constexpr float kUniform = 0.0f;
   static constexpr float kZero = kNeg;
+  static constexpr bool kShareSimd = true;  // alternate priority with the SIMD's other wave
   __device__ static __forceinline__ float from_log(float v) { return v * kLog2e; }
   // the staged parity LLR, as the branch metrics take it
   __device__ static __forceinline__ float stage_par(float par) { return par; }
@@ -180,6 +181,8 @@ struct LinDomain {
   };
   static constexpr float kUniform = 1.0f;
   static constexpr float kZero = 0.0f;
+  // measured slower with the alternating priority (3.23 -> 3.33 ms on 4096 x 6144)
+  static constexpr bool kShareSimd = false;
   __device__ static __forceinline__ float from_log(float v) { return hw_exp2(v * kLog2e); }
   // The parity weight depends on the channel only, so it is staged once per
   // job: e^-|par| (y = par/2, y + y = par exactly) with the sign bit set
@@ -399,7 +402,7 @@ __device__ __forceinline__ void beta_pass(const Lane& L, const _Float16* __restr
     load_chunk_lds<H, true, false>(L, ex, qtab, qq, by, c0, hi - c0, nxt, nullptr);
   }
   for (int c = (int)n - 1; c >= 0; --c) {
-    share_simd();
+    if constexpr (D::kShareSimd) share_simd();
     const uint32_t clo = lo + c * kSubWin;
     const uint32_t len = min((uint32_t)kSubWin, hi - clo);
     const Chunk cur = nxt;
@@ -587,7 +590,7 @@ __device__ __forceinline__ void siso_half(const Lane& L, _Float16* __restrict__ 
   load_chunk(sc, L.le, 0, min((uint32_t)kSubWin, S), nxt);
   load8(ck, nck);
   for (uint32_t w = 0; w < L.nW; ++w) {
-    share_simd();
+    if constexpr (D::kShareSimd) share_simd();
     const uint32_t t0 = w * kSubWin;
     const uint32_t len = min((uint32_t)kSubWin, S - t0);
     Chunk cur;
