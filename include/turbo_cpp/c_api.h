@@ -211,6 +211,23 @@ tc_status tc_tb_plan_soft_combine(tc_tb_plan* plan, const float* rx_dev, int32_t
  * combine retransmissions; accumulate != 0 is then rejected.  rx NULL and
  * tc_tb_plan_soft_combine keep working either way. */
 tc_status tc_tb_plan_set_hold(tc_tb_plan* plan, int32_t hold);
+/* Where the window decoders de-rate match received transmissions (the exact
+ * decoder always uses the second form):
+ *   TC_STAGING_FUSED    - inside the decoder's input staging: one launch, the
+ *                         decoder gathers each LLR through the sub-block
+ *                         interleaver;
+ *   TC_STAGING_SEPARATE - one de-rate kernel per transmission into the plan's
+ *                         decoder input (coalesced, LDS-staged), then the
+ *                         decode of that input;
+ *   TC_STAGING_AUTO     - the default, the separate form (measured as fast or
+ *                         faster on every workload tried: one full-rate
+ *                         transmission, punctured ones, HARQ pairs).
+ * Both give the same results bit for bit.  No counterpart in the reference
+ * (whose de-rate matcher is a host function, lte_turbo.cpp:280-325). */
+#define TC_STAGING_AUTO 0
+#define TC_STAGING_FUSED 1
+#define TC_STAGING_SEPARATE 2
+tc_status tc_tb_plan_set_staging(tc_tb_plan* plan, int32_t mode);
 /* HARQ in one call: the held decoder input becomes the combination of n_tx
  * (1..4) transmissions (rx_dev[t] with rv[t], rv NULL = the descriptors'),
  * then decode and assembly as tc_tb_plan_decode. */

@@ -542,3 +542,56 @@ def test_tb_plan_without_held_input():
             torch.cuda.synchronize()
             assert torch.equal(post2, post)
     assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])
+def test_tb_staging_modes_agree(decoder):
+    """De-rate matching in the window decoder's input staging
+    (STAGING_FUSED), as de-rate kernels before the decode (STAGING_SEPARATE)
+    and the default (STAGING_AUTO) give the same payload bits, CRC flags,
+    iteration counts and posteriors, bit for bit: one transmission, two
+    combined in one call, and a transmission accumulated onto the held
+    input.  The batch mixes punctured and repeated blocks (E at 0.5-1.3 of
+    the circular buffer)."""
+    import torch
+    tbs, rx, _ = _tb_batch(61, n_tb=10)
+    rng = np.random.default_rng(7)
+    r0 = torch.from_numpy(rx).cuda()
+    r2 = torch.from_numpy(rx[rng.permutation(rx.size)]).cuda()
+    s = torch.cuda.current_stream().cuda_stream
+    res = []
+    for mode in (T.STAGING_FUSED, T.STAGING_SEPARATE, T.STAGING_AUTO):
+        plan = T.TbPlan(tbs, decoder, max_iterations=6, early_stop=True)
+        plan.set_staging(mode)
+        outs = []
+        for case in ("one", "multi", "accumulate"):
+            bits = torch.zeros(plan.bits_total, dtype=torch.uint8, device="cuda")
+            post = torch.zeros(plan.post_floats, dtype=torch.float32, device="cuda")
+            ok = torch.zeros(plan.n_tb, dtype=torch.int32, device="cuda")
+            cb_ok = torch.zeros(plan.n_cb, dtype=torch.int32, device="cuda")
+            its = torch.zeros(plan.n_tb, dtype=torch.int32, device="cuda")
+            args = dict(bits_ptr=bits.data_ptr(), tb_crc_ptr=ok.data_ptr(), cb_crc_ptr=cb_ok.data_ptr(),
+                        iters_ptr=its.data_ptr(), post_ptr=post.data_ptr(), stream=s)
+            _say("staging", mode, "decoder", decoder, case)
+            if case == "one":
+                plan.decode(r0.data_ptr(), **args)
+            elif case == "multi":
+                plan.decode_multi([r0.data_ptr(), r2.data_ptr()], [0, 2], **args)
+            else:
+                plan.decode(r0.data_ptr(), rv=1, **args)
+                plan.decode(r2.data_ptr(), rv=3, accumulate=True, **args)
+            torch.cuda.synchronize()
+            outs.append((bits.cpu(), ok.cpu(), cb_ok.cpu(), its.cpu(), post.cpu()))
+        res.append(outs)
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            for x, y in zip(a, b):
+                assert torch.equal(x, y)
+
+
+def test_tb_staging_mode_errors():
+    tbs, _, _ = _tb_batch(62, n_tb=2)
+    plan = T.TbPlan(tbs)
+    for bad in (-1, 3):
+        with pytest.raises(ValueError):
+            plan.set_staging(bad)

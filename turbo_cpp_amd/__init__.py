@@ -26,6 +26,7 @@ LIB_PATH = os.path.join(_HERE, "libturbo_cpp_amd.so")
 
 TC_OK, TC_ERR_INVALID_ARGUMENT, TC_ERR_OUT_OF_RANGE, TC_ERR_RUNTIME = 0, 1, 2, 3
 DECODER_EXACT, DECODER_WINDOW_LOG, DECODER_WINDOW_LIN = 0, 1, 2
+STAGING_AUTO, STAGING_FUSED, STAGING_SEPARATE = 0, 1, 2  # tc_tb_plan_set_staging
 CRC24A, CRC24B, CRC16, CRC8 = 0, 1, 2, 3
 
 
@@ -116,6 +117,7 @@ def lib() -> C.CDLL:
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.tc_tb_plan_decode_multi.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p] + [C.c_void_p] * 6
         L.tc_tb_plan_set_hold.argtypes = [C.c_void_p, C.c_int32]
+        L.tc_tb_plan_set_staging.argtypes = [C.c_void_p, C.c_int32]
         L.tc_tb_plan_soft_combine.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
         L.tc_tb_plan_encode.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.tc_encode_transport_blocks_host.argtypes = [C.POINTER(TcTbDesc), C.c_size_t, _u8p, _u8p, C.c_void_p]
@@ -645,6 +647,13 @@ class TbPlan:
     def set_hold(self, hold: bool):
         """Whether decodes from received LLRs keep the decoder input for HARQ (default True)."""
         _check(lib().tc_tb_plan_set_hold(self._h, int(hold)))
+
+    def set_staging(self, mode: int):
+        """Where the window decoders de-rate match received transmissions:
+        STAGING_FUSED (in the decoder's input staging), STAGING_SEPARATE (a
+        de-rate kernel per transmission first) or STAGING_AUTO (the default,
+        the separate form).  Same results."""
+        _check(lib().tc_tb_plan_set_staging(self._h, int(mode)))
 
     def soft_combine(self, rx_ptr: int, rv: int = -1, accumulate: bool = False, stream: int = 0):
         """De-rate match one transmission into the held decoder input (no decode)."""

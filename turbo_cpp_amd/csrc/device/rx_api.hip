@@ -24,6 +24,7 @@ __global__ void derate_soft_kernel(const DerateCb*, const uint16_t*, const float
 __global__ void tb_assemble_kernel(const TbInfo*, const TbCb*, const float*, const uint32_t*, uint32_t,
                                    const uint32_t*, const uint32_t*, uint8_t*, int32_t*, int32_t*, uint32_t*);
 __global__ void tb_acc_zero_kernel(uint32_t*, uint32_t);
+__global__ void tb_finalize_kernel(const uint32_t*, uint32_t, uint32_t, int32_t*, uint32_t*);
 __global__ void tb_crc24a_kernel(const TbInfo*, const uint8_t*, const uint32_t*, uint32_t*);
 __global__ void cb_encode_kernel(const TxCb*, const TbInfo*, const uint8_t*, const uint32_t*, const uint32_t*,
                                  uint8_t*, uint8_t*);
@@ -112,6 +113,7 @@ struct tc_tb_plan {
   uint32_t* d_pow24b = nullptr;
   float* d_soft = nullptr;  // decoder input; the HARQ soft buffer between calls
   bool hold = true;         // decodes from received LLRs refresh d_soft
+  int32_t staging = TC_STAGING_AUTO;  // window decoders: where de-rate matching runs
   float* d_post = nullptr;  // posteriors when the caller passes none
   uint32_t* d_cb_iters = nullptr;
   // transmit side
@@ -302,6 +304,17 @@ void run_tb(tc_tb_plan& p, uint32_t n_tx, const float* const* rx, const int32_t*
   if (accumulate && !p.hold)
     throw Failure{TC_ERR_INVALID_ARGUMENT, "the plan does not hold its decoder input (tc_tb_plan_set_hold)"};
   const bool exact = p.exact;
+  const bool separate = n_tx && p.staging != TC_STAGING_FUSED;
+  if (!exact && separate) {
+    // window decoders, de-rate kernels first: each transmission into d_soft (added in order onto
+    // the held input or the transmissions before it, the fused staging's order), then the decode
+    // of d_soft with the TB assembly in its epilogue
+    for (uint32_t t = 0; t < n_tx; ++t) {
+      if (!rx[t]) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+      run_derate(*p.derate, rx[t], rv ? rv[t] : -1, (t > 0 || accumulate) ? 1 : 0, p.d_soft, s);
+    }
+    n_tx = 0;
+  }
   if (!exact) {
     // window decoders: de-rate matching in the input staging, TB assembly in the epilogue
     if (p.n_tb > 0x7FFFFFFFull / 3) throw Failure{TC_ERR_INVALID_ARGUMENT, "too many transport blocks"};
@@ -328,6 +341,11 @@ void run_tb(tc_tb_plan& p, uint32_t n_tx, const float* const* rx, const int32_t*
     if (!tchost::launch_window(p.decoder, n_tx ? nullptr : p.d_soft, post_buf, nullptr, p.d_cb_iters, s,
                                n_tx ? &x : nullptr, &tb))
       throw Failure{TC_ERR_RUNTIME, "window decoder expected"};
+    if (tb_ok || iters) {
+      hipLaunchKernelGGL(tcdev::tb_finalize_kernel, dim3(static_cast<uint32_t>((p.n_tb + 255) / 256)), dim3(256), 0, s,
+                         p.d_acc, static_cast<uint32_t>(p.n_tb), p.max_iterations, tb_ok, iters);
+      hip_check(hipGetLastError(), "transport block verdict launch");
+    }
     return;
   }
   if (n_tx)
@@ -431,6 +449,15 @@ tc_status tc_tb_plan_set_hold(tc_tb_plan* plan, int32_t hold) {
   return guarded([&] {
     if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
     plan->hold = hold != 0;
+  });
+}
+
+tc_status tc_tb_plan_set_staging(tc_tb_plan* plan, int32_t mode) {
+  return guarded([&] {
+    if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    if (mode < TC_STAGING_AUTO || mode > TC_STAGING_SEPARATE)
+      throw Failure{TC_ERR_INVALID_ARGUMENT, "staging must be TC_STAGING_AUTO, _FUSED or _SEPARATE"};
+    plan->staging = mode;
   });
 }
 

@@ -145,4 +145,16 @@ __global__ void tb_acc_zero_kernel(uint32_t* acc, uint32_t n) {
   if (i < n) acc[i] = 0u;
 }
 
+// TB verdicts of the decoder-epilogue assembly, after the decode: every block
+// of TB i has XORed its CRC24A share (or, for a code-block-CRC descriptor,
+// its CRC24B verdict) into acc[3i] and max-ed its iteration count into
+// acc[3i + 2].
+__global__ void tb_finalize_kernel(const uint32_t* acc, uint32_t n_tb, uint32_t max_iterations, int32_t* tb_ok,
+                                   uint32_t* tb_iters) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_tb) return;
+  if (tb_ok) tb_ok[i] = acc[3 * i] == 0u ? 1 : 0;
+  if (tb_iters) tb_iters[i] = max_iterations ? acc[3 * i + 2] : 0u;
+}
+
 }  // namespace tcdev

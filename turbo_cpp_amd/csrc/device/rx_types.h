@@ -65,7 +65,7 @@ struct TbOut {
   int32_t* tb_ok;
   int32_t* cb_ok;
   uint32_t* tb_iters;
-  uint32_t* acc;  // per TB: CRC24A remainder (XOR), blocks done, max iterations
+  uint32_t* acc;  // per TB: CRC24A remainder (XOR), unused, max iterations (tb_finalize_kernel reads them)
 };
 // D^m mod G(D) tables: CRC24B for m < 6144 (a code block), CRC24A up to the
 // longest transport block of a plan.

@@ -768,15 +768,11 @@ __device__ __forceinline__ void tb_epilogue(const WinArgs& A, uint32_t cb, const
     if (T.cb_ok) T.cb_ok[cb] = cb_pass ? 1 : 0;
     uint32_t* acc = T.acc + 3 * m.tb;
     if (t.cb_crc) tbx = cb_pass ? 0u : 1u;  // a code-block-CRC descriptor is checked by its CRC24B
+    // order-free device-scope atomics; tb_finalize_kernel reads them after this launch.  No
+    // fence: on this GPU a device-scope fence writes back and invalidates the XCD's L2
+    // (buffer_wbl2 / buffer_inv), which at two per block cost 8 ms on 65536 blocks
     atomicXor(acc, tbx);
     atomicMax(acc + 2, A.iters ? A.iters[cb] : 0u);
-    __threadfence();
-    if (atomicAdd(acc + 1, 1u) == t.c - 1) {  // last block of this TB
-      __threadfence();
-      const uint32_t rem = atomicOr(acc, 0u), itmax = atomicOr(acc + 2, 0u);
-      if (T.tb_ok) T.tb_ok[m.tb] = rem == 0 ? 1 : 0;
-      if (T.tb_iters) T.tb_iters[m.tb] = T.max_iterations ? itmax : 0u;
-    }
   }
   (void)len;
 }
