@@ -206,10 +206,11 @@ tc_status tc_encode_transport_blocks_host(const tc_tb_desc* tbs, size_t n_tb, co
 tc_status tc_tb_plan_soft_combine(tc_tb_plan* plan, const float* rx_dev, int32_t rv, int32_t accumulate,
                                   void* hip_stream);
 /* hold != 0 (the default): decodes from received LLRs leave the combined
- * decoder input in the plan for later accumulation.  hold == 0 skips that
- * write (one pass over the decoder input fewer) for receivers that do not
- * combine retransmissions; accumulate != 0 is then rejected.  rx NULL and
- * tc_tb_plan_soft_combine keep working either way. */
+ * decoder input in the plan for later accumulation.  hold == 0 is for
+ * receivers that do not combine retransmissions: accumulate != 0 is then
+ * rejected, and with TC_STAGING_FUSED the decode skips writing the decoder
+ * input (one pass over it fewer; the default staging writes it either way).
+ * rx NULL and tc_tb_plan_soft_combine keep working either way. */
 tc_status tc_tb_plan_set_hold(tc_tb_plan* plan, int32_t hold);
 /* Where the window decoders de-rate match received transmissions (the exact
  * decoder always uses the second form):
