@@ -106,10 +106,11 @@ struct LogDomain {
   static constexpr float kZero = kNeg;
   static constexpr bool kShareSimd = true;  // alternate priority with the SIMD's other wave
   __device__ static __forceinline__ float from_log(float v) { return v * kLog2e; }
-  // the staged parity LLR, as the branch metrics take it
-  __device__ static __forceinline__ float stage_par(float par) { return par; }
-  __device__ static __forceinline__ Br branch(float sa, float par) {
-    const float x = sa * kHalfLog2e, y = par * kHalfLog2e;
+  // the staged parity LLR, as the branch metrics take it: scaled once per job
+  // (the same product the branch metric would form at every use)
+  __device__ static __forceinline__ float stage_par(float par) { return par * kHalfLog2e; }
+  __device__ static __forceinline__ Br branch(float sa, float y) {
+    const float x = sa * kHalfLog2e;
     return {x + y, x - y};
   }
   __device__ static __forceinline__ float maxstar(float a, float b) {
