@@ -993,12 +993,14 @@ __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layo
       if (jj < len) {
         if (j < L.g + L.S) {
           load8(L, Y.bo, Y.S, j - L.g, bor[jj]);
+          dsr[jj] = L.at(Y.dsa, j);
+          dpr[jj] = L.at(Y.dpr, j);
         } else {
+          // above the own steps this sweep is the only writer: its sums start at zero
 #pragma unroll
           for (int s = 0; s < 8; ++s) bor[jj][s] = 0;
+          dsr[jj] = dpr[jj] = R(0);
         }
-        dsr[jj] = L.at(Y.dsa, j);
-        dpr[jj] = L.at(Y.dpr, j);
       }
     }
     R breg[8][8];
@@ -1354,8 +1356,9 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
           }
         }
         // the alpha-adjoint sweep is the first writer of local steps below
-        // g + S; the beta-adjoint sweep adds to those above
-        for (uint32_t j = L.g + L.S; j < Y.P; ++j) L.at(Y.dsa, j) = L.at(Y.dpr, j) = 0;
+        // g + S, the beta-adjoint sweep of those up to 2g + S; zeros beyond
+        // (a plan's longer windows) keep the combine's reads defined
+        for (uint32_t j = 2 * L.g + L.S; j < Y.P; ++j) L.at(Y.dsa, j) = L.at(Y.dpr, j) = 0;
         // beta over the window's warm-up above the own steps (local 2g + S - 1
         // down to g + S), run beside the alpha checkpoint pass (alpha at every
         // 8th local step only, al[c] = alpha at local 8c; the adjoint sweeps
