@@ -43,13 +43,18 @@ constexpr float kInf = __builtin_inff();
 
 __device__ __forceinline__ bool fin(float v) { return __builtin_isfinite(v); }
 
-// max* with the reference's infinity rules (lte_turbo.cpp:380-390).
+// max* with the reference's infinity rules (lte_turbo.cpp:380-390): y when
+// x is not finite, else x when y is not finite, else hi + log1pf(expf(lo -
+// hi)).  Written without branches (the Jacobian term is evaluated in every
+// case and selected away when an input is not finite) so that the alpha and
+// beta steps of a loop iteration, two independent chains of ~80 dependent
+// instructions each, can interleave; the selected value is the reference's
+// in every case.
 __device__ __forceinline__ float jac(float x, float y) {
-  if (!fin(x)) return y;
-  if (!fin(y)) return x;
   const float hi = (x < y) ? y : x;
   const float lo = (y < x) ? y : x;
-  return hi + log1pf(expf(lo - hi));
+  const float r = hi + log1pf(expf(lo - hi));
+  return !fin(x) ? y : (!fin(y) ? x : r);
 }
 
 __device__ __forceinline__ float bm(unsigned u, unsigned p, float sa, float lp) {
