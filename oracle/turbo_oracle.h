@@ -31,7 +31,7 @@
  *   or_assemble_tb      equality with the reference's decode_transport_block
  *                       (golden vectors with corrupted code blocks: bits,
  *                       per-block and TB CRC flags).
- *   or_decode_window    the decoder this build runs on the GPU in windowed
+ *   or_decode_window    the decoder the library runs on the GPU in windowed
  *                       mode; pinned by equality with or_decode when there is
  *                       one window per code block, and by BER.
  *   or_round_half       equality with numpy's float32 -> float16 -> float32
