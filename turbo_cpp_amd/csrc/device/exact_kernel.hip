@@ -41,6 +41,21 @@ namespace {
 
 constexpr float kInf = __builtin_inff();
 
+// Two waves share each SIMD; the issue arbitration favours the older one, so
+// it finishes its blocks first and leaves the other to run alone, which for
+// this latency-bound recursion halves the SIMD's throughput.  The two
+// alternate priority on a global slice of 16384 ticks of the 100 MHz
+// reference clock, by the parity of their wave slots (0 and 1 of the SIMD).
+__device__ __forceinline__ void share_simd() {
+  const uint32_t slot = __builtin_amdgcn_s_getreg((3 << 11) | 4) & 1u;  // HW_ID, wave slot bits [3:0]
+  const uint32_t t = (uint32_t)(__builtin_amdgcn_s_memrealtime() >> 14);
+  if ((t ^ slot) & 1u) {
+    __builtin_amdgcn_s_setprio(1);
+  } else {
+    __builtin_amdgcn_s_setprio(0);
+  }
+}
+
 __device__ __forceinline__ bool fin(float v) { return __builtin_isfinite(v); }
 
 // max* with the reference's infinity rules (lte_turbo.cpp:380-390): y when
@@ -156,6 +171,7 @@ __device__ void siso(unsigned r, uint32_t gbase, const float* sys, const float* 
   };
   load_chunk(0, ca_s, ca_p, cb_s, cb_p);
   for (uint32_t i0 = 0; i0 < k; i0 += kExactLanes) {
+    share_simd();
     if (i0 + kExactLanes < k) load_chunk(i0 + kExactLanes, na_s, na_p, nb_s, nb_p);
     const uint32_t n = min(kExactLanes, k - i0);
     for (uint32_t j = 0; j < n; ++j) {
