@@ -81,6 +81,9 @@ def lib():
                                        C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, _f32p,
                                        C.POINTER(C.c_int), C.POINTER(C.c_uint32)]
         L.or_decode_window.restype = C.c_int
+        L.or_decode_exact_seg.argtypes = [_f32p, _f32p, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
+                                          C.c_uint32, _f32p, C.POINTER(C.c_int), C.POINTER(C.c_uint32), C.c_void_p]
+        L.or_decode_exact_seg.restype = C.c_int
         L.or_round_half.argtypes = [C.c_float]
         L.or_round_half.restype = C.c_float
         L.or_brute_app.argtypes = [_f32p, _f32p, _f32p, C.c_uint32, _f32p, _f32p, _f64p]
@@ -258,6 +261,19 @@ def decode_window(d0, d1, d2, k, seg_len, warmup=0, linear=False, filler=0, max_
     if lib().or_decode_window(_f32(d0), _f32(d1), _f32(d2), k, filler, max_iterations, int(early_stop),
                               seg_len, warmup, int(linear), ex_bits, post, C.byref(ok), C.byref(it)) != 0:
         raise ValueError("bad K / segment length / extrinsic width")
+    return post, bool(ok.value), it.value
+
+
+def decode_exact_seg(d0, d1, d2, k, seg_len, filler=0, max_iterations=8, early_stop=False, work=None):
+    """Segment-parallel decode with exact boundaries (or_decode_exact_seg).
+    work: optional uint64 array of 4, accumulated (halves, fix-up rounds,
+    chunks recomputed by the slowest segment per round, chunks per pass)."""
+    post = np.zeros(k, np.float32)
+    ok, it = C.c_int(0), C.c_uint32(0)
+    w = None if work is None else work.ctypes.data
+    if lib().or_decode_exact_seg(_f32(d0), _f32(d1), _f32(d2), k, filler, max_iterations, int(early_stop),
+                                 seg_len, post, C.byref(ok), C.byref(it), w) != 0:
+        raise ValueError("bad K / segment length")
     return post, bool(ok.value), it.value
 
 

@@ -592,6 +592,243 @@ int or_decode_cb(const float* d0, const float* d1, const float* d2, uint32_t k,
   return 0;
 }
 
+/* ------------------------------- exact segment-parallel decoder (GPU path) ---
+ * Not a reference function: the schedule the library's exact GPU decoder
+ * runs, stated serially.  Every SISO half computes the reference's recursions
+ * (lte_turbo.cpp:410-486, same operations in the same order) split into
+ * M = K / seg_len segments that run side by side:
+ *   1. speculation: each segment runs beta down and alpha up its own steps
+ *      from a guess of its boundary metrics (the exact values of the same
+ *      boundary in the previous half of the same kind, uniform at first),
+ *      keeping the metrics at every kChunk-th step (checkpoints);
+ *   2. fix-up rounds: each segment takes its neighbour's boundary value as it
+ *      now stands (segment 0's alpha_0 and the last segment's tail beta are
+ *      exact) and, if it differs from the value it started from, recomputes
+ *      chunk by chunk until the recomputed metrics equal the stored
+ *      checkpoint bit for bit: from there on the recursion is the same
+ *      deterministic function of the same inputs, so nothing further changes.
+ *      A segment that never meets its checkpoints hands a new boundary value
+ *      on; rounds repeat until no boundary value changes;
+ *   3. the extrinsics, per chunk, from the checkpoints: beta of the chunk
+ *      recomputed downwards, alpha stepped upwards, the reference's 16-term
+ *      LLR accumulation per step.
+ * The metrics every step ends with are therefore those of the full-length
+ * serial recursion, and the posteriors equal or_decode_cb's bit for bit
+ * (test_exact_segmented_equals_serial).  `work` (may be NULL) accumulates
+ * [0] halves, [1] fix-up rounds, [2] chunks recomputed counting the slowest
+ * segment of each round (the cost on a GPU whose lanes run in lockstep),
+ * [3] chunks of one full pass (for scale). */
+enum { kChunk = 8 };
+
+static void exact_alpha_step(const float* a0, float sa, float par, float* a1) {
+  for (int i = 0; i < 8; ++i) a1[i] = -INFINITY;
+  for (unsigned s = 0; s < 8; ++s) {
+    if (!isfinite(a0[s])) continue;
+    for (unsigned u = 0; u < 2; ++u) {
+      unsigned p, n;
+      rsc_step(s, u, &p, &n);
+      float c = a0[s] + branch(u, p, sa, par);
+      a1[n] = isfinite(a1[n]) ? maxstar(a1[n], c) : c;
+    }
+  }
+  normalize8(a1);
+}
+
+static void exact_beta_step(const float* b1, float sa, float par, float* b0) {
+  for (int i = 0; i < 8; ++i) b0[i] = -INFINITY;
+  for (unsigned s = 0; s < 8; ++s) {
+    for (unsigned u = 0; u < 2; ++u) {
+      unsigned p, n;
+      rsc_step(s, u, &p, &n);
+      if (!isfinite(b1[n])) continue;
+      float c = b1[n] + branch(u, p, sa, par);
+      b0[s] = isfinite(b0[s]) ? maxstar(b0[s], c) : c;
+    }
+  }
+  normalize8(b0);
+}
+
+/* beta over chunk w of segment q (steps [lo, hi)), from b (beta at hi) down to beta at lo */
+static void exact_beta_chunk(const float* sys, const float* par, const float* ap, uint32_t lo, uint32_t hi,
+                             float* b) {
+  float nb[8];
+  for (uint32_t t = hi; t-- > lo;) {
+    exact_beta_step(b, sys[t] + ap[t], par[t], nb);
+    memcpy(b, nb, sizeof nb);
+  }
+}
+
+static void exact_alpha_chunk(const float* sys, const float* par, const float* ap, uint32_t lo, uint32_t hi,
+                              float* a) {
+  float na[8];
+  for (uint32_t t = lo; t < hi; ++t) {
+    exact_alpha_step(a, sys[t] + ap[t], par[t], na);
+    memcpy(a, na, sizeof na);
+  }
+}
+
+/* One SISO half, segment-parallel with exact boundaries.  nii: per segment
+ * [alpha at its first step | beta after its last step], in/out (the guesses
+ * in, the exact values out). */
+static void exact_siso_seg(const float* sys, const float* par, const float* ap, uint32_t k, uint32_t seg,
+                           const float* ts, const float* tp, float* nii, float* ext, uint64_t* work) {
+  const uint32_t m = k / seg, nw = (seg + kChunk - 1) / kChunk;
+  float* ack = (float*)malloc(sizeof(float) * 8 * (size_t)m * nw);
+  float* bck = (float*)malloc(sizeof(float) * 8 * (size_t)m * nw);
+  float* edge = (float*)malloc(sizeof(float) * 16 * (size_t)m);  /* [alpha at end | beta at start] */
+  float* prev = (float*)malloc(sizeof(float) * 16 * (size_t)m);
+  float tail[8], a0[8];
+  tail_beta(ts, tp, tail);
+  for (int i = 0; i < 8; ++i) a0[i] = i ? -INFINITY : 0.0f;
+#define CLO(w) ((w) * kChunk)
+#define CHI(w) (((w) + 1) * kChunk < seg ? ((w) + 1) * kChunk : seg)
+  /* 1. speculation */
+  for (uint32_t q = 0; q < m; ++q) {
+    const uint32_t base = q * seg;
+    float b[8], a[8];
+    memcpy(b, q == m - 1 ? tail : nii + 16 * q + 8, sizeof b);
+    for (uint32_t w = nw; w-- > 0;) {
+      memcpy(bck + 8 * ((size_t)q * nw + w), b, sizeof b);
+      exact_beta_chunk(sys, par, ap, base + CLO(w), base + CHI(w), b);
+    }
+    memcpy(edge + 16 * q + 8, b, sizeof b);
+    memcpy(a, q == 0 ? a0 : nii + 16 * q, sizeof a);
+    for (uint32_t w = 0; w < nw; ++w) {
+      memcpy(ack + 8 * ((size_t)q * nw + w), a, sizeof a);
+      exact_alpha_chunk(sys, par, ap, base + CLO(w), base + CHI(w), a);
+    }
+    memcpy(edge + 16 * q, a, sizeof a);
+  }
+  if (work) {
+    work[0] += 1;
+    work[3] += nw;
+  }
+  /* 2. fix-up rounds: every segment reads its neighbours' edges as the
+   * previous round left them (lanes in lockstep), beta and alpha alike */
+  for (;;) {
+    memcpy(prev, edge, sizeof(float) * 16 * (size_t)m);
+    int changed = 0;
+    uint32_t slowest = 0;
+    for (uint32_t q = 0; q < m; ++q) {
+      const uint32_t base = q * seg;
+      uint32_t done_b = 0, done_a = 0;
+      /* beta: new value after the last step = next segment's beta at its start */
+      float b[8], a[8];
+      memcpy(b, q == m - 1 ? tail : prev + 16 * (q + 1) + 8, sizeof b);
+      int met = 0;
+      for (uint32_t w = nw; w-- > 0;) {
+        float* ck = bck + 8 * ((size_t)q * nw + w);
+        if (!memcmp(ck, b, sizeof b)) {
+          met = 1;
+          break;
+        }
+        memcpy(ck, b, sizeof b);
+        exact_beta_chunk(sys, par, ap, base + CLO(w), base + CHI(w), b);
+        ++done_b;
+      }
+      if (!met && memcmp(edge + 16 * q + 8, b, sizeof b)) {
+        memcpy(edge + 16 * q + 8, b, sizeof b);
+        changed = 1;
+      }
+      memcpy(a, q == 0 ? a0 : prev + 16 * (q - 1), sizeof a);
+      met = 0;
+      for (uint32_t w = 0; w < nw; ++w) {
+        float* ck = ack + 8 * ((size_t)q * nw + w);
+        if (!memcmp(ck, a, sizeof a)) {
+          met = 1;
+          break;
+        }
+        memcpy(ck, a, sizeof a);
+        exact_alpha_chunk(sys, par, ap, base + CLO(w), base + CHI(w), a);
+        ++done_a;
+      }
+      if (!met && memcmp(edge + 16 * q, a, sizeof a)) {
+        memcpy(edge + 16 * q, a, sizeof a);
+        changed = 1;
+      }
+      const uint32_t d = done_a > done_b ? done_a : done_b;
+      if (d > slowest) slowest = d;
+    }
+    if (work) {
+      work[1] += 1;
+      work[2] += slowest;
+    }
+    if (!changed) break;
+  }
+  /* 3. extrinsics from the checkpoints */
+  for (uint32_t q = 0; q < m; ++q) {
+    const uint32_t base = q * seg;
+    for (uint32_t w = 0; w < nw; ++w) {
+      const uint32_t lo = base + CLO(w), hi = base + CHI(w);
+      float bs[kChunk + 1][8], a[8], na[8];
+      memcpy(bs[hi - lo], bck + 8 * ((size_t)q * nw + w), sizeof a);
+      for (uint32_t t = hi; t-- > lo;) exact_beta_step(bs[t - lo + 1], sys[t] + ap[t], par[t], bs[t - lo]);
+      memcpy(a, ack + 8 * ((size_t)q * nw + w), sizeof a);
+      for (uint32_t t = lo; t < hi; ++t) {
+        const float sa = sys[t] + ap[t];
+        float l[2] = {-INFINITY, -INFINITY};
+        for (unsigned s = 0; s < 8; ++s)
+          for (unsigned u = 0; u < 2; ++u) {
+            unsigned p, n;
+            rsc_step(s, u, &p, &n);
+            float mm = a[s] + branch(u, p, sa, par[t]) + bs[t - lo + 1][n];
+            l[u] = isfinite(l[u]) ? maxstar(l[u], mm) : mm;
+          }
+        ext[t] = (l[1] - l[0]) - sys[t] - ap[t];
+        exact_alpha_step(a, sa, par[t], na);
+        memcpy(a, na, sizeof a);
+      }
+    }
+    memcpy(nii + 16 * q, ack + 8 * ((size_t)q * nw), sizeof(float) * 8);
+    memcpy(nii + 16 * q + 8, bck + 8 * ((size_t)q * nw + nw - 1), sizeof(float) * 8);
+  }
+#undef CLO
+#undef CHI
+  free(prev);
+  free(edge);
+  free(bck);
+  free(ack);
+}
+
+int or_decode_exact_seg(const float* d0, const float* d1, const float* d2, uint32_t k, uint32_t filler,
+                        uint32_t max_iterations, int early_stop, uint32_t seg_len, float* post, int* crc_ok,
+                        uint32_t* iterations_used, uint64_t* work) {
+  if (or_qpp_index(k) < 0 || seg_len == 0 || k % seg_len != 0) return -1;
+  const uint32_t m = k / seg_len;
+  uint32_t* perm = (uint32_t*)malloc(sizeof(uint32_t) * k);
+  or_qpp_perm(k, perm);
+  float* buf = (float*)calloc(6 * (size_t)k, sizeof(float));
+  float *ap = buf, *e1 = buf + k, *e2 = buf + 2 * k, *si = buf + 3 * k, *ia = buf + 4 * k, *e2i = buf + 5 * k;
+  float* nii = (float*)calloc(32 * (size_t)m, sizeof(float)); /* upper | lower halves; uniform guesses */
+  uint8_t* bits = (uint8_t*)malloc(k);
+  float us[4], up[4], ls[4], lp[4];
+  tail_llrs(d0, d1, d2, k, us, up, ls, lp);
+  for (uint32_t i = 0; i < k; ++i) si[i] = d0[perm[i]];
+  for (uint32_t i = 0; i < k; ++i) post[i] = 0.0f;
+  int ok = 0;
+  for (uint32_t it = 0; it < max_iterations; ++it) {
+    exact_siso_seg(d0, d1, ap, k, seg_len, us, up, nii, e1, work);
+    for (uint32_t i = 0; i < k; ++i) ia[i] = e1[perm[i]];
+    exact_siso_seg(si, d2, ia, k, seg_len, ls, lp, nii + 16 * (size_t)m, e2i, work);
+    for (uint32_t j = 0; j < k; ++j) e2[perm[j]] = e2i[j];
+    for (uint32_t i = 0; i < k; ++i) {
+      ap[i] = e2[i];
+      post[i] = d0[i] + e1[i] + e2[i];
+    }
+    if (iterations_used) *iterations_used = it + 1;
+    if (early_stop) {
+      ok = cb_crc_passes(post, k, filler, bits);
+      if (ok) break;
+    }
+  }
+  if (crc_ok) *crc_ok = ok;
+  free(bits);
+  free(nii);
+  free(buf);
+  free(perm);
+  return 0;
+}
+
 /* ------------------------------------------- windowed decoder (GPU path) ---
  * The algorithm the batched HIP kernel runs; not a reference function.  The
  * code block is split into M = K / seg_len segments.  Each SISO half runs an

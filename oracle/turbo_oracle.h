@@ -102,6 +102,15 @@ int or_decode_window(const float* d0, const float* d1, const float* d2, uint32_t
                      uint32_t seg_len, uint32_t warmup, int linear_domain, uint32_t ex_bits,
                      float* posterior, int* crc_ok, uint32_t* iterations_used);
 
+/* Segment-parallel decode with exact segment boundaries (speculation and
+ * bit-exact fix-up, see turbo_oracle.c): the schedule of the library's exact
+ * GPU decoder.  Equals or_decode_cb bit for bit.  work (may be NULL, 4
+ * entries, accumulated): halves, fix-up rounds, chunks recomputed by the
+ * slowest segment of each round, chunks of one full pass. */
+int or_decode_exact_seg(const float* d0, const float* d1, const float* d2, uint32_t k, uint32_t filler,
+                        uint32_t max_iterations, int early_stop, uint32_t seg_len, float* posterior,
+                        int* crc_ok, uint32_t* iterations_used, uint64_t* work);
+
 /* x saturated at +-65504 and rounded to the nearest fp16 value (ties to
  * even, fp16 subnormals kept), returned as a float. */
 float or_round_half(float x);

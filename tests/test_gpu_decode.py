@@ -204,6 +204,24 @@ def test_reference_roundtrips_gpu(tbs, pattern):
             np.testing.assert_allclose(dec.posterior_llr, cpu.posterior_llr, atol=2e-5, rtol=1e-6)
 
 
+@pytest.mark.parametrize("backend", ["GpuStage1Exact", "GpuStage2Windowed"])
+def test_gpu_roundtrip_rv_above_3(backend):
+    """rv = 4: the reference takes k0 of any rv modulo the circular buffer
+    (lte_turbo.cpp:263-264, 294-295); the device de-rate tables cover rv 0..3,
+    so such TBs take the host de-rate + batched decode path and still decode,
+    matching the CPU decoder (Stage1 within 2e-5 as test_main.cpp:142)."""
+    bits = np.array([(i * 13 + 7) & 1 for i in range(12000)], np.uint8)
+    seg = T.segment_transport_block(T.append_crc24a(bits))
+    p = T.LteTurboCodecParams(transport_block_bits=12000, g_total=sum(3 * (len(c) + 4) for c in seg.code_blocks),
+                              rv=4, max_iterations=6)
+    enc = T.encode_transport_block(p, bits)
+    cpu = T.decode_transport_block(p, enc, T.DecoderBackend.CpuReference)
+    dec = T.decode_transport_block(p, enc, getattr(T.DecoderBackend, backend))
+    assert np.array_equal(dec.tb_bits, bits) and dec.transport_block_crc_ok
+    if backend == "GpuStage1Exact":
+        np.testing.assert_allclose(dec.posterior_llr, cpu.posterior_llr, atol=2e-5, rtol=1e-6)
+
+
 def test_tile8_alias_hard_decisions():
     # reference tests/test_main.cpp:153-175: tile8 hard decisions equal cb_bits
     bits, p, enc = _tb(40, (7, 3))
@@ -306,6 +324,46 @@ def test_window_early_stop_matches_oracle(decoder):
         assert_llr_parity(post[off:off + k], ref, atol=atol, rtol=rtol, hard_tol=hard)
         off += k
     assert crc.sum() >= 6 and not crc[2]
+
+
+@pytest.mark.parametrize("early_stop", [False, True])
+@pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])
+@pytest.mark.parametrize("k", [96, 352, 544, 768])
+def test_window_spare_lane_jobs_match_oracle(k, decoder, early_stop):
+    """Jobs that pack several code blocks of several segments each into one
+    wave and leave lanes idle (M = 3/11/17/24 segments, 21/5/3/2 blocks per
+    wave, 1/9/13/16 spare lanes), with 2 x blocks_per_wave + 1 blocks so the
+    last job is partly empty too: the idle lanes' shadowing, the per-lane
+    staging and the strided copy-out.  Each block equals the oracle's
+    restatement at the window bar (_window_bar); with early stop, stopping
+    iteration and CRC verdict equal the oracle's."""
+    geo = T.segment_geometry(k, decoder)
+    m, bpw = geo["segments"], geo["blocks_per_wave"]
+    assert 1 < m and bpw > 1 and m * bpw < 64
+    n = 2 * bpw + 1
+    rng = np.random.default_rng(1000 * k + 10 * decoder + early_stop)
+    ebs = [float(x) for x in rng.uniform(0.0, 3.0, n)]
+    if early_stop:
+        _, llrs = _crc_blocks([k] * n, [0] * n, ebs, 1000 + k)
+    else:
+        llrs = []
+        for e in ebs:
+            cb = rng.integers(0, 2, k).astype(np.uint8)
+            llrs.append(np.concatenate(bpsk_awgn_llr(O.encode_cb(cb), k, e, rng)))
+    _say(f"spare lanes: K={k} M={m} blocks/wave={bpw} n={n} decoder={decoder} es={early_stop}")
+    post, crc, its = T.decode_batch_host([k] * n, np.concatenate(llrs), decoder, 8, early_stop)
+    lin = decoder == T.DECODER_WINDOW_LIN
+    for i, l in enumerate(llrs):
+        ref, ok, used = O.decode_window(*_split(l, k), k, geo["segment_length"], geo["warmup"], lin, 0, 8,
+                                        early_stop, geo["extrinsic_bits"])
+        atol, rtol, hard = _window_bar(geo, ref)
+        assert_llr_parity(post[i * k:(i + 1) * k], ref, atol=atol, rtol=rtol, hard_tol=hard)
+        if early_stop:
+            assert (int(its[i]), bool(crc[i])) == (used, ok), i
+        else:
+            assert its[i] == 8
+    if early_stop:
+        assert crc.sum() > 0
 
 
 def test_window_early_stop_equals_fixed_iterations():

@@ -308,24 +308,54 @@ def _random_tbs(seed, n):
     return tbs, pays
 
 
-def test_gpu_encoder_matches_host_encoder():
-    """GPU CRC24A, segmentation + CRC24B, turbo encoder and rate matcher are
-    bit-exact with encode_transport_block (the host codec, itself bit-exact
-    with the reference's golden vectors) for mixed sizes, fillers, rv, Qm,
-    layers and E above and below the circular buffer."""
-    tbs, pays = _random_tbs(5, 40)
-    _say("gpu encoder", len(tbs), "TBs")
+def test_gpu_encoder_matches_reference_golden():
+    """tc_tb_plan_encode reproduces the reference's encode_transport_block
+    (lte_turbo.cpp:690-720: CRC24A, segmentation with filler and CRC24B, the
+    encoder with its tail multiplexing, rate matching) bit for bit on the
+    golden transport blocks: 8 sizes (16 .. 12000 bits, fillers, C = 2,
+    punctured and full-rate g_total) x rv 0..3, rate-matched bits of every code
+    block and the code-block bits."""
+    g = golden()
+    meta = g["tb_meta"]
+    cases = sorted({(int(r[0]), int(r[1]), int(r[2]), int(r[3])) for r in meta})
+    tbs = [(b, gt, rv, 1, 2) for _, b, gt, rv in cases]
+    pays = [g[f"tb{ci}_bits"] for ci, _, _, _ in cases]
+    _say("gpu encoder vs golden:", len(tbs), "TBs")
     rm, cb = T.encode_transport_blocks(tbs, pays)
     roff = coff = 0
-    for (b, g, rv, nl, qm), pay in zip(tbs, pays):
-        enc = T.encode_transport_block(T.LteTurboCodecParams(transport_block_bits=b, g_total=g, rv=rv,
-                                                             num_layers=nl, mod_order=qm), pay)
-        for blk in enc.code_blocks:
-            e, k = len(blk.rate_matched_bits), len(blk.cb_bits)
-            assert np.array_equal(rm[roff:roff + e], blk.rate_matched_bits), (b, g, rv)
-            assert np.array_equal(cb[coff:coff + k], blk.cb_bits), (b, g, rv)
+    for ci, b, gt, rv in cases:
+        blocks = sorted(int(r[4]) for r in meta if (int(r[0]), int(r[3])) == (ci, rv))
+        for r in blocks:
+            want = g[f"tb{ci}_rv{rv}_cb{r}_rm"]
+            assert np.array_equal(rm[roff:roff + want.size], want), (ci, rv, r)
+            roff += want.size
+            cbb = g[f"tb{ci}_rv0_cb{r}_cbbits"]
+            assert np.array_equal(cb[coff:coff + cbb.size], cbb), (ci, rv, r)
+            coff += cbb.size
+    assert roff == rm.size and coff == cb.size
+
+
+def test_gpu_encoder_matches_oracle_chain():
+    """The same on 40 random TBs (16 .. 75376 bits, so up to 13 code blocks;
+    Qm 2/4/6, 1-2 layers, rv 0-3, E above and below the circular buffer)
+    against the oracle's chain: or_crc_append(CRC24A) -> or_segment ->
+    or_encode_cb -> or_compute_e / or_rate_match (lte_turbo.cpp:104-325,
+    540-720)."""
+    tbs, pays = _random_tbs(5, 40)
+    assert max(t[0] for t in tbs) == 75376
+    _say("gpu encoder vs oracle:", len(tbs), "TBs")
+    rm, cb = T.encode_transport_blocks(tbs, pays)
+    roff = coff = 0
+    for (b, gt, rv, nl, qm), pay in zip(tbs, pays):
+        seg, blocks = O.segment(O.crc_append(O.CRC24A, pay))
+        for r, blk in enumerate(blocks):
+            f = seg["filler_bits"] if r == 0 else 0
+            e = O.compute_e(r, gt, seg["c"], nl, qm)
+            want = O.rate_match(*O.encode_cb(blk, f), f, e, rv)
+            assert np.array_equal(rm[roff:roff + e], want), (b, gt, rv, r)
+            assert np.array_equal(cb[coff:coff + blk.size], blk), (b, gt, rv, r)
             roff += e
-            coff += k
+            coff += blk.size
     assert roff == rm.size and coff == cb.size
 
 
@@ -595,3 +625,97 @@ def test_tb_staging_mode_errors():
     for bad in (-1, 3):
         with pytest.raises(ValueError):
             plan.set_staging(bad)
+
+
+# -------------------------------------------------- config 5: mixed load ---
+def test_mixed_load_harq_rayleigh_16qam_matches_reference_chain():
+    """The large mixed load at test size: 320 code blocks, 70% K=6144 and 30%
+    drawn from the QPP table, each K - 24 payload bits + CRC24B (code-block
+    CRC descriptors, so early stop applies as to the reference's segmented
+    blocks, lte_turbo.cpp:792-804); GPU encoder with rv 0 and rv 2, 16QAM over
+    i.i.d. Rayleigh fading with known CSI at Eb/N0 U[2,6] dB, both
+    transmissions soft-combined (lte_turbo.cpp:280-325 in fp32) and decoded
+    with early stop in one TbPlan.decode_multi call.
+
+    Bars: posteriors, CRC flags and iteration counts bit-exact with the
+    oracle's soft de-rate matching (or_derate_match_soft, rv 0 then rv 2 added)
+    followed by the batched decode of those inputs; each block's TB flag is the
+    CRC24B of its hard decisions and its payload the first K - 24 of them
+    (or_crc_check, the reference's polynomial); CRC verdicts and iteration
+    counts equal the oracle's windowed decoder (or_decode_window, fp16-held
+    extrinsics, early stop) on all but at most 1% of the blocks, where one
+    fp16 rounding step of a held extrinsic (hardware exp2/log2 against libm)
+    can move a marginal block's stopping iteration."""
+    import concurrent.futures as cf
+
+    import torch
+    rng = np.random.default_rng(505)
+    n = 320
+    table = [int(e[0]) for e in T.qpp_interleaver_table()]
+    ks = np.where(rng.random(n) < 0.7, 6144, rng.choice(table, n)).astype(np.int64)
+    es = (3 * (ks + 4) // 2) // 4 * 4
+    ebn0 = rng.uniform(2.0, 6.0, n)
+    tbs = [(int(k) - 24, int(e), 0, 1, 4, 1) for k, e in zip(ks, es)]
+    plan = T.TbPlan(tbs, T.DECODER_WINDOW_LOG, 8, early_stop=True)
+    assert list(plan.ks) == list(ks) and list(plan.es) == list(es) and not plan.fillers.any()
+    pay_h = rng.integers(0, 2, plan.bits_total).astype(np.uint8)
+    pay = torch.from_numpy(pay_h).cuda()
+    n0 = 1.0 / (10.0 ** (ebn0 / 10.0) * 4.0 * (ks - 24) / es)
+    ch = T.ChannelPlan([(int(e), 4, 1, float(v)) for e, v in zip(es, n0)])
+    s = torch.cuda.current_stream().cuda_stream
+    rx = []
+    for rv, seed in ((0, 71), (2, 72)):
+        rm = torch.empty(plan.rx_floats, dtype=torch.uint8, device="cuda")
+        plan.encode(pay.data_ptr(), rm.data_ptr(), rv=rv, stream=s)
+        y = torch.empty(plan.rx_floats, dtype=torch.float32, device="cuda")
+        ch.run(seed, rm.data_ptr(), y.data_ptr(), stream=s)
+        rx.append(y)
+    bits = torch.zeros(plan.bits_total, dtype=torch.uint8, device="cuda")
+    tb_ok = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+    cb_ok = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+    its = torch.zeros(n, dtype=torch.int32, device="cuda")
+    post = torch.zeros(plan.post_floats, dtype=torch.float32, device="cuda")
+    _say("mixed load", n, "blocks: decode_multi rv0+rv2")
+    plan.decode_multi([rx[0].data_ptr(), rx[1].data_ptr()], [0, 2], bits.data_ptr(), tb_ok.data_ptr(),
+                      cb_ok.data_ptr(), its.data_ptr(), post.data_ptr(), stream=s)
+    torch.cuda.synchronize()
+    post_h, bits_h = post.cpu().numpy(), bits.cpu().numpy()
+    tb_h, cb_h, its_h = tb_ok.cpu().numpy(), cb_ok.cpu().numpy(), its.cpu().numpy()
+    rx_h = [r.cpu().numpy() for r in rx]
+    # reference chain: the oracle's soft de-rate matching of both transmissions, added
+    llr, ioff = [], 0
+    for k, e in zip(ks, es):
+        l0 = O.derate_match_soft(rx_h[0][ioff:ioff + int(e)], int(k) + 4, 0, 0)
+        llr.append(O.derate_match_soft(rx_h[1][ioff:ioff + int(e)], int(k) + 4, 0, 2, out=l0))
+        ioff += int(e)
+    _say("mixed load: batched decode of the oracle-combined inputs")
+    post_ref, crc_ref, its_ref = T.decode_batch_host(ks, np.concatenate(llr), T.DECODER_WINDOW_LOG, 8,
+                                                     early_stop=True)
+    assert np.array_equal(post_h.view(np.uint32), post_ref.view(np.uint32))
+    assert np.array_equal(tb_h.astype(bool), crc_ref) and np.array_equal(cb_h, tb_h)
+    assert np.array_equal(its_h, its_ref.astype(np.int32))
+    poff = boff = 0
+    for i, k in enumerate(ks):
+        hard = (post_h[poff:poff + k] >= 0).astype(np.uint8)
+        assert bool(tb_h[i]) == O.crc_check(O.CRC24B, hard), i
+        assert np.array_equal(bits_h[boff:boff + k - 24], hard[:k - 24]), i
+        poff += int(k)
+        boff += int(k) - 24
+    bler = 1.0 - tb_h.mean()
+    _say(f"mixed load: BLER {bler:.4f}, mean iterations {its_h.mean():.2f}")
+    assert bler < 0.1 and its_h.mean() < 4.0
+    # the oracle's windowed decoder, same geometry and held-extrinsic width
+
+    def one(i):
+        k = int(ks[i])
+        geo = T.segment_geometry(k)
+        d = k + 4
+        _, ok, used = O.decode_window(llr[i][:d], llr[i][d:2 * d], llr[i][2 * d:], k, geo["segment_length"],
+                                      geo["warmup"], False, 0, 8, True, geo["extrinsic_bits"])
+        return ok, used
+
+    with cf.ThreadPoolExecutor(8) as ex:
+        res = list(ex.map(one, range(n)))
+    differ = [i for i, (ok, used) in enumerate(res) if ok != bool(tb_h[i]) or used != its_h[i]]
+    _say(f"mixed load: oracle window decoder differs on {len(differ)} of {n} blocks")
+    assert len(differ) <= n // 100, differ

@@ -224,3 +224,18 @@ def test_channel_plan_argument_errors_before_device():
     with pytest.raises(ValueError):
         T.ChannelPlan([(12, 4, 0, 0.0)])  # n0
 
+
+
+def test_transport_block_entry_points_check_buffer_lengths():
+    """The batched TB entry points take no lengths in C; the Python mirror
+    checks them against the descriptors first (a short array would otherwise
+    be read past its end)."""
+    tbs = [(1000, 3108, 0, 1, 2), (40, 150, 1, 1, 2), (6120, 6000, 0, 1, 4, 1)]
+    ks, es = T._tb_sizes(T._tb_descs(tbs), len(tbs))
+    assert ks == [1024, 64, 6144] and es[1] == 150 and sum(es) == 3108 + 150 + 6000
+    with pytest.raises(ValueError):
+        T.decode_transport_blocks(tbs, np.zeros(sum(es) - 1, np.float32))
+    with pytest.raises(ValueError):
+        T.encode_transport_blocks(tbs, [np.zeros(1000, np.uint8), np.zeros(39, np.uint8), np.zeros(6120, np.uint8)])
+    with pytest.raises(ValueError):
+        T.encode_transport_blocks(tbs, [np.zeros(1000, np.uint8)])

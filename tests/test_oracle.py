@@ -493,3 +493,23 @@ def test_channel_oracle_constellations_and_closed_forms():
     g = np.abs(h) ** 2 / 0.5
     assert np.allclose(llr[:, 0], -2 * np.sqrt(2) * z.real * g) and np.allclose(llr[:, 1], -2 * np.sqrt(2) * z.imag * g)
 
+
+
+@pytest.mark.parametrize("k,seg", [(40, 8), (40, 40), (352, 32), (1024, 32), (6144, 96), (6144, 192)])
+def test_exact_segmented_equals_serial(k, seg):
+    """or_decode_exact_seg (segment-parallel recursions whose boundaries are
+    fixed up until every segment meets its own checkpoints bit for bit) gives
+    the serial decoder's posteriors (or_decode_cb, lte_turbo.cpp:736-813) bit
+    for bit, with and without early stop, at low and high SNR; the fix-up
+    work is reported in `work`."""
+    rng = np.random.default_rng(k + seg)
+    for ebn0, es in ((0.0, False), (0.75, True), (2.0, True)):
+        payload = rng.integers(0, 2, k - 24).astype(np.uint8)
+        cb = O.crc_append(O.CRC24B, payload)
+        d0, d1, d2 = bpsk_awgn_llr(O.encode_cb(cb), k, ebn0, rng)
+        want, ok, used = O.decode_cb(d0, d1, d2, k, 0, 8, es)
+        work = np.zeros(4, np.uint64)
+        got, ok2, used2 = O.decode_exact_seg(d0, d1, d2, k, seg, 0, 8, es, work=work)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (ebn0, es)
+        assert (ok2, used2) == (ok, used)
+        assert work[0] == 2 * used and work[1] >= work[0]

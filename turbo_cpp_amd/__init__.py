@@ -687,14 +687,12 @@ def decode_transport_blocks(tbs, rx, decoder: int = DECODER_WINDOW_LOG, max_iter
     n = len(tbs)
     rx = _f32(rx).reshape(-1)
     b = [int(descs[i].transport_block_bits) for i in range(n)]
+    ks, es = _tb_sizes(descs, n)
+    if rx.size != sum(es):  # the C entry point reads sum(E_r) floats
+        raise ValueError(f"rx holds {rx.size} LLRs, the transport blocks' code blocks take {sum(es)}")
     bits = np.zeros(max(sum(b), 1), np.uint8)
     tb = np.zeros(max(n, 1), np.int32)
     o = options(decoder, max_iterations, early_stop)
-    # sizes come from a plan-free query: segmentation on the host
-    ks, ncb = [], 0
-    for i in range(n):
-        seg = _segment_sizes(b[i] + 24)
-        ks += [seg[4]] * seg[2] + [seg[3]] * seg[1]
     cb = np.zeros(max(len(ks), 1), np.int32)
     its = np.zeros(max(n, 1), np.uint32)
     post = np.zeros(max(sum(ks), 1), np.float32)
@@ -710,7 +708,7 @@ def decode_transport_blocks(tbs, rx, decoder: int = DECODER_WINDOW_LOG, max_iter
 def _standard_geometry(params: LteTurboCodecParams, encoded: EncodedTransportBlock) -> bool:
     """True when the blocks are those encode_transport_block makes for params."""
     try:
-        if params.transport_block_bits == 0:
+        if params.transport_block_bits == 0 or params.rv > 3:  # device de-rate tables cover rv 0..3
             return False
         c, cp, cm, kp, km, f, _ = _segment_sizes(params.transport_block_bits + 24)
         if c != len(encoded.code_blocks):
@@ -733,18 +731,36 @@ def encode_transport_blocks(tbs, payloads):
     every code block back to back, code-block bits back to back)."""
     descs = _tb_descs(tbs)
     n = len(tbs)
+    if len(payloads) != n:
+        raise ValueError("one payload per transport block")
+    for i, p in enumerate(payloads):  # the C entry point reads sum(B) bytes
+        if np.asarray(p).size != int(descs[i].transport_block_bits):
+            raise ValueError(f"payload {i} has {np.asarray(p).size} bits, its descriptor says "
+                             f"{int(descs[i].transport_block_bits)}")
     pay = np.concatenate([_u8(p) for p in payloads]) if n else np.zeros(1, np.uint8)
-    ne = nk = 0
-    for i in range(n):
-        c, cp, cm, kp, km, f, _ = _segment_sizes(int(descs[i].transport_block_bits) + 24)
-        ks = [km] * cm + [kp] * cp
-        nk += sum(ks)
-        ne += sum(compute_e(r, int(descs[i].g_total), c, int(descs[i].num_layers), int(descs[i].mod_order))
-                  for r in range(c))
+    ks, es = _tb_sizes(descs, n)
+    nk, ne = sum(ks), sum(es)
     rm = np.zeros(max(ne, 1), np.uint8)
     cb = np.zeros(max(nk, 1), np.uint8)
     _check(lib().tc_encode_transport_blocks_host(descs, n, pay, rm, cb.ctypes.data))
     return rm[:ne], cb[:nk]
+
+
+def _tb_sizes(descs, n):
+    """K_r and E_r of every code block of n TB descriptors, as tc_tb_plan_create
+    derives them (a code_block_crc descriptor is one block of B + 24 bits)."""
+    ks, es = [], []
+    for i in range(n):
+        d = descs[i]
+        b = int(d.transport_block_bits)
+        if d.code_block_crc:
+            c, blocks = 1, [b + 24]
+        else:
+            c, cp, cm, kp, km, _, _ = _segment_sizes(b + 24)
+            blocks = [km] * cm + [kp] * cp
+        ks += blocks
+        es += [compute_e(r, int(d.g_total), c, int(d.num_layers), int(d.mod_order)) for r in range(c)]
+    return ks, es
 
 
 def _segment_sizes(b: int):

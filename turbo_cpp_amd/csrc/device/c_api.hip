@@ -72,13 +72,12 @@ WinKernel window_kernel(const tc_options& o) {
 
 }  // namespace
 
-struct tc_plan {
+// What a launch reads: the device tables and work space of a plan.  A tc_plan
+// owns its own (hipMalloc); the no-allocation entry point places a one-block
+// plan in a static device array (noalloc.hip).
+struct PlanDev {
   tc_options opts{};
-  int device = 0;
   size_t n_cb = 0;
-  size_t llr_floats = 0, post_floats = 0;
-  std::vector<tcdev::CbInfo> cbs;
-  // device
   tcdev::CbInfo* d_cbs = nullptr;
   // exact
   float* d_work = nullptr;
@@ -88,20 +87,67 @@ struct tc_plan {
   tcdev::WinJob* d_jobs = nullptr;
   uint32_t* d_cb_index = nullptr;
   uint32_t* d_qtab = nullptr;
-  uint32_t* d_crc_pow = nullptr;
+  const uint32_t* d_crc_pow = nullptr;
   float2* d_scratch = nullptr;
   uint32_t* d_sched = nullptr;
   uint32_t njobs = 0, grid = 0;
   size_t lds_bytes = 0;
+};
+
+struct tc_plan : PlanDev {
+  int device = 0;
+  size_t llr_floats = 0, post_floats = 0;
+  std::vector<tcdev::CbInfo> cbs;
 
   ~tc_plan() {
-    for (void* p : {(void*)d_cbs, (void*)d_work, (void*)d_groups, (void*)d_jobs, (void*)d_cb_index, (void*)d_qtab, (void*)d_crc_pow,
-                    (void*)d_scratch, (void*)d_sched})
-      if (p) (void)hipFree(p);
+    for (const void* p : {(const void*)d_cbs, (const void*)d_work, (const void*)d_groups, (const void*)d_jobs,
+                          (const void*)d_cb_index, (const void*)d_qtab, (const void*)d_crc_pow,
+                          (const void*)d_scratch, (const void*)d_sched})
+      if (p) (void)hipFree(const_cast<void*>(p));
   }
 };
 
 namespace {
+
+// Per-block fields of a plan (QPP parameters, no offsets).
+tcdev::CbInfo cb_info(uint32_t k, uint32_t f) {
+  const int qi = turbo_cpp::detail::qpp_index(k);
+  if (qi < 0) throw Failure{TC_ERR_OUT_OF_RANGE, "unsupported QPP block size"};
+  if (f >= k) throw Failure{TC_ERR_INVALID_ARGUMENT, "filler count must be below K"};
+  tcdev::CbInfo c{};
+  c.k = k;
+  c.filler = f;
+  c.f1 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f1);
+  c.f2 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f2);
+  return c;
+}
+
+// Exact-decoder work space of one block, in floats.
+size_t exact_work_floats(uint32_t k) { return 16 * (size_t)(k + 1) + 4 * (size_t)k; }
+
+// The window decoders' per-K table: entry t < S holds (pi(t) mod S) << 16 | pi(t) div S.
+void fill_qtab(uint32_t k, uint32_t seg, uint32_t f1, uint32_t f2, uint32_t* out) {
+  for (uint32_t t = 0; t < seg; ++t) {
+    const uint64_t pi = (static_cast<uint64_t>(f1) * t + static_cast<uint64_t>(f2) * t * t) % k;
+    out[t] = (static_cast<uint32_t>(pi % seg) << 16) | static_cast<uint32_t>(pi / seg);
+  }
+}
+
+tcdev::WinJob make_job(const tc_segment_info& g, uint32_t f1, uint32_t f2, uint32_t nblk, uint32_t cb_off,
+                       uint32_t qtab_off) {
+  tcdev::WinJob j{};
+  j.k = g.k;
+  j.seg = g.segment_length;
+  j.nseg = g.segments;
+  j.nblk = nblk;
+  j.warm = g.warmup;
+  j.f1 = f1;
+  j.f2 = f2;
+  j.cb_off = cb_off;
+  j.qtab_off = qtab_off;
+  j.div_magic = static_cast<uint32_t>(((1ull << 32) + g.segment_length - 1) / g.segment_length);
+  return j;
+}
 
 void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n, const tc_options& o) {
   require_device();
@@ -111,21 +157,14 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   p.cbs.resize(n);
   size_t llr = 0, post = 0, work = 0;
   for (size_t i = 0; i < n; ++i) {
-    const int qi = turbo_cpp::detail::qpp_index(k[i]);
-    if (qi < 0) throw Failure{TC_ERR_OUT_OF_RANGE, "unsupported QPP block size"};
-    const uint32_t f = filler ? filler[i] : 0;
-    if (f >= k[i]) throw Failure{TC_ERR_INVALID_ARGUMENT, "filler count must be below K"};
     auto& c = p.cbs[i];
-    c.k = k[i];
-    c.filler = f;
-    c.f1 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f1);
-    c.f2 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f2);
+    c = cb_info(k[i], filler ? filler[i] : 0);
     c.llr_off = llr;
     c.post_off = post;
     c.work_off = work;
     llr += 3 * (size_t)(k[i] + 4);
     post += k[i];
-    work += 16 * (size_t)(k[i] + 1) + 4 * (size_t)k[i];
+    work += exact_work_floats(k[i]);
   }
   p.llr_floats = llr;
   p.post_floats = post;
@@ -159,23 +198,11 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
     const uint32_t f1 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f1);
     const uint32_t f2 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f2);
     const uint32_t qoff = static_cast<uint32_t>(qtab.size());
-    for (uint32_t t = 0; t < g.segment_length; ++t) {
-      const uint64_t pi = (static_cast<uint64_t>(f1) * t + static_cast<uint64_t>(f2) * t * t) % kk;
-      const uint32_t r = static_cast<uint32_t>(pi % g.segment_length), c0 = static_cast<uint32_t>(pi / g.segment_length);
-      qtab.push_back((r << 16) | c0);
-    }
+    qtab.resize(qoff + g.segment_length);
+    fill_qtab(kk, g.segment_length, f1, f2, qtab.data() + qoff);
     for (size_t s = 0; s < list.size(); s += g.blocks_per_wave) {
-      tcdev::WinJob j{};
-      j.k = kk;
-      j.seg = g.segment_length;
-      j.nseg = g.segments;
-      j.nblk = static_cast<uint32_t>(std::min<size_t>(g.blocks_per_wave, list.size() - s));
-      j.warm = g.warmup;
-      j.f1 = f1;
-      j.f2 = f2;
-      j.cb_off = static_cast<uint32_t>(cb_index.size());
-      j.qtab_off = qoff;
-      j.div_magic = static_cast<uint32_t>(((1ull << 32) + g.segment_length - 1) / g.segment_length);
+      const tcdev::WinJob j = make_job(g, f1, f2, static_cast<uint32_t>(std::min<size_t>(g.blocks_per_wave, list.size() - s)),
+                                       static_cast<uint32_t>(cb_index.size()), qoff);
       for (uint32_t b = 0; b < j.nblk; ++b) cb_index.push_back(list[s + b]);
       jobs.push_back(j);
     }
@@ -208,18 +235,14 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   p.d_scratch = dev_alloc<float2>(static_cast<size_t>(p.grid) * tcdev::kSlotFloat2);
   p.d_sched = dev_alloc<uint32_t>(2);
   hip_check(hipMemset(p.d_sched, 0, 2 * sizeof(uint32_t)), "hipMemset");
-  // D^m mod G(D) for the CRC24B generator (lte_turbo.cpp:108-111)
   std::vector<uint32_t> pw(tcdev::kCrcPowLen);
-  uint32_t r = 1;
-  for (int m = 0; m < tcdev::kCrcPowLen; ++m) {
-    pw[m] = r;
-    r = ((r << 1) & 0xFFFFFFu) ^ (((r >> 23) & 1u) ? 0xC00063u : 0u);
-  }
-  p.d_crc_pow = dev_alloc<uint32_t>(pw.size());
-  dev_upload(p.d_crc_pow, pw);
+  tchost::crc24b_powers(pw.data());
+  uint32_t* crc_pow = dev_alloc<uint32_t>(pw.size());
+  p.d_crc_pow = crc_pow;
+  dev_upload(crc_pow, pw);
 }
 
-void launch(tc_plan& p, const float* llr, float* post, int32_t* crc, uint32_t* iters, hipStream_t s,
+void launch(const PlanDev& p, const float* llr, float* post, int32_t* crc, uint32_t* iters, hipStream_t s,
             const tcdev::RxIn* rx = nullptr, const tcdev::TbOut* tb = nullptr) {
   if (p.n_cb == 0) return;
   const tc_options& o = p.opts;
@@ -247,9 +270,16 @@ bool tchost::launch_window(tc_plan* plan, const float* llr, float* post, int32_t
   return true;
 }
 
-namespace {
+void tchost::crc24b_powers(uint32_t* out) {
+  // D^m mod G(D) for the CRC24B generator (lte_turbo.cpp:108-111)
+  uint32_t r = 1;
+  for (int m = 0; m < tcdev::kCrcPowLen; ++m) {
+    out[m] = r;
+    r = ((r << 1) & 0xFFFFFFu) ^ (((r >> 23) & 1u) ? 0xC00063u : 0u);
+  }
+}
 
-void check_early_stop_support(const tc_options&) {}
+namespace {
 
 void check_status(tc_status st) {
   if (st != TC_OK) throw Failure{st, turbo_cpp::detail::last_error()};
@@ -294,7 +324,6 @@ tc_status tc_plan_create(const uint32_t* k, const uint32_t* filler, size_t n_cb,
   return guarded([&] {
     if (!plan || (n_cb && !k)) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
     const tc_options o = normalised(opts);
-    check_early_stop_support(o);
     auto p = std::make_unique<tc_plan>();
     build_plan(*p, k, filler, n_cb, o);
     *plan = p.release();
@@ -357,7 +386,6 @@ tc_status tc_decode_batch_host(const uint32_t* k, const uint32_t* filler, size_t
                                const float* llr, float* post, int32_t* crc_ok, uint32_t* iterations_used) {
   return guarded([&] {
     const tc_options o = normalised(opts);
-    check_early_stop_support(o);
     tc_plan p;
     build_plan(p, k, filler, n_cb, o);
     if (n_cb == 0) return;
@@ -393,7 +421,6 @@ tc_status tc_decode_code_block(const tc_cb_desc* desc, const float* d0, size_t n
     if (turbo_cpp::detail::qpp_index(k) < 0) throw Failure{TC_ERR_OUT_OF_RANGE, "unsupported QPP block size"};
     if (desc->d_r < desc->k_r + 4) throw Failure{TC_ERR_INVALID_ARGUMENT, "d_r must be at least K + 4"};
     const tc_options o = normalised(opts);
-    check_early_stop_support(o);
     require_device();
     const uint32_t f = static_cast<uint32_t>(desc->filler_count);
     int dev = 0;
@@ -407,10 +434,14 @@ tc_status tc_decode_code_block(const tc_cb_desc* desc, const float* d0, size_t n
     }
     const size_t need = 3 * (size_t)(k + 4);
     if (g_one.cap < need) {
-      for (void* q : {(void*)g_one.d_llr, (void*)g_one.d_post})
-        if (q) (void)hipFree(q);
-      g_one.d_llr = dev_alloc<float>(need);
-      g_one.d_post = dev_alloc<float>(need);
+      // both new buffers first: a failed allocation leaves the cache as it was
+      DevBuf<float> llr_buf(need), post_buf(need);
+      for (float** q : {&g_one.d_llr, &g_one.d_post}) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+      }
+      std::swap(g_one.d_llr, llr_buf.p);
+      std::swap(g_one.d_post, post_buf.p);
       g_one.cap = need;
     }
     if (!g_one.d_crc) g_one.d_crc = dev_alloc<int32_t>(1);

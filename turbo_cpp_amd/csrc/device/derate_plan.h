@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "rx_types.h"
+#include "turbo_cpp/c_api.h"
 
 struct tc_derate_plan {
   size_t n_cb = 0;
@@ -36,5 +37,8 @@ namespace tchost {
 // exact-decoder plan.
 bool launch_window(tc_plan* plan, const float* llr, float* post, int32_t* crc_ok, uint32_t* iters,
                    hipStream_t stream, const tcdev::RxIn* rx, const tcdev::TbOut* tb);
+
+// D^m mod G(D) of the CRC24B generator for m < tcdev::kCrcPowLen.
+void crc24b_powers(uint32_t* out);
 }  // namespace tchost
 

@@ -85,6 +85,9 @@ namespace {
 bool standard_geometry(const LteTurboCodecParams& params, const EncodedTransportBlock& encoded) {
   try {
     if (params.transport_block_bits == 0) return false;
+    // the device de-rate tables cover rv 0..3; the host path takes k0 of any rv
+    // (lte_turbo.cpp:263-264)
+    if (params.rv > 3) return false;
     const auto seg = detail::segmentation_sizes(params.transport_block_bits + 24);
     if (seg.c != encoded.code_blocks.size()) return false;
     for (std::size_t r = 0; r < seg.c; ++r) {
