@@ -83,6 +83,20 @@ tc_status tc_decode_code_block(const tc_cb_desc* desc, const float* d0, size_t n
                                const tc_options* opts, float* posterior, int32_t* crc_ok,
                                uint32_t* iterations_used);
 
+/* The same decode without allocating device memory.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (serving loops that must
+ * not allocate, harnesses that forbid it).  Same arguments, checks, kernels
+ * and results as tc_decode_code_block; the inputs, outputs and work space
+ * live in a static device array of 640 KiB that each GPU the library runs on
+ * holds until the process ends, sized for the largest code block (K = 6144).
+ * Calls on one GPU share that array: calls that overlap in time on one GPU
+ * (e.g. from several host threads) race and must be serialised by the
+ * caller. */
+tc_status tc_decode_code_block_noalloc(const tc_cb_desc* desc, const float* d0, size_t n0,
+                                       const float* d1, size_t n1, const float* d2, size_t n2,
+                                       const tc_options* opts, float* posterior, int32_t* crc_ok,
+                                       uint32_t* iterations_used);
+
 /* ---- batched decode, device-resident buffers ------------------------------
  * New: no counterpart in the reference, which decodes one code block per call
  * (lte_turbo.cpp:941-951 runs those calls on std::async threads).  A plan

@@ -85,6 +85,7 @@ def lib() -> C.CDLL:
         L.tc_decode_code_block.argtypes = [C.POINTER(TcCbDesc), _f32p, C.c_size_t, _f32p, C.c_size_t, _f32p,
                                            C.c_size_t, C.POINTER(TcOptions), _f32p, C.POINTER(C.c_int32),
                                            C.POINTER(C.c_uint32)]
+        L.tc_decode_code_block_noalloc.argtypes = L.tc_decode_code_block.argtypes
         L.tc_plan_create.argtypes = [_u32p, C.c_void_p, C.c_size_t, C.POINTER(TcOptions), C.POINTER(C.c_void_p)]
         L.tc_plan_destroy.argtypes = [C.c_void_p]
         L.tc_plan_destroy.restype = None
@@ -413,6 +414,20 @@ def _gpu_decode(decoder, descriptor, d0, d1, d2, max_iterations, enable_early_st
     cd = descriptor.c()
     _check(lib().tc_decode_code_block(C.byref(cd), d0, len(d0), d1, len(d1), d2, len(d2), C.byref(o), post,
                                       C.byref(ok), C.byref(it)))
+    return post, bool(ok.value), it.value
+
+
+def decode_code_block_noalloc(decoder, descriptor, d0, d1, d2, max_iterations, enable_early_stop, **kw):
+    """tc_decode_code_block_noalloc: the per-block GPU decode without device
+    allocation (inputs, outputs and work space in the library's static device
+    array).  Calls that overlap on one GPU race."""
+    d0, d1, d2 = _spans(d0, d1, d2)
+    post = np.zeros(descriptor.k_r, np.float32)
+    ok, it = C.c_int32(0), C.c_uint32(0)
+    o = options(decoder, max_iterations, enable_early_stop, **kw)
+    cd = descriptor.c()
+    _check(lib().tc_decode_code_block_noalloc(C.byref(cd), d0, len(d0), d1, len(d1), d2, len(d2), C.byref(o), post,
+                                              C.byref(ok), C.byref(it)))
     return post, bool(ok.value), it.value
 
 

@@ -24,7 +24,7 @@ HOST_SOURCES = ["host/codec.cpp", "host/api.cpp"]
 DEVICE_SOURCES = ["device/exact_kernel.hip", "device/window_kernel.hip", "device/c_api.hip",
                   "device/rx_kernels.hip", "device/rx_api.hip", "device/tx_kernels.hip", "device/channel.hip",
                   "device/diff_kernel.hip",
-                  "device/diff_api.hip"]
+                  "device/diff_api.hip", "device/noalloc.hip"]
 
 COMMON = ["-O3", "-std=c++20", "-fPIC", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include")]
 # Packed FP32 (v_pk_*_f32) stays off for all device code.

@@ -279,6 +279,76 @@ void tchost::crc24b_powers(uint32_t* out) {
   }
 }
 
+tc_options tchost::normalised_options(const tc_options* o) { return normalised(o); }
+
+namespace {
+// Tables of a one-block decode, uploaded in one copy to the start of the
+// caller's area; the decoder's work space follows at kOneBlockWorkOff.
+struct OneBlockHeader {
+  tcdev::CbInfo cb;
+  tcdev::WinJob job;
+  uint32_t groups[tcdev::kExactBlocksPerWave];
+  uint32_t cb_index;
+  uint32_t sched[2];  // the window decoders' job queue: zero at launch
+  uint32_t qtab[tcdev::kMaxSeg + tcdev::kSubWin];
+};
+constexpr size_t round256(size_t n) { return (n + 255) / 256 * 256; }
+constexpr size_t kOneBlockWorkOff = round256(sizeof(OneBlockHeader));
+constexpr size_t kMaxK = 6144;
+}  // namespace
+
+size_t tchost::one_block_area_bytes() {
+  const size_t exact = exact_work_floats(kMaxK) * sizeof(float);
+  const size_t window = tcdev::kSlotFloat2 * sizeof(float2);
+  return kOneBlockWorkOff + round256(std::max(exact, window));
+}
+
+void tchost::decode_one_block(uint8_t* area, size_t area_bytes, const uint32_t* crc_pow_dev, const tc_options& o,
+                              uint32_t k, uint32_t filler, const float* llr_dev, float* post_dev, int32_t* crc_dev,
+                              uint32_t* it_dev, hipStream_t s) {
+  OneBlockHeader h{};
+  h.cb = cb_info(k, filler);
+  PlanDev p{};
+  p.opts = o;
+  p.n_cb = 1;
+  auto at = [&](const void* field) {
+    return area + (static_cast<const uint8_t*>(field) - reinterpret_cast<const uint8_t*>(&h));
+  };
+  p.d_cbs = reinterpret_cast<tcdev::CbInfo*>(at(&h.cb));
+  size_t work_bytes = 0;
+  if (o.decoder == TC_DECODER_EXACT) {
+    work_bytes = exact_work_floats(k) * sizeof(float);
+    h.groups[0] = 0;
+    for (uint32_t b = 1; b < tcdev::kExactBlocksPerWave; ++b) h.groups[b] = tcdev::kNoBlock;
+    p.d_groups = reinterpret_cast<uint32_t*>(at(h.groups));
+    p.n_groups = 1;
+    p.d_work = reinterpret_cast<float*>(area + kOneBlockWorkOff);
+  } else {
+    const tc_segment_info g = geometry(k, o);
+    work_bytes = tcdev::kSlotFloat2 * sizeof(float2);
+    fill_qtab(k, g.segment_length, h.cb.f1, h.cb.f2, h.qtab);  // padding entries stay zero
+    h.job = make_job(g, h.cb.f1, h.cb.f2, 1, 0, 0);
+    h.cb_index = 0;
+    p.d_jobs = reinterpret_cast<tcdev::WinJob*>(at(&h.job));
+    p.d_cb_index = reinterpret_cast<uint32_t*>(at(&h.cb_index));
+    p.d_qtab = reinterpret_cast<uint32_t*>(at(h.qtab));
+    p.d_sched = reinterpret_cast<uint32_t*>(at(h.sched));
+    p.d_crc_pow = crc_pow_dev;
+    p.d_scratch = reinterpret_cast<float2*>(area + kOneBlockWorkOff);
+    p.njobs = 1;
+    p.grid = 1;
+    p.lds_bytes = static_cast<size_t>(tcdev::job_lds_bytes(static_cast<int>(g.segment_length)));
+  }
+  if (kOneBlockWorkOff + work_bytes > area_bytes)
+    throw Failure{TC_ERR_INVALID_ARGUMENT, "code block needs " + std::to_string(kOneBlockWorkOff + work_bytes) +
+                                               " bytes of work space, above the " + std::to_string(area_bytes) +
+                                               " bytes available"};
+  hip_check(hipMemcpyAsync(area, &h, sizeof h, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+  // the header lives on this thread's stack: the copy must be done before it goes
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  launch(p, llr_dev, post_dev, crc_dev, it_dev, s);
+}
+
 namespace {
 
 void check_status(tc_status st) {

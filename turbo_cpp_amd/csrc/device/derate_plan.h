@@ -38,7 +38,20 @@ namespace tchost {
 bool launch_window(tc_plan* plan, const float* llr, float* post, int32_t* crc_ok, uint32_t* iters,
                    hipStream_t stream, const tcdev::RxIn* rx, const tcdev::TbOut* tb);
 
+// tc_options with defaults filled in (min_segment, warmup); throws on an
+// unknown decoder.
+tc_options normalised_options(const tc_options* o);
 // D^m mod G(D) of the CRC24B generator for m < tcdev::kCrcPowLen.
 void crc24b_powers(uint32_t* out);
+// Device bytes a one-code-block decode needs (tables + work space, K up to 6144).
+size_t one_block_area_bytes();
+// Decode one code block (K, filler) of llr_dev ([d0|d1|d2], K+4 each) into
+// post_dev on stream s, with the plan's tables and work space placed in
+// `area` (256-byte aligned, area_bytes long) instead of allocated: copies the
+// tables in (waiting for that copy) and launches.  crc_pow_dev: the CRC24B
+// power table (window decoders).  Allocates no device memory.
+void decode_one_block(uint8_t* area, size_t area_bytes, const uint32_t* crc_pow_dev, const tc_options& o,
+                      uint32_t k, uint32_t filler, const float* llr_dev, float* post_dev, int32_t* crc_dev,
+                      uint32_t* it_dev, hipStream_t s);
 }  // namespace tchost
 
