@@ -34,9 +34,12 @@ typedef int32_t tc_status;
 #define TC_ERR_RUNTIME 3
 
 /* Decoder algorithm. */
-#define TC_DECODER_EXACT 0      /* full-length Log-MAP with the reference's arithmetic */
-#define TC_DECODER_WINDOW_LOG 1 /* segment-parallel Log-MAP (max* in the log domain) */
-#define TC_DECODER_WINDOW_LIN 2 /* segment-parallel MAP in the probability domain */
+#define TC_DECODER_EXACT 0        /* the reference's Log-MAP arithmetic, segment-parallel with exact
+                                     segment boundaries (bit-identical to TC_DECODER_EXACT_SERIAL) */
+#define TC_DECODER_WINDOW_LOG 1   /* segment-parallel Log-MAP (max* in the log domain), warm-up +
+                                     next-iteration boundaries */
+#define TC_DECODER_WINDOW_LIN 2   /* segment-parallel MAP in the probability domain */
+#define TC_DECODER_EXACT_SERIAL 3 /* the reference's arithmetic, one full-length recursion per block */
 
 /* Mirrors turbo_cpp::CodeBlockDescriptor (lte_turbo.h:38-45). */
 typedef struct tc_cb_desc {
@@ -55,6 +58,9 @@ typedef struct tc_options {
   uint32_t min_segment;    /* window decoders: lower bound on the segment length, 0 = 32 */
   uint32_t warmup;         /* window decoders: warm-up steps per boundary, 0 = 32 */
   uint32_t waves_per_cu;   /* window decoders: resident waves per CU, 0 = as many as fit */
+  uint32_t extrinsic_bits; /* width of the extrinsics held between half-iterations: 16
+                              (fp16) or 32 (fp32); 0 = default (window decoders 16, exact
+                              decoders 32, which accept only 32) */
 } tc_options;
 
 /* Geometry the window decoders chose for one K (for tests and reports). */
@@ -66,7 +72,7 @@ typedef struct tc_segment_info {
   uint32_t warmup;         /* effective warm-up steps (0 when M == 1) */
   uint32_t extrinsic_bits; /* width of the extrinsics held between
                               half-iterations: 16 (fp16, saturated at
-                              +-65504, round to nearest even) */
+                              +-65504, round to nearest even) or 32 */
 } tc_segment_info;
 
 const char* tc_last_error(void);

@@ -1024,9 +1024,15 @@ int or_decode_window(const float* d0, const float* d1, const float* d2, uint32_t
                      int* crc_ok, uint32_t* iterations_used) {
   if (or_qpp_index(k) < 0 || seg_len == 0 || k % seg_len != 0) return -1;
   if (ex_bits != 32 && ex_bits != 16) return -1;
-  if (k / seg_len > 1 && warmup >= seg_len) return -1; /* warm-up stays inside the neighbour */
   const uint32_t nseg = k / seg_len;
-  const int g = (nseg > 1) ? (int)warmup : 0;
+  /* warmup >= k: every segment's warm-ups reach the block ends (alpha from
+   * alpha_0 at step 0, beta from the tail beta at step K), so each segment
+   * computes the full-length recursions' metrics and the result equals the
+   * one-segment decode (seg_len == k) bit for bit -- a check of the segment
+   * plumbing; otherwise the warm-up stays inside the neighbouring segment */
+  const int full = nseg > 1 && warmup >= k;
+  if (nseg > 1 && !full && warmup >= seg_len) return -1;
+  const int g = (nseg > 1 && !full) ? (int)warmup : 0;
   uint32_t* perm = (uint32_t*)malloc(sizeof(uint32_t) * k);
   or_qpp_perm(k, perm);
   /* E1 | E2 as held between halves | ext | E2 as computed */
@@ -1069,9 +1075,10 @@ int or_decode_window(const float* d0, const float* d1, const float* d2, uint32_t
       for (uint32_t i = 0; i < k; ++i) sa[i] = h ? (si[i] + e1[perm[i]]) : (d0[i] + e2[i]);
       memcpy(nab, ab, sizeof(float) * nb);
       for (uint32_t m = 0; m < nseg; ++m) {
-        const int ga = (m > 0) ? g : 0, gb = (m + 1 < nseg) ? g : 0;
-        float* ain = ab + ((size_t)(h * 2 + 0) * nseg + m) * 8;
-        float* bin = ab + ((size_t)(h * 2 + 1) * nseg + m) * 8;
+        const int ga = full ? (int)(m * seg_len) : ((m > 0) ? g : 0);
+        const int gb = full ? (int)((nseg - 1 - m) * seg_len) : ((m + 1 < nseg) ? g : 0);
+        float* ain = ab + ((size_t)(h * 2 + 0) * nseg + (full ? 0 : m)) * 8;
+        float* bin = ab + ((size_t)(h * 2 + 1) * nseg + (full ? nseg - 1 : m)) * 8;
         float aexp[8], bexp[8];
         seg_siso(sa + (size_t)m * seg_len, par + (size_t)m * seg_len, (int)seg_len, ga, gb, g, lin,
                  ain, bin, ex + (size_t)m * seg_len, aexp, bexp, bst);

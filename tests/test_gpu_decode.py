@@ -71,6 +71,86 @@ def test_exact_batched_equals_single_calls():
     assert list(its) == [6] * len(ks)
 
 
+# -------------------------------------- exact decoders: segmented == serial
+def _table_ks():
+    return [int(e[0]) for e in T.qpp_interleaver_table()]
+
+
+def _bits_equal(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
+def test_exact_serial_matches_reference_golden():
+    """The serial form (one recursion per block, behind the tile8 entry point)
+    against the reference's golden posteriors: 1e-4 absolute, hard decisions
+    equal where |LLR| >= 1e-3."""
+    g = golden()
+    for k, eb, it in GOLDEN_DECODES:
+        tag = f"dec_k{k}_eb{eb}_it{it}"
+        _say("serial", tag)
+        llr = g[tag + "_llr"]
+        post, _, used = T.turbo_decode_code_block_gpu_stage1_tile8_experimental(T.CodeBlockDescriptor(0, k, k + 4),
+                                                                                llr[0], llr[1], llr[2], it, False)
+        assert used == it
+        assert_llr_parity(post, g[tag + "_post"], atol=1e-4)
+
+
+def test_exact_segmented_equals_serial_all_sizes():
+    """TC_DECODER_EXACT (segment-parallel recursions with boundaries fixed up
+    until every segment meets its checkpoints bit for bit) and
+    TC_DECODER_EXACT_SERIAL (one full-length recursion per block) give the
+    same posteriors bit for bit: one block of each of the 188 QPP sizes at
+    Eb/N0 0..2 dB, 8 iterations, one batch each."""
+    ks = _table_ks()
+    rng = np.random.default_rng(188)
+    llrs = []
+    for k in ks:
+        cb = rng.integers(0, 2, k).astype(np.uint8)
+        llrs.append(np.concatenate(bpsk_awgn_llr(O.encode_cb(cb), k, float(rng.uniform(0.0, 2.0)), rng)))
+    llr = np.concatenate(llrs)
+    _say("exact segmented vs serial: 188 sizes")
+    seg, _, its = T.decode_batch_host(ks, llr, T.DECODER_EXACT, 8)
+    ser, _, its2 = T.decode_batch_host(ks, llr, T.DECODER_EXACT_SERIAL, 8)
+    assert _bits_equal(seg, ser)
+    assert list(its) == [8] * len(ks) and list(its2) == [8] * len(ks)
+
+
+@pytest.mark.parametrize("ebn0", [-0.5, 0.0, 0.5, 1.0])
+def test_exact_segmented_equals_serial_k6144(ebn0):
+    """64 blocks of K=6144 (64 segments of 96 steps each) from below the
+    decoding threshold to above it, where boundary fix-ups need the most
+    rounds: bit for bit the serial decoder's posteriors."""
+    bits, llr = random_blocks(64, 6144, ebn0, 6144 + int(100 * ebn0), O.encode_cb)
+    _say(f"exact segmented vs serial K=6144 ebn0={ebn0}")
+    seg, _, _ = T.decode_batch_host([6144] * 64, llr, T.DECODER_EXACT, 8)
+    ser, _, _ = T.decode_batch_host([6144] * 64, llr, T.DECODER_EXACT_SERIAL, 8)
+    assert _bits_equal(seg, ser)
+
+
+def test_exact_segmented_equals_serial_early_stop():
+    """With CRC24B early stop on mixed K (several blocks per wave for K <= 64,
+    fillers): posteriors, CRC verdicts and stopping iterations equal the
+    serial decoder's, and the verdicts the CPU reference decoder's."""
+    rng = np.random.default_rng(96)
+    table = _table_ks()
+    ks = [6144] * 8 + [40] * 9 + [int(x) for x in rng.choice(table, 47)]
+    fs = [8 if i % 5 == 0 and k > 64 else 0 for i, k in enumerate(ks)]
+    eb = [float(x) for x in rng.uniform(-0.5, 2.5, len(ks))]
+    _, llrs = _crc_blocks(ks, fs, eb, 96)
+    llr = np.concatenate(llrs)
+    _say("exact segmented vs serial, early stop:", len(ks), "blocks")
+    seg, c1, i1 = T.decode_batch_host(ks, llr, T.DECODER_EXACT, 8, True, fillers=fs)
+    ser, c2, i2 = T.decode_batch_host(ks, llr, T.DECODER_EXACT_SERIAL, 8, True, fillers=fs)
+    assert _bits_equal(seg, ser)
+    assert np.array_equal(c1, c2) and np.array_equal(i1, i2)
+    assert 0 < c1.sum() < len(ks)
+    for j in range(0, len(ks), 9):  # verdicts of the reference decoder on the CPU (a sample)
+        k, f, l = ks[j], fs[j], llrs[j]
+        _, ok, used = T.turbo_decode_code_block_cpu(T.CodeBlockDescriptor(0, k, k + 4, filler_count=f), *_split(l, k),
+                                                    8, True)
+        assert (bool(c1[j]), int(i1[j])) == (ok, used), j
+
+
 # ------------------------------------------------------------ window decoders
 def _window_bar(geo, ref):
     """(atol, rtol, hard_tol) of window decoder vs oracle.  exp2/log2
@@ -108,6 +188,41 @@ def test_window_decoder_matches_oracle(k, ebn0, decoder):
         atol, rtol, hard = _window_bar(geo, ref)
         assert_llr_parity(post[i * k:(i + 1) * k], ref, atol=atol, rtol=rtol, hard_tol=hard)
     assert list(its) == [8, 8, 8]
+
+
+@pytest.mark.parametrize("early_stop", [False, True])
+@pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])
+def test_window_fp32_held_all_sizes_match_oracle(decoder, early_stop):
+    """Extrinsics held in fp32 between half-iterations (extrinsic_bits=32):
+    one block of each of the 188 QPP sizes (Eb/N0 0..2.5 dB) against the
+    oracle's restatement with 32-bit held extrinsics, at the 32-bit bar of
+    _window_bar: 1e-3 absolute + 1e-4 relative, hard decisions equal where
+    |LLR| >= 1e-3.  With early stop (CRC24B blocks) the stopping iteration and
+    the verdict equal the oracle's."""
+    import concurrent.futures as cf
+    ks = _table_ks()
+    rng = np.random.default_rng(32 + decoder + 2 * early_stop)
+    eb = [float(x) for x in rng.uniform(0.0, 2.5, len(ks))]
+    _, llrs = _crc_blocks(ks, [0] * len(ks), eb, 320 + decoder)
+    _say(f"fp32-held window decoder={decoder} es={early_stop}: 188 sizes")
+    post, crc, its = T.decode_batch_host(ks, np.concatenate(llrs), decoder, 8, early_stop, extrinsic_bits=32)
+    lin = decoder == T.DECODER_WINDOW_LIN
+
+    def ref(i):
+        k = ks[i]
+        geo = T.segment_geometry(k, decoder, extrinsic_bits=32)
+        assert geo["extrinsic_bits"] == 32
+        return O.decode_window(*_split(llrs[i], k), k, geo["segment_length"], geo["warmup"], lin, 0, 8, early_stop, 32)
+
+    with cf.ThreadPoolExecutor(8) as ex:
+        refs = list(ex.map(ref, range(len(ks))))
+    off = 0
+    for i, k in enumerate(ks):
+        r, ok, used = refs[i]
+        assert_llr_parity(post[off:off + k], r, atol=1e-3, rtol=1e-4, hard_tol=1e-3)
+        if early_stop:
+            assert (bool(crc[i]), int(its[i])) == (ok, used), (i, k)
+        off += k
 
 
 @pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])

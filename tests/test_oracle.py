@@ -513,3 +513,26 @@ def test_exact_segmented_equals_serial(k, seg):
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (ebn0, es)
         assert (ok2, used2) == (ok, used)
         assert work[0] == 2 * used and work[1] >= work[0]
+
+
+@pytest.mark.parametrize("linear", [False, True])
+@pytest.mark.parametrize("k,seg", [(544, 32), (1024, 32), (6144, 96)])
+def test_window_full_warmup_equals_one_segment(k, seg, linear):
+    """or_decode_window with warm-ups that reach both block ends (warmup >=
+    K: alpha from alpha_0 at step 0, beta from the tail beta at step K) makes
+    every segment compute the full-length recursions, so the multi-segment
+    decode equals the one-segment decode bit for bit, for fp32- and
+    fp16-held extrinsics and with early stop: an independent check of the
+    oracle's segment offsets, per-segment interleaving and extrinsic
+    assembly (the one-segment decode itself equals the exact decoder to
+    rounding, test_window_single_segment_equals_exact)."""
+    rng = np.random.default_rng(k + seg + linear)
+    payload = rng.integers(0, 2, k - 24).astype(np.uint8)
+    cb = O.crc_append(O.CRC24B, payload)
+    for ebn0, es in ((0.25, False), (1.0, True)):
+        d0, d1, d2 = bpsk_awgn_llr(O.encode_cb(cb), k, ebn0, rng)
+        for ex_bits in (32, 16):
+            one = O.decode_window(d0, d1, d2, k, k, 0, linear, 0, 8, es, ex_bits)
+            many = O.decode_window(d0, d1, d2, k, seg, k, linear, 0, 8, es, ex_bits)
+            assert np.array_equal(many[0].view(np.uint32), one[0].view(np.uint32)), (ebn0, ex_bits)
+            assert many[1:] == one[1:]

@@ -40,6 +40,7 @@ CASES = [
     # decoder, early stop, (K, Eb/N0) of the first and the second (smaller) call
     (T.DECODER_EXACT, False, (6144, 0.5), (1024, 0.5)),
     (T.DECODER_EXACT, True, (1024, 1.5), (40, 3.0)),
+    (T.DECODER_EXACT_SERIAL, False, (6144, 0.5), (544, 0.5)),
     (T.DECODER_WINDOW_LOG, False, (6144, 0.5), (544, 0.5)),
     (T.DECODER_WINDOW_LOG, True, (6144, 1.5), (104, 3.0)),
     (T.DECODER_WINDOW_LIN, True, (2048, 1.0), (352, 2.0)),

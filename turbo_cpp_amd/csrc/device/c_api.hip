@@ -23,9 +23,15 @@ namespace tcdev {
 __global__ void exact_decode_kernel(const CbInfo*, const uint32_t*, uint32_t, const float*, float*, float*, int32_t*,
                                     uint32_t*, uint32_t, int32_t);
 __global__ void window_decode_log_kernel(WinArgs);
-__global__ void window_decode_lin_kernel(WinArgs);
 __global__ void window_decode_log_es_kernel(WinArgs);
+__global__ void window_decode_log_f32_kernel(WinArgs);
+__global__ void window_decode_log_es_f32_kernel(WinArgs);
+__global__ void window_decode_lin_kernel(WinArgs);
 __global__ void window_decode_lin_es_kernel(WinArgs);
+__global__ void window_decode_lin_f32_kernel(WinArgs);
+__global__ void window_decode_lin_es_f32_kernel(WinArgs);
+__global__ void window_decode_exact_kernel(WinArgs);
+__global__ void window_decode_exact_es_kernel(WinArgs);
 }  // namespace tcdev
 
 namespace {
@@ -37,8 +43,16 @@ tc_options normalised(const tc_options* o) {
   if (o) r = *o;
   if (r.min_segment == 0) r.min_segment = 32;
   if (r.warmup == 0) r.warmup = 32;
-  if (r.decoder < TC_DECODER_EXACT || r.decoder > TC_DECODER_WINDOW_LIN)
+  if (r.decoder < TC_DECODER_EXACT || r.decoder > TC_DECODER_EXACT_SERIAL)
     throw Failure{TC_ERR_INVALID_ARGUMENT, "unknown decoder"};
+  // extrinsics held between half-iterations: the window decoders default to
+  // fp16, the exact decoders hold them in fp32 (the reference's width)
+  const bool exact = r.decoder == TC_DECODER_EXACT || r.decoder == TC_DECODER_EXACT_SERIAL;
+  if (r.extrinsic_bits == 0) r.extrinsic_bits = exact ? 32 : tcdev::kExtrinsicBits;
+  if (r.extrinsic_bits != 16 && r.extrinsic_bits != 32)
+    throw Failure{TC_ERR_INVALID_ARGUMENT, "extrinsic_bits must be 16 or 32"};
+  if (exact && r.extrinsic_bits != 32)
+    throw Failure{TC_ERR_INVALID_ARGUMENT, "the exact decoders hold extrinsics in fp32"};
   return r;
 }
 
@@ -58,16 +72,29 @@ tc_segment_info geometry(uint32_t k, const tc_options& o) {
   g.segment_length = s;
   g.segments = k / s;
   g.blocks_per_wave = tcdev::kWaveLanes / g.segments;
-  g.warmup = (g.segments > 1) ? std::min(o.warmup, s - 1) : 0;
-  g.extrinsic_bits = tcdev::kExtrinsicBits;
+  // the exact decoder's boundaries are fixed up exactly, with no warm-up
+  g.warmup = (g.segments > 1 && o.decoder != TC_DECODER_EXACT) ? std::min(o.warmup, s - 1) : 0;
+  g.extrinsic_bits = o.extrinsic_bits;
   return g;
 }
 
 using WinKernel = void (*)(tcdev::WinArgs);
 WinKernel window_kernel(const tc_options& o) {
-  if (o.decoder == TC_DECODER_WINDOW_LOG)
-    return o.early_stop ? tcdev::window_decode_log_es_kernel : tcdev::window_decode_log_kernel;
-  return o.early_stop ? tcdev::window_decode_lin_es_kernel : tcdev::window_decode_lin_kernel;
+  const bool es = o.early_stop != 0, f32 = o.extrinsic_bits == 32;
+  switch (o.decoder) {
+    case TC_DECODER_EXACT:
+      return es ? tcdev::window_decode_exact_es_kernel : tcdev::window_decode_exact_kernel;
+    case TC_DECODER_WINDOW_LOG:
+      if (f32) return es ? tcdev::window_decode_log_es_f32_kernel : tcdev::window_decode_log_f32_kernel;
+      return es ? tcdev::window_decode_log_es_kernel : tcdev::window_decode_log_kernel;
+    default:
+      if (f32) return es ? tcdev::window_decode_lin_es_f32_kernel : tcdev::window_decode_lin_f32_kernel;
+      return es ? tcdev::window_decode_lin_es_kernel : tcdev::window_decode_lin_kernel;
+  }
+}
+
+size_t lds_bytes_of(uint32_t seg, const tc_options& o) {
+  return static_cast<size_t>(tcdev::job_lds_bytes(static_cast<int>(seg), o.extrinsic_bits / 8));
 }
 
 }  // namespace
@@ -170,7 +197,7 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   p.post_floats = post;
   p.d_cbs = dev_alloc<tcdev::CbInfo>(n);
   dev_upload(p.d_cbs, p.cbs);
-  if (o.decoder == TC_DECODER_EXACT) {
+  if (o.decoder == TC_DECODER_EXACT_SERIAL) {
     p.d_work = dev_alloc<float>(work);
     // groups of up to kExactBlocksPerWave blocks of one K, one wave each
     constexpr size_t B = tcdev::kExactBlocksPerWave;
@@ -216,7 +243,7 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   p.njobs = static_cast<uint32_t>(jobs.size());
   p.lds_bytes = 0;
   for (const auto& j : jobs)
-    p.lds_bytes = std::max<size_t>(p.lds_bytes, static_cast<size_t>(tcdev::job_lds_bytes(static_cast<int>(j.seg))));
+    p.lds_bytes = std::max<size_t>(p.lds_bytes, lds_bytes_of(j.seg, o));
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
   int per_cu = 0;
@@ -246,7 +273,7 @@ void launch(const PlanDev& p, const float* llr, float* post, int32_t* crc, uint3
             const tcdev::RxIn* rx = nullptr, const tcdev::TbOut* tb = nullptr) {
   if (p.n_cb == 0) return;
   const tc_options& o = p.opts;
-  if (o.decoder == TC_DECODER_EXACT) {
+  if (o.decoder == TC_DECODER_EXACT_SERIAL) {
     hipLaunchKernelGGL(tcdev::exact_decode_kernel, dim3(p.n_groups), dim3(64), 0, s, p.d_cbs, p.d_groups,
                        p.n_groups, llr, post, p.d_work, crc, iters, o.max_iterations, o.early_stop);
   } else {
@@ -265,7 +292,7 @@ void launch(const PlanDev& p, const float* llr, float* post, int32_t* crc, uint3
 
 bool tchost::launch_window(tc_plan* plan, const float* llr, float* post, int32_t* crc_ok, uint32_t* iters,
                            hipStream_t stream, const tcdev::RxIn* rx, const tcdev::TbOut* tb) {
-  if (plan->opts.decoder == TC_DECODER_EXACT) return false;
+  if (plan->opts.decoder == TC_DECODER_EXACT_SERIAL) return false;
   launch(*plan, llr, post, crc_ok, iters, stream, rx, tb);
   return true;
 }
@@ -316,7 +343,7 @@ void tchost::decode_one_block(uint8_t* area, size_t area_bytes, const uint32_t* 
   };
   p.d_cbs = reinterpret_cast<tcdev::CbInfo*>(at(&h.cb));
   size_t work_bytes = 0;
-  if (o.decoder == TC_DECODER_EXACT) {
+  if (o.decoder == TC_DECODER_EXACT_SERIAL) {
     work_bytes = exact_work_floats(k) * sizeof(float);
     h.groups[0] = 0;
     for (uint32_t b = 1; b < tcdev::kExactBlocksPerWave; ++b) h.groups[b] = tcdev::kNoBlock;
@@ -337,7 +364,7 @@ void tchost::decode_one_block(uint8_t* area, size_t area_bytes, const uint32_t* 
     p.d_scratch = reinterpret_cast<float2*>(area + kOneBlockWorkOff);
     p.njobs = 1;
     p.grid = 1;
-    p.lds_bytes = static_cast<size_t>(tcdev::job_lds_bytes(static_cast<int>(g.segment_length)));
+    p.lds_bytes = lds_bytes_of(g.segment_length, o);
   }
   if (kOneBlockWorkOff + work_bytes > area_bytes)
     throw Failure{TC_ERR_INVALID_ARGUMENT, "code block needs " + std::to_string(kOneBlockWorkOff + work_bytes) +
@@ -358,7 +385,7 @@ void check_status(tc_status st) {
 // Per-thread cache of single-block plans and staging buffers, so repeated
 // per-code-block calls do not allocate.
 struct OneBlockCache {
-  std::map<std::tuple<uint32_t, uint32_t, int32_t, uint32_t, uint32_t, uint32_t, int32_t, uint32_t>,
+  std::map<std::tuple<uint32_t, uint32_t, int32_t, uint32_t, uint32_t, uint32_t, int32_t, uint32_t, uint32_t>,
            std::unique_ptr<tc_plan>>
       plans;
   float* d_llr = nullptr;
@@ -439,9 +466,9 @@ tc_status tc_plan_decode_rx(tc_plan* plan, const tc_derate_plan* derate, uint32_
     x.rank = derate->d_rank;
     x.dcb = derate->d_cbs;
     const auto s = static_cast<hipStream_t>(hip_stream);
-    if (plan->opts.decoder == TC_DECODER_EXACT) {
-      // no fused staging in the exact decoder: de-rate into keep, then decode it
-      if (!keep_dev) throw Failure{TC_ERR_INVALID_ARGUMENT, "the exact decoder needs keep"};
+    if (plan->opts.decoder == TC_DECODER_EXACT_SERIAL) {
+      // no fused staging in the serial exact decoder: de-rate into keep, then decode it
+      if (!keep_dev) throw Failure{TC_ERR_INVALID_ARGUMENT, "the serial exact decoder needs keep"};
       for (uint32_t t = 0; t < n_tx; ++t)
         check_status(tc_derate_plan_run(const_cast<tc_derate_plan*>(derate), rx_dev[t], rv ? rv[t] : -1,
                                         (t > 0 || accumulate) ? 1 : 0, keep_dev, hip_stream));
@@ -496,7 +523,7 @@ tc_status tc_decode_code_block(const tc_cb_desc* desc, const float* d0, size_t n
     int dev = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     auto key = std::make_tuple(k, f, o.decoder, o.max_iterations, o.min_segment, o.warmup * 2 + (o.early_stop != 0), dev,
-                               o.waves_per_cu);
+                               o.waves_per_cu, o.extrinsic_bits);
     auto& slot = g_one.plans[key];
     if (!slot) {
       slot = std::make_unique<tc_plan>();

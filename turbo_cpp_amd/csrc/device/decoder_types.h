@@ -35,10 +35,11 @@ constexpr int kMaxSeg = kSubWin * kMaxSubWins;
 constexpr int kExRow = kWaveLanes;  // LDS extrinsic row: one entry per lane
 // Per-wave scratch slot, in float2: staged channel LLRs for both halves
 // ([t][lane], S <= kMaxSeg), then in floats the boundary metrics
-// [lane][half][16], the beta checkpoints [lane][w][8] and the posteriors
-// [t][lane] plus one dummy slot per lane.
+// [lane][half][16], the beta checkpoints [lane][w][8], the posteriors
+// [t][lane] plus one dummy slot per lane, and the alpha checkpoints
+// [lane][w][8] (exact segment boundaries).
 constexpr int kSlotFloat2 = 2 * kMaxSeg * kWaveLanes + 16 * kWaveLanes + 4 * kMaxSubWins * kWaveLanes +
-                            (kMaxSeg + 1) * kWaveLanes / 2;
+                            (kMaxSeg + 1) * kWaveLanes / 2 + 4 * kMaxSubWins * kWaveLanes;
 
 struct WinJob {
   uint32_t k;
@@ -95,13 +96,15 @@ struct WinArgs {
   uint32_t* sched;
 };
 constexpr int kCrcPowLen = 6144;
-// LDS of one window job: fp16 extrinsics [S][64] plus one dummy slot per
-// lane, a hard-decision bitmap over the same index space, and one CRC
-// partial per lane.  At S = 96 this is 13.3 KiB, so LDS does not keep a CU
-// below the eight waves (two per SIMD) that the register budget allows.
-constexpr int job_lds_bytes(int s) {
-  return (s * kExRow + kWaveLanes) * 2 + (s * kExRow + kWaveLanes) / 8 + kWaveLanes * 4;
+// LDS of one window job: the extrinsics held between half-iterations
+// [S][64] plus one dummy slot per lane (ex_bytes each: 2 for fp16, 4 for
+// fp32), a hard-decision bitmap over the same index space, and one CRC
+// partial per lane.  At S = 96 with fp16 this is 13.3 KiB, so LDS does not
+// keep a CU below the eight waves (two per SIMD) that the register budget
+// allows; fp32 (25.7 KiB) allows six.
+constexpr int job_lds_bytes(int s, int ex_bytes) {
+  return (s * kExRow + kWaveLanes) * ex_bytes + (s * kExRow + kWaveLanes) / 8 + kWaveLanes * 4;
 }
-constexpr int kExtrinsicBits = 16;
+constexpr int kExtrinsicBits = 16;  // window decoders' default held width
 
 }  // namespace tcdev

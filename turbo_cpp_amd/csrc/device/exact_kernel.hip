@@ -1,10 +1,12 @@
-// Exact Log-MAP turbo decoder: full-length forward and backward recursions
-// with no windowing, evaluated in the reference's operation order
-// (lte_turbo.cpp:380-486, 736-813) so that posterior LLRs agree with
-// turbo_decode_code_block_cpu to rounding of expf/log1pf.
+// Exact Log-MAP turbo decoder, serial form: full-length forward and backward
+// recursions, one per code block, evaluated in the reference's operation
+// order (lte_turbo.cpp:380-486, 736-813) so that posterior LLRs agree with
+// turbo_decode_code_block_cpu to the rounding of max* (jac_ref against
+// expf/log1pf).
 //
-// This is the parity path behind GpuStage1Exact; the window kernel is the
-// throughput path.
+// TC_DECODER_EXACT_SERIAL and the tile8 entry point run it; the
+// segment-parallel exact decoder (window_kernel.h, siso_half_exact) computes
+// the same metrics bit for bit and backs GpuStage1Exact.
 //
 // Layout: one wave decodes up to kExactBlocksPerWave code blocks of one K,
 // kExactLanes lanes per block; the first eight lanes of a block carry one
@@ -59,16 +61,12 @@ __device__ __forceinline__ void share_simd() {
 __device__ __forceinline__ bool fin(float v) { return __builtin_isfinite(v); }
 
 // max* with the reference's infinity rules (lte_turbo.cpp:380-390): y when
-// x is not finite, else x when y is not finite, else hi + log1pf(expf(lo -
-// hi)).  Written without branches (the Jacobian term is evaluated in every
-// case and selected away when an input is not finite) so that the alpha and
-// beta steps of a loop iteration, two independent chains of ~80 dependent
-// instructions each, can interleave; the selected value is the reference's
-// in every case.
+// x is not finite, else x when y is not finite, else jac_ref (trellis.h).
+// Written without branches (the Jacobian term is evaluated in every case and
+// selected away when an input is not finite) so that the alpha and beta steps
+// of a loop iteration, two independent chains, can interleave.
 __device__ __forceinline__ float jac(float x, float y) {
-  const float hi = (x < y) ? y : x;
-  const float lo = (y < x) ? y : x;
-  const float r = hi + log1pf(expf(lo - hi));
+  const float r = jac_ref(x, y);
   return !fin(x) ? y : (!fin(y) ? x : r);
 }
 

@@ -248,7 +248,7 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
   p.bits_total = bits;
   const tc_options o = opts ? *opts : tc_options{8, 0, TC_DECODER_WINDOW_LOG, 0, 0, 0};
   p.max_iterations = o.max_iterations;
-  p.exact = o.decoder == TC_DECODER_EXACT;
+  p.exact = o.decoder == TC_DECODER_EXACT_SERIAL;
   check_status(tc_derate_plan_create(p.ks.data(), p.fs.data(), p.es.data(), rvs.data(), p.n_cb, &p.derate));
   check_status(tc_plan_create(p.ks.data(), p.fs.data(), p.n_cb, &o, &p.decoder));
   if (n_tb == 0) return;

@@ -37,6 +37,18 @@ __host__ __device__ constexpr unsigned term_in(unsigned s) { return ((s >> 1) ^ 
 __device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float hw_log2(float x) { return __builtin_amdgcn_logf(x); }
 
+// max*(x, y) = hi + log1p(exp(lo - hi)) of two natural-log metrics (the
+// reference's max_star for finite operands, lte_turbo.cpp:380-390), evaluated
+// as hi + ln2 * log2(1 + 2^((lo - hi) log2e)) on v_exp_f32 / v_log_f32.  Both
+// exact decoders use this one function, so their metrics agree bit for bit;
+// against the reference's expf/log1pf it differs by rounding (~1e-7 per call).
+__device__ __forceinline__ float jac_ref(float x, float y) {
+#pragma clang fp contract(off)
+  const float hi = (x < y) ? y : x;
+  const float lo = (y < x) ? y : x;
+  return hi + hw_log2(1.0f + hw_exp2((lo - hi) * kLog2e)) * kLn2;
+}
+
 // Branch metric of input u and parity c from g1 = x + y, g2 = x - y
 // (x = (Ls+La)/2, y = Lp/2): +-g1 for (1,1)/(0,0), +-g2 for (1,0)/(0,1).
 __device__ __forceinline__ float gsel(unsigned u, unsigned c, float g1, float g2) {

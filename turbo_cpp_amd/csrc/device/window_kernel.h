@@ -42,6 +42,8 @@
 // Arithmetic: either max* in the log domain or the probability domain with
 // power-of-two renormalisation; oracle/turbo_oracle.c (or_decode_window)
 // states both step by step and the test suite compares the two.
+#pragma once
+
 #include <hip/hip_runtime.h>
 
 #include "decoder_types.h"
@@ -105,6 +107,7 @@ struct LogDomain {
   static constexpr float kUniform = 0.0f;
   static constexpr float kZero = kNeg;
   static constexpr bool kShareSimd = true;  // alternate priority with the SIMD's other wave
+  static constexpr bool kExact = false;
   __device__ static __forceinline__ float from_log(float v) { return v * kLog2e; }
   // the staged parity LLR, as the branch metrics take it: scaled once per job
   // (the same product the branch metric would form at every use)
@@ -140,7 +143,8 @@ struct LogDomain {
   }
   // alpha_{t+1} from alpha_t; returns the extrinsic of step t when WANT
   template <bool WANT>
-  __device__ static __forceinline__ float astep(float* a, const Br& r, float sa, const float* bn) {
+  __device__ static __forceinline__ float astep(float* a, const Br& r, float sa, float /*sys*/, float /*ap*/,
+                                                const float* bn) {
     float na[8];
 #pragma unroll
     for (unsigned n = 0; n < 8; ++n) {
@@ -184,6 +188,7 @@ struct LinDomain {
   static constexpr float kZero = 0.0f;
   // measured slower with the alternating priority (3.23 -> 3.33 ms on 4096 x 6144)
   static constexpr bool kShareSimd = false;
+  static constexpr bool kExact = false;
   __device__ static __forceinline__ float from_log(float v) { return hw_exp2(v * kLog2e); }
   // The parity weight depends on the channel only, so it is staged once per
   // job: e^-|par| (y = par/2, y + y = par exactly) with the sign bit set
@@ -220,7 +225,8 @@ struct LinDomain {
     for (int i = 0; i < 8; ++i) b[i] = nb[i];
   }
   template <bool WANT>
-  __device__ static __forceinline__ float astep(float* a, const Br& r, float /*sa*/, const float* bn) {
+  __device__ static __forceinline__ float astep(float* a, const Br& r, float /*sa*/, float /*sys*/, float /*ap*/,
+                                                const float* bn) {
     const float q[2] = {r.q0, r.q1}, uu[2] = {r.u0, r.u1};
     float na[8];
     float s1 = 0.0f, s0 = 0.0f;
@@ -239,6 +245,75 @@ struct LinDomain {
       s1 = fmaxf(s1, kFltMin);
       s0 = fmaxf(s0, kFltMin);
       ext = kLn2 * (hw_log2(s1) - hw_log2(s0));
+    }
+    norm(na);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = na[i];
+    return ext;
+  }
+};
+
+// ------------------------------------------------ the reference's arithmetic --
+// Natural-log metrics with the operations of lte_turbo.cpp:410-486 in their
+// order: branch metric 0.5 (+-(sys + ap) +- par), candidates alpha + gamma
+// (beta + gamma) combined by max* (jac_ref: hi + log1p(exp(lo - hi))),
+// metrics renormalised by their maximum every step, the LLR of a step as the
+// sequential max* accumulation of (alpha + gamma) + beta over (state, input)
+// in the reference's order, extrinsic = ((L1 - L0) - sys) - ap.  A finite
+// stand-in kNeg replaces the reference's -inf for the states alpha_0 ..
+// alpha_2 cannot reach: max* with it returns the other operand (its exp2
+// underflows to 0) and subtracting the maximum leaves it a stand-in, so every
+// finite metric, and every extrinsic, is the one the reference's rules give
+// with this max*.  Segment boundaries are exact (siso_half_exact).
+struct ExactDomain {
+  struct Br {
+    float g1, g2;  // 0.5 (sa + par), 0.5 (sa - par): gamma = +-g1 for (u,c) = (1,1)/(0,0), +-g2 for (1,0)/(0,1)
+  };
+  static constexpr float kUniform = 0.0f;
+  static constexpr float kZero = kNeg;
+  static constexpr bool kShareSimd = true;
+  static constexpr bool kExact = true;
+  __device__ static __forceinline__ float from_log(float v) { return v; }
+  __device__ static __forceinline__ float stage_par(float par) { return par; }
+  __device__ static __forceinline__ Br branch(float sa, float par) { return {0.5f * (sa + par), 0.5f * (sa - par)}; }
+  __device__ static __forceinline__ void norm(float* v) {
+    const float m = max8(v);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = v[i] - m;
+  }
+  __device__ static __forceinline__ void bstep(float* b, const Br& r) {
+    float nb[8];
+#pragma unroll
+    for (unsigned p = 0; p < 8; ++p) {
+      const Tr t0 = rsc(p, 0), t1 = rsc(p, 1);
+      nb[p] = jac_ref(b[t0.next] + gsel(0, t0.parity, r.g1, r.g2), b[t1.next] + gsel(1, t1.parity, r.g1, r.g2));
+    }
+    norm(nb);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b[i] = nb[i];
+  }
+  template <bool WANT>
+  __device__ static __forceinline__ float astep(float* a, const Br& r, float /*sa*/, float sys, float ap,
+                                                const float* bn) {
+    float ext = 0.0f;
+    if constexpr (WANT) {
+      float l[2];
+#pragma unroll
+      for (unsigned s = 0; s < 8; ++s) {
+#pragma unroll
+        for (unsigned u = 0; u < 2; ++u) {
+          const Tr tr = rsc(s, u);
+          const float m = (a[s] + gsel(u, tr.parity, r.g1, r.g2)) + bn[tr.next];
+          l[u] = (s == 0) ? m : jac_ref(l[u], m);
+        }
+      }
+      ext = ((l[1] - l[0]) - sys) - ap;
+    }
+    float na[8];
+#pragma unroll
+    for (unsigned n = 0; n < 8; ++n) {
+      const unsigned u = in_of(n), p = pred0(n), c = par0(n);
+      na[n] = jac_ref(a[p] + gsel(u, c, r.g1, r.g2), a[p + 1] + gsel(u, c ^ 1u, r.g1, r.g2));
     }
     norm(na);
 #pragma unroll
@@ -310,10 +385,16 @@ __device__ __forceinline__ uint32_t lds_addr(const Lane& L, const uint32_t* __re
   return lds_addr_q<H>(L, H == 1 ? qtab_at(qtab, L.qtab_off + t) : 0u, qq, t, y);
 }
 
-// Extrinsics in LDS are _Float16, saturated at +-65504 and rounded to
-// nearest even (v_cvt_f16_f32 in the default rounding mode).
-__device__ __forceinline__ _Float16 to_ex(float v) {
-  return static_cast<_Float16>(__builtin_amdgcn_fmed3f(v, -65504.0f, 65504.0f));
+// Extrinsics held in LDS between half-iterations: X = _Float16 (saturated
+// at +-65504 and rounded to nearest even, v_cvt_f16_f32 in the default
+// rounding mode) or float (held exactly).
+template <class X>
+__device__ __forceinline__ X to_ex(float v) {
+  if constexpr (sizeof(X) == 2) {
+    return static_cast<X>(__builtin_amdgcn_fmed3f(v, -65504.0f, 65504.0f));
+  } else {
+    return v;
+  }
 }
 
 __device__ __forceinline__ uint32_t y_back(uint32_t y, uint32_t b, uint32_t m) { return (y >= b) ? y - b : y + m - b; }
@@ -354,8 +435,8 @@ __device__ __forceinline__ void load_chunk(const float* __restrict__ sc, uint32_
 
 // LDS extrinsics of the chunk, issued in the order the pass consumes them;
 // addr (may be null) keeps the addresses for the write-back.
-template <int H, bool BACKWARD, bool FULL>
-__device__ __forceinline__ void load_chunk_lds(const Lane& L, const _Float16* __restrict__ ex,
+template <int H, bool BACKWARD, bool FULL, class X>
+__device__ __forceinline__ void load_chunk_lds(const Lane& L, const X* __restrict__ ex,
                                                const uint32_t* __restrict__ qtab, uint32_t qq, uint32_t by,
                                                uint32_t lo, uint32_t len, Chunk& c, uint32_t* addr) {
   uint32_t y = (H == 1) ? qpp_col0(L, qq, BACKWARD ? lo + len - 1 : lo) : 0u;
@@ -388,8 +469,8 @@ __device__ __forceinline__ void load_chunk_lds(const Lane& L, const _Float16* __
 // chunk c; on_hand(beta) right after step hand_t (beta then is beta_hand_t;
 // hand_t outside [lo, hi): never).  A chunk of kSubWin steps that does not
 // hold hand_t strictly inside runs without per-step guards.
-template <class D, int H, class OnChunk, class OnHand>
-__device__ __forceinline__ void beta_pass(const Lane& L, const _Float16* __restrict__ ex,
+template <class D, int H, class X, class OnChunk, class OnHand>
+__device__ __forceinline__ void beta_pass(const Lane& L, const X* __restrict__ ex,
                                           const float* __restrict__ sc, const uint32_t* __restrict__ qtab,
                                           uint32_t qq, uint32_t col, uint32_t lo, uint32_t hi, float* bc,
                                           uint32_t hand_t, OnChunk&& on_chunk, OnHand&& on_hand) {
@@ -434,8 +515,8 @@ __device__ __forceinline__ void beta_pass(const Lane& L, const _Float16* __restr
 
 // Forward recursion of alpha without extrinsics (the warm-up) over steps
 // [lo, hi) of segment qq, chunks of kSubWin aligned at lo.
-template <class D, int H>
-__device__ __forceinline__ void alpha_warmup(const Lane& L, const _Float16* __restrict__ ex,
+template <class D, int H, class X>
+__device__ __forceinline__ void alpha_warmup(const Lane& L, const X* __restrict__ ex,
                                              const float* __restrict__ sc, const uint32_t* __restrict__ qtab,
                                              uint32_t qq, uint32_t col, uint32_t lo, uint32_t hi, float* ac) {
   if (hi <= lo) return;
@@ -458,7 +539,7 @@ __device__ __forceinline__ void alpha_warmup(const Lane& L, const _Float16* __re
       for (int j = 0; j < kSubWin; ++j) {
         __builtin_amdgcn_sched_barrier(0);
         const float sa = cur.v[j].x + cur.e[j];
-        D::template astep<false>(ac, D::branch(sa, cur.v[j].y), sa, nullptr);
+        D::template astep<false>(ac, D::branch(sa, cur.v[j].y), sa, 0.0f, 0.0f, nullptr);
       }
     } else {
 #pragma unroll
@@ -466,7 +547,7 @@ __device__ __forceinline__ void alpha_warmup(const Lane& L, const _Float16* __re
         if ((uint32_t)j < len) {
           __builtin_amdgcn_sched_barrier(0);
           const float sa = cur.v[j].x + cur.e[j];
-          D::template astep<false>(ac, D::branch(sa, cur.v[j].y), sa, nullptr);
+          D::template astep<false>(ac, D::branch(sa, cur.v[j].y), sa, 0.0f, 0.0f, nullptr);
         }
       }
     }
@@ -489,8 +570,8 @@ static_assert(kExRow == 64, "extrinsic address = row * 64 + column");
 // extrinsic address, or a per-lane dummy slot for lanes that must not write:
 // the fp16 extrinsic to LDS (upper half: E1 of the bit; lower: E2) and, in
 // the lower half, the fp32 posterior to post_sc.
-template <class D, int H, bool ES, bool FULL>
-__device__ __forceinline__ void alpha_subwindow(const Lane& L, _Float16* __restrict__ ex, const Chunk& cur,
+template <class D, int H, bool ES, bool FULL, class X>
+__device__ __forceinline__ void alpha_subwindow(const Lane& L, X* __restrict__ ex, const Chunk& cur,
                                                 const uint32_t* addr, float* bb, float* ac, uint32_t t0,
                                                 uint32_t len, uint32_t hand_t, float* __restrict__ nii,
                                                 uint32_t dummy, bool wr, float* __restrict__ post_sc,
@@ -522,9 +603,9 @@ __device__ __forceinline__ void alpha_subwindow(const Lane& L, _Float16* __restr
     if (FULL || (uint32_t)j < len) {
       if (!FULL && t0 + j == hand_t) hand();
       __builtin_amdgcn_sched_barrier(0);  // keep steps in order: bounds register pressure
-      const float ext = D::template astep<true>(ac, br[j], isa[j], bs[j]);
+      const float ext = D::template astep<true>(ac, br[j], isa[j], cur.v[j].x, cur.e[j], bs[j]);
       const uint32_t a = wr ? addr[j] : dummy;
-      ex[a] = to_ex(ext);
+      ex[a] = to_ex<X>(ext);
       if constexpr (H == 1) {
         const float p = isa[j] + ext;
         post_sc[a] = p;
@@ -536,13 +617,58 @@ __device__ __forceinline__ void alpha_subwindow(const Lane& L, _Float16* __restr
   }
 }
 
+// Per sub-window of this lane's segment, from alpha ac at its first step:
+// recompute beta from the checkpoint ck[w] (beta after the sub-window's last
+// step), then alpha + extrinsic per step (alpha_subwindow).  hand_t: the step
+// whose alpha goes to the next segment's record (kNever: none).
+template <class D, int H, bool ES, class X>
+__device__ __forceinline__ void extrinsic_pass(const Lane& L, X* __restrict__ ex, const float* __restrict__ sc,
+                                               const uint32_t* __restrict__ qtab, const float* __restrict__ ck,
+                                               float* ac, uint32_t hand_t, float* __restrict__ nii,
+                                               float* __restrict__ post_sc, const EarlyStop& es) {
+  const uint32_t S = L.S;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
+  const uint32_t dummy = S * kExRow + lane;
+  const bool wr = L.active && !es.done;
+  // global loads (staged LLRs, beta checkpoint) run one sub-window ahead;
+  // the LDS extrinsics of a sub-window are read at its start
+  Chunk nxt;
+  float nck[8];
+  load_chunk(sc, L.le, 0, min((uint32_t)kSubWin, S), nxt);
+  load8(ck, nck);
+  for (uint32_t w = 0; w < L.nW; ++w) {
+    if constexpr (D::kShareSimd) share_simd();
+    const uint32_t t0 = w * kSubWin;
+    const uint32_t len = min((uint32_t)kSubWin, S - t0);
+    Chunk cur;
+#pragma unroll
+    for (int j = 0; j < kSubWin; ++j) cur.v[j] = nxt.v[j];
+    float bb[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) bb[i] = nck[i];
+    if (w + 1 < L.nW) {
+      load_chunk(sc, L.le, t0 + kSubWin, min((uint32_t)kSubWin, S - t0 - kSubWin), nxt);
+      load8(ck + 8 * (w + 1), nck);
+    }
+    uint32_t addr[kSubWin];
+    if (len == kSubWin && (hand_t <= t0 || hand_t >= t0 + kSubWin)) {
+      load_chunk_lds<H, true, true>(L, ex, qtab, L.q, by, t0, len, cur, addr);
+      alpha_subwindow<D, H, ES, true>(L, ex, cur, addr, bb, ac, t0, len, hand_t, nii, dummy, wr, post_sc, es);
+    } else {
+      load_chunk_lds<H, true, false>(L, ex, qtab, L.q, by, t0, len, cur, addr);
+      alpha_subwindow<D, H, ES, false>(L, ex, cur, addr, bb, ac, t0, len, hand_t, nii, dummy, wr, post_sc, es);
+    }
+  }
+}
+
 // One SISO half (H = 0 upper, 1 lower) for this lane's segment.
 // nii: this lane's boundary metrics of this half, [alpha 8 | beta 8]
 // (alpha at step -g, beta at step S+g of the segment); ck: this lane's beta
 // checkpoints at sub-window boundaries, [w][8].  The lower half also leaves
 // each bit's posterior in post_sc (fp32, at the bit's extrinsic address).
-template <class D, int H, bool ES>
-__device__ __forceinline__ void siso_half(const Lane& L, _Float16* __restrict__ ex, const float* __restrict__ sc,
+template <class D, int H, bool ES, class X>
+__device__ __forceinline__ void siso_half(const Lane& L, X* __restrict__ ex, const float* __restrict__ sc,
                                           const uint32_t* __restrict__ qtab, float* __restrict__ nii,
                                           float* __restrict__ ck, float* __restrict__ post_sc,
                                           const EarlyStop& es) {
@@ -580,39 +706,179 @@ __device__ __forceinline__ void siso_half(const Lane& L, _Float16* __restrict__ 
     alpha_warmup<D, H>(L, ex, sc, qtab, ql, col, S - g, S, ac);
     if (L.first) load8(nii, ac);
   }
-  const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
-  const uint32_t hand_t = g > 0 ? S - g : kNever;
-  const uint32_t dummy = S * kExRow + lane;
-  const bool wr = L.active && !es.done;
-  // global loads (staged LLRs, beta checkpoint) run one sub-window ahead;
-  // the LDS extrinsics of a sub-window are read at its start
-  Chunk nxt;
-  float nck[8];
-  load_chunk(sc, L.le, 0, min((uint32_t)kSubWin, S), nxt);
-  load8(ck, nck);
-  for (uint32_t w = 0; w < L.nW; ++w) {
-    if constexpr (D::kShareSimd) share_simd();
-    const uint32_t t0 = w * kSubWin;
-    const uint32_t len = min((uint32_t)kSubWin, S - t0);
-    Chunk cur;
+  extrinsic_pass<D, H, ES>(L, ex, sc, qtab, ck, ac, g > 0 ? S - g : kNever, nii, post_sc, es);
+}
+
+// ---------------------------------------------- exact segment boundaries --
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+// the 8 metrics are the same bit patterns
+__device__ __forceinline__ bool same8(const float* a, const float* b) {
+  uint32_t d = 0;
 #pragma unroll
-    for (int j = 0; j < kSubWin; ++j) cur.v[j] = nxt.v[j];
-    float bb[8];
+  for (int i = 0; i < 8; ++i) d |= __float_as_uint(a[i]) ^ __float_as_uint(b[i]);
+  return d == 0;
+}
+
+// beta down / alpha up one chunk of len <= kSubWin steps
+template <class D>
+__device__ __forceinline__ void beta_chunk(const Chunk& c, uint32_t len, float* b) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) bb[i] = nck[i];
-    if (w + 1 < L.nW) {
-      load_chunk(sc, L.le, t0 + kSubWin, min((uint32_t)kSubWin, S - t0 - kSubWin), nxt);
-      load8(ck + 8 * (w + 1), nck);
-    }
-    uint32_t addr[kSubWin];
-    if (len == kSubWin && (hand_t <= t0 || hand_t >= t0 + kSubWin)) {
-      load_chunk_lds<H, true, true>(L, ex, qtab, L.q, by, t0, len, cur, addr);
-      alpha_subwindow<D, H, ES, true>(L, ex, cur, addr, bb, ac, t0, len, hand_t, nii, dummy, wr, post_sc, es);
-    } else {
-      load_chunk_lds<H, true, false>(L, ex, qtab, L.q, by, t0, len, cur, addr);
-      alpha_subwindow<D, H, ES, false>(L, ex, cur, addr, bb, ac, t0, len, hand_t, nii, dummy, wr, post_sc, es);
+  for (int j = kSubWin - 1; j >= 0; --j) {
+    if ((uint32_t)j < len) {
+      __builtin_amdgcn_sched_barrier(0);
+      D::bstep(b, D::branch(c.v[j].x + c.e[j], c.v[j].y));
     }
   }
+}
+template <class D>
+__device__ __forceinline__ void alpha_chunk(const Chunk& c, uint32_t len, float* a) {
+#pragma unroll
+  for (int j = 0; j < kSubWin; ++j) {
+    if ((uint32_t)j < len) {
+      __builtin_amdgcn_sched_barrier(0);
+      const float sa = c.v[j].x + c.e[j];
+      D::template astep<false>(a, D::branch(sa, c.v[j].y), sa, 0.0f, 0.0f, nullptr);
+    }
+  }
+}
+
+// alpha over this lane's segment from ac, keeping ack[w] = alpha at the
+// first step of chunk w; ac ends as alpha after the segment's last step.
+template <class D, int H, class X>
+__device__ __forceinline__ void alpha_pass_ck(const Lane& L, const X* __restrict__ ex, const float* __restrict__ sc,
+                                              const uint32_t* __restrict__ qtab, float* ac, float* __restrict__ ack) {
+  const uint32_t S = L.S, n = L.nW;
+  const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
+  Chunk nxt;
+  load_chunk(sc, L.le, 0, min((uint32_t)kSubWin, S), nxt);
+  load_chunk_lds<H, false, false>(L, ex, qtab, L.q, by, 0, min((uint32_t)kSubWin, S), nxt, nullptr);
+  for (uint32_t c = 0; c < n; ++c) {
+    if constexpr (D::kShareSimd) share_simd();
+    const uint32_t clo = c * kSubWin;
+    const uint32_t len = min((uint32_t)kSubWin, S - clo);
+    const Chunk cur = nxt;
+    if (c + 1 < n) {
+      const uint32_t nlen = min((uint32_t)kSubWin, S - clo - kSubWin);
+      load_chunk(sc, L.le, clo + kSubWin, nlen, nxt);
+      load_chunk_lds<H, false, false>(L, ex, qtab, L.q, by, clo + kSubWin, nlen, nxt, nullptr);
+    }
+    store8(ack + 8 * c, ac);
+    alpha_chunk<D>(cur, len, ac);
+  }
+}
+
+// One SISO half with exact segment boundaries (the reference's recursions,
+// lte_turbo.cpp:410-486, computed segment-parallel; or_decode_exact_seg
+// states the schedule serially):
+//  1. speculation: beta down and alpha up this lane's segment from its
+//     record nii (the exact boundary metrics of the previous iteration;
+//     uniform at first; alpha_0 and the tail beta are exact), keeping
+//     checkpoints ck[w] (beta after chunk w) and ack[w] (alpha before it);
+//  2. fix-up rounds: each lane takes its neighbours' edge metrics as they
+//     stand (wave_shr / wave_shl) and, where one differs from the value its
+//     own pass started from, re-runs chunk by chunk until its metrics equal
+//     the stored checkpoint bit for bit -- from there on the recursion is the
+//     same function of the same inputs, so nothing further changes.  A lane
+//     that never meets its checkpoints has a new edge value; rounds repeat
+//     until no edge changes.  Every checkpoint then holds the metric of the
+//     full-length serial recursion;
+//  3. extrinsics per chunk from the checkpoints (extrinsic_pass).
+// The extrinsics held between halves are fp32, so the decoder computes the
+// serial decoder's numbers bit for bit.
+template <int H, bool ES, class X>
+__device__ __forceinline__ void siso_half_exact(const Lane& L, X* __restrict__ ex, const float* __restrict__ sc,
+                                                const uint32_t* __restrict__ qtab, float* __restrict__ nii,
+                                                float* __restrict__ ck, float* __restrict__ ack,
+                                                float* __restrict__ post_sc, const EarlyStop& es) {
+  using D = ExactDomain;
+  const uint32_t S = L.S, nW = L.nW;
+  constexpr uint32_t kNever = 0xFFFFFFFFu;
+  const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
+  // 1. speculation
+  float bc[8];
+  load8(nii + 8, bc);
+  beta_pass<D, H>(
+      L, ex, sc, qtab, L.q, L.le, 0, S, bc, kNever, [&](uint32_t w, float* b) { store8(ck + 8 * w, b); },
+      [](const float*) {});
+  float ac[8];
+  load8(nii, ac);
+  alpha_pass_ck<D, H>(L, ex, sc, qtab, ac, ack);
+  // 2. fix-up rounds (idle lanes and the exact ends never need one).  After
+  // round r the segments within r of an exact end hold exact metrics, so
+  // M + 1 rounds always suffice; the bound only guarantees every wave leaves.
+  const bool may_b = L.active && !L.last, may_a = L.active && !L.first;
+  for (uint32_t round = 0; round <= L.M; ++round) {
+    float b[8], a[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      b[i] = dpp_from_next_lane(bc[i]);
+      a[i] = dpp_from_prev_lane(ac[i]);
+    }
+    bool db = may_b, da = may_a;
+    for (uint32_t c = 0; c < nW; ++c) {
+      if constexpr (D::kShareSimd) share_simd();
+      const uint32_t wb = nW - 1 - c;
+      if (db) {
+        float k8[8];
+        load8(ck + 8 * wb, k8);
+        if (same8(k8, b)) {
+          db = false;
+        } else {
+          store8(ck + 8 * wb, b);
+        }
+      }
+      if (da) {
+        float k8[8];
+        load8(ack + 8 * c, k8);
+        if (same8(k8, a)) {
+          da = false;
+        } else {
+          store8(ack + 8 * c, a);
+        }
+      }
+      if (!wave_any(db || da)) break;
+      const uint32_t lb = min((uint32_t)kSubWin, S - wb * kSubWin), la = min((uint32_t)kSubWin, S - c * kSubWin);
+      Chunk xb, xa;
+      load_chunk(sc, L.le, wb * kSubWin, lb, xb);
+      load_chunk(sc, L.le, c * kSubWin, la, xa);
+      load_chunk_lds<H, true, false>(L, ex, qtab, L.q, by, wb * kSubWin, lb, xb, nullptr);
+      load_chunk_lds<H, false, false>(L, ex, qtab, L.q, by, c * kSubWin, la, xa, nullptr);
+      float nb[8], na[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        nb[i] = b[i];
+        na[i] = a[i];
+      }
+      beta_chunk<D>(xb, lb, nb);
+      alpha_chunk<D>(xa, la, na);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        b[i] = db ? nb[i] : b[i];
+        a[i] = da ? na[i] : a[i];
+      }
+    }
+    bool changed = false;
+    if (db && !same8(b, bc)) {
+      changed = true;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) bc[i] = b[i];
+    }
+    if (da && !same8(a, ac)) {
+      changed = true;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) ac[i] = a[i];
+    }
+    if (!wave_any(changed)) break;
+  }
+  // the exact edge metrics: next iteration's starting values
+  float t8[8];
+  load8(ck + 8 * (nW - 1), t8);
+  store8(nii + 8, t8);
+  load8(ack, ac);
+  store8(nii, ac);
+  // 3. extrinsics
+  extrinsic_pass<D, H, ES>(L, ex, sc, qtab, ck, ac, kNever, nii, post_sc, es);
 }
 
 // remainder test of CRC24B (generator of lte_turbo.cpp:108-111) on the
@@ -916,11 +1182,11 @@ __device__ __forceinline__ void copy_out_tiled(const Lane& L, bool any, const fl
 }
 
 // One job: stage its inputs, run every iteration, write its outputs.
-template <class D, bool ES>
-__device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _Float16* __restrict__ ex,
+template <class D, bool ES, class X>
+__device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X* __restrict__ ex,
                                            float* __restrict__ sc_up, float* __restrict__ sc_lo,
                                            float* __restrict__ nii, float* __restrict__ ck,
-                                           float* __restrict__ post_sc) {
+                                           float* __restrict__ post_sc, float* __restrict__ ack) {
   const uint32_t lane = threadIdx.x;
   const uint32_t* __restrict__ cb_index = A.cb_index;
   const CbInfo* __restrict__ cbs = A.cbs;
@@ -1051,14 +1317,18 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _F
   }
   {
     uint32_t* exw = reinterpret_cast<uint32_t*>(ex);
-    for (uint32_t i = lane; i < ex_slots / 2; i += kWaveLanes) exw[i] = 0u;
+    for (uint32_t i = lane; i < ex_slots * sizeof(X) / 4; i += kWaveLanes) exw[i] = 0u;
   }
   __syncthreads();
 
   EarlyStop es{es_on, false, bitmap};
   uint32_t used_it = A.max_iterations;
   for (uint32_t it = 0; it < A.max_iterations; ++it) {
-    siso_half<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
+    if constexpr (D::kExact) {
+      siso_half_exact<0, ES>(L, ex, sc_up, qtab, nii, ck, ack, post_sc, es);
+    } else {
+      siso_half<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
+    }
     __syncthreads();
     if (es_on) {
       for (uint32_t i = lane; i < bitmap_words; i += kWaveLanes) bitmap[i] = 0u;
@@ -1066,7 +1336,11 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _F
     }
     // every lower half leaves the posteriors in post_sc; a block that has
     // stopped no longer overwrites its own
-    siso_half<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
+    if constexpr (D::kExact) {
+      siso_half_exact<1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, ack, post_sc, es);
+    } else {
+      siso_half<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
+    }
     __syncthreads();
     if (es_on) {
       const bool ok = crc24b_job(L, bitmap, partial, A.crc_pow, ci.filler);
@@ -1120,7 +1394,7 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, _F
   __syncthreads();
 }
 
-template <class D, bool ES>
+template <class D, bool ES, class X>
 __device__ __forceinline__ void window_decode(const WinArgs& A) {
   extern __shared__ float lds[];
   const uint32_t lane = threadIdx.x;
@@ -1133,30 +1407,14 @@ __device__ __forceinline__ void window_decode(const WinArgs& A) {
   float* ck = recs + kWaveLanes * 32 + lane * (8 * kMaxSubWins);
   // posteriors [S][64] + one dummy slot per lane
   float* post_sc = recs + kWaveLanes * 32 + kWaveLanes * 8 * kMaxSubWins;
+  // alpha checkpoints [lane][w][8] (exact segment boundaries)
+  float* ack = post_sc + (kMaxSeg + 1) * kWaveLanes + lane * (8 * kMaxSubWins);
   for (uint32_t jb = next_job(A.sched); jb < A.njobs; jb = next_job(A.sched)) {
     const WinJob J = A.jobs[jb];
-    window_job<D, ES>(A, J, reinterpret_cast<_Float16*>(lds), sc_up, sc_lo, nii, ck, post_sc);
+    window_job<D, ES, X>(A, J, reinterpret_cast<X*>(lds), sc_up, sc_lo, nii, ck, post_sc, ack);
   }
   leave_queue(A.sched);
 }
 
 }  // namespace
-
-__global__ __launch_bounds__(64, 2) void window_decode_log_kernel(WinArgs args) {
-  window_decode<LogDomain, false>(args);
-}
-
-__global__ __launch_bounds__(64, 2) void window_decode_lin_kernel(WinArgs args) {
-  window_decode<LinDomain, false>(args);
-}
-
-// with CRC24B early stop
-__global__ __launch_bounds__(64, 2) void window_decode_log_es_kernel(WinArgs args) {
-  window_decode<LogDomain, true>(args);
-}
-
-__global__ __launch_bounds__(64, 2) void window_decode_lin_es_kernel(WinArgs args) {
-  window_decode<LinDomain, true>(args);
-}
-
 }  // namespace tcdev

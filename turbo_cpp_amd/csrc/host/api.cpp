@@ -62,12 +62,14 @@ std::vector<float> turbo_decode_code_block_gpu_stage2(const CodeBlockDescriptor&
   return gpu_decode(TC_DECODER_WINDOW_LOG, desc, d0, d1, d2, max_iterations, early_stop, crc_ok, iterations_used);
 }
 
-// The reference's tile8 variant computes the exact decoder's maths with a
-// different thread mapping (lte_turbo_cuda.cu:298-401); here it is the exact decoder.
+// The reference's tile8 variant computes the exact decoder's maths with one
+// lane per trellis state (lte_turbo_cuda.cu:298-401); here it is the serial
+// exact decoder, which maps lanes to states the same way (same numbers as
+// Stage1, bit for bit).
 std::vector<float> turbo_decode_code_block_gpu_stage1_tile8_experimental(
     const CodeBlockDescriptor& desc, std::span<const float> d0, std::span<const float> d1, std::span<const float> d2,
     std::uint32_t max_iterations, bool early_stop, bool* crc_ok, std::uint32_t* iterations_used) {
-  return gpu_decode(TC_DECODER_EXACT, desc, d0, d1, d2, max_iterations, early_stop, crc_ok, iterations_used);
+  return gpu_decode(TC_DECODER_EXACT_SERIAL, desc, d0, d1, d2, max_iterations, early_stop, crc_ok, iterations_used);
 }
 
 namespace {
