@@ -5,6 +5,7 @@
 // TURBO_CPP_HAS_CUDA.
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <iostream>
 #include <stdexcept>
 #include <vector>
@@ -128,6 +129,56 @@ void test_errors() {
   expect(threw, "GPU decode with wrong span size must throw");
 }
 
+// The reference's decode_transport_block runs one GPU decode per code block
+// on std::async threads (lte_turbo.cpp:941-951): eight host threads decode
+// different blocks at once through both GPU entry points; every result equals
+// the same call made alone, bit for bit.
+void test_concurrent_calls() {
+  if (!g_gpu) return;
+  constexpr int kBlocks = 8;
+  const std::size_t k = 1024, d = k + 4;
+  std::vector<std::vector<float>> llr(kBlocks);
+  std::uint32_t lcg = 12345;
+  for (int b = 0; b < kBlocks; ++b) {
+    std::vector<std::uint8_t> cb(k);
+    for (auto& x : cb) x = (lcg = lcg * 1664525U + 1013904223U) >> 31;
+    tc::CodeBlockDescriptor cd;
+    cd.k_r = k;
+    cd.d_r = d;
+    const auto enc = tc::turbo_encode_code_block(cd, cb);
+    expect(enc.systematic.size() == d && enc.parity0.size() == d && enc.parity1.size() == d, "encoder stream size");
+    llr[b].resize(3 * d);
+    for (std::size_t i = 0; i < d; ++i) {
+      const std::uint8_t bits[3] = {enc.systematic[i], enc.parity0[i], enc.parity1[i]};
+      for (int s = 0; s < 3; ++s) {
+        const float noise = static_cast<float>((lcg = lcg * 1664525U + 1013904223U) >> 8) / 16777216.0F - 0.5F;
+        llr[b][s * d + i] = (bits[s] ? 1.0F : -1.0F) * 1.5F + 3.0F * noise;  // bit 1 > 0
+      }
+    }
+  }
+  tc::CodeBlockDescriptor desc;
+  desc.k_r = k;
+  desc.d_r = d;
+  auto decode = [&](int b, bool windowed) {
+    const std::span<const float> d0(llr[b].data(), d), d1(llr[b].data() + d, d), d2(llr[b].data() + 2 * d, d);
+    return windowed ? tc::turbo_decode_code_block_gpu_stage2(desc, d0, d1, d2, 6, false)
+                    : tc::turbo_decode_code_block_gpu_stage1(desc, d0, d1, d2, 6, false);
+  };
+  std::vector<std::vector<float>> alone[2];
+  for (int w = 0; w < 2; ++w)
+    for (int b = 0; b < kBlocks; ++b) alone[w].push_back(decode(b, w == 1));
+  for (int round = 0; round < 3; ++round) {
+    std::vector<std::future<std::pair<std::vector<float>, std::vector<float>>>> jobs;
+    for (int b = 0; b < kBlocks; ++b)
+      jobs.push_back(std::async(std::launch::async, [&, b] { return std::make_pair(decode(b, false), decode(b, true)); }));
+    for (int b = 0; b < kBlocks; ++b) {
+      const auto r = jobs[b].get();
+      expect(std::memcmp(r.first.data(), alone[0][b].data(), k * sizeof(float)) == 0, "concurrent Stage1 decode differs");
+      expect(std::memcmp(r.second.data(), alone[1][b].data(), k * sizeof(float)) == 0, "concurrent Stage2 decode differs");
+    }
+  }
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -141,6 +192,7 @@ int main(int argc, char** argv) {
     roundtrip(6112, 19, 11, false);  // :257-303
     roundtrip(12000, 17, 9, false);  // :305-344
     test_errors();
+    test_concurrent_calls();
     std::cout << "All tests passed (" << (g_gpu ? "gpu" : "cpu") << ")\n";
     return 0;
   } catch (const std::exception& e) {

@@ -269,27 +269,50 @@ def test_window_decoder_is_deterministic():
 
 
 def _reference_posteriors(llr, k, iters):
+    """The reference's CPU decoder (oracle/_ref, its own lte_turbo.cpp) on all
+    host threads, or the oracle's restatement (bit-exact with it) where that
+    build is absent."""
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
     if O.ref_available():
-        return O.ref_decode_many(llr, k, iters, min(16, os.cpu_count() or 1))
-    return np.stack([O.decode_cb(*_split(l, k), k, 0, iters, False)[0] for l in llr])
+        return O.ref_decode_many(llr, k, iters, threads)
+    import concurrent.futures as cf
+    with cf.ThreadPoolExecutor(threads) as ex:
+        return np.stack(list(ex.map(lambda l: O.decode_cb(*_split(l, k), k, 0, iters, False)[0], llr)))
 
 
-@pytest.mark.parametrize("ebn0", [0.5, 0.75])
-def test_window_ber_inside_reference_interval(ebn0):
-    """BER and BLER of the window decoders lie inside the 95% interval of the
-    reference decoder on the same 128 noisy K=6144 blocks (8 iterations)."""
-    n, k = 128, 6144
+# Eb/N0 (dB) and blocks of K=6144: enough for >= 100 bit errors of the reference
+# decoder where it makes them at feasible sample sizes (its failures cluster in
+# a few blocks, ~1 in 100 blocks at 0.75-1.0 dB); at 1.25 and 1.5 dB the
+# decoders made no error in 585 blocks each (3.6 Mbit) of the benchmark's sweep,
+# so >= 100 errors would need a sample beyond a test's reach: those two points
+# are checked against the interval of the reference's (zero or few) errors on
+# 512 blocks and reported as unresolvable.
+BER_POINTS = [(0.0, 64), (0.25, 64), (0.5, 256), (0.75, 8192), (1.0, 8192), (1.25, 512), (1.5, 512)]
+
+
+@pytest.mark.parametrize("ebn0,n", BER_POINTS)
+def test_ber_inside_reference_interval(ebn0, n):
+    """BER and BLER of the exact decoder and of both window decoders (the
+    headline geometry: fp16-held extrinsics) lie inside the 95%
+    Clopper-Pearson intervals of the reference decoder's bit and block
+    errors on the same noisy K=6144 blocks, 8 iterations (BASELINE config 2's
+    Eb/N0 sweep, 0 to 1.5 dB in 0.25 dB steps)."""
+    k = 6144
     bits, llr = random_blocks(n, k, ebn0, 1234 + int(ebn0 * 100), O.encode_cb)
+    _say(f"ebn0={ebn0}: {n} blocks, reference decoder on the CPU")
     ref = _reference_posteriors(llr, k, 8)
     ref_err = ((ref >= 0) != bits).sum(1)
     lo_b, hi_b = clopper_pearson(int(ref_err.sum()), n * k)
     lo_f, hi_f = clopper_pearson(int((ref_err > 0).sum()), n)
-    _say(f"ebn0={ebn0} reference BER={ref_err.sum() / (n * k):.3g} BLER={(ref_err > 0).mean():.3g}")
+    resolvable = int(ref_err.sum()) >= 100
+    _say(f"ebn0={ebn0} reference bit errors={int(ref_err.sum())} BER={ref_err.sum() / (n * k):.3g} "
+         f"BLER={(ref_err > 0).mean():.3g}" + ("" if resolvable else " (unresolvable: fewer than 100 errors)"))
+    assert resolvable or ebn0 >= 1.25, "a point planned as resolvable made fewer than 100 reference errors"
     for dec in (T.DECODER_EXACT, T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN):
-        post, _, _ = T.decode_batch_host([k] * n, llr, dec, 8)
+        post, _, _ = T.decode_batch_host([k] * n, llr, dec, 8, min_segment=32, warmup=16)
         err = ((post.reshape(n, k) >= 0) != bits).sum(1)
         ber, bler = err.sum() / (n * k), (err > 0).mean()
-        _say(f"  decoder={dec} BER={ber:.3g} BLER={bler:.3g}")
+        _say(f"  decoder={dec} bit errors={int(err.sum())} BER={ber:.3g} BLER={bler:.3g}")
         assert lo_b <= ber <= hi_b, (dec, ber, lo_b, hi_b)
         assert lo_f <= bler <= hi_f, (dec, bler, lo_f, hi_f)
 
@@ -509,3 +532,36 @@ def test_transport_block_early_stop(backend):
     if backend == "GpuStage1Exact":
         assert dec.iterations_used == cpu.iterations_used
     assert all(dec.code_block_crc_ok)
+
+
+# ------------------------------------------------ host threads at once --
+def test_concurrent_host_threads_match_sequential():
+    """Eight host threads decode at once through the host-buffer entry points
+    (the per-block GPU calls, as the reference's decode_transport_block issues
+    them from std::async threads, lte_turbo.cpp:941-951, and the batched host
+    call).  Each thread runs on a stream of its own; every result equals the
+    same call made alone, bit for bit."""
+    import concurrent.futures as cf
+    rng = np.random.default_rng(808)
+    ks = [6144, 1024, 544, 40, 2048, 6144, 104, 3200]
+    llrs = []
+    for k in ks:
+        cb = rng.integers(0, 2, k).astype(np.uint8)
+        llrs.append(np.concatenate(bpsk_awgn_llr(O.encode_cb(cb), k, 0.75, rng)))
+
+    def work(i):
+        k, l = ks[i], llrs[i]
+        d = T.CodeBlockDescriptor(0, k, k + 4)
+        a = T.turbo_decode_code_block_gpu_stage1(d, *_split(l, k), 8, False)[0]
+        b = T.turbo_decode_code_block_gpu_stage2(d, *_split(l, k), 8, False)[0]
+        c = T.decode_batch_host([k, k], np.concatenate([l, l]), T.DECODER_WINDOW_LIN, 8)[0]
+        return a, b, c
+
+    alone = [work(i) for i in range(len(ks))]
+    _say("8 host threads decoding at once")
+    for _ in range(3):
+        with cf.ThreadPoolExecutor(len(ks)) as ex:
+            together = list(ex.map(work, range(len(ks))))
+        for x, y in zip(together, alone):
+            for u, v in zip(x, y):
+                assert _bits_equal(u, v)

@@ -64,7 +64,7 @@ def build(verbose: bool = False) -> str:
     # C++ API test program (tests/cpp/api_test.cpp) linked against the library
     test_src = os.path.join(ROOT, "tests", "cpp", "api_test.cpp")
     if os.path.exists(test_src):
-        cmd = ["g++"] + COMMON + [test_src, "-o", API_TEST, "-L", HERE, "-lturbo_cpp_amd",
+        cmd = ["g++"] + COMMON + ["-pthread", test_src, "-o", API_TEST, "-L", HERE, "-lturbo_cpp_amd",
                                   "-Wl,-rpath,$ORIGIN/.."]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -212,6 +212,7 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
     p.n_groups = static_cast<uint32_t>(groups.size() / B);
     p.d_groups = dev_alloc<uint32_t>(groups.size());
     dev_upload(p.d_groups, groups);
+    uploads_done();
     return;
   }
   // ---- window decoders: group by K, pack blocks into 64-lane jobs ----
@@ -267,6 +268,7 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   uint32_t* crc_pow = dev_alloc<uint32_t>(pw.size());
   p.d_crc_pow = crc_pow;
   dev_upload(crc_pow, pw);
+  uploads_done();
 }
 
 void launch(const PlanDev& p, const float* llr, float* post, int32_t* crc, uint32_t* iters, hipStream_t s,
@@ -382,25 +384,80 @@ void check_status(tc_status st) {
   if (st != TC_OK) throw Failure{st, turbo_cpp::detail::last_error()};
 }
 
-// Per-thread cache of single-block plans and staging buffers, so repeated
-// per-code-block calls do not allocate.
-struct OneBlockCache {
-  std::map<std::tuple<uint32_t, uint32_t, int32_t, uint32_t, uint32_t, uint32_t, int32_t, uint32_t, uint32_t>,
-           std::unique_ptr<tc_plan>>
-      plans;
-  float* d_llr = nullptr;
-  float* d_post = nullptr;
-  int32_t* d_crc = nullptr;
-  uint32_t* d_it = nullptr;
+// Device buffer that only grows (freed by its owner).
+struct GrowBuf {
+  void* p = nullptr;
   size_t cap = 0;
-  ~OneBlockCache() {
-    for (void* q : {(void*)d_llr, (void*)d_post, (void*)d_crc, (void*)d_it})
-      if (q) (void)hipFree(q);
+  template <typename T>
+  T* get(size_t n) {
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    if (cap < bytes) {
+      void* q = nullptr;
+      hip_check(hipMalloc(&q, bytes), "hipMalloc");
+      if (p) (void)hipFree(p);
+      p = q;
+      cap = bytes;
+    }
+    return static_cast<T*>(p);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
   }
 };
-thread_local OneBlockCache g_one;
+
+// Per-thread state of the host-buffer entry points: a stream of the thread's
+// own, so host threads that decode at once -- as the reference's
+// decode_transport_block does with std::async (lte_turbo.cpp:941-951) -- do
+// not wait on each other's work (each call waits for its own stream only);
+// single-block plans by shape; the last batch plan; device staging buffers
+// that only grow.  A call that repeats an earlier shape allocates nothing.
+using OneKey = std::tuple<uint32_t, uint32_t, int32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>;
+struct HostCalls {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  std::map<OneKey, std::unique_ptr<tc_plan>> one_plans;
+  std::unique_ptr<tc_plan> batch;
+  std::vector<uint32_t> batch_k, batch_f;
+  tc_options batch_o{};
+  GrowBuf llr, post, crc, it;
+  void reset() {
+    one_plans.clear();
+    batch.reset();
+    batch_k.clear();
+    batch_f.clear();
+    for (GrowBuf* g : {&llr, &post, &crc, &it}) g->release();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+    device = -1;
+  }
+  ~HostCalls() { reset(); }
+};
+thread_local HostCalls g_host;
+
+// This thread's state on the current device (set up on first use).
+HostCalls& host_calls() {
+  require_device();
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (g_host.device != dev) {
+    g_host.reset();
+    g_host.device = dev;
+  }
+  if (!g_host.stream) hip_check(hipStreamCreateWithFlags(&g_host.stream, hipStreamNonBlocking), "hipStreamCreate");
+  return g_host;
+}
+
+bool same_options(const tc_options& a, const tc_options& b) {
+  return a.max_iterations == b.max_iterations && a.early_stop == b.early_stop && a.decoder == b.decoder &&
+         a.min_segment == b.min_segment && a.warmup == b.warmup && a.waves_per_cu == b.waves_per_cu &&
+         a.extrinsic_bits == b.extrinsic_bits;
+}
 
 }  // namespace
+
+hipStream_t tchost::host_stream() { return host_calls().stream; }
 
 extern "C" {
 
@@ -483,26 +540,38 @@ tc_status tc_decode_batch_host(const uint32_t* k, const uint32_t* filler, size_t
                                const float* llr, float* post, int32_t* crc_ok, uint32_t* iterations_used) {
   return guarded([&] {
     const tc_options o = normalised(opts);
-    tc_plan p;
-    build_plan(p, k, filler, n_cb, o);
     if (n_cb == 0) return;
-    float* d_llr = dev_alloc<float>(p.llr_floats);
-    float* d_post = dev_alloc<float>(p.post_floats);
-    int32_t* d_crc = dev_alloc<int32_t>(n_cb);
-    uint32_t* d_it = dev_alloc<uint32_t>(n_cb);
-    try {
-      hip_check(hipMemcpy(d_llr, llr, p.llr_floats * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
-      launch(p, d_llr, d_post, d_crc, d_it, nullptr);
-      hip_check(hipDeviceSynchronize(), "decode");
-      hip_check(hipMemcpy(post, d_post, p.post_floats * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-      if (crc_ok) hip_check(hipMemcpy(crc_ok, d_crc, n_cb * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-      if (iterations_used)
-        hip_check(hipMemcpy(iterations_used, d_it, n_cb * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    } catch (...) {
-      (void)hipFree(d_llr), (void)hipFree(d_post), (void)hipFree(d_crc), (void)hipFree(d_it);
-      throw;
+    if (!k || !llr || !post) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+    HostCalls& h = host_calls();
+    // the last batch's plan is reused when the blocks and options repeat
+    std::vector<uint32_t> fs(n_cb, 0u);
+    if (filler) fs.assign(filler, filler + n_cb);
+    if (!h.batch || !same_options(h.batch_o, o) || h.batch_k.size() != n_cb ||
+        !std::equal(h.batch_k.begin(), h.batch_k.end(), k) || h.batch_f != fs) {
+      h.batch.reset();
+      auto p = std::make_unique<tc_plan>();
+      build_plan(*p, k, fs.data(), n_cb, o);
+      h.batch = std::move(p);
+      h.batch_k.assign(k, k + n_cb);
+      h.batch_f = fs;
+      h.batch_o = o;
     }
-    (void)hipFree(d_llr), (void)hipFree(d_post), (void)hipFree(d_crc), (void)hipFree(d_it);
+    const tc_plan& p = *h.batch;
+    float* d_llr = h.llr.get<float>(p.llr_floats);
+    float* d_post = h.post.get<float>(p.post_floats);
+    int32_t* d_crc = h.crc.get<int32_t>(n_cb);
+    uint32_t* d_it = h.it.get<uint32_t>(n_cb);
+    const hipStream_t s = h.stream;
+    hip_check(hipMemcpyAsync(d_llr, llr, p.llr_floats * sizeof(float), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    launch(p, d_llr, d_post, d_crc, d_it, s);
+    hip_check(hipMemcpyAsync(post, d_post, p.post_floats * sizeof(float), hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync D2H");
+    if (crc_ok)
+      hip_check(hipMemcpyAsync(crc_ok, d_crc, n_cb * sizeof(int32_t), hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    if (iterations_used)
+      hip_check(hipMemcpyAsync(iterations_used, d_it, n_cb * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(s), "decode");
   });
 }
 
@@ -518,45 +587,32 @@ tc_status tc_decode_code_block(const tc_cb_desc* desc, const float* d0, size_t n
     if (turbo_cpp::detail::qpp_index(k) < 0) throw Failure{TC_ERR_OUT_OF_RANGE, "unsupported QPP block size"};
     if (desc->d_r < desc->k_r + 4) throw Failure{TC_ERR_INVALID_ARGUMENT, "d_r must be at least K + 4"};
     const tc_options o = normalised(opts);
-    require_device();
+    HostCalls& h = host_calls();
     const uint32_t f = static_cast<uint32_t>(desc->filler_count);
-    int dev = 0;
-    hip_check(hipGetDevice(&dev), "hipGetDevice");
-    auto key = std::make_tuple(k, f, o.decoder, o.max_iterations, o.min_segment, o.warmup * 2 + (o.early_stop != 0), dev,
-                               o.waves_per_cu, o.extrinsic_bits);
-    auto& slot = g_one.plans[key];
+    const OneKey key{k, f, o.decoder, o.max_iterations, o.min_segment, o.warmup * 2 + (o.early_stop != 0),
+                     o.waves_per_cu, o.extrinsic_bits};
+    auto& slot = h.one_plans[key];
     if (!slot) {
-      slot = std::make_unique<tc_plan>();
-      build_plan(*slot, &k, &f, 1, o);
+      auto p = std::make_unique<tc_plan>();
+      build_plan(*p, &k, &f, 1, o);
+      slot = std::move(p);
     }
-    const size_t need = 3 * (size_t)(k + 4);
-    if (g_one.cap < need) {
-      // both new buffers first: a failed allocation leaves the cache as it was
-      DevBuf<float> llr_buf(need), post_buf(need);
-      for (float** q : {&g_one.d_llr, &g_one.d_post}) {
-        if (*q) (void)hipFree(*q);
-        *q = nullptr;
-      }
-      std::swap(g_one.d_llr, llr_buf.p);
-      std::swap(g_one.d_post, post_buf.p);
-      g_one.cap = need;
-    }
-    if (!g_one.d_crc) g_one.d_crc = dev_alloc<int32_t>(1);
-    if (!g_one.d_it) g_one.d_it = dev_alloc<uint32_t>(1);
+    const size_t d = k + 4;
+    float* d_llr = h.llr.get<float>(3 * d);
+    float* d_post = h.post.get<float>(k);
+    int32_t* d_flags = h.crc.get<int32_t>(2);  // CRC flag, iterations
+    const hipStream_t s = h.stream;
     // only the first K+4 entries of each stream are read
-    hip_check(hipMemcpy(g_one.d_llr, d0, (k + 4) * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
-    hip_check(hipMemcpy(g_one.d_llr + (k + 4), d1, (k + 4) * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
-    hip_check(hipMemcpy(g_one.d_llr + 2 * (k + 4), d2, (k + 4) * sizeof(float), hipMemcpyHostToDevice),
-              "hipMemcpy H2D");
-    launch(*slot, g_one.d_llr, g_one.d_post, g_one.d_crc, g_one.d_it, nullptr);
-    hip_check(hipDeviceSynchronize(), "decode");
-    hip_check(hipMemcpy(posterior, g_one.d_post, k * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    int32_t ok = 0;
-    uint32_t it = 0;
-    hip_check(hipMemcpy(&ok, g_one.d_crc, sizeof ok, hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    hip_check(hipMemcpy(&it, g_one.d_it, sizeof it, hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    if (crc_ok) *crc_ok = ok;
-    if (iterations_used) *iterations_used = it;
+    hip_check(hipMemcpyAsync(d_llr, d0, d * sizeof(float), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    hip_check(hipMemcpyAsync(d_llr + d, d1, d * sizeof(float), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    hip_check(hipMemcpyAsync(d_llr + 2 * d, d2, d * sizeof(float), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    launch(*slot, d_llr, d_post, d_flags, reinterpret_cast<uint32_t*>(d_flags + 1), s);
+    int32_t flags[2] = {0, 0};
+    hip_check(hipMemcpyAsync(posterior, d_post, k * sizeof(float), hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    hip_check(hipMemcpyAsync(flags, d_flags, sizeof flags, hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(s), "decode");
+    if (crc_ok) *crc_ok = flags[0];
+    if (iterations_used) *iterations_used = static_cast<uint32_t>(flags[1]);
   });
 }
 

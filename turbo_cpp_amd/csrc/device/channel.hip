@@ -194,6 +194,7 @@ tc_status tc_channel_plan_create(const tc_channel_seg* segs, size_t n_seg, tc_ch
       p->d_end = dev_alloc<uint64_t>(n_seg);
       dev_upload(p->d_end, end);
     }
+    uploads_done();
     *plan = p.release();
   });
 }

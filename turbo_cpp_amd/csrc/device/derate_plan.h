@@ -41,6 +41,9 @@ bool launch_window(tc_plan* plan, const float* llr, float* post, int32_t* crc_ok
 // tc_options with defaults filled in (min_segment, warmup); throws on an
 // unknown decoder.
 tc_options normalised_options(const tc_options* o);
+// The calling host thread's own stream on the current device (created on
+// first use): host-buffer entry points enqueue on it and wait for it alone.
+hipStream_t host_stream();
 // D^m mod G(D) of the CRC24B generator for m < tcdev::kCrcPowLen.
 void crc24b_powers(uint32_t* out);
 // Device bytes a one-code-block decode needs (tables + work space, K up to 6144).

@@ -186,6 +186,7 @@ tc_status tc_diff_plan_create(const uint32_t* k, size_t n_cb, uint32_t iteration
     if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
     auto p = std::make_unique<tc_diff_plan>();
     build(*p, k, n_cb, iterations, min_segment, warmup, fp64 != 0);
+    uploads_done();
     *plan = p.release();
   });
 }

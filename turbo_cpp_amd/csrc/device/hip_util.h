@@ -48,6 +48,12 @@ inline void require_device() {
     throw Failure{TC_ERR_RUNTIME, "no HIP device available (the GPU backends need an MI355X / gfx950)"};
 }
 
+// Plan construction uploads tables and clears counters on the null stream;
+// callers launch plans on streams of their own, which are not ordered after
+// it (non-blocking streams), so a builder waits for that work before
+// returning the plan.
+inline void uploads_done() { hip_check(hipStreamSynchronize(nullptr), "plan upload"); }
+
 template <typename T>
 T* dev_alloc(size_t n) {
   void* p = nullptr;

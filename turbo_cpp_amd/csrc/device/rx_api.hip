@@ -83,6 +83,7 @@ void build_derate_plan(tc_derate_plan& p, const uint32_t* k, const uint32_t* fil
   dev_upload(p.d_cbs, cbs);
   p.d_rank = dev_alloc<uint16_t>(rank.size());
   dev_upload(p.d_rank, rank);
+  uploads_done();
 }
 
 void run_derate(const tc_derate_plan& p, const float* rx, int32_t rv, int32_t accumulate, float* llr, hipStream_t s) {
@@ -274,6 +275,7 @@ void build_tb_plan(tc_tb_plan& p, const tc_tb_desc* tbs, size_t n_tb, const tc_o
   p.d_map = dev_alloc<tcdev::TbCbMap>(map.size());
   dev_upload(p.d_map, map);
   p.d_acc = dev_alloc<uint32_t>(3 * n_tb);
+  uploads_done();
 }
 
 void run_encode(tc_tb_plan& p, const uint8_t* payload, int32_t rv, uint8_t* rm, uint8_t* cb_bits, hipStream_t s) {
@@ -370,6 +372,7 @@ tc_status tc_derate_plan_create(const uint32_t* k, const uint32_t* filler, const
     if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
     auto p = std::make_unique<tc_derate_plan>();
     build_derate_plan(*p, k, filler, e, rv, n_cb);
+    uploads_done();
     *plan = p.release();
   });
 }
@@ -392,13 +395,14 @@ tc_status tc_derate_match_soft(const float* rx, uint64_t e, uint32_t k, uint32_t
     if (!rx || !out) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
     tc_derate_plan p;
     build_derate_plan(p, &k, &filler, &e, &rv, 1);
+    const hipStream_t s = tchost::host_stream();
     DevBuf<float> d_rx(e), d_out(p.llr_floats);
-    hip_check(hipMemcpy(d_rx.p, rx, e * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    hip_check(hipMemcpyAsync(d_rx.p, rx, e * sizeof(float), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
     if (accumulate)
-      hip_check(hipMemcpy(d_out.p, out, p.llr_floats * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
-    run_derate(p, d_rx.p, -1, accumulate, d_out.p, nullptr);
-    hip_check(hipDeviceSynchronize(), "de-rate matching");
-    hip_check(hipMemcpy(out, d_out.p, p.llr_floats * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+      hip_check(hipMemcpyAsync(d_out.p, out, p.llr_floats * sizeof(float), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    run_derate(p, d_rx.p, -1, accumulate, d_out.p, s);
+    hip_check(hipMemcpyAsync(out, d_out.p, p.llr_floats * sizeof(float), hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(s), "de-rate matching");
   });
 }
 
@@ -407,6 +411,7 @@ tc_status tc_tb_plan_create(const tc_tb_desc* tbs, size_t n_tb, const tc_options
     if (!plan) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
     auto p = std::make_unique<tc_tb_plan>();
     build_tb_plan(*p, tbs, n_tb, opts);
+    uploads_done();
     *plan = p.release();
   });
 }
@@ -499,12 +504,13 @@ tc_status tc_encode_transport_blocks_host(const tc_tb_desc* tbs, size_t n_tb, co
     if (n_tb == 0) return;
     if (!payload || !rm_bits) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
     const size_t nrm = tc_derate_plan_rx_floats(p.derate), ncb = tc_plan_post_floats(p.decoder);
+    const hipStream_t s = tchost::host_stream();
     DevBuf<uint8_t> d_pay(p.bits_total), d_rm(nrm), d_cb(ncb);
-    hip_check(hipMemcpy(d_pay.p, payload, p.bits_total, hipMemcpyHostToDevice), "hipMemcpy H2D");
-    run_encode(p, d_pay.p, -1, d_rm.p, cb_bits ? d_cb.p : nullptr, nullptr);
-    hip_check(hipDeviceSynchronize(), "transport block encode");
-    hip_check(hipMemcpy(rm_bits, d_rm.p, nrm, hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    if (cb_bits) hip_check(hipMemcpy(cb_bits, d_cb.p, ncb, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    hip_check(hipMemcpyAsync(d_pay.p, payload, p.bits_total, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    run_encode(p, d_pay.p, -1, d_rm.p, cb_bits ? d_cb.p : nullptr, s);
+    hip_check(hipMemcpyAsync(rm_bits, d_rm.p, nrm, hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    if (cb_bits) hip_check(hipMemcpyAsync(cb_bits, d_cb.p, ncb, hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(s), "transport block encode");
   });
 }
 
@@ -517,19 +523,20 @@ tc_status tc_decode_transport_blocks_host(const tc_tb_desc* tbs, size_t n_tb, co
     if (n_tb == 0) return;
     if (!rx || !bits || !tb_crc_ok) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
     const size_t nrx = tc_derate_plan_rx_floats(p.derate), npost = tc_plan_post_floats(p.decoder);
+    const hipStream_t s = tchost::host_stream();
     DevBuf<float> d_rx(nrx), d_post(npost);
     DevBuf<uint8_t> d_bits(p.bits_total);
     DevBuf<int32_t> d_tb(n_tb), d_cb(p.n_cb);
     DevBuf<uint32_t> d_it(n_tb);
-    hip_check(hipMemcpy(d_rx.p, rx, nrx * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    hip_check(hipMemcpyAsync(d_rx.p, rx, nrx * sizeof(float), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
     const float* rxs[1] = {d_rx.p};
-    run_tb(p, 1, rxs, nullptr, 0, d_bits.p, d_tb.p, d_cb.p, d_it.p, d_post.p, nullptr);
-    hip_check(hipDeviceSynchronize(), "transport block decode");
-    hip_check(hipMemcpy(bits, d_bits.p, p.bits_total, hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    hip_check(hipMemcpy(tb_crc_ok, d_tb.p, n_tb * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    if (cb_crc_ok) hip_check(hipMemcpy(cb_crc_ok, d_cb.p, p.n_cb * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    if (iterations) hip_check(hipMemcpy(iterations, d_it.p, n_tb * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    if (posterior) hip_check(hipMemcpy(posterior, d_post.p, npost * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    run_tb(p, 1, rxs, nullptr, 0, d_bits.p, d_tb.p, d_cb.p, d_it.p, d_post.p, s);
+    hip_check(hipMemcpyAsync(bits, d_bits.p, p.bits_total, hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    hip_check(hipMemcpyAsync(tb_crc_ok, d_tb.p, n_tb * sizeof(int32_t), hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    if (cb_crc_ok) hip_check(hipMemcpyAsync(cb_crc_ok, d_cb.p, p.n_cb * sizeof(int32_t), hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    if (iterations) hip_check(hipMemcpyAsync(iterations, d_it.p, n_tb * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    if (posterior) hip_check(hipMemcpyAsync(posterior, d_post.p, npost * sizeof(float), hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(s), "transport block decode");
   });
 }
 
