@@ -98,12 +98,11 @@ struct WinArgs {
 constexpr int kCrcPowLen = 6144;
 // LDS of one window job: the extrinsics held between half-iterations
 // [S][64] plus one dummy slot per lane (ex_bytes each: 2 for fp16, 4 for
-// fp32), a hard-decision bitmap over the same index space, and one CRC
-// partial per lane.  At S = 96 with fp16 this is 13.3 KiB, so LDS does not
-// keep a CU below the eight waves (two per SIMD) that the register budget
-// allows; fp32 (25.7 KiB) allows six.
+// fp32) and one CRC partial per lane.  At S = 96 with fp16 this is 12.4 KiB,
+// so LDS does not keep a CU below the eight waves (two per SIMD) that the
+// register budget allows; fp32 (24.5 KiB) allows six.
 constexpr int job_lds_bytes(int s, int ex_bytes) {
-  return (s * kExRow + kWaveLanes) * ex_bytes + (s * kExRow + kWaveLanes) / 8 + kWaveLanes * 4;
+  return (s * kExRow + kWaveLanes) * ex_bytes + kWaveLanes * 4;
 }
 constexpr int kExtrinsicBits = 16;  // window decoders' default held width
 

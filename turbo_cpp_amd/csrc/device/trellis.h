@@ -39,14 +39,14 @@ __device__ __forceinline__ float hw_log2(float x) { return __builtin_amdgcn_logf
 
 // max*(x, y) = hi + log1p(exp(lo - hi)) of two natural-log metrics (the
 // reference's max_star for finite operands, lte_turbo.cpp:380-390), evaluated
-// as hi + ln2 * log2(1 + 2^((lo - hi) log2e)) on v_exp_f32 / v_log_f32.  Both
+// as hi + ln2 * log2(1 + 2^((lo - hi) log2e)) on v_exp_f32 / v_log_f32, the
+// last multiply-add fused.  Both
 // exact decoders use this one function, so their metrics agree bit for bit;
 // against the reference's expf/log1pf it differs by rounding (~1e-7 per call).
 __device__ __forceinline__ float jac_ref(float x, float y) {
 #pragma clang fp contract(off)
-  const float hi = (x < y) ? y : x;
-  const float lo = (y < x) ? y : x;
-  return hi + hw_log2(1.0f + hw_exp2((lo - hi) * kLog2e)) * kLn2;
+  const float hi = fmaxf(x, y), lo = fminf(x, y);
+  return __builtin_fmaf(hw_log2(1.0f + hw_exp2((lo - hi) * kLog2e)), kLn2, hi);
 }
 
 // Branch metric of input u and parity c from g1 = x + y, g2 = x - y

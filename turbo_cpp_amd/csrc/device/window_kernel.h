@@ -561,7 +561,6 @@ __device__ __forceinline__ void alpha_warmup(const Lane& L, const X* __restrict_
 struct EarlyStop {
   bool on;
   bool done;
-  uint32_t* bitmap;  // hard decisions, indexed like the extrinsics
 };
 static_assert(kExRow == 64, "extrinsic address = row * 64 + column");
 
@@ -609,9 +608,6 @@ __device__ __forceinline__ void alpha_subwindow(const Lane& L, X* __restrict__ e
       if constexpr (H == 1) {
         const float p = isa[j] + ext;
         post_sc[a] = p;
-        if constexpr (ES) {
-          if (es.on && p >= 0.0f) atomicOr(&es.bitmap[a >> 5], 1u << (a & 31));
-        }
       }
     }
   }
@@ -795,15 +791,45 @@ __device__ __forceinline__ void siso_half_exact(const Lane& L, X* __restrict__ e
   const uint32_t S = L.S, nW = L.nW;
   constexpr uint32_t kNever = 0xFFFFFFFFu;
   const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
-  // 1. speculation
-  float bc[8];
+  // 1. speculation: beta down and alpha up in one loop, one chunk of each per
+  // trip (two independent chains), inputs loaded a trip ahead
+  float bc[8], ac[8];
   load8(nii + 8, bc);
-  beta_pass<D, H>(
-      L, ex, sc, qtab, L.q, L.le, 0, S, bc, kNever, [&](uint32_t w, float* b) { store8(ck + 8 * w, b); },
-      [](const float*) {});
-  float ac[8];
   load8(nii, ac);
-  alpha_pass_ck<D, H>(L, ex, sc, qtab, ac, ack);
+  {
+    Chunk nb, na;
+    const uint32_t lb0 = S - (nW - 1) * kSubWin, la0 = min((uint32_t)kSubWin, S);
+    load_chunk(sc, L.le, (nW - 1) * kSubWin, lb0, nb);
+    load_chunk_lds<H, true, false>(L, ex, qtab, L.q, by, (nW - 1) * kSubWin, lb0, nb, nullptr);
+    load_chunk(sc, L.le, 0, la0, na);
+    load_chunk_lds<H, false, false>(L, ex, qtab, L.q, by, 0, la0, na, nullptr);
+    for (uint32_t c = 0; c < nW; ++c) {
+      if constexpr (D::kShareSimd) share_simd();
+      const uint32_t wb = nW - 1 - c;
+      const uint32_t lb = min((uint32_t)kSubWin, S - wb * kSubWin), la = min((uint32_t)kSubWin, S - c * kSubWin);
+      const Chunk xb = nb, xa = na;
+      if (c + 1 < nW) {
+        const uint32_t wb1 = wb - 1, wa1 = c + 1;
+        const uint32_t lb1 = min((uint32_t)kSubWin, S - wb1 * kSubWin), la1 = min((uint32_t)kSubWin, S - wa1 * kSubWin);
+        load_chunk(sc, L.le, wb1 * kSubWin, lb1, nb);
+        load_chunk_lds<H, true, false>(L, ex, qtab, L.q, by, wb1 * kSubWin, lb1, nb, nullptr);
+        load_chunk(sc, L.le, wa1 * kSubWin, la1, na);
+        load_chunk_lds<H, false, false>(L, ex, qtab, L.q, by, wa1 * kSubWin, la1, na, nullptr);
+      }
+      store8(ck + 8 * wb, bc);
+      store8(ack + 8 * c, ac);
+#pragma unroll
+      for (int j = 0; j < kSubWin; ++j) {
+        const int jb = kSubWin - 1 - j;
+        __builtin_amdgcn_sched_barrier(0);
+        if ((uint32_t)jb < lb) D::bstep(bc, D::branch(xb.v[jb].x + xb.e[jb], xb.v[jb].y));
+        if ((uint32_t)j < la) {
+          const float sa = xa.v[j].x + xa.e[j];
+          D::template astep<false>(ac, D::branch(sa, xa.v[j].y), sa, 0.0f, 0.0f, nullptr);
+        }
+      }
+    }
+  }
   // 2. fix-up rounds (idle lanes and the exact ends never need one).  After
   // round r the segments within r of an exact end hold exact metrics, so
   // M + 1 rounds always suffice; the bound only guarantees every wave leaves.
@@ -882,20 +908,32 @@ __device__ __forceinline__ void siso_half_exact(const Lane& L, X* __restrict__ e
 }
 
 // remainder test of CRC24B (generator of lte_turbo.cpp:108-111) on the
-// hard decisions of bits [F, K) of every block of the job, from the LDS
-// bitmap: bit i contributes D^(K-1-i) mod G.  Each lane folds its own
-// segment (natural order: r(i+1) = r(i) * D^-1 mod G), then the first lane of
-// each block XORs its M partials.  Returns this lane's block verdict.
-__device__ __forceinline__ bool crc24b_job(const Lane& L, const uint32_t* __restrict__ bitmap, uint32_t* __restrict__ partial,
-                           const uint32_t* __restrict__ crc_pow, uint32_t filler) {
+// hard decisions (posterior >= 0) of bits [F, K) of every block of the job,
+// read from the posteriors the lower half left in the wave's scratch slot
+// (bit i = q S + t at row t, column b M + q): bit i contributes D^(K-1-i) mod
+// G.  Each lane folds its own segment (natural order: r(i+1) = r(i) * D^-1
+// mod G), then the first lane of each block XORs its M partials.  Returns this
+// lane's block verdict.  Reading the posteriors after the half keeps the
+// early-stop work out of the recursion loops.
+__device__ __forceinline__ bool crc24b_job(const Lane& L, const float* __restrict__ post_sc,
+                                           uint32_t* __restrict__ partial, const uint32_t* __restrict__ crc_pow,
+                                           uint32_t filler) {
   const uint32_t S = L.S, K = L.K, lane = threadIdx.x;
   uint32_t x = 0;
   uint32_t r = crc_pow[K - 1 - L.q * S];
-  for (uint32_t t = 0; t < S; ++t) {
-    const uint32_t i = L.q * S + t;
-    const uint32_t a = t * kExRow + L.colb + L.q;
-    if (i >= filler && ((bitmap[a >> 5] >> (a & 31)) & 1u)) x ^= r;
-    r = (r & 1u) ? ((r ^ (0xC00063u | (1u << 24))) >> 1) : (r >> 1);
+  for (uint32_t t0 = 0; t0 < S; t0 += kSubWin) {
+    float pv[kSubWin];
+#pragma unroll
+    for (uint32_t j = 0; j < kSubWin; ++j)
+      pv[j] = (t0 + j < S) ? post_sc[(t0 + j) * kExRow + L.colb + L.q] : -1.0f;
+#pragma unroll
+    for (uint32_t j = 0; j < kSubWin; ++j) {
+      if (t0 + j < S) {
+        const uint32_t i = L.q * S + t0 + j;
+        if (i >= filler && pv[j] >= 0.0f) x ^= r;
+        r = (r & 1u) ? ((r ^ (0xC00063u | (1u << 24))) >> 1) : (r >> 1);
+      }
+    }
   }
   partial[lane] = x;
   __syncthreads();
@@ -1213,13 +1251,10 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
   const uint32_t my_cb = cb_index[J.cb_off + b];
   const CbInfo ci = cbs[my_cb];
   const bool fused = A.rx.n_tx > 0;
-  // LDS: extrinsics [S][64] + one dummy slot per lane (fp16), then the
-  // hard-decision bitmap over the same index space and the CRC partials
-  // (early stop only)
+  // LDS: extrinsics [S][64] + one dummy slot per lane, then the CRC
+  // partials (early stop only)
   const uint32_t ex_slots = S * kExRow + kWaveLanes;
-  uint32_t* bitmap = reinterpret_cast<uint32_t*>(ex + ex_slots);
-  const uint32_t bitmap_words = ex_slots / 32;
-  uint32_t* partial = bitmap + bitmap_words;
+  uint32_t* partial = reinterpret_cast<uint32_t*>(ex + ex_slots);
   RxBlk Bm{};
   if (fused) rx_block(A.rx, my_cb, Bm);
   const uint32_t D4 = K + 4;
@@ -1321,7 +1356,7 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
   }
   __syncthreads();
 
-  EarlyStop es{es_on, false, bitmap};
+  EarlyStop es{es_on, false};
   uint32_t used_it = A.max_iterations;
   for (uint32_t it = 0; it < A.max_iterations; ++it) {
     if constexpr (D::kExact) {
@@ -1330,10 +1365,6 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
       siso_half<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
     }
     __syncthreads();
-    if (es_on) {
-      for (uint32_t i = lane; i < bitmap_words; i += kWaveLanes) bitmap[i] = 0u;
-      __syncthreads();
-    }
     // every lower half leaves the posteriors in post_sc; a block that has
     // stopped no longer overwrites its own
     if constexpr (D::kExact) {
@@ -1343,7 +1374,7 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
     }
     __syncthreads();
     if (es_on) {
-      const bool ok = crc24b_job(L, bitmap, partial, A.crc_pow, ci.filler);
+      const bool ok = crc24b_job(L, post_sc, partial, A.crc_pow, ci.filler);
       if (ok && !es.done) {
         es.done = true;
         used_it = it + 1;
