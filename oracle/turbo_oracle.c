@@ -844,7 +844,9 @@ int or_decode_exact_seg(const float* d0, const float* d1, const float* d2, uint3
  *                x = sa*(log2e/2), y = par*(log2e/2), g1 = x + y, g2 = x - y
  *                (branch metrics +-g1, +-g2),
  *                max*(a,b) = max(a,b) + log2(1 + 2^-|a-b|),
- *                metrics renormalised to state 0's metric each step,
+ *                metrics renormalised to state 0's metric once per chunk of 8
+ *                steps of a pass (chunks aligned at the pass's first step:
+ *                beta after the chunk's lowest step, alpha after its 8th),
  *                extrinsic = ln2 * (LSE{alpha~(n)+beta(n) : u(n)=1}
  *                          - LSE{...: u(n)=0}) - sa,
  *                LSE(v) = m + log2(sum 2^(v_i-m)).
@@ -926,7 +928,9 @@ static void seg_siso(const float* in_sa, const float* in_par, int len, int ga, i
         rsc_step(p, 1, &c1, &n1);
         nb[p] = lg_maxstar(b[n0] + gsel(0, c0, g1, g2), b[n1] + gsel(1, c1, g1, g2));
       }
-      lg_norm(nb);
+      /* chunks of the own pass start at local step 0, those of the warm-up
+       * over the next segment at local step len */
+      if ((t >= len ? t - len : t) % 8 == 0) lg_norm(nb);
     } else {
       float u1, u0, q1, q0;
       lin_weights(x, y, &u1, &u0, &q1, &q0);
@@ -973,7 +977,8 @@ static void seg_siso(const float* in_sa, const float* in_par, int len, int ga, i
         float l1 = m1 + log2f(s1), l0 = m0 + log2f(s0);
         ext[t] = (l1 - l0) * kLn2 - in_sa[t];
       }
-      lg_norm(na);
+      /* chunks of the warm-up start at local step -ga, the own pass's at 0 */
+      if ((t < 0 ? t + ga : t) % 8 == 7) lg_norm(na);
     } else {
       float u1, u0, q1, q0;
       lin_weights(x, y, &u1, &u0, &q1, &q0);

@@ -121,8 +121,11 @@ struct LogDomain {
   }
   // metrics relative to state 0 (any reference state keeps max* exact; the
   // spread between states stays bounded because every state reaches every
-  // other in three steps); state 0 is then exactly 0, which the next step's
-  // additions fold away
+  // other in three steps).  Done once per chunk of kSubWin steps, not every
+  // step (nrm): the metrics then drift by at most kSubWin branch metrics
+  // between renormalisations, and the chunk-aligned positions (the chunk's
+  // last beta step, its kSubWin-th alpha step) keep the beta recompute
+  // bit-identical to the first pass
   __device__ static __forceinline__ void norm(float* v) {
     const float m = v[0];
 #pragma unroll
@@ -130,21 +133,21 @@ struct LogDomain {
     v[0] = 0.0f;
   }
   // beta_t from beta_{t+1}
-  __device__ static __forceinline__ void bstep(float* b, const Br& r) {
+  __device__ static __forceinline__ void bstep(float* b, const Br& r, bool nrm = true) {
     float nb[8];
 #pragma unroll
     for (unsigned p = 0; p < 8; ++p) {
       const Tr t0 = rsc(p, 0), t1 = rsc(p, 1);
       nb[p] = maxstar(b[t0.next] + gsel(0, t0.parity, r.g1, r.g2), b[t1.next] + gsel(1, t1.parity, r.g1, r.g2));
     }
-    norm(nb);
+    if (nrm) norm(nb);
 #pragma unroll
     for (int i = 0; i < 8; ++i) b[i] = nb[i];
   }
   // alpha_{t+1} from alpha_t; returns the extrinsic of step t when WANT
   template <bool WANT>
   __device__ static __forceinline__ float astep(float* a, const Br& r, float sa, float /*sys*/, float /*ap*/,
-                                                const float* bn) {
+                                                const float* bn, bool nrm = true) {
     float na[8];
 #pragma unroll
     for (unsigned n = 0; n < 8; ++n) {
@@ -172,7 +175,7 @@ struct LogDomain {
       const float l0 = m0 + hw_log2(s0);
       ext = (l1 - l0) * kLn2 - sa;
     }
-    norm(na);
+    if (nrm) norm(na);
 #pragma unroll
     for (int i = 0; i < 8; ++i) a[i] = na[i];
     return ext;
@@ -212,7 +215,9 @@ struct LinDomain {
 #pragma unroll
     for (int i = 0; i < 8; ++i) v[i] = __builtin_amdgcn_ldexpf(v[i], -e);
   }
-  __device__ static __forceinline__ void bstep(float* b, const Br& r) {
+  // rescaled every step (nrm ignored): products of probabilities under- or
+  // overflow within a few steps otherwise
+  __device__ static __forceinline__ void bstep(float* b, const Br& r, bool /*nrm*/ = true) {
     const float w[2][2] = {{r.u0 * r.q0, r.u0 * r.q1}, {r.u1 * r.q0, r.u1 * r.q1}};
     float nb[8];
 #pragma unroll
@@ -226,7 +231,7 @@ struct LinDomain {
   }
   template <bool WANT>
   __device__ static __forceinline__ float astep(float* a, const Br& r, float /*sa*/, float /*sys*/, float /*ap*/,
-                                                const float* bn) {
+                                                const float* bn, bool /*nrm*/ = true) {
     const float q[2] = {r.q0, r.q1}, uu[2] = {r.u0, r.u1};
     float na[8];
     float s1 = 0.0f, s0 = 0.0f;
@@ -281,7 +286,8 @@ struct ExactDomain {
 #pragma unroll
     for (int i = 0; i < 8; ++i) v[i] = v[i] - m;
   }
-  __device__ static __forceinline__ void bstep(float* b, const Br& r) {
+  // renormalised every step, as the reference does (nrm ignored)
+  __device__ static __forceinline__ void bstep(float* b, const Br& r, bool /*nrm*/ = true) {
     float nb[8];
 #pragma unroll
     for (unsigned p = 0; p < 8; ++p) {
@@ -294,7 +300,7 @@ struct ExactDomain {
   }
   template <bool WANT>
   __device__ static __forceinline__ float astep(float* a, const Br& r, float /*sa*/, float sys, float ap,
-                                                const float* bn) {
+                                                const float* bn, bool /*nrm*/ = true) {
     float ext = 0.0f;
     if constexpr (WANT) {
       float l[2];
@@ -497,7 +503,7 @@ __device__ __forceinline__ void beta_pass(const Lane& L, const X* __restrict__ e
 #pragma unroll
       for (int j = kSubWin - 1; j >= 0; --j) {
         __builtin_amdgcn_sched_barrier(0);
-        D::bstep(bc, D::branch(cur.v[j].x + cur.e[j], cur.v[j].y));
+        D::bstep(bc, D::branch(cur.v[j].x + cur.e[j], cur.v[j].y), j == 0);
       }
       if (hand_t == clo) on_hand(bc);
     } else {
@@ -505,7 +511,7 @@ __device__ __forceinline__ void beta_pass(const Lane& L, const X* __restrict__ e
       for (int j = kSubWin - 1; j >= 0; --j) {
         if ((uint32_t)j < len) {
           __builtin_amdgcn_sched_barrier(0);
-          D::bstep(bc, D::branch(cur.v[j].x + cur.e[j], cur.v[j].y));
+          D::bstep(bc, D::branch(cur.v[j].x + cur.e[j], cur.v[j].y), j == 0);
           if (clo + j == hand_t) on_hand(bc);
         }
       }
@@ -539,7 +545,7 @@ __device__ __forceinline__ void alpha_warmup(const Lane& L, const X* __restrict_
       for (int j = 0; j < kSubWin; ++j) {
         __builtin_amdgcn_sched_barrier(0);
         const float sa = cur.v[j].x + cur.e[j];
-        D::template astep<false>(ac, D::branch(sa, cur.v[j].y), sa, 0.0f, 0.0f, nullptr);
+        D::template astep<false>(ac, D::branch(sa, cur.v[j].y), sa, 0.0f, 0.0f, nullptr, j == kSubWin - 1);
       }
     } else {
 #pragma unroll
@@ -547,7 +553,7 @@ __device__ __forceinline__ void alpha_warmup(const Lane& L, const X* __restrict_
         if ((uint32_t)j < len) {
           __builtin_amdgcn_sched_barrier(0);
           const float sa = cur.v[j].x + cur.e[j];
-          D::template astep<false>(ac, D::branch(sa, cur.v[j].y), sa, 0.0f, 0.0f, nullptr);
+          D::template astep<false>(ac, D::branch(sa, cur.v[j].y), sa, 0.0f, 0.0f, nullptr, j == kSubWin - 1);
         }
       }
     }
@@ -586,7 +592,7 @@ __device__ __forceinline__ void alpha_subwindow(const Lane& L, X* __restrict__ e
 #pragma unroll
       for (int i = 0; i < 8; ++i) bs[j][i] = bb[i];
       __builtin_amdgcn_sched_barrier(0);
-      if (j > 0) D::bstep(bb, br[j]);
+      if (j > 0) D::bstep(bb, br[j], false);  // steps 1.. of the chunk: not renormalised in the first pass either
     }
   }
   auto hand = [&]() {
@@ -602,7 +608,7 @@ __device__ __forceinline__ void alpha_subwindow(const Lane& L, X* __restrict__ e
     if (FULL || (uint32_t)j < len) {
       if (!FULL && t0 + j == hand_t) hand();
       __builtin_amdgcn_sched_barrier(0);  // keep steps in order: bounds register pressure
-      const float ext = D::template astep<true>(ac, br[j], isa[j], cur.v[j].x, cur.e[j], bs[j]);
+      const float ext = D::template astep<true>(ac, br[j], isa[j], cur.v[j].x, cur.e[j], bs[j], j == kSubWin - 1);
       const uint32_t a = wr ? addr[j] : dummy;
       ex[a] = to_ex<X>(ext);
       if constexpr (H == 1) {
