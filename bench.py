@@ -584,7 +584,7 @@ def main():
     else:
         from turbo_cpp_amd import autotune
         log("autotuning segment length / warm-up / waves per CU")
-        tuned = autotune.tune(K, dec, ITERS, n_blocks=N_CB, ebn0_db=0.5)
+        tuned = autotune.tune(K, dec, ITERS, n_blocks=N_CB, ebn0_db=(0.25, 0.5, 0.75), extrinsic_bits=(16, 32))
         tuned_opts = autotune.options_for(tuned)
         log("autotune choice:", tuned["best"])
     log("generating", N_CB, "blocks")
@@ -594,7 +594,8 @@ def main():
     y = torch.empty(N_CB * K, dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
     plan = T.Plan(ks, dec, ITERS, **tuned_opts)
-    geo = T.segment_geometry(K, dec, tuned_opts.get("min_segment", 0), tuned_opts.get("warmup", 0))
+    geo = T.segment_geometry(K, dec, tuned_opts.get("min_segment", 0), tuned_opts.get("warmup", 0),
+                             tuned_opts.get("extrinsic_bits", 0))
 
     def step(p=plan, out=y):
         p.decode(x.data_ptr(), out.data_ptr(), 0, 0, stream.cuda_stream)
@@ -650,13 +651,14 @@ def main():
     # the same geometry with the extrinsics held in fp32 between half-iterations (what fp16
     # holding buys, at equal BER)
     for name, d in (("log_f32", T.DECODER_WINDOW_LOG), ("lin_f32", T.DECODER_WINDOW_LIN)):
-        pf = T.Plan(ks, d, ITERS, extrinsic_bits=32, **tuned_opts)
+        pf = T.Plan(ks, d, ITERS, **{**tuned_opts, "extrinsic_bits": 32})
         ms = timed(pf, 5, 2)
         modes[name] = {"ms": ms, "gbit_s": N_CB * K / ms / 1e6, "extrinsics_held": "fp32"}
         ber[name] = ber_table(y.cpu().numpy(), bits, ebn0)
         log(f"{name}: {ms:.3f} ms")
         del pf
-    modes[args.decoder]["extrinsics_held"] = modes[other_name]["extrinsics_held"] = "fp16"
+    held = f"fp{geo['extrinsic_bits']}"
+    modes[args.decoder]["extrinsics_held"] = modes[other_name]["extrinsics_held"] = held
 
     # exact decoders (the reference's arithmetic and order): segment-parallel with exact
     # boundaries, and one full-length recursion per block; the BER reference on the GPU
@@ -683,7 +685,7 @@ def main():
     plain_gbits = K / plain_s / 1e9
     log(f"plain: {plain_s * 1e3:.3f} ms per block -> {plain_gbits * 1e3:.3f} Mbit/s")
 
-    geo_kw = {k: v for k, v in tuned_opts.items() if k in ("min_segment", "warmup")}
+    geo_kw = {k: v for k, v in tuned_opts.items() if k in ("min_segment", "warmup", "extrinsic_bits")}
     rx_chain = None if args.no_rx_chain else rx_chain_leg(T, torch, stream, llr, kernel_ms, geo_kw)
     diff = diff_leg(T, torch, args.train_steps) if args.train_steps > 0 else None
     uneven = None if args.no_scenarios else uneven_leg(T, torch, geo_kw)
@@ -743,14 +745,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 arithmetic, fp16-held extrinsics",
+        "dtype": f"fp32 arithmetic, fp{geo['extrinsic_bits']}-held extrinsics",
         "data": "synthetic: seeded Bernoulli bits, reference encoder, BPSK/AWGN 0.0-1.5 dB",
         "config": {"workload": "uniform batch: 4096 code blocks K=6144, rate 1/3 full buffer, BPSK/AWGN "
                                "Eb/N0 0.0..1.5 dB (0.25 dB steps), fp32 LLRs, 8 iterations, no early stop",
                    "code_blocks": N_CB, "k": K, "iterations": ITERS, "decoder": f"window-{args.decoder}",
                    "segment_length": geo["segment_length"], "segments_per_block": geo["segments"],
                    "warmup_steps": geo["warmup"], "waves_per_cu": tuned_opts.get("waves_per_cu", 0),
-                   "extrinsics_held": "fp16 (modes.log_f32: fp32-held, same geometry)"},
+                   "extrinsics_held": f"fp{geo['extrinsic_bits']} (modes.log_f32: fp32-held, same geometry)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain": {"gbit_s": plain_gbits, "ms_per_block": plain_s * 1e3, "blocks_timed": nplain,
@@ -766,9 +768,12 @@ def main():
         "bwd_gbit_s": diff["bwd_gbit_s"] if diff else None,
         "differentiation": diff,
         "autotune": None if tuned is None else {
-            "choice": tuned["best"], "exact_bit_errors": tuned["exact_bit_errors"], "interval": tuned["interval"],
-            "calibration": f"{tuned['n_blocks']} blocks K={K}, QPSK/AWGN {tuned['ebn0_db']} dB",
-            "table": [{k: c[k] for k in ("segment_length", "warmup", "waves_per_cu", "ms", "bit_errors", "qualifies")}
+            "choice": tuned["best"], "points": tuned["points"],
+            "calibration": f"{tuned['n_blocks']} blocks K={K} per point, QPSK/AWGN at {tuned['ebn0_db']} dB; "
+                           f"qualifies: bit and block errors inside the exact decoder's 95% intervals at every "
+                           f"point; timed at {tuned['timed_at_db']} dB",
+            "table": [{k: c[k] for k in ("segment_length", "warmup", "waves_per_cu", "extrinsic_bits", "ms",
+                                         "per_point", "qualifies")}
                       for c in tuned["table"]]},
         "uneven_batch": uneven,
         "mixed_load": mixed,

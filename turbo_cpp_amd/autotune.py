@@ -3,16 +3,18 @@
 Knobs (tc_options): the segment length S (via min_segment; S is the smallest
 divisor of K >= max(min_segment, K/64), which also fixes the segments per
 block M = K/S and the blocks per 64-lane wave 64/M), the warm-up steps per
-segment boundary, and the resident waves per CU (occupancy).
+segment boundary, the resident waves per CU (occupancy) and the width of the
+extrinsics held between half-iterations (fp16 or fp32).
 
-For one K the tuner makes a calibration batch on the GPU (random payloads of
-K - 24 bits with CRC24A, the GPU encoder, full circular buffer at rv 0, QPSK
-over AWGN at the given Eb/N0, max-log LLRs, soft de-rate matching), decodes it
-with the exact full-length decoder (the reference's algorithm) and with every
-candidate, and times each candidate's decode with HIP events.  A candidate
-qualifies when its bit-error count lies inside the 95% Clopper-Pearson
-interval of the exact decoder's count on the same blocks; the fastest
-qualifying candidate is returned with the whole table.
+For one K the tuner makes calibration batches on the GPU at several Eb/N0
+points (random payloads of K - 24 bits with CRC24A, the GPU encoder, full
+circular buffer at rv 0, QPSK over AWGN, max-log LLRs, soft de-rate
+matching), decodes them with the exact decoder (the reference's algorithm)
+and with every candidate, and times each candidate's decode with HIP events
+on the middle point's batch.  A candidate qualifies when, at every point,
+both its bit-error and its block-error counts lie inside the 95%
+Clopper-Pearson intervals of the exact decoder's counts on the same blocks;
+the fastest qualifying candidate is returned with the whole table.
 """
 from __future__ import annotations
 
@@ -31,11 +33,14 @@ class Candidate:
     min_segment: int
     warmup: int
     waves_per_cu: int
+    extrinsic_bits: int = 16
     segment_length: int = 0
     segments: int = 0
     blocks_per_wave: int = 0
     ms: float = float("nan")
-    bit_errors: int = -1
+    bit_errors: int = -1  # summed over the calibration points
+    block_errors: int = -1
+    per_point: Optional[list] = None  # [(bit errors, block errors)] per Eb/N0 point
     qualifies: bool = False
 
 
@@ -86,47 +91,67 @@ def _decode(plan: Plan, llr, post, reps: int):
 
 
 def tune(k: int, decoder: int = DECODER_WINDOW_LOG, iterations: int = 8, n_blocks: int = 2048,
-         ebn0_db: float = 0.5, min_segments: Sequence[int] = (32, 64, 96, 128, 192),
-         warmups: Sequence[int] = (8, 16, 24, 32), waves: Sequence[int] = (0, 4, 2), reps: int = 3,
-         seed: int = 1) -> Dict:
-    """Returns {"best": {...}, "table": [...], "exact_bit_errors": n, "interval": [lo, hi], ...}."""
+         ebn0_db=(0.25, 0.5, 0.75), min_segments: Sequence[int] = (32, 64, 96, 128, 192),
+         warmups: Sequence[int] = (8, 16, 24, 32), waves: Sequence[int] = (0, 4, 2),
+         extrinsic_bits: Sequence[int] = (16,), reps: int = 3, seed: int = 1) -> Dict:
+    """Returns {"best": {...}, "table": [...], "points": [per Eb/N0: exact errors and intervals], ...}."""
     import torch
-    llr, cb = calibration_batch(k, n_blocks, ebn0_db, seed)
-    bits = cb.to(torch.bool)
+    pts = [float(ebn0_db)] if np.isscalar(ebn0_db) else [float(e) for e in ebn0_db]
+    batches = [calibration_batch(k, n_blocks, e, seed + i) for i, e in enumerate(pts)]
+    timed_at = len(pts) // 2
     post = torch.empty(n_blocks * k, dtype=torch.float32, device="cuda")
-    exact = Plan([k] * n_blocks, DECODER_EXACT, iterations)
-    exact.decode(llr.data_ptr(), post.data_ptr(), 0, 0, torch.cuda.current_stream().cuda_stream)
-    torch.cuda.synchronize()
-    ref_err = int(((post >= 0) != bits).sum())
     nbits = n_blocks * k
-    lo, hi = clopper_pearson(ref_err, nbits)
+
+    def errors(plan):
+        out = []
+        for llr, cb in batches:
+            plan.decode(llr.data_ptr(), post.data_ptr(), 0, 0, torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            wrong = (post >= 0) != cb.to(torch.bool)
+            out.append((int(wrong.sum()), int(wrong.view(n_blocks, k).any(dim=1).sum())))
+        return out
+
+    exact = errors(Plan([k] * n_blocks, DECODER_EXACT, iterations))
+    points = []
+    for e, (be, ke) in zip(pts, exact):
+        lo_b, hi_b = clopper_pearson(be, nbits)
+        lo_k, hi_k = clopper_pearson(ke, n_blocks)
+        points.append({"ebn0_db": e, "exact_bit_errors": be, "exact_block_errors": ke,
+                       "bit_interval": [math.floor(lo_b * nbits), math.ceil(hi_b * nbits)],
+                       "block_interval": [math.floor(lo_k * n_blocks), math.ceil(hi_k * n_blocks)]})
     seen = set()
     table: List[Candidate] = []
-    for ms, g, w in itertools.product(min_segments, warmups, waves):
+    for ms, g, w, xb in itertools.product(min_segments, warmups, waves, extrinsic_bits):
         try:
-            geo = segment_geometry(k, decoder, ms, g)
+            geo = segment_geometry(k, decoder, ms, g, xb)
         except ValueError:
             continue  # segment length outside what the kernel supports
-        key = (geo["segment_length"], geo["warmup"], w)
+        key = (geo["segment_length"], geo["warmup"], w, xb)
         if key in seen:
             continue
         seen.add(key)
-        c = Candidate(ms, g, w, geo["segment_length"], geo["segments"], geo["blocks_per_wave"])
-        plan = Plan([k] * n_blocks, decoder, iterations, min_segment=ms, warmup=g, waves_per_cu=w)
-        c.ms = _decode(plan, llr, post, reps)
-        c.bit_errors = int(((post >= 0) != bits).sum())
-        c.qualifies = lo * nbits <= c.bit_errors <= hi * nbits
+        c = Candidate(ms, g, w, xb, geo["segment_length"], geo["segments"], geo["blocks_per_wave"])
+        plan = Plan([k] * n_blocks, decoder, iterations, min_segment=ms, warmup=g, waves_per_cu=w, extrinsic_bits=xb)
+        c.per_point = errors(plan)
+        c.bit_errors = sum(x[0] for x in c.per_point)
+        c.block_errors = sum(x[1] for x in c.per_point)
+        c.qualifies = all(p["bit_interval"][0] <= be <= p["bit_interval"][1] and
+                          p["block_interval"][0] <= ke <= p["block_interval"][1]
+                          for p, (be, ke) in zip(points, c.per_point))
+        c.ms = _decode(plan, batches[timed_at][0], post, reps)
         table.append(c)
         del plan
     ok = [c for c in table if c.qualifies]
     best = min(ok or table, key=lambda c: c.ms)
-    return {"k": k, "decoder": decoder, "iterations": iterations, "n_blocks": n_blocks, "ebn0_db": ebn0_db,
-            "exact_bit_errors": ref_err, "bits": nbits,
-            "interval": [math.floor(lo * nbits), math.ceil(hi * nbits)],
+    return {"k": k, "decoder": decoder, "iterations": iterations, "n_blocks": n_blocks, "ebn0_db": pts,
+            "timed_at_db": pts[timed_at], "bits": nbits, "points": points,
+            "exact_bit_errors": [p["exact_bit_errors"] for p in points],
+            "interval": [p["bit_interval"] for p in points],
             "best": asdict(best), "table": [asdict(c) for c in table]}
 
 
 def options_for(result: Dict) -> Dict[str, int]:
     """Keyword arguments for Plan / options() from a tune() result."""
     b = result["best"]
-    return {"min_segment": b["min_segment"], "warmup": b["warmup"], "waves_per_cu": b["waves_per_cu"]}
+    return {"min_segment": b["min_segment"], "warmup": b["warmup"], "waves_per_cu": b["waves_per_cu"],
+            "extrinsic_bits": b["extrinsic_bits"]}
