@@ -447,7 +447,9 @@ def turbo_decode_code_block_gpu_stage2(descriptor, d0, d1, d2, max_iterations, e
 
 def turbo_decode_code_block_gpu_stage1_tile8_experimental(descriptor, d0, d1, d2, max_iterations,
                                                           enable_early_stop):
-    return _gpu_decode(DECODER_EXACT, descriptor, d0, d1, d2, max_iterations, enable_early_stop)
+    """Exact Log-MAP on the GPU, one full-length recursion per block (the
+    serial form; the same posteriors bit for bit as the segment-parallel one)."""
+    return _gpu_decode(DECODER_EXACT_SERIAL, descriptor, d0, d1, d2, max_iterations, enable_early_stop)
 
 
 def segment_geometry(k: int, decoder: int = DECODER_WINDOW_LOG, min_segment: int = 0, warmup: int = 0,
