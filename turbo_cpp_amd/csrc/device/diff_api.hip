@@ -44,12 +44,12 @@ namespace {
 
 constexpr uint32_t kMaxHalves = 32;
 
-// Scratch elements per lane: staged sys+la and par, the own extrinsic
-// adjoints, alpha and beta over the window, the beta output adjoints, the
-// sys/par adjoints and six tail adjoints.
+// Scratch elements per lane: staged sys+la and par, alpha and beta over the
+// window, the beta output adjoints, the sys/par adjoints and six tail
+// adjoints.
 uint32_t slot_elems(uint32_t s, uint32_t g) {
   const uint32_t p = s + 2 * g;
-  return 4 * p + 9 * s + 16 * (g + s + 1) + 6;
+  return 4 * p + 8 * s + 16 * (g + s + 1) + 6;
 }
 
 void build(tc_diff_plan& p, const uint32_t* k, size_t n, uint32_t iterations, uint32_t min_segment, uint32_t warmup,

@@ -41,22 +41,34 @@
 namespace tcdev {
 namespace {
 
+// The recursions run in base 2: state metrics, branch metrics and LLR terms
+// are the natural-log quantities times log2(e), so each Jacobian logarithm is
+// one exp2 and one log2 with no scaling; an LLR difference is taken back to
+// natural units once (times ln 2).  The max* weights exp(x - y) / (1 + ...)
+// are the same numbers in either base, so the adjoints, which are built from
+// those weights and from the natural-unit derivatives of gamma (+-1/2 per
+// input), stay in natural units.
 template <typename R>
 struct Num;
 // float: the hardware exp2 / log2 / reciprocal (about 1 ulp); double: libm
 // and IEEE division, for the finite-difference and FP64 reference checks.
 template <>
 struct Num<float> {
-  static __device__ float ex(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
-  static __device__ float ln(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+  static __device__ float ex2(float x) { return __builtin_amdgcn_exp2f(x); }
+  static __device__ float lg2(float x) { return __builtin_amdgcn_logf(x); }
   static __device__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 };
 template <>
 struct Num<double> {
-  static __device__ double ex(double x) { return exp(x); }
-  static __device__ double ln(double x) { return log(x); }
+  static __device__ double ex2(double x) { return exp2(x); }
+  static __device__ double lg2(double x) { return log2(x); }
   static __device__ double rcp(double x) { return 1.0 / x; }
 };
+// half of log2(e): a branch metric's input scale in base 2 (gamma = +-x/2)
+template <typename R>
+constexpr R kHalfL2e = R(0.72134752044448170368);
+template <typename R>
+constexpr R kLn2 = R(0.69314718055994530942);
 
 constexpr float kNegF = -1.0e30f;
 
@@ -78,12 +90,12 @@ template <typename R>
 __device__ inline R jac(R a, R b) {
   const R m = fmax(a, b);
   const R d = fabs(a - b);
-  return m + Num<R>::ln(R(1) + Num<R>::ex(-d));
+  return m + Num<R>::lg2(R(1) + Num<R>::ex2(-d));
 }
 template <typename R>
 __device__ inline void jac_w(R a, R b, R& wa, R& wb) {
   const R d = fabs(a - b);
-  const R e = Num<R>::ex(-d);
+  const R e = Num<R>::ex2(-d);
   const R big = Num<R>::rcp(R(1) + e);
   const R small = e * big;
   wa = a >= b ? big : small;
@@ -105,9 +117,14 @@ __device__ inline void normalise(R (&v)[8]) {
 }
 
 // alpha_{t+1} from alpha_t: each state's two incoming transitions in order
-// of increasing predecessor state.
+// of increasing predecessor state.  Metrics are renormalised (maximum 0) only
+// where nrm is set: when the step ends a chunk of 8 local steps, so every
+// value a recompute from a checkpoint reproduces was computed the same way
+// the first time (the LLRs and the adjoints do not depend on where the
+// constant offsets were taken out; within 8 steps the metrics drift by at most
+// 8 branch metrics).
 template <typename R>
-__device__ inline void alpha_step(const R (&a)[8], R hs, R hp, R (&o)[8]) {
+__device__ inline void alpha_step(const R (&a)[8], R hs, R hp, R (&o)[8], bool nrm) {
   bool seen[8] = {false, false, false, false, false, false, false, false};
 #pragma unroll
   for (unsigned s = 0; s < 8; ++s)
@@ -118,19 +135,19 @@ __device__ inline void alpha_step(const R (&a)[8], R hs, R hp, R (&o)[8]) {
       o[n] = seen[n] ? jac(o[n], c) : c;
       seen[n] = true;
     }
-  normalise(o);
+  if (nrm) normalise(o);
 }
 
 template <typename R>
-__device__ inline void beta_step(const R (&b)[8], R hs, R hp, R (&o)[8]) {
+__device__ inline void beta_step(const R (&b)[8], R hs, R hp, R (&o)[8], bool nrm) {
 #pragma unroll
   for (unsigned s = 0; s < 8; ++s)
     o[s] = jac(b[nxt(s, 0)] + gam(0u, par(s, 0), hs, hp), b[nxt(s, 1)] + gam(1u, par(s, 1), hs, hp));
-  normalise(o);
+  if (nrm) normalise(o);
 }
 
-// L_u = log sum_s exp(alpha(s) + gamma(s,u) + beta(next(s,u))) and, when
-// P is given, the weights P[u][s] = exp(m(s,u) - L_u).
+// L_u = log2 sum_s 2^(alpha(s) + gamma(s,u) + beta(next(s,u))) (base-2
+// metrics) and, when P is given, the weights P[u][s] = 2^(m(s,u) - L_u).
 template <typename R>
 __device__ inline void llr_terms(const R (&a)[8], const R (&b)[8], R hs, R hp, R (&L)[2], R (*P)[8]) {
 #pragma unroll
@@ -145,10 +162,10 @@ __device__ inline void llr_terms(const R (&a)[8], const R (&b)[8], R hs, R hp, R
     R sum = 0;
 #pragma unroll
     for (unsigned s = 0; s < 8; ++s) {
-      m[s] = Num<R>::ex(m[s] - mx);
+      m[s] = Num<R>::ex2(m[s] - mx);
       sum += m[s];
     }
-    L[u] = mx + Num<R>::ln(sum);
+    L[u] = mx + Num<R>::lg2(sum);
     if (P) {
       const R inv = Num<R>::rcp(sum);
 #pragma unroll
@@ -167,7 +184,7 @@ __device__ inline void tail_beta(const R (&ts)[3], const R (&tp)[3], R (&o)[8]) 
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const unsigned u = term_in(cur);
-      m += gam(u, par(cur, u), R(0.5) * ts[j], R(0.5) * tp[j]);
+      m += gam(u, par(cur, u), kHalfL2e<R> * ts[j], kHalfL2e<R> * tp[j]);
       cur = nxt(cur, u);
     }
     o[s] = m;  // every state reaches 0 in three termination steps
@@ -210,12 +227,11 @@ struct QppWalk {
 // Per-lane scratch fields, each [j][lane] with j < its length.
 struct Layout {
   uint32_t P, S, g;
-  uint32_t sa, pr, eb, al, be, bo, dsa, dpr, tl, total;
+  uint32_t sa, pr, al, be, bo, dsa, dpr, tl, total;
   __device__ Layout(uint32_t s, uint32_t gw) : P(s + 2 * gw), S(s), g(gw) {
     sa = 0;
     pr = sa + P;
-    eb = pr + P;
-    al = eb + S;                 // (g + S + 1) x 8
+    al = pr + P;                 // (g + S + 1) x 8
     be = al + (g + S + 1) * 8;   // (S + g + 1) x 8
     bo = be + (S + g + 1) * 8;   // S x 8
     dsa = bo + S * 8;
@@ -264,10 +280,10 @@ __device__ __forceinline__ Lane<R> fresh(const Lane<R>& L0) {
   return L;
 }
 
-// Inlining: the recursion passes are inlined into the kernels; the LDS fills
-// and write-backs and the global-memory fallbacks of staging and combine are
-// separate functions, so their batches of loads do not raise the register
-// count of the kernel around the passes.
+// Inlining: everything is inlined into the kernels.  A job is one wave and
+// the kernels run one wave per SIMD, so a lane has 512 registers (256 VGPRs
+// and 256 AGPRs); a call would instead save and restore the callee's
+// registers through private memory on every half-iteration.
 //
 // Memory order and control flow of every sequential pass below.  A job is
 // one wave and a batch rarely has more jobs than SIMDs, so no other wave
@@ -361,7 +377,7 @@ struct StageIn {
 };
 
 template <typename R>
-__device__ __attribute__((noinline)) void stage(const Lane<R> L0, const Layout Y, uint32_t h, const R* d0, const R* d1,
+__device__ __forceinline__ void stage(const Lane<R> L0, const Layout Y, uint32_t h, const R* d0, const R* d1,
                                       const R* d2, const R* sv, const R* scales) {
   const Lane<R> L = fresh(L0);
   const R w = h ? scales[h - 1] : R(0);
@@ -440,37 +456,47 @@ constexpr uint32_t kFillBatch = 16;  // loads in flight per lane and array
 
 // Fill of the staging arrays: 0 = sys + la at natural position i (the lower
 // half reads it at Pi(t)), 1 = the half's parity stream.  Same arithmetic as
-// the per-lane staging.
+// the per-lane staging.  Two blocks per batch of loads, so a job of two
+// blocks waits for memory once.
 template <typename R>
-__device__ __attribute__((noinline)) void stage_fill(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
+__device__ __forceinline__ void stage_fill(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const R w = h ? A.scales[h - 1] : R(0);
   const bool lower = (h & 1u) != 0;
   const uint32_t K = J.k;
   const LdsPtr<R> xl = lds_view(X);
-  for (uint32_t b = 0; b < J.nblk; ++b) {
-    const DiffCb C = A.cbs[J.cb_first + b];
-    const GlobalPtr<const R> d0 = global_view(A.llr + C.llr_off);
-    const GlobalPtr<const R> dp = d0 + (size_t)(lower ? 2 : 1) * (K + 4);
-    const GlobalPtr<const R> ep = global_view(A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K);
+  for (uint32_t b0 = 0; b0 < J.nblk; b0 += 2) {
+    const uint32_t nb = min(2u, J.nblk - b0);
+    GlobalPtr<const R> d0[2], dp[2], ep[2];
+#pragma unroll
+    for (uint32_t x = 0; x < 2; ++x) {
+      const DiffCb C = A.cbs[J.cb_first + b0 + (x < nb ? x : 0u)];
+      d0[x] = global_view(A.llr + C.llr_off);
+      dp[x] = d0[x] + (size_t)(lower ? 2 : 1) * (K + 4);
+      ep[x] = global_view(A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K);
+    }
     for (uint32_t i0 = threadIdx.x; i0 < K; i0 += 64 * kFillBatch) {
-      R sv[kFillBatch], ev[kFillBatch], pv[kFillBatch];
+      R sv[2][kFillBatch], ev[2][kFillBatch], pv[2][kFillBatch];
 #pragma unroll
-      for (uint32_t u = 0; u < kFillBatch; ++u) {
-        const uint32_t i = i0 + 64 * u;
-        if (i < K) {
-          sv[u] = d0[i];
-          ev[u] = h ? ep[i] : R(0);
-          pv[u] = dp[i];
-        }
-      }
+      for (uint32_t x = 0; x < 2; ++x)
 #pragma unroll
-      for (uint32_t u = 0; u < kFillBatch; ++u) {
-        const uint32_t i = i0 + 64 * u;
-        if (i < K) {
-          xl[X.idx(0, b, i)] = lower ? sv[u] + w * ev[u] : sv[u] + (h ? w * ev[u] : R(0));
-          xl[X.idx(1, b, i)] = pv[u];
+        for (uint32_t u = 0; u < kFillBatch; ++u) {
+          const uint32_t i = i0 + 64 * u;
+          if (x < nb && i < K) {
+            sv[x][u] = d0[x][i];
+            ev[x][u] = h ? ep[x][i] : R(0);
+            pv[x][u] = dp[x][i];
+          }
         }
-      }
+#pragma unroll
+      for (uint32_t x = 0; x < 2; ++x)
+#pragma unroll
+        for (uint32_t u = 0; u < kFillBatch; ++u) {
+          const uint32_t i = i0 + 64 * u;
+          if (x < nb && i < K) {
+            xl[X.idx(0, b0 + x, i)] = lower ? sv[x][u] + w * ev[x][u] : sv[x][u] + (h ? w * ev[x][u] : R(0));
+            xl[X.idx(1, b0 + x, i)] = pv[x][u];
+          }
+        }
     }
   }
 }
@@ -480,7 +506,7 @@ __device__ __attribute__((noinline)) void stage_fill(const DiffArgs<R> A, const 
 // half's order, zeros outside [0, K).  Each lane takes a run of consecutive
 // positions (the lower half's interleaver walked by its recurrence).
 template <typename R>
-__device__ __attribute__((noinline)) void stage_gather(const DiffJob J, uint32_t h, const BlockLds<R> X) {
+__device__ __forceinline__ void stage_gather(const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const LdsPtr<R> xl = lds_view(X);
   const bool lower = (h & 1u) != 0;
   const uint32_t K = J.k, g = J.warm, n = K + 2 * g;
@@ -580,7 +606,7 @@ __device__ __forceinline__ void beta_ext_pass(const Lane<R>& L0, const Layout& Y
         for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
         if (jj + 1 < 8 && j0 + jj + 1 < own_end) {
           R o[8];
-          alpha_step(a, R(0.5) * x.in.sa[jj], R(0.5) * x.in.pr[jj], o);
+          alpha_step(a, kHalfL2e<R> * x.in.sa[jj], kHalfL2e<R> * x.in.pr[jj], o, false);
           select8(L.t_of((uint32_t)(j0 + jj)) >= 0, o, a);
         }
       }
@@ -593,12 +619,12 @@ __device__ __forceinline__ void beta_ext_pass(const Lane<R>& L0, const Layout& Y
         const int t = L.t_of((uint32_t)j);
         {
           R Lu[2];
-          llr_terms(areg[jj], b, R(0.5) * sa, R(0.5) * pr, Lu, (R(*)[8]) nullptr);
-          e[(h & 1u) ? qw.pi : (uint32_t)t] = Lu[1] - Lu[0] - sa;
+          llr_terms(areg[jj], b, kHalfL2e<R> * sa, kHalfL2e<R> * pr, Lu, (R(*)[8]) nullptr);
+          e[(h & 1u) ? qw.pi : (uint32_t)t] = (Lu[1] - Lu[0]) * kLn2<R> - sa;
           if (j > g) qw.prev();
         }
         R o[8];
-        beta_step(b, R(0.5) * sa, R(0.5) * pr, o);
+        beta_step(b, kHalfL2e<R> * sa, kHalfL2e<R> * pr, o, ((j - g) & 7) == 0);
         select8(t < (int)L.K, o, b);
       }
     }
@@ -649,14 +675,14 @@ __device__ __forceinline__ void alpha_ckpt_beta_warm_pass(const Lane<R>& L0, con
       const uint32_t j = j0 + jj;
       if (j < n) {
         R o[8];
-        alpha_step(a, R(0.5) * c.up.sa[jj], R(0.5) * c.up.pr[jj], o);
+        alpha_step(a, kHalfL2e<R> * c.up.sa[jj], kHalfL2e<R> * c.up.pr[jj], o, ((j + 1) & 7u) == 0);
         select8(L.t_of(j) >= 0, o, a);
       }
       if (j < L.g) {
         const int jb = top - 1 - (int)j;
         beta_ck((uint32_t)jb + 1 - L.g, bcur);
         R o[8];
-        beta_step(bcur, R(0.5) * c.dn.sa[jj], R(0.5) * c.dn.pr[jj], o);
+        beta_step(bcur, kHalfL2e<R> * c.dn.sa[jj], kHalfL2e<R> * c.dn.pr[jj], o, (((uint32_t)jb - L.g) & 7u) == 0);
         select8(L.t_of((uint32_t)jb) < (int)L.K, o, bcur);
       }
     }
@@ -888,7 +914,10 @@ __device__ __forceinline__ void post_adjoint(const DiffArgs<R>& A, const Lane<R>
 // 8 (alpha recomputed from the checkpoint), the output adjoints of the own
 // steps on the way, and beta of the own steps (bcur: beta at local g + S on
 // entry), whose checkpoints beta_ck keeps for the beta-adjoint sweep.
-// Writes dsa / dpr of local steps below g + S (their first writer).
+// Writes dsa / dpr of local steps below g + S (their first writer).  The
+// adjoints of the own steps' extrinsics are read from ecur (natural order:
+// position t, or Pi(t) in a lower half), a sub-window ahead like the other
+// inputs.
 template <typename R>
 struct AlphaIn {
   R a0[8], e[8];
@@ -896,19 +925,29 @@ struct AlphaIn {
 };
 
 template <typename R, class Ck>
-__device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Layout& Y, R (&bcur)[8], Ck&& beta_ck) {
+__device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Layout& Y, R (&bcur)[8], Ck&& beta_ck,
+                                                    bool lower, const R* ecur) {
   const Lane<R> L = fresh(L0);
   R ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const uint32_t n_al = L.g + L.S;
+  const GlobalPtr<const R> eg = global_view(ecur);
+  // own steps' positions, walked down from the last one (loads go in
+  // descending order of steps)
+  QppWalk qw((uint32_t)L.t_of(n_al - 1), L.K, L.f1, L.f2);
   auto load = [&](int c, AlphaIn<R>& x) {
     const uint32_t j0 = 8u * (uint32_t)c, len = min(8u, n_al - j0);
     load8(L, Y.al, Y.g + Y.S + 1, (uint32_t)c, x.a0);
     load_in8(L, Y, (int)j0, (int)len, x.in);
 #pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
-      if (jj < len) {
-        const uint32_t j = j0 + jj;
-        x.e[jj] = (j >= L.g) ? L.at(Y.eb, j - L.g) : R(0);
+    for (int jj = 7; jj >= 0; --jj) {
+      if ((uint32_t)jj < len) {
+        const uint32_t j = j0 + (uint32_t)jj;
+        if (j >= L.g) {
+          x.e[jj] = eg[lower ? qw.pi : (uint32_t)L.t_of(j)];
+          if (j > L.g) qw.prev();
+        } else {
+          x.e[jj] = R(0);
+        }
       }
     }
   };
@@ -926,7 +965,7 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
           for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
           if (jj + 1 < len) {
             R o[8];
-            alpha_step(a, R(0.5) * x.in.sa[jj], R(0.5) * x.in.pr[jj], o);
+            alpha_step(a, kHalfL2e<R> * x.in.sa[jj], kHalfL2e<R> * x.in.pr[jj], o, false);
             select8(L.t_of(j0 + jj) >= 0, o, a);
           }
         }
@@ -939,13 +978,13 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
         R b[8];
 #pragma unroll
         for (int s = 0; s < 8; ++s) b[s] = bcur[s];  // beta_{j+1} while j >= g
-        const R hs = R(0.5) * x.in.sa[jj], hp = R(0.5) * x.in.pr[jj];
+        const R hs = kHalfL2e<R> * x.in.sa[jj], hp = kHalfL2e<R> * x.in.pr[jj];
         if (j >= L.g) {
           // own step (inside the block for every lane): checkpoint
           // beta_{j+1}, step beta to beta_j
           beta_ck(j + 1 - L.g, b);
           R o[8];
-          beta_step(b, hs, hp, o);
+          beta_step(b, hs, hp, o, ((j - L.g) & 7u) == 0);
 #pragma unroll
           for (int s = 0; s < 8; ++s) bcur[s] = o[s];
         }
@@ -1015,7 +1054,7 @@ __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layo
           for (int s = 0; s < 8; ++s) breg[jj][s] = b[s];  // beta at local j + 1
           if (jj > 0) {
             R o[8];
-            beta_step(b, R(0.5) * x.in.sa[jj], R(0.5) * x.in.pr[jj], o);
+            beta_step(b, kHalfL2e<R> * x.in.sa[jj], kHalfL2e<R> * x.in.pr[jj], o, false);
             select8(L.t_of(jb0 + (uint32_t)jj) < (int)L.K, o, b);
           }
         }
@@ -1031,7 +1070,7 @@ __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layo
           nb[s] = bb[s];
           bn[s] = bor[jj][s];
         }
-        beta_adjoint_step(breg[jj], R(0.5) * x.in.sa[jj], R(0.5) * x.in.pr[jj], nb, bn, ds, dp);
+        beta_adjoint_step(breg[jj], kHalfL2e<R> * x.in.sa[jj], kHalfL2e<R> * x.in.pr[jj], nb, bn, ds, dp);
         // past the block end the adjoint stops (beta is not stepped there)
         select8(L.t_of(j) < (int)L.K, nb, bb);
         L.at(Y.dsa, j) = dsr[jj] + ds;
@@ -1054,7 +1093,7 @@ struct CombIn {
 };
 
 template <typename R>
-__device__ __attribute__((noinline)) R combine(const Lane<R> L0, const Layout Y, uint32_t h, R w, const R* eprev, R* g0,
+__device__ __forceinline__ R combine(const Lane<R> L0, const Layout Y, uint32_t h, R w, const R* eprev, R* g0,
                                      R* gpar, R* prev) {
   const Lane<R> L = fresh(L0);
   const bool lower = (h & 1u) != 0;
@@ -1124,48 +1163,57 @@ __device__ __attribute__((noinline)) R combine(const Lane<R> L0, const Layout Y,
 // half's parity stream, 2 = the adjoint of e_{h-1} started as the per-lane
 // path starts it (the posterior term for h - 1 == H - 2, else zero), 3 =
 // e_{h-1}.  The lanes then add into the copies in the order of combine; the
-// write-back stores arrays 0-2.
+// write-back stores arrays 0-2.  Two blocks per batch of loads.
 template <typename R>
-__device__ __attribute__((noinline)) void combine_fill(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
+__device__ __forceinline__ void combine_fill(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const uint32_t H = A.halves, K = J.k;
   const bool lower = (h & 1u) != 0, with_prev = h != 0;
   const bool from_post = with_prev && h - 1 == H - 2;
   const R wp = A.scales[H - 2];
   const LdsPtr<R> xl = lds_view(X);
-  for (uint32_t b = 0; b < J.nblk; ++b) {
-    const DiffCb C = A.cbs[J.cb_first + b];
-    const GlobalPtr<const R> g0 = global_view<R>(A.grad_llr + C.llr_off);
-    const GlobalPtr<const R> gq = g0 + (size_t)(lower ? 2 : 1) * (K + 4);
-    const GlobalPtr<const R> gp = global_view(A.grad_post + C.post_off);
-    const GlobalPtr<const R> ep = global_view(A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K);
+  for (uint32_t b0 = 0; b0 < J.nblk; b0 += 2) {
+    const uint32_t nb = min(2u, J.nblk - b0);
+    GlobalPtr<const R> g0[2], gq[2], gp[2], ep[2];
+#pragma unroll
+    for (uint32_t x = 0; x < 2; ++x) {
+      const DiffCb C = A.cbs[J.cb_first + b0 + (x < nb ? x : 0u)];
+      g0[x] = global_view<R>(A.grad_llr + C.llr_off);
+      gq[x] = g0[x] + (size_t)(lower ? 2 : 1) * (K + 4);
+      gp[x] = global_view(A.grad_post + C.post_off);
+      ep[x] = global_view(A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K);
+    }
     for (uint32_t i0 = threadIdx.x; i0 < K; i0 += 64 * kFillBatch) {
-      R x0[kFillBatch], x1[kFillBatch], x2[kFillBatch], x3[kFillBatch];
+      R x0[2][kFillBatch], x1[2][kFillBatch], x2[2][kFillBatch], x3[2][kFillBatch];
 #pragma unroll
-      for (uint32_t u = 0; u < kFillBatch; ++u) {
-        const uint32_t i = i0 + 64 * u;
-        if (i < K) {
-          x0[u] = g0[i];
-          x1[u] = gq[i];
-          x2[u] = from_post ? gp[i] : R(0);
-          x3[u] = with_prev ? ep[i] : R(0);
-        }
-      }
+      for (uint32_t x = 0; x < 2; ++x)
 #pragma unroll
-      for (uint32_t u = 0; u < kFillBatch; ++u) {
-        const uint32_t i = i0 + 64 * u;
-        if (i < K) {
-          xl[X.idx(0, b, i)] = x0[u];
-          xl[X.idx(1, b, i)] = x1[u];
-          xl[X.idx(2, b, i)] = from_post ? wp * x2[u] : R(0);
-          xl[X.idx(3, b, i)] = x3[u];
+        for (uint32_t u = 0; u < kFillBatch; ++u) {
+          const uint32_t i = i0 + 64 * u;
+          if (x < nb && i < K) {
+            x0[x][u] = g0[x][i];
+            x1[x][u] = gq[x][i];
+            x2[x][u] = from_post ? gp[x][i] : R(0);
+            x3[x][u] = with_prev ? ep[x][i] : R(0);
+          }
         }
-      }
+#pragma unroll
+      for (uint32_t x = 0; x < 2; ++x)
+#pragma unroll
+        for (uint32_t u = 0; u < kFillBatch; ++u) {
+          const uint32_t i = i0 + 64 * u;
+          if (x < nb && i < K) {
+            xl[X.idx(0, b0 + x, i)] = x0[x][u];
+            xl[X.idx(1, b0 + x, i)] = x1[x][u];
+            xl[X.idx(2, b0 + x, i)] = from_post ? wp * x2[x][u] : R(0);
+            xl[X.idx(3, b0 + x, i)] = x3[x][u];
+          }
+        }
     }
   }
 }
 
 template <typename R>
-__device__ __attribute__((noinline)) void combine_store(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
+__device__ __forceinline__ void combine_store(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const uint32_t K = J.k;
   const bool lower = (h & 1u) != 0, with_prev = h != 0;
   const LdsPtr<R> xl = lds_view(X);
@@ -1183,7 +1231,13 @@ __device__ __attribute__((noinline)) void combine_store(const DiffArgs<R> A, con
 }
 
 // Per-lane part of the LDS combine (the same sums, in the same order, as
-// combine).
+// combine), with the next chunk's scratch loads in flight while a chunk is
+// added into the LDS copies.
+template <typename R>
+struct CombLdsIn {
+  R ds[8], dp[8], dsn[8], dpn[8], dsp[8], dpp[8];
+};
+
 template <typename R>
 __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uint32_t h, R w, const BlockLds<R>& X,
                                          uint32_t blk) {
@@ -1195,24 +1249,25 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
   const uint32_t ln = min(L.lane + 1, 63u), lp = L.lane > 0 ? L.lane - 1 : 0u;
   QppWalk qw(L.q * L.S, L.K, L.f1, L.f2);
   R wsum = 0;
-  for (uint32_t j0 = L.g; j0 < own_end; j0 += 8) {
-    R ds[8], dp[8], dsn[8], dpn[8], dsp[8], dpp[8];
+  auto load = [&](uint32_t j0, CombLdsIn<R>& x) {
 #pragma unroll
     for (uint32_t jj = 0; jj < 8; ++jj) {
       const uint32_t j = j0 + jj;
       if (j < own_end) {
-        ds[jj] = L.at(Y.dsa, j);
-        dp[jj] = L.at(Y.dpr, j);
+        x.ds[jj] = L.at(Y.dsa, j);
+        x.dp[jj] = L.at(Y.dpr, j);
         if (j >= L.S) {  // t in [qS + S - g, qS + S): lane q+1's local j - S
-          dsn[jj] = L.nb(Y.dsa, j - L.S, ln);
-          dpn[jj] = L.nb(Y.dpr, j - L.S, ln);
+          x.dsn[jj] = L.nb(Y.dsa, j - L.S, ln);
+          x.dpn[jj] = L.nb(Y.dpr, j - L.S, ln);
         }
         if (j < 2 * L.g) {  // t in [qS, qS + g): lane q-1's local j + S
-          dsp[jj] = L.nb(Y.dsa, j + L.S, lp);
-          dpp[jj] = L.nb(Y.dpr, j + L.S, lp);
+          x.dsp[jj] = L.nb(Y.dsa, j + L.S, lp);
+          x.dpp[jj] = L.nb(Y.dpr, j + L.S, lp);
         }
       }
     }
+  };
+  auto body = [&](uint32_t j0, const CombLdsIn<R>& x) {
 #pragma unroll
     for (uint32_t jj = 0; jj < 8; ++jj) {
       const uint32_t j = j0 + jj;
@@ -1220,14 +1275,14 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
         const uint32_t t = (uint32_t)L.t_of(j);
         const uint32_t i = lower ? qw.pi : t;
         qw.next();
-        R a = ds[jj], c = dp[jj];
+        R a = x.ds[jj], c = x.dp[jj];
         if (j >= L.S) {
-          a = has_next ? a + dsn[jj] : a;
-          c = has_next ? c + dpn[jj] : c;
+          a = has_next ? a + x.dsn[jj] : a;
+          c = has_next ? c + x.dpn[jj] : c;
         }
         if (j < 2 * L.g) {
-          a = has_prev ? a + dsp[jj] : a;
-          c = has_prev ? c + dpp[jj] : c;
+          a = has_prev ? a + x.dsp[jj] : a;
+          c = has_prev ? c + x.dpp[jj] : c;
         }
         X.at(0, blk, i) = X.at(0, blk, i) + a;
         X.at(1, blk, t) = X.at(1, blk, t) + c;
@@ -1237,28 +1292,9 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
         }
       }
     }
-  }
+  };
+  chunks_up<CombLdsIn<R>>(L.g, own_end, load, body);
   return wsum;
-}
-
-// LDS path of the extrinsic-adjoint gather: array 0 = the adjoint of e_h.
-template <typename R>
-__device__ __attribute__((noinline)) void eb_fill(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
-  const uint32_t K = J.k;
-  const LdsPtr<R> xl = lds_view(X);
-  for (uint32_t b = 0; b < J.nblk; ++b) {
-    const DiffCb C = A.cbs[J.cb_first + b];
-    const GlobalPtr<const R> cur = global_view<R>(A.ebar + C.ebar_off + (size_t)(h & 1u) * K);
-    for (uint32_t i0 = threadIdx.x; i0 < K; i0 += 64 * kFillBatch) {
-      R x[kFillBatch];
-#pragma unroll
-      for (uint32_t u = 0; u < kFillBatch; ++u)
-        if (i0 + 64 * u < K) x[u] = cur[i0 + 64 * u];
-#pragma unroll
-      for (uint32_t u = 0; u < kFillBatch; ++u)
-        if (i0 + 64 * u < K) xl[X.idx(0, b, i0 + 64 * u)] = x[u];
-    }
-  }
 }
 
 }  // namespace
@@ -1312,8 +1348,6 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
         stage_gather(J, h, X);
         if (L.active) tail_inputs(h, d0, d1, d2, K, ts, tp);
         __syncthreads();
-        eb_fill(A, J, h, X);
-        __syncthreads();
       } else if (L.active) {
         // adjoint of e_{h-1}: starts at the posterior term for h - 1 == H - 2
         // (the LDS path starts it in its combine fill)
@@ -1337,24 +1371,6 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
         stage(L, Y, h, d0, d1, d2, sv, A.scales);
       }
       if (L.active) {
-        {
-          // the adjoints of the own steps' extrinsics, into the eb field
-          QppWalk qw(lo, K, L.f1, L.f2);
-          for (uint32_t j0 = L.g; j0 < L.g + L.S; j0 += 8) {
-            R v[8];
-#pragma unroll
-            for (uint32_t jj = 0; jj < 8; ++jj)
-              if (j0 + jj < L.g + L.S) {
-                const uint32_t t = (uint32_t)L.t_of(j0 + jj);
-                const uint32_t i = lower ? qw.pi : t;
-                v[jj] = A.use_lds ? X.at(0, blk, i) : cur[i];
-                qw.next();
-              }
-#pragma unroll
-            for (uint32_t jj = 0; jj < 8; ++jj)
-              if (j0 + jj < L.g + L.S) L.at(Y.eb, j0 + jj - L.g) = v[jj];
-          }
-        }
         // the alpha-adjoint sweep is the first writer of local steps below
         // g + S, the beta-adjoint sweep of those up to 2g + S; zeros beyond
         // (a plan's longer windows) keep the combine's reads defined
@@ -1383,7 +1399,7 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
           }
         };
         alpha_ckpt_beta_warm_pass(L, Y, bcur, beta_ck);
-        alpha_adjoint_sweep(L, Y, bcur, beta_ck);
+        alpha_adjoint_sweep(L, Y, bcur, beta_ck, lower, cur);
         R bb[8];
         beta_adjoint_sweep(L, Y, bb);
         // beta of the window top: the tail beta when it reaches K
