@@ -13,9 +13,9 @@
 #include "turbo_cpp/c_api.h"
 
 namespace tcdev {
-template <typename R>
+template <typename R, bool kLds>
 __global__ void diff_forward_kernel(DiffArgs<R>);
-template <typename R>
+template <typename R, bool kLds>
 __global__ void diff_backward_kernel(DiffArgs<R>);
 template <typename R>
 __global__ void diff_scale_reduce_kernel(const R*, uint32_t, uint32_t, R*);
@@ -44,13 +44,16 @@ namespace {
 
 constexpr uint32_t kMaxHalves = 32;
 
-// Scratch elements per lane: staged sys+la and par, alpha and beta over the
-// window, the beta output adjoints, the sys/par adjoints and six tail
-// adjoints.
+// Scratch elements per lane: staged sys+la and par, alpha and beta
+// checkpoints, the LLR's adjoint terms of the own steps, the sys/par adjoints
+// of the alpha and the beta side and six tail adjoints.
 uint32_t slot_elems(uint32_t s, uint32_t g) {
   const uint32_t p = s + 2 * g;
-  return 4 * p + 8 * s + 16 * (g + s + 1) + 6;
+  return 6 * p + 8 * s + 16 * (g + s + 1) + 6;
 }
+
+// A job runs on a workgroup of two waves (alpha side and beta side).
+constexpr uint32_t kDiffThreads = 128;
 
 void build(tc_diff_plan& p, const uint32_t* k, size_t n, uint32_t iterations, uint32_t min_segment, uint32_t warmup,
            bool fp64) {
@@ -160,16 +163,22 @@ tcdev::DiffArgs<R> args(const tc_diff_plan& p, const void* llr, const void* scal
 
 template <typename R>
 void forward(const tc_diff_plan& p, const void* llr, const void* scales, void* post, void* saved, hipStream_t s) {
-  hipLaunchKernelGGL(tcdev::diff_forward_kernel<R>, dim3(p.grid), dim3(64), p.lds_bytes, s,
-                     args<R>(p, llr, scales, post, saved, nullptr, nullptr));
+  const auto a = args<R>(p, llr, scales, post, saved, nullptr, nullptr);
+  if (p.lds_bytes)
+    hipLaunchKernelGGL((tcdev::diff_forward_kernel<R, true>), dim3(p.grid), dim3(kDiffThreads), p.lds_bytes, s, a);
+  else
+    hipLaunchKernelGGL((tcdev::diff_forward_kernel<R, false>), dim3(p.grid), dim3(kDiffThreads), 0, s, a);
   hip_check(hipGetLastError(), "differentiable decoder forward launch");
 }
 
 template <typename R>
 void backward(const tc_diff_plan& p, const void* llr, const void* scales, const void* saved, const void* grad_post,
               void* grad_llr, void* grad_scales, hipStream_t s) {
-  hipLaunchKernelGGL(tcdev::diff_backward_kernel<R>, dim3(p.grid), dim3(64), p.lds_bytes, s,
-                     args<R>(p, llr, scales, nullptr, const_cast<void*>(saved), grad_post, grad_llr));
+  const auto a = args<R>(p, llr, scales, nullptr, const_cast<void*>(saved), grad_post, grad_llr);
+  if (p.lds_bytes)
+    hipLaunchKernelGGL((tcdev::diff_backward_kernel<R, true>), dim3(p.grid), dim3(kDiffThreads), p.lds_bytes, s, a);
+  else
+    hipLaunchKernelGGL((tcdev::diff_backward_kernel<R, false>), dim3(p.grid), dim3(kDiffThreads), 0, s, a);
   hip_check(hipGetLastError(), "differentiable decoder backward launch");
   hipLaunchKernelGGL(tcdev::diff_scale_reduce_kernel<R>, dim3(p.halves), dim3(256), 0, s,
                      static_cast<const R*>(p.d_part), p.njobs, p.halves, static_cast<R*>(grad_scales));

@@ -8,27 +8,34 @@
 // (lte_turbo.cpp:410-486): gamma = (x_u (sys + la) + z_p par) / 2,
 // alpha/beta by the Jacobian logarithm, ext = L1 - L0 - sys - la.
 //
-// Parallel layout: one wave per job of nblk code blocks of one K; each block
-// is cut into M = K / S segments, lane = block * M + segment.  Segment q owns
-// steps [qS, qS + S): its alpha starts from zero metrics g steps early (from
-// state 0 when that reaches step 0) and its beta from zero metrics g steps
-// late (from the tail beta when that reaches K).  A segment therefore
-// depends only on the half-iteration's inputs in [qS - g, qS + S + g), so
-// each half-iteration is a function of the previous one's extrinsics and the
-// adjoint is local too: a segment's backward recomputes its alpha and beta
-// over that window (the checkpoints are the segment boundaries; nothing but
-// the extrinsics of every half is stored by the forward pass), then runs
-//   - the output adjoint of the extrinsic (softmax weights of L1 and L0),
-//   - the alpha adjoint backwards in time (weights of each state's two
-//     incoming transitions),
-//   - the beta adjoint forwards in time (weights of each state's two
-//     outgoing transitions),
-// and reduces the branch-metric adjoints onto sys + la and par.  Warm-up
-// steps belong to neighbouring segments; their contributions are added in a
-// fixed order (own, next segment's alpha warm-up, previous segment's beta
-// warm-up), so the result is deterministic.  Per-step renormalisation does
-// not change any output (the LLRs are invariant to adding a constant to all
-// eight metrics of a step), so it has no adjoint.
+// Parallel layout: a job is nblk code blocks of one K run by a workgroup of
+// two waves; each block is cut into M = K / S segments and lane =
+// block * M + segment in both waves.  Segment q owns steps [qS, qS + S): its
+// alpha starts from zero metrics g steps early (from state 0 when that
+// reaches step 0) and its beta from zero metrics g steps late (from the tail
+// beta when that reaches K).  A segment therefore depends only on the
+// half-iteration's inputs in [qS - g, qS + S + g), so each half-iteration is
+// a function of the previous one's extrinsics and the adjoint is local too.
+// Wave 0 carries a segment's alpha side and wave 1 its beta side; per half:
+//   1. wave 0 runs alpha over the segment's window, wave 1 beta, each keeping
+//      a checkpoint every 8 steps;
+//   2. the two waves split the own steps: from the checkpoints each
+//      recomputes alpha_{t+1} and beta_{t+1} of its steps and forms the
+//      LLR from the eight states at t + 1 (the input bit of a transition is
+//      fixed by the state it enters): the forward stores the extrinsic, the
+//      adjoint stores G(t, n) = dloss/d(alpha_{t+1}(n) + beta_{t+1}(n)), the
+//      LLR's share of both metrics' adjoints;
+//   3. (adjoint) wave 0 runs the alpha adjoint backwards in time (weights of
+//      each state's two incoming transitions, seeded with G), wave 1 the
+//      beta adjoint forwards in time (weights of each state's two outgoing
+//      transitions, seeded with G), each into its own sums of the
+//      branch-metric adjoints on sys + la and par;
+//   4. (adjoint) the two waves split the own steps again and add the sums
+//      into the LLR adjoints in a fixed order (own alpha side + own beta
+//      side, next segment's alpha warm-up, previous segment's beta warm-up),
+//      so the result is deterministic.
+// Renormalisation does not change any output (the LLRs are invariant to
+// adding a constant to all eight metrics of a step), so it has no adjoint.
 //
 // Per-lane scratch is laid out [field][j][lane] (j = local step), so lanes in
 // lockstep access consecutive addresses.  Templated on the real type: the
@@ -146,34 +153,6 @@ __device__ inline void beta_step(const R (&b)[8], R hs, R hp, R (&o)[8], bool nr
   if (nrm) normalise(o);
 }
 
-// L_u = log2 sum_s 2^(alpha(s) + gamma(s,u) + beta(next(s,u))) (base-2
-// metrics) and, when P is given, the weights P[u][s] = 2^(m(s,u) - L_u).
-template <typename R>
-__device__ inline void llr_terms(const R (&a)[8], const R (&b)[8], R hs, R hp, R (&L)[2], R (*P)[8]) {
-#pragma unroll
-  for (unsigned u = 0; u < 2; ++u) {
-    R m[8];
-    R mx = R(kNegF);
-#pragma unroll
-    for (unsigned s = 0; s < 8; ++s) {
-      m[s] = a[s] + gam(u, par(s, u), hs, hp) + b[nxt(s, u)];
-      mx = fmax(m[s], mx);
-    }
-    R sum = 0;
-#pragma unroll
-    for (unsigned s = 0; s < 8; ++s) {
-      m[s] = Num<R>::ex2(m[s] - mx);
-      sum += m[s];
-    }
-    L[u] = mx + Num<R>::lg2(sum);
-    if (P) {
-      const R inv = Num<R>::rcp(sum);
-#pragma unroll
-      for (unsigned s = 0; s < 8; ++s) P[u][s] = m[s] * inv;
-    }
-  }
-}
-
 // Tail beta (lte_turbo.cpp:357-378): three forced steps to state 0.
 template <typename R>
 __device__ inline void tail_beta(const R (&ts)[3], const R (&tp)[3], R (&o)[8]) {
@@ -227,16 +206,18 @@ struct QppWalk {
 // Per-lane scratch fields, each [j][lane] with j < its length.
 struct Layout {
   uint32_t P, S, g;
-  uint32_t sa, pr, al, be, bo, dsa, dpr, tl, total;
+  uint32_t sa, pr, al, be, gl, dsa, dpr, dsb, dpb, tl, total;
   __device__ Layout(uint32_t s, uint32_t gw) : P(s + 2 * gw), S(s), g(gw) {
     sa = 0;
     pr = sa + P;
-    al = pr + P;                 // (g + S + 1) x 8
-    be = al + (g + S + 1) * 8;   // (S + g + 1) x 8
-    bo = be + (S + g + 1) * 8;   // S x 8
-    dsa = bo + S * 8;
+    al = pr + P;                 // alpha checkpoints, 8 x (g + S + 1)
+    be = al + (g + S + 1) * 8;   // beta checkpoints, 8 x (S + g + 1)
+    gl = be + (S + g + 1) * 8;   // G of the own steps, 8 x S
+    dsa = gl + S * 8;            // alpha side: sys + la and par adjoints
     dpr = dsa + P;
-    tl = dpr + P;                // 6
+    dsb = dpr + P;               // beta side
+    dpb = dsb + P;
+    tl = dpb + P;                // 6
     total = tl + 6;
   }
 };
@@ -254,12 +235,12 @@ struct Lane {
   uint32_t K, S, M, g, q, f1, f2;
   bool active;
   uint32_t lane;
-  R* sc;  // this wave's scratch (uniform), element (field + j) * 64 + lane
+  R* sc;  // the job's scratch (uniform), element (field + j) * 64 + lane
   // staged inputs: the block's LDS pairs (lds; local step j is position
   // p = qS + j, pair at p + p / 32) or the per-lane sa / pr fields
   bool lds;
   LdsPtr<R> xs;
-  // 32-bit byte offsets from the wave's base (a slot is far below 4 GiB):
+  // 32-bit byte offsets from the job's base (a slot is far below 4 GiB):
   // one 32-bit offset per field and the steps as immediate offsets, not a
   // 64-bit address per access
   __device__ R& nb(uint32_t field, uint32_t j, uint32_t other) const {
@@ -280,25 +261,74 @@ __device__ __forceinline__ Lane<R> fresh(const Lane<R>& L0) {
   return L;
 }
 
-// Inlining: everything is inlined into the kernels.  A job is one wave and
-// the kernels run one wave per SIMD, so a lane has 512 registers (256 VGPRs
-// and 256 AGPRs); a call would instead save and restore the callee's
-// registers through private memory on every half-iteration.
+// Inlining: everything is inlined into the kernels, so no call saves and
+// restores registers through private memory.
 //
-// Memory order and control flow of every sequential pass below.  A job is
-// one wave and a batch rarely has more jobs than SIMDs, so no other wave
-// hides a wait.  Each pass therefore walks its steps in chunks of 8 with two
-// input buffers used in turn: the loads of chunk c + 1 go out before chunk c
-// is computed and stored, and no register copy sits between a load and its
-// use (a copy would wait for the load at the copy).  The vector-memory
-// counter retires loads and stores in issue order, so a load issued behind a
-// store would also wait for that store; issued ahead, it waits only for
-// stores of earlier chunks.  Loop bounds are wave-uniform (all lanes of a job
-// share K, S and g); steps that are outside the block for some lanes
-// (t < 0 in the first segment, t >= K in the last) are computed by every
-// lane and their results selected away, so no per-lane branch splits a
-// chunk.  Values computed that way are only ever written to positions that
-// no later pass reads.
+// Memory order and control flow of every sequential pass below.  Each pass
+// walks its steps in chunks of at most 8 with two input buffers used in turn:
+// the loads of the next chunk go out before a chunk is computed and stored,
+// and no register copy sits between a load and its use (a copy would wait for
+// the load at the copy).  The vector-memory counter retires loads and stores
+// in issue order, so a load issued behind a store would also wait for that
+// store; issued ahead, it waits only for stores of earlier chunks.  Loop
+// bounds are wave-uniform (all lanes of a job share K, S and g); steps that
+// are outside the block for some lanes (t < 0 in the first segment, t >= K
+// in the last) are computed by every lane and their results selected away,
+// so no per-lane branch splits a chunk.  Values computed that way are only
+// ever written to positions that no later pass reads.
+//
+// Chunks.  Alpha runs over local steps [0, g + S) in chunks whose boundaries
+// are multiples of 8 counted from the first own step (local g): chunk k
+// covers [max(0, 8k - off), min(8k + 8 - off, g + S)) with off =
+// (8 - g mod 8) mod 8.  Beta runs over [g, 2g + S) in chunks
+// [g + 8m, min(g + 8m + 8, 2g + S)).  Over the own steps the two grids
+// coincide: own chunk m is alpha chunk (g + off) / 8 + m and beta chunk m.
+// Metrics are renormalised at the chunk boundaries only (alpha when it
+// reaches local g + 8i, beta when it reaches local g + 8i), so a recompute
+// from a checkpoint reproduces the first pass's values bit for bit.
+// Checkpoints: al[k] = alpha at the start of alpha chunk k; be[m] = beta at
+// local g + 8m for 0 < m < nb and be[nb] = beta at the window top 2g + S, so
+// beta chunk m is recomputed downwards from be[m + 1].
+struct Chunks {
+  uint32_t off, na, nb, nown;
+  __device__ Chunks(uint32_t S, uint32_t g)
+      : off((8u - (g & 7u)) & 7u), na((g + S + ((8u - (g & 7u)) & 7u) + 7u) >> 3), nb((g + S + 7u) >> 3),
+        nown((S + 7u) >> 3) {}
+  __device__ uint32_t a_lo(uint32_t k) const { return 8u * k > off ? 8u * k - off : 0u; }
+  __device__ uint32_t a_hi(uint32_t k, uint32_t end) const { return min(8u * k + 8u - off, end); }
+  __device__ uint32_t own_alpha(uint32_t m, uint32_t g) const { return (g + off) / 8u + m; }
+};
+__device__ __forceinline__ bool alpha_nrm(uint32_t j, uint32_t off) { return ((j + 1u + off) & 7u) == 0; }
+__device__ __forceinline__ bool beta_nrm(uint32_t j, uint32_t g) { return ((j - g) & 7u) == 0; }
+
+// body(k, buf) for k = lo .. hi - 1 (up) or hi - 1 .. lo (down), with
+// load(k, buf) one chunk ahead, alternating two buffers.
+template <class Buf, class Load, class Body>
+__device__ __forceinline__ void loop_up(uint32_t lo, uint32_t hi, Load&& load, Body&& body) {
+  if (lo >= hi) return;
+  Buf b0, b1;
+  load(lo, b0);
+  for (uint32_t k = lo; k < hi; k += 2) {
+    if (k + 1 < hi) load(k + 1, b1);
+    body(k, b0);
+    if (k + 1 >= hi) break;
+    if (k + 2 < hi) load(k + 2, b0);
+    body(k + 1, b1);
+  }
+}
+template <class Buf, class Load, class Body>
+__device__ __forceinline__ void loop_down(uint32_t lo, uint32_t hi, Load&& load, Body&& body) {
+  if (lo >= hi) return;
+  Buf b0, b1;
+  load(hi - 1, b0);
+  for (int k = (int)hi - 1; k >= (int)lo; k -= 2) {
+    if (k - 1 >= (int)lo) load((uint32_t)k - 1, b1);
+    body((uint32_t)k, b0);
+    if (k - 1 < (int)lo) break;
+    if (k - 2 >= (int)lo) load((uint32_t)k - 2, b0);
+    body((uint32_t)k - 1, b1);
+  }
+}
 
 // Runs body(j0, buf) for j0 = lo, lo + 8, ... < hi with load(j0, buf) one
 // chunk ahead, alternating two buffers.
@@ -315,22 +345,6 @@ __device__ __forceinline__ void chunks_up(uint32_t lo, uint32_t hi, Load&& load,
     body(j0 + 8, b1);
   }
 }
-// The same from the top down: body(top, buf) covers steps top - 1 down to
-// max(top - 8, lo).
-template <class Buf, class Load, class Body>
-__device__ __forceinline__ void chunks_down(int lo, int hi, Load&& load, Body&& body) {
-  if (hi <= lo) return;
-  Buf b0, b1;
-  load(hi, b0);
-  for (int top = hi; top > lo; top -= 16) {
-    if (top - 8 > lo) load(top - 8, b1);
-    body(top, b0);
-    if (top - 8 <= lo) break;
-    if (top - 16 > lo) load(top - 16, b0);
-    body(top - 8, b1);
-  }
-}
-
 template <typename R>
 __device__ __forceinline__ void select8(bool c, const R (&x)[8], R (&v)[8]) {
 #pragma unroll
@@ -452,7 +466,9 @@ __device__ __forceinline__ BlockLds<R> block_lds(R* base, const DiffJob& J) {
   return BlockLds<R>{base, diff_lds_stride(J.k), J.nblk, diff_stage_pairs(J.k, J.warm)};
 }
 
-constexpr uint32_t kFillBatch = 16;  // loads in flight per lane and array
+// Loads in flight per thread, block and array in the LDS fills (a workgroup
+// of 128 threads covers 1024 positions per batch).
+constexpr uint32_t kFillBatch = 8;
 
 // Fill of the staging arrays: 0 = sys + la at natural position i (the lower
 // half reads it at Pi(t)), 1 = the half's parity stream.  Same arithmetic as
@@ -462,7 +478,7 @@ template <typename R>
 __device__ __forceinline__ void stage_fill(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const R w = h ? A.scales[h - 1] : R(0);
   const bool lower = (h & 1u) != 0;
-  const uint32_t K = J.k;
+  const uint32_t K = J.k, nt = blockDim.x;
   const LdsPtr<R> xl = lds_view(X);
   for (uint32_t b0 = 0; b0 < J.nblk; b0 += 2) {
     const uint32_t nb = min(2u, J.nblk - b0);
@@ -474,13 +490,13 @@ __device__ __forceinline__ void stage_fill(const DiffArgs<R> A, const DiffJob J,
       dp[x] = d0[x] + (size_t)(lower ? 2 : 1) * (K + 4);
       ep[x] = global_view(A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K);
     }
-    for (uint32_t i0 = threadIdx.x; i0 < K; i0 += 64 * kFillBatch) {
+    for (uint32_t i0 = threadIdx.x; i0 < K; i0 += nt * kFillBatch) {
       R sv[2][kFillBatch], ev[2][kFillBatch], pv[2][kFillBatch];
 #pragma unroll
       for (uint32_t x = 0; x < 2; ++x)
 #pragma unroll
         for (uint32_t u = 0; u < kFillBatch; ++u) {
-          const uint32_t i = i0 + 64 * u;
+          const uint32_t i = i0 + nt * u;
           if (x < nb && i < K) {
             sv[x][u] = d0[x][i];
             ev[x][u] = h ? ep[x][i] : R(0);
@@ -491,7 +507,7 @@ __device__ __forceinline__ void stage_fill(const DiffArgs<R> A, const DiffJob J,
       for (uint32_t x = 0; x < 2; ++x)
 #pragma unroll
         for (uint32_t u = 0; u < kFillBatch; ++u) {
-          const uint32_t i = i0 + 64 * u;
+          const uint32_t i = i0 + nt * u;
           if (x < nb && i < K) {
             xl[X.idx(0, b0 + x, i)] = lower ? sv[x][u] + w * ev[x][u] : sv[x][u] + (h ? w * ev[x][u] : R(0));
             xl[X.idx(1, b0 + x, i)] = pv[x][u];
@@ -503,14 +519,14 @@ __device__ __forceinline__ void stage_fill(const DiffArgs<R> A, const DiffJob J,
 
 // Staged pairs of the half from arrays 0 and 1 (see stage for the values):
 // pair p of block b holds (sys + la, parity) at position t = p - g of the
-// half's order, zeros outside [0, K).  Each lane takes a run of consecutive
+// half's order, zeros outside [0, K).  Each thread takes a run of consecutive
 // positions (the lower half's interleaver walked by its recurrence).
 template <typename R>
 __device__ __forceinline__ void stage_gather(const DiffJob J, uint32_t h, const BlockLds<R> X) {
   const LdsPtr<R> xl = lds_view(X);
   const bool lower = (h & 1u) != 0;
   const uint32_t K = J.k, g = J.warm, n = K + 2 * g;
-  const uint32_t run = (n + 63) / 64, p0 = threadIdx.x * run;
+  const uint32_t run = (n + blockDim.x - 1) / blockDim.x, p0 = threadIdx.x * run;
   const int t0 = (int)p0 - (int)g;
   for (uint32_t b = 0; b < J.nblk; ++b) {
     const uint32_t sb = X.stage_base(b);
@@ -552,176 +568,50 @@ __device__ void tail_inputs(uint32_t h, const R* d0, const R* d1, const R* d2, u
   }
 }
 
-// Runs body(c, buf) for c = hi - 1 down to lo with load(c, buf) one ahead,
-// alternating two buffers.
-template <class Buf, class Load, class Body>
-__device__ __forceinline__ void subwindows_down(int lo, int hi, Load&& load, Body&& body) {
-  if (hi <= lo) return;
-  Buf b0, b1;
-  load(hi - 1, b0);
-  for (int c = hi - 1; c >= lo; c -= 2) {
-    if (c - 1 >= lo) load(c - 1, b1);
-    body(c, b0);
-    if (c - 1 < lo) break;
-    if (c - 2 >= lo) load(c - 2, b0);
-    body(c - 1, b1);
-  }
-}
-
-// Forward beta pass over the own steps that also emits their extrinsics:
-// beta from local g + S (b: beta there, after the warm-up of
-// alpha_ckpt_beta_warm_pass) down to local g, in sub-windows [8c, 8c + 8) of
-// local steps; alpha of each sub-window is recomputed from its checkpoint
-// (al[c], alpha at local 8c) into registers first.  Past the block end
-// (t >= K) beta stays at its start value; beta is not stored.
-template <typename R>
-struct BetaExtIn {
-  In8<R> in;  // steps 8c + jj
-  R a0[8];    // alpha at local 8c
-};
-
-template <typename R>
-__device__ __forceinline__ void beta_ext_pass(const Lane<R>& L0, const Layout& Y, R (&b)[8], uint32_t h, R* e) {
-  const Lane<R> L = fresh(L0);
-  const int own_end = (int)(L.g + L.S);  // own steps: g <= j < g + S
-  const int g = (int)L.g;
-  const uint32_t st = Y.g + Y.S + 1;
-  // Pi of the own steps, from the last one down
-  QppWalk qw((uint32_t)L.t_of((uint32_t)own_end - 1), L.K, L.f1, L.f2);
-  auto load = [&](int c, BetaExtIn<R>& x) {
-    const int j0 = 8 * c;
-    load_in8(L, Y, j0, min(8, own_end - j0), x.in);
-    load8(L, Y.al, st, (uint32_t)c, x.a0);
-  };
-  auto body = [&](int c, const BetaExtIn<R>& x) {
-    const int j0 = 8 * c;
-    R areg[8][8];
-    {
-      R a[8];
-#pragma unroll
-      for (int s = 0; s < 8; ++s) a[s] = x.a0[s];
-#pragma unroll
-      for (int jj = 0; jj < 8; ++jj) {
-#pragma unroll
-        for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
-        if (jj + 1 < 8 && j0 + jj + 1 < own_end) {
-          R o[8];
-          alpha_step(a, kHalfL2e<R> * x.in.sa[jj], kHalfL2e<R> * x.in.pr[jj], o, false);
-          select8(L.t_of((uint32_t)(j0 + jj)) >= 0, o, a);
-        }
-      }
-    }
-#pragma unroll
-    for (int jj = 7; jj >= 0; --jj) {
-      const int j = j0 + jj;
-      if (j >= g && j < own_end) {
-        const R sa = x.in.sa[jj], pr = x.in.pr[jj];
-        const int t = L.t_of((uint32_t)j);
-        {
-          R Lu[2];
-          llr_terms(areg[jj], b, kHalfL2e<R> * sa, kHalfL2e<R> * pr, Lu, (R(*)[8]) nullptr);
-          e[(h & 1u) ? qw.pi : (uint32_t)t] = (Lu[1] - Lu[0]) * kLn2<R> - sa;
-          if (j > g) qw.prev();
-        }
-        R o[8];
-        beta_step(b, kHalfL2e<R> * sa, kHalfL2e<R> * pr, o, ((j - g) & 7) == 0);
-        select8(t < (int)L.K, o, b);
-      }
-    }
-  };
-  subwindows_down<BetaExtIn<R>>(g / 8, (own_end + 7) / 8, load, body);
-}
-
 template <typename R>
 __device__ __forceinline__ void load8(const Lane<R>& L, uint32_t field, uint32_t stride, uint32_t j, R (&v)[8]) {
 #pragma unroll
   for (int s = 0; s < 8; ++s) v[s] = L.at(field + s * stride, j);
 }
 
-// Alpha over local steps 0 .. g + S with only every 8th stored (al[c] =
-// alpha at local 8c; alpha stays at its start value before step 0 of the
-// block, t < 0) and, in the same loop, beta over the warm-up above the own
-// steps: local 2g + S - 1 down to g + S, from bcur (beta at local 2g + S) to
-// beta at local g + S, with beta_ck(i, beta) called before each beta step
-// (i = j + 1 - g for step j) to keep checkpoints.  The two recursions are
-// independent; a job is one wave and rarely shares its SIMD, so each chunk
-// interleaves them to give the SIMD two dependency chains to issue from.
-// Every value is the one the two recursions compute run one after the other.
-template <typename R>
-struct AlphaBetaIn {
-  In8<R> up;  // alpha inputs, local j0 + jj
-  In8<R> dn;  // beta inputs, local 2g + S - 1 - (j0 + jj)
-};
+// Input bit of the transitions into state n (n = (fb, s2, s1) with
+// fb = u ^ s1 ^ s2): the parity of n's bits.
+__host__ __device__ constexpr unsigned in_of(unsigned n) { return (n ^ (n >> 1) ^ (n >> 2)) & 1u; }
 
-template <typename R, class Ck>
-__device__ __forceinline__ void alpha_ckpt_beta_warm_pass(const Lane<R>& L0, const Layout& Y, R (&bcur)[8],
-                                                          Ck&& beta_ck) {
-  const Lane<R> L = fresh(L0);
-  R a[8];
-  const bool from_zero_state = (int)(L.q * L.S) - (int)L.g <= 0;
+// Per input-bit class of the eight states at t + 1: m[u] = max T(n),
+// s[u] = sum 2^(T(n) - m[u]), p[n] = 2^(T(n) - m[u(n)]).  L_u = m[u] +
+// log2 s[u] is the base-2 LLR term of bit u: every transition into a state of
+// class u carries input u, so summing alpha_t + gamma + beta_{t+1} over the
+// 16 transitions equals summing alpha_{t+1} + beta_{t+1} over the 8 states.
+template <typename R>
+__device__ __forceinline__ void class_sums(const R (&T)[8], R (&m)[2], R (&s)[2], R (&p)[8]) {
+  m[0] = m[1] = R(kNegF);
 #pragma unroll
-  for (int s = 0; s < 8; ++s) a[s] = from_zero_state ? (s == 0 ? R(0) : R(kNegF)) : R(0);
-  const uint32_t n = L.g + L.S;
-  const int top = (int)(2 * L.g + L.S);
-  auto load = [&](uint32_t j0, AlphaBetaIn<R>& c) {
-    load_in8(L, Y, (int)j0, (int)min(8u, n - j0), c.up);
-    if (j0 < L.g) load_in8<R, true>(L, Y, top - 1 - (int)j0, min(8, (int)L.g - (int)j0), c.dn);
-  };
-  auto body = [&](uint32_t j0, const AlphaBetaIn<R>& c) {
+  for (unsigned n = 0; n < 8; ++n) m[in_of(n)] = fmax(m[in_of(n)], T[n]);
+  s[0] = s[1] = R(0);
 #pragma unroll
-    for (int s = 0; s < 8; ++s) L.at(Y.al + s * (Y.g + Y.S + 1), j0 / 8) = a[s];
-#pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
-      const uint32_t j = j0 + jj;
-      if (j < n) {
-        R o[8];
-        alpha_step(a, kHalfL2e<R> * c.up.sa[jj], kHalfL2e<R> * c.up.pr[jj], o, ((j + 1) & 7u) == 0);
-        select8(L.t_of(j) >= 0, o, a);
-      }
-      if (j < L.g) {
-        const int jb = top - 1 - (int)j;
-        beta_ck((uint32_t)jb + 1 - L.g, bcur);
-        R o[8];
-        beta_step(bcur, kHalfL2e<R> * c.dn.sa[jj], kHalfL2e<R> * c.dn.pr[jj], o, (((uint32_t)jb - L.g) & 7u) == 0);
-        select8(L.t_of((uint32_t)jb) < (int)L.K, o, bcur);
-      }
-    }
-  };
-  chunks_up<AlphaBetaIn<R>>(0, n, load, body);
+  for (unsigned n = 0; n < 8; ++n) {
+    p[n] = Num<R>::ex2(T[n] - m[in_of(n)]);
+    s[in_of(n)] += p[n];
+  }
 }
 
-// One step of the alpha adjoint at local j (alpha_j = a, beta_{j+1} = b,
-// half gammas hs, hp): the output adjoint of the extrinsic e when the step is
-// the lane's own (then the beta output adjoints of the step go to bo[j - g]),
-// then the adjoint of alpha_{j+1} pulled back through its two incoming
-// transitions; ab holds the adjoint of alpha_{j+1} on entry and of alpha_j on
-// exit; ds, dp: the adjoints of sys + la and par of the step.
+// The weight of a in jac(a, b): 2^a / (2^a + 2^b) (base-2 metrics).
 template <typename R>
-__device__ inline void alpha_adjoint_step(const R (&a)[8], const R (&b)[8], R hs, R hp, R e, bool own, R (&ab)[8],
-                                          R& ds_out, R& dp_out, const Lane<R>& L, const Layout& Y, uint32_t j) {
-  R dg[8][2];
-#pragma unroll
-  for (int s = 0; s < 8; ++s) dg[s][0] = dg[s][1] = 0;
+__device__ inline R jac_w0(R a, R b) {
+  const R e = Num<R>::ex2(-fabs(a - b));
+  const R big = Num<R>::rcp(R(1) + e);
+  return a >= b ? big : e * big;
+}
+
+// One step of the alpha adjoint at local j (alpha_j = a, half gammas hs, hp
+// in base 2): ab holds the adjoint of alpha_{j+1} on entry (the LLR's share
+// included) and of alpha_j on exit; ds, dp: the step's adjoints of sys + la
+// and par through its transitions (gamma = +-(sys + la)/2 +- par/2).
+template <typename R>
+__device__ inline void alpha_adjoint_step(const R (&a)[8], R hs, R hp, R (&ab)[8], R& ds_out, R& dp_out) {
   R abn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  R dsa_direct = 0;
-  if (own) {
-    R Lu[2], P[2][8];
-    llr_terms(a, b, hs, hp, Lu, P);
-    R bo[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (unsigned s = 0; s < 8; ++s)
-#pragma unroll
-      for (unsigned u = 0; u < 2; ++u) {
-        const R dm = u ? e * P[1][s] : -e * P[0][s];
-        dg[s][u] += dm;
-        abn[s] += dm;
-        bo[nxt(s, u)] += dm;
-      }
-#pragma unroll
-    for (int s = 0; s < 8; ++s) L.at(Y.bo + s * Y.S, j - L.g) = bo[s];
-    dsa_direct = -e;
-  }
-  // alpha_{t+1}(n) = jac over its two incoming transitions
+  R ds = 0, dp = 0;
   bool seen[8] = {false, false, false, false, false, false, false, false};
   unsigned fs[8], fu[8];
 #pragma unroll
@@ -735,56 +625,345 @@ __device__ inline void alpha_adjoint_step(const R (&a)[8], const R (&b)[8], R hs
         seen[n] = true;
       } else {
         const unsigned s0 = fs[n], u0 = fu[n];
-        R w0, w1;
-        jac_w(a[s0] + gam(u0, par(s0, u0), hs, hp), a[s] + gam(u, par(s, u), hs, hp), w0, w1);
-        dg[s0][u0] += ab[n] * w0;
-        abn[s0] += ab[n] * w0;
-        dg[s][u] += ab[n] * w1;
-        abn[s] += ab[n] * w1;
+        const R w0 = jac_w0(a[s0] + gam(u0, par(s0, u0), hs, hp), a[s] + gam(u, par(s, u), hs, hp));
+        const R d0 = ab[n] * w0, d1 = ab[n] - d0;  // the two weights add to 1
+        abn[s0] += d0;
+        abn[s] += d1;
+        ds += u0 ? d0 : -d0;
+        ds += u ? d1 : -d1;
+        dp += par(s0, u0) ? d0 : -d0;
+        dp += par(s, u) ? d1 : -d1;
       }
     }
 #pragma unroll
   for (int s = 0; s < 8; ++s) ab[s] = abn[s];
-  R ds = dsa_direct, dp = 0;
-#pragma unroll
-  for (unsigned s = 0; s < 8; ++s)
-#pragma unroll
-    for (unsigned u = 0; u < 2; ++u) {
-      ds += u ? R(0.5) * dg[s][u] : R(-0.5) * dg[s][u];
-      dp += par(s, u) ? R(0.5) * dg[s][u] : R(-0.5) * dg[s][u];
-    }
-  ds_out = ds;
-  dp_out = dp;
+  ds_out = R(0.5) * ds;
+  dp_out = R(0.5) * dp;
 }
 
 // One step of the beta adjoint at local j (beta_{j+1} = b): bb holds the
-// adjoint of beta_j on entry (bn: the step's output adjoints of beta_{j+1})
-// and of beta_{j+1} on exit.
+// adjoint of beta_j on entry (bn: the output adjoints of beta_{j+1}, the
+// LLR's share) and of beta_{j+1} on exit.
 template <typename R>
 __device__ inline void beta_adjoint_step(const R (&b)[8], R hs, R hp, R (&bb)[8], R (&bn)[8], R& ds_out,
                                          R& dp_out) {
   R ds = 0, dp = 0;
 #pragma unroll
   for (unsigned s = 0; s < 8; ++s) {
-    R w0, w1;
-    jac_w(b[nxt(s, 0)] + gam(0u, par(s, 0), hs, hp), b[nxt(s, 1)] + gam(1u, par(s, 1), hs, hp), w0, w1);
-    const R d0g = bb[s] * w0, d1g = bb[s] * w1;
+    const R w0 = jac_w0(b[nxt(s, 0)] + gam(0u, par(s, 0), hs, hp), b[nxt(s, 1)] + gam(1u, par(s, 1), hs, hp));
+    const R d0g = bb[s] * w0, d1g = bb[s] - d0g;
     bn[nxt(s, 0)] += d0g;
     bn[nxt(s, 1)] += d1g;
-    ds += R(0.5) * (d1g - d0g);
-    dp += par(s, 0) ? R(0.5) * d0g : R(-0.5) * d0g;
-    dp += par(s, 1) ? R(0.5) * d1g : R(-0.5) * d1g;
+    ds += d1g - d0g;
+    dp += par(s, 0) ? d0g : -d0g;
+    dp += par(s, 1) ? d1g : -d1g;
   }
 #pragma unroll
   for (int s = 0; s < 8; ++s) bb[s] = bn[s];
-  ds_out = ds;
-  dp_out = dp;
+  ds_out = R(0.5) * ds;
+  dp_out = R(0.5) * dp;
+}
+
+// Phase 1, alpha side: alpha over local steps [0, g + S) from its start
+// (state 0 when the window reaches the block start, else zero metrics; alpha
+// stays at its start value before step 0 of the block, t < 0), al[k] kept at
+// the start of each chunk.
+template <typename R>
+__device__ __forceinline__ void alpha_ckpt_pass(const Lane<R>& L0, const Layout& Y, const Chunks& C) {
+  const Lane<R> L = fresh(L0);
+  R a[8];
+  const bool from_zero_state = (int)(L.q * L.S) - (int)L.g <= 0;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) a[s] = from_zero_state ? (s == 0 ? R(0) : R(kNegF)) : R(0);
+  const uint32_t end = L.g + L.S, ast = Y.g + Y.S + 1;
+  auto load = [&](uint32_t k, In8<R>& c) {
+    const uint32_t lo = C.a_lo(k);
+    load_in8(L, Y, (int)lo, (int)(C.a_hi(k, end) - lo), c);
+  };
+  auto body = [&](uint32_t k, const In8<R>& c) {
+    const uint32_t lo = C.a_lo(k), n = C.a_hi(k, end) - lo;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) L.at(Y.al + s * ast, k) = a[s];
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj)
+      if (jj < n) {
+        const uint32_t j = lo + jj;
+        R o[8];
+        alpha_step(a, kHalfL2e<R> * c.sa[jj], kHalfL2e<R> * c.pr[jj], o, alpha_nrm(j, C.off));
+        select8(L.t_of(j) >= 0, o, a);
+      }
+  };
+  loop_up<In8<R>>(0, C.na, load, body);
+}
+
+// Phase 1, beta side: beta from the window top (b: beta at local 2g + S)
+// down to local g; beta stays at its start value past the block end
+// (t >= K).  be[m + 1] is kept at the top of each chunk m.
+template <typename R>
+__device__ __forceinline__ void beta_ckpt_pass(const Lane<R>& L0, const Layout& Y, const Chunks& C, R (&b)[8]) {
+  const Lane<R> L = fresh(L0);
+  const uint32_t top = 2 * L.g + L.S, bst = Y.S + Y.g + 1;
+  auto load = [&](uint32_t m, In8<R>& c) {
+    const uint32_t lo = L.g + 8u * m, hi = min(lo + 8u, top);
+    load_in8(L, Y, (int)lo, (int)(hi - lo), c);
+  };
+  auto body = [&](uint32_t m, const In8<R>& c) {
+    const uint32_t lo = L.g + 8u * m, n = min(lo + 8u, top) - lo;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) L.at(Y.be + s * bst, m + 1) = b[s];
+#pragma unroll
+    for (int jj = 7; jj >= 0; --jj)
+      if ((uint32_t)jj < n) {
+        const uint32_t j = lo + (uint32_t)jj;
+        R o[8];
+        beta_step(b, kHalfL2e<R> * c.sa[jj], kHalfL2e<R> * c.pr[jj], o, beta_nrm(j, L.g));
+        select8(L.t_of(j) < (int)L.K, o, b);
+      }
+  };
+  loop_down<In8<R>>(0, C.nb, load, body);
+}
+
+// Phase 2: for own chunks m in [m0, m1), alpha_{j+1} and beta_{j+1} of each
+// own step j recomputed from the checkpoints (alpha up from al, beta down
+// from be[m + 1]), then term(j, T, sys + la of j) with T(n) =
+// alpha_{j+1}(n) + beta_{j+1}(n), in ascending order of j.
+template <typename R>
+struct OwnIn {
+  R a0[8], b0[8];
+};
+
+template <typename R, class Term>
+__device__ __forceinline__ void own_terms(const Lane<R>& L0, const Layout& Y, const Chunks& C, uint32_t m0, uint32_t m1,
+                                          Term&& term) {
+  const Lane<R> L = fresh(L0);
+  const uint32_t ast = Y.g + Y.S + 1, bst = Y.S + Y.g + 1;
+  const uint32_t own_end = L.g + L.S, top = 2 * L.g + L.S;
+  auto load = [&](uint32_t m, OwnIn<R>& x) {
+    const uint32_t lo = L.g + 8u * m;
+    (void)lo;
+    load8(L, Y.al, ast, C.own_alpha(m, L.g), x.a0);
+    load8(L, Y.be, bst, m + 1, x.b0);
+  };
+  auto body = [&](uint32_t m, const OwnIn<R>& xc) {
+    const uint32_t lo = L.g + 8u * m, n = min(8u, own_end - lo), nb = min(lo + 8u, top) - lo;
+    // staged inputs of steps lo .. lo + nb - 1 (LDS, or the lane's scratch)
+    In8<R> in;
+    load_in8(L, Y, (int)lo, (int)nb, in);
+
+    // beta of the chunk into registers first (downwards), then alpha stepped
+    // up with each step's terms formed as soon as alpha_{j+1} is known
+    R breg[8][8];
+    {
+      R b[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) b[s] = xc.b0[s];
+#pragma unroll
+      for (int jj = 7; jj >= 0; --jj)
+        if ((uint32_t)jj < nb) {
+          if ((uint32_t)jj < n) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) breg[jj][s] = b[s];  // beta at local lo + jj + 1
+          }
+          if (jj > 0) {
+            const uint32_t j = lo + (uint32_t)jj;
+            R o[8];
+            beta_step(b, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, beta_nrm(j, L.g));
+            select8(L.t_of(j) < (int)L.K, o, b);
+          }
+        }
+    }
+    R a[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) a[s] = xc.a0[s];
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj)
+      if (jj < n) {
+        R o[8];
+        alpha_step(a, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, alpha_nrm(lo + jj, C.off));
+        R T[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          a[s] = o[s];  // own steps lie inside the block
+          T[s] = o[s] + breg[jj][s];
+        }
+        term(lo + jj, T, in.sa[jj]);
+      }
+  };
+  loop_up<OwnIn<R>>(m0, m1, load, body);
+}
+
+// Phase 3, alpha side: the alpha adjoint backwards over local steps
+// g + S - 1 .. 0, chunk by chunk (alpha recomputed from al into registers),
+// seeded at each own step with G (the LLR's share of the adjoint of
+// alpha_{j+1}); writes dsa / dpr of local steps [0, g + S), with the direct
+// term -e of the extrinsic (ext = ... - (sys + la)) at the own steps.  e:
+// the adjoints of the own steps' extrinsics, read from ecur (natural order:
+// position t, or Pi(t) in a lower half) a chunk ahead.
+template <typename R>
+struct AlphaIn {
+  R a0[8], e[8];
+};
+
+template <typename R>
+__device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Layout& Y, const Chunks& C, bool lower,
+                                                    const R* ecur) {
+  const Lane<R> L = fresh(L0);
+  R ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t end = L.g + L.S, ast = Y.g + Y.S + 1;
+  const GlobalPtr<const R> eg = global_view(ecur);
+  // own steps' positions, walked down from the last one (loads go in
+  // descending order of steps)
+  QppWalk qw((uint32_t)L.t_of(end - 1), L.K, L.f1, L.f2);
+  auto load = [&](uint32_t k, AlphaIn<R>& x) {
+    const uint32_t lo = C.a_lo(k), n = C.a_hi(k, end) - lo;
+    load8(L, Y.al, ast, k, x.a0);
+#pragma unroll
+    for (int jj = 7; jj >= 0; --jj) {
+      if ((uint32_t)jj < n) {
+        const uint32_t j = lo + (uint32_t)jj;
+        if (j >= L.g) {
+          x.e[jj] = eg[lower ? qw.pi : (uint32_t)L.t_of(j)];
+          if (j > L.g) qw.prev();
+        } else {
+          x.e[jj] = R(0);
+        }
+      }
+    }
+  };
+  auto body = [&](uint32_t k, const AlphaIn<R>& xc) {
+    const uint32_t lo = C.a_lo(k), n = C.a_hi(k, end) - lo;
+
+    In8<R> in;
+    load_in8(L, Y, (int)lo, (int)n, in);
+    // G of the chunk's own steps, loaded ahead of the recompute
+    R gv[8][8];
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj)
+      if (jj < n) {
+        const uint32_t j = lo + jj;
+        if (j >= L.g) {
+          load8(L, Y.gl, Y.S, j - L.g, gv[jj]);
+        } else {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) gv[jj][s] = R(0);
+        }
+      }
+    R areg[8][8];
+    {
+      R a[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) a[s] = xc.a0[s];
+#pragma unroll
+      for (uint32_t jj = 0; jj < 8; ++jj) {
+        if (jj < n) {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
+          if (jj + 1 < n) {
+            R o[8];
+            alpha_step(a, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, alpha_nrm(lo + jj, C.off));
+            select8(L.t_of(lo + jj) >= 0, o, a);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int jj = 7; jj >= 0; --jj) {
+      if ((uint32_t)jj < n) {
+        const uint32_t j = lo + (uint32_t)jj;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) ab[s] += gv[jj][s];
+        R ds, dp;
+        alpha_adjoint_step(areg[jj], kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], ab, ds, dp);
+        L.at(Y.dsa, j) = ds - xc.e[jj];
+        L.at(Y.dpr, j) = dp;
+      }
+    }
+  };
+  loop_down<AlphaIn<R>>(0, C.na, load, body);
+}
+
+// Phase 3, beta side: the beta adjoint forwards over local steps
+// g .. 2g + S - 1, chunk by chunk (beta recomputed downwards from be into
+// registers), with G as the output adjoints of beta_{j+1} at the own steps;
+// writes dsb / dpb of local steps [g, 2g + S).  Returns in bb the adjoint of
+// beta at the window top (at the block end, that of the tail beta).
+template <typename R>
+struct BetaRe {
+  R b0[8];
+};
+
+template <typename R>
+__device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layout& Y, const Chunks& C, R (&bb)[8]) {
+  const Lane<R> L = fresh(L0);
+  const uint32_t top = 2 * L.g + L.S, own_end = L.g + L.S, bst = Y.S + Y.g + 1;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) bb[s] = 0;
+  auto load = [&](uint32_t m, BetaRe<R>& x) {
+    const uint32_t lo = L.g + 8u * m;
+    (void)lo;
+    load8(L, Y.be, bst, m + 1, x.b0);
+  };
+  auto body = [&](uint32_t m, const BetaRe<R>& xc) {
+    const uint32_t lo = L.g + 8u * m, n = min(lo + 8u, top) - lo;
+
+    In8<R> in;
+    load_in8(L, Y, (int)lo, (int)n, in);
+    R gv[8][8];
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj)
+      if (jj < n) {
+        const uint32_t j = lo + jj;
+        if (j < own_end) {
+          load8(L, Y.gl, Y.S, j - L.g, gv[jj]);
+        } else {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) gv[jj][s] = R(0);
+        }
+      }
+    R breg[8][8];
+    {
+      R b[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) b[s] = xc.b0[s];
+#pragma unroll
+      for (int jj = 7; jj >= 0; --jj) {
+        if ((uint32_t)jj < n) {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) breg[jj][s] = b[s];  // beta at local lo + jj + 1
+          if (jj > 0) {
+            const uint32_t j = lo + (uint32_t)jj;
+            R o[8];
+            beta_step(b, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, beta_nrm(j, L.g));
+            select8(L.t_of(j) < (int)L.K, o, b);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      if (jj < n) {
+        const uint32_t j = lo + jj;
+        R nb[8], bn[8], ds, dp;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          nb[s] = bb[s];
+          bn[s] = gv[jj][s];
+        }
+        beta_adjoint_step(breg[jj], kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], nb, bn, ds, dp);
+        // past the block end the adjoint stops (beta is not stepped there)
+        select8(L.t_of(j) < (int)L.K, nb, bb);
+        L.at(Y.dsb, j) = ds;
+        L.at(Y.dpb, j) = dp;
+      }
+    }
+  };
+  loop_up<BetaRe<R>>(0, C.nb, load, body);
 }
 
 template <typename R>
 __device__ Lane<R> make_lane(const DiffJob& J, const DiffArgs<R>& A) {
   Lane<R> L;
-  L.lane = threadIdx.x;
+  L.lane = threadIdx.x & 63u;
   L.K = J.k;
   L.S = J.seg;
   L.M = J.nseg;
@@ -807,60 +986,127 @@ __device__ __forceinline__ Lane<R> with_lds(Lane<R> L, const BlockLds<R>& X, uin
   return L;
 }
 
+// The own chunks a wave takes in phases 2 and 4: wave 0 the first half
+// (rounded up), wave 1 the rest.
+__device__ __forceinline__ void own_range(const Chunks& C, uint32_t wave, uint32_t& m0, uint32_t& m1) {
+  const uint32_t h = (C.nown + 1) / 2;
+  m0 = wave ? h : 0u;
+  m1 = wave ? C.nown : h;
+}
+
+// A lane's code block: its descriptor and the pointers derived from it.
+template <typename R>
+struct BlockPtrs {
+  DiffCb c;
+  const R *d0, *d1, *d2, *sv;
+  R* sv_mut;
+  __device__ BlockPtrs(const DiffArgs<R>& A, const DiffJob& J, uint32_t blk) : c(A.cbs[J.cb_first + blk]) {
+    const uint32_t D = J.k + 4;
+    d0 = A.llr + c.llr_off;
+    d1 = d0 + D;
+    d2 = d1 + D;
+    sv_mut = A.saved + c.save_off;
+    sv = sv_mut;
+  }
+  __device__ R* g(const DiffArgs<R>& A, uint32_t stream, uint32_t k) const { return A.grad_llr + c.llr_off + (size_t)stream * (k + 4); }
+  __device__ R* ebar(const DiffArgs<R>& A, uint32_t which, uint32_t k) const { return A.ebar + c.ebar_off + (size_t)which * k; }
+};
+
+// Phase 1 of a half on both waves: staging is done; wave 0 runs alpha, wave 1
+// beta from its start (the tail beta when the window reaches the block end).
+template <typename R>
+__device__ __forceinline__ void checkpoint_passes(const Lane<R>& L, const Layout& Y, const Chunks& C, uint32_t wave,
+                                                  uint32_t h, const R* d0, const R* d1, const R* d2) {
+  if (!L.active) return;
+  if (wave == 0) {
+    alpha_ckpt_pass(L, Y, C);
+  } else {
+    R b[8];
+    if (L.q * L.S + L.S + L.g >= L.K) {
+      R ts[3], tp[3];
+      tail_inputs(h, d0, d1, d2, L.K, ts, tp);
+      tail_beta(ts, tp, b);
+    } else {
+#pragma unroll
+      for (int s = 0; s < 8; ++s) b[s] = 0;
+    }
+    beta_ckpt_pass(L, Y, C, b);
+  }
+}
+
 }  // namespace
 
-
+// Waves per SIMD the kernels are built for: two (one job's two waves, or two
+// jobs) in float, one in double (whose registers would not fit two).
 template <typename R>
-__global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs<R> A) {
+constexpr int kDiffWavesPerSimd = sizeof(R) == 4 ? 2 : 1;
+
+// kLds: the plan's jobs fit their block copies in LDS (A.use_lds); the
+// global-memory path is a kernel of its own, so its code does not take
+// registers from the LDS path.
+template <typename R, bool kLds>
+__global__ __launch_bounds__(128, kDiffWavesPerSimd<R>) void diff_forward_kernel(DiffArgs<R> A) {
   extern __shared__ __attribute__((aligned(16))) char diff_lds[];
   const Layout Y(A.max_seg, A.max_warm);
+  // the wave's index in the workgroup, as a scalar (uniform in a wave;
+  // read through readfirstlane, everything derived from it stays in scalar
+  // registers and scalar branches)
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (uint32_t jb = blockIdx.x; jb < A.njobs; jb += gridDim.x) {
     const DiffJob J = A.jobs[jb];
     const Lane<R> L0 = make_lane(J, A);
     const uint32_t blk = L0.active ? L0.lane / J.nseg : 0;
     const BlockLds<R> X = block_lds(reinterpret_cast<R*>(diff_lds), J);
-    const Lane<R> L = A.use_lds ? with_lds(L0, X, blk) : L0;
-    const DiffCb C = A.cbs[J.cb_first + blk];
-    const R* d0 = A.llr + C.llr_off;
-    const R* d1 = d0 + (L.K + 4);
-    const R* d2 = d1 + (L.K + 4);
-    R* sv = A.saved + C.save_off;
+    const Lane<R> L = kLds ? with_lds(L0, X, blk) : L0;
+    // the lane's block, fetched where a phase needs it (its pointers then
+    // hold no registers through the passes)
+    auto cb = [&]() { return BlockPtrs<R>(A, J, blk); };
+    const Chunks CH(L.S, L.g);
+    uint32_t m0, m1;
+    own_range(CH, wave, m0, m1);
     for (uint32_t h = 0; h < A.halves; ++h) {
-      if (A.use_lds) {
+      if constexpr (kLds) {
         stage_fill(A, J, h, X);
         __syncthreads();
         stage_gather(J, h, X);
-        __syncthreads();
+      } else if (wave == 0 && L.active) {
+        const BlockPtrs<R> B = cb();
+        stage(L, Y, h, B.d0, B.d1, B.d2, B.sv, A.scales);
       }
+      __syncthreads();
       if (L.active) {
-        R ts[3], tp[3];
-        tail_inputs(h, d0, d1, d2, L.K, ts, tp);
-        if (!A.use_lds) stage(L, Y, h, d0, d1, d2, sv, A.scales);
-        R b[8];
-        if (L.q * L.S + L.S + L.g >= L.K) {
-          tail_beta(ts, tp, b);
-        } else {
-#pragma unroll
-          for (int s = 0; s < 8; ++s) b[s] = 0;
-        }
-        alpha_ckpt_beta_warm_pass(L, Y, b, [](uint32_t, const R (&)[8]) {});
-        beta_ext_pass(L, Y, b, h, sv + (size_t)h * L.K);
+        const BlockPtrs<R> B = cb();
+        checkpoint_passes(L, Y, CH, wave, h, B.d0, B.d1, B.d2);
+      }
+      __syncthreads();
+      if (L.active) {
+        const bool lower = (h & 1u) != 0;
+        R* e = cb().sv_mut + (size_t)h * L.K;
+        QppWalk qw(L.q * L.S + 8u * m0, L.K, L.f1, L.f2);
+        own_terms(L, Y, CH, m0, m1, [&](uint32_t j, const R (&T)[8], R sa) {
+          R m[2], s[2], p[8];
+          class_sums(T, m, s, p);
+          const R l1 = m[1] + Num<R>::lg2(s[1]), l0 = m[0] + Num<R>::lg2(s[0]);
+          e[lower ? qw.pi : (uint32_t)L.t_of(j)] = (l1 - l0) * kLn2<R> - sa;
+          qw.next();
+        });
       }
       __syncthreads();
     }
     if (L.active) {
+      const BlockPtrs<R> B = cb();
       const uint32_t H = A.halves;
-      const R* e1 = sv + (size_t)(H - 2) * L.K;
-      const R* e2 = sv + (size_t)(H - 1) * L.K;
-      R* post = A.post + C.post_off;
+      const R* e1 = B.sv + (size_t)(H - 2) * L.K;
+      const R* e2 = B.sv + (size_t)(H - 1) * L.K;
+      R* post = A.post + B.c.post_off;
       const R w1 = A.scales[H - 2], w2 = A.scales[H - 1];
-      const uint32_t hi = L.q * L.S + L.S;
-      for (uint32_t t0 = L.q * L.S; t0 < hi; t0 += 8) {
+      const uint32_t lo = L.q * L.S + 8u * m0, hi = min(L.q * L.S + 8u * m1, L.q * L.S + L.S);
+      for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
         R x[8], y1[8], y2[8];
 #pragma unroll
         for (uint32_t jj = 0; jj < 8; ++jj)
           if (t0 + jj < hi) {
-            x[jj] = d0[t0 + jj];
+            x[jj] = B.d0[t0 + jj];
             y1[jj] = e1[t0 + jj];
             y2[jj] = e2[t0 + jj];
           }
@@ -875,13 +1121,15 @@ __global__ __launch_bounds__(64) void diff_forward_kernel(DiffArgs<R> A) {
 
 namespace {
 
-// Posterior adjoint of one block segment: grad d0 = grad post, the adjoint
-// of the last half's extrinsic, and this lane's terms of the last two scales'
-// adjoints (post = d0 + w[H-2] e_{H-2} + w[H-1] e_{H-1}).
+// Posterior adjoint of the positions [lo, hi) of a block segment: grad d0 =
+// grad post, the adjoint of the last half's extrinsic, and this lane's terms
+// of the last two scales' adjoints (post = d0 + w[H-2] e_{H-2} +
+// w[H-1] e_{H-1}); the tail positions are zeroed by the last segment's lane
+// when tail is set.
 template <typename R>
 __device__ __forceinline__ void post_adjoint(const DiffArgs<R>& A, const Lane<R>& L, const R* sv, const R* gp, R* g0,
-                                             R* g1, R* g2, R* el, R& s1, R& s2) {
-  const uint32_t H = A.halves, K = L.K, lo = L.q * L.S, hi = lo + L.S;
+                                             R* g1, R* g2, R* el, uint32_t lo, uint32_t hi, bool tail, R& s1, R& s2) {
+  const uint32_t H = A.halves, K = L.K;
   const R* e1 = sv + (size_t)(H - 2) * K;
   const R* e2 = sv + (size_t)(H - 1) * K;
   const R wl = A.scales[H - 1];
@@ -906,221 +1154,56 @@ __device__ __forceinline__ void post_adjoint(const DiffArgs<R>& A, const Lane<R>
         s2 += gv[jj] * x2[jj];
       }
   }
-  if (L.q == L.M - 1)
+  if (tail && L.q == L.M - 1)
     for (uint32_t t = K; t < K + 4; ++t) g0[t] = g1[t] = g2[t] = 0;
 }
 
-// Alpha adjoint, backwards over local steps g + S - 1 .. 0 in sub-windows of
-// 8 (alpha recomputed from the checkpoint), the output adjoints of the own
-// steps on the way, and beta of the own steps (bcur: beta at local g + S on
-// entry), whose checkpoints beta_ck keeps for the beta-adjoint sweep.
-// Writes dsa / dpr of local steps below g + S (their first writer).  The
-// adjoints of the own steps' extrinsics are read from ecur (natural order:
-// position t, or Pi(t) in a lower half), a sub-window ahead like the other
-// inputs.
-template <typename R>
-struct AlphaIn {
-  R a0[8], e[8];
-  In8<R> in;  // staged inputs (halved where used)
-};
-
-template <typename R, class Ck>
-__device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Layout& Y, R (&bcur)[8], Ck&& beta_ck,
-                                                    bool lower, const R* ecur) {
-  const Lane<R> L = fresh(L0);
-  R ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const uint32_t n_al = L.g + L.S;
-  const GlobalPtr<const R> eg = global_view(ecur);
-  // own steps' positions, walked down from the last one (loads go in
-  // descending order of steps)
-  QppWalk qw((uint32_t)L.t_of(n_al - 1), L.K, L.f1, L.f2);
-  auto load = [&](int c, AlphaIn<R>& x) {
-    const uint32_t j0 = 8u * (uint32_t)c, len = min(8u, n_al - j0);
-    load8(L, Y.al, Y.g + Y.S + 1, (uint32_t)c, x.a0);
-    load_in8(L, Y, (int)j0, (int)len, x.in);
-#pragma unroll
-    for (int jj = 7; jj >= 0; --jj) {
-      if ((uint32_t)jj < len) {
-        const uint32_t j = j0 + (uint32_t)jj;
-        if (j >= L.g) {
-          x.e[jj] = eg[lower ? qw.pi : (uint32_t)L.t_of(j)];
-          if (j > L.g) qw.prev();
-        } else {
-          x.e[jj] = R(0);
-        }
-      }
-    }
-  };
-  auto body = [&](int c, const AlphaIn<R>& x) {
-    const uint32_t j0 = 8u * (uint32_t)c, len = min(8u, n_al - j0);
-    R areg[8][8];
-    {
-      R a[8];
-#pragma unroll
-      for (int s = 0; s < 8; ++s) a[s] = x.a0[s];
-#pragma unroll
-      for (uint32_t jj = 0; jj < 8; ++jj) {
-        if (jj < len) {
-#pragma unroll
-          for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
-          if (jj + 1 < len) {
-            R o[8];
-            alpha_step(a, kHalfL2e<R> * x.in.sa[jj], kHalfL2e<R> * x.in.pr[jj], o, false);
-            select8(L.t_of(j0 + jj) >= 0, o, a);
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int jj = 7; jj >= 0; --jj) {
-      if ((uint32_t)jj < len) {
-        const uint32_t j = j0 + (uint32_t)jj;
-        R b[8];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) b[s] = bcur[s];  // beta_{j+1} while j >= g
-        const R hs = kHalfL2e<R> * x.in.sa[jj], hp = kHalfL2e<R> * x.in.pr[jj];
-        if (j >= L.g) {
-          // own step (inside the block for every lane): checkpoint
-          // beta_{j+1}, step beta to beta_j
-          beta_ck(j + 1 - L.g, b);
-          R o[8];
-          beta_step(b, hs, hp, o, ((j - L.g) & 7u) == 0);
-#pragma unroll
-          for (int s = 0; s < 8; ++s) bcur[s] = o[s];
-        }
-        R ds, dp;
-        alpha_adjoint_step(areg[jj], b, hs, hp, x.e[jj], j >= L.g, ab, ds, dp, L, Y, j);
-        L.at(Y.dsa, j) = ds;
-        L.at(Y.dpr, j) = dp;
-      }
-    }
-  };
-  subwindows_down<AlphaIn<R>>(0, (int)((n_al + 7) / 8), load, body);
-}
-
-// Beta adjoint, forwards over local steps g .. g + S + g - 1 in sub-windows
-// of 8 (beta recomputed from the checkpoint at the top); adds to dsa / dpr.
-// Returns in bb the adjoint of beta at the window top (at the block end,
-// that of the tail beta).
-template <typename R>
-struct BetaRe {
-  R b0[8];
-  In8<R> in;  // staged inputs (halved where used)
-};
-
-template <typename R>
-__device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layout& Y, R (&bb)[8]) {
-  const Lane<R> L = fresh(L0);
-  const uint32_t jend = 2 * L.g + L.S;
-  const uint32_t bst = Y.S + Y.g + 1;
-#pragma unroll
-  for (int s = 0; s < 8; ++s) bb[s] = 0;
-  auto load = [&](uint32_t jb0, BetaRe<R>& x) {
-    const uint32_t len = min(8u, jend - jb0);
-    const uint32_t itop = jb0 + len - L.g;  // beta at local jb0 + len = g + itop
-    load8(L, Y.be, bst, (itop + 7) / 8, x.b0);
-    load_in8(L, Y, (int)jb0, (int)len, x.in);
-  };
-  auto body = [&](uint32_t jb0, const BetaRe<R>& x) {
-    const uint32_t len = min(8u, jend - jb0);
-    // the sub-window's output adjoints and running sys/par adjoints, loaded
-    // ahead of the recompute
-    R bor[8][8], dsr[8], dpr[8];
-#pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
-      const uint32_t j = jb0 + jj;
-      if (jj < len) {
-        if (j < L.g + L.S) {
-          load8(L, Y.bo, Y.S, j - L.g, bor[jj]);
-          dsr[jj] = L.at(Y.dsa, j);
-          dpr[jj] = L.at(Y.dpr, j);
-        } else {
-          // above the own steps this sweep is the only writer: its sums start at zero
-#pragma unroll
-          for (int s = 0; s < 8; ++s) bor[jj][s] = 0;
-          dsr[jj] = dpr[jj] = R(0);
-        }
-      }
-    }
-    R breg[8][8];
-    {
-      R b[8];
-#pragma unroll
-      for (int s = 0; s < 8; ++s) b[s] = x.b0[s];
-#pragma unroll
-      for (int jj = 7; jj >= 0; --jj) {
-        if ((uint32_t)jj < len) {
-#pragma unroll
-          for (int s = 0; s < 8; ++s) breg[jj][s] = b[s];  // beta at local j + 1
-          if (jj > 0) {
-            R o[8];
-            beta_step(b, kHalfL2e<R> * x.in.sa[jj], kHalfL2e<R> * x.in.pr[jj], o, false);
-            select8(L.t_of(jb0 + (uint32_t)jj) < (int)L.K, o, b);
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
-      if (jj < len) {
-        const uint32_t j = jb0 + jj;
-        R nb[8], bn[8], ds, dp;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-          nb[s] = bb[s];
-          bn[s] = bor[jj][s];
-        }
-        beta_adjoint_step(breg[jj], kHalfL2e<R> * x.in.sa[jj], kHalfL2e<R> * x.in.pr[jj], nb, bn, ds, dp);
-        // past the block end the adjoint stops (beta is not stepped there)
-        select8(L.t_of(j) < (int)L.K, nb, bb);
-        L.at(Y.dsa, j) = dsr[jj] + ds;
-        L.at(Y.dpr, j) = dpr[jj] + dp;
-      }
-    }
-  };
-  chunks_up<BetaRe<R>>(L.g, jend, load, body);
-}
-
-// Combine of one half: own window, next segment's alpha warm-up, previous
-// segment's beta warm-up (in that order), added into the LLR adjoints and the
-// previous half's extrinsic adjoint; returns this lane's term of the scale
-// adjoint.  Read-modify-writes in chunks of 8 steps (t, or Pi(t), is a
-// permutation, so no two steps share a position).
+// Own steps [jlo, jhi) (local) of the combine of one half: the alpha- and
+// beta-side sums of the own window, then the next segment's alpha warm-up
+// and the previous segment's beta warm-up (in that order), added into the
+// LLR adjoints and the previous half's extrinsic adjoint; returns this lane's
+// term of the scale adjoint.  Read-modify-writes in chunks of 8 steps (t, or
+// Pi(t), is a permutation, so no two steps share a position).
 template <typename R>
 struct CombIn {
   R ds[8], dp[8], dsn[8], dpn[8], dsp[8], dpp[8], gs[8], gq[8], pv[8], ep[8];
   uint32_t at[8];
 };
 
+// the own-window sums of local step j: alpha side + beta side
 template <typename R>
-__device__ __forceinline__ R combine(const Lane<R> L0, const Layout Y, uint32_t h, R w, const R* eprev, R* g0,
-                                     R* gpar, R* prev) {
+__device__ __forceinline__ void own_sums(const Lane<R>& L, const Layout& Y, uint32_t j, R& ds, R& dp) {
+  ds = L.at(Y.dsa, j) + L.at(Y.dsb, j);
+  dp = L.at(Y.dpr, j) + L.at(Y.dpb, j);
+}
+
+template <typename R>
+__device__ __forceinline__ R combine(const Lane<R> L0, const Layout Y, uint32_t h, R w, const R* eprev, R* g0, R* gpar,
+                                     R* prev, uint32_t jlo, uint32_t jhi) {
   const Lane<R> L = fresh(L0);
   const bool lower = (h & 1u) != 0;
   const bool with_prev = h != 0;  // the first upper half has no a-priori input
-  const uint32_t own_end = L.g + L.S;
   const bool has_next = L.q + 1 < L.M, has_prev = L.q > 0;
   const uint32_t ln = min(L.lane + 1, 63u), lp = L.lane > 0 ? L.lane - 1 : 0u;
-  QppWalk qw(L.q * L.S, L.K, L.f1, L.f2);
+  QppWalk qw((uint32_t)L.t_of(jlo), L.K, L.f1, L.f2);
   R wsum = 0;
   auto load = [&](uint32_t j0, CombIn<R>& x) {
 #pragma unroll
     for (uint32_t jj = 0; jj < 8; ++jj) {
       const uint32_t j = j0 + jj;
-      if (j < own_end) {
+      if (j < jhi) {
         const uint32_t t = (uint32_t)L.t_of(j);
         const uint32_t i = lower ? qw.pi : t;
         qw.next();
         x.at[jj] = i;
-        x.ds[jj] = L.at(Y.dsa, j);
-        x.dp[jj] = L.at(Y.dpr, j);
+        own_sums(L, Y, j, x.ds[jj], x.dp[jj]);
         if (j >= L.S) {  // t in [qS + S - g, qS + S): lane q+1's local j - S
           x.dsn[jj] = L.nb(Y.dsa, j - L.S, ln);
           x.dpn[jj] = L.nb(Y.dpr, j - L.S, ln);
         }
         if (j < 2 * L.g) {  // t in [qS, qS + g): lane q-1's local j + S
-          x.dsp[jj] = L.nb(Y.dsa, j + L.S, lp);
-          x.dpp[jj] = L.nb(Y.dpr, j + L.S, lp);
+          x.dsp[jj] = L.nb(Y.dsb, j + L.S, lp);
+          x.dpp[jj] = L.nb(Y.dpb, j + L.S, lp);
         }
         x.gs[jj] = g0[i];
         x.gq[jj] = gpar[t];
@@ -1135,7 +1218,7 @@ __device__ __forceinline__ R combine(const Lane<R> L0, const Layout Y, uint32_t 
 #pragma unroll
     for (uint32_t jj = 0; jj < 8; ++jj) {
       const uint32_t j = j0 + jj;
-      if (j < own_end) {
+      if (j < jhi) {
         const uint32_t t = (uint32_t)L.t_of(j), i = x.at[jj];
         R ds = x.ds[jj], dp = x.dp[jj];
         if (j >= L.S) {
@@ -1155,7 +1238,7 @@ __device__ __forceinline__ R combine(const Lane<R> L0, const Layout Y, uint32_t 
       }
     }
   };
-  chunks_up<CombIn<R>>(L.g, own_end, load, body);
+  chunks_up<CombIn<R>>(jlo, jhi, load, body);
   return wsum;
 }
 
@@ -1166,7 +1249,7 @@ __device__ __forceinline__ R combine(const Lane<R> L0, const Layout Y, uint32_t 
 // write-back stores arrays 0-2.  Two blocks per batch of loads.
 template <typename R>
 __device__ __forceinline__ void combine_fill(const DiffArgs<R> A, const DiffJob J, uint32_t h, const BlockLds<R> X) {
-  const uint32_t H = A.halves, K = J.k;
+  const uint32_t H = A.halves, K = J.k, nt = blockDim.x;
   const bool lower = (h & 1u) != 0, with_prev = h != 0;
   const bool from_post = with_prev && h - 1 == H - 2;
   const R wp = A.scales[H - 2];
@@ -1182,13 +1265,13 @@ __device__ __forceinline__ void combine_fill(const DiffArgs<R> A, const DiffJob 
       gp[x] = global_view(A.grad_post + C.post_off);
       ep[x] = global_view(A.saved + C.save_off + (size_t)(h ? h - 1 : 0) * K);
     }
-    for (uint32_t i0 = threadIdx.x; i0 < K; i0 += 64 * kFillBatch) {
+    for (uint32_t i0 = threadIdx.x; i0 < K; i0 += nt * kFillBatch) {
       R x0[2][kFillBatch], x1[2][kFillBatch], x2[2][kFillBatch], x3[2][kFillBatch];
 #pragma unroll
       for (uint32_t x = 0; x < 2; ++x)
 #pragma unroll
         for (uint32_t u = 0; u < kFillBatch; ++u) {
-          const uint32_t i = i0 + 64 * u;
+          const uint32_t i = i0 + nt * u;
           if (x < nb && i < K) {
             x0[x][u] = g0[x][i];
             x1[x][u] = gq[x][i];
@@ -1200,7 +1283,7 @@ __device__ __forceinline__ void combine_fill(const DiffArgs<R> A, const DiffJob 
       for (uint32_t x = 0; x < 2; ++x)
 #pragma unroll
         for (uint32_t u = 0; u < kFillBatch; ++u) {
-          const uint32_t i = i0 + 64 * u;
+          const uint32_t i = i0 + nt * u;
           if (x < nb && i < K) {
             xl[X.idx(0, b0 + x, i)] = x0[x][u];
             xl[X.idx(1, b0 + x, i)] = x1[x][u];
@@ -1222,7 +1305,7 @@ __device__ __forceinline__ void combine_store(const DiffArgs<R> A, const DiffJob
     const GlobalPtr<R> g0 = global_view(A.grad_llr + C.llr_off);
     const GlobalPtr<R> gq = g0 + (size_t)(lower ? 2 : 1) * (K + 4);
     const GlobalPtr<R> prev = global_view(A.ebar + C.ebar_off + (size_t)((h + 1) & 1u) * K);
-    for (uint32_t i = threadIdx.x; i < K; i += 64) {
+    for (uint32_t i = threadIdx.x; i < K; i += blockDim.x) {
       g0[i] = xl[X.idx(0, b, i)];
       gq[i] = xl[X.idx(1, b, i)];
       if (with_prev) prev[i] = xl[X.idx(2, b, i)];
@@ -1230,9 +1313,9 @@ __device__ __forceinline__ void combine_store(const DiffArgs<R> A, const DiffJob
   }
 }
 
-// Per-lane part of the LDS combine (the same sums, in the same order, as
-// combine), with the next chunk's scratch loads in flight while a chunk is
-// added into the LDS copies.
+// Per-lane part of the LDS combine over own local steps [jlo, jhi) (the same
+// sums, in the same order, as combine), with the next chunk's scratch loads
+// in flight while a chunk is added into the LDS copies.
 template <typename R>
 struct CombLdsIn {
   R ds[8], dp[8], dsn[8], dpn[8], dsp[8], dpp[8];
@@ -1240,29 +1323,27 @@ struct CombLdsIn {
 
 template <typename R>
 __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uint32_t h, R w, const BlockLds<R>& X,
-                                         uint32_t blk) {
+                                         uint32_t blk, uint32_t jlo, uint32_t jhi) {
   const Lane<R> L = fresh(L0);
   const bool lower = (h & 1u) != 0;
   const bool with_prev = h != 0;
-  const uint32_t own_end = L.g + L.S;
   const bool has_next = L.q + 1 < L.M, has_prev = L.q > 0;
   const uint32_t ln = min(L.lane + 1, 63u), lp = L.lane > 0 ? L.lane - 1 : 0u;
-  QppWalk qw(L.q * L.S, L.K, L.f1, L.f2);
+  QppWalk qw((uint32_t)L.t_of(jlo), L.K, L.f1, L.f2);
   R wsum = 0;
   auto load = [&](uint32_t j0, CombLdsIn<R>& x) {
 #pragma unroll
     for (uint32_t jj = 0; jj < 8; ++jj) {
       const uint32_t j = j0 + jj;
-      if (j < own_end) {
-        x.ds[jj] = L.at(Y.dsa, j);
-        x.dp[jj] = L.at(Y.dpr, j);
+      if (j < jhi) {
+        own_sums(L, Y, j, x.ds[jj], x.dp[jj]);
         if (j >= L.S) {  // t in [qS + S - g, qS + S): lane q+1's local j - S
           x.dsn[jj] = L.nb(Y.dsa, j - L.S, ln);
           x.dpn[jj] = L.nb(Y.dpr, j - L.S, ln);
         }
         if (j < 2 * L.g) {  // t in [qS, qS + g): lane q-1's local j + S
-          x.dsp[jj] = L.nb(Y.dsa, j + L.S, lp);
-          x.dpp[jj] = L.nb(Y.dpr, j + L.S, lp);
+          x.dsp[jj] = L.nb(Y.dsb, j + L.S, lp);
+          x.dpp[jj] = L.nb(Y.dpb, j + L.S, lp);
         }
       }
     }
@@ -1271,7 +1352,7 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
 #pragma unroll
     for (uint32_t jj = 0; jj < 8; ++jj) {
       const uint32_t j = j0 + jj;
-      if (j < own_end) {
+      if (j < jhi) {
         const uint32_t t = (uint32_t)L.t_of(j);
         const uint32_t i = lower ? qw.pi : t;
         qw.next();
@@ -1293,43 +1374,59 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
       }
     }
   };
-  chunks_up<CombLdsIn<R>>(L.g, own_end, load, body);
+  chunks_up<CombLdsIn<R>>(jlo, jhi, load, body);
   return wsum;
+}
+
+// Sum over the wave, then over the workgroup's two waves (wave 1's partial
+// through LDS, added to wave 0's in that order); the result is valid in
+// thread 0.
+template <typename R>
+__device__ __forceinline__ R workgroup_sum(R v, R* slot) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (threadIdx.x == 64) *slot = v;
+  __syncthreads();
+  return v + *slot;
 }
 
 }  // namespace
 
-template <typename R>
-__global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
+template <typename R, bool kLds>
+__global__ __launch_bounds__(128, kDiffWavesPerSimd<R>) void diff_backward_kernel(DiffArgs<R> A) {
   extern __shared__ __attribute__((aligned(16))) char diff_lds[];
+  __shared__ R wave1_part[2];
   const Layout Y(A.max_seg, A.max_warm);
   const uint32_t H = A.halves;
+  // the wave's index in the workgroup, as a scalar (uniform in a wave;
+  // read through readfirstlane, everything derived from it stays in scalar
+  // registers and scalar branches)
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (uint32_t jb = blockIdx.x; jb < A.njobs; jb += gridDim.x) {
     const DiffJob J = A.jobs[jb];
     const Lane<R> L0 = make_lane(J, A);
     const uint32_t blk = L0.active ? L0.lane / J.nseg : 0;
     const BlockLds<R> X = block_lds(reinterpret_cast<R*>(diff_lds), J);
-    const Lane<R> L = A.use_lds ? with_lds(L0, X, blk) : L0;
-    const DiffCb C = A.cbs[J.cb_first + blk];
-    const uint32_t K = L.K, D = K + 4;
-    const R* d0 = A.llr + C.llr_off;
-    const R* d1 = d0 + D;
-    const R* d2 = d1 + D;
-    const R* sv = A.saved + C.save_off;
-    const R* gp = A.grad_post + C.post_off;
-    R* g0 = A.grad_llr + C.llr_off;
-    R* g1 = g0 + D;
-    R* g2 = g1 + D;
-    R* eb[2] = {A.ebar + C.ebar_off, A.ebar + C.ebar_off + K};
+    const Lane<R> L = kLds ? with_lds(L0, X, blk) : L0;
+    // the lane's block, fetched where a phase needs it (its pointers then
+    // hold no registers through the passes)
+    auto cb = [&]() { return BlockPtrs<R>(A, J, blk); };
+    const uint32_t K = L.K;
     R* dsc = A.dscale_part + (size_t)jb * H;
-    const uint32_t lo = L.q * L.S, hi = lo + L.S;
+    const Chunks CH(L.S, L.g);
+    uint32_t m0, m1;
+    own_range(CH, wave, m0, m1);
+    // own local steps of this wave in phase 4, and their natural positions
+    const uint32_t jlo = L.g + min(8u * m0, L.S), jhi = L.g + min(8u * m1, L.S);
+    const uint32_t lo = L.q * L.S + (jlo - L.g), hi = L.q * L.S + (jhi - L.g);
     {
       R s1 = 0, s2 = 0;
-      if (L.active) post_adjoint(A, L, sv, gp, g0, g1, g2, eb[(H - 1) & 1u], s1, s2);
-      for (int o = 32; o > 0; o >>= 1) {
-        s1 += __shfl_xor(s1, o, 64);
-        s2 += __shfl_xor(s2, o, 64);
+      if (L.active) {
+        const BlockPtrs<R> B = cb();
+        post_adjoint(A, L, B.sv, A.grad_post + B.c.post_off, B.g(A, 0, K), B.g(A, 1, K), B.g(A, 2, K),
+                     B.ebar(A, (H - 1) & 1u, K), lo, hi, wave == 0, s1, s2);
       }
+      s1 = workgroup_sum(s1, &wave1_part[0]);
+      s2 = workgroup_sum(s2, &wave1_part[1]);
       if (threadIdx.x == 0) {
         for (uint32_t h = 0; h < H; ++h) dsc[h] = 0;
         dsc[H - 2] = s1;
@@ -1338,132 +1435,125 @@ __global__ __launch_bounds__(64) void diff_backward_kernel(DiffArgs<R> A) {
     }
     __syncthreads();
     for (uint32_t h = H; h-- > 0;) {
-      R* cur = eb[h & 1u];
-      R* prev = eb[(h + 1) & 1u];
       const bool lower = (h & 1u) != 0;
-      R ts[3], tp[3];
-      if (A.use_lds) {
+      if constexpr (kLds) {
         stage_fill(A, J, h, X);
         __syncthreads();
         stage_gather(J, h, X);
-        if (L.active) tail_inputs(h, d0, d1, d2, K, ts, tp);
-        __syncthreads();
       } else if (L.active) {
+        const BlockPtrs<R> B = cb();
         // adjoint of e_{h-1}: starts at the posterior term for h - 1 == H - 2
         // (the LDS path starts it in its combine fill)
+        R* prev = B.ebar(A, (h + 1) & 1u, K);
+        const R* gp = A.grad_post + B.c.post_off;
         if (h >= 1) {
           if (h - 1 == H - 2) {
             const R wp = A.scales[H - 2];
-            for (uint32_t t0 = lo; t0 < hi; t0 += 8) {
-              R gv[8];
-#pragma unroll
-              for (uint32_t jj = 0; jj < 8; ++jj)
-                if (t0 + jj < hi) gv[jj] = gp[t0 + jj];
-#pragma unroll
-              for (uint32_t jj = 0; jj < 8; ++jj)
-                if (t0 + jj < hi) prev[t0 + jj] = wp * gv[jj];
-            }
+            for (uint32_t t = lo; t < hi; ++t) prev[t] = wp * gp[t];
           } else {
             for (uint32_t t = lo; t < hi; ++t) prev[t] = 0;
           }
         }
-        tail_inputs(h, d0, d1, d2, K, ts, tp);
-        stage(L, Y, h, d0, d1, d2, sv, A.scales);
+        if (wave == 0) stage(L, Y, h, B.d0, B.d1, B.d2, B.sv, A.scales);
       }
+      __syncthreads();
+      // phase 1: alpha and beta checkpoints
       if (L.active) {
-        // the alpha-adjoint sweep is the first writer of local steps below
-        // g + S, the beta-adjoint sweep of those up to 2g + S; zeros beyond
-        // (a plan's longer windows) keep the combine's reads defined
-        for (uint32_t j = 2 * L.g + L.S; j < Y.P; ++j) L.at(Y.dsa, j) = L.at(Y.dpr, j) = 0;
-        // beta over the window's warm-up above the own steps (local 2g + S - 1
-        // down to g + S), run beside the alpha checkpoint pass (alpha at every
-        // 8th local step only, al[c] = alpha at local 8c; the adjoint sweeps
-        // recompute each sub-window's metrics into registers, which keeps the
-        // wave's scratch small enough to stay in the caches).  The own steps'
-        // beta runs inside the alpha-adjoint sweep, an independent chain
-        // beside the adjoint recursion.  Checkpoints for the beta-adjoint
-        // sweep: beta at local g + i for i a multiple of 8 or i = S + g, at
-        // be[(i + 7) / 8]
-        R bcur[8];
-        if (lo + L.S + L.g >= K) {
-          tail_beta(ts, tp, bcur);
+        const BlockPtrs<R> B = cb();
+        checkpoint_passes(L, Y, CH, wave, h, B.d0, B.d1, B.d2);
+      }
+      __syncthreads();
+      // phase 2: G(j, n) = e_j x (+-1 by the input bit of n) x the weight of
+      // state n in its class's LLR term (e_j: the adjoint of the extrinsic)
+      if (L.active) {
+        const GlobalPtr<const R> eg = global_view<R>(cb().ebar(A, h & 1u, K));
+        QppWalk qw(L.q * L.S + 8u * m0, L.K, L.f1, L.f2);
+        own_terms(L, Y, CH, m0, m1, [&](uint32_t j, const R (&T)[8], R) {
+          const R e = eg[lower ? qw.pi : (uint32_t)L.t_of(j)];
+          qw.next();
+          R m[2], s[2], p[8];
+          class_sums(T, m, s, p);
+          const R f1 = e * Num<R>::rcp(s[1]), f0 = -e * Num<R>::rcp(s[0]);
+#pragma unroll
+          for (unsigned n = 0; n < 8; ++n) L.at(Y.gl + n * Y.S, j - L.g) = (in_of(n) ? f1 : f0) * p[n];
+        });
+      }
+      __syncthreads();
+      // phase 3: the two adjoint sweeps
+      if (L.active) {
+        if (wave == 0) {
+          alpha_adjoint_sweep(L, Y, CH, lower, cb().ebar(A, h & 1u, K));
         } else {
+          R bb[8];
+          beta_adjoint_sweep(L, Y, CH, bb);
+          // beta of the window top: the tail beta when it reaches K
+          R dts[3] = {0, 0, 0}, dtp[3] = {0, 0, 0};
+          if (L.q * L.S + L.S + L.g >= K) {
 #pragma unroll
-          for (int s = 0; s < 8; ++s) bcur[s] = 0;
-        }
-        const uint32_t bst = Y.S + Y.g + 1;
-        auto beta_ck = [&](uint32_t i, const R (&b)[8]) {
-          if ((i & 7u) == 0 || i == L.S + L.g) {
+            for (unsigned s = 0; s < 8; ++s) {
+              unsigned c = s;
 #pragma unroll
-            for (int s = 0; s < 8; ++s) L.at(Y.be + s * bst, (i + 7) / 8) = b[s];
-          }
-        };
-        alpha_ckpt_beta_warm_pass(L, Y, bcur, beta_ck);
-        alpha_adjoint_sweep(L, Y, bcur, beta_ck, lower, cur);
-        R bb[8];
-        beta_adjoint_sweep(L, Y, bb);
-        // beta of the window top: the tail beta when it reaches K
-        R dts[3] = {0, 0, 0}, dtp[3] = {0, 0, 0};
-        if (lo + L.S + L.g >= K) {
-#pragma unroll
-          for (unsigned s = 0; s < 8; ++s) {
-            unsigned c = s;
-#pragma unroll
-            for (int k3 = 0; k3 < 3; ++k3) {
-              const unsigned u = term_in(c);
-              dts[k3] += u ? R(0.5) * bb[s] : R(-0.5) * bb[s];
-              dtp[k3] += par(c, u) ? R(0.5) * bb[s] : R(-0.5) * bb[s];
-              c = nxt(c, u);
+              for (int k3 = 0; k3 < 3; ++k3) {
+                const unsigned u = term_in(c);
+                dts[k3] += u ? R(0.5) * bb[s] : R(-0.5) * bb[s];
+                dtp[k3] += par(c, u) ? R(0.5) * bb[s] : R(-0.5) * bb[s];
+                c = nxt(c, u);
+              }
             }
           }
-        }
-        for (int k3 = 0; k3 < 3; ++k3) {
-          L.at(Y.tl, k3) = dts[k3];
-          L.at(Y.tl, 3 + k3) = dtp[k3];
+          for (int k3 = 0; k3 < 3; ++k3) {
+            L.at(Y.tl, k3) = dts[k3];
+            L.at(Y.tl, 3 + k3) = dtp[k3];
+          }
         }
       }
       __syncthreads();
+      // phase 4: combine
       R wsum = 0;
       const R w = h ? A.scales[h - 1] : R(0);
-      if (A.use_lds) {
+      if constexpr (kLds) {
         combine_fill(A, J, h, X);
         __syncthreads();
-        if (L.active) wsum = combine_lds(L, Y, h, w, X, blk);
+        if (L.active) wsum = combine_lds(L, Y, h, w, X, blk, jlo, jhi);
         __syncthreads();
         combine_store(A, J, h, X);
       } else if (L.active) {
-        const R* eprev = sv + (size_t)(h ? h - 1 : 0) * K;
-        wsum = combine(L, Y, h, w, eprev, g0, lower ? g2 : g1, prev);
+        const BlockPtrs<R> B = cb();
+        const R* eprev = B.sv + (size_t)(h ? h - 1 : 0) * K;
+        wsum = combine(L, Y, h, w, eprev, B.g(A, 0, K), B.g(A, lower ? 2 : 1, K), B.ebar(A, (h + 1) & 1u, K), jlo,
+                       jhi);
       }
-      if (L.active) {
-        if (L.q == L.M - 1) {
-          // tail LLR adjoints, in segment order
-          R dts[3] = {0, 0, 0}, dtp[3] = {0, 0, 0};
-          for (uint32_t m = 0; m < L.M; ++m) {
-            const uint32_t other = L.lane - (L.M - 1) + m;
-            if (m * L.S + L.S + L.g < K) continue;
-            for (int k3 = 0; k3 < 3; ++k3) {
-              dts[k3] += L.nb(Y.tl, k3, other);
-              dtp[k3] += L.nb(Y.tl, 3 + k3, other);
-            }
-          }
-          if (!lower) {
-            g0[K] += dts[0];
-            g0[K + 2] += dts[1];
-            g1[K] += dtp[0];
-            g0[K + 1] += dtp[1];
-            g1[K + 2] += dtp[2];
-          } else {
-            g1[K + 3] += dts[0];
-            g2[K] += dts[1];
-            g2[K + 2] += dts[2];
-            g2[K + 1] += dtp[0];
-            g0[K + 3] += dtp[1];
-            g2[K + 3] += dtp[2];
+      if (L.active && wave == 0 && L.q == L.M - 1) {
+        const BlockPtrs<R> B = cb();
+        R* g0 = B.g(A, 0, K);
+        R* g1 = B.g(A, 1, K);
+        R* g2 = B.g(A, 2, K);
+        // tail LLR adjoints, in segment order
+        R dts[3] = {0, 0, 0}, dtp[3] = {0, 0, 0};
+        for (uint32_t m = 0; m < L.M; ++m) {
+          const uint32_t other = L.lane - (L.M - 1) + m;
+          if (m * L.S + L.S + L.g < K) continue;
+          for (int k3 = 0; k3 < 3; ++k3) {
+            dts[k3] += L.nb(Y.tl, k3, other);
+            dtp[k3] += L.nb(Y.tl, 3 + k3, other);
           }
         }
+        if (!lower) {
+          g0[K] += dts[0];
+          g0[K + 2] += dts[1];
+          g1[K] += dtp[0];
+          g0[K + 1] += dtp[1];
+          g1[K + 2] += dtp[2];
+        } else {
+          g1[K + 3] += dts[0];
+          g2[K] += dts[1];
+          g2[K + 2] += dts[2];
+          g2[K + 1] += dtp[0];
+          g0[K + 3] += dtp[1];
+          g2[K + 3] += dtp[2];
+        }
       }
-      for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o, 64);
+      wsum = workgroup_sum(wsum, &wave1_part[0]);
       if (threadIdx.x == 0 && h) dsc[h - 1] += wsum;
       __syncthreads();
     }
@@ -1489,10 +1579,14 @@ __global__ __launch_bounds__(256) void diff_scale_reduce_kernel(const R* part, u
   if (threadIdx.x == 0) out[h] = red[0];
 }
 
-template __global__ void diff_forward_kernel<float>(DiffArgs<float>);
-template __global__ void diff_forward_kernel<double>(DiffArgs<double>);
-template __global__ void diff_backward_kernel<float>(DiffArgs<float>);
-template __global__ void diff_backward_kernel<double>(DiffArgs<double>);
+template __global__ void diff_forward_kernel<float, true>(DiffArgs<float>);
+template __global__ void diff_forward_kernel<float, false>(DiffArgs<float>);
+template __global__ void diff_forward_kernel<double, true>(DiffArgs<double>);
+template __global__ void diff_forward_kernel<double, false>(DiffArgs<double>);
+template __global__ void diff_backward_kernel<float, true>(DiffArgs<float>);
+template __global__ void diff_backward_kernel<float, false>(DiffArgs<float>);
+template __global__ void diff_backward_kernel<double, true>(DiffArgs<double>);
+template __global__ void diff_backward_kernel<double, false>(DiffArgs<double>);
 template __global__ void diff_scale_reduce_kernel<float>(const float*, uint32_t, uint32_t, float*);
 template __global__ void diff_scale_reduce_kernel<double>(const double*, uint32_t, uint32_t, double*);
 
