@@ -300,7 +300,9 @@ tc_status tc_qam_demap(tc_channel_plan* plan, const float* y_dev, const float* h
  * beta recursions `warmup` (<= S) steps outside the segment from zero metrics.
  * fp64 != 0 runs every buffer in double (for finite-difference checks),
  * else float.  Layouts: llr = [d0|d1|d2] of K+4 per block, post K per block,
- * scales 2I, saved = the forward's extrinsics for the backward (sizes from
+ * scales 2I, saved = the forward's extrinsics and its alpha/beta checkpoints
+ * (every 8 steps of each segment's window, every half-iteration) for the
+ * backward, which recomputes the metrics between checkpoints (sizes from
  * tc_diff_plan_sizes: [0] llr, [1] post, [2] scales, [3] saved elements).
  * grad_llr is overwritten (same layout as llr); grad_scales gets 2I values.
  * Calls enqueue on the given stream and allocate nothing; calls on one plan

@@ -29,6 +29,7 @@ struct tc_diff_plan {
   uint32_t halves = 0, njobs = 0, grid = 0, max_seg = 0, max_warm = 0, slot_elems = 0;
   size_t lds_bytes = 0;  // dynamic LDS per workgroup (0: no LDS path)
   uint64_t llr_elems = 0, post_elems = 0, saved_elems = 0, ebar_elems = 0;
+  uint64_t ext_elems = 0, ckpt_job_half = 0;  // saved: extrinsics, then checkpoints
   tcdev::DiffJob* d_jobs = nullptr;
   tcdev::DiffCb* d_cbs = nullptr;
   void* d_ebar = nullptr;
@@ -44,12 +45,12 @@ namespace {
 
 constexpr uint32_t kMaxHalves = 32;
 
-// Scratch elements per lane: staged sys+la and par, alpha and beta
-// checkpoints, the LLR's adjoint terms of the own steps, the sys/par adjoints
-// of the alpha and the beta side and six tail adjoints.
+// Scratch elements per lane: staged sys+la and par, the LLR's adjoint terms
+// of the own steps, the sys/par adjoints of the alpha and the beta side and
+// six tail adjoints.
 uint32_t slot_elems(uint32_t s, uint32_t g) {
   const uint32_t p = s + 2 * g;
-  return 6 * p + 8 * s + 16 * (g + s + 1) + 6;
+  return 6 * p + 8 * s + 6;
 }
 
 // A job runs on a workgroup of two waves (alpha side and beta side).
@@ -115,6 +116,10 @@ void build(tc_diff_plan& p, const uint32_t* k, size_t n, uint32_t iterations, ui
   }
   p.njobs = static_cast<uint32_t>(jobs.size());
   p.slot_elems = slot_elems(p.max_seg, p.max_warm);
+  // the forward's checkpoints follow the extrinsics in the saved state
+  p.ext_elems = p.saved_elems;
+  p.ckpt_job_half = 64ull * tcdev::diff_ck_lane_elems(p.max_seg, p.max_warm);
+  p.saved_elems += (uint64_t)p.njobs * p.halves * p.ckpt_job_half;
   if (n == 0) return;
   require_device();
   int dev = 0, cus = 0;
@@ -154,6 +159,8 @@ tcdev::DiffArgs<R> args(const tc_diff_plan& p, const void* llr, const void* scal
   a.ebar = static_cast<R*>(p.d_ebar);
   a.dscale_part = static_cast<R*>(p.d_part);
   a.scratch = static_cast<R*>(p.d_scratch);
+  a.ckpt = saved ? static_cast<R*>(saved) + p.ext_elems : nullptr;
+  a.ckpt_job_half = p.ckpt_job_half;
   a.slot_elems = p.slot_elems;
   a.max_seg = p.max_seg;
   a.max_warm = p.max_warm;

@@ -17,8 +17,9 @@
 // half-iteration's inputs in [qS - g, qS + S + g), so each half-iteration is
 // a function of the previous one's extrinsics and the adjoint is local too.
 // Wave 0 carries a segment's alpha side and wave 1 its beta side; per half:
-//   1. wave 0 runs alpha over the segment's window, wave 1 beta, each keeping
-//      a checkpoint every 8 steps;
+//   1. (forward) wave 0 runs alpha over the segment's window, wave 1 beta,
+//      each keeping a checkpoint every 8 steps in the saved state, which the
+//      adjoint reads back instead of running this phase again;
 //   2. the two waves split the own steps: from the checkpoints each
 //      recomputes alpha_{t+1} and beta_{t+1} of its steps and forms the
 //      LLR from the eight states at t + 1 (the input bit of a transition is
@@ -203,16 +204,21 @@ struct QppWalk {
   }
 };
 
-// Per-lane scratch fields, each [j][lane] with j < its length.
+// Per-lane scratch fields, each [j][lane] with j < its length, and the
+// fields of a lane's checkpoints of one half (alpha: cna per state, beta: cnb
+// per state; kept by the forward for the adjoint).
 struct Layout {
   uint32_t P, S, g;
-  uint32_t sa, pr, al, be, gl, dsa, dpr, dsb, dpb, tl, total;
+  uint32_t sa, pr, gl, dsa, dpr, dsb, dpb, tl, total;
+  uint32_t cna, cnb, cal, cbe;
   __device__ Layout(uint32_t s, uint32_t gw) : P(s + 2 * gw), S(s), g(gw) {
+    cna = diff_ck_alpha(s, gw);
+    cnb = diff_ck_beta(s, gw);
+    cal = 0;
+    cbe = 8 * cna;
     sa = 0;
     pr = sa + P;
-    al = pr + P;                 // alpha checkpoints, 8 x (g + S + 1)
-    be = al + (g + S + 1) * 8;   // beta checkpoints, 8 x (S + g + 1)
-    gl = be + (S + g + 1) * 8;   // G of the own steps, 8 x S
+    gl = pr + P;                 // G of the own steps, 8 x S
     dsa = gl + S * 8;            // alpha side: sys + la and par adjoints
     dpr = dsa + P;
     dsb = dpr + P;               // beta side
@@ -236,6 +242,7 @@ struct Lane {
   bool active;
   uint32_t lane;
   R* sc;  // the job's scratch (uniform), element (field + j) * 64 + lane
+  R* ck;  // the job's checkpoints of the current half, the same order
   // staged inputs: the block's LDS pairs (lds; local step j is position
   // p = qS + j, pair at p + p / 32) or the per-lane sa / pr fields
   bool lds;
@@ -247,6 +254,9 @@ struct Lane {
     return *reinterpret_cast<R*>(reinterpret_cast<char*>(sc) + (uint32_t)(((field + j) * 64u + other) * sizeof(R)));
   }
   __device__ R& at(uint32_t field, uint32_t j) const { return nb(field, j, lane); }
+  __device__ R& ckat(uint32_t field, uint32_t j) const {
+    return *reinterpret_cast<R*>(reinterpret_cast<char*>(ck) + (uint32_t)(((field + j) * 64u + lane) * sizeof(R)));
+  }
   // local step j covers t = qS - g + j
   __device__ int t_of(uint32_t j) const { return (int)(q * S) - (int)g + (int)j; }
 };
@@ -569,6 +579,12 @@ __device__ void tail_inputs(uint32_t h, const R* d0, const R* d1, const R* d2, u
 }
 
 template <typename R>
+__device__ __forceinline__ void ckload8(const Lane<R>& L, uint32_t field, uint32_t stride, uint32_t j, R (&v)[8]) {
+#pragma unroll
+  for (int s = 0; s < 8; ++s) v[s] = L.ckat(field + s * stride, j);
+}
+
+template <typename R>
 __device__ __forceinline__ void load8(const Lane<R>& L, uint32_t field, uint32_t stride, uint32_t j, R (&v)[8]) {
 #pragma unroll
   for (int s = 0; s < 8; ++s) v[s] = L.at(field + s * stride, j);
@@ -675,7 +691,7 @@ __device__ __forceinline__ void alpha_ckpt_pass(const Lane<R>& L0, const Layout&
   const bool from_zero_state = (int)(L.q * L.S) - (int)L.g <= 0;
 #pragma unroll
   for (int s = 0; s < 8; ++s) a[s] = from_zero_state ? (s == 0 ? R(0) : R(kNegF)) : R(0);
-  const uint32_t end = L.g + L.S, ast = Y.g + Y.S + 1;
+  const uint32_t end = L.g + L.S;
   auto load = [&](uint32_t k, In8<R>& c) {
     const uint32_t lo = C.a_lo(k);
     load_in8(L, Y, (int)lo, (int)(C.a_hi(k, end) - lo), c);
@@ -683,7 +699,7 @@ __device__ __forceinline__ void alpha_ckpt_pass(const Lane<R>& L0, const Layout&
   auto body = [&](uint32_t k, const In8<R>& c) {
     const uint32_t lo = C.a_lo(k), n = C.a_hi(k, end) - lo;
 #pragma unroll
-    for (int s = 0; s < 8; ++s) L.at(Y.al + s * ast, k) = a[s];
+    for (int s = 0; s < 8; ++s) L.ckat(Y.cal + s * Y.cna, k) = a[s];
 #pragma unroll
     for (uint32_t jj = 0; jj < 8; ++jj)
       if (jj < n) {
@@ -702,7 +718,7 @@ __device__ __forceinline__ void alpha_ckpt_pass(const Lane<R>& L0, const Layout&
 template <typename R>
 __device__ __forceinline__ void beta_ckpt_pass(const Lane<R>& L0, const Layout& Y, const Chunks& C, R (&b)[8]) {
   const Lane<R> L = fresh(L0);
-  const uint32_t top = 2 * L.g + L.S, bst = Y.S + Y.g + 1;
+  const uint32_t top = 2 * L.g + L.S;
   auto load = [&](uint32_t m, In8<R>& c) {
     const uint32_t lo = L.g + 8u * m, hi = min(lo + 8u, top);
     load_in8(L, Y, (int)lo, (int)(hi - lo), c);
@@ -710,7 +726,7 @@ __device__ __forceinline__ void beta_ckpt_pass(const Lane<R>& L0, const Layout& 
   auto body = [&](uint32_t m, const In8<R>& c) {
     const uint32_t lo = L.g + 8u * m, n = min(lo + 8u, top) - lo;
 #pragma unroll
-    for (int s = 0; s < 8; ++s) L.at(Y.be + s * bst, m + 1) = b[s];
+    for (int s = 0; s < 8; ++s) L.ckat(Y.cbe + s * Y.cnb, m + 1) = b[s];
 #pragma unroll
     for (int jj = 7; jj >= 0; --jj)
       if ((uint32_t)jj < n) {
@@ -736,13 +752,12 @@ template <typename R, class Term>
 __device__ __forceinline__ void own_terms(const Lane<R>& L0, const Layout& Y, const Chunks& C, uint32_t m0, uint32_t m1,
                                           Term&& term) {
   const Lane<R> L = fresh(L0);
-  const uint32_t ast = Y.g + Y.S + 1, bst = Y.S + Y.g + 1;
   const uint32_t own_end = L.g + L.S, top = 2 * L.g + L.S;
   auto load = [&](uint32_t m, OwnIn<R>& x) {
     const uint32_t lo = L.g + 8u * m;
     (void)lo;
-    load8(L, Y.al, ast, C.own_alpha(m, L.g), x.a0);
-    load8(L, Y.be, bst, m + 1, x.b0);
+    ckload8(L, Y.cal, Y.cna, C.own_alpha(m, L.g), x.a0);
+    ckload8(L, Y.cbe, Y.cnb, m + 1, x.b0);
   };
   auto body = [&](uint32_t m, const OwnIn<R>& xc) {
     const uint32_t lo = L.g + 8u * m, n = min(8u, own_end - lo), nb = min(lo + 8u, top) - lo;
@@ -809,14 +824,14 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
                                                     const R* ecur) {
   const Lane<R> L = fresh(L0);
   R ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const uint32_t end = L.g + L.S, ast = Y.g + Y.S + 1;
+  const uint32_t end = L.g + L.S;
   const GlobalPtr<const R> eg = global_view(ecur);
   // own steps' positions, walked down from the last one (loads go in
   // descending order of steps)
   QppWalk qw((uint32_t)L.t_of(end - 1), L.K, L.f1, L.f2);
   auto load = [&](uint32_t k, AlphaIn<R>& x) {
     const uint32_t lo = C.a_lo(k), n = C.a_hi(k, end) - lo;
-    load8(L, Y.al, ast, k, x.a0);
+    ckload8(L, Y.cal, Y.cna, k, x.a0);
 #pragma unroll
     for (int jj = 7; jj >= 0; --jj) {
       if ((uint32_t)jj < n) {
@@ -895,13 +910,13 @@ struct BetaRe {
 template <typename R>
 __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layout& Y, const Chunks& C, R (&bb)[8]) {
   const Lane<R> L = fresh(L0);
-  const uint32_t top = 2 * L.g + L.S, own_end = L.g + L.S, bst = Y.S + Y.g + 1;
+  const uint32_t top = 2 * L.g + L.S, own_end = L.g + L.S;
 #pragma unroll
   for (int s = 0; s < 8; ++s) bb[s] = 0;
   auto load = [&](uint32_t m, BetaRe<R>& x) {
     const uint32_t lo = L.g + 8u * m;
     (void)lo;
-    load8(L, Y.be, bst, m + 1, x.b0);
+    ckload8(L, Y.cbe, Y.cnb, m + 1, x.b0);
   };
   auto body = [&](uint32_t m, const BetaRe<R>& xc) {
     const uint32_t lo = L.g + 8u * m, n = min(lo + 8u, top) - lo;
@@ -973,6 +988,7 @@ __device__ Lane<R> make_lane(const DiffJob& J, const DiffArgs<R>& A) {
   L.f2 = J.f2;
   L.active = L.lane / J.nseg < J.nblk;
   L.sc = A.scratch + (size_t)blockIdx.x * 64 * A.slot_elems;
+  L.ck = nullptr;
   L.lds = false;
   L.xs = nullptr;
   return L;
@@ -1074,16 +1090,19 @@ __global__ __launch_bounds__(128, kDiffWavesPerSimd<R>) void diff_forward_kernel
         stage(L, Y, h, B.d0, B.d1, B.d2, B.sv, A.scales);
       }
       __syncthreads();
+      // this half's checkpoints, kept for the adjoint
+      Lane<R> Lh = L;
+      Lh.ck = A.ckpt + ((size_t)jb * A.halves + h) * A.ckpt_job_half;
       if (L.active) {
         const BlockPtrs<R> B = cb();
-        checkpoint_passes(L, Y, CH, wave, h, B.d0, B.d1, B.d2);
+        checkpoint_passes(Lh, Y, CH, wave, h, B.d0, B.d1, B.d2);
       }
       __syncthreads();
       if (L.active) {
         const bool lower = (h & 1u) != 0;
         R* e = cb().sv_mut + (size_t)h * L.K;
         QppWalk qw(L.q * L.S + 8u * m0, L.K, L.f1, L.f2);
-        own_terms(L, Y, CH, m0, m1, [&](uint32_t j, const R (&T)[8], R sa) {
+        own_terms(Lh, Y, CH, m0, m1, [&](uint32_t j, const R (&T)[8], R sa) {
           R m[2], s[2], p[8];
           class_sums(T, m, s, p);
           const R l1 = m[1] + Num<R>::lg2(s[1]), l0 = m[0] + Num<R>::lg2(s[0]);
@@ -1457,18 +1476,15 @@ __global__ __launch_bounds__(128, kDiffWavesPerSimd<R>) void diff_backward_kerne
         if (wave == 0) stage(L, Y, h, B.d0, B.d1, B.d2, B.sv, A.scales);
       }
       __syncthreads();
-      // phase 1: alpha and beta checkpoints
-      if (L.active) {
-        const BlockPtrs<R> B = cb();
-        checkpoint_passes(L, Y, CH, wave, h, B.d0, B.d1, B.d2);
-      }
-      __syncthreads();
+      // the forward's checkpoints of this half (phase 1 ran there)
+      Lane<R> Lh = L;
+      Lh.ck = A.ckpt + ((size_t)jb * H + h) * A.ckpt_job_half;
       // phase 2: G(j, n) = e_j x (+-1 by the input bit of n) x the weight of
       // state n in its class's LLR term (e_j: the adjoint of the extrinsic)
       if (L.active) {
         const GlobalPtr<const R> eg = global_view<R>(cb().ebar(A, h & 1u, K));
         QppWalk qw(L.q * L.S + 8u * m0, L.K, L.f1, L.f2);
-        own_terms(L, Y, CH, m0, m1, [&](uint32_t j, const R (&T)[8], R) {
+        own_terms(Lh, Y, CH, m0, m1, [&](uint32_t j, const R (&T)[8], R) {
           const R e = eg[lower ? qw.pi : (uint32_t)L.t_of(j)];
           qw.next();
           R m[2], s[2], p[8];
@@ -1482,10 +1498,10 @@ __global__ __launch_bounds__(128, kDiffWavesPerSimd<R>) void diff_backward_kerne
       // phase 3: the two adjoint sweeps
       if (L.active) {
         if (wave == 0) {
-          alpha_adjoint_sweep(L, Y, CH, lower, cb().ebar(A, h & 1u, K));
+          alpha_adjoint_sweep(Lh, Y, CH, lower, cb().ebar(A, h & 1u, K));
         } else {
           R bb[8];
-          beta_adjoint_sweep(L, Y, CH, bb);
+          beta_adjoint_sweep(Lh, Y, CH, bb);
           // beta of the window top: the tail beta when it reaches K
           R dts[3] = {0, 0, 0}, dtp[3] = {0, 0, 0};
           if (L.q * L.S + L.S + L.g >= K) {

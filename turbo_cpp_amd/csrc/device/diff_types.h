@@ -36,6 +36,11 @@ struct DiffArgs {
   R* dscale_part;  // [njobs][2I]
   R* scratch;      // per lane, slot_elems each
   uint32_t slot_elems;
+  // the forward's alpha / beta checkpoints, kept for the adjoint: job jb's
+  // half h at ckpt + (jb * halves + h) * ckpt_job_half, per lane
+  // diff_ck_lane_elems elements in [element][lane] order
+  R* ckpt;
+  uint64_t ckpt_job_half;
   uint32_t max_seg, max_warm;
   // 1: every job's blocks fit their LDS copies (diff_lds_elems) in the
   // dynamic LDS of its workgroup (staging and the adjoint combine then go
@@ -43,6 +48,15 @@ struct DiffArgs {
   // read their inputs from LDS)
   int32_t use_lds;
 };
+
+// Checkpoints of one lane and half: alpha at the start of each of its chunks
+// (at most (g + S + 14) / 8 of them) and beta at the top of each chunk (at
+// most (g + S + 7) / 8 + 1 slots), 8 states each.
+__host__ __device__ constexpr uint32_t diff_ck_alpha(uint32_t s, uint32_t g) { return (g + s + 14) / 8; }
+__host__ __device__ constexpr uint32_t diff_ck_beta(uint32_t s, uint32_t g) { return (g + s + 7) / 8 + 1; }
+__host__ __device__ constexpr uint32_t diff_ck_lane_elems(uint32_t s, uint32_t g) {
+  return 8 * (diff_ck_alpha(s, g) + diff_ck_beta(s, g));
+}
 
 // LDS copies per block: kDiffLdsArrays arrays of diff_lds_stride(K)
 // elements, element i at i + i / 32 (one pad per 32 elements, so lanes that
