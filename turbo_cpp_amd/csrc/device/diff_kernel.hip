@@ -20,16 +20,16 @@
 //   1. (forward) wave 0 runs alpha over the segment's window, wave 1 beta,
 //      each keeping a checkpoint every 8 steps in the saved state, which the
 //      adjoint reads back instead of running this phase again;
-//   2. the two waves split the own steps: from the checkpoints each
-//      recomputes alpha_{t+1} and beta_{t+1} of its steps and forms the
+//   2. (forward) the two waves split the own steps: from the checkpoints
+//      each recomputes alpha_{t+1} and beta_{t+1} of its steps and forms the
 //      LLR from the eight states at t + 1 (the input bit of a transition is
-//      fixed by the state it enters): the forward stores the extrinsic, the
-//      adjoint stores G(t, n) = dloss/d(alpha_{t+1}(n) + beta_{t+1}(n)), the
-//      LLR's share of both metrics' adjoints;
+//      fixed by the state it enters);
 //   3. (adjoint) wave 0 runs the alpha adjoint backwards in time (weights of
-//      each state's two incoming transitions, seeded with G), wave 1 the
-//      beta adjoint forwards in time (weights of each state's two outgoing
-//      transitions, seeded with G), each into its own sums of the
+//      each state's two incoming transitions), wave 1 the beta adjoint
+//      forwards in time (weights of each state's two outgoing transitions),
+//      both seeded at the own steps with G(t, n) = dloss/d(alpha_{t+1}(n) +
+//      beta_{t+1}(n)), the LLR's share of both metrics' adjoints, which each
+//      wave forms from metrics it recomputes; each writes its own sums of the
 //      branch-metric adjoints on sys + la and par;
 //   4. (adjoint) the two waves split the own steps again and add the sums
 //      into the LLR adjoints in a fixed order (own alpha side + own beta
@@ -209,7 +209,7 @@ struct QppWalk {
 // per state; kept by the forward for the adjoint).
 struct Layout {
   uint32_t P, S, g;
-  uint32_t sa, pr, gl, dsa, dpr, dsb, dpb, tl, total;
+  uint32_t sa, pr, dsa, dpr, dsb, dpb, tl, total;
   uint32_t cna, cnb, cal, cbe;
   __device__ Layout(uint32_t s, uint32_t gw) : P(s + 2 * gw), S(s), g(gw) {
     cna = diff_ck_alpha(s, gw);
@@ -218,8 +218,7 @@ struct Layout {
     cbe = 8 * cna;
     sa = 0;
     pr = sa + P;
-    gl = pr + P;                 // G of the own steps, 8 x S
-    dsa = gl + S * 8;            // alpha side: sys + la and par adjoints
+    dsa = pr + P;                // alpha side: sys + la and par adjoints
     dpr = dsa + P;
     dsb = dpr + P;               // beta side
     dpb = dsb + P;
@@ -657,11 +656,11 @@ __device__ inline void alpha_adjoint_step(const R (&a)[8], R hs, R hp, R (&ab)[8
   dp_out = R(0.5) * dp;
 }
 
-// One step of the beta adjoint at local j (beta_{j+1} = b): bb holds the
-// adjoint of beta_j on entry (bn: the output adjoints of beta_{j+1}, the
-// LLR's share) and of beta_{j+1} on exit.
+// One step of the beta adjoint at local j (beta_{j+1} = b): bb is the
+// adjoint of beta_j; bn holds the output adjoints of beta_{j+1} (the LLR's
+// share) on entry and the whole adjoint of beta_{j+1} on exit.
 template <typename R>
-__device__ inline void beta_adjoint_step(const R (&b)[8], R hs, R hp, R (&bb)[8], R (&bn)[8], R& ds_out,
+__device__ inline void beta_adjoint_step(const R (&b)[8], R hs, R hp, const R (&bb)[8], R (&bn)[8], R& ds_out,
                                          R& dp_out) {
   R ds = 0, dp = 0;
 #pragma unroll
@@ -674,8 +673,6 @@ __device__ inline void beta_adjoint_step(const R (&b)[8], R hs, R hp, R (&bb)[8]
     dp += par(s, 0) ? d0g : -d0g;
     dp += par(s, 1) ? d1g : -d1g;
   }
-#pragma unroll
-  for (int s = 0; s < 8; ++s) bb[s] = bn[s];
   ds_out = R(0.5) * ds;
   dp_out = R(0.5) * dp;
 }
@@ -807,16 +804,30 @@ __device__ __forceinline__ void own_terms(const Lane<R>& L0, const Layout& Y, co
   loop_up<OwnIn<R>>(m0, m1, load, body);
 }
 
+// G(n) of one own step, the LLR's share of the adjoints of alpha_{j+1}(n) and
+// beta_{j+1}(n): from T(n) = alpha_{j+1}(n) + beta_{j+1}(n) and the adjoint e
+// of the step's extrinsic, e x (+-1 by n's input bit) x the weight of n in
+// its class's LLR term.
+template <typename R>
+__device__ __forceinline__ void llr_adjoint(const R (&T)[8], R e, R (&G)[8]) {
+  R m[2], s[2], p[8];
+  class_sums(T, m, s, p);
+  const R f1 = e * Num<R>::rcp(s[1]), f0 = -e * Num<R>::rcp(s[0]);
+#pragma unroll
+  for (unsigned n = 0; n < 8; ++n) G[n] = (in_of(n) ? f1 : f0) * p[n];
+}
+
 // Phase 3, alpha side: the alpha adjoint backwards over local steps
-// g + S - 1 .. 0, chunk by chunk (alpha recomputed from al into registers),
-// seeded at each own step with G (the LLR's share of the adjoint of
-// alpha_{j+1}); writes dsa / dpr of local steps [0, g + S), with the direct
-// term -e of the extrinsic (ext = ... - (sys + la)) at the own steps.  e:
-// the adjoints of the own steps' extrinsics, read from ecur (natural order:
+// g + S - 1 .. 0, chunk by chunk (alpha recomputed from its checkpoint into
+// registers), seeded at each own step with G, formed here from alpha_{j+1}
+// and beta_{j+1} (beta recomputed downwards from its checkpoint at the
+// chunk's top); writes dsa / dpr of local steps [0, g + S), with the direct
+// term -e of the extrinsic (ext = ... - (sys + la)) at the own steps.  e: the
+// adjoints of the own steps' extrinsics, read from ecur (natural order:
 // position t, or Pi(t) in a lower half) a chunk ahead.
 template <typename R>
 struct AlphaIn {
-  R a0[8], e[8];
+  R a0[8], b0[8], e[8];
 };
 
 template <typename R>
@@ -824,7 +835,7 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
                                                     const R* ecur) {
   const Lane<R> L = fresh(L0);
   R ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const uint32_t end = L.g + L.S;
+  const uint32_t end = L.g + L.S, top = 2 * L.g + L.S;
   const GlobalPtr<const R> eg = global_view(ecur);
   // own steps' positions, walked down from the last one (loads go in
   // descending order of steps)
@@ -832,6 +843,7 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
   auto load = [&](uint32_t k, AlphaIn<R>& x) {
     const uint32_t lo = C.a_lo(k), n = C.a_hi(k, end) - lo;
     ckload8(L, Y.cal, Y.cna, k, x.a0);
+    if (lo >= L.g) ckload8(L, Y.cbe, Y.cnb, (lo - L.g) / 8u + 1u, x.b0);
 #pragma unroll
     for (int jj = 7; jj >= 0; --jj) {
       if ((uint32_t)jj < n) {
@@ -847,23 +859,11 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
   };
   auto body = [&](uint32_t k, const AlphaIn<R>& xc) {
     const uint32_t lo = C.a_lo(k), n = C.a_hi(k, end) - lo;
-
+    const bool own = lo >= L.g;  // alpha chunks start at the own steps' grid
+    const uint32_t nin = own ? min(lo + 8u, top) - lo : n;
     In8<R> in;
-    load_in8(L, Y, (int)lo, (int)n, in);
-    // G of the chunk's own steps, loaded ahead of the recompute
-    R gv[8][8];
-#pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj)
-      if (jj < n) {
-        const uint32_t j = lo + jj;
-        if (j >= L.g) {
-          load8(L, Y.gl, Y.S, j - L.g, gv[jj]);
-        } else {
-#pragma unroll
-          for (int s = 0; s < 8; ++s) gv[jj][s] = R(0);
-        }
-      }
-    R areg[8][8];
+    load_in8(L, Y, (int)lo, (int)nin, in);
+    R areg[8][8], anext[8];
     {
       R a[8];
 #pragma unroll
@@ -873,10 +873,35 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
         if (jj < n) {
 #pragma unroll
           for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
-          if (jj + 1 < n) {
+          if (jj + 1 < n || own) {
             R o[8];
             alpha_step(a, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, alpha_nrm(lo + jj, C.off));
             select8(L.t_of(lo + jj) >= 0, o, a);
+          }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 8; ++s) anext[s] = a[s];  // alpha_{lo + n} in an own chunk
+    }
+    R gv[8][8];
+    if (own) {
+      R b[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) b[s] = xc.b0[s];
+#pragma unroll
+      for (int jj = 7; jj >= 0; --jj) {
+        if ((uint32_t)jj < nin) {
+          if ((uint32_t)jj < n) {  // b: beta at local lo + jj + 1
+            R T[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) T[s] = (jj == 7 || (uint32_t)jj + 1 >= n ? anext[s] : areg[jj + 1 < 8 ? jj + 1 : 7][s]) + b[s];
+            llr_adjoint(T, xc.e[jj], gv[jj]);
+          }
+          if (jj > 0) {
+            const uint32_t j = lo + (uint32_t)jj;
+            R o[8];
+            beta_step(b, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, beta_nrm(j, L.g));
+            select8(L.t_of(j) < (int)L.K, o, b);
           }
         }
       }
@@ -885,8 +910,10 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
     for (int jj = 7; jj >= 0; --jj) {
       if ((uint32_t)jj < n) {
         const uint32_t j = lo + (uint32_t)jj;
+        if (own) {
 #pragma unroll
-        for (int s = 0; s < 8; ++s) ab[s] += gv[jj][s];
+          for (int s = 0; s < 8; ++s) ab[s] += gv[jj][s];
+        }
         R ds, dp;
         alpha_adjoint_step(areg[jj], kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], ab, ds, dp);
         L.at(Y.dsa, j) = ds - xc.e[jj];
@@ -898,43 +925,46 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
 }
 
 // Phase 3, beta side: the beta adjoint forwards over local steps
-// g .. 2g + S - 1, chunk by chunk (beta recomputed downwards from be into
-// registers), with G as the output adjoints of beta_{j+1} at the own steps;
-// writes dsb / dpb of local steps [g, 2g + S).  Returns in bb the adjoint of
-// beta at the window top (at the block end, that of the tail beta).
+// g .. 2g + S - 1, chunk by chunk (beta recomputed downwards from its
+// checkpoint into registers), with G as the output adjoints of beta_{j+1} at
+// the own steps, formed here from beta_{j+1} and alpha_{j+1} (alpha stepped
+// up from its checkpoint at the chunk's start); writes dsb / dpb of local
+// steps [g, 2g + S).  Returns in bb the adjoint of beta at the window top (at
+// the block end, that of the tail beta).  e as for the alpha side.
 template <typename R>
 struct BetaRe {
-  R b0[8];
+  R b0[8], a0[8], e[8];
 };
 
 template <typename R>
-__device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layout& Y, const Chunks& C, R (&bb)[8]) {
+__device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layout& Y, const Chunks& C, bool lower,
+                                                   const R* ecur, R (&bb)[8]) {
   const Lane<R> L = fresh(L0);
   const uint32_t top = 2 * L.g + L.S, own_end = L.g + L.S;
+  const GlobalPtr<const R> eg = global_view(ecur);
+  // own steps' positions, walked up from the first one
+  QppWalk qw(L.q * L.S, L.K, L.f1, L.f2);
 #pragma unroll
   for (int s = 0; s < 8; ++s) bb[s] = 0;
   auto load = [&](uint32_t m, BetaRe<R>& x) {
     const uint32_t lo = L.g + 8u * m;
-    (void)lo;
     ckload8(L, Y.cbe, Y.cnb, m + 1, x.b0);
+    if (lo < own_end) {
+      ckload8(L, Y.cal, Y.cna, C.own_alpha(m, L.g), x.a0);
+#pragma unroll
+      for (uint32_t jj = 0; jj < 8; ++jj)
+        if (lo + jj < own_end) {
+          x.e[jj] = eg[lower ? qw.pi : (uint32_t)L.t_of(lo + jj)];
+          qw.next();
+        }
+    }
   };
   auto body = [&](uint32_t m, const BetaRe<R>& xc) {
     const uint32_t lo = L.g + 8u * m, n = min(lo + 8u, top) - lo;
-
+    const bool own = lo < own_end;
+    const uint32_t nown = own ? min(8u, own_end - lo) : 0u;
     In8<R> in;
     load_in8(L, Y, (int)lo, (int)n, in);
-    R gv[8][8];
-#pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj)
-      if (jj < n) {
-        const uint32_t j = lo + jj;
-        if (j < own_end) {
-          load8(L, Y.gl, Y.S, j - L.g, gv[jj]);
-        } else {
-#pragma unroll
-          for (int s = 0; s < 8; ++s) gv[jj][s] = R(0);
-        }
-      }
     R breg[8][8];
     {
       R b[8];
@@ -954,19 +984,35 @@ __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layo
         }
       }
     }
+    R gv[8][8];
+    if (own) {
+      R a[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) a[s] = xc.a0[s];
+#pragma unroll
+      for (uint32_t jj = 0; jj < 8; ++jj)
+        if (jj < nown) {
+          R o[8];
+          alpha_step(a, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, alpha_nrm(lo + jj, C.off));
+          R T[8];
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            a[s] = o[s];  // own steps lie inside the block
+            T[s] = o[s] + breg[jj][s];
+          }
+          llr_adjoint(T, xc.e[jj], gv[jj]);
+        }
+    }
 #pragma unroll
     for (uint32_t jj = 0; jj < 8; ++jj) {
       if (jj < n) {
         const uint32_t j = lo + jj;
-        R nb[8], bn[8], ds, dp;
+        R bn[8], ds, dp;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-          nb[s] = bb[s];
-          bn[s] = gv[jj][s];
-        }
-        beta_adjoint_step(breg[jj], kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], nb, bn, ds, dp);
+        for (int s = 0; s < 8; ++s) bn[s] = jj < nown ? gv[jj][s] : R(0);
+        beta_adjoint_step(breg[jj], kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], bb, bn, ds, dp);
         // past the block end the adjoint stops (beta is not stepped there)
-        select8(L.t_of(j) < (int)L.K, nb, bb);
+        select8(L.t_of(j) < (int)L.K, bn, bb);
         L.at(Y.dsb, j) = ds;
         L.at(Y.dpb, j) = dp;
       }
@@ -1479,29 +1525,14 @@ __global__ __launch_bounds__(128, kDiffWavesPerSimd<R>) void diff_backward_kerne
       // the forward's checkpoints of this half (phase 1 ran there)
       Lane<R> Lh = L;
       Lh.ck = A.ckpt + ((size_t)jb * H + h) * A.ckpt_job_half;
-      // phase 2: G(j, n) = e_j x (+-1 by the input bit of n) x the weight of
-      // state n in its class's LLR term (e_j: the adjoint of the extrinsic)
-      if (L.active) {
-        const GlobalPtr<const R> eg = global_view<R>(cb().ebar(A, h & 1u, K));
-        QppWalk qw(L.q * L.S + 8u * m0, L.K, L.f1, L.f2);
-        own_terms(Lh, Y, CH, m0, m1, [&](uint32_t j, const R (&T)[8], R) {
-          const R e = eg[lower ? qw.pi : (uint32_t)L.t_of(j)];
-          qw.next();
-          R m[2], s[2], p[8];
-          class_sums(T, m, s, p);
-          const R f1 = e * Num<R>::rcp(s[1]), f0 = -e * Num<R>::rcp(s[0]);
-#pragma unroll
-          for (unsigned n = 0; n < 8; ++n) L.at(Y.gl + n * Y.S, j - L.g) = (in_of(n) ? f1 : f0) * p[n];
-        });
-      }
-      __syncthreads();
-      // phase 3: the two adjoint sweeps
+      // phase 3: the two adjoint sweeps, each forming the LLR terms G of the
+      // own steps it needs
       if (L.active) {
         if (wave == 0) {
           alpha_adjoint_sweep(Lh, Y, CH, lower, cb().ebar(A, h & 1u, K));
         } else {
           R bb[8];
-          beta_adjoint_sweep(Lh, Y, CH, bb);
+          beta_adjoint_sweep(Lh, Y, CH, lower, cb().ebar(A, h & 1u, K), bb);
           // beta of the window top: the tail beta when it reaches K
           R dts[3] = {0, 0, 0}, dtp[3] = {0, 0, 0};
           if (L.q * L.S + L.S + L.g >= K) {
