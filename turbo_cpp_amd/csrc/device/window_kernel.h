@@ -711,6 +711,256 @@ __device__ __forceinline__ void siso_half(const Lane& L, X* __restrict__ ex, con
   extrinsic_pass<D, H, ES>(L, ex, sc, qtab, ck, ac, g > 0 ? S - g : kNever, nii, post_sc, es);
 }
 
+// ------------------------------------------------------------- lean path --
+// The geometry of the long blocks: one block per wave in M = 64 segments
+// (every K >= 2048) with S and the warm-up g multiples of kSubWin.  Every
+// chunk is then full and every hand-off falls on a chunk edge, so the passes
+// below run without per-step guards; a pass's chunks alternate between two
+// register sets (no copies at the loop edge); and the lower half's extrinsic
+// column (c_t + A_q + B_q t) mod 64 needs no conditional subtract.  The
+// arithmetic is siso_half's, step for step, so the results are bit-identical.
+
+// One segment's extrinsic addressing as this lane sees it (its own or a
+// neighbour's), in bytes of the LDS rows of X (XB bytes per entry): the upper
+// half reads column q, byte (t * 64 + q) XB; the lower half reads bit
+// pi(q S + t) at row r_t, column (c_t + A_q + B_q t) mod 64 (r_t, c_t from
+// the per-K table), byte ((y + XB c_t) & (64 XB - 1)) | r_t 64 XB with
+// y = XB (A_q + B_q t) kept unreduced: 2^32 is a multiple of 64 XB.
+struct LeanSeg {
+  uint32_t col;  // column of the segment in the staged planes
+  uint32_t qb;   // XB q
+  uint32_t y0;   // XB A_q (mod 2^32)
+  uint32_t dy;   // XB B_q (mod 2^32)
+};
+
+template <class X>
+__device__ __forceinline__ LeanSeg lean_seg(const Lane& L, uint32_t qq, uint32_t col) {
+  constexpr uint32_t XB = sizeof(X);
+  return {col, XB * qq, XB * (L.f1 * qq + L.f2 * qq * qq * L.S), XB * (2u * L.f2 * qq)};
+}
+
+// The extrinsics of chunk [lo, lo + kSubWin) of segment G, issued in the
+// order the pass consumes them (BACKWARD: last step first); ab (may be null)
+// keeps their byte addresses for the write-back.
+template <int H, bool BACKWARD, class X>
+__device__ __forceinline__ void lean_lds(const X* __restrict__ ex, const uint32_t* __restrict__ qtab, const Lane& L,
+                                         const LeanSeg& G, uint32_t lo, Chunk& c, uint32_t* ab) {
+  constexpr uint32_t XB = sizeof(X);
+  constexpr uint32_t kRowB = kExRow * XB;
+  const char* base = reinterpret_cast<const char*>(ex);
+  if constexpr (H == 0) {
+    const uint32_t a0 = lo * kRowB + G.qb;
+#pragma unroll
+    for (int jj = 0; jj < kSubWin; ++jj) {
+      const int j = BACKWARD ? kSubWin - 1 - jj : jj;
+      const uint32_t a = a0 + j * kRowB;
+      c.e[j] = static_cast<float>(*reinterpret_cast<const X*>(base + a));
+      if (ab) ab[j] = a;
+    }
+  } else {
+    // the chunk's table entries in one scalar batch (opaque per chunk, as in
+    // load_chunk_lds, so the addresses of every chunk are not hoisted)
+    uint32_t off = __builtin_amdgcn_readfirstlane(L.qtab_off + lo);
+    asm volatile("" : "+s"(off));
+    const uint32_t* qt = qtab + off;
+    uint32_t qv[kSubWin];
+#pragma unroll
+    for (int j = 0; j < kSubWin; ++j) qv[j] = qtab_at(qt, j);
+    uint32_t y = G.y0 + (lo + (BACKWARD ? kSubWin - 1 : 0)) * G.dy;
+#pragma unroll
+    for (int jj = 0; jj < kSubWin; ++jj) {
+      const int j = BACKWARD ? kSubWin - 1 - jj : jj;
+      const uint32_t a = ((y + XB * (qv[j] & 0xFFFFu)) & (kRowB - 1u)) | ((qv[j] >> 16) * kRowB);
+      c.e[j] = static_cast<float>(*reinterpret_cast<const X*>(base + a));
+      if (ab) ab[j] = a;
+      y = BACKWARD ? y - G.dy : y + G.dy;
+    }
+  }
+}
+
+template <int H, bool BACKWARD, class X>
+__device__ __forceinline__ void lean_load(const X* __restrict__ ex, const float* __restrict__ sc,
+                                          const uint32_t* __restrict__ qtab, const Lane& L, const LeanSeg& G,
+                                          uint32_t c, Chunk& x) {
+  load_chunk<true>(sc, G.col, c * kSubWin, kSubWin, x);
+  lean_lds<H, BACKWARD>(ex, qtab, L, G, c * kSubWin, x, nullptr);
+}
+
+// beta down chunks n-1 .. 0 of segment G from bc; on_chunk(c, beta) before
+// chunk c, on_hand(beta) after chunk hand_c (beta then is beta at its first step)
+template <class D, int H, class X, class OnChunk, class OnHand>
+__device__ __forceinline__ void lean_beta(const Lane& L, const X* __restrict__ ex, const float* __restrict__ sc,
+                                          const uint32_t* __restrict__ qtab, const LeanSeg& G, uint32_t n, float* bc,
+                                          uint32_t hand_c, OnChunk&& on_chunk, OnHand&& on_hand) {
+  auto run = [&](uint32_t c, const Chunk& x) {
+    if constexpr (D::kShareSimd) share_simd();
+    on_chunk(c, bc);
+#pragma unroll
+    for (int j = kSubWin - 1; j >= 0; --j) {
+      __builtin_amdgcn_sched_barrier(0);
+      D::bstep(bc, D::branch(x.v[j].x + x.e[j], x.v[j].y), j == 0);
+    }
+    if (c == hand_c) on_hand(bc);
+  };
+  Chunk P, Q;
+  lean_load<H, true>(ex, sc, qtab, L, G, n - 1, P);
+  for (uint32_t c = n - 1;; c -= 2) {
+    if (c >= 1) lean_load<H, true>(ex, sc, qtab, L, G, c - 1, Q);
+    run(c, P);
+    if (c == 0) break;
+    if (c >= 2) lean_load<H, true>(ex, sc, qtab, L, G, c - 2, P);
+    run(c - 1, Q);
+    if (c == 1) break;
+  }
+}
+
+// alpha up chunks c0 .. c0+n-1 of segment G (the warm-up: no extrinsics)
+template <class D, int H, class X>
+__device__ __forceinline__ void lean_alpha(const Lane& L, const X* __restrict__ ex, const float* __restrict__ sc,
+                                           const uint32_t* __restrict__ qtab, const LeanSeg& G, uint32_t c0,
+                                           uint32_t n, float* ac) {
+  auto run = [&](const Chunk& x) {
+#pragma unroll
+    for (int j = 0; j < kSubWin; ++j) {
+      __builtin_amdgcn_sched_barrier(0);
+      const float sa = x.v[j].x + x.e[j];
+      D::template astep<false>(ac, D::branch(sa, x.v[j].y), sa, 0.0f, 0.0f, nullptr, j == kSubWin - 1);
+    }
+  };
+  Chunk P, Q;
+  lean_load<H, false>(ex, sc, qtab, L, G, c0, P);
+  for (uint32_t c = 0;; c += 2) {
+    if (c + 1 < n) lean_load<H, false>(ex, sc, qtab, L, G, c0 + c + 1, Q);
+    run(P);
+    if (c + 1 >= n) break;
+    if (c + 2 < n) lean_load<H, false>(ex, sc, qtab, L, G, c0 + c + 2, P);
+    run(Q);
+    if (c + 2 >= n) break;
+  }
+}
+
+// Sub-window of the forward pass (alpha_subwindow's full-chunk path with byte
+// addresses ab): recompute beta from the checkpoint bb, then alpha +
+// extrinsic per step; the extrinsic to LDS, in the lower half also the
+// posterior to post_sc (float index = byte address / XB).
+template <class D, int H, class X>
+__device__ __forceinline__ void lean_subwindow(const Lane& L, X* __restrict__ ex, const float2* v, const float* e,
+                                               const uint32_t* ab, float* bb, float* ac, bool hand,
+                                               float* __restrict__ nii, uint32_t dummy_b, bool wr,
+                                               float* __restrict__ post_sc) {
+  constexpr uint32_t XB = sizeof(X);
+  float bs[kSubWin][8];
+  typename D::Br br[kSubWin];
+  float isa[kSubWin];
+#pragma unroll
+  for (int j = kSubWin - 1; j >= 0; --j) {
+    isa[j] = v[j].x + e[j];
+    br[j] = D::branch(isa[j], v[j].y);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) bs[j][i] = bb[i];
+    __builtin_amdgcn_sched_barrier(0);
+    if (j > 0) D::bstep(bb, br[j], false);
+  }
+  if (hand) {
+    // uniform: hand alpha at step S-g to the next segment
+    float na[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) na[i] = dpp_from_prev_lane(ac[i]);
+    if (!L.first) store8(nii, na);
+  }
+  char* exb = reinterpret_cast<char*>(ex);
+  char* pb = reinterpret_cast<char*>(post_sc);
+#pragma unroll
+  for (int j = 0; j < kSubWin; ++j) {
+    __builtin_amdgcn_sched_barrier(0);
+    const float ext = D::template astep<true>(ac, br[j], isa[j], v[j].x, e[j], bs[j], j == kSubWin - 1);
+    const uint32_t a = wr ? ab[j] : dummy_b;
+    *reinterpret_cast<X*>(exb + a) = to_ex<X>(ext);
+    if constexpr (H == 1) *reinterpret_cast<float*>(pb + a * (4 / XB)) = isa[j] + ext;
+  }
+}
+
+// Forward pass of the lane's own segment G: per sub-window, the staged pair
+// and the beta checkpoint one sub-window ahead (alternating register sets),
+// the extrinsics at its start; hand_c: the sub-window whose first alpha goes
+// to the next segment's record.
+template <class D, int H, bool ES, class X>
+__device__ __forceinline__ void lean_extrinsic(const Lane& L, X* __restrict__ ex, const float* __restrict__ sc,
+                                               const uint32_t* __restrict__ qtab, const LeanSeg& G,
+                                               const float* __restrict__ ck, float* ac, uint32_t hand_c,
+                                               float* __restrict__ nii, float* __restrict__ post_sc,
+                                               const EarlyStop& es) {
+  const uint32_t n = L.nW;
+  const uint32_t dummy_b = (L.S * kExRow + threadIdx.x) * sizeof(X);
+  const bool wr = L.active && !es.done;
+  struct In {
+    float2 v[kSubWin];
+    float k[8];
+  };
+  auto load = [&](uint32_t w, In& in) {
+    const GlobalFloat* p = (const GlobalFloat*)(sc + w * kSubWin * kWaveLanes + G.col);
+    asm volatile("" : "+v"(p));
+#pragma unroll
+    for (int j = 0; j < kSubWin; ++j) in.v[j] = make_float2(p[j * kWaveLanes], p[kPlane + j * kWaveLanes]);
+    load8(ck + 8 * w, in.k);
+  };
+  auto run = [&](uint32_t w, In& in) {
+    if constexpr (D::kShareSimd) share_simd();
+    Chunk x;
+    uint32_t ab[kSubWin];
+    lean_lds<H, true>(ex, qtab, L, G, w * kSubWin, x, ab);
+    lean_subwindow<D, H>(L, ex, in.v, x.e, ab, in.k, ac, w == hand_c, nii, dummy_b, wr, post_sc);
+  };
+  In P, Q;
+  load(0, P);
+  for (uint32_t w = 0;; w += 2) {
+    if (w + 1 < n) load(w + 1, Q);
+    run(w, P);
+    if (w + 1 >= n) break;
+    if (w + 2 < n) load(w + 2, P);
+    run(w + 1, Q);
+    if (w + 2 >= n) break;
+  }
+}
+
+// siso_half on the lean geometry (same records, same order of operations).
+template <class D, int H, bool ES, class X>
+__device__ __forceinline__ void siso_half_lean(const Lane& L, X* __restrict__ ex, const float* __restrict__ sc,
+                                               const uint32_t* __restrict__ qtab, float* __restrict__ nii,
+                                               float* __restrict__ ck, float* __restrict__ post_sc,
+                                               const EarlyStop& es) {
+  const uint32_t nW = L.nW, ng = L.g / kSubWin;
+  constexpr uint32_t kNever = 0xFFFFFFFFu;
+  const LeanSeg own = lean_seg<X>(L, L.q, L.le);
+  // backward pass: warm-up over the next segment's first g steps, then the
+  // segment, beta checkpoints at chunk starts, beta at step g handed back
+  float bc[8];
+  load8(nii + 8, bc);
+  {
+    const LeanSeg nx = L.last ? own : lean_seg<X>(L, L.q + 1, L.le + 1);
+    lean_beta<D, H>(L, ex, sc, qtab, nx, ng, bc, kNever, [](uint32_t, float*) {}, [](const float*) {});
+    if (L.last) load8(nii + 8, bc);
+  }
+  lean_beta<D, H>(
+      L, ex, sc, qtab, own, nW, bc, ng, [&](uint32_t w, float* b) { store8(ck + 8 * w, b); },
+      [&](const float* b) {
+        float nb[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) nb[i] = dpp_from_next_lane(b[i]);
+        if (!L.last) store8(nii + 8, nb);
+      });
+  // forward pass: warm-up over the previous segment's last g steps, then per
+  // sub-window beta recomputed, alpha + extrinsic
+  float ac[8];
+  load8(nii, ac);
+  {
+    const LeanSeg pv = L.first ? own : lean_seg<X>(L, L.q - 1, L.le - 1);
+    lean_alpha<D, H>(L, ex, sc, qtab, pv, nW - ng, ng, ac);
+    if (L.first) load8(nii, ac);
+  }
+  lean_extrinsic<D, H, ES>(L, ex, sc, qtab, own, ck, ac, nW - ng, nii, post_sc, es);
+}
+
 // ---------------------------------------------- exact segment boundaries --
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 
@@ -1364,11 +1614,19 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
 
   EarlyStop es{es_on, false};
   uint32_t used_it = A.max_iterations;
+  // wave-uniform: the lean geometry (one block in 64 segments, chunk-aligned
+  // S and warm-up)
+  const bool lean = !D::kExact && J.nblk == 1 && M == kWaveLanes && S % kSubWin == 0 && L.g > 0 &&
+                    L.g % kSubWin == 0;
   for (uint32_t it = 0; it < A.max_iterations; ++it) {
     if constexpr (D::kExact) {
       siso_half_exact<0, ES>(L, ex, sc_up, qtab, nii, ck, ack, post_sc, es);
     } else {
-      siso_half<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
+      if (lean) {
+        siso_half_lean<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
+      } else {
+        siso_half<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
+      }
     }
     __syncthreads();
     // every lower half leaves the posteriors in post_sc; a block that has
@@ -1376,7 +1634,11 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
     if constexpr (D::kExact) {
       siso_half_exact<1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, ack, post_sc, es);
     } else {
-      siso_half<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
+      if (lean) {
+        siso_half_lean<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
+      } else {
+        siso_half<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
+      }
     }
     __syncthreads();
     if (es_on) {
