@@ -26,6 +26,10 @@ __global__ void window_decode_log_kernel(WinArgs);
 __global__ void window_decode_log_es_kernel(WinArgs);
 __global__ void window_decode_log_f32_kernel(WinArgs);
 __global__ void window_decode_log_es_f32_kernel(WinArgs);
+__global__ void window_decode_log_lean_kernel(WinArgs);
+__global__ void window_decode_log_lean_es_kernel(WinArgs);
+__global__ void window_decode_log_lean_f32_kernel(WinArgs);
+__global__ void window_decode_log_lean_es_f32_kernel(WinArgs);
 __global__ void window_decode_lin_kernel(WinArgs);
 __global__ void window_decode_lin_es_kernel(WinArgs);
 __global__ void window_decode_lin_f32_kernel(WinArgs);
@@ -78,9 +82,23 @@ tc_segment_info geometry(uint32_t k, const tc_options& o) {
   return g;
 }
 
+// The lean geometry of the window decoders (window_kernel.h): one code block
+// per job in 64 segments, S and the warm-up multiples of 8 (the K >= 2048
+// that are multiples of 512, at warm-ups 8, 16, 24, ...).
+bool lean_job(const tcdev::WinJob& j) {
+  return j.nblk == 1 && j.nseg == static_cast<uint32_t>(tcdev::kWaveLanes) && j.seg % tcdev::kSubWin == 0 &&
+         j.warm > 0 && j.warm % tcdev::kSubWin == 0;
+}
+
 using WinKernel = void (*)(tcdev::WinArgs);
-WinKernel window_kernel(const tc_options& o) {
+// lean: every job of the plan has the lean geometry; the log-domain decoder
+// then runs its lean kernels (the same results, bit for bit)
+WinKernel window_kernel(const tc_options& o, bool lean = false) {
   const bool es = o.early_stop != 0, f32 = o.extrinsic_bits == 32;
+  if (lean && o.decoder == TC_DECODER_WINDOW_LOG) {
+    if (f32) return es ? tcdev::window_decode_log_lean_es_f32_kernel : tcdev::window_decode_log_lean_f32_kernel;
+    return es ? tcdev::window_decode_log_lean_es_kernel : tcdev::window_decode_log_lean_kernel;
+  }
   switch (o.decoder) {
     case TC_DECODER_EXACT:
       return es ? tcdev::window_decode_exact_es_kernel : tcdev::window_decode_exact_kernel;
@@ -119,6 +137,7 @@ struct PlanDev {
   uint32_t* d_sched = nullptr;
   uint32_t njobs = 0, grid = 0;
   size_t lds_bytes = 0;
+  bool lean = false;  // every job has the lean geometry (window_kernel)
 };
 
 struct tc_plan : PlanDev {
@@ -243,12 +262,15 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   });
   p.njobs = static_cast<uint32_t>(jobs.size());
   p.lds_bytes = 0;
-  for (const auto& j : jobs)
+  p.lean = !jobs.empty();
+  for (const auto& j : jobs) {
     p.lds_bytes = std::max<size_t>(p.lds_bytes, lds_bytes_of(j.seg, o));
+    p.lean = p.lean && lean_job(j);
+  }
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
   int per_cu = 0;
-  const void* fn = reinterpret_cast<const void*>(window_kernel(o));
+  const void* fn = reinterpret_cast<const void*>(window_kernel(o, p.lean));
   hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, tcdev::kWaveLanes, p.lds_bytes),
             "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   per_cu = std::max(per_cu, 1);
@@ -285,7 +307,7 @@ void launch(const PlanDev& p, const float* llr, float* post, int32_t* crc, uint3
     a.tb = tcdev::TbOut{};
     if (tb) a.tb = *tb;
     a.sched = p.d_sched;
-    hipLaunchKernelGGL(window_kernel(o), dim3(p.grid), dim3(64), p.lds_bytes, s, a);
+    hipLaunchKernelGGL(window_kernel(o, p.lean), dim3(p.grid), dim3(64), p.lds_bytes, s, a);
   }
   hip_check(hipGetLastError(), "kernel launch");
 }
@@ -367,6 +389,7 @@ void tchost::decode_one_block(uint8_t* area, size_t area_bytes, const uint32_t* 
     p.njobs = 1;
     p.grid = 1;
     p.lds_bytes = lds_bytes_of(g.segment_length, o);
+    p.lean = lean_job(h.job);
   }
   if (kOneBlockWorkOff + work_bytes > area_bytes)
     throw Failure{TC_ERR_INVALID_ARGUMENT, "code block needs " + std::to_string(kOneBlockWorkOff + work_bytes) +

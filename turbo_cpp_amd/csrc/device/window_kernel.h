@@ -713,7 +713,8 @@ __device__ __forceinline__ void siso_half(const Lane& L, X* __restrict__ ex, con
 
 // ------------------------------------------------------------- lean path --
 // The geometry of the long blocks: one block per wave in M = 64 segments
-// (every K >= 2048) with S and the warm-up g multiples of kSubWin.  Every
+// (every K >= 2048; S and the warm-up g multiples of kSubWin at K a multiple
+// of 512 and warm-ups 8, 16, 24, ...).  Every
 // chunk is then full and every hand-off falls on a chunk edge, so the passes
 // below run without per-step guards; a pass's chunks alternate between two
 // register sets (no copies at the loop edge); and the lower half's extrinsic
@@ -1476,7 +1477,10 @@ __device__ __forceinline__ void copy_out_tiled(const Lane& L, bool any, const fl
 }
 
 // One job: stage its inputs, run every iteration, write its outputs.
-template <class D, bool ES, class X>
+// LEAN: every job of the launch has the lean geometry (siso_half_lean); the
+// host picks such a kernel only for plans whose jobs all have it, so that
+// neither path's registers constrain the other's.
+template <class D, bool ES, class X, bool LEAN = false>
 __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X* __restrict__ ex,
                                            float* __restrict__ sc_up, float* __restrict__ sc_lo,
                                            float* __restrict__ nii, float* __restrict__ ck,
@@ -1614,31 +1618,23 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
 
   EarlyStop es{es_on, false};
   uint32_t used_it = A.max_iterations;
-  // wave-uniform: the lean geometry (one block in 64 segments, chunk-aligned
-  // S and warm-up)
-  const bool lean = !D::kExact && J.nblk == 1 && M == kWaveLanes && S % kSubWin == 0 && L.g > 0 &&
-                    L.g % kSubWin == 0;
   for (uint32_t it = 0; it < A.max_iterations; ++it) {
     if constexpr (D::kExact) {
       siso_half_exact<0, ES>(L, ex, sc_up, qtab, nii, ck, ack, post_sc, es);
+    } else if constexpr (LEAN) {
+      siso_half_lean<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
     } else {
-      if (lean) {
-        siso_half_lean<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
-      } else {
-        siso_half<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
-      }
+      siso_half<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
     }
     __syncthreads();
     // every lower half leaves the posteriors in post_sc; a block that has
     // stopped no longer overwrites its own
     if constexpr (D::kExact) {
       siso_half_exact<1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, ack, post_sc, es);
+    } else if constexpr (LEAN) {
+      siso_half_lean<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
     } else {
-      if (lean) {
-        siso_half_lean<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
-      } else {
-        siso_half<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
-      }
+      siso_half<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
     }
     __syncthreads();
     if (es_on) {
@@ -1693,7 +1689,7 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
   __syncthreads();
 }
 
-template <class D, bool ES, class X>
+template <class D, bool ES, class X, bool LEAN = false>
 __device__ __forceinline__ void window_decode(const WinArgs& A) {
   extern __shared__ float lds[];
   const uint32_t lane = threadIdx.x;
@@ -1710,7 +1706,7 @@ __device__ __forceinline__ void window_decode(const WinArgs& A) {
   float* ack = post_sc + (kMaxSeg + 1) * kWaveLanes + lane * (8 * kMaxSubWins);
   for (uint32_t jb = next_job(A.sched); jb < A.njobs; jb = next_job(A.sched)) {
     const WinJob J = A.jobs[jb];
-    window_job<D, ES, X>(A, J, reinterpret_cast<X*>(lds), sc_up, sc_lo, nii, ck, post_sc, ack);
+    window_job<D, ES, X, LEAN>(A, J, reinterpret_cast<X*>(lds), sc_up, sc_lo, nii, ck, post_sc, ack);
   }
   leave_queue(A.sched);
 }
