@@ -42,17 +42,48 @@ N_CB = 4096
 ITERS = 8
 EBN0 = [0.25 * i for i in range(7)]
 # Algorithmic work per info bit per iteration in the reference's pairwise
-# Log-MAP form: 384 simple ops + 120 transcendental ops, each transcendental
-# priced at the two issue cycles v_exp_f32 / v_log_f32 take on gfx950
-# (SQ_ACTIVE_INST_VALU - SQ_INSTS_VALU of the counter runs) = 624 VALU
-# lane-slots; HBM bytes per block: 12(K+4) in + 4K out.
-VALU_SLOTS_PER_BIT_ITER = 384 + 120 * 2
-VALU_PEAK = 256 * 64 * 2.4e9  # lane-slots/s: 256 CUs x 64 lanes x 2.4 GHz, unpacked FP32
+# Log-MAP form: 384 simple ops + 120 transcendental ops.  A lane-slot is the
+# SIMD time of one lane of a simple FP32 op: MI355X's published FP32 vector
+# peak (157.3 TFLOP/s, i.e. 78.6 T FMA lanes/s = 256 CUs x 4 SIMDs x 32 lanes
+# x 2.4 GHz: a wave64 op every 2 cycles); transcendentals (v_exp_f32,
+# v_log_f32) run at quarter rate, 8 SIMD cycles per wave64 op (the counter
+# calibration: SQ_ACTIVE_INST_VALU charges them twice a simple op and a
+# kernel of nothing but v_exp_f32 keeps its SIMDs 96% busy at that price), so
+# 4 lane-slots each: 384 + 4 x 120 = 864 lane-slots per bit and iteration.
+# HBM bytes per block: 12(K+4) in + 4K out.
+VALU_SLOTS_PER_BIT_ITER = 384 + 120 * 4
+VALU_PEAK = 256 * 4 * 32 * 2.4e9  # lane-slots/s
 HBM_PEAK_GBS = 8000.0
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def valu_calibration():
+    """SIMD cycles one wave64 v_add_f32 / v_exp_f32 occupies, measured with
+    independent operations at 1, 2, 4 and 8 one-wave work-groups per SIMD
+    (turbo_cpp_amd/csrc/calib/valu_rate.hip)."""
+    import ctypes as C
+    path = os.path.join(ROOT, "turbo_cpp_amd", "libturbo_calib.so")
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    lib.tc_calib_valu.argtypes = [C.c_int, C.c_int, C.c_int] + [C.POINTER(C.c_double)] * 4
+    rows = []
+    for op, name in ((0, "v_add_f32"), (1, "v_exp_f32")):
+        for per_simd in (1, 2, 4, 8):
+            v = [C.c_double() for _ in range(4)]
+            rc = lib.tc_calib_valu(op, 1024 * per_simd, 4000, *[C.byref(x) for x in v])
+            if rc:
+                return {"error": f"tc_calib_valu returned {rc}"}
+            rows.append({"op": name, "workgroups_per_simd": per_simd, "clock_mhz": round(v[0].value),
+                         "resident_waves_per_simd": round(v[1].value, 2),
+                         "wave_cycles_per_instr": round(v[2].value, 2),
+                         "simd_cycles_per_instr": round(v[3].value, 2)})
+    return {"what": "16 independent ops per round, 4000 rounds per wave; SIMD cycles per wave64 instruction = "
+                    "a wave's cycles per instruction / resident waves per SIMD",
+            "rows": rows}
 
 
 def make_inputs(n, seed=12345):
@@ -521,13 +552,13 @@ def diff_leg(T, torch, train_steps):
     ber_exact = (float((hard != y_te).float().mean()), float((hard != y_te).any(dim=1).float().mean()))
     log(f"diff: fwd {fwd_ms:.2f} ms bwd {bwd_ms:.2f} ms; train {train_steps} steps {train_s:.1f} s; "
         f"BER exact Log-MAP {ber_exact[0]:.2e} unit scales {ber_plain[0]:.2e} weighted {ber_w[0]:.2e}")
-    # VALU busy, memory-wait share and HBM-side traffic of the two kernels at this workload, from
+    # SIMD VALU time, memory-wait share and HBM-side traffic of the two kernels at this workload, from
     # their rocprofv3 counter runs (copy-calibrated), recorded in profiles/
     prof = os.path.join(ROOT, "profiles", "diff_decoder_k1024_x2048.md")
     counters = {}
     if os.path.exists(prof):
-        keys = ("valu_busy_fwd", "valu_busy_bwd", "wait_frac_fwd", "wait_frac_bwd", "traffic_bytes_fwd",
-                "traffic_bytes_bwd")
+        keys = ("simd_valu_time_frac_fwd", "simd_valu_time_frac_bwd", "wait_frac_fwd", "wait_frac_bwd",
+                "traffic_bytes_fwd", "traffic_bytes_bwd")
         for line in open(prof):
             for key in keys:
                 if line.startswith(key + ":"):
@@ -708,7 +739,7 @@ def main():
         for line in open(prof):
             if line.startswith("traffic_bytes_per_launch_log:"):
                 traffic = float(line.split(":")[1])
-            for key in ("valu_busy_log", "sq_insts_valu_log"):
+            for key in ("simd_valu_time_frac_log", "sq_insts_valu_log"):
                 if line.startswith(key + ":"):
                     counters[key[:-4]] = float(line.split(":")[1])
     bits_per_launch = N_CB * K
@@ -718,17 +749,23 @@ def main():
                 "frac": achieved / VALU_PEAK, "traffic": traffic,
                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
                                 "profiles/window_decode_counters_k6144_x4096.md)",
-                "note": "VALU lane-slots in the reference's pairwise Log-MAP form (624/bit/iteration: 384 "
-                        "simple + 120 transcendental at 2 issue cycles) per second; no MFMA work exists in "
-                        "this path",
+                "note": "VALU lane-slots of the reference's pairwise Log-MAP form per second (864 per bit and "
+                        "iteration: 384 simple FP32 ops + 120 transcendentals at quarter rate, 4 slots each); "
+                        "peak = MI355X's FP32 vector peak, 256 CUs x 4 SIMDs x 32 lanes x 2.4 GHz (157.3 "
+                        "TFLOP/s as FMAs); no MFMA work exists in this path",
                 "hbm": {"achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": hbm_achieved / HBM_PEAK_GBS}}
     if counters:
-        # the reference-form fraction can exceed 1 (this kernel needs fewer lane-slots per bit than
-        # the pairwise form); the measured VALU busy fraction and instruction count are bounded
-        roofline["valu_busy"] = counters.get("valu_busy")
+        # the kernel's own VALU issue from its counter run: simple instructions at 2 SIMD cycles,
+        # transcendentals (SQ_ACTIVE_INST_VALU - SQ_INSTS_VALU) at 8, over the SIMDs' cycles
+        roofline["simd_valu_time_frac"] = counters.get("simd_valu_time_frac")
         roofline["sq_insts_valu"] = counters.get("sq_insts_valu")
         roofline["counters_from"] = "profiles/window_decode_counters_k6144_x4096.md"
+    calib = valu_calibration()
+    if calib and "rows" in calib:
+        log("valu calibration:", [(r["op"], r["resident_waves_per_simd"], r["simd_cycles_per_instr"])
+                                  for r in calib["rows"]])
+    roofline["valu_calibration"] = calib
     e2e = None if args.no_e2e else e2e_leg(T, llr, tuned_opts)
     ref_cases = None if args.no_reference_cases else reference_bench_leg(T)
     cpu = None if args.no_cpu_baseline else cpu_baseline(llr)

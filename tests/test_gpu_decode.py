@@ -465,6 +465,32 @@ def test_window_early_stop_matches_oracle(decoder):
 
 
 @pytest.mark.parametrize("early_stop", [False, True])
+@pytest.mark.parametrize("xbits", [16, 32])
+@pytest.mark.parametrize("warm", [8, 16, 24])
+def test_window_lean_kernel_equals_general(warm, xbits, early_stop):
+    """A plan whose every job has the lean geometry (one block in 64
+    segments, segment length and warm-up multiples of 8: K >= 2048 a
+    multiple of 512) runs the lean log-domain kernel; one K=40 block more
+    makes the same blocks run the general kernel.  Posteriors, CRC verdicts
+    and stopping iterations are the same bits (fillers, blocks that stop at
+    different iterations and ones that never pass included)."""
+    ks = [2048, 6144, 3072, 2560, 6144, 4096, 5632, 6144, 2048, 3584]
+    fs = [8, 0, 0, 24, 0, 0, 0, 16, 0, 0]
+    eb = [1.5, 0.4, -1.0, 1.0, 0.7, 2.0, 0.9, 0.5, -0.5, 1.2]
+    _, llrs = _crc_blocks(ks, fs, eb, 60 + warm)
+    _, short = _crc_blocks([40], [0], [2.0], 61)
+    lean = T.decode_batch_host(ks, np.concatenate(llrs), T.DECODER_WINDOW_LOG, 8, early_stop, fs, 0, warm, xbits)
+    gen = T.decode_batch_host(ks + [40], np.concatenate(llrs + short), T.DECODER_WINDOW_LOG, 8, early_stop,
+                              fs + [0], 0, warm, xbits)
+    n = sum(ks)
+    _say(f"warm-up {warm}, fp{xbits} held, early stop {early_stop}: iterations {list(lean[2])}")
+    assert np.array_equal(lean[0].view(np.uint32), gen[0][:n].view(np.uint32))
+    assert list(lean[1]) == list(gen[1][:-1]) and list(lean[2]) == list(gen[2][:-1])
+    if early_stop:
+        assert lean[1].sum() >= 5 and not lean[1][2] and min(lean[2]) < 8
+
+
+@pytest.mark.parametrize("early_stop", [False, True])
 @pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])
 @pytest.mark.parametrize("k", [96, 352, 544, 768])
 def test_window_spare_lane_jobs_match_oracle(k, decoder, early_stop):

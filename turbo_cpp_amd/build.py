@@ -15,6 +15,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libturbo_cpp_amd.so")
+# VALU issue calibration for bench.py's roofline (not part of the decoder library)
+CALIB_SRC = os.path.join(CSRC, "calib", "valu_rate.hip")
+CALIB_LIB = os.path.join(HERE, "libturbo_calib.so")
 API_TEST = os.path.join(OUT_DIR, "api_test")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
@@ -62,6 +65,11 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE_FLAGS + ["-shared", CALIB_SRC, "-o", CALIB_LIB + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"calibration build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(CALIB_LIB + ".tmp", CALIB_LIB)
     # C++ API test program (tests/cpp/api_test.cpp) linked against the library
     test_src = os.path.join(ROOT, "tests", "cpp", "api_test.cpp")
     if os.path.exists(test_src):
