@@ -619,12 +619,24 @@ __device__ inline R jac_w0(R a, R b) {
   return a >= b ? big : e * big;
 }
 
+// The same weight from the Jacobian logarithm of the pair already at hand
+// (j = jac(a, b), e.g. a recomputed metric before its renormalisation):
+// 2^(a - j), one exp2 instead of an exp2 and a reciprocal; at most 1.
+template <typename R>
+__device__ inline R jac_wj(R a, R j) {
+  return fmin(Num<R>::ex2(a - j), R(1));
+}
+
 // One step of the alpha adjoint at local j (alpha_j = a, half gammas hs, hp
 // in base 2): ab holds the adjoint of alpha_{j+1} on entry (the LLR's share
 // included) and of alpha_j on exit; ds, dp: the step's adjoints of sys + la
-// and par through its transitions (gamma = +-(sys + la)/2 +- par/2).
-template <typename R>
-__device__ inline void alpha_adjoint_step(const R (&a)[8], R hs, R hp, R (&ab)[8], R& ds_out, R& dp_out) {
+// and par through its transitions (gamma = +-(sys + la)/2 +- par/2).  With
+// NEXT, an holds alpha_{j+1} before any renormalisation (each state's jac of
+// its two candidates) and the weights come from it (jac_wj); else from the
+// two candidates (jac_w0).
+template <typename R, bool NEXT = false>
+__device__ inline void alpha_adjoint_step(const R (&a)[8], const R (&an)[8], R hs, R hp, R (&ab)[8], R& ds_out,
+                                          R& dp_out) {
   R abn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   R ds = 0, dp = 0;
   bool seen[8] = {false, false, false, false, false, false, false, false};
@@ -640,7 +652,8 @@ __device__ inline void alpha_adjoint_step(const R (&a)[8], R hs, R hp, R (&ab)[8
         seen[n] = true;
       } else {
         const unsigned s0 = fs[n], u0 = fu[n];
-        const R w0 = jac_w0(a[s0] + gam(u0, par(s0, u0), hs, hp), a[s] + gam(u, par(s, u), hs, hp));
+        const R c0 = a[s0] + gam(u0, par(s0, u0), hs, hp);
+        const R w0 = NEXT ? jac_wj(c0, an[n]) : jac_w0(c0, a[s] + gam(u, par(s, u), hs, hp));
         const R d0 = ab[n] * w0, d1 = ab[n] - d0;  // the two weights add to 1
         abn[s0] += d0;
         abn[s] += d1;
@@ -658,14 +671,16 @@ __device__ inline void alpha_adjoint_step(const R (&a)[8], R hs, R hp, R (&ab)[8
 
 // One step of the beta adjoint at local j (beta_{j+1} = b): bb is the
 // adjoint of beta_j; bn holds the output adjoints of beta_{j+1} (the LLR's
-// share) on entry and the whole adjoint of beta_{j+1} on exit.
-template <typename R>
-__device__ inline void beta_adjoint_step(const R (&b)[8], R hs, R hp, const R (&bb)[8], R (&bn)[8], R& ds_out,
-                                         R& dp_out) {
+// share) on entry and the whole adjoint of beta_{j+1} on exit.  With NEXT,
+// bj holds beta_j before any renormalisation and the weights come from it.
+template <typename R, bool NEXT = false>
+__device__ inline void beta_adjoint_step(const R (&b)[8], const R (&bj)[8], R hs, R hp, const R (&bb)[8],
+                                         R (&bn)[8], R& ds_out, R& dp_out) {
   R ds = 0, dp = 0;
 #pragma unroll
   for (unsigned s = 0; s < 8; ++s) {
-    const R w0 = jac_w0(b[nxt(s, 0)] + gam(0u, par(s, 0), hs, hp), b[nxt(s, 1)] + gam(1u, par(s, 1), hs, hp));
+    const R c0 = b[nxt(s, 0)] + gam(0u, par(s, 0), hs, hp);
+    const R w0 = NEXT ? jac_wj(c0, bj[s]) : jac_w0(c0, b[nxt(s, 1)] + gam(1u, par(s, 1), hs, hp));
     const R d0g = bb[s] * w0, d1g = bb[s] - d0g;
     bn[nxt(s, 0)] += d0g;
     bn[nxt(s, 1)] += d1g;
@@ -863,7 +878,10 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
     const uint32_t nin = own ? min(lo + 8u, top) - lo : n;
     In8<R> in;
     load_in8(L, Y, (int)lo, (int)nin, in);
-    R areg[8][8], anext[8];
+    // areg[jj] = alpha at local lo + jj; within a chunk only its last step is
+    // renormalised, so areg[jj + 1] is also alpha_{lo+jj+1} before
+    // renormalisation, and anext + amx that of the last step in an own chunk
+    R areg[8][8], anext[8], amx = R(0);
     {
       R a[8];
 #pragma unroll
@@ -875,7 +893,15 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
           for (int s = 0; s < 8; ++s) areg[jj][s] = a[s];
           if (jj + 1 < n || own) {
             R o[8];
-            alpha_step(a, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, alpha_nrm(lo + jj, C.off));
+            alpha_step(a, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, false);
+            if (alpha_nrm(lo + jj, C.off)) {  // normalise(), keeping the maximum it subtracts
+              R m = o[0];
+#pragma unroll
+              for (int s = 1; s < 8; ++s) m = fmax(o[s], m);
+#pragma unroll
+              for (int s = 0; s < 8; ++s) o[s] -= m;
+              amx = m;
+            }
             select8(L.t_of(lo + jj) >= 0, o, a);
           }
         }
@@ -915,7 +941,19 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
           for (int s = 0; s < 8; ++s) ab[s] += gv[jj][s];
         }
         R ds, dp;
-        alpha_adjoint_step(areg[jj], kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], ab, ds, dp);
+        // weights from alpha_{j+1} before renormalisation (steps whose alpha
+        // was not stepped, t < 0, write only values no later pass reads)
+        if ((uint32_t)jj + 1 < n) {
+          alpha_adjoint_step<R, true>(areg[jj], areg[jj < 7 ? jj + 1 : 7], kHalfL2e<R> * in.sa[jj],
+                                      kHalfL2e<R> * in.pr[jj], ab, ds, dp);
+        } else if (own) {
+          R an[8];
+#pragma unroll
+          for (int s = 0; s < 8; ++s) an[s] = anext[s] + amx;
+          alpha_adjoint_step<R, true>(areg[jj], an, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], ab, ds, dp);
+        } else {
+          alpha_adjoint_step(areg[jj], areg[jj], kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], ab, ds, dp);
+        }
         L.at(Y.dsa, j) = ds - xc.e[jj];
         L.at(Y.dpr, j) = dp;
       }
@@ -1010,7 +1048,16 @@ __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layo
         R bn[8], ds, dp;
 #pragma unroll
         for (int s = 0; s < 8; ++s) bn[s] = jj < nown ? gv[jj][s] : R(0);
-        beta_adjoint_step(breg[jj], kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], bb, bn, ds, dp);
+        // weights from beta_j = breg[jj - 1]: within a chunk only beta at its
+        // first step (jj = 0) is renormalised, and that one is not in breg
+        // (steps past the block end, whose beta was not stepped, are dropped
+        // by the select below)
+        if (jj > 0) {
+          beta_adjoint_step<R, true>(breg[jj], breg[jj > 0 ? jj - 1 : 0], kHalfL2e<R> * in.sa[jj],
+                                     kHalfL2e<R> * in.pr[jj], bb, bn, ds, dp);
+        } else {
+          beta_adjoint_step(breg[jj], breg[jj], kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], bb, bn, ds, dp);
+        }
         // past the block end the adjoint stops (beta is not stepped there)
         select8(L.t_of(j) < (int)L.K, bn, bb);
         L.at(Y.dsb, j) = ds;
