@@ -619,6 +619,67 @@ def test_tb_staging_modes_agree(decoder):
                 assert torch.equal(x, y)
 
 
+def _tbs_rx(sizes, seed):
+    """Transport blocks of the given payload sizes, full circular buffer at
+    rv 0, BPSK/AWGN LLRs of the rate-matched bits (noisy enough that some
+    blocks need several iterations and a few fail)."""
+    rng = np.random.default_rng(seed)
+    tbs, rx = [], []
+    for i, b in enumerate(sizes):
+        seg = T._segment_sizes(b + 24)
+        ks = [seg[4]] * seg[2] + [seg[3]] * seg[1]
+        g_total = sum(3 * (k + 4) for k in ks) - 2 * seg[5]
+        p = T.LteTurboCodecParams(transport_block_bits=b, g_total=g_total, rv=0)
+        enc = T.encode_transport_block(p, rng.integers(0, 2, b).astype(np.uint8))
+        sigma = (1.45, 1.15, 1.0)[i % 3]  # Eb/N0 about -1.5, 0.5 and 1.7 dB
+        for blk in enc.code_blocks:
+            y = 2.0 * blk.rate_matched_bits.astype(np.float64) - 1.0 + sigma * rng.standard_normal(len(blk.rate_matched_bits))
+            rx.append((2.0 * y / sigma ** 2).astype(np.float32))
+        tbs.append((b, g_total, 0, 1, 2))
+    return tbs, np.concatenate(rx)
+
+
+@pytest.mark.parametrize("early", [False, True])
+def test_lean_tb_plans_equal_general(early):
+    """Transport blocks whose code blocks all have the lean geometry (K a
+    multiple of 512 from 2048, warm-up 16: K = 6144, 4096, 3072, 5120 here,
+    one or two blocks per TB) run the lean log kernel, with de-rate matching
+    in its input staging or as separate kernels and the TB assembly in its
+    epilogue.  The same TBs plus one short TB run the general kernel.
+    Payload bits, TB and block CRC flags, iterations and posteriors are the
+    same bits in all four runs."""
+    import torch
+    sizes = [6120, 4072, 3048, 12216, 8120, 5096, 6120, 12216]
+    tbs, rx = _tbs_rx(sizes, 90 + early)
+    short_tbs, short_rx = _tbs_rx([16], 3)
+    s = torch.cuda.current_stream().cuda_stream
+    res = []
+    for extra in (False, True):
+        all_tbs = tbs + (short_tbs if extra else [])
+        r = torch.from_numpy(np.concatenate([rx, short_rx]) if extra else rx).cuda()
+        for mode in (T.STAGING_FUSED, T.STAGING_SEPARATE):
+            plan = T.TbPlan(all_tbs, T.DECODER_WINDOW_LOG, max_iterations=8, early_stop=early, warmup=16)
+            plan.set_staging(mode)
+            nb = sum(a for a, *_ in tbs)
+            bits = torch.zeros(plan.bits_total, dtype=torch.uint8, device="cuda")
+            post = torch.zeros(plan.post_floats, dtype=torch.float32, device="cuda")
+            ok = torch.zeros(plan.n_tb, dtype=torch.int32, device="cuda")
+            cb_ok = torch.zeros(plan.n_cb, dtype=torch.int32, device="cuda")
+            its = torch.zeros(plan.n_tb, dtype=torch.int32, device="cuda")
+            _say("lean TB plan: short TB", extra, "staging", mode, "early stop", early)
+            plan.decode(r.data_ptr(), bits_ptr=bits.data_ptr(), tb_crc_ptr=ok.data_ptr(), cb_crc_ptr=cb_ok.data_ptr(),
+                        iters_ptr=its.data_ptr(), post_ptr=post.data_ptr(), stream=s)
+            torch.cuda.synchronize()
+            ncb = len(plan.ks) - (1 if extra else 0)  # the short TB's block comes last
+            nk = sum(int(k) for k in plan.ks[:ncb])
+            res.append((bits[:nb].cpu(), ok[:len(tbs)].cpu(), cb_ok[:ncb].cpu(), its[:len(tbs)].cpu(),
+                        post[:nk].cpu()))
+    assert 0 < int(res[0][1].sum()) < len(tbs)
+    for other in res[1:]:
+        for x, y in zip(res[0], other):
+            assert torch.equal(x, y)
+
+
 def test_tb_staging_mode_errors():
     tbs, _, _ = _tb_batch(62, n_tb=2)
     plan = T.TbPlan(tbs)
