@@ -21,6 +21,8 @@
 
 namespace tcdev {
 __global__ void derate_soft_kernel(const DerateCb*, const uint16_t*, const float*, float*, int32_t, int32_t);
+__global__ void derate_soft2_kernel(const DerateCb*, const uint16_t*, const float*, int32_t, const float*, int32_t,
+                                    float*, int32_t);
 __global__ void tb_assemble_kernel(const TbInfo*, const TbCb*, const float*, const uint32_t*, uint32_t,
                                    const uint32_t*, const uint32_t*, uint8_t*, int32_t*, int32_t*, uint32_t*);
 __global__ void tb_acc_zero_kernel(uint32_t*, uint32_t);
@@ -94,6 +96,21 @@ void run_derate(const tc_derate_plan& p, const float* rx, int32_t rv, int32_t ac
   const size_t lds = static_cast<size_t>(p.max_valid) * sizeof(float);
   hipLaunchKernelGGL(tcdev::derate_soft_kernel, dim3(static_cast<uint32_t>(p.n_cb)), dim3(tcdev::kDerateThreads), lds,
                      s, p.d_cbs, p.d_rank, rx, llr, rv, accumulate ? 1 : 0);
+  hip_check(hipGetLastError(), "de-rate matching launch");
+}
+
+// Two transmissions in one pass (HARQ combining): the same result as
+// run_derate(rx_a, rv_a, accumulate) then run_derate(rx_b, rv_b, 1).
+void run_derate2(const tc_derate_plan& p, const float* rx_a, int32_t rv_a, const float* rx_b, int32_t rv_b,
+                 int32_t accumulate, float* llr, hipStream_t s) {
+  if (rv_a < -1 || rv_a > 3 || rv_b < -1 || rv_b > 3)
+    throw Failure{TC_ERR_INVALID_ARGUMENT, "rv must be -1 (per block) or 0..3"};
+  if (p.n_cb == 0) return;
+  if (!rx_a || !rx_b || !llr) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+  if (p.n_cb > 0x7FFFFFFFull) throw Failure{TC_ERR_INVALID_ARGUMENT, "too many code blocks"};
+  const size_t lds = 2 * static_cast<size_t>(p.max_valid) * sizeof(float);
+  hipLaunchKernelGGL(tcdev::derate_soft2_kernel, dim3(static_cast<uint32_t>(p.n_cb)), dim3(tcdev::kDerateThreads),
+                     lds, s, p.d_cbs, p.d_rank, rx_a, rv_a, rx_b, rv_b, llr, accumulate ? 1 : 0);
   hip_check(hipGetLastError(), "de-rate matching launch");
 }
 
@@ -311,9 +328,18 @@ void run_tb(tc_tb_plan& p, uint32_t n_tx, const float* const* rx, const int32_t*
     // window decoders, de-rate kernels first: each transmission into d_soft (added in order onto
     // the held input or the transmissions before it, the fused staging's order), then the decode
     // of d_soft with the TB assembly in its epilogue
-    for (uint32_t t = 0; t < n_tx; ++t) {
+    // (transmissions taken two at a time in one pass each, the same left-to-right sums)
+    for (uint32_t t = 0; t < n_tx;) {
       if (!rx[t]) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
-      run_derate(*p.derate, rx[t], rv ? rv[t] : -1, (t > 0 || accumulate) ? 1 : 0, p.d_soft, s);
+      const int32_t acc = (t > 0 || accumulate) ? 1 : 0;
+      if (t + 1 < n_tx) {
+        if (!rx[t + 1]) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
+        run_derate2(*p.derate, rx[t], rv ? rv[t] : -1, rx[t + 1], rv ? rv[t + 1] : -1, acc, p.d_soft, s);
+        t += 2;
+      } else {
+        run_derate(*p.derate, rx[t], rv ? rv[t] : -1, acc, p.d_soft, s);
+        t += 1;
+      }
     }
     n_tx = 0;
   }

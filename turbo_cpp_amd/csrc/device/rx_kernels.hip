@@ -80,6 +80,87 @@ __global__ __launch_bounds__(kDerateThreads) void derate_soft_kernel(const Derat
   }
 }
 
+// The received values of one transmission summed per rank in k order into
+// acc[0 .. min(e, nv)) (derate_soft_kernel's first step).
+__device__ __forceinline__ void derate_fill(const float* __restrict__ in, uint32_t e, uint32_t nv, float* acc) {
+  constexpr uint32_t U = 4;
+  if (e <= nv) {
+    for (uint32_t base = 0; base < e; base += U * kDerateThreads) {
+      float v[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t m = base + u * kDerateThreads + threadIdx.x;
+        v[u] = m < e ? in[m] : 0.0f;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t m = base + u * kDerateThreads + threadIdx.x;
+        if (m < e) acc[m] = 0.0f + v[u];
+      }
+    }
+  } else {
+    for (uint32_t m = threadIdx.x; m < nv; m += kDerateThreads) {
+      float v = 0.0f;
+      for (uint32_t k = m; k < e; k += nv) v += in[k];
+      acc[m] = v;
+    }
+  }
+}
+
+// Two transmissions (HARQ combining) in one pass: what derate_soft_kernel
+// leaves after a run on rx_a (accumulating or not) followed by an
+// accumulating run on rx_b, bit for bit (each position: the first value, or
+// the held one plus it, then plus the second), with one write of the output
+// instead of a write, a read and a second write.  LDS: the two per-rank sums.
+__global__ __launch_bounds__(kDerateThreads) void derate_soft2_kernel(const DerateCb* __restrict__ cbs,
+                                                                      const uint16_t* __restrict__ rank,
+                                                                      const float* __restrict__ rx_a, int32_t rv_a,
+                                                                      const float* __restrict__ rx_b, int32_t rv_b,
+                                                                      float* __restrict__ out, int32_t accumulate) {
+  extern __shared__ float acc2[];
+  const DerateCb c = cbs[blockIdx.x];
+  const uint32_t nv = c.n_valid, e = c.e;
+  const uint32_t ne = e < nv ? e : nv;
+  derate_fill(rx_a + c.in_off, e, nv, acc2);
+  derate_fill(rx_b + c.in_off, e, nv, acc2 + ne);
+  __syncthreads();
+  const uint32_t ra = (rv_a >= 0 ? static_cast<uint32_t>(rv_a) : c.rv) & 3u;
+  const uint32_t rb = (rv_b >= 0 ? static_cast<uint32_t>(rv_b) : c.rv) & 3u;
+  const uint32_t na = cbs[blockIdx.x].start[ra], nb = cbs[blockIdx.x].start[rb];
+  const uint16_t* rk = rank + c.rank_off;
+  float* o = out + c.out_off;
+  const uint32_t n = 3 * c.d;
+  constexpr uint32_t U = 4;
+  for (uint32_t base = 0; base < n; base += U * kDerateThreads) {
+    uint32_t r[U];
+    float prev[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t i = base + u * kDerateThreads + threadIdx.x;
+      r[u] = i < n ? rk[i] : kNullRank;
+      prev[u] = (accumulate && i < n) ? o[i] : 0.0f;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t i = base + u * kDerateThreads + threadIdx.x;
+      if (i >= n) continue;
+      const bool valid = r[u] != kNullRank;
+      const uint32_t ma = (r[u] >= na) ? r[u] - na : r[u] + nv - na;
+      const uint32_t mb = (r[u] >= nb) ? r[u] - nb : r[u] + nv - nb;
+      const bool sa = valid && ma < e, sb = valid && mb < e;
+      float v;
+      if (accumulate) {
+        v = sa ? prev[u] + acc2[ma] : prev[u];
+      } else {
+        v = sa ? acc2[ma] : 0.0f;
+      }
+      if (sb) v = v + acc2[ne + mb];
+      if (accumulate && !sa && !sb) continue;  // untouched, as in the two runs
+      o[i] = v;
+    }
+  }
+}
+
 // XOR over the workgroup (kTbThreads lanes); every thread gets the result.
 __device__ uint32_t block_xor(uint32_t x, uint32_t* red) {
   for (int o = 32; o > 0; o >>= 1) x ^= __shfl_xor(x, o, 64);
