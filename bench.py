@@ -180,7 +180,7 @@ def e2e_leg(T, llr, opts):
     (host LLRs in, pageable H2D copy, decode, D2H copy of the posteriors; the
     thread's cached plan and buffers are reused after the first call)."""
     ks = [K] * N_CB
-    opts = {k: v for k, v in opts.items() if k in ("min_segment", "warmup")}
+    opts = {k: v for k, v in opts.items() if k in ("min_segment", "warmup", "max_waves_per_block")}
     host = np.ascontiguousarray(llr.reshape(-1))
     T.decode_batch_host(ks, host, T.DECODER_WINDOW_LOG, ITERS, **opts)  # plan + buffers
     ts = []
@@ -292,6 +292,57 @@ def rx_chain_leg(T, torch, stream, llr, decode_ms, tuned):
             "derate_roofline": {"bound": "hbm", "achieved": der_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": der_gbs / HBM_PEAK_GBS, "bytes_per_launch": der_bytes},
             "tb_crc_pass": {f"{x:.2f}": float(okh[ebn0 == x].mean()) for x in EBN0}}
+
+
+# window lengths at K=6144: (segment length, waves per block); 32 and 64 span three and two waves
+WINDOW_LENGTHS = [(32, 3), (64, 2), (96, 1), (128, 1)]
+
+
+def window_length_leg(T, torch, stream, x, bits, ebn0, dec, warm, shares=(4096, 512, 64, 1), tune_share=512):
+    """Window (segment) lengths 32/64/96/128 at K=6144 on the headline
+    inputs: decode time at the full batch and at smaller per-GPU shares (the
+    first n blocks; n = 1: one block's latency), BER/BLER per Eb/N0 at the
+    full batch; and the
+    autotuner's BER-gated choice at a 512-block share, where three-wave
+    blocks of 32-step windows fill the GPU that 512 one-wave blocks leave
+    half idle."""
+    from turbo_cpp_amd import autotune
+    rows = []
+    y = torch.empty(N_CB * K, dtype=torch.float32, device="cuda")
+    for s_len, waves in WINDOW_LENGTHS:
+        geo = T.segment_geometry(K, dec, s_len, warm, 0, waves)
+        row = {"segment_length": geo["segment_length"], "segments": geo["segments"],
+               "waves_per_block": geo["waves_per_block"], "warmup": geo["warmup"], "shares": {}}
+        for n in shares:
+            plan = T.Plan([K] * n, dec, ITERS, min_segment=s_len, warmup=warm, max_waves_per_block=waves)
+            for _ in range(2):
+                plan.decode(x.data_ptr(), y.data_ptr(), 0, 0, stream.cuda_stream)
+            evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(5)]
+            for a, b in evs:
+                a.record(stream)
+                plan.decode(x.data_ptr(), y.data_ptr(), 0, 0, stream.cuda_stream)
+                b.record(stream)
+            torch.cuda.synchronize()
+            ms = float(np.median([a.elapsed_time(b) for a, b in evs]))
+            row["shares"][str(n)] = {"ms": ms, "gbit_s": n * K / ms / 1e6}
+            if n == N_CB:
+                row["ber"] = ber_table(y.cpu().numpy(), bits, ebn0)
+            del plan
+        log("window length", row["segment_length"], "on", row["waves_per_block"], "waves:",
+            {n: round(v["ms"], 3) for n, v in row["shares"].items()}, "ms")
+        rows.append(row)
+    t = autotune.tune(K, dec, ITERS, n_blocks=tune_share, ebn0_db=(0.25, 0.5, 0.75), warmups=(16, 24),
+                      waves=(0,), extrinsic_bits=(16,))
+    log(f"autotune at {tune_share} blocks:", {k: t["best"][k] for k in ("segment_length", "waves_per_block", "warmup",
+                                                                         "ms", "qualifies")})
+    return {"k": K, "decoder": "window-log" if dec == T.DECODER_WINDOW_LOG else "window-lin",
+            "what": "the headline's inputs (first n blocks for n < 4096); median of 5 launches",
+            "rows": rows,
+            "autotune_small_share": {
+                "n_blocks": tune_share, "choice": {k: t["best"][k] for k in (
+                    "segment_length", "waves_per_block", "warmup", "ms", "per_point", "qualifies")},
+                "table": [{k: c[k] for k in ("segment_length", "waves_per_block", "warmup", "ms", "per_point",
+                                             "qualifies")} for c in t["table"]]}}
 
 
 def _graph_time(torch, fn, reps):
@@ -639,6 +690,8 @@ def main():
     ap.add_argument("--no-rx-chain", action="store_true")
     ap.add_argument("--train-steps", type=int, default=500, help="Adam steps of the differentiation leg (0: skip it)")
     ap.add_argument("--no-scenarios", action="store_true", help="skip the uneven-batch and mixed-load legs")
+    ap.add_argument("--no-window-lengths", action="store_true",
+                    help="skip the window-length leg (32/64/96/128 at 4096, 512, 64 and 1 blocks)")
     ap.add_argument("--no-autotune", action="store_true", help="skip the tuner; use --min-segment/--warmup-steps")
     ap.add_argument("--min-segment", type=int, default=0, help="with --no-autotune (0: library default 32)")
     ap.add_argument("--warmup-steps", type=int, default=0, help="with --no-autotune (0: library default 32)")
@@ -673,7 +726,7 @@ def main():
     stream = torch.cuda.current_stream()
     plan = T.Plan(ks, dec, ITERS, **tuned_opts)
     geo = T.segment_geometry(K, dec, tuned_opts.get("min_segment", 0), tuned_opts.get("warmup", 0),
-                             tuned_opts.get("extrinsic_bits", 0))
+                             tuned_opts.get("extrinsic_bits", 0), tuned_opts.get("max_waves_per_block", 0))
 
     def step(p=plan, out=y):
         p.decode(x.data_ptr(), out.data_ptr(), 0, 0, stream.cuda_stream)
@@ -763,7 +816,9 @@ def main():
     plain_gbits = K / plain_s / 1e9
     log(f"plain: {plain_s * 1e3:.3f} ms per block -> {plain_gbits * 1e3:.3f} Mbit/s")
 
-    geo_kw = {k: v for k, v in tuned_opts.items() if k in ("min_segment", "warmup", "extrinsic_bits")}
+    geo_kw = {k: v for k, v in tuned_opts.items()
+              if k in ("min_segment", "warmup", "extrinsic_bits", "max_waves_per_block")}
+    wl = None if args.no_window_lengths else window_length_leg(T, torch, stream, x, bits, ebn0, dec, geo["warmup"])
     rx_chain = None if args.no_rx_chain else rx_chain_leg(T, torch, stream, llr, kernel_ms, geo_kw)
     diff = diff_leg(T, torch, args.train_steps) if args.train_steps > 0 else None
     uneven = None if args.no_scenarios else uneven_leg(T, torch, geo_kw)
@@ -836,6 +891,7 @@ def main():
                    "code_blocks": N_CB, "k": K, "iterations": ITERS, "decoder": f"window-{args.decoder}",
                    "segment_length": geo["segment_length"], "segments_per_block": geo["segments"],
                    "warmup_steps": geo["warmup"], "waves_per_cu": tuned_opts.get("waves_per_cu", 0),
+                   "waves_per_block": geo["waves_per_block"],
                    "extrinsics_held": f"fp{geo['extrinsic_bits']} (modes.log_f32: fp32-held, same geometry)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -856,9 +912,10 @@ def main():
             "calibration": f"{tuned['n_blocks']} blocks K={K} per point, QPSK/AWGN at {tuned['ebn0_db']} dB; "
                            f"qualifies: bit and block errors inside the exact decoder's 95% intervals at every "
                            f"point; timed at {tuned['timed_at_db']} dB",
-            "table": [{k: c[k] for k in ("segment_length", "warmup", "waves_per_cu", "extrinsic_bits", "ms",
-                                         "per_point", "qualifies")}
+            "table": [{k: c[k] for k in ("segment_length", "waves_per_block", "warmup", "waves_per_cu",
+                                         "extrinsic_bits", "ms", "per_point", "qualifies")}
                       for c in tuned["table"]]},
+        "window_lengths": wl,
         "uneven_batch": uneven,
         "mixed_load": mixed,
     }

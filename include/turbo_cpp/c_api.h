@@ -61,6 +61,10 @@ typedef struct tc_options {
   uint32_t extrinsic_bits; /* width of the extrinsics held between half-iterations: 16
                               (fp16) or 32 (fp32); 0 = default (window decoders 16, exact
                               decoders 32, which accept only 32) */
+  uint32_t max_waves_per_block; /* window decoders (log, lin): a code block may span up to
+                              this many waves of one work-group, 1-3 (0 = 1), so the
+                              segment length may go down to ceil(K / (64 n)): window
+                              lengths 32 and 64 at K = 6144 need 3 and 2 */
 } tc_options;
 
 /* Geometry the window decoders chose for one K (for tests and reports). */
@@ -68,11 +72,13 @@ typedef struct tc_segment_info {
   uint32_t k;
   uint32_t segment_length; /* S, divides K */
   uint32_t segments;       /* M = K / S: lanes per code block */
-  uint32_t blocks_per_wave;/* code blocks packed into one 64-lane wavefront */
+  uint32_t blocks_per_wave;/* code blocks per job of waves_per_block waves (one job: 64 x
+                              waves_per_block lanes) */
   uint32_t warmup;         /* effective warm-up steps (0 when M == 1) */
   uint32_t extrinsic_bits; /* width of the extrinsics held between
                               half-iterations: 16 (fp16, saturated at
                               +-65504, round to nearest even) or 32 */
+  uint32_t waves_per_block;/* waves one code block spans: ceil(M / 64) */
 } tc_segment_info;
 
 const char* tc_last_error(void);

@@ -530,6 +530,99 @@ def test_window_spare_lane_jobs_match_oracle(k, decoder, early_stop):
         assert crc.sum() > 0
 
 
+# ------------------------------------------------ blocks on several waves
+# (K, segment length, waves per block, warm-up): 192, 96, 128, 126 and 160
+# segments; the last two leave lanes of their last wave idle
+WIDE_CASES = [(6144, 32, 3, 24), (6144, 64, 2, 16), (6144, 48, 2, 32), (4032, 32, 2, 16), (5120, 32, 3, 8)]
+
+
+def _wide_geo(k, decoder, seg, waves, warm, xbits=0):
+    geo = T.segment_geometry(k, decoder, seg, warm, xbits, waves)
+    assert (geo["segment_length"], geo["waves_per_block"]) == (seg, waves), geo
+    return geo
+
+
+@pytest.mark.parametrize("xbits", [16, 32])
+@pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])
+@pytest.mark.parametrize("k,seg,waves,warm", WIDE_CASES)
+def test_window_multi_wave_matches_oracle(k, seg, waves, warm, decoder, xbits):
+    """Code blocks of more than 64 segments span 2 or 3 waves of one
+    work-group (max_waves_per_block): window lengths 32, 48 and 64 at
+    K=6144.  Each block equals the oracle's restatement with the same
+    segments and warm-up at the window bar (_window_bar: fp32-held 1e-3 +
+    1e-4 |LLR|, hard decisions equal where |LLR| >= 1e-3; fp16-held scaled
+    by max |LLR|); a second decode of the same blocks gives the same bits
+    (the waves' hand-offs through memory are ordered by barriers, not by
+    timing)."""
+    geo = _wide_geo(k, decoder, seg, waves, warm, xbits)
+    rng = np.random.default_rng(k + seg + 7 * decoder + xbits)
+    llrs = []
+    for e in (0.25, 0.6, 1.0):
+        cb = rng.integers(0, 2, k).astype(np.uint8)
+        llrs.append(np.concatenate(bpsk_awgn_llr(O.encode_cb(cb), k, e, rng)))
+    _say(f"several waves: K={k} S={seg} waves={waves} g={warm} decoder={decoder} fp{xbits}")
+    args = ([k] * 3, np.concatenate(llrs), decoder, 8, False, None, seg, warm, xbits, waves)
+    post, _, its = T.decode_batch_host(*args)
+    again, _, _ = T.decode_batch_host(*args)
+    assert np.array_equal(post.view(np.uint32), again.view(np.uint32))
+    lin = decoder == T.DECODER_WINDOW_LIN
+    for i, l in enumerate(llrs):
+        ref = O.decode_window(*_split(l, k), k, seg, geo["warmup"], lin, 0, 8, False, geo["extrinsic_bits"])[0]
+        atol, rtol, hard = _window_bar(geo, ref)
+        assert_llr_parity(post[i * k:(i + 1) * k], ref, atol=atol, rtol=rtol, hard_tol=hard)
+    assert list(its) == [8, 8, 8]
+
+
+@pytest.mark.parametrize("decoder", [T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN])
+def test_window_multi_wave_plan_early_stop_matches_oracle(decoder):
+    """One plan of 3-wave jobs (max_waves_per_block 3, segment length from
+    32): K=6144 and 5120 blocks span three waves; shorter blocks are packed
+    192 lanes to a job (K=1024: 6 blocks of 32 segments, K=544: 11 of 17, so
+    blocks straddle waves; K=40: one segment each), with CRC24B early stop,
+    fillers and a block at -1 dB that never passes.  Stopping iteration and
+    CRC verdict equal the oracle's, posteriors at the window bar."""
+    ks = [6144, 1024, 6144, 40, 5120, 544, 6144, 1024, 40, 2048, 544, 544]
+    fs = [0, 8, 24, 0, 0, 0, 16, 0, 0, 8, 0, 0]
+    eb = [1.0, 1.2, -1.0, 2.0, 0.8, 1.5, 0.6, 0.3, 0.0, 1.1, 2.5, 0.9]
+    _, llrs = _crc_blocks(ks, fs, eb, 88 + decoder)
+    post, crc, its = T.decode_batch_host(ks, np.concatenate(llrs), decoder, 8, True, fs, 32, 16, 0, 3)
+    lin = decoder == T.DECODER_WINDOW_LIN
+    off = 0
+    for i, (k, f, l) in enumerate(zip(ks, fs, llrs)):
+        geo = T.segment_geometry(k, decoder, 32, 16, 0, 3)
+        ref, ok, used = O.decode_window(*_split(l, k), k, geo["segment_length"], geo["warmup"], lin, f, 8, True,
+                                        geo["extrinsic_bits"])
+        _say(f"block {i}: K={k} M={geo['segments']} gpu it={its[i]} crc={crc[i]}  oracle it={used} crc={ok}")
+        assert (int(its[i]), bool(crc[i])) == (used, ok)
+        atol, rtol, hard = _window_bar(geo, ref)
+        assert_llr_parity(post[off:off + k], ref, atol=atol, rtol=rtol, hard_tol=hard)
+        off += k
+    assert crc.sum() >= 6 and not crc[2] and min(its) < 8
+
+
+@pytest.mark.parametrize("early_stop", [False, True])
+def test_window_blocks_in_multi_wave_jobs_equal_one_wave(early_stop):
+    """Blocks of at most 64 segments decode to the same bits in a plan of
+    3-wave jobs (192 lanes; blocks packed across the waves, a K=6144 block of
+    192 segments makes the plan wide) as in one-wave jobs of the same
+    segment length and warm-up: posteriors, CRC verdicts, iterations."""
+    ks = [1024] * 7 + [40] * 5 + [544] * 12 + [2048] * 2 + [352] * 6
+    for k in set(ks):
+        a, b = (T.segment_geometry(k, T.DECODER_WINDOW_LOG, 32, 16, 0, n) for n in (1, 3))
+        assert (a["segment_length"], a["warmup"]) == (b["segment_length"], b["warmup"])
+    rng = np.random.default_rng(5 + early_stop)
+    eb = [float(x) for x in rng.uniform(-0.5, 2.5, len(ks))]
+    _, llrs = _crc_blocks(ks, [0] * len(ks), eb, 70 + early_stop)
+    _, big = _crc_blocks([6144], [0], [1.0], 71)
+    one = T.decode_batch_host(ks, np.concatenate(llrs), T.DECODER_WINDOW_LOG, 8, early_stop, None, 32, 16, 0, 1)
+    wide = T.decode_batch_host(ks + [6144], np.concatenate(llrs + big), T.DECODER_WINDOW_LOG, 8, early_stop,
+                               None, 32, 16, 0, 3)
+    n = sum(ks)
+    _say(f"small blocks in 3-wave jobs, early stop {early_stop}: iterations {list(one[2])}")
+    assert np.array_equal(one[0].view(np.uint32), wide[0][:n].view(np.uint32))
+    assert list(one[1]) == list(wide[1][:-1]) and list(one[2]) == list(wide[2][:-1])
+
+
 def test_window_early_stop_equals_fixed_iterations():
     """A block that stops at iteration n has bit-for-bit the posterior of the
     same decoder run for exactly n iterations without early stop."""

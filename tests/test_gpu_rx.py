@@ -197,16 +197,24 @@ def _tb_batch(seed, n_tb=24):
     return tbs, np.concatenate(rx_all), payloads
 
 
-@pytest.mark.parametrize("decoder,early", [(T.DECODER_WINDOW_LOG, False), (T.DECODER_WINDOW_LIN, True),
-                                           (T.DECODER_EXACT, True)])
-def test_tb_pipeline_batch_matches_oracle_assembly(decoder, early):
+@pytest.mark.parametrize("decoder,early,waves,staging", [
+    (T.DECODER_WINDOW_LOG, False, 1, None), (T.DECODER_WINDOW_LIN, True, 1, None), (T.DECODER_EXACT, True, 1, None),
+    (T.DECODER_WINDOW_LOG, True, 3, "fused"), (T.DECODER_WINDOW_LOG, False, 3, "separate"),
+    (T.DECODER_WINDOW_LIN, True, 2, "fused")])
+def test_tb_pipeline_batch_matches_oracle_assembly(decoder, early, waves, staging):
     """Many TBs of mixed sizes, rv and code rates in one plan: the device
     assembly equals the oracle's (lte_turbo.cpp:958-979) on the same
     posteriors bit for bit, and the posteriors equal the batched decoder's on
-    the oracle's soft de-rate matching."""
+    the oracle's soft de-rate matching.  waves > 1: segment length from 32 on
+    jobs of 2 or 3 waves (the K=6144 blocks span them, shorter blocks are
+    packed across them; the TB assembly runs on the first wave), with de-rate
+    matching fused into the decoder's staging or in separate kernels."""
     import torch
-    tbs, rx, payloads = _tb_batch(21 + decoder)
-    plan = T.TbPlan(tbs, decoder=decoder, max_iterations=6, early_stop=early)
+    tbs, rx, payloads = _tb_batch(21 + decoder + (10 * waves if waves > 1 else 0))
+    geo = dict(min_segment=32, max_waves_per_block=waves) if waves > 1 else {}
+    plan = T.TbPlan(tbs, decoder=decoder, max_iterations=6, early_stop=early, **geo)
+    if staging:
+        plan.set_staging(T.STAGING_FUSED if staging == "fused" else T.STAGING_SEPARATE)
     assert plan.rx_floats == rx.size
     dev = dict(rx=torch.from_numpy(rx).cuda(),
                bits=torch.zeros(plan.bits_total, dtype=torch.uint8, device="cuda"),
@@ -226,7 +234,9 @@ def test_tb_pipeline_batch_matches_oracle_assembly(decoder, early):
         llr.append(O.derate_match_soft(rx[ioff:ioff + int(e)], int(k) + 4, int(f), int(rv)))
         ioff += int(e)
     post_ref, _, its_ref = T.decode_batch_host(plan.ks, np.concatenate(llr), decoder, 6, early,
-                                               fillers=plan.fillers)
+                                               fillers=plan.fillers, **geo)
+    if waves > 1:
+        assert max(T.segment_geometry(int(k), decoder, 32, 0, 0, waves)["waves_per_block"] for k in plan.ks) == waves
     assert np.array_equal(h["post"].view(np.uint32), post_ref.view(np.uint32))
     boff = coff = poff = 0
     n_ok = 0

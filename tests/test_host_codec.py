@@ -193,6 +193,31 @@ def test_segment_geometry_all_table_sizes():
     assert T.segment_geometry(6144)["segment_length"] == 96
 
 
+def test_segment_geometry_several_waves_per_block():
+    """max_waves_per_block n (window log/lin decoders) lowers the bound on
+    S to ceil(K / (64 n)): window lengths 32 and 64 at K=6144 on 3 and 2
+    waves; every table K keeps S | K, M <= 64 n, waves_per_block =
+    ceil(M / 64) <= n and blocks_per_wave = 64 waves_per_block // M; n above
+    3, or above 1 for the exact decoders, is an argument error."""
+    L, N = T.DECODER_WINDOW_LOG, T.DECODER_WINDOW_LIN
+    g = T.segment_geometry(6144, L, 32, 16, 0, 3)
+    assert (g["segment_length"], g["segments"], g["waves_per_block"], g["blocks_per_wave"]) == (32, 192, 3, 1)
+    g = T.segment_geometry(6144, N, 64, 16, 32, 3)
+    assert (g["segment_length"], g["segments"], g["waves_per_block"], g["extrinsic_bits"]) == (64, 96, 2, 32)
+    assert T.segment_geometry(6144, L, 32, 16, 0, 2)["segment_length"] == 48
+    for n in (1, 2, 3):
+        for (k, _, _) in T.qpp_interleaver_table():
+            g = T.segment_geometry(k, L, 32, 16, 0, n)
+            s, m, w = g["segment_length"], g["segments"], g["waves_per_block"]
+            assert k % s == 0 and s * m == k and m <= 64 * n and w == (m + 63) // 64 <= n
+            assert g["blocks_per_wave"] == 64 * w // m
+            assert s == T.segment_geometry(k, L, 32, 16, 0, 1)["segment_length"] or m > 64
+    with pytest.raises(ValueError):
+        T.segment_geometry(6144, L, 32, 16, 0, 4)
+    with pytest.raises(ValueError):
+        T.segment_geometry(6144, T.DECODER_EXACT, 32, 16, 0, 2)
+
+
 def test_single_hip_runtime_per_process():
     """Loading the library (and torch after it) maps one HIP runtime only."""
     import subprocess

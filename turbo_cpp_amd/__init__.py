@@ -37,7 +37,7 @@ class TcCbDesc(C.Structure):
 class TcOptions(C.Structure):
     _fields_ = [("max_iterations", C.c_uint32), ("early_stop", C.c_int32), ("decoder", C.c_int32),
                 ("min_segment", C.c_uint32), ("warmup", C.c_uint32), ("waves_per_cu", C.c_uint32),
-                ("extrinsic_bits", C.c_uint32)]
+                ("extrinsic_bits", C.c_uint32), ("max_waves_per_block", C.c_uint32)]
 
 
 class TcTbDesc(C.Structure):
@@ -52,7 +52,7 @@ class TcChannelSeg(C.Structure):
 
 class TcSegmentInfo(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("k", "segment_length", "segments", "blocks_per_wave", "warmup",
-                                          "extrinsic_bits")]
+                                          "extrinsic_bits", "waves_per_block")]
 
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
@@ -403,11 +403,15 @@ def turbo_decode_code_block_cpu(descriptor: CodeBlockDescriptor, d0, d1, d2, max
 
 
 def options(decoder: int = DECODER_WINDOW_LOG, max_iterations: int = 8, early_stop: bool = False,
-            min_segment: int = 0, warmup: int = 0, waves_per_cu: int = 0, extrinsic_bits: int = 0) -> TcOptions:
+            min_segment: int = 0, warmup: int = 0, waves_per_cu: int = 0, extrinsic_bits: int = 0,
+            max_waves_per_block: int = 0) -> TcOptions:
     """tc_options; extrinsic_bits: width of the extrinsics held between
     half-iterations (16 or 32; 0 = the decoder's default: 16 for the window
-    decoders, 32 for the exact ones)."""
-    return TcOptions(max_iterations, int(early_stop), decoder, min_segment, warmup, waves_per_cu, extrinsic_bits)
+    decoders, 32 for the exact ones); max_waves_per_block: waves one code
+    block may span (1-3, 0 = 1; window log/lin decoders), which lets the
+    segment length go down to ceil(K / (64 n))."""
+    return TcOptions(max_iterations, int(early_stop), decoder, min_segment, warmup, waves_per_cu, extrinsic_bits,
+                     max_waves_per_block)
 
 
 def _gpu_decode(decoder, descriptor, d0, d1, d2, max_iterations, enable_early_stop, **kw):
@@ -453,17 +457,19 @@ def turbo_decode_code_block_gpu_stage1_tile8_experimental(descriptor, d0, d1, d2
 
 
 def segment_geometry(k: int, decoder: int = DECODER_WINDOW_LOG, min_segment: int = 0, warmup: int = 0,
-                     extrinsic_bits: int = 0):
-    o = options(decoder, 8, False, min_segment, warmup, 0, extrinsic_bits)
+                     extrinsic_bits: int = 0, max_waves_per_block: int = 0):
+    o = options(decoder, 8, False, min_segment, warmup, 0, extrinsic_bits, max_waves_per_block)
     g = TcSegmentInfo()
     _check(lib().tc_segment_geometry(k, C.byref(o), C.byref(g)))
     return dict(k=g.k, segment_length=g.segment_length, segments=g.segments,
-                blocks_per_wave=g.blocks_per_wave, warmup=g.warmup, extrinsic_bits=g.extrinsic_bits)
+                blocks_per_wave=g.blocks_per_wave, warmup=g.warmup, extrinsic_bits=g.extrinsic_bits,
+                waves_per_block=g.waves_per_block)
 
 
 def decode_batch_host(ks: Sequence[int], llr: np.ndarray, decoder: int = DECODER_WINDOW_LOG,
                       max_iterations: int = 8, early_stop: bool = False, fillers=None,
-                      min_segment: int = 0, warmup: int = 0, extrinsic_bits: int = 0):
+                      min_segment: int = 0, warmup: int = 0, extrinsic_bits: int = 0,
+                      max_waves_per_block: int = 0):
     """Decode packed host LLRs (block r: [d0|d1|d2], 3(K_r+4) floats).
 
     Returns (posterior (sum K_r floats), crc_ok[n], iterations_used[n])."""
@@ -476,7 +482,7 @@ def decode_batch_host(ks: Sequence[int], llr: np.ndarray, decoder: int = DECODER
     crc = np.zeros(n, np.int32)
     its = np.zeros(n, np.uint32)
     fil = None if fillers is None else np.ascontiguousarray(fillers, dtype=np.uint32)
-    o = options(decoder, max_iterations, early_stop, min_segment, warmup, 0, extrinsic_bits)
+    o = options(decoder, max_iterations, early_stop, min_segment, warmup, 0, extrinsic_bits, max_waves_per_block)
     _check(lib().tc_decode_batch_host(ks, None if fil is None else fil.ctypes.data, n, C.byref(o), llr, post,
                                       crc.ctypes.data, its.ctypes.data))
     return post, crc.astype(bool), its
@@ -491,10 +497,11 @@ class Plan:
 
     def __init__(self, ks: Sequence[int], decoder: int = DECODER_WINDOW_LOG, max_iterations: int = 8,
                  early_stop: bool = False, fillers=None, min_segment: int = 0, warmup: int = 0,
-                 waves_per_cu: int = 0, extrinsic_bits: int = 0):
+                 waves_per_cu: int = 0, extrinsic_bits: int = 0, max_waves_per_block: int = 0):
         self.ks = np.ascontiguousarray(ks, dtype=np.uint32)
         self._fil = None if fillers is None else np.ascontiguousarray(fillers, dtype=np.uint32)
-        o = options(decoder, max_iterations, early_stop, min_segment, warmup, waves_per_cu, extrinsic_bits)
+        o = options(decoder, max_iterations, early_stop, min_segment, warmup, waves_per_cu, extrinsic_bits,
+                    max_waves_per_block)
         h = C.c_void_p()
         _check(lib().tc_plan_create(self.ks, None if self._fil is None else self._fil.ctypes.data, len(self.ks),
                                     C.byref(o), C.byref(h)))
@@ -634,10 +641,11 @@ class TbPlan:
     may be 0."""
 
     def __init__(self, tbs, decoder: int = DECODER_WINDOW_LOG, max_iterations: int = 8,
-                 early_stop: bool = False, min_segment: int = 0, warmup: int = 0, extrinsic_bits: int = 0):
+                 early_stop: bool = False, min_segment: int = 0, warmup: int = 0, extrinsic_bits: int = 0,
+                 max_waves_per_block: int = 0):
         self.n_tb = len(tbs)
         self._descs = _tb_descs(tbs)
-        o = options(decoder, max_iterations, early_stop, min_segment, warmup, 0, extrinsic_bits)
+        o = options(decoder, max_iterations, early_stop, min_segment, warmup, 0, extrinsic_bits, max_waves_per_block)
         h = C.c_void_p()
         _check(lib().tc_tb_plan_create(self._descs, self.n_tb, C.byref(o), C.byref(h)))
         self._h = h

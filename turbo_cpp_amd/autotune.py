@@ -1,10 +1,12 @@
 """Autotuner of the window decoders' launch geometry.
 
-Knobs (tc_options): the segment length S (via min_segment; S is the smallest
-divisor of K >= max(min_segment, K/64), which also fixes the segments per
-block M = K/S and the blocks per 64-lane wave 64/M), the warm-up steps per
+Knobs (tc_options): the segment length S (via min_segment and
+max_waves_per_block n; S is the smallest divisor of K >= max(min_segment,
+K/(64 n)), which also fixes the segments per block M = K/S, the waves one
+block spans, ceil(M/64), and the blocks per job), the warm-up steps per
 segment boundary, the resident waves per CU (occupancy) and the width of the
-extrinsics held between half-iterations (fp16 or fp32).
+extrinsics held between half-iterations (fp16 or fp32).  Window lengths 32
+and 64 at K=6144 are blocks on three and two waves.
 
 For one K the tuner makes calibration batches on the GPU at several Eb/N0
 points (random payloads of K - 24 bits with CRC24A, the GPU encoder, full
@@ -34,6 +36,8 @@ class Candidate:
     warmup: int
     waves_per_cu: int
     extrinsic_bits: int = 16
+    max_waves_per_block: int = 1
+    waves_per_block: int = 1
     segment_length: int = 0
     segments: int = 0
     blocks_per_wave: int = 0
@@ -93,7 +97,8 @@ def _decode(plan: Plan, llr, post, reps: int):
 def tune(k: int, decoder: int = DECODER_WINDOW_LOG, iterations: int = 8, n_blocks: int = 2048,
          ebn0_db=(0.25, 0.5, 0.75), min_segments: Sequence[int] = (32, 64, 96, 128, 192),
          warmups: Sequence[int] = (8, 16, 24, 32), waves: Sequence[int] = (0, 4, 2),
-         extrinsic_bits: Sequence[int] = (16,), reps: int = 3, seed: int = 1) -> Dict:
+         extrinsic_bits: Sequence[int] = (16,), block_waves: Sequence[int] = (1, 3), reps: int = 3,
+         seed: int = 1) -> Dict:
     """Returns {"best": {...}, "table": [...], "points": [per Eb/N0: exact errors and intervals], ...}."""
     import torch
     pts = [float(ebn0_db)] if np.isscalar(ebn0_db) else [float(e) for e in ebn0_db]
@@ -121,17 +126,19 @@ def tune(k: int, decoder: int = DECODER_WINDOW_LOG, iterations: int = 8, n_block
                        "block_interval": [math.floor(lo_k * n_blocks), math.ceil(hi_k * n_blocks)]})
     seen = set()
     table: List[Candidate] = []
-    for ms, g, w, xb in itertools.product(min_segments, warmups, waves, extrinsic_bits):
+    for ms, g, w, xb, bw in itertools.product(min_segments, warmups, waves, extrinsic_bits, block_waves):
         try:
-            geo = segment_geometry(k, decoder, ms, g, xb)
+            geo = segment_geometry(k, decoder, ms, g, xb, bw)
         except ValueError:
             continue  # segment length outside what the kernel supports
         key = (geo["segment_length"], geo["warmup"], w, xb)
         if key in seen:
             continue
         seen.add(key)
-        c = Candidate(ms, g, w, xb, geo["segment_length"], geo["segments"], geo["blocks_per_wave"])
-        plan = Plan([k] * n_blocks, decoder, iterations, min_segment=ms, warmup=g, waves_per_cu=w, extrinsic_bits=xb)
+        c = Candidate(ms, g, w, xb, bw, geo["waves_per_block"], geo["segment_length"], geo["segments"],
+                      geo["blocks_per_wave"])
+        plan = Plan([k] * n_blocks, decoder, iterations, min_segment=ms, warmup=g, waves_per_cu=w, extrinsic_bits=xb,
+                    max_waves_per_block=bw)
         c.per_point = errors(plan)
         c.bit_errors = sum(x[0] for x in c.per_point)
         c.block_errors = sum(x[1] for x in c.per_point)
@@ -154,4 +161,4 @@ def options_for(result: Dict) -> Dict[str, int]:
     """Keyword arguments for Plan / options() from a tune() result."""
     b = result["best"]
     return {"min_segment": b["min_segment"], "warmup": b["warmup"], "waves_per_cu": b["waves_per_cu"],
-            "extrinsic_bits": b["extrinsic_bits"]}
+            "extrinsic_bits": b["extrinsic_bits"], "max_waves_per_block": b.get("max_waves_per_block", 1)}

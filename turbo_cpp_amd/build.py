@@ -25,7 +25,7 @@ ARCH = "gfx950"
 
 HOST_SOURCES = ["host/codec.cpp", "host/api.cpp"]
 DEVICE_SOURCES = ["device/exact_kernel.hip", "device/window_log.hip", "device/window_log_lean.hip",
-                  "device/window_lin.hip",
+                  "device/window_lin.hip", "device/window_wide2.hip", "device/window_wide3.hip",
                   "device/window_exact.hip", "device/c_api.hip",
                   "device/rx_kernels.hip", "device/rx_api.hip", "device/tx_kernels.hip", "device/channel.hip",
                   "device/diff_kernel.hip",

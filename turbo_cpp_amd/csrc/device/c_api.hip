@@ -36,11 +36,27 @@ __global__ void window_decode_lin_f32_kernel(WinArgs);
 __global__ void window_decode_lin_es_f32_kernel(WinArgs);
 __global__ void window_decode_exact_kernel(WinArgs);
 __global__ void window_decode_exact_es_kernel(WinArgs);
+// jobs on two and three waves (window_wide2.hip, window_wide3.hip)
+#define TC_WIDE_KERNELS(W)                                  \
+  __global__ void window_decode_log_##W##_kernel(WinArgs);        \
+  __global__ void window_decode_log_es_##W##_kernel(WinArgs);     \
+  __global__ void window_decode_log_f32_##W##_kernel(WinArgs);    \
+  __global__ void window_decode_log_es_f32_##W##_kernel(WinArgs); \
+  __global__ void window_decode_lin_##W##_kernel(WinArgs);        \
+  __global__ void window_decode_lin_es_##W##_kernel(WinArgs);     \
+  __global__ void window_decode_lin_f32_##W##_kernel(WinArgs);    \
+  __global__ void window_decode_lin_es_f32_##W##_kernel(WinArgs);
+TC_WIDE_KERNELS(w2)
+TC_WIDE_KERNELS(w3)
+#undef TC_WIDE_KERNELS
 }  // namespace tcdev
 
 namespace {
 
 using namespace tchost;
+
+// Waves one window job may span (window_wide2.hip, window_wide3.hip).
+constexpr int kMaxJobWaves = 3;
 
 tc_options normalised(const tc_options* o) {
   tc_options r{};
@@ -57,14 +73,21 @@ tc_options normalised(const tc_options* o) {
     throw Failure{TC_ERR_INVALID_ARGUMENT, "extrinsic_bits must be 16 or 32"};
   if (exact && r.extrinsic_bits != 32)
     throw Failure{TC_ERR_INVALID_ARGUMENT, "the exact decoders hold extrinsics in fp32"};
+  if (r.max_waves_per_block == 0) r.max_waves_per_block = 1;
+  if (r.max_waves_per_block > static_cast<uint32_t>(kMaxJobWaves))
+    throw Failure{TC_ERR_INVALID_ARGUMENT, "max_waves_per_block must be 1 to " + std::to_string(kMaxJobWaves)};
+  if (exact && r.max_waves_per_block != 1)
+    throw Failure{TC_ERR_INVALID_ARGUMENT, "code blocks on several waves: window decoders (log, lin) only"};
   return r;
 }
 
 // Segment geometry of the window decoders: S = the smallest divisor of K
-// with S >= max(min_segment, ceil(K / 64)), so M = K / S <= 64 lanes.
+// with S >= max(min_segment, ceil(K / (64 n))), n = max_waves_per_block, so
+// M = K / S <= 64 n lanes: ceil(M / 64) waves per block.
 tc_segment_info geometry(uint32_t k, const tc_options& o) {
   if (turbo_cpp::detail::qpp_index(k) < 0) throw Failure{TC_ERR_OUT_OF_RANGE, "unsupported QPP block size"};
-  const uint32_t lo = std::max<uint32_t>(o.min_segment, (k + 63) / 64);
+  const uint32_t lanes = tcdev::kWaveLanes * std::max<uint32_t>(o.max_waves_per_block, 1);
+  const uint32_t lo = std::max<uint32_t>(o.min_segment, (k + lanes - 1) / lanes);
   uint32_t s = lo;
   while (s < k && k % s != 0) ++s;
   if (s > k) s = k;
@@ -75,7 +98,8 @@ tc_segment_info geometry(uint32_t k, const tc_options& o) {
   g.k = k;
   g.segment_length = s;
   g.segments = k / s;
-  g.blocks_per_wave = tcdev::kWaveLanes / g.segments;
+  g.waves_per_block = (g.segments + tcdev::kWaveLanes - 1) / tcdev::kWaveLanes;
+  g.blocks_per_wave = tcdev::kWaveLanes * g.waves_per_block / g.segments;
   // the exact decoder's boundaries are fixed up exactly, with no warm-up
   g.warmup = (g.segments > 1 && o.decoder != TC_DECODER_EXACT) ? std::min(o.warmup, s - 1) : 0;
   g.extrinsic_bits = o.extrinsic_bits;
@@ -92,9 +116,27 @@ bool lean_job(const tcdev::WinJob& j) {
 
 using WinKernel = void (*)(tcdev::WinArgs);
 // lean: every job of the plan has the lean geometry; the log-domain decoder
-// then runs its lean kernels (the same results, bit for bit)
-WinKernel window_kernel(const tc_options& o, bool lean = false) {
+// then runs its lean kernels (the same results, bit for bit).  waves: the
+// waves of one job (2, 3: the wide kernels).
+WinKernel window_kernel(const tc_options& o, bool lean = false, uint32_t waves = 1) {
   const bool es = o.early_stop != 0, f32 = o.extrinsic_bits == 32;
+  const bool log = o.decoder == TC_DECODER_WINDOW_LOG;
+  if (waves == 2) {
+    if (log) {
+      if (f32) return es ? tcdev::window_decode_log_es_f32_w2_kernel : tcdev::window_decode_log_f32_w2_kernel;
+      return es ? tcdev::window_decode_log_es_w2_kernel : tcdev::window_decode_log_w2_kernel;
+    }
+    if (f32) return es ? tcdev::window_decode_lin_es_f32_w2_kernel : tcdev::window_decode_lin_f32_w2_kernel;
+    return es ? tcdev::window_decode_lin_es_w2_kernel : tcdev::window_decode_lin_w2_kernel;
+  }
+  if (waves == 3) {
+    if (log) {
+      if (f32) return es ? tcdev::window_decode_log_es_f32_w3_kernel : tcdev::window_decode_log_f32_w3_kernel;
+      return es ? tcdev::window_decode_log_es_w3_kernel : tcdev::window_decode_log_w3_kernel;
+    }
+    if (f32) return es ? tcdev::window_decode_lin_es_f32_w3_kernel : tcdev::window_decode_lin_f32_w3_kernel;
+    return es ? tcdev::window_decode_lin_es_w3_kernel : tcdev::window_decode_lin_w3_kernel;
+  }
   if (lean && o.decoder == TC_DECODER_WINDOW_LOG) {
     if (f32) return es ? tcdev::window_decode_log_lean_es_f32_kernel : tcdev::window_decode_log_lean_f32_kernel;
     return es ? tcdev::window_decode_log_lean_es_kernel : tcdev::window_decode_log_lean_kernel;
@@ -111,8 +153,9 @@ WinKernel window_kernel(const tc_options& o, bool lean = false) {
   }
 }
 
-size_t lds_bytes_of(uint32_t seg, const tc_options& o) {
-  return static_cast<size_t>(tcdev::job_lds_bytes(static_cast<int>(seg), o.extrinsic_bits / 8));
+size_t lds_bytes_of(uint32_t seg, const tc_options& o, uint32_t waves = 1) {
+  return static_cast<size_t>(
+      tcdev::job_lds_bytes(static_cast<int>(seg), o.extrinsic_bits / 8, static_cast<int>(waves)));
 }
 
 }  // namespace
@@ -138,6 +181,7 @@ struct PlanDev {
   uint32_t njobs = 0, grid = 0;
   size_t lds_bytes = 0;
   bool lean = false;  // every job has the lean geometry (window_kernel)
+  uint32_t waves = 1;  // waves per job (work-group)
 };
 
 struct tc_plan : PlanDev {
@@ -234,13 +278,20 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
     uploads_done();
     return;
   }
-  // ---- window decoders: group by K, pack blocks into 64-lane jobs ----
+  // ---- window decoders: group by K, pack blocks into jobs of 64 lanes per
+  // wave, as many waves per job as the widest block needs ----
   std::map<uint32_t, std::vector<uint32_t>> by_k;
   for (size_t i = 0; i < n; ++i) by_k[k[i]].push_back(static_cast<uint32_t>(i));
+  p.waves = 1;
+  for (const auto& [kk, list] : by_k) {
+    (void)list;
+    p.waves = std::max(p.waves, geometry(kk, o).waves_per_block);
+  }
   std::vector<tcdev::WinJob> jobs;
   std::vector<uint32_t> cb_index, qtab;
   for (const auto& [kk, list] : by_k) {
-    const tc_segment_info g = geometry(kk, o);
+    tc_segment_info g = geometry(kk, o);
+    g.blocks_per_wave = tcdev::kWaveLanes * p.waves / g.segments;  // blocks per job
     const int qi = turbo_cpp::detail::qpp_index(kk);
     const uint32_t f1 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f1);
     const uint32_t f2 = static_cast<uint32_t>(turbo_cpp::detail::kQppTable[qi].f2);
@@ -264,17 +315,18 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   p.lds_bytes = 0;
   p.lean = !jobs.empty();
   for (const auto& j : jobs) {
-    p.lds_bytes = std::max<size_t>(p.lds_bytes, lds_bytes_of(j.seg, o));
-    p.lean = p.lean && lean_job(j);
+    p.lds_bytes = std::max<size_t>(p.lds_bytes, lds_bytes_of(j.seg, o, p.waves));
+    p.lean = p.lean && p.waves == 1 && lean_job(j);
   }
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, p.device), "hipGetDeviceProperties");
-  int per_cu = 0;
-  const void* fn = reinterpret_cast<const void*>(window_kernel(o, p.lean));
-  hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, tcdev::kWaveLanes, p.lds_bytes),
+  int per_cu = 0;  // work-groups (jobs) per CU
+  const void* fn = reinterpret_cast<const void*>(window_kernel(o, p.lean, p.waves));
+  hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, tcdev::kWaveLanes * p.waves, p.lds_bytes),
             "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   per_cu = std::max(per_cu, 1);
-  if (o.waves_per_cu) per_cu = std::min<int>(per_cu, static_cast<int>(o.waves_per_cu));
+  if (o.waves_per_cu)
+    per_cu = std::min<int>(per_cu, std::max<int>(1, static_cast<int>(o.waves_per_cu / p.waves)));
   p.grid = std::max<uint32_t>(1, std::min<uint32_t>(p.njobs, static_cast<uint32_t>(per_cu) * prop.multiProcessorCount));
   p.d_jobs = dev_alloc<tcdev::WinJob>(jobs.size());
   dev_upload(p.d_jobs, jobs);
@@ -282,7 +334,7 @@ void build_plan(tc_plan& p, const uint32_t* k, const uint32_t* filler, size_t n,
   dev_upload(p.d_cb_index, cb_index);
   p.d_qtab = dev_alloc<uint32_t>(qtab.size());
   dev_upload(p.d_qtab, qtab);
-  p.d_scratch = dev_alloc<float2>(static_cast<size_t>(p.grid) * tcdev::kSlotFloat2);
+  p.d_scratch = dev_alloc<float2>(static_cast<size_t>(p.grid) * p.waves * tcdev::kSlotFloat2);
   p.d_sched = dev_alloc<uint32_t>(2);
   hip_check(hipMemset(p.d_sched, 0, 2 * sizeof(uint32_t)), "hipMemset");
   std::vector<uint32_t> pw(tcdev::kCrcPowLen);
@@ -307,7 +359,8 @@ void launch(const PlanDev& p, const float* llr, float* post, int32_t* crc, uint3
     a.tb = tcdev::TbOut{};
     if (tb) a.tb = *tb;
     a.sched = p.d_sched;
-    hipLaunchKernelGGL(window_kernel(o, p.lean), dim3(p.grid), dim3(64), p.lds_bytes, s, a);
+    hipLaunchKernelGGL(window_kernel(o, p.lean, p.waves), dim3(p.grid), dim3(tcdev::kWaveLanes * p.waves),
+                       p.lds_bytes, s, a);
   }
   hip_check(hipGetLastError(), "kernel launch");
 }
@@ -376,7 +429,7 @@ void tchost::decode_one_block(uint8_t* area, size_t area_bytes, const uint32_t* 
     p.d_work = reinterpret_cast<float*>(area + kOneBlockWorkOff);
   } else {
     const tc_segment_info g = geometry(k, o);
-    work_bytes = tcdev::kSlotFloat2 * sizeof(float2);
+    work_bytes = g.waves_per_block * tcdev::kSlotFloat2 * sizeof(float2);
     fill_qtab(k, g.segment_length, h.cb.f1, h.cb.f2, h.qtab);  // padding entries stay zero
     h.job = make_job(g, h.cb.f1, h.cb.f2, 1, 0, 0);
     h.cb_index = 0;
@@ -388,8 +441,9 @@ void tchost::decode_one_block(uint8_t* area, size_t area_bytes, const uint32_t* 
     p.d_scratch = reinterpret_cast<float2*>(area + kOneBlockWorkOff);
     p.njobs = 1;
     p.grid = 1;
-    p.lds_bytes = lds_bytes_of(g.segment_length, o);
-    p.lean = lean_job(h.job);
+    p.waves = g.waves_per_block;
+    p.lds_bytes = lds_bytes_of(g.segment_length, o, p.waves);
+    p.lean = p.waves == 1 && lean_job(h.job);
   }
   if (kOneBlockWorkOff + work_bytes > area_bytes)
     throw Failure{TC_ERR_INVALID_ARGUMENT, "code block needs " + std::to_string(kOneBlockWorkOff + work_bytes) +
@@ -475,7 +529,7 @@ HostCalls& host_calls() {
 bool same_options(const tc_options& a, const tc_options& b) {
   return a.max_iterations == b.max_iterations && a.early_stop == b.early_stop && a.decoder == b.decoder &&
          a.min_segment == b.min_segment && a.warmup == b.warmup && a.waves_per_cu == b.waves_per_cu &&
-         a.extrinsic_bits == b.extrinsic_bits;
+         a.extrinsic_bits == b.extrinsic_bits && a.max_waves_per_block == b.max_waves_per_block;
 }
 
 }  // namespace
@@ -613,7 +667,7 @@ tc_status tc_decode_code_block(const tc_cb_desc* desc, const float* d0, size_t n
     HostCalls& h = host_calls();
     const uint32_t f = static_cast<uint32_t>(desc->filler_count);
     const OneKey key{k, f, o.decoder, o.max_iterations, o.min_segment, o.warmup * 2 + (o.early_stop != 0),
-                     o.waves_per_cu, o.extrinsic_bits};
+                     o.waves_per_cu, o.extrinsic_bits * 4 + o.max_waves_per_block};
     auto& slot = h.one_plans[key];
     if (!slot) {
       auto p = std::make_unique<tc_plan>();

@@ -25,9 +25,10 @@ struct CbInfo {
 constexpr uint32_t kNoBlock = 0xFFFFFFFFu;
 constexpr uint32_t kExactBlocksPerWave = 2;
 
-// Window decoder: a job is what one 64-lane wavefront decodes at a time:
+// Window decoder: a job is what one work-group of 64-lane wavefronts (one,
+// or two or three for blocks of more than 64 segments) decodes at a time:
 // nblk code blocks of the same K, each split into M segments of length S
-// (lane = block * M + segment).
+// (lane = block * M + segment, lanes numbered across the waves).
 constexpr int kWaveLanes = 64;
 constexpr int kSubWin = 8;       // W: sub-window of the beta recompute
 constexpr int kMaxSubWins = 24;  // nW <= 24, so S <= 192
@@ -90,19 +91,19 @@ struct WinArgs {
   int32_t early_stop;
   RxIn rx;  // n_tx == 0: the decoder input is llr
   TbOut tb;  // on == 0: posteriors only
-  // Job queue: [0] next job, [1] waves that found it empty.  Both read zero
-  // at launch; the last wave to leave puts them back to zero, so the kernel
+  // Job queue: [0] next job, [1] work-groups that found it empty.  Both read
+  // zero at launch; the last work-group to leave puts them back to zero, so the kernel
   // needs no clearing launch (and no memset node under graph capture).
   uint32_t* sched;
 };
 constexpr int kCrcPowLen = 6144;
 // LDS of one window job: the extrinsics held between half-iterations
-// [S][64] plus one dummy slot per lane (ex_bytes each: 2 for fp16, 4 for
+// [S][64 waves] plus one dummy slot per lane (ex_bytes each: 2 for fp16, 4 for
 // fp32) and one CRC partial per lane.  At S = 96 with fp16 this is 12.4 KiB,
 // so LDS does not keep a CU below the eight waves (two per SIMD) that the
 // register budget allows; fp32 (24.5 KiB) allows six.
-constexpr int job_lds_bytes(int s, int ex_bytes) {
-  return (s * kExRow + kWaveLanes) * ex_bytes + kWaveLanes * 4;
+constexpr int job_lds_bytes(int s, int ex_bytes, int waves = 1) {
+  return (s * kExRow + kWaveLanes) * ex_bytes * waves + kWaveLanes * 4 * waves;
 }
 constexpr int kExtrinsicBits = 16;  // window decoders' default held width
 

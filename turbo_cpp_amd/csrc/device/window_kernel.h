@@ -108,6 +108,7 @@ struct LogDomain {
   static constexpr float kZero = kNeg;
   static constexpr bool kShareSimd = true;  // alternate priority with the SIMD's other wave
   static constexpr bool kExact = false;
+  static constexpr int kWaves = 1;  // waves per job (Wide<D, NW>: NW)
   __device__ static __forceinline__ float from_log(float v) { return v * kLog2e; }
   // the staged parity LLR, as the branch metrics take it: scaled once per job
   // (the same product the branch metric would form at every use)
@@ -192,6 +193,7 @@ struct LinDomain {
   // measured slower with the alternating priority (3.23 -> 3.33 ms on 4096 x 6144)
   static constexpr bool kShareSimd = false;
   static constexpr bool kExact = false;
+  static constexpr int kWaves = 1;  // waves per job (Wide<D, NW>: NW)
   __device__ static __forceinline__ float from_log(float v) { return hw_exp2(v * kLog2e); }
   // The parity weight depends on the channel only, so it is staged once per
   // job: e^-|par| (y = par/2, y + y = par exactly) with the sign bit set
@@ -278,6 +280,7 @@ struct ExactDomain {
   static constexpr float kZero = kNeg;
   static constexpr bool kShareSimd = true;
   static constexpr bool kExact = true;
+  static constexpr int kWaves = 1;  // waves per job (Wide<D, NW>: NW)
   __device__ static __forceinline__ float from_log(float v) { return v; }
   __device__ static __forceinline__ float stage_par(float par) { return par; }
   __device__ static __forceinline__ Br branch(float sa, float par) { return {0.5f * (sa + par), 0.5f * (sa - par)}; }
@@ -328,6 +331,18 @@ struct ExactDomain {
   }
 };
 
+// A job on NW waves of one work-group (code blocks of more than 64
+// segments): D's arithmetic, with the job's lanes numbered 0 .. 64 NW - 1
+// across the waves.  Every row of the job's LDS extrinsics, of its staged
+// inputs and of its posteriors then holds 64 NW columns.  Waves are not in
+// lockstep, so the boundary metrics move between neighbouring segments
+// through their records in memory instead of DPP, and the passes that read a
+// neighbour's extrinsics finish before any lane overwrites its own.
+template <class D, int NW>
+struct Wide : D {
+  static constexpr int kWaves = NW;
+};
+
 // Tail beta in the log domain, the reference's rules (lte_turbo.cpp:357-378):
 // every state's forced three-step path to state 0, renormalised to max 0.
 __device__ __forceinline__ void tail_beta(float ts0, float ts1, float ts2, float tp0, float tp1, float tp2, float* beta) {
@@ -374,21 +389,22 @@ __device__ __forceinline__ uint32_t qtab_at(const uint32_t* qtab, uint32_t i) {
   return ((ConstU32*)qtab)[i];
 }
 
-// u: the table entry of step t (lower half only)
-template <int H>
+// u: the table entry of step t (lower half only); R: entries per row (the
+// job's lanes)
+template <int H, int R = kExRow>
 __device__ __forceinline__ uint32_t lds_addr_q(const Lane& L, uint32_t u, uint32_t qq, uint32_t t, uint32_t y) {
   if constexpr (H == 0) {
-    return t * kExRow + L.colb + qq;
+    return t * R + L.colb + qq;
   } else {
     uint32_t c = (u & 0xFFFFu) + y;
     c = (c >= L.M) ? c - L.M : c;
-    return (u >> 16) * kExRow + L.colb + c;
+    return (u >> 16) * R + L.colb + c;
   }
 }
-template <int H>
+template <int H, int R = kExRow>
 __device__ __forceinline__ uint32_t lds_addr(const Lane& L, const uint32_t* __restrict__ qtab, uint32_t qq,
                                              uint32_t t, uint32_t y) {
-  return lds_addr_q<H>(L, H == 1 ? qtab_at(qtab, L.qtab_off + t) : 0u, qq, t, y);
+  return lds_addr_q<H, R>(L, H == 1 ? qtab_at(qtab, L.qtab_off + t) : 0u, qq, t, y);
 }
 
 // Extrinsics held in LDS between half-iterations: X = _Float16 (saturated
@@ -420,13 +436,18 @@ struct Chunk {
 
 typedef __attribute__((address_space(1))) const float GlobalFloat;
 // A half's staged inputs are two planes of [t][lane] floats, kPlane apart:
-// systematic (+ a-priori source) and parity.
+// systematic (+ a-priori source) and parity.  A job on NW waves has rows of
+// 64 NW floats and planes NW times as large (plane_of<R>).
 constexpr uint32_t kPlane = kMaxSeg * kWaveLanes;
+template <int R>
+constexpr uint32_t plane_of() {
+  return kMaxSeg * R;
+}
 
-template <bool FULL = false>
+template <bool FULL = false, int R = kWaveLanes>
 __device__ __forceinline__ void load_chunk(const float* __restrict__ sc, uint32_t col, uint32_t lo, uint32_t len,
                                            Chunk& c) {
-  const GlobalFloat* base = (const GlobalFloat*)(sc + lo * kWaveLanes + col);
+  const GlobalFloat* base = (const GlobalFloat*)(sc + lo * R + col);
   // Opaque to the optimiser: without this the per-lane 64-bit address of
   // every (chunk, step, column) combination is hoisted out of the loops and
   // spilled.  One address pair per chunk; steps use immediate offsets.  The
@@ -436,12 +457,12 @@ __device__ __forceinline__ void load_chunk(const float* __restrict__ sc, uint32_
   asm volatile("" : "+v"(base));
 #pragma unroll
   for (int j = 0; j < kSubWin; ++j)
-    if (FULL || (uint32_t)j < len) c.v[j] = make_float2(base[j * kWaveLanes], base[kPlane + j * kWaveLanes]);
+    if (FULL || (uint32_t)j < len) c.v[j] = make_float2(base[j * R], base[plane_of<R>() + j * R]);
 }
 
 // LDS extrinsics of the chunk, issued in the order the pass consumes them;
 // addr (may be null) keeps the addresses for the write-back.
-template <int H, bool BACKWARD, bool FULL, class X>
+template <int H, bool BACKWARD, bool FULL, class X, int R = kExRow>
 __device__ __forceinline__ void load_chunk_lds(const Lane& L, const X* __restrict__ ex,
                                                const uint32_t* __restrict__ qtab, uint32_t qq, uint32_t by,
                                                uint32_t lo, uint32_t len, Chunk& c, uint32_t* addr) {
@@ -462,7 +483,7 @@ __device__ __forceinline__ void load_chunk_lds(const Lane& L, const X* __restric
   for (int jj = 0; jj < kSubWin; ++jj) {
     const int j = BACKWARD ? kSubWin - 1 - jj : jj;
     if (FULL || (uint32_t)j < len) {
-      const uint32_t a = lds_addr_q<H>(L, qv[j], qq, lo + j, y);
+      const uint32_t a = lds_addr_q<H, R>(L, qv[j], qq, lo + j, y);
       c.e[j] = static_cast<float>(ex[a]);
       if (addr) addr[j] = a;
       if constexpr (H == 1) y = BACKWARD ? y_back(y, by, L.M) : y_fwd(y, by, L.M);
@@ -481,13 +502,14 @@ __device__ __forceinline__ void beta_pass(const Lane& L, const X* __restrict__ e
                                           uint32_t qq, uint32_t col, uint32_t lo, uint32_t hi, float* bc,
                                           uint32_t hand_t, OnChunk&& on_chunk, OnHand&& on_hand) {
   if (hi <= lo) return;
+  constexpr int R = kWaveLanes * D::kWaves;
   const uint32_t n = (hi - lo + kSubWin - 1) / kSubWin;
   const uint32_t by = (H == 1) ? qpp_stride(L, qq) : 0u;
   Chunk nxt;
   {
     const uint32_t c0 = lo + (n - 1) * kSubWin;
-    load_chunk(sc, col, c0, hi - c0, nxt);
-    load_chunk_lds<H, true, false>(L, ex, qtab, qq, by, c0, hi - c0, nxt, nullptr);
+    load_chunk<false, R>(sc, col, c0, hi - c0, nxt);
+    load_chunk_lds<H, true, false, X, R>(L, ex, qtab, qq, by, c0, hi - c0, nxt, nullptr);
   }
   for (int c = (int)n - 1; c >= 0; --c) {
     if constexpr (D::kShareSimd) share_simd();
@@ -495,8 +517,8 @@ __device__ __forceinline__ void beta_pass(const Lane& L, const X* __restrict__ e
     const uint32_t len = min((uint32_t)kSubWin, hi - clo);
     const Chunk cur = nxt;
     if (c > 0) {
-      load_chunk<true>(sc, col, clo - kSubWin, kSubWin, nxt);
-      load_chunk_lds<H, true, true>(L, ex, qtab, qq, by, clo - kSubWin, kSubWin, nxt, nullptr);
+      load_chunk<true, R>(sc, col, clo - kSubWin, kSubWin, nxt);
+      load_chunk_lds<H, true, true, X, R>(L, ex, qtab, qq, by, clo - kSubWin, kSubWin, nxt, nullptr);
     }
     on_chunk((uint32_t)c, bc);
     if (len == kSubWin && (hand_t <= clo || hand_t >= clo + kSubWin)) {
@@ -526,19 +548,20 @@ __device__ __forceinline__ void alpha_warmup(const Lane& L, const X* __restrict_
                                              const float* __restrict__ sc, const uint32_t* __restrict__ qtab,
                                              uint32_t qq, uint32_t col, uint32_t lo, uint32_t hi, float* ac) {
   if (hi <= lo) return;
+  constexpr int R = kWaveLanes * D::kWaves;
   const uint32_t by = (H == 1) ? qpp_stride(L, qq) : 0u;
   const uint32_t n = (hi - lo + kSubWin - 1) / kSubWin;
   Chunk nxt;
-  load_chunk(sc, col, lo, min((uint32_t)kSubWin, hi - lo), nxt);
-  load_chunk_lds<H, false, false>(L, ex, qtab, qq, by, lo, min((uint32_t)kSubWin, hi - lo), nxt, nullptr);
+  load_chunk<false, R>(sc, col, lo, min((uint32_t)kSubWin, hi - lo), nxt);
+  load_chunk_lds<H, false, false, X, R>(L, ex, qtab, qq, by, lo, min((uint32_t)kSubWin, hi - lo), nxt, nullptr);
   for (uint32_t c = 0; c < n; ++c) {
     const uint32_t clo = lo + c * kSubWin;
     const uint32_t len = min((uint32_t)kSubWin, hi - clo);
     const Chunk cur = nxt;
     if (c + 1 < n) {
       const uint32_t nlen = min((uint32_t)kSubWin, hi - clo - kSubWin);
-      load_chunk(sc, col, clo + kSubWin, nlen, nxt);
-      load_chunk_lds<H, false, false>(L, ex, qtab, qq, by, clo + kSubWin, nlen, nxt, nullptr);
+      load_chunk<false, R>(sc, col, clo + kSubWin, nlen, nxt);
+      load_chunk_lds<H, false, false, X, R>(L, ex, qtab, qq, by, clo + kSubWin, nlen, nxt, nullptr);
     }
     if (len == kSubWin) {
 #pragma unroll
@@ -597,10 +620,14 @@ __device__ __forceinline__ void alpha_subwindow(const Lane& L, X* __restrict__ e
   }
   auto hand = [&]() {
     // uniform: hand alpha at step S-g to the next segment
-    float na[8];
+    if constexpr (D::kWaves == 1) {
+      float na[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) na[i] = dpp_from_prev_lane(ac[i]);
-    if (!L.first) store8(nii, na);
+      for (int i = 0; i < 8; ++i) na[i] = dpp_from_prev_lane(ac[i]);
+      if (!L.first) store8(nii, na);
+    } else {
+      if (L.active && !L.last) store8(nii, ac);  // nii: the next segment's record
+    }
   };
   if (FULL && hand_t == t0) hand();
 #pragma unroll
@@ -628,16 +655,17 @@ __device__ __forceinline__ void extrinsic_pass(const Lane& L, X* __restrict__ ex
                                                const uint32_t* __restrict__ qtab, const float* __restrict__ ck,
                                                float* ac, uint32_t hand_t, float* __restrict__ nii,
                                                float* __restrict__ post_sc, const EarlyStop& es) {
+  constexpr int R = kWaveLanes * D::kWaves;
   const uint32_t S = L.S;
   const uint32_t lane = threadIdx.x;
   const uint32_t by = (H == 1) ? qpp_stride(L, L.q) : 0u;
-  const uint32_t dummy = S * kExRow + lane;
+  const uint32_t dummy = S * R + lane;
   const bool wr = L.active && !es.done;
   // global loads (staged LLRs, beta checkpoint) run one sub-window ahead;
   // the LDS extrinsics of a sub-window are read at its start
   Chunk nxt;
   float nck[8];
-  load_chunk(sc, L.le, 0, min((uint32_t)kSubWin, S), nxt);
+  load_chunk<false, R>(sc, L.le, 0, min((uint32_t)kSubWin, S), nxt);
   load8(ck, nck);
   for (uint32_t w = 0; w < L.nW; ++w) {
     if constexpr (D::kShareSimd) share_simd();
@@ -650,15 +678,15 @@ __device__ __forceinline__ void extrinsic_pass(const Lane& L, X* __restrict__ ex
 #pragma unroll
     for (int i = 0; i < 8; ++i) bb[i] = nck[i];
     if (w + 1 < L.nW) {
-      load_chunk(sc, L.le, t0 + kSubWin, min((uint32_t)kSubWin, S - t0 - kSubWin), nxt);
+      load_chunk<false, R>(sc, L.le, t0 + kSubWin, min((uint32_t)kSubWin, S - t0 - kSubWin), nxt);
       load8(ck + 8 * (w + 1), nck);
     }
     uint32_t addr[kSubWin];
     if (len == kSubWin && (hand_t <= t0 || hand_t >= t0 + kSubWin)) {
-      load_chunk_lds<H, true, true>(L, ex, qtab, L.q, by, t0, len, cur, addr);
+      load_chunk_lds<H, true, true, X, R>(L, ex, qtab, L.q, by, t0, len, cur, addr);
       alpha_subwindow<D, H, ES, true>(L, ex, cur, addr, bb, ac, t0, len, hand_t, nii, dummy, wr, post_sc, es);
     } else {
-      load_chunk_lds<H, true, false>(L, ex, qtab, L.q, by, t0, len, cur, addr);
+      load_chunk_lds<H, true, false, X, R>(L, ex, qtab, L.q, by, t0, len, cur, addr);
       alpha_subwindow<D, H, ES, false>(L, ex, cur, addr, bb, ac, t0, len, hand_t, nii, dummy, wr, post_sc, es);
     }
   }
@@ -669,14 +697,26 @@ __device__ __forceinline__ void extrinsic_pass(const Lane& L, X* __restrict__ ex
 // (alpha at step -g, beta at step S+g of the segment); ck: this lane's beta
 // checkpoints at sub-window boundaries, [w][8].  The lower half also leaves
 // each bit's posterior in post_sc (fp32, at the bit's extrinsic address).
+// A job on several waves (D::kWaves > 1) reads its boundary metrics from nii
+// and hands its own to the neighbours' records at push (the records of the
+// next iteration, kRecFloats apart per lane, at this half's offset).
+constexpr uint32_t kRecFloats = 32;  // per-lane record: [half][alpha 8 | beta 8]
 template <class D, int H, bool ES, class X>
 __device__ __forceinline__ void siso_half(const Lane& L, X* __restrict__ ex, const float* __restrict__ sc,
                                           const uint32_t* __restrict__ qtab, float* __restrict__ nii,
                                           float* __restrict__ ck, float* __restrict__ post_sc,
-                                          const EarlyStop& es) {
+                                          const EarlyStop& es, float* __restrict__ push = nullptr) {
   const uint32_t S = L.S, g = L.g;
   const uint32_t lane = threadIdx.x;
   constexpr uint32_t kNever = 0xFFFFFFFFu;
+  // where this lane's hand-offs go: alpha to the next segment, beta to the
+  // previous one (one wave: its own record, the value pulled by DPP)
+  float* a_to = nii;
+  float* b_to = nii + 8;
+  if constexpr (D::kWaves > 1) {
+    a_to = L.last ? nii : push + (L.le + 1) * kRecFloats;
+    b_to = L.first ? nii + 8 : push + (L.le - 1) * kRecFloats + 8;
+  }
   // ---------------- backward pass: warm-up over the next segment's first g
   // steps, then the segment, storing beta at sub-window boundaries
   float bc[8];
@@ -693,10 +733,14 @@ __device__ __forceinline__ void siso_half(const Lane& L, X* __restrict__ ex, con
       [&](const float* b) {
         // the whole wave hands beta at step g to the previous segment (used
         // next iteration); the last segment keeps its tail beta
-        float nb[8];
+        if constexpr (D::kWaves == 1) {
+          float nb[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) nb[i] = dpp_from_next_lane(b[i]);
-        if (!L.last) store8(nii + 8, nb);
+          for (int i = 0; i < 8; ++i) nb[i] = dpp_from_next_lane(b[i]);
+          if (!L.last) store8(b_to, nb);
+        } else {
+          if (L.active && !L.first) store8(b_to, b);
+        }
       });
   // ---------------- forward pass: warm-up over the previous segment's last g
   // steps, then per sub-window: recompute beta, alpha + extrinsic
@@ -708,7 +752,10 @@ __device__ __forceinline__ void siso_half(const Lane& L, X* __restrict__ ex, con
     alpha_warmup<D, H>(L, ex, sc, qtab, ql, col, S - g, S, ac);
     if (L.first) load8(nii, ac);
   }
-  extrinsic_pass<D, H, ES>(L, ex, sc, qtab, ck, ac, g > 0 ? S - g : kNever, nii, post_sc, es);
+  // several waves: every warm-up has read its neighbours' extrinsics before
+  // any lane overwrites its own (one wave runs in lockstep)
+  if constexpr (D::kWaves > 1) __syncthreads();
+  extrinsic_pass<D, H, ES>(L, ex, sc, qtab, ck, ac, g > 0 ? S - g : kNever, a_to, post_sc, es);
 }
 
 // ------------------------------------------------------------- lean path --
@@ -1172,6 +1219,7 @@ __device__ __forceinline__ void siso_half_exact(const Lane& L, X* __restrict__ e
 // mod G), then the first lane of each block XORs its M partials.  Returns this
 // lane's block verdict.  Reading the posteriors after the half keeps the
 // early-stop work out of the recursion loops.
+template <int R = kExRow>
 __device__ __forceinline__ bool crc24b_job(const Lane& L, const float* __restrict__ post_sc,
                                            uint32_t* __restrict__ partial, const uint32_t* __restrict__ crc_pow,
                                            uint32_t filler) {
@@ -1182,7 +1230,7 @@ __device__ __forceinline__ bool crc24b_job(const Lane& L, const float* __restric
     float pv[kSubWin];
 #pragma unroll
     for (uint32_t j = 0; j < kSubWin; ++j)
-      pv[j] = (t0 + j < S) ? post_sc[(t0 + j) * kExRow + L.colb + L.q] : -1.0f;
+      pv[j] = (t0 + j < S) ? post_sc[(t0 + j) * R + L.colb + L.q] : -1.0f;
 #pragma unroll
     for (uint32_t j = 0; j < kSubWin; ++j) {
       if (t0 + j < S) {
@@ -1342,10 +1390,20 @@ __device__ __forceinline__ void tb_epilogue(const WinArgs& A, uint32_t cb, const
 // Jobs are taken from a counter rather than by a fixed stride: waves that
 // share a SIMD run slower than waves alone on one, and early-stopped blocks
 // finish early, so a fixed assignment leaves the fast waves idle at the end.
+template <int NW = 1>
 __device__ __forceinline__ uint32_t next_job(uint32_t* sched) {
-  uint32_t j = 0;
-  if (threadIdx.x == 0) j = atomicAdd(sched, 1u);
-  return __builtin_amdgcn_readfirstlane(j);
+  if constexpr (NW == 1) {
+    uint32_t j = 0;
+    if (threadIdx.x == 0) j = atomicAdd(sched, 1u);
+    return __builtin_amdgcn_readfirstlane(j);
+  } else {
+    // one ticket per work-group, passed to its waves through LDS
+    __shared__ uint32_t ticket;
+    __syncthreads();  // every wave has read the previous ticket
+    if (threadIdx.x == 0) ticket = atomicAdd(sched, 1u);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(ticket);
+  }
 }
 // Every wave has taken its last (empty) ticket before it counts itself out,
 // so the last one out can reset the queue for the next launch.
@@ -1364,7 +1422,10 @@ __device__ __forceinline__ void leave_queue(uint32_t* sched) {
 // row across the columns are both conflict-free.  Global accesses are then
 // runs of consecutive floats on the natural side and whole rows on the
 // scratch side.
-__device__ __forceinline__ uint32_t tile_at(uint32_t u, uint32_t c) { return u * kWaveLanes + (c ^ (u & 63u)); }
+template <int R = kWaveLanes>
+__device__ __forceinline__ uint32_t tile_at(uint32_t u, uint32_t c) {
+  return u * R + (c ^ (u & 63u));
+}
 constexpr uint32_t kTileBatch = 16;
 
 // One-block job (M segments): the decoder input of the block into the scratch
@@ -1375,23 +1436,24 @@ template <class D>
 __device__ __forceinline__ void stage_tiled(const WinArgs& A, const Lane& L, const CbInfo& ci, const RxBlk& B,
                                             bool fused, const uint32_t* __restrict__ qtab, float* __restrict__ tile,
                                             float* __restrict__ up, float* __restrict__ lo) {
+  constexpr int R = kWaveLanes * D::kWaves;  // the job's lanes: columns of a row
   const uint32_t lane = threadIdx.x, S = L.S, M = L.M, D4 = L.K + 4;
-  const uint32_t R = (S + 1) / 2;
+  const uint32_t rows = (S + 1) / 2;
   float* keep = (fused && A.rx.keep) ? A.rx.keep + ci.llr_off : nullptr;
   const uint32_t by = qpp_stride(L, L.q);
   for (uint32_t sgm = 0; sgm < 3; ++sgm) {
-    for (uint32_t r0 = 0; r0 < S; r0 += R) {
-      const uint32_t rp = min(R, S - r0);
+    for (uint32_t r0 = 0; r0 < S; r0 += rows) {
+      const uint32_t rp = min(rows, S - r0);
       const uint32_t mg = 0xFFFFFFFFu / rp + 1u;  // ceil(2^32 / rp): exact idx / rp for idx < 2^16
       const uint32_t n = M * rp;
-      for (uint32_t base = 0; base < n; base += kTileBatch * kWaveLanes) {
+      for (uint32_t base = 0; base < n; base += kTileBatch * R) {
         uint32_t pos[kTileBatch], tix[kTileBatch], mask = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kTileBatch; ++k) {
-          const uint32_t idx = base + k * kWaveLanes + lane;
+          const uint32_t idx = base + k * R + lane;
           const uint32_t q = __umulhi(idx, mg), u = idx - q * rp;
           pos[k] = sgm * D4 + q * S + r0 + u;
-          tix[k] = tile_at(u, q);
+          tix[k] = tile_at<R>(u, q);
           mask |= (idx < n ? 1u : 0u) << k;
         }
         float v[kTileBatch];
@@ -1411,16 +1473,16 @@ __device__ __forceinline__ void stage_tiled(const WinArgs& A, const Lane& L, con
       }
       __syncthreads();
       // rows r0 .. r0+rp-1 of this lane's column
-      float* dst = (sgm == 0) ? up : (sgm == 1 ? up + kPlane : lo + kPlane);
+      float* dst = (sgm == 0) ? up : (sgm == 1 ? up + plane_of<R>() : lo + plane_of<R>());
       for (uint32_t u0 = 0; u0 < rp; u0 += kSubWin) {
         float v[kSubWin];
 #pragma unroll
         for (uint32_t k = 0; k < kSubWin; ++k)
-          if (u0 + k < rp) v[k] = tile[tile_at(u0 + k, lane)];
+          if (u0 + k < rp) v[k] = tile[tile_at<R>(u0 + k, lane)];
 #pragma unroll
         for (uint32_t k = 0; k < kSubWin; ++k) {
           if (u0 + k < rp && L.active)
-            dst[(r0 + u0 + k) * kWaveLanes + lane] = (sgm == 0) ? v[k] : D::stage_par(v[k]);
+            dst[(r0 + u0 + k) * R + lane] = (sgm == 0) ? v[k] : D::stage_par(v[k]);
         }
       }
       if (sgm == 0) {
@@ -1433,8 +1495,8 @@ __device__ __forceinline__ void stage_tiled(const WinArgs& A, const Lane& L, con
           if (row >= r0 && row < r0 + rp) {
             uint32_t c = (qv & 0xFFFFu) + y;
             c = (c >= M) ? c - M : c;
-            const float x = tile[tile_at(row - r0, c)];
-            if (L.active) lo[t * kWaveLanes + lane] = x;
+            const float x = tile[tile_at<R>(row - r0, c)];
+            if (L.active) lo[t * R + lane] = x;
           }
           y = y_fwd(y, by, M);
         }
@@ -1444,32 +1506,33 @@ __device__ __forceinline__ void stage_tiled(const WinArgs& A, const Lane& L, con
   }
 }
 
-// One-block job: posteriors from the scratch rows (post_sc) to dst in
-// natural order, zero when no iteration ran.
+// One-block job: posteriors from the scratch rows (post_sc, R per row) to
+// dst in natural order, zero when no iteration ran.
+template <int R = kWaveLanes>
 __device__ __forceinline__ void copy_out_tiled(const Lane& L, bool any, const float* __restrict__ post_sc,
                                                float* __restrict__ tile, float* __restrict__ dst) {
   const uint32_t lane = threadIdx.x, S = L.S, M = L.M;
-  const uint32_t R = (S + 1) / 2;
-  for (uint32_t r0 = 0; r0 < S; r0 += R) {
-    const uint32_t rp = min(R, S - r0);
+  const uint32_t rows = (S + 1) / 2;
+  for (uint32_t r0 = 0; r0 < S; r0 += rows) {
+    const uint32_t rp = min(rows, S - r0);
     for (uint32_t u0 = 0; u0 < rp; u0 += kTileBatch) {
       float v[kTileBatch];
 #pragma unroll
       for (uint32_t k = 0; k < kTileBatch; ++k)
-        if (u0 + k < rp) v[k] = post_sc[(r0 + u0 + k) * kExRow + lane];
+        if (u0 + k < rp) v[k] = post_sc[(r0 + u0 + k) * R + lane];
 #pragma unroll
       for (uint32_t k = 0; k < kTileBatch; ++k)
-        if (u0 + k < rp) tile[tile_at(u0 + k, lane)] = v[k];
+        if (u0 + k < rp) tile[tile_at<R>(u0 + k, lane)] = v[k];
     }
     __syncthreads();
     const uint32_t mg = 0xFFFFFFFFu / rp + 1u;
     const uint32_t n = M * rp;
-    for (uint32_t base = 0; base < n; base += kTileBatch * kWaveLanes) {
+    for (uint32_t base = 0; base < n; base += kTileBatch * R) {
 #pragma unroll
       for (uint32_t k = 0; k < kTileBatch; ++k) {
-        const uint32_t idx = base + k * kWaveLanes + lane;
+        const uint32_t idx = base + k * R + lane;
         const uint32_t q = __umulhi(idx, mg), u = idx - q * rp;
-        if (idx < n) dst[q * S + r0 + u] = any ? tile[tile_at(u, q)] : 0.0f;
+        if (idx < n) dst[q * S + r0 + u] = any ? tile[tile_at<R>(u, q)] : 0.0f;
       }
     }
     __syncthreads();
@@ -1480,11 +1543,17 @@ __device__ __forceinline__ void copy_out_tiled(const Lane& L, bool any, const fl
 // LEAN: every job of the launch has the lean geometry (siso_half_lean); the
 // host picks such a kernel only for plans whose jobs all have it, so that
 // neither path's registers constrain the other's.
+// A job on several waves (D::kWaves > 1) keeps two sets of boundary records,
+// [lane][kRecFloats] each, from recs: iteration it reads set it mod 2 and hands
+// its metrics to the other (nii: this lane's record in set 0).
 template <class D, bool ES, class X, bool LEAN = false>
 __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X* __restrict__ ex,
                                            float* __restrict__ sc_up, float* __restrict__ sc_lo,
                                            float* __restrict__ nii, float* __restrict__ ck,
-                                           float* __restrict__ post_sc, float* __restrict__ ack) {
+                                           float* __restrict__ post_sc, float* __restrict__ ack,
+                                           float* __restrict__ recs = nullptr) {
+  constexpr int R = kWaveLanes * D::kWaves;  // the job's lanes
+  constexpr uint32_t kRecSet = R * kRecFloats;
   const uint32_t lane = threadIdx.x;
   const uint32_t* __restrict__ cb_index = A.cb_index;
   const CbInfo* __restrict__ cbs = A.cbs;
@@ -1511,9 +1580,9 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
   const uint32_t my_cb = cb_index[J.cb_off + b];
   const CbInfo ci = cbs[my_cb];
   const bool fused = A.rx.n_tx > 0;
-  // LDS: extrinsics [S][64] + one dummy slot per lane, then the CRC
+  // LDS: extrinsics [S][R] + one dummy slot per lane, then the CRC
   // partials (early stop only)
-  const uint32_t ex_slots = S * kExRow + kWaveLanes;
+  const uint32_t ex_slots = S * R + R;
   uint32_t* partial = reinterpret_cast<uint32_t*>(ex + ex_slots);
   RxBlk Bm{};
   if (fused) rx_block(A.rx, my_cb, Bm);
@@ -1550,9 +1619,9 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
       for (uint32_t u = 0; u < kStageSteps; ++u) {
         const uint32_t t = t0 + u;
         if (((mask >> (3 * u)) & 1u) == 0) continue;
-        sc_up[t * kWaveLanes + lane] = v[3 * u];
-        sc_up[kPlane + t * kWaveLanes + lane] = D::stage_par(v[3 * u + 1]);
-        sc_lo[kPlane + t * kWaveLanes + lane] = D::stage_par(v[3 * u + 2]);
+        sc_up[t * R + lane] = v[3 * u];
+        sc_up[plane_of<R>() + t * R + lane] = D::stage_par(v[3 * u + 1]);
+        sc_lo[plane_of<R>() + t * R + lane] = D::stage_par(v[3 * u + 2]);
         if (keep) {
           keep[pos[3 * u]] = v[3 * u];
           keep[pos[3 * u + 1]] = v[3 * u + 1];
@@ -1578,6 +1647,10 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
       for (int s = 0; s < 8; ++s) {
         nii[h * 16 + s] = L.first ? (s == 0 ? D::kUniform : D::kZero) : D::kUniform;
         nii[h * 16 + 8 + s] = L.last ? D::from_log(tb[h][s]) : D::kUniform;
+        if constexpr (D::kWaves > 1) {  // both record sets
+          nii[kRecSet + h * 16 + s] = nii[h * 16 + s];
+          nii[kRecSet + h * 16 + 8 + s] = nii[h * 16 + 8 + s];
+        }
       }
     }
   }
@@ -1601,30 +1674,33 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
 #pragma unroll
       for (uint32_t u = 0; u < kSubWin; ++u) {
         if (t0 + u < S) {
-          xp[u] = sc_up[lds_addr<1>(L, qtab, L.q, t0 + u, y)];
+          xp[u] = sc_up[lds_addr<1, R>(L, qtab, L.q, t0 + u, y)];
           y = y_fwd(y, by, M);
         }
       }
 #pragma unroll
       for (uint32_t u = 0; u < kSubWin; ++u)
-        if (t0 + u < S && L.active) sc_lo[(t0 + u) * kWaveLanes + lane] = xp[u];
+        if (t0 + u < S && L.active) sc_lo[(t0 + u) * R + lane] = xp[u];
     }
   }
   {
     uint32_t* exw = reinterpret_cast<uint32_t*>(ex);
-    for (uint32_t i = lane; i < ex_slots * sizeof(X) / 4; i += kWaveLanes) exw[i] = 0u;
+    for (uint32_t i = lane; i < ex_slots * sizeof(X) / 4; i += R) exw[i] = 0u;
   }
   __syncthreads();
 
   EarlyStop es{es_on, false};
   uint32_t used_it = A.max_iterations;
   for (uint32_t it = 0; it < A.max_iterations; ++it) {
+    // several waves: this iteration's records and the next one's
+    float* const own = (D::kWaves > 1 && (it & 1u)) ? nii + kRecSet : nii;
+    float* const push = (D::kWaves > 1) ? recs + ((it & 1u) ? 0u : kRecSet) : nullptr;
     if constexpr (D::kExact) {
       siso_half_exact<0, ES>(L, ex, sc_up, qtab, nii, ck, ack, post_sc, es);
     } else if constexpr (LEAN) {
       siso_half_lean<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
     } else {
-      siso_half<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
+      siso_half<D, 0, ES>(L, ex, sc_up, qtab, own, ck, post_sc, es, push);
     }
     __syncthreads();
     // every lower half leaves the posteriors in post_sc; a block that has
@@ -1634,11 +1710,11 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
     } else if constexpr (LEAN) {
       siso_half_lean<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
     } else {
-      siso_half<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
+      siso_half<D, 1, ES>(L, ex, sc_lo, qtab, own + 16, ck, post_sc, es, D::kWaves > 1 ? push + 16 : nullptr);
     }
     __syncthreads();
     if (es_on) {
-      const bool ok = crc24b_job(L, post_sc, partial, A.crc_pow, ci.filler);
+      const bool ok = crc24b_job<R>(L, post_sc, partial, A.crc_pow, ci.filler);
       if (ok && !es.done) {
         es.done = true;
         used_it = it + 1;
@@ -1654,21 +1730,21 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
   // ---- posterior out, natural order (zero when no iteration ran), with
   // kCopyBatch loads in flight per lane ----
   if (J.nblk == 1)
-    copy_out_tiled(L, A.max_iterations > 0, post_sc, reinterpret_cast<float*>(ex), A.post + ci.post_off);
+    copy_out_tiled<R>(L, A.max_iterations > 0, post_sc, reinterpret_cast<float*>(ex), A.post + ci.post_off);
   else for (uint32_t bb = 0; bb < J.nblk; ++bb) {
     float* dst = A.post + cbs[cb_index[J.cb_off + bb]].post_off;
-    for (uint32_t base = 0; base < K; base += kCopyBatch * kWaveLanes) {
+    for (uint32_t base = 0; base < K; base += kCopyBatch * R) {
       float v[kCopyBatch];
 #pragma unroll
       for (uint32_t u = 0; u < kCopyBatch; ++u) {
-        const uint32_t i = base + u * kWaveLanes + lane;
+        const uint32_t i = base + u * R + lane;
         const uint32_t qq = __umulhi(i, J.div_magic);
         const uint32_t t = i - qq * S;
-        v[u] = (i < K && A.max_iterations) ? post_sc[t * kExRow + bb * M + qq] : 0.0f;
+        v[u] = (i < K && A.max_iterations) ? post_sc[t * R + bb * M + qq] : 0.0f;
       }
 #pragma unroll
       for (uint32_t u = 0; u < kCopyBatch; ++u) {
-        const uint32_t i = base + u * kWaveLanes + lane;
+        const uint32_t i = base + u * R + lane;
         if (i < K) dst[i] = v[u];
       }
     }
@@ -1680,33 +1756,44 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
   if (A.tb.on) {
     __threadfence_block();  // this wave's posterior and iteration stores before it reads them back
     __syncthreads();
-    for (uint32_t bb = 0; bb < J.nblk; ++bb) {
-      const uint32_t cbb = cb_index[J.cb_off + bb];
-      const CbInfo cbi = cbs[cbb];
-      tb_epilogue(A, cbb, A.post + cbi.post_off, K, cbi.filler);
+    // one wave assembles (the job's first, when it has several)
+    if (D::kWaves == 1 || lane < (uint32_t)kWaveLanes) {
+      for (uint32_t bb = 0; bb < J.nblk; ++bb) {
+        const uint32_t cbb = cb_index[J.cb_off + bb];
+        const CbInfo cbi = cbs[cbb];
+        tb_epilogue(A, cbb, A.post + cbi.post_off, K, cbi.filler);
+      }
     }
   }
   __syncthreads();
 }
 
+// A work-group of D::kWaves waves runs one job at a time; its scratch slot is
+// kSlotFloat2 per wave.
 template <class D, bool ES, class X, bool LEAN = false>
 __device__ __forceinline__ void window_decode(const WinArgs& A) {
   extern __shared__ float lds[];
+  constexpr int NW = D::kWaves, R = kWaveLanes * NW;
   const uint32_t lane = threadIdx.x;
   // staged inputs: two planes per half
-  float* sc_up = reinterpret_cast<float*>(A.scratch + (size_t)blockIdx.x * kSlotFloat2);
-  float* sc_lo = sc_up + 2 * kPlane;
-  // per-lane records: boundary metrics [lane][half][16], checkpoints [lane][w][8]
-  float* recs = sc_lo + 2 * kPlane;
-  float* nii = recs + lane * 32;
-  float* ck = recs + kWaveLanes * 32 + lane * (8 * kMaxSubWins);
-  // posteriors [S][64] + one dummy slot per lane
-  float* post_sc = recs + kWaveLanes * 32 + kWaveLanes * 8 * kMaxSubWins;
-  // alpha checkpoints [lane][w][8] (exact segment boundaries)
-  float* ack = post_sc + (kMaxSeg + 1) * kWaveLanes + lane * (8 * kMaxSubWins);
-  for (uint32_t jb = next_job(A.sched); jb < A.njobs; jb = next_job(A.sched)) {
+  float* sc_up = reinterpret_cast<float*>(A.scratch + (size_t)blockIdx.x * NW * kSlotFloat2);
+  float* sc_lo = sc_up + 2 * plane_of<R>();
+  // per-lane records: boundary metrics [lane][half][16] (two sets on several
+  // waves), checkpoints [lane][w][8]
+  constexpr uint32_t kRecs = (NW > 1 ? 2 : 1) * R * kRecFloats;
+  float* recs = sc_lo + 2 * plane_of<R>();
+  float* nii = recs + lane * kRecFloats;
+  float* ck = recs + kRecs + lane * (8 * kMaxSubWins);
+  // posteriors [S][R] + one dummy slot per lane
+  float* post_sc = recs + kRecs + R * 8 * kMaxSubWins;
+  // alpha checkpoints [lane][w][8] (exact segment boundaries; one wave only)
+  float* ack = post_sc + (kMaxSeg + 1) * R + lane * (8 * kMaxSubWins);
+  static_assert(4 * plane_of<R>() + kRecs + R * 8 * kMaxSubWins + (kMaxSeg + 1) * R +
+                        (NW == 1 ? R * 8 * kMaxSubWins : 0) <= 2 * NW * kSlotFloat2,
+                "the job's scratch fits its waves' slots");
+  for (uint32_t jb = next_job<NW>(A.sched); jb < A.njobs; jb = next_job<NW>(A.sched)) {
     const WinJob J = A.jobs[jb];
-    window_job<D, ES, X, LEAN>(A, J, reinterpret_cast<X*>(lds), sc_up, sc_lo, nii, ck, post_sc, ack);
+    window_job<D, ES, X, LEAN>(A, J, reinterpret_cast<X*>(lds), sc_up, sc_lo, nii, ck, post_sc, ack, recs);
   }
   leave_queue(A.sched);
 }
