@@ -74,3 +74,11 @@ def test_noalloc_rejects_bad_sizes():
     st = L.tc_decode_code_block_noalloc(C.byref(d), llr[0], 44, llr[1], 44, llr[2], 44, C.byref(opts), post,
                                         C.byref(ok), C.byref(it))
     assert st == T.TC_ERR_OUT_OF_RANGE
+    # a block on several waves needs that many waves' work space: more than the static array holds
+    big = _block(6144, 1.0, 2)
+    d = T.TcCbDesc(0, 6144, 6148, 0, 0, 0)
+    post = np.zeros(6144, np.float32)
+    opts = T.options(T.DECODER_WINDOW_LOG, 8, False, 32, 16, 0, 0, 3)
+    st = L.tc_decode_code_block_noalloc(C.byref(d), big[0], 6148, big[1], 6148, big[2], 6148, C.byref(opts), post,
+                                        C.byref(ok), C.byref(it))
+    assert st == T.TC_ERR_INVALID_ARGUMENT and b"waves" in L.tc_last_error()

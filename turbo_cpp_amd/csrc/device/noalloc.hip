@@ -91,6 +91,15 @@ extern "C" tc_status tc_decode_code_block_noalloc(const tc_cb_desc* desc, const 
       throw Failure{TC_ERR_INVALID_ARGUMENT, "K=" + std::to_string(k) + " above the no-allocation maximum " +
                                                  std::to_string(kMaxK)};
     const tc_options o = normalised_options(opts);
+    if (o.decoder == TC_DECODER_WINDOW_LOG || o.decoder == TC_DECODER_WINDOW_LIN) {
+      tc_segment_info g{};
+      if (tc_segment_geometry(k, &o, &g) != TC_OK) throw Failure{TC_ERR_INVALID_ARGUMENT, tc_last_error()};
+      if (g.waves_per_block > 1)
+        throw Failure{TC_ERR_INVALID_ARGUMENT, "K=" + std::to_string(k) + " at this segment length spans " +
+                                                   std::to_string(g.waves_per_block) +
+                                                   " waves, whose work space exceeds the no-allocation array "
+                                                   "(one wave's): use max_waves_per_block 1"};
+    }
     require_device();
     uint8_t* base = device_scratch();
     float* io = reinterpret_cast<float*>(base + kIoOff);
