@@ -16,7 +16,7 @@
 struct tc_derate_plan {
   size_t n_cb = 0;
   size_t rx_floats = 0, llr_floats = 0;
-  uint32_t max_valid = 0;
+  uint32_t max_held = 0;  // max over blocks of min(E, N_valid): LDS floats per transmission
   std::vector<uint32_t> ks, fs;  // per block, to match a decode plan
   tcdev::DerateCb* d_cbs = nullptr;
   uint16_t* d_rank = nullptr;

@@ -70,7 +70,7 @@ void build_derate_plan(tc_derate_plan& p, const uint32_t* k, const uint32_t* fil
     std::copy(g.start, g.start + 4, c.start);
     c.rv = rv ? rv[i] : 0;
     c.pad = 0;
-    p.max_valid = std::max(p.max_valid, g.n_valid);
+    p.max_held = std::max(p.max_held, std::min(c.e, g.n_valid));
     p.ks.push_back(k[i]);
     p.fs.push_back(f);
     in += e[i];
@@ -93,7 +93,7 @@ void run_derate(const tc_derate_plan& p, const float* rx, int32_t rv, int32_t ac
   if (p.n_cb == 0) return;
   if (!rx || !llr) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
   if (p.n_cb > 0x7FFFFFFFull) throw Failure{TC_ERR_INVALID_ARGUMENT, "too many code blocks"};
-  const size_t lds = static_cast<size_t>(p.max_valid) * sizeof(float);
+  const size_t lds = static_cast<size_t>(p.max_held) * sizeof(float);
   hipLaunchKernelGGL(tcdev::derate_soft_kernel, dim3(static_cast<uint32_t>(p.n_cb)), dim3(tcdev::kDerateThreads), lds,
                      s, p.d_cbs, p.d_rank, rx, llr, rv, accumulate ? 1 : 0);
   hip_check(hipGetLastError(), "de-rate matching launch");
@@ -108,7 +108,7 @@ void run_derate2(const tc_derate_plan& p, const float* rx_a, int32_t rv_a, const
   if (p.n_cb == 0) return;
   if (!rx_a || !rx_b || !llr) throw Failure{TC_ERR_INVALID_ARGUMENT, "null argument"};
   if (p.n_cb > 0x7FFFFFFFull) throw Failure{TC_ERR_INVALID_ARGUMENT, "too many code blocks"};
-  const size_t lds = 2 * static_cast<size_t>(p.max_valid) * sizeof(float);
+  const size_t lds = 2 * static_cast<size_t>(p.max_held) * sizeof(float);
   hipLaunchKernelGGL(tcdev::derate_soft2_kernel, dim3(static_cast<uint32_t>(p.n_cb)), dim3(tcdev::kDerateThreads),
                      lds, s, p.d_cbs, p.d_rank, rx_a, rv_a, rx_b, rv_b, llr, accumulate ? 1 : 0);
   hip_check(hipGetLastError(), "de-rate matching launch");
