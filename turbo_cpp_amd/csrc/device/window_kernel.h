@@ -1226,18 +1226,25 @@ __device__ __forceinline__ bool crc24b_job(const Lane& L, const float* __restric
   const uint32_t S = L.S, K = L.K, lane = threadIdx.x;
   uint32_t x = 0;
   uint32_t r = crc_pow[K - 1 - L.q * S];
-  for (uint32_t t0 = 0; t0 < S; t0 += kSubWin) {
-    float pv[kSubWin];
+  // kCrcBatch posteriors in flight per lane (the registers of the recursions
+  // are free here): two round trips to the cache for S = 96 instead of one
+  // per 8 steps, whose latency the fold could not cover
+  constexpr uint32_t kCrcBatch = 48;
+  const float* base = post_sc + L.colb + L.q;
+  for (uint32_t t0 = 0; t0 < S; t0 += kCrcBatch) {
+    // every load unconditional (rows past the segment re-read row S-1 and
+    // are masked below): a load under a per-step guard becomes a branch that
+    // waits for it, one round trip per step
+    float pv[kCrcBatch];
 #pragma unroll
-    for (uint32_t j = 0; j < kSubWin; ++j)
-      pv[j] = (t0 + j < S) ? post_sc[(t0 + j) * R + L.colb + L.q] : -1.0f;
+    for (uint32_t j = 0; j < kCrcBatch; ++j) pv[j] = base[min(t0 + j, S - 1) * R];
 #pragma unroll
-    for (uint32_t j = 0; j < kSubWin; ++j) {
-      if (t0 + j < S) {
-        const uint32_t i = L.q * S + t0 + j;
-        if (i >= filler && pv[j] >= 0.0f) x ^= r;
-        r = (r & 1u) ? ((r ^ (0xC00063u | (1u << 24))) >> 1) : (r >> 1);
-      }
+    for (uint32_t j = 0; j < kCrcBatch; ++j) {
+      const bool in = t0 + j < S;  // uniform
+      const uint32_t i = L.q * S + t0 + j;
+      x ^= (in && i >= filler && pv[j] >= 0.0f) ? r : 0u;
+      const uint32_t rn = (r >> 1) ^ ((0u - (r & 1u)) & ((0xC00063u | (1u << 24)) >> 1));  // r * D^-1 mod G
+      r = in ? rn : r;
     }
   }
   partial[lane] = x;
