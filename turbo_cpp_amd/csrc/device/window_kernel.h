@@ -1350,17 +1350,28 @@ __device__ __forceinline__ void tb_epilogue(const WinArgs& A, uint32_t cb, const
   const TbInfo t = T.tbs[m.tb];
   const uint32_t n = K - filler, len = t.b + 24;
   uint32_t cbx = 0, tbx = 0;
-  // kCopyBatch bits per lane and batch: all loads first, then the XORs
+  // kCopyBatch bits per lane and batch: all loads first, then the XORs.  The
+  // loads are unconditional, at indices clamped into range (n >= 1), and
+  // masked afterwards: a load under a per-lane condition becomes a branch
+  // that waits for it.
   for (uint32_t base = 0; base < n; base += kCopyBatch * kWaveLanes) {
     float pv[kCopyBatch];
     uint32_t pb[kCopyBatch], pa[kCopyBatch];
 #pragma unroll
     for (uint32_t u = 0; u < kCopyBatch; ++u) {
       const uint32_t q = base + u * kWaveLanes + lane;
+      const uint32_t qn = min(q, n - 1);
+      pv[u] = post[filler + qn];
+      pb[u] = T.pow24b[n - 1 - qn];
+      pa[u] = T.pow24a[m.np ? len - 1 - (m.pos + min(q, m.np - 1)) : 0u];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kCopyBatch; ++u) {
+      const uint32_t q = base + u * kWaveLanes + lane;
       const bool in = q < n, in_tb = q < m.np;
-      pv[u] = in ? post[filler + q] : -1.0f;
-      pb[u] = (in && m.seg) ? T.pow24b[n - 1 - q] : 0u;
-      pa[u] = in_tb ? T.pow24a[len - 1 - (m.pos + q)] : 0u;
+      pv[u] = in ? pv[u] : -1.0f;
+      pb[u] = (in && m.seg) ? pb[u] : 0u;
+      pa[u] = in_tb ? pa[u] : 0u;
     }
 #pragma unroll
     for (uint32_t u = 0; u < kCopyBatch; ++u) {
@@ -1467,8 +1478,16 @@ __device__ __forceinline__ void stage_tiled(const WinArgs& A, const Lane& L, con
         if (fused) {
           rx_batch(A.rx, B, ci.llr_off, pos, mask, v);
         } else {
+          // unconditional loads (a slot past the pass re-reads its last
+          // entry; the mask drops it below): a load under a per-lane
+          // condition becomes a branch that waits for it
+          const uint32_t nlast = n - 1;
 #pragma unroll
-          for (uint32_t k = 0; k < kTileBatch; ++k) v[k] = ((mask >> k) & 1u) ? A.llr[ci.llr_off + pos[k]] : 0.0f;
+          for (uint32_t k = 0; k < kTileBatch; ++k) {
+            const uint32_t idx = min(base + k * R + lane, nlast);
+            const uint32_t q = __umulhi(idx, mg), u = idx - q * rp;
+            v[k] = A.llr[ci.llr_off + sgm * D4 + q * S + r0 + u];
+          }
         }
 #pragma unroll
         for (uint32_t k = 0; k < kTileBatch; ++k) {
