@@ -33,10 +33,7 @@ __global__ __launch_bounds__(kDerateThreads) void derate_soft_kernel(const Derat
     for (uint32_t base = 0; base < e; base += U * kDerateThreads) {
       float v[U];
 #pragma unroll
-      for (uint32_t u = 0; u < U; ++u) {
-        const uint32_t m = base + u * kDerateThreads + threadIdx.x;
-        v[u] = m < e ? in[m] : 0.0f;
-      }
+      for (uint32_t u = 0; u < U; ++u) v[u] = in[min(base + u * kDerateThreads + threadIdx.x, e - 1)];
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) {
         const uint32_t m = base + u * kDerateThreads + threadIdx.x;
@@ -61,9 +58,11 @@ __global__ __launch_bounds__(kDerateThreads) void derate_soft_kernel(const Derat
     float prev[U];
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
-      const uint32_t i = base + u * kDerateThreads + threadIdx.x;
-      r[u] = i < n ? rk[i] : kNullRank;
-      prev[u] = (accumulate && i < n) ? o[i] : 0.0f;
+      // unconditional loads at clamped indices (the i >= n slots are skipped
+      // below): a load under a per-lane condition becomes a branch and a wait
+      const uint32_t i = min(base + u * kDerateThreads + threadIdx.x, n - 1);
+      r[u] = rk[i];
+      prev[u] = accumulate ? o[i] : 0.0f;
     }
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
@@ -88,10 +87,7 @@ __device__ __forceinline__ void derate_fill(const float* __restrict__ in, uint32
     for (uint32_t base = 0; base < e; base += U * kDerateThreads) {
       float v[U];
 #pragma unroll
-      for (uint32_t u = 0; u < U; ++u) {
-        const uint32_t m = base + u * kDerateThreads + threadIdx.x;
-        v[u] = m < e ? in[m] : 0.0f;
-      }
+      for (uint32_t u = 0; u < U; ++u) v[u] = in[min(base + u * kDerateThreads + threadIdx.x, e - 1)];
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) {
         const uint32_t m = base + u * kDerateThreads + threadIdx.x;
@@ -136,9 +132,11 @@ __global__ __launch_bounds__(kDerateThreads) void derate_soft2_kernel(const Dera
     float prev[U];
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
-      const uint32_t i = base + u * kDerateThreads + threadIdx.x;
-      r[u] = i < n ? rk[i] : kNullRank;
-      prev[u] = (accumulate && i < n) ? o[i] : 0.0f;
+      // unconditional loads at clamped indices (the i >= n slots are skipped
+      // below): a load under a per-lane condition becomes a branch and a wait
+      const uint32_t i = min(base + u * kDerateThreads + threadIdx.x, n - 1);
+      r[u] = rk[i];
+      prev[u] = accumulate ? o[i] : 0.0f;
     }
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) {
