@@ -246,9 +246,9 @@ tc_status tc_tb_plan_set_hold(tc_tb_plan* plan, int32_t hold);
  *   TC_STAGING_SEPARATE - one de-rate kernel per transmission into the plan's
  *                         decoder input (coalesced, LDS-staged), then the
  *                         decode of that input;
- *   TC_STAGING_AUTO     - the default, the separate form (measured as fast or
- *                         faster on every workload tried: one full-rate
- *                         transmission, punctured ones, HARQ pairs).
+ *   TC_STAGING_AUTO     - the default, the fused form (measured faster on
+ *                         every workload tried: one full-rate transmission,
+ *                         punctured ones, HARQ pairs).
  * Both give the same results bit for bit.  No counterpart in the reference
  * (whose de-rate matcher is a host function, lte_turbo.cpp:280-325). */
 #define TC_STAGING_AUTO 0

@@ -323,7 +323,7 @@ void run_tb(tc_tb_plan& p, uint32_t n_tx, const float* const* rx, const int32_t*
   if (accumulate && !p.hold)
     throw Failure{TC_ERR_INVALID_ARGUMENT, "the plan does not hold its decoder input (tc_tb_plan_set_hold)"};
   const bool exact = p.exact;
-  const bool separate = n_tx && p.staging != TC_STAGING_FUSED;
+  const bool separate = n_tx && p.staging == TC_STAGING_SEPARATE;
   if (!exact && separate) {
     // window decoders, de-rate kernels first: each transmission into d_soft (added in order onto
     // the held input or the transmissions before it, the fused staging's order), then the decode

@@ -1280,59 +1280,70 @@ __device__ __forceinline__ void rx_block(const RxIn& X, uint32_t cb, RxBlk& r) {
   r.n03 = c->start[(X.rv[3] == kBlockRv ? own : X.rv[3]) & 3u];
 }
 
-// Soft de-rate value of decoder-input position pos (= s (K+4) + i) of one
-// block: received values landing on its circular-buffer rank summed in
-// reception order per transmission, transmissions added in order (the
-// order of tc_derate_plan_run, so the result is bit-identical to it).
-__device__ __forceinline__ float rx_from_rank(const RxIn& X, const RxBlk& B, const float* keep_blk, uint32_t pos,
-                                             uint32_t r) {
-  float v = X.base ? keep_blk[pos] : 0.0f;
-  if (r == kNullRank) return v;
-#pragma unroll
-  for (int t = 0; t < kMaxTx; ++t) {
-    if ((uint32_t)t >= X.n_tx) break;
-    const uint32_t n0 = start_of(B, t);
-    const uint32_t m = r >= n0 ? r - n0 : r + B.nv - n0;
-    if (m < B.e) {
-      const float* in = X.rx[t] + B.in_off;
-      float a = 0.0f;
-      for (uint32_t k = m; k < B.e; k += B.nv) a += in[k];
-      v = (t == 0 && !X.base) ? a : v + a;
-    }
-  }
-  return v;
-}
-
-__device__ __forceinline__ float rx_value(const RxIn& X, const RxBlk& B, const float* keep_blk, uint32_t pos) {
-  return rx_from_rank(X, B, keep_blk, pos, B.rank[pos]);
-}
-
-// Decoder input value at position pos of block cb: from llr, or de-rate
-// matched from the received LLRs (then also written to keep if set).
-__device__ __forceinline__ float chan_in(const WinArgs& A, const RxBlk& B, uint64_t llr_off, uint32_t pos,
-                                         bool write_keep) {
-  if (A.rx.n_tx == 0) return A.llr[llr_off + pos];
-  float* keep = A.rx.keep ? A.rx.keep + llr_off : nullptr;
-  const float v = rx_value(A.rx, B, keep, pos);
-  if (keep && write_keep) keep[pos] = v;
-  return v;
-}
-
 // kStageSteps steps of the three streams at once: all rank loads, then all
 // value loads, so a lane keeps that many gathers in flight while staging.
 constexpr uint32_t kStageSteps = 8;
 // posteriors copied out per lane and batch
 constexpr uint32_t kCopyBatch = 16;
+// Soft de-rate values of N decoder-input positions (pos = s (K+4) + i) of
+// one block, every one in range (callers clamp the slots they drop): the
+// received values landing on each position's circular-buffer rank summed in
+// reception order per transmission, transmissions added in order onto the
+// held input or the first of them -- the operations and order of
+// tc_derate_plan_run, so the result is bit-identical to it.  The loads of a
+// batch are unconditional and issued together (a load under a per-lane
+// condition compiles to a branch that waits for it): the ranks, the held
+// input when it is added, then per transmission one received value per
+// position at a clamped index, selected afterwards.  Repetition (E >
+// N_valid, several values per rank) keeps a per-position loop.
 template <uint32_t N>
 __device__ __forceinline__ void rx_batch(const RxIn& X, const RxBlk& B, uint64_t llr_off, const uint32_t (&pos)[N],
-                                         uint32_t valid_mask, float (&out)[N]) {
-  const float* keep = X.keep ? X.keep + llr_off : nullptr;
+                                         float (&out)[N]) {
   uint32_t r[N];
 #pragma unroll
-  for (uint32_t u = 0; u < N; ++u) r[u] = ((valid_mask >> u) & 1u) ? B.rank[pos[u]] : kNullRank;
+  for (uint32_t u = 0; u < N; ++u) r[u] = B.rank[pos[u]];
+  if (X.base) {
+    const float* keep = X.keep + llr_off;
 #pragma unroll
-  for (uint32_t u = 0; u < N; ++u)
-    out[u] = ((valid_mask >> u) & 1u) ? rx_from_rank(X, B, keep, pos[u], r[u]) : 0.0f;
+    for (uint32_t u = 0; u < N; ++u) out[u] = keep[pos[u]];
+  } else {
+#pragma unroll
+    for (uint32_t u = 0; u < N; ++u) out[u] = 0.0f;
+  }
+#pragma unroll
+  for (int t = 0; t < kMaxTx; ++t) {
+    if ((uint32_t)t >= X.n_tx) break;
+    const uint32_t n0 = start_of(B, t);
+    const float* in = X.rx[t] + B.in_off;
+    const bool first = t == 0 && !X.base;
+    if (B.e <= B.nv) {
+      float a[N];
+#pragma unroll
+      for (uint32_t u = 0; u < N; ++u) {
+        const uint32_t m = r[u] >= n0 ? r[u] - n0 : r[u] + B.nv - n0;
+        a[u] = in[min(m, B.e - 1)];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < N; ++u) {
+        const uint32_t m = r[u] >= n0 ? r[u] - n0 : r[u] + B.nv - n0;
+        const float x = 0.0f + a[u];  // the de-rate kernel's sum of one value (-0 -> +0)
+        const float sum = first ? x : out[u] + x;
+        // a select, not a branch: under a branch the load would be sunk into it
+        out[u] = (r[u] != kNullRank && m < B.e) ? sum : out[u];
+      }
+    } else {
+#pragma unroll
+      for (uint32_t u = 0; u < N; ++u) {
+        if (r[u] == kNullRank) continue;
+        const uint32_t m = r[u] >= n0 ? r[u] - n0 : r[u] + B.nv - n0;
+        if (m < B.e) {
+          float x = 0.0f;
+          for (uint32_t k = m; k < B.e; k += B.nv) x += in[k];
+          out[u] = first ? x : out[u] + x;
+        }
+      }
+    }
+  }
 }
 
 // Transport-block assembly of one decoded block (lte_turbo.cpp:958-979), run
@@ -1476,7 +1487,14 @@ __device__ __forceinline__ void stage_tiled(const WinArgs& A, const Lane& L, con
         }
         float v[kTileBatch];
         if (fused) {
-          rx_batch(A.rx, B, ci.llr_off, pos, mask, v);
+          uint32_t posl[kTileBatch];  // in range: a slot past the pass takes the pass's last entry
+#pragma unroll
+          for (uint32_t k = 0; k < kTileBatch; ++k) {
+            const uint32_t idx = min(base + k * R + lane, n - 1);
+            const uint32_t q = __umulhi(idx, mg), u = idx - q * rp;
+            posl[k] = sgm * D4 + q * S + r0 + u;
+          }
+          rx_batch(A.rx, B, ci.llr_off, posl, v);
         } else {
           // unconditional loads (a slot past the pass re-reads its last
           // entry; the mask drops it below): a load under a per-lane
@@ -1634,12 +1652,22 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
         pos[3 * u + 2] = 2 * D4 + i;
         mask |= ((L.active && t0 + u < S) ? 7u : 0u) << (3 * u);
       }
+      // loads at in-range positions (steps past the segment re-read its last
+      // step; the mask drops them below), unconditional
+      uint32_t posl[N];
+#pragma unroll
+      for (uint32_t u = 0; u < kStageSteps; ++u) {
+        const uint32_t i = L.q * S + min(t0 + u, S - 1);
+        posl[3 * u] = i;
+        posl[3 * u + 1] = D4 + i;
+        posl[3 * u + 2] = 2 * D4 + i;
+      }
       float v[N];
       if (fused) {
-        rx_batch(A.rx, Bm, ci.llr_off, pos, mask, v);
+        rx_batch(A.rx, Bm, ci.llr_off, posl, v);
       } else {
 #pragma unroll
-        for (uint32_t u = 0; u < N; ++u) v[u] = ((mask >> u) & 1u) ? A.llr[ci.llr_off + pos[u]] : 0.0f;
+        for (uint32_t u = 0; u < N; ++u) v[u] = A.llr[ci.llr_off + posl[u]];
       }
 #pragma unroll
       for (uint32_t u = 0; u < kStageSteps; ++u) {
@@ -1657,14 +1685,24 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
     }
   }
   // ---- boundary metrics: alpha_0 / tail beta / uniform ----
+  float tl[3][4];  // d0, d1, d2 at K .. K+3
+  {
+    uint32_t tp[12];
+    float tv[12];
+#pragma unroll
+    for (uint32_t u = 0; u < 12; ++u) tp[u] = (u / 4) * D4 + K + u % 4;
+    if (fused) {
+      rx_batch(A.rx, Bm, ci.llr_off, tp, tv);
+    } else {
+#pragma unroll
+      for (uint32_t u = 0; u < 12; ++u) tv[u] = A.llr[ci.llr_off + tp[u]];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 12; ++u) tl[u / 4][u % 4] = tv[u];
+  }
   {
     float tb[2][8];
-    float tl[3][4];  // d0, d1, d2 at K .. K+3
     // decoder tail read-out of lte_turbo.cpp:758-773
-#pragma unroll
-    for (uint32_t sgm = 0; sgm < 3; ++sgm)
-#pragma unroll
-      for (uint32_t j = 0; j < 4; ++j) tl[sgm][j] = chan_in(A, Bm, ci.llr_off, sgm * D4 + K + j, false);
     tail_beta(tl[0][0], tl[0][2], 0.0f, tl[1][0], tl[0][1], tl[1][2], tb[0]);
     tail_beta(tl[1][3], tl[2][0], tl[2][2], tl[2][1], tl[0][3], tl[2][3], tb[1]);
 #pragma unroll
@@ -1682,12 +1720,9 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
   }
   __syncthreads();  // every lane's d0 in sc_up before the gather below
   if (fused && A.rx.keep && L.active && L.first) {
-    // every lane of the block has read the tails; one recomputes and writes them
+    // every lane of the block has read the tails; one writes them
     for (uint32_t sgm = 0; sgm < 3; ++sgm)
-      for (uint32_t j = 0; j < 4; ++j) {
-        const uint32_t pos = sgm * D4 + K + j;
-        A.rx.keep[ci.llr_off + pos] = chan_in(A, Bm, ci.llr_off, pos, false);
-      }
+      for (uint32_t j = 0; j < 4; ++j) A.rx.keep[ci.llr_off + sgm * D4 + K + j] = tl[sgm][j];
   }
   // ---- several-block jobs: interleaved systematic, the lower half's plane
   // at step t = d0[pi(q S + t)], which sits in the upper half's plane at the
