@@ -1521,8 +1521,7 @@ __device__ __forceinline__ void stage_tiled(const WinArgs& A, const Lane& L, con
       for (uint32_t u0 = 0; u0 < rp; u0 += kSubWin) {
         float v[kSubWin];
 #pragma unroll
-        for (uint32_t k = 0; k < kSubWin; ++k)
-          if (u0 + k < rp) v[k] = tile[tile_at<R>(u0 + k, lane)];
+        for (uint32_t k = 0; k < kSubWin; ++k) v[k] = tile[tile_at<R>(min(u0 + k, rp - 1), lane)];
 #pragma unroll
         for (uint32_t k = 0; k < kSubWin; ++k) {
           if (u0 + k < rp && L.active)
@@ -1531,18 +1530,32 @@ __device__ __forceinline__ void stage_tiled(const WinArgs& A, const Lane& L, con
       }
       if (sgm == 0) {
         // lower-half systematic: step t of segment q reads d0 at pi(q S + t),
-        // i.e. row pi(t) mod S (uniform) at this lane's permuted column
+        // i.e. row pi(t) mod S (uniform) at this lane's permuted column.
+        // kSubWin steps at a time: their table entries in one scalar batch
+        // (the table is padded by kSubWin entries), then every tile read at
+        // a row clamped into the pass, then the stores of the steps whose
+        // row the pass holds -- not one dependent round trip per step
         uint32_t y = qpp_col0(L, L.q, 0);
-        for (uint32_t t = 0; t < S; ++t) {
-          const uint32_t qv = qtab_at(qtab, L.qtab_off + t);
-          const uint32_t row = qv >> 16;
-          if (row >= r0 && row < r0 + rp) {
-            uint32_t c = (qv & 0xFFFFu) + y;
+        for (uint32_t t0 = 0; t0 < S; t0 += kSubWin) {
+          uint32_t off = __builtin_amdgcn_readfirstlane(L.qtab_off + t0);  // wave-uniform
+          asm volatile("" : "+s"(off));
+          const uint32_t* qt = qtab + off;
+          uint32_t qv[kSubWin];
+#pragma unroll
+          for (uint32_t j = 0; j < kSubWin; ++j) qv[j] = qtab_at(qt, j);
+          float x[kSubWin];
+#pragma unroll
+          for (uint32_t j = 0; j < kSubWin; ++j) {
+            uint32_t c = (qv[j] & 0xFFFFu) + y;
             c = (c >= M) ? c - M : c;
-            const float x = tile[tile_at<R>(row - r0, c)];
-            if (L.active) lo[t * R + lane] = x;
+            x[j] = tile[tile_at<R>(min((qv[j] >> 16) - r0, rp - 1), c)];
+            y = y_fwd(y, by, M);
           }
-          y = y_fwd(y, by, M);
+#pragma unroll
+          for (uint32_t j = 0; j < kSubWin; ++j) {
+            const uint32_t row = qv[j] >> 16;
+            if (t0 + j < S && row >= r0 && row < r0 + rp && L.active) lo[(t0 + j) * R + lane] = x[j];
+          }
         }
       }
       __syncthreads();
@@ -1562,8 +1575,7 @@ __device__ __forceinline__ void copy_out_tiled(const Lane& L, bool any, const fl
     for (uint32_t u0 = 0; u0 < rp; u0 += kTileBatch) {
       float v[kTileBatch];
 #pragma unroll
-      for (uint32_t k = 0; k < kTileBatch; ++k)
-        if (u0 + k < rp) v[k] = post_sc[(r0 + u0 + k) * R + lane];
+      for (uint32_t k = 0; k < kTileBatch; ++k) v[k] = post_sc[(r0 + min(u0 + k, rp - 1)) * R + lane];
 #pragma unroll
       for (uint32_t k = 0; k < kTileBatch; ++k)
         if (u0 + k < rp) tile[tile_at<R>(u0 + k, lane)] = v[k];
@@ -1572,11 +1584,18 @@ __device__ __forceinline__ void copy_out_tiled(const Lane& L, bool any, const fl
     const uint32_t mg = 0xFFFFFFFFu / rp + 1u;
     const uint32_t n = M * rp;
     for (uint32_t base = 0; base < n; base += kTileBatch * R) {
+      float v[kTileBatch];  // read at clamped slots, so no read waits under a branch
+#pragma unroll
+      for (uint32_t k = 0; k < kTileBatch; ++k) {
+        const uint32_t idx = min(base + k * R + lane, n - 1);
+        const uint32_t q = __umulhi(idx, mg), u = idx - q * rp;
+        v[k] = tile[tile_at<R>(u, q)];
+      }
 #pragma unroll
       for (uint32_t k = 0; k < kTileBatch; ++k) {
         const uint32_t idx = base + k * R + lane;
         const uint32_t q = __umulhi(idx, mg), u = idx - q * rp;
-        if (idx < n) dst[q * S + r0 + u] = any ? tile[tile_at<R>(u, q)] : 0.0f;
+        if (idx < n) dst[q * S + r0 + u] = any ? v[k] : 0.0f;
       }
     }
     __syncthreads();
