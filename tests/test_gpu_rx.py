@@ -397,14 +397,21 @@ def test_gpu_encode_decode_round_trip_on_device():
         assert bool(ok.all()) and torch.equal(bits, pay)
 
 
-def test_tb_pipeline_graph_capture_replays_match_eager():
+@pytest.mark.parametrize("waves", [1, 3])
+def test_tb_pipeline_graph_capture_replays_match_eager(waves):
     """The TB pipeline (de-rate matching, decode with CRC24B early stop,
     assembly) allocates nothing and records no memset, so it can be captured
     in a HIP graph: three replays, with eager work in between, each equal
-    the eager run bit for bit."""
+    the eager run bit for bit.  waves = 3: segment length from 32 on jobs of
+    three waves (the job queue's per-work-group tickets and its reset by the
+    last work-group out, replayed)."""
     import torch
     tbs, rx, _ = _tb_batch(31, n_tb=12)
-    plan = T.TbPlan(tbs, decoder=T.DECODER_WINDOW_LOG, max_iterations=8, early_stop=True)
+    geo = dict(min_segment=32, max_waves_per_block=waves) if waves > 1 else {}
+    plan = T.TbPlan(tbs, decoder=T.DECODER_WINDOW_LOG, max_iterations=8, early_stop=True, **geo)
+    if waves > 1:
+        assert max(T.segment_geometry(int(k), T.DECODER_WINDOW_LOG, 32, 0, 0, waves)["waves_per_block"]
+                   for k in plan.ks) == waves
     d_rx = torch.from_numpy(rx).cuda()
 
     def outputs():
