@@ -37,10 +37,13 @@ constexpr int kExRow = kWaveLanes;  // LDS extrinsic row: one entry per lane
 // Per-wave scratch slot, in float2: staged channel LLRs for both halves
 // ([t][lane], S <= kMaxSeg), then in floats the boundary metrics
 // [lane][half][16], the beta checkpoints [lane][w][8], the posteriors
-// [t][lane] plus one dummy slot per lane, and the alpha checkpoints
-// [lane][w][8] (exact segment boundaries).
+// [t][lane] plus one dummy slot per lane, the alpha checkpoints [lane][w][8]
+// and the extrinsics held between halves [t][lane] plus one dummy slot per
+// lane (the last two: exact segment boundaries only, whose fp32 extrinsics
+// live here rather than in LDS).
 constexpr int kSlotFloat2 = 2 * kMaxSeg * kWaveLanes + 16 * kWaveLanes + 4 * kMaxSubWins * kWaveLanes +
-                            (kMaxSeg + 1) * kWaveLanes / 2 + 4 * kMaxSubWins * kWaveLanes;
+                            (kMaxSeg + 1) * kWaveLanes / 2 + 4 * kMaxSubWins * kWaveLanes +
+                            (kMaxSeg + 1) * kWaveLanes / 2;
 
 struct WinJob {
   uint32_t k;
@@ -104,6 +107,14 @@ constexpr int kCrcPowLen = 6144;
 // register budget allows; fp32 (24.5 KiB) allows six.
 constexpr int job_lds_bytes(int s, int ex_bytes, int waves = 1) {
   return (s * kExRow + kWaveLanes) * ex_bytes * waves + kWaveLanes * 4 * waves;
+}
+// LDS of one job of the exact segment-parallel decoder, whose extrinsics sit
+// in the wave's scratch slot: the staging / copy-out tile (ceil(S/2) rows of
+// 64 floats), which also holds the CRC partials during the iterations.  At
+// S = 96 this is 12 KiB, so eight waves (two per SIMD, what its registers
+// allow) fit a CU, where the 24.8 KiB of fp32 extrinsics in LDS allowed six.
+constexpr int exact_job_lds_bytes(int s) {
+  return ((s + 1) / 2) * kWaveLanes * 4 > kWaveLanes * 4 ? ((s + 1) / 2) * kWaveLanes * 4 : kWaveLanes * 4;
 }
 constexpr int kExtrinsicBits = 16;  // window decoders' default held width
 

@@ -1614,7 +1614,7 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
                                            float* __restrict__ sc_up, float* __restrict__ sc_lo,
                                            float* __restrict__ nii, float* __restrict__ ck,
                                            float* __restrict__ post_sc, float* __restrict__ ack,
-                                           float* __restrict__ recs = nullptr) {
+                                           float* __restrict__ recs = nullptr, X* __restrict__ ex_slot = nullptr) {
   constexpr int R = kWaveLanes * D::kWaves;  // the job's lanes
   constexpr uint32_t kRecSet = R * kRecFloats;
   const uint32_t lane = threadIdx.x;
@@ -1646,7 +1646,11 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
   // LDS: extrinsics [S][R] + one dummy slot per lane, then the CRC
   // partials (early stop only)
   const uint32_t ex_slots = S * R + R;
-  uint32_t* partial = reinterpret_cast<uint32_t*>(ex + ex_slots);
+  // exact boundaries: the extrinsics in the wave's scratch slot (ex_slot),
+  // LDS only for the staging tile, in which the CRC partials sit during the
+  // iterations
+  X* const exh = D::kExact ? ex_slot : ex;
+  uint32_t* partial = D::kExact ? reinterpret_cast<uint32_t*>(ex) : reinterpret_cast<uint32_t*>(ex + ex_slots);
   RxBlk Bm{};
   if (fused) rx_block(A.rx, my_cb, Bm);
   const uint32_t D4 = K + 4;
@@ -1764,7 +1768,7 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
     }
   }
   {
-    uint32_t* exw = reinterpret_cast<uint32_t*>(ex);
+    uint32_t* exw = reinterpret_cast<uint32_t*>(exh);
     for (uint32_t i = lane; i < ex_slots * sizeof(X) / 4; i += R) exw[i] = 0u;
   }
   __syncthreads();
@@ -1776,7 +1780,7 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
     float* const own = (D::kWaves > 1 && (it & 1u)) ? nii + kRecSet : nii;
     float* const push = (D::kWaves > 1) ? recs + ((it & 1u) ? 0u : kRecSet) : nullptr;
     if constexpr (D::kExact) {
-      siso_half_exact<0, ES>(L, ex, sc_up, qtab, nii, ck, ack, post_sc, es);
+      siso_half_exact<0, ES>(L, exh, sc_up, qtab, nii, ck, ack, post_sc, es);
     } else if constexpr (LEAN) {
       siso_half_lean<D, 0, ES>(L, ex, sc_up, qtab, nii, ck, post_sc, es);
     } else {
@@ -1786,7 +1790,7 @@ __device__ __forceinline__ void window_job(const WinArgs& A, const WinJob& J, X*
     // every lower half leaves the posteriors in post_sc; a block that has
     // stopped no longer overwrites its own
     if constexpr (D::kExact) {
-      siso_half_exact<1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, ack, post_sc, es);
+      siso_half_exact<1, ES>(L, exh, sc_lo, qtab, nii + 16, ck, ack, post_sc, es);
     } else if constexpr (LEAN) {
       siso_half_lean<D, 1, ES>(L, ex, sc_lo, qtab, nii + 16, ck, post_sc, es);
     } else {
@@ -1866,14 +1870,16 @@ __device__ __forceinline__ void window_decode(const WinArgs& A) {
   float* ck = recs + kRecs + lane * (8 * kMaxSubWins);
   // posteriors [S][R] + one dummy slot per lane
   float* post_sc = recs + kRecs + R * 8 * kMaxSubWins;
-  // alpha checkpoints [lane][w][8] (exact segment boundaries; one wave only)
+  // alpha checkpoints [lane][w][8] and the held extrinsics [S][64] + one
+  // dummy slot per lane (exact segment boundaries; one wave only)
   float* ack = post_sc + (kMaxSeg + 1) * R + lane * (8 * kMaxSubWins);
+  X* ex_slot = reinterpret_cast<X*>(post_sc + (kMaxSeg + 1) * R + R * 8 * kMaxSubWins);
   static_assert(4 * plane_of<R>() + kRecs + R * 8 * kMaxSubWins + (kMaxSeg + 1) * R +
-                        (NW == 1 ? R * 8 * kMaxSubWins : 0) <= 2 * NW * kSlotFloat2,
+                        (NW == 1 ? R * 8 * kMaxSubWins + (kMaxSeg + 1) * R : 0) <= 2 * NW * kSlotFloat2,
                 "the job's scratch fits its waves' slots");
   for (uint32_t jb = next_job<NW>(A.sched); jb < A.njobs; jb = next_job<NW>(A.sched)) {
     const WinJob J = A.jobs[jb];
-    window_job<D, ES, X, LEAN>(A, J, reinterpret_cast<X*>(lds), sc_up, sc_lo, nii, ck, post_sc, ack, recs);
+    window_job<D, ES, X, LEAN>(A, J, reinterpret_cast<X*>(lds), sc_up, sc_lo, nii, ck, post_sc, ack, recs, ex_slot);
   }
   leave_queue(A.sched);
 }
