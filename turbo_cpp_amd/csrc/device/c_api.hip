@@ -154,7 +154,10 @@ WinKernel window_kernel(const tc_options& o, bool lean = false, uint32_t waves =
 }
 
 size_t lds_bytes_of(uint32_t seg, const tc_options& o, uint32_t waves = 1) {
-  if (o.decoder == TC_DECODER_EXACT) return static_cast<size_t>(tcdev::exact_job_lds_bytes(static_cast<int>(seg)));
+  // fp32-held extrinsics of one-wave jobs live in the scratch slot (window_job,
+  // kF32HeldInSlot: exact and log decoders)
+  if (waves == 1 && o.extrinsic_bits == 32 && o.decoder != TC_DECODER_WINDOW_LIN)
+    return static_cast<size_t>(tcdev::tile_job_lds_bytes(static_cast<int>(seg)));
   return static_cast<size_t>(
       tcdev::job_lds_bytes(static_cast<int>(seg), o.extrinsic_bits / 8, static_cast<int>(waves)));
 }

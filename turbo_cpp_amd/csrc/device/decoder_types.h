@@ -104,16 +104,19 @@ constexpr int kCrcPowLen = 6144;
 // [S][64 waves] plus one dummy slot per lane (ex_bytes each: 2 for fp16, 4 for
 // fp32) and one CRC partial per lane.  At S = 96 with fp16 this is 12.4 KiB,
 // so LDS does not keep a CU below the eight waves (two per SIMD) that the
-// register budget allows; fp32 (24.5 KiB) allows six.
+// register budget allows; fp32 (24.5 KiB) allows six (the probability-domain
+// decoder's fp32 variant; the others hold fp32 in the scratch slot,
+// tile_job_lds_bytes).
 constexpr int job_lds_bytes(int s, int ex_bytes, int waves = 1) {
   return (s * kExRow + kWaveLanes) * ex_bytes * waves + kWaveLanes * 4 * waves;
 }
-// LDS of one job of the exact segment-parallel decoder, whose extrinsics sit
-// in the wave's scratch slot: the staging / copy-out tile (ceil(S/2) rows of
-// 64 floats), which also holds the CRC partials during the iterations.  At
-// S = 96 this is 12 KiB, so eight waves (two per SIMD, what its registers
-// allow) fit a CU, where the 24.8 KiB of fp32 extrinsics in LDS allowed six.
-constexpr int exact_job_lds_bytes(int s) {
+// LDS of a one-wave job whose fp32-held extrinsics sit in the wave's scratch
+// slot (the exact segment-parallel decoder; the log window decoder with
+// extrinsic_bits = 32): the staging / copy-out tile (ceil(S/2) rows of 64
+// floats), which also holds the CRC partials during the iterations.  At
+// S = 96 this is 12 KiB, so eight waves (two per SIMD, what the registers
+// allow) fit a CU, where 24.8 KiB of fp32 extrinsics in LDS allowed six.
+constexpr int tile_job_lds_bytes(int s) {
   return ((s + 1) / 2) * kWaveLanes * 4 > kWaveLanes * 4 ? ((s + 1) / 2) * kWaveLanes * 4 : kWaveLanes * 4;
 }
 constexpr int kExtrinsicBits = 16;  // window decoders' default held width
