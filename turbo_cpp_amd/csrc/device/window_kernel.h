@@ -4,7 +4,8 @@
 // to finish (all iterations): nblk code blocks of one K, each cut into M
 // segments of S = K / M steps; lane = block * M + segment.  Each lane runs
 // both SISO halves of every iteration over its own segment, lane-serially,
-// with all 8 state metrics in VGPRs.
+// with all 8 state metrics in VGPRs.  Blocks of more than 64 segments run on
+// jobs of two or three waves of one work-group (Wide<D, NW>, below).
 //
 // Segment boundaries.  A segment's initial alpha (beta) comes from a g-step
 // warm-up recursion over the neighbouring segment's last (first) g steps,
@@ -13,10 +14,11 @@
 // alpha_0 = state 0 and the last segment from the reference's tail beta
 // (lte_turbo.cpp:357-378, tail read-out :758-773), so M == 1 is a plain
 // full-length pass.  The exported metrics move between lanes with DPP
-// wave_shr:1 / wave_shl:1.
+// wave_shr:1 / wave_shl:1 (on several waves: through the lanes' records).
 //
 // Memory.  The wavefront's extrinsics live in LDS for the whole decode, one
-// fp16 per bit in a transposed layout: bit i = q*S + t of block b sits at
+// fp16 per bit (fp32-held variants of the log and exact decoders: in the
+// wave's scratch slot, see window_job) in a transposed layout: bit i = q*S + t of block b sits at
 // row t, column b*M + q (rows of 64).  The upper half reads and writes
 // its own column; the lower half reads and writes bit pi(j) for its steps
 // j = q*S + t.  Because S divides K, pi(q*S + t) mod S = pi(t) mod S for
