@@ -587,7 +587,9 @@ def diff_leg(T, torch, train_steps):
     a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
     x_req = x_tr.clone().requires_grad_(True)
     fwd, bwd = [], []
-    for rep in range(4):
+    # 3 untimed passes, then the median of 20 (the first passes after a
+    # plan is built run slower)
+    for rep in range(23):
         dec.zero_grad()
         a.record(stream)
         loss = bce(dec(x_req), y_tr)
@@ -595,10 +597,10 @@ def diff_leg(T, torch, train_steps):
         loss.backward()
         c.record(stream)
         torch.cuda.synchronize()
-        if rep:
+        if rep >= 3:
             fwd.append(a.elapsed_time(b))
             bwd.append(b.elapsed_time(c))
-    fwd_ms, bwd_ms = float(np.mean(fwd)), float(np.mean(bwd))
+    fwd_ms, bwd_ms = float(np.median(fwd)), float(np.median(bwd))
 
     held = [gen.batch(1000 + i) for i in range(4)]  # 8192 held-out blocks, never trained on
 

@@ -85,13 +85,17 @@ def test_adjoint_fp64_gradcheck_central_differences(min_segment):
                                     rtol=1e-4, nondet_tol=0.0)
 
 
-@pytest.mark.parametrize("k,min_segment,warm,ebn0", [(40, 8, 8, -1.0), (1024, 0, 0, 0.5), (1024, 0, 16, 0.5)])
+@pytest.mark.parametrize("k,min_segment,warm,ebn0",
+                         [(40, 8, 8, -1.0), (1024, 0, 0, 0.5), (1024, 0, 16, 0.5), (4096, 0, 0, 0.5)])
 def test_adjoint_fp32_matches_fp64_reverse_mode(k, min_segment, warm, ebn0):
     """float32 adjoint against torch.autograd in FP64 on the CPU reference:
     relative L2 error of both gradients below 2e-3.  The SNRs leave
     decision errors, so the gradients are not vanishingly small (where every
     bit is right the BCE gradients underflow towards 0 and any float32
-    evaluation, the CPU reference's included, is off by ~1e-2 relative)."""
+    evaluation, the CPU reference's included, is off by ~1e-2 relative).
+    K <= 2048 runs the LDS-staged kernels; at K=4096 a job's block copies
+    (68 KB) exceed the 40 KB of LDS the plan gives a job, so the float32
+    kernels that stage inputs per lane in global memory run."""
     bits, llr = _blocks(k, 2, ebn0, 11)
     sc = np.linspace(0.8, 1.1, 12)
     x, w, post = _ref_post(llr, sc, k, 6, min_segment, warm, grad=True)
@@ -115,7 +119,7 @@ def test_adjoint_fp64_global_staging_matches_reverse_mode():
     reference: relative L2 error of both gradients below 1e-9.  Two FP64
     blocks of K=1024 need 70 KB of block copies in LDS, more than the plan
     gives a job, so this runs the path that stages inputs per lane in global
-    memory (the float32 tests above run the LDS-staged one)."""
+    memory in FP64 (the float32 test above covers both paths)."""
     k = 1024
     bits, llr = _blocks(k, 2, 0.5, 13)
     sc = np.linspace(0.8, 1.1, 12)

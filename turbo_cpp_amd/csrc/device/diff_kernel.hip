@@ -202,6 +202,19 @@ struct QppWalk {
     d = d >= d2 ? d - d2 : d + k - d2;
     pi = pi >= d ? pi - d : pi + k - d;
   }
+  // The walk's state as the optimiser's barrier: steps after it cannot be
+  // computed before it, so a gather walked step by step computes each
+  // position where it loads (otherwise a chunk's positions are computed
+  // ahead and parked in spill slots, and each reload waits for every load
+  // before it).
+  __device__ void pin() { asm volatile("" : "+v"(pi), "+v"(d)); }
+  // prev() when c, else nothing (no branch)
+  __device__ void prev_if(bool c) {
+    const uint32_t dd = d >= d2 ? d - d2 : d + k - d2;
+    const uint32_t p = pi >= dd ? pi - dd : pi + k - dd;
+    pi = c ? p : pi;
+    d = c ? dd : d;
+  }
 };
 
 // Per-lane scratch fields, each [j][lane] with j < its length, and the
@@ -298,6 +311,23 @@ __device__ __forceinline__ Lane<R> fresh(const Lane<R>& L0) {
 // Checkpoints: al[k] = alpha at the start of alpha chunk k; be[m] = beta at
 // local g + 8m for 0 < m < nb and be[nb] = beta at the window top 2g + S, so
 // beta chunk m is recomputed downwards from be[m + 1].
+// Lane src's value of v (a wave-wide read: every lane of the wave takes
+// part); float and double.
+__device__ __forceinline__ float shfl_r(float v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ double shfl_r(double v, int src) { return __shfl(v, src, 64); }
+
+// t_of(j) of a chunk's first step as a value the optimiser can neither hoist
+// nor share with other uses of t_of: the base of the chunk's contiguous row of
+// loads (immediate offsets from one address, where per-step positions kept
+// live across the loop would take registers and come back from spill slots,
+// each reload waiting for every load before it).
+template <typename R>
+__device__ __forceinline__ uint32_t row_t(const Lane<R>& L, uint32_t j) {
+  uint32_t t = (uint32_t)L.t_of(j);
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 struct Chunks {
   uint32_t off, na, nb, nown;
   __device__ Chunks(uint32_t S, uint32_t g)
@@ -366,26 +396,29 @@ template <typename R>
 struct In8 {
   R sa[8], pr[8];
 };
+// Entries jj >= n repeat entry n - 1: every entry loads (one wait for the
+// batch, where a branch per entry would wait at each) and no later code reads
+// them.
 template <typename R, bool DOWN = false>
 __device__ __forceinline__ void load_in8(const Lane<R>& L, const Layout& Y, int j0, int n, In8<R>& c) {
   if (L.lds) {
     const uint32_t pb = L.q * L.S;
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj)
-      if (jj < n) {
-        const uint32_t p = pb + (uint32_t)(DOWN ? j0 - jj : j0 + jj);
-        const uint32_t i = 2 * (p + (p >> 5));
-        c.sa[jj] = L.xs[i];
-        c.pr[jj] = L.xs[i + 1];
-      }
+    for (int jj = 0; jj < 8; ++jj) {
+      const int jc = jj < n ? jj : (n > 0 ? n - 1 : 0);
+      const uint32_t p = pb + (uint32_t)(DOWN ? j0 - jc : j0 + jc);
+      const uint32_t i = 2 * (p + (p >> 5));
+      c.sa[jj] = L.xs[i];
+      c.pr[jj] = L.xs[i + 1];
+    }
   } else {
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj)
-      if (jj < n) {
-        const uint32_t j = (uint32_t)(DOWN ? j0 - jj : j0 + jj);
-        c.sa[jj] = L.at(Y.sa, j);
-        c.pr[jj] = L.at(Y.pr, j);
-      }
+    for (int jj = 0; jj < 8; ++jj) {
+      const int jc = jj < n ? jj : (n > 0 ? n - 1 : 0);
+      const uint32_t j = (uint32_t)(DOWN ? j0 - jc : j0 + jc);
+      c.sa[jj] = L.at(Y.sa, j);
+      c.pr[jj] = L.at(Y.pr, j);
+    }
   }
 }
 
@@ -855,20 +888,31 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
   // own steps' positions, walked down from the last one (loads go in
   // descending order of steps)
   QppWalk qw((uint32_t)L.t_of(end - 1), L.K, L.f1, L.f2);
+  // b0 and e are loaded in own chunks only, and nothing else defines them
+  // (the body reads them at own steps only): a buffer entry that another path
+  // defines too (zeros for the warm-up chunks, say) is copied where the paths
+  // join, and the copy waits for its load, so the load one chunk ahead would
+  // wait at once.
   auto load = [&](uint32_t k, AlphaIn<R>& x) {
     const uint32_t lo = C.a_lo(k), n = C.a_hi(k, end) - lo;
     ckload8(L, Y.cal, Y.cna, k, x.a0);
-    if (lo >= L.g) ckload8(L, Y.cbe, Y.cnb, (lo - L.g) / 8u + 1u, x.b0);
+    if (lo >= L.g) {  // an own chunk (alpha chunks start at the own steps' grid)
+      ckload8(L, Y.cbe, Y.cnb, (lo - L.g) / 8u + 1u, x.b0);
+      if (lower) {
+        // the walk stays in [0, K), so every entry loads unconditionally
+        // (entries past a short chunk, jj >= n, are never used); a branch
+        // per entry would make each load wait for the ones before it
 #pragma unroll
-    for (int jj = 7; jj >= 0; --jj) {
-      if ((uint32_t)jj < n) {
-        const uint32_t j = lo + (uint32_t)jj;
-        if (j >= L.g) {
-          x.e[jj] = eg[lower ? qw.pi : (uint32_t)L.t_of(j)];
-          if (j > L.g) qw.prev();
-        } else {
-          x.e[jj] = R(0);
+        for (int jj = 7; jj >= 0; --jj) {
+          qw.pin();
+          x.e[jj] = eg[qw.pi];
+          qw.prev_if((uint32_t)jj < n && lo + (uint32_t)jj > L.g);
         }
+      } else {
+        const GlobalPtr<const R> row = eg + row_t(L, lo);
+#pragma unroll
+        for (int jj = 7; jj >= 0; --jj)
+          if ((uint32_t)jj < n) x.e[jj] = row[jj];
       }
     }
   };
@@ -909,53 +953,50 @@ __device__ __forceinline__ void alpha_adjoint_sweep(const Lane<R>& L0, const Lay
 #pragma unroll
       for (int s = 0; s < 8; ++s) anext[s] = a[s];  // alpha_{lo + n} in an own chunk
     }
-    R gv[8][8];
-    if (own) {
-      R b[8];
+    // Descending over the chunk: at an own step, G from alpha_{j+1} and b =
+    // beta_{j+1} (beta recomputed downwards from the checkpoint at the chunk's
+    // top, nin steps in an own chunk, so also above the last own step of a
+    // short chunk) joins the adjoint, then the adjoint step of j.
+    R b[8];
 #pragma unroll
-      for (int s = 0; s < 8; ++s) b[s] = xc.b0[s];
-#pragma unroll
-      for (int jj = 7; jj >= 0; --jj) {
-        if ((uint32_t)jj < nin) {
-          if ((uint32_t)jj < n) {  // b: beta at local lo + jj + 1
-            R T[8];
-#pragma unroll
-            for (int s = 0; s < 8; ++s) T[s] = (jj == 7 || (uint32_t)jj + 1 >= n ? anext[s] : areg[jj + 1 < 8 ? jj + 1 : 7][s]) + b[s];
-            llr_adjoint(T, xc.e[jj], gv[jj]);
-          }
-          if (jj > 0) {
-            const uint32_t j = lo + (uint32_t)jj;
-            R o[8];
-            beta_step(b, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, beta_nrm(j, L.g));
-            select8(L.t_of(j) < (int)L.K, o, b);
-          }
-        }
-      }
-    }
+    for (int s = 0; s < 8; ++s) b[s] = xc.b0[s];
 #pragma unroll
     for (int jj = 7; jj >= 0; --jj) {
-      if ((uint32_t)jj < n) {
+      if ((uint32_t)jj < nin) {
         const uint32_t j = lo + (uint32_t)jj;
-        if (own) {
+        if ((uint32_t)jj < n) {
+          if (own) {
+            R T[8], gv[8];
 #pragma unroll
-          for (int s = 0; s < 8; ++s) ab[s] += gv[jj][s];
-        }
-        R ds, dp;
-        // weights from alpha_{j+1} before renormalisation (steps whose alpha
-        // was not stepped, t < 0, write only values no later pass reads)
-        if ((uint32_t)jj + 1 < n) {
-          alpha_adjoint_step<R, true>(areg[jj], areg[jj < 7 ? jj + 1 : 7], kHalfL2e<R> * in.sa[jj],
-                                      kHalfL2e<R> * in.pr[jj], ab, ds, dp);
-        } else if (own) {
-          R an[8];
+            for (int s = 0; s < 8; ++s) T[s] = (jj == 7 || (uint32_t)jj + 1 >= n ? anext[s] : areg[jj + 1 < 8 ? jj + 1 : 7][s]) + b[s];
+            llr_adjoint(T, xc.e[jj], gv);
 #pragma unroll
-          for (int s = 0; s < 8; ++s) an[s] = anext[s] + amx;
-          alpha_adjoint_step<R, true>(areg[jj], an, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], ab, ds, dp);
-        } else {
-          alpha_adjoint_step(areg[jj], areg[jj], kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], ab, ds, dp);
+            for (int s = 0; s < 8; ++s) ab[s] += gv[s];
+          }
+          // weights from alpha_{j+1} before renormalisation (steps whose
+          // alpha was not stepped, t < 0, write only values no later pass
+          // reads)
+          R ds, dp;
+          if ((uint32_t)jj + 1 < n) {
+            alpha_adjoint_step<R, true>(areg[jj], areg[jj < 7 ? jj + 1 : 7], kHalfL2e<R> * in.sa[jj],
+                                        kHalfL2e<R> * in.pr[jj], ab, ds, dp);
+          } else if (own) {
+            R an[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) an[s] = anext[s] + amx;
+            alpha_adjoint_step<R, true>(areg[jj], an, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], ab, ds,
+                                        dp);
+          } else {
+            alpha_adjoint_step(areg[jj], areg[jj], kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], ab, ds, dp);
+          }
+          L.at(Y.dsa, j) = own ? ds - xc.e[jj] : ds;
+          L.at(Y.dpr, j) = dp;
         }
-        L.at(Y.dsa, j) = ds - xc.e[jj];
-        L.at(Y.dpr, j) = dp;
+        if (own && jj > 0) {  // b: beta at local lo + jj
+          R o[8];
+          beta_step(b, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, beta_nrm(j, L.g));
+          select8(L.t_of(j) < (int)L.K, o, b);
+        }
       }
     }
   };
@@ -984,17 +1025,27 @@ __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layo
   QppWalk qw(L.q * L.S, L.K, L.f1, L.f2);
 #pragma unroll
   for (int s = 0; s < 8; ++s) bb[s] = 0;
+  // a0 and e: own chunks only, as on the alpha side
   auto load = [&](uint32_t m, BetaRe<R>& x) {
     const uint32_t lo = L.g + 8u * m;
     ckload8(L, Y.cbe, Y.cnb, m + 1, x.b0);
     if (lo < own_end) {
       ckload8(L, Y.cal, Y.cna, C.own_alpha(m, L.g), x.a0);
+      if (lower) {
+        // unconditional loads, as on the alpha side (entries past the own
+        // steps are never used)
 #pragma unroll
-      for (uint32_t jj = 0; jj < 8; ++jj)
-        if (lo + jj < own_end) {
-          x.e[jj] = eg[lower ? qw.pi : (uint32_t)L.t_of(lo + jj)];
-          qw.next();
+        for (uint32_t jj = 0; jj < 8; ++jj) {
+          qw.pin();
+          x.e[jj] = eg[qw.pi];
+          qw.next_if(lo + jj < own_end);
         }
+      } else {
+        const GlobalPtr<const R> row = eg + row_t(L, lo);
+#pragma unroll
+        for (uint32_t jj = 0; jj < 8; ++jj)
+          if (lo + jj < own_end) x.e[jj] = row[jj];
+      }
     }
   };
   auto body = [&](uint32_t m, const BetaRe<R>& xc) {
@@ -1022,13 +1073,17 @@ __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layo
         }
       }
     }
-    R gv[8][8];
-    if (own) {
-      R a[8];
+    // Ascending over the chunk: at an own step, alpha_{j+1} stepped up from
+    // the checkpoint at the chunk's start and G from it and beta_{j+1} seed
+    // the output adjoints of beta_{j+1}, then the adjoint step of j.
+    R a[8];
 #pragma unroll
-      for (int s = 0; s < 8; ++s) a[s] = xc.a0[s];
+    for (int s = 0; s < 8; ++s) a[s] = xc.a0[s];
 #pragma unroll
-      for (uint32_t jj = 0; jj < 8; ++jj)
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      if (jj < n) {
+        const uint32_t j = lo + jj;
+        R bn[8], ds, dp;
         if (jj < nown) {
           R o[8];
           alpha_step(a, kHalfL2e<R> * in.sa[jj], kHalfL2e<R> * in.pr[jj], o, alpha_nrm(lo + jj, C.off));
@@ -1038,16 +1093,11 @@ __device__ __forceinline__ void beta_adjoint_sweep(const Lane<R>& L0, const Layo
             a[s] = o[s];  // own steps lie inside the block
             T[s] = o[s] + breg[jj][s];
           }
-          llr_adjoint(T, xc.e[jj], gv[jj]);
+          llr_adjoint(T, xc.e[jj], bn);
+        } else {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) bn[s] = R(0);
         }
-    }
-#pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
-      if (jj < n) {
-        const uint32_t j = lo + jj;
-        R bn[8], ds, dp;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) bn[s] = jj < nown ? gv[jj][s] : R(0);
         // weights from beta_j = breg[jj - 1]: within a chunk only beta at its
         // first step (jj = 0) is renormalised, and that one is not in breg
         // (steps past the block end, whose beta was not stepped, are dropped
@@ -1430,7 +1480,7 @@ __device__ __forceinline__ void combine_store(const DiffArgs<R> A, const DiffJob
 // in flight while a chunk is added into the LDS copies.
 template <typename R>
 struct CombLdsIn {
-  R ds[8], dp[8], dsn[8], dpn[8], dsp[8], dpp[8];
+  R dsa[8], dsb[8], dpa[8], dpb[8], dsn[8], dpn[8], dsp[8], dpp[8];
 };
 
 template <typename R>
@@ -1443,45 +1493,74 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
   const uint32_t ln = min(L.lane + 1, 63u), lp = L.lane > 0 ? L.lane - 1 : 0u;
   QppWalk qw((uint32_t)L.t_of(jlo), L.K, L.f1, L.f2);
   R wsum = 0;
+  // Loads: every entry, at steps clamped into [j0, jhi) and neighbour
+  // indices clamped into their fields (entries of steps past jhi, and
+  // neighbour terms a step does not take, are loaded and not used), so the
+  // batch goes out without a branch per load.
+  const uint32_t top = L.S + 2 * L.g;
   auto load = [&](uint32_t j0, CombLdsIn<R>& x) {
 #pragma unroll
     for (uint32_t jj = 0; jj < 8; ++jj) {
-      const uint32_t j = j0 + jj;
-      if (j < jhi) {
-        own_sums(L, Y, j, x.ds[jj], x.dp[jj]);
-        if (j >= L.S) {  // t in [qS + S - g, qS + S): lane q+1's local j - S
-          x.dsn[jj] = L.nb(Y.dsa, j - L.S, ln);
-          x.dpn[jj] = L.nb(Y.dpr, j - L.S, ln);
-        }
-        if (j < 2 * L.g) {  // t in [qS, qS + g): lane q-1's local j + S
-          x.dsp[jj] = L.nb(Y.dsb, j + L.S, lp);
-          x.dpp[jj] = L.nb(Y.dpb, j + L.S, lp);
-        }
-      }
+      const uint32_t j = min(j0 + jj, jhi - 1);
+      // the own-window sums' terms, added in the body (an add here would
+      // wait for the loads)
+      x.dsa[jj] = L.at(Y.dsa, j);
+      x.dsb[jj] = L.at(Y.dsb, j);
+      x.dpa[jj] = L.at(Y.dpr, j);
+      x.dpb[jj] = L.at(Y.dpb, j);
+      // t in [qS + S - g, qS + S) (j >= S): lane q+1's local j - S
+      const uint32_t jn = j >= L.S ? j - L.S : 0u;
+      x.dsn[jj] = L.nb(Y.dsa, jn, ln);
+      x.dpn[jj] = L.nb(Y.dpr, jn, ln);
+      // t in [qS, qS + g) (j < 2g): lane q-1's local j + S
+      const uint32_t jp = min(j + L.S, top - 1);
+      x.dsp[jj] = L.nb(Y.dsb, jp, lp);
+      x.dpp[jj] = L.nb(Y.dpb, jp, lp);
     }
   };
+  // Each step updates its own positions (t, and Pi(t) in a lower half, a
+  // permutation: no two steps share one), so a chunk reads all its LDS
+  // values first and then writes them (one wait, where read-add-write per
+  // step waits at every step); entries past jhi read and do not write.
   auto body = [&](uint32_t j0, const CombLdsIn<R>& x) {
+    uint32_t ii[8], tt[8];
+    R aa[8], cc[8];
 #pragma unroll
     for (uint32_t jj = 0; jj < 8; ++jj) {
       const uint32_t j = j0 + jj;
-      if (j < jhi) {
-        const uint32_t t = (uint32_t)L.t_of(j);
-        const uint32_t i = lower ? qw.pi : t;
-        qw.next();
-        R a = x.ds[jj], c = x.dp[jj];
-        if (j >= L.S) {
-          a = has_next ? a + x.dsn[jj] : a;
-          c = has_next ? c + x.dpn[jj] : c;
-        }
-        if (j < 2 * L.g) {
-          a = has_prev ? a + x.dsp[jj] : a;
-          c = has_prev ? c + x.dpp[jj] : c;
-        }
-        X.at(0, blk, i) = X.at(0, blk, i) + a;
-        X.at(1, blk, t) = X.at(1, blk, t) + c;
+      tt[jj] = (uint32_t)L.t_of(min(j, jhi - 1));
+      ii[jj] = lower ? qw.pi : tt[jj];
+      qw.next_if(j < jhi);
+      R a = x.dsa[jj] + x.dsb[jj], c = x.dpa[jj] + x.dpb[jj];  // own_sums
+      if (j >= L.S) {
+        a = has_next ? a + x.dsn[jj] : a;
+        c = has_next ? c + x.dpn[jj] : c;
+      }
+      if (j < 2 * L.g) {
+        a = has_prev ? a + x.dsp[jj] : a;
+        c = has_prev ? c + x.dpp[jj] : c;
+      }
+      aa[jj] = a;
+      cc[jj] = c;
+    }
+    R r0[8], r1[8], r2[8], r3[8];
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      r0[jj] = X.at(0, blk, ii[jj]);
+      r1[jj] = X.at(1, blk, tt[jj]);
+      if (with_prev) {
+        r2[jj] = X.at(2, blk, ii[jj]);
+        r3[jj] = X.at(3, blk, ii[jj]);
+      }
+    }
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; ++jj) {
+      if (j0 + jj < jhi) {
+        X.at(0, blk, ii[jj]) = r0[jj] + aa[jj];
+        X.at(1, blk, tt[jj]) = r1[jj] + cc[jj];
         if (with_prev) {
-          X.at(2, blk, i) = X.at(2, blk, i) + w * a;
-          wsum += a * X.at(3, blk, i);
+          X.at(2, blk, ii[jj]) = r2[jj] + w * aa[jj];
+          wsum += aa[jj] * r3[jj];
         }
       }
     }
@@ -1490,14 +1569,25 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
   return wsum;
 }
 
+// A workgroup barrier that orders LDS accesses only: global stores issued
+// before it may still be in flight after it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Sum over the wave, then over the workgroup's two waves (wave 1's partial
 // through LDS, added to wave 0's in that order); the result is valid in
-// thread 0.
-template <typename R>
+// thread 0.  kLdsOnly: its barrier orders LDS only (lds_barrier).
+template <bool kLdsOnly = false, typename R>
 __device__ __forceinline__ R workgroup_sum(R v, R* slot) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   if (threadIdx.x == 64) *slot = v;
-  __syncthreads();
+  if constexpr (kLdsOnly)
+    lds_barrier();
+  else
+    __syncthreads();
   return v + *slot;
 }
 
@@ -1573,9 +1663,16 @@ __global__ __launch_bounds__(128, kDiffWavesPerSimd<R>) void diff_backward_kerne
       Lane<R> Lh = L;
       Lh.ck = A.ckpt + ((size_t)jb * H + h) * A.ckpt_job_half;
       // phase 3: the two adjoint sweeps, each forming the LLR terms G of the
-      // own steps it needs
+      // own steps it needs.  Wave 1 takes the alpha side, whose sweep has the
+      // more work: of a workgroup's two waves, wave 0 was measured slower on
+      // the same code (both ways round), so the pairing that ends the two
+      // together is this one.
+      // The tail LLR adjoints of the block, summed in segment order into the
+      // last segment's lane of wave 0 (the beta side, whose lanes hold each
+      // segment's share), for phase 4.
+      R tail_s[3] = {0, 0, 0}, tail_p[3] = {0, 0, 0};
       if (L.active) {
-        if (wave == 0) {
+        if (wave == 1) {
           alpha_adjoint_sweep(Lh, Y, CH, lower, cb().ebar(A, h & 1u, K));
         } else {
           R bb[8];
@@ -1595,9 +1692,17 @@ __global__ __launch_bounds__(128, kDiffWavesPerSimd<R>) void diff_backward_kerne
               }
             }
           }
-          for (int k3 = 0; k3 < 3; ++k3) {
-            L.at(Y.tl, k3) = dts[k3];
-            L.at(Y.tl, 3 + k3) = dtp[k3];
+          // segments m0 .. M-1 reach the block end (m S + S + g >= K); lane
+          // q = M - 1 of each block reads their shares from the block's
+          // lanes, ascending (the other lanes' sums are not used)
+          const uint32_t m0 = K > L.S + L.g ? (K - L.g - 1) / L.S : 0u;
+          for (uint32_t m = m0; m < L.M; ++m) {
+            const int src = (int)L.lane - (int)(L.M - 1) + (int)m;
+#pragma unroll
+            for (int k3 = 0; k3 < 3; ++k3) {
+              tail_s[k3] += shfl_r(dts[k3], src);
+              tail_p[k3] += shfl_r(dtp[k3], src);
+            }
           }
         }
       }
@@ -1622,34 +1727,31 @@ __global__ __launch_bounds__(128, kDiffWavesPerSimd<R>) void diff_backward_kerne
         R* g0 = B.g(A, 0, K);
         R* g1 = B.g(A, 1, K);
         R* g2 = B.g(A, 2, K);
-        // tail LLR adjoints, in segment order
-        R dts[3] = {0, 0, 0}, dtp[3] = {0, 0, 0};
-        for (uint32_t m = 0; m < L.M; ++m) {
-          const uint32_t other = L.lane - (L.M - 1) + m;
-          if (m * L.S + L.S + L.g < K) continue;
-          for (int k3 = 0; k3 < 3; ++k3) {
-            dts[k3] += L.nb(Y.tl, k3, other);
-            dtp[k3] += L.nb(Y.tl, 3 + k3, other);
-          }
-        }
+        // tail LLR adjoints added in: every load before any store (distinct
+        // positions), so the loads go out together
         if (!lower) {
-          g0[K] += dts[0];
-          g0[K + 2] += dts[1];
-          g1[K] += dtp[0];
-          g0[K + 1] += dtp[1];
-          g1[K + 2] += dtp[2];
+          const R x0 = g0[K], x1 = g0[K + 1], x2 = g0[K + 2], y0 = g1[K], y2 = g1[K + 2];
+          g0[K] = x0 + tail_s[0];
+          g0[K + 2] = x2 + tail_s[1];
+          g1[K] = y0 + tail_p[0];
+          g0[K + 1] = x1 + tail_p[1];
+          g1[K + 2] = y2 + tail_p[2];
         } else {
-          g1[K + 3] += dts[0];
-          g2[K] += dts[1];
-          g2[K + 2] += dts[2];
-          g2[K + 1] += dtp[0];
-          g0[K + 3] += dtp[1];
-          g2[K + 3] += dtp[2];
+          const R y3 = g1[K + 3], z0 = g2[K], z1 = g2[K + 1], z2 = g2[K + 2], z3 = g2[K + 3], x3 = g0[K + 3];
+          g1[K + 3] = y3 + tail_s[0];
+          g2[K] = z0 + tail_s[1];
+          g2[K + 2] = z2 + tail_s[2];
+          g2[K + 1] = z1 + tail_p[0];
+          g0[K + 3] = x3 + tail_p[1];
+          g2[K + 3] = z3 + tail_p[2];
         }
       }
-      wsum = workgroup_sum(wsum, &wave1_part[0]);
+      // the combine's global stores need not be complete here (the next
+      // half's first barrier, after its fill, waits for them): these two
+      // barriers order LDS only
+      wsum = workgroup_sum<true>(wsum, &wave1_part[0]);
       if (threadIdx.x == 0 && h) dsc[h - 1] += wsum;
-      __syncthreads();
+      lds_barrier();
     }
   }
 }
