@@ -16,7 +16,7 @@
 // beta when that reaches K).  A segment therefore depends only on the
 // half-iteration's inputs in [qS - g, qS + S + g), so each half-iteration is
 // a function of the previous one's extrinsics and the adjoint is local too.
-// Wave 0 carries a segment's alpha side and wave 1 its beta side; per half:
+// Each segment has an alpha side and a beta side, one per wave; per half:
 //   1. (forward) wave 0 runs alpha over the segment's window, wave 1 beta,
 //      each keeping a checkpoint every 8 steps in the saved state, which the
 //      adjoint reads back instead of running this phase again;
@@ -24,8 +24,8 @@
 //      each recomputes alpha_{t+1} and beta_{t+1} of its steps and forms the
 //      LLR from the eight states at t + 1 (the input bit of a transition is
 //      fixed by the state it enters);
-//   3. (adjoint) wave 0 runs the alpha adjoint backwards in time (weights of
-//      each state's two incoming transitions), wave 1 the beta adjoint
+//   3. (adjoint) wave 1 runs the alpha adjoint backwards in time (weights of
+//      each state's two incoming transitions), wave 0 the beta adjoint
 //      forwards in time (weights of each state's two outgoing transitions),
 //      both seeded at the own steps with G(t, n) = dloss/d(alpha_{t+1}(n) +
 //      beta_{t+1}(n)), the LLR's share of both metrics' adjoints, which each
