@@ -46,10 +46,10 @@ namespace {
 constexpr uint32_t kMaxHalves = 32;
 
 // Scratch elements per lane: staged sys+la and par, the sys/par adjoints of
-// the alpha and the beta side and six tail adjoints.
+// the alpha and the beta side.
 uint32_t slot_elems(uint32_t s, uint32_t g) {
   const uint32_t p = s + 2 * g;
-  return 6 * p + 6;
+  return 6 * p;
 }
 
 // A job runs on a workgroup of two waves (alpha side and beta side).

@@ -198,17 +198,14 @@ struct QppWalk {
     pi = c ? p : pi;
     d = c ? dd : d;
   }
-  __device__ void prev() {
-    d = d >= d2 ? d - d2 : d + k - d2;
-    pi = pi >= d ? pi - d : pi + k - d;
-  }
   // The walk's state as the optimiser's barrier: steps after it cannot be
   // computed before it, so a gather walked step by step computes each
   // position where it loads (otherwise a chunk's positions are computed
   // ahead and parked in spill slots, and each reload waits for every load
   // before it).
   __device__ void pin() { asm volatile("" : "+v"(pi), "+v"(d)); }
-  // prev() when c, else nothing (no branch)
+  // a step back, Pi(i-1) and its difference, when c, else nothing (no
+  // branch)
   __device__ void prev_if(bool c) {
     const uint32_t dd = d >= d2 ? d - d2 : d + k - d2;
     const uint32_t p = pi >= dd ? pi - dd : pi + k - dd;
@@ -222,7 +219,7 @@ struct QppWalk {
 // per state; kept by the forward for the adjoint).
 struct Layout {
   uint32_t P, S, g;
-  uint32_t sa, pr, dsa, dpr, dsb, dpb, tl, total;
+  uint32_t sa, pr, dsa, dpr, dsb, dpb;
   uint32_t cna, cnb, cal, cbe;
   __device__ Layout(uint32_t s, uint32_t gw) : P(s + 2 * gw), S(s), g(gw) {
     cna = diff_ck_alpha(s, gw);
@@ -235,8 +232,6 @@ struct Layout {
     dpr = dsa + P;
     dsb = dpr + P;               // beta side
     dpb = dsb + P;
-    tl = dpb + P;                // 6
-    total = tl + 6;
   }
 };
 
