@@ -601,6 +601,24 @@ def diff_leg(T, torch, train_steps):
             fwd.append(a.elapsed_time(b))
             bwd.append(b.elapsed_time(c))
     fwd_ms, bwd_ms = float(np.median(fwd)), float(np.median(bwd))
+    # the decoder's own calls alone (forward kernel; adjoint kernel + scale
+    # reduction), same batch and scales, without torch's loss around them
+    plan = dec.plan
+    flat = x_req.detach().reshape(-1).contiguous()
+    sc = dec.scales.detach().contiguous()
+    gpost = torch.full((plan.post_elems,), 1e-3, dtype=torch.float32, device="cuda")
+    kf, kb = [], []
+    for rep in range(23):
+        a.record(stream)
+        _, saved = plan.forward(flat, sc)
+        b.record(stream)
+        plan.backward(flat, sc, saved, gpost)
+        c.record(stream)
+        torch.cuda.synchronize()
+        if rep >= 3:
+            kf.append(a.elapsed_time(b))
+            kb.append(b.elapsed_time(c))
+    kfwd_ms, kbwd_ms = float(np.median(kf)), float(np.median(kb))
 
     held = [gen.batch(1000 + i) for i in range(4)]  # 8192 held-out blocks, never trained on
 
@@ -643,7 +661,7 @@ def diff_leg(T, torch, train_steps):
     b_, c_ = int(np.sum(w_blk & ~ex_blk)), int(np.sum(~w_blk & ex_blk))
     from scipy.stats import binomtest
     p_blk = float(binomtest(b_, b_ + c_, 0.5).pvalue) if b_ + c_ else 1.0
-    log(f"diff: fwd {fwd_ms:.2f} ms bwd {bwd_ms:.2f} ms; train {train_steps} steps {train_s:.1f} s; held-out BER "
+    log(f"diff: fwd {fwd_ms:.2f} ms bwd {bwd_ms:.2f} ms (decoder calls {kfwd_ms:.2f} / {kbwd_ms:.2f} ms); train {train_steps} steps {train_s:.1f} s; held-out BER "
         f"exact {ex_bits / nbits:.2e} unit {unit_bits / nbits:.2e} weighted {w_bits / nbits:.2e}; block errors "
         f"exact {int(ex_blk.sum())} weighted {int(w_blk.sum())} (only weighted {b_}, only exact {c_}, p {p_blk:.2f})")
     # SIMD VALU time, memory-wait share and HBM-side traffic of the two kernels at this workload, from
@@ -664,6 +682,9 @@ def diff_leg(T, torch, train_steps):
                       "gradient), HIP events on the launch stream, median of 20 after 3 untimed",
             "counters": counters,
             "fwd_ms": fwd_ms, "bwd_ms": bwd_ms, "fwd_gbit_s": n * k / fwd_ms / 1e6, "bwd_gbit_s": n * k / bwd_ms / 1e6,
+            "decoder_calls_ms": {"forward": kfwd_ms, "backward": kbwd_ms,
+                                 "what": "DiffPlan.forward / DiffPlan.backward alone (forward kernel; adjoint "
+                                         "kernel + scale reduction), median of 20 after 3 untimed"},
             "adam_steps": train_steps, "train_s": train_s, "training": "a fresh batch of 2048 blocks per step",
             "loss_every_50": losses,
             "scales": [round(float(v), 4) for v in dec.scales.detach().cpu()],
