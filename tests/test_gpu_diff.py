@@ -140,6 +140,31 @@ def test_adjoint_fp64_global_staging_matches_reverse_mode():
         assert rel < 1e-9, rel
 
 
+@pytest.mark.parametrize("k,n", [(1024, 64), (4096, 6)])
+def test_adjoint_fp32_is_deterministic(k, n):
+    """The float32 gradients are a fixed sum order throughout (combine terms
+    in a fixed order per position, the tail adjoints by segment, the scale
+    partials per job and then in job order): two forward + backward runs on
+    the same inputs give the same bits.  K=1024: 32 jobs of the LDS-staged
+    kernels; K=4096: 6 jobs of the global-staging ones."""
+    bits, llr = _blocks(k, n, 0.5, 17)
+    plan = DiffPlan([k] * n, 6)
+    x = torch.tensor(llr, device="cuda").reshape(-1).contiguous()
+    w = torch.linspace(0.8, 1.1, 12, device="cuda")
+    gp = torch.tensor(np.random.default_rng(3).standard_normal(plan.post_elems), dtype=torch.float32,
+                      device="cuda")
+    _say("fp32 determinism", k, n)
+    outs = []
+    for _ in range(2):
+        post, saved = plan.forward(x, w)
+        g_llr, g_sc = plan.backward(x, w, saved, gp)
+        torch.cuda.synchronize()
+        outs.append((post.cpu().numpy(), g_llr.cpu().numpy(), g_sc.cpu().numpy()))
+    for a, b in zip(*outs):
+        assert np.isfinite(a).all()
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_training_lowers_the_loss():
     """A few Adam steps on the scales lower the BCE loss (float32)."""
     k, n = 1024, 64
