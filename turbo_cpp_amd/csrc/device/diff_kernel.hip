@@ -364,19 +364,19 @@ __device__ __forceinline__ void loop_down(uint32_t lo, uint32_t hi, Load&& load,
   }
 }
 
-// Runs body(j0, buf) for j0 = lo, lo + 8, ... < hi with load(j0, buf) one
+// Runs body(j0, buf) for j0 = lo, lo + C, ... < hi with load(j0, buf) one
 // chunk ahead, alternating two buffers.
-template <class Buf, class Load, class Body>
+template <class Buf, uint32_t C = 8, class Load, class Body>
 __device__ __forceinline__ void chunks_up(uint32_t lo, uint32_t hi, Load&& load, Body&& body) {
   if (lo >= hi) return;
   Buf b0, b1;
   load(lo, b0);
-  for (uint32_t j0 = lo; j0 < hi; j0 += 16) {
-    if (j0 + 8 < hi) load(j0 + 8, b1);
+  for (uint32_t j0 = lo; j0 < hi; j0 += 2 * C) {
+    if (j0 + C < hi) load(j0 + C, b1);
     body(j0, b0);
-    if (j0 + 8 >= hi) break;
-    if (j0 + 16 < hi) load(j0 + 16, b0);
-    body(j0 + 8, b1);
+    if (j0 + C >= hi) break;
+    if (j0 + 2 * C < hi) load(j0 + 2 * C, b0);
+    body(j0 + C, b1);
   }
 }
 template <typename R>
@@ -1473,9 +1473,14 @@ __device__ __forceinline__ void combine_store(const DiffArgs<R> A, const DiffJob
 // Per-lane part of the LDS combine over own local steps [jlo, jhi) (the same
 // sums, in the same order, as combine), with the next chunk's scratch loads
 // in flight while a chunk is added into the LDS copies.
+// Chunks of four steps: the raw terms of two chunks in flight stay within the
+// registers the kernel has (with eight, the combine's buffers made the
+// allocator spill values that the sweeps then reload).
+constexpr uint32_t kCombStep = 4;
 template <typename R>
 struct CombLdsIn {
-  R dsa[8], dsb[8], dpa[8], dpb[8], dsn[8], dpn[8], dsp[8], dpp[8];
+  R dsa[kCombStep], dsb[kCombStep], dpa[kCombStep], dpb[kCombStep], dsn[kCombStep], dpn[kCombStep],
+      dsp[kCombStep], dpp[kCombStep];
 };
 
 template <typename R>
@@ -1495,7 +1500,7 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
   const uint32_t top = L.S + 2 * L.g;
   auto load = [&](uint32_t j0, CombLdsIn<R>& x) {
 #pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
+    for (uint32_t jj = 0; jj < kCombStep; ++jj) {
       const uint32_t j = min(j0 + jj, jhi - 1);
       // the own-window sums' terms, added in the body (an add here would
       // wait for the loads)
@@ -1518,10 +1523,10 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
   // values first and then writes them (one wait, where read-add-write per
   // step waits at every step); entries past jhi read and do not write.
   auto body = [&](uint32_t j0, const CombLdsIn<R>& x) {
-    uint32_t ii[8], tt[8];
-    R aa[8], cc[8];
+    uint32_t ii[kCombStep], tt[kCombStep];
+    R aa[kCombStep], cc[kCombStep];
 #pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
+    for (uint32_t jj = 0; jj < kCombStep; ++jj) {
       const uint32_t j = j0 + jj;
       tt[jj] = (uint32_t)L.t_of(min(j, jhi - 1));
       ii[jj] = lower ? qw.pi : tt[jj];
@@ -1538,9 +1543,9 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
       aa[jj] = a;
       cc[jj] = c;
     }
-    R r0[8], r1[8], r2[8], r3[8];
+    R r0[kCombStep], r1[kCombStep], r2[kCombStep], r3[kCombStep];
 #pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
+    for (uint32_t jj = 0; jj < kCombStep; ++jj) {
       r0[jj] = X.at(0, blk, ii[jj]);
       r1[jj] = X.at(1, blk, tt[jj]);
       if (with_prev) {
@@ -1549,7 +1554,7 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
       }
     }
 #pragma unroll
-    for (uint32_t jj = 0; jj < 8; ++jj) {
+    for (uint32_t jj = 0; jj < kCombStep; ++jj) {
       if (j0 + jj < jhi) {
         X.at(0, blk, ii[jj]) = r0[jj] + aa[jj];
         X.at(1, blk, tt[jj]) = r1[jj] + cc[jj];
@@ -1560,7 +1565,7 @@ __device__ __forceinline__ R combine_lds(const Lane<R>& L0, const Layout& Y, uin
       }
     }
   };
-  chunks_up<CombLdsIn<R>>(jlo, jhi, load, body);
+  chunks_up<CombLdsIn<R>, kCombStep>(jlo, jhi, load, body);
   return wsum;
 }
 
